@@ -1,0 +1,338 @@
+"""hitomi_amd -- MI355X-native FFT estimator path of hitomi_measurement.
+
+Python view of libhitomi_amd.so (include/hitomi_amd.h) through ctypes.  The
+compute happens in the gfx950 kernels of the shared library; this module
+only moves arrays and unpacks results.  Importing it does not touch the GPU;
+creating a Context does, and fails loudly when the library or a GPU is
+missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhitomi_amd.so")
+
+NGP, CIC, TSC = 0, 1, 2
+FP32, FP64 = 0, 1
+ASSIGN = {"NGP": NGP, "CIC": CIC, "TSC": TSC}
+MAX_ELLS = 4
+
+
+class HitomiError(RuntimeError):
+    pass
+
+
+class Mesh(ct.Structure):
+    _fields_ = [("n_mesh", ct.c_int * 3), ("boxsize", ct.c_double * 3), ("assign", ct.c_int)]
+
+
+class BoxConfig(ct.Structure):
+    _fields_ = [
+        ("mesh", Mesh),
+        ("precision", ct.c_int),
+        ("n_kbin", ct.c_int),
+        ("kmin", ct.c_double),
+        ("kmax", ct.c_double),
+        ("n_pk_ells", ct.c_int),
+        ("pk_ells", ct.c_int * MAX_ELLS),
+        ("bk", ct.c_int),
+        ("regrid", ct.c_int),
+        ("prune", ct.c_int),
+    ]
+
+
+class BoxResult(ct.Structure):
+    _fields_ = [
+        ("pk", ct.POINTER(ct.c_double)),
+        ("pk_raw", ct.POINTER(ct.c_double)),
+        ("pk_nmode", ct.POINTER(ct.c_longlong)),
+        ("bk", ct.POINTER(ct.c_double)),
+        ("bk_sum", ct.POINTER(ct.c_double)),
+        ("sn_sum", ct.POINTER(ct.c_double)),
+        ("shell_nmode", ct.POINTER(ct.c_longlong)),
+    ]
+
+
+# Every symbol include/hitomi_amd.h declares, with (restype, argtypes).
+_VP = ct.c_void_p
+_SIGS = {
+    "hm_ctx_create": (_VP, [ct.c_int, ct.c_int]),
+    "hm_ctx_destroy": (None, [_VP]),
+    "hm_last_error": (ct.c_char_p, [_VP]),
+    "hm_sync": (ct.c_int, [_VP]),
+    "hm_dev_alloc": (_VP, [_VP, ct.c_size_t]),
+    "hm_dev_free": (ct.c_int, [_VP, _VP]),
+    "hm_memcpy_h2d": (ct.c_int, [_VP, _VP, _VP, ct.c_size_t]),
+    "hm_memcpy_d2h": (ct.c_int, [_VP, _VP, _VP, ct.c_size_t]),
+    "hm_memset_zero": (ct.c_int, [_VP, _VP, ct.c_size_t]),
+    "hm_timing_enable": (ct.c_int, [_VP, ct.c_int]),
+    "hm_timing_get": (ct.c_int, [_VP, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]),
+    "hm_timing_reset": (ct.c_int, [_VP]),
+    "hm_assign": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
+    "hm_cell_coords": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_longlong, _VP]),
+    "hm_fft_c2c": (ct.c_int, [_VP, ct.c_int, _VP, ct.POINTER(ct.c_int), ct.c_int]),
+    "hm_fft_r2c": (ct.c_int, [_VP, ct.c_int, _VP, _VP, ct.POINTER(ct.c_int)]),
+    "hm_fft_c2r": (ct.c_int, [_VP, ct.c_int, _VP, _VP, ct.POINTER(ct.c_int)]),
+    "hm_box_plan_create": (_VP, [_VP, ct.POINTER(BoxConfig)]),
+    "hm_box_plan_destroy": (None, [_VP, _VP]),
+    "hm_box_plan_execute": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.POINTER(BoxResult)]),
+    "hm_box_plan_bytes": (ct.c_size_t, [_VP]),
+    "hm_box_plan_triple_mesh": (ct.c_int, [_VP, ct.POINTER(ct.c_int)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ct.CDLL:
+    """Load libhitomi_amd.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise HitomiError(f"{path} not found: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ct.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def make_mesh(n_mesh, boxsize, assign="TSC") -> Mesh:
+    m = Mesh()
+    n = list(n_mesh) if np.ndim(n_mesh) else [int(n_mesh)] * 3
+    L = list(boxsize) if np.ndim(boxsize) else [float(boxsize)] * 3
+    for a in range(3):
+        m.n_mesh[a] = int(n[a])
+        m.boxsize[a] = float(L[a])
+    m.assign = ASSIGN[assign] if isinstance(assign, str) else int(assign)
+    return m
+
+
+class DeviceArray:
+    """A library-owned device allocation with its shape and dtype."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctx.lib.hm_dev_alloc(ctx.h, max(self.nbytes, 1))
+        if not self.ptr:
+            ctx.raise_error("hm_dev_alloc")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        self.ctx.check(self.ctx.lib.hm_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "h2d")
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.ctx.check(self.ctx.lib.hm_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.hm_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device, one stream (hm_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self.h = self.lib.hm_ctx_create(device, 0)
+        if not self.h:
+            raise HitomiError(f"hm_ctx_create({device}) failed: no usable GPU")
+
+    def close(self):
+        if self.h:
+            self.lib.hm_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def raise_error(self, what):
+        msg = self.lib.hm_last_error(self.h)
+        raise HitomiError(f"{what}: {msg.decode() if msg else '?'}")
+
+    def check(self, rc, what):
+        if rc != 0:
+            self.raise_error(what)
+
+    def array(self, a, dtype=None) -> DeviceArray:
+        a = np.ascontiguousarray(a, dtype=dtype)
+        return DeviceArray(self, a.shape, a.dtype).upload(a)
+
+    def empty(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def sync(self):
+        self.check(self.lib.hm_sync(self.h), "hm_sync")
+
+    # timing
+    def timing(self, on=True):
+        self.check(self.lib.hm_timing_enable(self.h, 1 if on else 0), "hm_timing_enable")
+
+    def kernel_time(self, name):
+        ms = ct.c_double()
+        n = ct.c_longlong()
+        self.check(self.lib.hm_timing_get(self.h, name.encode(), ct.byref(ms), ct.byref(n)), "hm_timing_get")
+        return ms.value, n.value
+
+    def timing_reset(self):
+        self.check(self.lib.hm_timing_reset(self.h), "hm_timing_reset")
+
+    # a6: mass assignment
+    def assign(self, mesh: Mesh, pos, weights=None, wabs_sum=0.0):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        npart = pos.shape[0]
+        dpos = self.array(pos)
+        dw = self.array(np.ascontiguousarray(weights, dtype=np.float64).reshape(-1, 2)) if weights is not None else None
+        ng = mesh.n_mesh[0] * mesh.n_mesh[1] * mesh.n_mesh[2]
+        shape = (mesh.n_mesh[0], mesh.n_mesh[1], mesh.n_mesh[2])
+        re = self.empty(shape, np.float64)
+        im = self.empty(shape, np.float64) if weights is not None else None
+        self.check(
+            self.lib.hm_assign(self.h, ct.byref(mesh), dpos.ptr, dw.ptr if dw else None, npart, float(wabs_sum),
+                               re.ptr, im.ptr if im else None), "hm_assign")
+        out_re = re.download()
+        out_im = im.download() if im is not None else np.zeros_like(out_re)
+        assert out_re.size == ng
+        return out_re, out_im
+
+    def cell_coords(self, mesh: Mesh, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        s = {NGP: 1, CIC: 8, TSC: 27}[mesh.assign]
+        dpos = self.array(pos)
+        out = self.empty((pos.shape[0], s), np.int64)
+        self.check(self.lib.hm_cell_coords(self.h, ct.byref(mesh), dpos.ptr, pos.shape[0], out.ptr), "hm_cell_coords")
+        return out.download()
+
+    # a8: FFTs
+    def fft_c2c(self, a, sign, precision=FP64):
+        dt = np.complex128 if precision == FP64 else np.complex64
+        a = np.ascontiguousarray(a, dtype=dt)
+        n = (ct.c_int * 3)(*a.shape)
+        d = self.array(a)
+        self.check(self.lib.hm_fft_c2c(self.h, precision, d.ptr, n, int(sign)), "hm_fft_c2c")
+        return d.download()
+
+    def fft_r2c(self, a, precision=FP64):
+        dt = np.float64 if precision == FP64 else np.float32
+        a = np.ascontiguousarray(a, dtype=dt)
+        n = (ct.c_int * 3)(*a.shape)
+        d = self.array(a)
+        out = self.empty((a.shape[0], a.shape[1], a.shape[2] // 2 + 1),
+                         np.complex128 if precision == FP64 else np.complex64)
+        self.check(self.lib.hm_fft_r2c(self.h, precision, d.ptr, out.ptr, n), "hm_fft_r2c")
+        return out.download()
+
+    def fft_c2r(self, h, nz, precision=FP64):
+        dt = np.complex128 if precision == FP64 else np.complex64
+        h = np.ascontiguousarray(h, dtype=dt)
+        n = (ct.c_int * 3)(h.shape[0], h.shape[1], int(nz))
+        d = self.array(h)
+        out = self.empty((h.shape[0], h.shape[1], int(nz)), np.float64 if precision == FP64 else np.float32)
+        self.check(self.lib.hm_fft_c2r(self.h, precision, d.ptr, out.ptr, n), "hm_fft_c2r")
+        return out.download()
+
+
+class BoxPlan:
+    """Periodic-box P_l + B_000 plan (hm_box_plan): calcPowerSpectrumForBOX
+    (powerspec.hpp:273-332) and calcBiSpectrumForBOX (bispec.hpp:877-1203) for
+    every ith_kbin at once."""
+
+    def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2,
+                 pk_ells=(0, 2, 4), bk=True, precision=FP32, regrid=True, prune=True):
+        self.ctx = ctx
+        cfg = BoxConfig()
+        cfg.mesh = make_mesh(n_mesh, boxsize, assign)
+        cfg.precision = precision
+        cfg.n_kbin = int(n_kbin)
+        cfg.kmin = float(kmin)
+        cfg.kmax = float(kmax)
+        cfg.n_pk_ells = len(pk_ells)
+        for i, l in enumerate(pk_ells):
+            cfg.pk_ells[i] = int(l)
+        cfg.bk = 1 if bk else 0
+        cfg.regrid = 1 if regrid else 0
+        cfg.prune = 1 if prune else 0
+        self.cfg = cfg
+        self.n_kbin = int(n_kbin)
+        self.pk_ells = tuple(pk_ells)
+        self.h = ctx.lib.hm_box_plan_create(ctx.h, ct.byref(cfg))
+        if not self.h:
+            ctx.raise_error("hm_box_plan_create")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hm_box_plan_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self):
+        return self.ctx.lib.hm_box_plan_bytes(self.h)
+
+    @property
+    def triple_mesh(self):
+        m = (ct.c_int * 3)()
+        self.ctx.lib.hm_box_plan_triple_mesh(self.h, m)
+        return tuple(m)
+
+    def execute(self, pos, npart=None):
+        """pos: host (N,3) float64 array or a DeviceArray of it."""
+        if isinstance(pos, DeviceArray):
+            dpos = pos
+            npart = pos.shape[0] if npart is None else npart
+        else:
+            pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+            dpos = self.ctx.array(pos)
+            npart = pos.shape[0]
+        nk, ne = self.n_kbin, len(self.pk_ells)
+        out = {
+            "pk": np.zeros((ne, nk, 2)), "pk_raw": np.zeros((ne, nk, 2)), "pk_nmode": np.zeros(nk, np.int64),
+            "bk": np.zeros((nk, nk, 2)), "bk_sum": np.zeros((nk, nk, 2)), "sn_sum": np.zeros((nk, nk, 2)),
+            "shell_nmode": np.zeros(nk, np.int64),
+        }
+        r = BoxResult()
+        dp = ct.POINTER(ct.c_double)
+        lp = ct.POINTER(ct.c_longlong)
+        r.pk = out["pk"].ctypes.data_as(dp)
+        r.pk_raw = out["pk_raw"].ctypes.data_as(dp)
+        r.pk_nmode = out["pk_nmode"].ctypes.data_as(lp)
+        r.bk = out["bk"].ctypes.data_as(dp)
+        r.bk_sum = out["bk_sum"].ctypes.data_as(dp)
+        r.sn_sum = out["sn_sum"].ctypes.data_as(dp)
+        r.shell_nmode = out["shell_nmode"].ctypes.data_as(lp)
+        self.ctx.check(self.ctx.lib.hm_box_plan_execute(self.ctx.h, self.h, dpos.ptr, int(npart), ct.byref(r)),
+                       "hm_box_plan_execute")
+        return out

@@ -1,0 +1,268 @@
+// hm_api.hip -- the extern "C" boundary of libhitomi_amd.so (declared in
+// include/hitomi_amd.h).  Every entry point catches its own failures and
+// reports them through hm_last_error().
+#include "hm_assign.hpp"
+#include "hm_internal.hpp"
+
+#include "../../include/hitomi_amd.h"
+
+#include <cstring>
+#include <new>
+
+namespace hm {
+struct BoxPlanImpl;
+BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg);
+void plan_destroy(BoxPlanImpl* p);
+int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res);
+size_t plan_bytes(const BoxPlanImpl* p);
+void plan_triple_mesh(const BoxPlanImpl* p, int m[3]);
+int cell_coords(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, long long np, long long* d_out);
+int fixed_to_density(hm_ctx* ctx, const long long* d_q, long long n, double inv_scale, double factor, double* d_out);
+}  // namespace hm
+
+struct hm_box_plan {
+    hm::BoxPlanImpl* impl;
+};
+
+static int bad(hm_ctx* ctx, const char* msg) {
+    if (ctx) ctx->set_error(msg);
+    return -1;
+}
+
+static bool mesh_ok(hm_ctx* ctx, const hm_mesh* m, hm::AssignGeom& g) {
+    if (!m) return bad(ctx, "mesh is NULL"), false;
+    for (int a = 0; a < 3; a++) {
+        if (m->n_mesh[a] <= 0) return bad(ctx, "n_mesh must be > 0"), false;
+        if (!(m->boxsize[a] > 0.0)) return bad(ctx, "boxsize must be > 0"), false;
+        g.n[a] = m->n_mesh[a];
+        g.L[a] = m->boxsize[a];
+    }
+    if (m->assign < HM_NGP || m->assign > HM_TSC) return bad(ctx, "assign must be NGP, CIC or TSC"), false;
+    g.assign = m->assign;
+    g.ng = (long long)m->n_mesh[0] * m->n_mesh[1] * m->n_mesh[2];
+    return true;
+}
+
+extern "C" {
+
+hm_ctx* hm_ctx_create(int device, int flags) {
+    (void)flags;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    hm_ctx* ctx = new (std::nothrow) hm_ctx();
+    if (!ctx) return nullptr;
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return nullptr;
+    }
+    return ctx;
+}
+
+void hm_ctx_destroy(hm_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+    for (auto& pe : ctx->pending) {
+        (void)hipEventDestroy(pe.a);
+        (void)hipEventDestroy(pe.b);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* hm_last_error(const hm_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int hm_sync(hm_ctx* ctx) {
+    if (!ctx) return -1;
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+void* hm_dev_alloc(hm_ctx* ctx, size_t bytes) {
+    if (!ctx) return nullptr;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        ctx->set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    return p;
+}
+
+int hm_dev_free(hm_ctx* ctx, void* d_ptr) {
+    if (!ctx) return -1;
+    if (!d_ptr) return 0;
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    HM_HIP_CHECK(ctx, hipFree(d_ptr));
+    return 0;
+}
+
+int hm_memcpy_h2d(hm_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+    if (!ctx) return -1;
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int hm_memcpy_d2h(hm_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
+    if (!ctx) return -1;
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int hm_memset_zero(hm_ctx* ctx, void* d_ptr, size_t bytes) {
+    if (!ctx) return -1;
+    return hm::fill_zero(ctx, d_ptr, bytes);
+}
+
+int hm_timing_enable(hm_ctx* ctx, int on) {
+    if (!ctx) return -1;
+    ctx->timing = on != 0;
+    return 0;
+}
+
+static int timing_collect(hm_ctx* ctx) {
+    if (ctx->pending.empty()) return 0;
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& pe : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+            auto& s = ctx->kstats[pe.name];
+            s.first += ms;
+            s.second += 1;
+        }
+        ctx->event_pool.push_back(pe.a);
+        ctx->event_pool.push_back(pe.b);
+    }
+    ctx->pending.clear();
+    return 0;
+}
+
+int hm_timing_get(hm_ctx* ctx, const char* name, double* total_ms, long long* launches) {
+    if (!ctx || !name) return -1;
+    if (timing_collect(ctx)) return -1;
+    auto it = ctx->kstats.find(name);
+    if (total_ms) *total_ms = it == ctx->kstats.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == ctx->kstats.end() ? 0 : it->second.second;
+    return 0;
+}
+
+int hm_timing_reset(hm_ctx* ctx) {
+    if (!ctx) return -1;
+    if (timing_collect(ctx)) return -1;
+    ctx->kstats.clear();
+    return 0;
+}
+
+int hm_assign(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
+              double wabs_sum, double* d_out_re, double* d_out_im) {
+    if (!ctx) return -1;
+    hm::AssignGeom g;
+    if (!mesh_ok(ctx, mesh, g)) return -1;
+    if (np < 0 || (np > 0 && !d_pos) || !d_out_re) return bad(ctx, "hm_assign: bad arguments");
+    if (!(wabs_sum > 0.0)) wabs_sum = (double)np;
+    const double scale = hm::fixed_scale_for(wabs_sum);
+    hm::DevBuf q;
+    const size_t bytes = sizeof(long long) * (size_t)g.ng * (d_out_im ? 2 : 1);
+    if (hm::dev_alloc(ctx, q, bytes)) return -1;
+    long long* qre = (long long*)q.p;
+    long long* qim = d_out_im ? qre + g.ng : nullptr;
+    int rc = hm::fill_zero(ctx, q.p, bytes);
+    if (!rc) rc = hm::assign_fixed(ctx, g, d_pos, d_w, np, scale, qre, qim);
+    const double dV = (g.L[0] * g.L[1] * g.L[2]) / (double)g.ng;
+    const double factor = 1.0 / dV;
+    if (!rc) rc = hm::fixed_to_density(ctx, qre, g.ng, 1.0 / scale, factor, d_out_re);
+    if (!rc && qim) rc = hm::fixed_to_density(ctx, qim, g.ng, 1.0 / scale, factor, d_out_im);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = bad(ctx, "hm_assign: stream error");
+    hm::dev_free(q);
+    return rc;
+}
+
+int hm_cell_coords(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, long long np, long long* d_coords) {
+    if (!ctx) return -1;
+    hm::AssignGeom g;
+    if (!mesh_ok(ctx, mesh, g)) return -1;
+    if (np < 0 || (np > 0 && (!d_pos || !d_coords))) return bad(ctx, "hm_cell_coords: bad arguments");
+    return hm::cell_coords(ctx, g, d_pos, np, d_coords);
+}
+
+static bool fft_dims_ok(hm_ctx* ctx, const int n[3], bool real) {
+    for (int a = 0; a < 3; a++)
+        if (!hm::fft_length_supported(n[a])) return bad(ctx, "fft: unsupported length"), false;
+    if (real && !hm::fft_length_supported(n[2] / 2)) return bad(ctx, "fft: unsupported real length"), false;
+    return true;
+}
+
+int hm_fft_c2c(hm_ctx* ctx, int precision, void* d_data, const int n[3], int sign) {
+    if (!ctx) return -1;
+    if (!n || !d_data || (sign != -1 && sign != 1)) return bad(ctx, "hm_fft_c2c: bad arguments");
+    if (!fft_dims_ok(ctx, n, false)) return -1;
+    return precision == HM_FP64 ? hm::fft3d_c2c<double>(ctx, (double2*)d_data, n, sign)
+                                : hm::fft3d_c2c<float>(ctx, (float2*)d_data, n, sign);
+}
+
+int hm_fft_r2c(hm_ctx* ctx, int precision, const void* d_in, void* d_out, const int n[3]) {
+    if (!ctx) return -1;
+    if (!n || !d_in || !d_out) return bad(ctx, "hm_fft_r2c: bad arguments");
+    if (!fft_dims_ok(ctx, n, true)) return -1;
+    return precision == HM_FP64 ? hm::fft3d_r2c<double>(ctx, (const double*)d_in, (double2*)d_out, n)
+                                : hm::fft3d_r2c<float>(ctx, (const float*)d_in, (float2*)d_out, n);
+}
+
+int hm_fft_c2r(hm_ctx* ctx, int precision, void* d_in, void* d_out, const int n[3]) {
+    if (!ctx) return -1;
+    if (!n || !d_in || !d_out) return bad(ctx, "hm_fft_c2r: bad arguments");
+    if (!fft_dims_ok(ctx, n, true)) return -1;
+    return precision == HM_FP64 ? hm::fft3d_c2r<double>(ctx, (double2*)d_in, (double*)d_out, n)
+                                : hm::fft3d_c2r<float>(ctx, (float2*)d_in, (float*)d_out, n);
+}
+
+hm_box_plan* hm_box_plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
+    if (!ctx) return nullptr;
+    if (!cfg) {
+        ctx->set_error("hm_box_plan_create: cfg is NULL");
+        return nullptr;
+    }
+    hm::BoxPlanImpl* impl = nullptr;
+    try {
+        impl = hm::plan_create(ctx, cfg);
+    } catch (const std::exception& e) {
+        ctx->set_error(std::string("hm_box_plan_create: ") + e.what());
+        return nullptr;
+    }
+    if (!impl) return nullptr;
+    return new hm_box_plan{impl};
+}
+
+void hm_box_plan_destroy(hm_ctx* ctx, hm_box_plan* plan) {
+    if (!plan) return;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    hm::plan_destroy(plan->impl);
+    delete plan;
+}
+
+int hm_box_plan_execute(hm_ctx* ctx, hm_box_plan* plan, const double* d_pos, long long np, hm_box_result* res) {
+    if (!ctx) return -1;
+    if (!plan || !d_pos) return bad(ctx, "hm_box_plan_execute: bad arguments");
+    try {
+        return hm::plan_execute_any(ctx, plan->impl, d_pos, np, res);
+    } catch (const std::exception& e) {
+        ctx->set_error(std::string("hm_box_plan_execute: ") + e.what());
+        return -1;
+    }
+}
+
+size_t hm_box_plan_bytes(const hm_box_plan* plan) { return plan ? hm::plan_bytes(plan->impl) : 0; }
+
+int hm_box_plan_triple_mesh(const hm_box_plan* plan, int m[3]) {
+    if (!plan || !m) return -1;
+    hm::plan_triple_mesh(plan->impl, m);
+    return 0;
+}
+
+}  // extern "C"

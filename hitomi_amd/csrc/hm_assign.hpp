@@ -1,0 +1,76 @@
+// hm_assign.hpp -- NGP/CIC/TSC mass assignment with 64-bit fixed-point cell
+// sums (DensityFieldClass::calcField, field.hpp:47-184).
+//
+// Each particle adds round(w * Wx*Wy*Wz * 2^s) to int64 cells with device
+// atomics.  Integer addition is associative, so the grid is bitwise
+// identical for any particle order or schedule; 2^s is the largest power of
+// two with sum|w| * 2^s < 2^62, so no cell can overflow.
+#pragma once
+
+#include "hm_internal.hpp"
+
+namespace hm {
+
+struct AssignGeom {
+    int n[3];
+    double L[3];
+    long long ng;
+    int assign;
+};
+
+// Stencil of one axis, FP64 exactly as field.hpp:84-86, 121-126, 162-169
+// (no contraction: the reference is built for x86-64 without FMA).
+__device__ __forceinline__ int stencil_axis(int assign, double xp, int* iw0, double* w) {
+#pragma clang fp contract(off)
+    if (assign == 0) {
+        *iw0 = (int)(xp + 0.5);
+        w[0] = 1.0;
+        return 1;
+    } else if (assign == 1) {
+        const int i0 = (int)(xp);
+        const double dx = xp - (double)i0;
+        w[0] = 1.0 - dx;
+        w[1] = dx;
+        *iw0 = i0;
+        return 2;
+    } else {
+        const int i1 = (int)(xp + 0.5);
+        const double dx = xp - (double)i1;
+        w[0] = 0.5 * (0.5 - dx) * (0.5 - dx);
+        w[1] = 0.75 - dx * dx;
+        w[2] = 0.5 * (0.5 + dx) * (0.5 + dx);
+        *iw0 = i1 - 1;
+        return 3;
+    }
+}
+
+__device__ __forceinline__ double axis_xp(double pos, int n, double L) {
+#pragma clang fp contract(off)
+    return pos * (double)n / L;  // field.hpp:162
+}
+
+// Fixed-point scale 2^s for a weight bound.
+double fixed_scale_for(double wabs_sum);
+
+// Real source for the r2c z-pass reading the fixed-point grid:
+// value = ((q * 2^-s) * (1/dV) - nbar) * dV, i.e. the box density fluctuation
+// of field.hpp:505-520 times dV as calcFourierTransform applies it
+// (field.hpp:313-318).
+template <typename T> struct FixedRealSrc {
+    const long long* g;
+    int L;  // half row length (complex pairs per row)
+    double inv_scale, factor, nbar, dV;
+    __device__ inline C<T> operator()(long long r, int m) const {
+#pragma clang fp contract(off)
+        const long long* q = g + r * (2 * (long long)L) + 2 * m;
+        const double a = (((double)q[0] * inv_scale) * factor - nbar) * dV;
+        const double b = (((double)q[1] * inv_scale) * factor - nbar) * dV;
+        return mk<T>((T)a, (T)b);
+    }
+};
+
+// Accumulate particles into a zeroed fixed-point grid (re and optional im).
+int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
+                 double scale, long long* d_re, long long* d_im);
+
+}  // namespace hm

@@ -1,0 +1,911 @@
+// hm_box.hip -- periodic-box P_l and B_000 on one MI355X.
+//
+// Pipeline for one mock (reference: calcPowerSpectrumForBOX,
+// powerspec.hpp:273-332, and calcBiSpectrumForBOX, bispec.hpp:877-1203,
+// for every ith_kbin at once):
+//   1. fixed-point TSC/CIC/NGP assignment into an int64 mesh;
+//   2. r2c 3D FFT whose first pass converts the fixed-point mesh to
+//      (rho - nbar) dV on the fly;
+//   3. P_l binning of the half spectrum over the low-k cube only, with the
+//      reference's fine (1e-4) histogram and coarse fold, FP64 sums;
+//   4. shot noise of B: xi_sn = IFFT[(|d|^2 - N S)/W^2 / V] (even part, the
+//      odd part sums to zero against the isotropic Bessel kernel), grouped
+//      by exact |n|^2, then sum_n2 j0(k1 r) j0(k2 r) H(n2);
+//   5. the 20 shell fields and delta/W inverse-transformed on the smallest
+//      mesh M >= 4 s + 1 that holds the triple product without aliasing
+//      (s = largest |n_a| in any shell), with pruned passes;
+//   6. all 210 triple sums sum_x F_a F_b F_3 in one pass (FP32 products,
+//      FP64 accumulation per 64 voxels, fixed-order block reduction).
+#include "hm_assign.hpp"
+#include "hm_fft_launch.hpp"
+
+#include "../../include/hitomi_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hm {
+
+constexpr int FINE_N = 100000;      // field.hpp:775
+constexpr double FINE_DK = 0.0001;  // field.hpp:776
+
+// ---------------------------------------------------------------------------
+// per-axis FP64 tables of the N-mesh: k component (field.hpp:797-799),
+// assignment-window factor and shot-noise factor with the reference's index
+// truncation (field.hpp:700-716, 1293-1309).
+// ---------------------------------------------------------------------------
+struct AxisTabs {
+    const double* kv[3];
+    const double* wf[3];
+    const double* sf[3];
+};
+
+static void host_axis_tables(const Mesh& m, std::vector<double>& out, long long off[3]) {
+    const long long ntot = (long long)m.n[0] + m.n[1] + m.n[2];
+    out.assign(3 * ntot, 0.0);
+    long long o = 0;
+    for (int a = 0; a < 3; a++) {
+        off[a] = o;
+        const int n = m.n[a];
+        const double dk = 2.0 * M_PI / m.L[a];
+        for (int i = 0; i < n; i++) {
+            double kv = (i < n / 2) ? (double)i * dk : (double)(i - n) * dk;
+            int ii = (int)(kv / dk + 1.0e-5);
+            double x = M_PI * (double)ii / (double)n;
+            double f = (ii != 0) ? std::sin(x) / x : 1.0;
+            double s = (ii != 0) ? std::sin(x) : 0.0;
+            double sfac;
+            if (m.assign == 0) sfac = 1.0;
+            else if (m.assign == 1) sfac = 1.0 - (2.0 / 3.0) * s * s;
+            else sfac = 1.0 - s * s + 2.0 * std::pow(s, 4) / 15.0;
+            out[o + i] = kv;
+            out[ntot + o + i] = f;
+            out[2 * ntot + o + i] = sfac;
+        }
+        o += n;
+    }
+}
+
+__device__ __forceinline__ double wpow(double r, int p) { return p == 1 ? r : (p == 2 ? r * r : r * r * r); }
+
+// |k| exactly as field.hpp:800 evaluates it (no contraction)
+__device__ __forceinline__ double kmag3(double a, double b, double c) {
+#pragma clang fp contract(off)
+    return sqrt(a * a + b * b + c * c);
+}
+
+// Legendre P_l(mu) by the three-term recurrence (the m = 0 reduced
+// spherical harmonic of tools.hpp:12-68 is sphPlm(l,0,mu) sqrt(4pi/(2l+1))).
+__device__ __forceinline__ double legendre(int l, double mu) {
+    if (l == 0) return 1.0;
+    double p0 = 1.0, p1 = mu;
+    for (int k = 2; k <= l; k++) {
+        double p2 = ((2.0 * k - 1.0) * mu * p1 - (k - 1.0) * p0) / (double)k;
+        p0 = p1;
+        p1 = p2;
+    }
+    return p1;
+}
+
+// ---------------------------------------------------------------------------
+// P_l binning (calcPowerSpectrum, field.hpp:765-852) on the half spectrum
+// ---------------------------------------------------------------------------
+struct BinArgs {
+    int n[3], nzh;
+    int s01;          // support on x, y (-1: all)
+    int nrow_y;       // compact count along y
+    long long nrows;
+    int nzmax;        // last kz index visited
+    AxisTabs t;
+    int wp;           // window power
+    const int2* f2b;  // fine index -> [lo, hi] coarse bins
+    int nk, ne;
+    int ells[HM_MAX_ELLS];
+    double sn;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, BinArgs a, double* __restrict__ part,
+                                                long long* __restrict__ cnt) {
+#pragma clang fp contract(off)
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.nrows) return;
+    const int i = support_map((int)(row / a.nrow_y), a.s01, a.n[0]);
+    const int j = support_map((int)(row % a.nrow_y), a.s01, a.n[1]);
+    double* P = part + row * (long long)a.nk * a.ne;
+    long long* Cn = cnt + row * (long long)a.nk;
+    const long long rbase = ((long long)i * a.n[1] + j) * a.nzh;
+    const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
+    for (int kz = 0; kz <= a.nzmax; kz++) {
+        const C<T> d = dk[rbase + kz];
+        const double re = (double)d.x, imv = (double)d.y;
+        const double p2 = re * re + imv * imv;  // Re f1 conj(f2), f1 = f2
+        const int nmir = (kz > 0 && kz < a.n[2] - kz) ? 2 : 1;
+        for (int m = 0; m < nmir; m++) {
+            const int ii = m ? im : i, jj = m ? jm : j, kk = m ? (a.n[2] - kz) : kz;
+            const double kx = a.t.kv[0][ii], ky = a.t.kv[1][jj], kzv = a.t.kv[2][kk];
+            const double kmag = kmag3(kx, ky, kzv);
+            const int kindex = (int)(kmag / FINE_DK + 0.5);
+            if (kindex >= FINE_N) continue;
+            const int2 b = a.f2b[kindex];
+            if (b.x < 0) continue;
+            const double S = (a.t.sf[0][ii] * a.t.sf[1][jj]) * a.t.sf[2][kk];
+            const double W = wpow((a.t.wf[0][ii] * a.t.wf[1][jj]) * a.t.wf[2][kk], a.wp);
+            const double val = (p2 - a.sn * S) / (W * W);
+            const double mu = (fabs(kmag) < 1.0e-15) ? 0.0 : kzv / kmag;
+            for (int bb = b.x; bb <= b.y; bb++) {
+                for (int e = 0; e < a.ne; e++) {
+                    const int l = a.ells[e];
+                    const double y = (l == 0) ? 1.0 : ((fabs(kmag) < 1.0e-15) ? 0.0 : legendre(l, mu));
+                    P[bb * a.ne + e] += val * y;
+                }
+                Cn[bb] += 1;
+            }
+        }
+    }
+}
+
+// Sum rows of a [nrows][ncols] array per column in a fixed order.
+template <typename V>
+__global__ __launch_bounds__(256) void k_reduce_cols(const V* __restrict__ part, long long nrows, int ncols,
+                                                    V* __restrict__ out) {
+    __shared__ V sm[256];
+    const int c = blockIdx.x;
+    V acc = 0;
+    for (long long r = threadIdx.x; r < nrows; r += 256) acc += part[r * ncols + c];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sm[0];
+}
+
+template <typename V> static int reduce_cols(hm_ctx* ctx, const char* name, const V* part, long long nrows, int ncols, V* out) {
+    if (ncols <= 0) return 0;
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL(k_reduce_cols<V>, dim3((unsigned)ncols), dim3(256), 0, ctx->stream, part, nrows, ncols, out);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// shell mode counts n_mode (field.hpp:354-384) on the full N mesh, exact
+// ---------------------------------------------------------------------------
+struct CountArgs {
+    int n[3];
+    int s[3];
+    int c[3];  // compact extents
+    long long total;
+    const double* kv[3];
+    const double* lower;
+    const double* upper;
+    int nk;
+};
+
+__global__ __launch_bounds__(256) void k_shell_count(CountArgs a, unsigned long long* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ unsigned int sc[64];
+    if (threadIdx.x < 64) sc[threadIdx.x] = 0;
+    __syncthreads();
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.total) {
+        const int kc = (int)(t % a.c[2]);
+        const long long r = t / a.c[2];
+        const int jc = (int)(r % a.c[1]);
+        const int ic = (int)(r / a.c[1]);
+        const int i = support_map(ic, a.s[0], a.n[0]);
+        const int j = support_map(jc, a.s[1], a.n[1]);
+        const int k = support_map(kc, a.s[2], a.n[2]);
+        const double kmag = kmag3(a.kv[0][i], a.kv[1][j], a.kv[2][k]);
+        for (int b = 0; b < a.nk; b++)
+            if (kmag > a.lower[b] && kmag <= a.upper[b]) atomicAdd(&sc[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < a.nk && sc[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)sc[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// spectrum source of the first (x) pass of a triple-product field on the
+// M mesh: value = [shell] * delta(k) / W(k) (field.hpp:371-382) read from the
+// N-mesh half spectrum, W with the reference's truncated index (Q2).
+// ---------------------------------------------------------------------------
+template <typename T> struct ShellSrc {
+    const C<T>* dk;
+    int n[3], nzh;
+    int M[3];
+    AxisTabs t;
+    int wp;
+    int shell;  // 1: keep modes with lower < |k| <= upper; 0: all modes (delta/W)
+    double lower, upper;
+    __device__ inline int to_n(int m, int a) const {
+        if (M[a] == n[a]) return m;
+        const int sgn = (m <= M[a] / 2) ? m : m - M[a];
+        return sgn >= 0 ? sgn : sgn + n[a];
+    }
+    // o = y, b = z, p = x on the M mesh
+    __device__ inline C<T> operator()(int o, int b, int p, long long) const {
+#pragma clang fp contract(off)
+        const int i = to_n(p, 0), j = to_n(o, 1), k = to_n(b, 2);
+        if (shell) {
+            const double kmag = kmag3(t.kv[0][i], t.kv[1][j], t.kv[2][k]);
+            if (!(kmag > lower && kmag <= upper)) return mk<T>((T)0, (T)0);
+        }
+        C<T> d;
+        if (k < nzh) {
+            d = dk[((long long)i * n[1] + j) * nzh + k];
+        } else {
+            const int im = (n[0] - i) % n[0], jm = (n[1] - j) % n[1];
+            d = cconj<T>(dk[((long long)im * n[1] + jm) * nzh + (n[2] - k)]);
+        }
+        const double W = wpow((t.wf[0][i] * t.wf[1][j]) * t.wf[2][k], wp);
+        return mk<T>((T)((double)d.x / W), (T)((double)d.y / W));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// shot-noise field source: even part of g(k) = (Re d conj(n) - N S)/W^2 / V
+// (calcShotNoiseForBispectrum_ijk, field.hpp:1116-1151), n(k) = d(k) + N [k=0]
+// ---------------------------------------------------------------------------
+template <typename T> struct XiSrc {
+    const C<T>* dk;
+    int n[3], nzh;
+    AxisTabs t;
+    int wp;
+    double sn, inv_v;
+    // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
+    __device__ inline C<T> operator()(int, int b, int p, long long addr) const {
+#pragma clang fp contract(off)
+        const int i = p, j = b / nzh, k = b % nzh;
+        const C<T> d = dk[addr];
+        const double re = (double)d.x, im = (double)d.y;
+        double p2;
+        if (i == 0 && j == 0 && k == 0) p2 = re * (re + sn) + im * im;
+        else p2 = re * re + im * im;
+        const int ii = (n[0] - i) % n[0], jj = (n[1] - j) % n[1], kk = (n[2] - k) % n[2];
+        const double S1 = (t.sf[0][i] * t.sf[1][j]) * t.sf[2][k];
+        const double W1 = wpow((t.wf[0][i] * t.wf[1][j]) * t.wf[2][k], wp);
+        const double S2 = (t.sf[0][ii] * t.sf[1][jj]) * t.sf[2][kk];
+        const double W2 = wpow((t.wf[0][ii] * t.wf[1][jj]) * t.wf[2][kk], wp);
+        const double g1 = inv_v * ((p2 - sn * S1) / (W1 * W1));
+        const double g2 = inv_v * ((p2 - sn * S2) / (W2 * W2));
+        return mk<T>((T)(0.5 * (g1 + g2)), (T)0);
+    }
+};
+
+// |x| max over a real mesh, as the bit pattern of a non-negative double
+template <typename T>
+__global__ __launch_bounds__(256) void k_absmax(const T* __restrict__ x, long long n, unsigned long long* __restrict__ out) {
+    __shared__ double sm[256];
+    double m = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        m = fmax(m, fabs((double)x[i]));
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+// 2^(62 - e) with absmax * maxcount < 2^e
+__device__ __forceinline__ double hist_scale(unsigned long long bits, double maxcount) {
+    const double amax = __longlong_as_double((long long)bits);
+    int e = 0;
+    frexp(amax * maxcount * (1.0 + 1e-9) + 1e-300, &e);
+    if (e < -900) e = -900;  // an all-zero field: any finite scale
+    return ldexp(1.0, 62 - e);
+}
+
+// H[n2] += xi(x) for the minimum-image |n|^2 of every voxel (rvec rule of
+// bispec.hpp:1040-1043), in 64-bit fixed point.
+template <typename T>
+__global__ __launch_bounds__(256) void k_hist_n2(const T* __restrict__ xi, int n0, int n1, int n2,
+                                                const unsigned long long* __restrict__ amax, double maxcount,
+                                                unsigned long long* __restrict__ H) {
+    const double sc = hist_scale(*amax, maxcount);
+    const long long ng = (long long)n0 * n1 * n2;
+    for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < ng; v += (long long)gridDim.x * 256) {
+        const int k = (int)(v % n2);
+        const long long r = v / n2;
+        const int j = (int)(r % n1), i = (int)(r / n1);
+        const long long a = (i < n0 / 2) ? i : i - n0;
+        const long long b = (j < n1 / 2) ? j : j - n1;
+        const long long c = (k < n2 / 2) ? k : k - n2;
+        const long long key = a * a + b * b + c * c;
+        const long long q = __double2ll_rn((double)xi[v] * sc);
+        if (q) atomicAdd(H + key, (unsigned long long)q);
+    }
+}
+
+// J[a][n2] = j0(k_a sqrt(n2) dr) (bessel.hpp spline of j0 replaced by the
+// closed form sin x / x)
+__global__ void k_j0_table(const double* __restrict__ kb, int nk, long long nkey, double dr, double* __restrict__ J) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nk * nkey) return;
+    const int a = (int)(t / nkey);
+    const long long key = t % nkey;
+    const double x = kb[a] * (sqrt((double)key) * dr);
+    J[t] = (x == 0.0) ? 1.0 : sin(x) / x;
+}
+
+// S[a][b] = dV * sum_n2 J[a][n2] J[b][n2] H[n2], one block per pair (a <= b)
+__global__ __launch_bounds__(256) void k_sn_pairs(const double* __restrict__ J, long long nkey,
+                                                 const unsigned long long* __restrict__ H,
+                                                 const unsigned long long* __restrict__ amax, double maxcount,
+                                                 const short2* __restrict__ pairs, double dV, double* __restrict__ out) {
+    __shared__ double sm[256];
+    const short2 pr = pairs[blockIdx.x];
+    const double inv = 1.0 / hist_scale(*amax, maxcount);
+    const double* Ja = J + (long long)pr.x * nkey;
+    const double* Jb = J + (long long)pr.y * nkey;
+    double acc = 0.0;
+    for (long long key = threadIdx.x; key < nkey; key += 256) {
+        const long long h = (long long)H[key];
+        if (h) acc += Ja[key] * Jb[key] * ((double)h * inv);
+    }
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0] * dV;
+}
+
+// ---------------------------------------------------------------------------
+// triple sums G[a][b] = sum_x F_a(x) F_b(x) F_3(x) for all pairs a <= b
+// ---------------------------------------------------------------------------
+constexpr int GRAM_CH = 64;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_sym(const C<T>* __restrict__ fields, long long fstride, int nf,
+                                                 const C<T>* __restrict__ f3, long long nvox, int vpb,
+                                                 const short2* __restrict__ pairs, int npairs,
+                                                 double2* __restrict__ part) {
+    extern __shared__ unsigned char smraw[];
+    C<T>* f3s = reinterpret_cast<C<T>*>(smraw);
+    C<T>* cs = f3s + GRAM_CH;                   // [nf][GRAM_CH + 1]: F_a F_3
+    C<T>* gs = cs + (size_t)nf * (GRAM_CH + 1);  // [nf][GRAM_CH + 1]: F_b
+    const int t = threadIdx.x;
+    const long long v0 = (long long)blockIdx.x * vpb;
+    const long long v1 = (v0 + vpb < nvox) ? v0 + vpb : nvox;
+    short2 pr = make_short2(0, 0);
+    if (t < npairs) pr = pairs[t];
+    double accr = 0.0, acci = 0.0;
+    for (long long c0 = v0; c0 < v1; c0 += GRAM_CH) {
+        if (t < GRAM_CH) {
+            const long long v = c0 + t;
+            f3s[t] = (v < v1) ? f3[v] : mk<T>((T)0, (T)0);
+        }
+        __syncthreads();
+        for (int e = t; e < nf * GRAM_CH; e += 256) {
+            const int f = e / GRAM_CH, x = e % GRAM_CH;
+            const long long v = c0 + x;
+            const C<T> g = (v < v1) ? fields[(long long)f * fstride + v] : mk<T>((T)0, (T)0);
+            gs[f * (GRAM_CH + 1) + x] = g;
+            cs[f * (GRAM_CH + 1) + x] = cmul<T>(g, f3s[x]);
+        }
+        __syncthreads();
+        if (t < npairs) {
+            T ar = 0, ai = 0;
+            const C<T>* ca = cs + pr.x * (GRAM_CH + 1);
+            const C<T>* gb = gs + pr.y * (GRAM_CH + 1);
+#pragma unroll 8
+            for (int x = 0; x < GRAM_CH; x++) {
+                const C<T> c = ca[x], g = gb[x];
+                ar += c.x * g.x - c.y * g.y;
+                ai += c.x * g.y + c.y * g.x;
+            }
+            accr += (double)ar;
+            acci += (double)ai;
+        }
+        __syncthreads();
+    }
+    if (t < npairs) part[(long long)blockIdx.x * npairs + t] = make_double2(accr, acci);
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct BoxPlanImpl {
+    hm_box_config cfg;
+    Mesh mesh;
+    int nk = 0, ne = 0;
+    std::vector<double> kbin, lower, upper;
+    std::vector<int2> f2b;
+    long long tab_off[3];
+    int s[3];   // shell support (signed |n| bound) per axis, -1 = all
+    int s3[3];  // support of delta/W on the triple mesh
+    int M[3];
+    int nzmax = 0, s01 = -1, nrow_y = 0;
+    long long nrows = 0;
+    long long nkey = 0;   // |n|^2 keys
+    double maxcount = 0;  // most voxels sharing one |n|^2
+    std::vector<short2> pairs;
+    std::vector<long long> shell_nmode;
+    bool counted = false;
+    size_t bytes = 0;
+    DevBuf tab, f2b_d, lu_d, kb_d, fixed, dk, pk_part, pk_cnt, pk_red, pk_cnt_red, xi_half, xi_real, hist, amax, jtab,
+        pairs_d, sn_pairs, fields, gram_part, gram_red, shell_cnt;
+    long long gram_blocks = 0;
+    int gram_vpb = 4096;
+};
+
+static int alloc(hm_ctx* ctx, BoxPlanImpl* p, DevBuf& b, size_t bytes) {
+    if (dev_alloc(ctx, b, bytes)) return -1;
+    p->bytes += bytes;
+    return 0;
+}
+
+template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
+    const Mesh& m = p->mesh;
+    if (alloc(ctx, p, p->fixed, sizeof(long long) * (size_t)m.ng())) return -1;
+    if (alloc(ctx, p, p->dk, sizeof(C<T>) * (size_t)m.nh())) return -1;
+    const size_t ncols = (size_t)p->nk * p->ne;
+    if (alloc(ctx, p, p->pk_part, sizeof(double) * ncols * (size_t)p->nrows)) return -1;
+    if (alloc(ctx, p, p->pk_cnt, sizeof(long long) * (size_t)p->nk * (size_t)p->nrows)) return -1;
+    if (alloc(ctx, p, p->pk_red, sizeof(double) * ncols)) return -1;
+    if (alloc(ctx, p, p->pk_cnt_red, sizeof(long long) * (size_t)p->nk)) return -1;
+    if (p->cfg.bk) {
+        if (alloc(ctx, p, p->xi_half, sizeof(C<T>) * (size_t)m.nh())) return -1;
+        if (alloc(ctx, p, p->xi_real, sizeof(T) * (size_t)m.ng())) return -1;
+        if (alloc(ctx, p, p->hist, sizeof(unsigned long long) * (size_t)p->nkey)) return -1;
+        if (alloc(ctx, p, p->amax, sizeof(unsigned long long))) return -1;
+        if (alloc(ctx, p, p->jtab, sizeof(double) * (size_t)p->nk * (size_t)p->nkey)) return -1;
+        if (alloc(ctx, p, p->pairs_d, sizeof(short2) * p->pairs.size())) return -1;
+        if (alloc(ctx, p, p->sn_pairs, sizeof(double) * p->pairs.size())) return -1;
+        const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
+        if (alloc(ctx, p, p->fields, sizeof(C<T>) * (size_t)nvox * (size_t)(p->nk + 1))) return -1;
+        p->gram_blocks = (nvox + p->gram_vpb - 1) / p->gram_vpb;
+        if (alloc(ctx, p, p->gram_part, sizeof(double2) * (size_t)p->gram_blocks * p->pairs.size())) return -1;
+        if (alloc(ctx, p, p->gram_red, sizeof(double2) * p->pairs.size())) return -1;
+    }
+    if (alloc(ctx, p, p->shell_cnt, sizeof(unsigned long long) * (size_t)p->nk)) return -1;
+    return 0;
+}
+
+static AxisTabs axis_tabs(const BoxPlanImpl* p) {
+    AxisTabs t;
+    const double* base = reinterpret_cast<const double*>(p->tab.p);
+    const long long ntot = (long long)p->mesh.n[0] + p->mesh.n[1] + p->mesh.n[2];
+    for (int a = 0; a < 3; a++) {
+        t.kv[a] = base + p->tab_off[a];
+        t.wf[a] = base + ntot + p->tab_off[a];
+        t.sf[a] = base + 2 * ntot + p->tab_off[a];
+    }
+    return t;
+}
+
+static int wpow_of(int assign) { return assign == 0 ? 1 : (assign == 1 ? 2 : 3); }
+
+BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
+    auto* p = new BoxPlanImpl();
+    p->cfg = *cfg;
+    Mesh& m = p->mesh;
+    for (int a = 0; a < 3; a++) {
+        m.n[a] = cfg->mesh.n_mesh[a];
+        m.L[a] = cfg->mesh.boxsize[a];
+    }
+    m.assign = cfg->mesh.assign;
+    auto fail = [&](const std::string& s) -> BoxPlanImpl* {
+        ctx->set_error(s);
+        delete p;
+        return nullptr;
+    };
+    for (int a = 0; a < 3; a++) {
+        if (!fft_length_supported(m.n[a]) || (m.n[a] & (m.n[a] - 1)))
+            return fail("box plan: n_mesh must be a power of two in [8, 1024]");
+        if (!(m.L[a] > 0)) return fail("box plan: boxsize must be > 0");
+    }
+    if (m.n[2] < 16) return fail("box plan: n_mesh_z must be >= 16");
+    if (m.assign < 0 || m.assign > 2) return fail("box plan: assign must be NGP, CIC or TSC");
+    if (cfg->n_kbin < 2 || cfg->n_kbin > 64) return fail("box plan: n_kbin must be in [2, 64]");
+    if (cfg->n_pk_ells < 0 || cfg->n_pk_ells > HM_MAX_ELLS) return fail("box plan: n_pk_ells out of range");
+    p->nk = cfg->n_kbin;
+    p->ne = cfg->n_pk_ells;
+    // k bins, ToolClass::setKbin (tools.hpp:118-124)
+    p->kbin.resize(p->nk);
+    const double dkb = (cfg->kmax - cfg->kmin) / (double)(p->nk - 1);
+    for (int i = 0; i < p->nk; i++) p->kbin[i] = cfg->kmin + dkb * (double)i;
+    const double dk_bin = p->kbin[1] - p->kbin[0];
+    p->lower.resize(p->nk);
+    p->upper.resize(p->nk);
+    for (int j = 0; j < p->nk; j++) {
+        p->lower[j] = (p->kbin[j] > dk_bin * 0.5) ? (p->kbin[j] - dk_bin * 0.5) : 0.0;  // field.hpp:828, 371
+        p->upper[j] = p->kbin[j] + dk_bin * 0.5;
+    }
+    // fine -> coarse fold (field.hpp:827-837)
+    p->f2b.assign(FINE_N, make_int2(-1, -2));
+    for (int i = 0; i < FINE_N; i++) {
+        const double kt = (double)i * FINE_DK;
+        int lo = -1, hi = -2;
+        for (int j = 0; j < p->nk; j++)
+            if ((kt > p->lower[j]) && (kt <= p->upper[j])) {
+                if (lo < 0) lo = j;
+                else if (j != hi + 1) return fail("box plan: non-contiguous k-bin overlap");
+                hi = j;
+            }
+        p->f2b[i] = make_int2(lo, hi);
+    }
+    // supports: every mode whose fine centre can land in a bin
+    double umax = 0;
+    for (int j = 0; j < p->nk; j++) umax = std::max(umax, p->upper[j]);
+    for (int a = 0; a < 3; a++) {
+        const double dk = 2.0 * M_PI / m.L[a];
+        long long s = (long long)std::floor((umax + FINE_DK) / dk) + 1;
+        p->s[a] = (!cfg->prune || 2 * s + 1 >= m.n[a] || s >= m.n[a] / 2) ? -1 : (int)s;
+    }
+    // P binning rows: x, y in the support, kz up to the support
+    p->s01 = (p->s[0] < 0 || p->s[1] < 0) ? -1 : std::max(p->s[0], p->s[1]);
+    if (p->s01 >= 0 && (2 * p->s01 + 1 >= m.n[0] || 2 * p->s01 + 1 >= m.n[1])) p->s01 = -1;
+    const int nrx = support_count(p->s01, m.n[0]);
+    p->nrow_y = support_count(p->s01, m.n[1]);
+    p->nrows = (long long)nrx * p->nrow_y;
+    p->nzmax = (p->s[2] < 0) ? m.n[2] / 2 : std::min(p->s[2], m.n[2] / 2);
+    // triple-product mesh
+    for (int a = 0; a < 3; a++) {
+        p->M[a] = m.n[a];
+        p->s3[a] = -1;
+        if (cfg->regrid && p->s[a] >= 0) {
+            const int want = fft_length_at_least(4 * p->s[a] + 1);
+            if (want > 0 && want < m.n[a]) {
+                p->M[a] = want;
+                p->s3[a] = 2 * p->s[a];
+            }
+        }
+    }
+    // pairs a <= b
+    for (int a = 0; a < p->nk; a++)
+        for (int b = a; b < p->nk; b++) p->pairs.push_back(make_short2((short)a, (short)b));
+    if (cfg->bk && p->pairs.size() > 256) return fail("box plan: at most 22 k-bins for the bispectrum");
+    if (cfg->bk) {
+        bool has0 = false;
+        for (int e = 0; e < p->ne; e++) has0 = has0 || (cfg->pk_ells[e] == 0);
+        if (!has0) return fail("box plan: the bispectrum shot noise needs l = 0 in pk_ells");
+    }
+    if (cfg->bk) {
+        const double dr0 = m.L[0] / m.n[0], dr1 = m.L[1] / m.n[1], dr2 = m.L[2] / m.n[2];
+        if (!(dr0 == dr1 && dr1 == dr2)) return fail("box plan: the bispectrum needs equal cell sizes on all axes");
+        const long long h0 = m.n[0] / 2, h1 = m.n[1] / 2, h2 = m.n[2] / 2;
+        p->nkey = h0 * h0 + h1 * h1 + h2 * h2 + 1;
+        // largest number of voxels sharing one |n|^2
+        std::vector<unsigned int> cxy((size_t)(h0 * h0 + h1 * h1 + 1), 0);
+        for (int i = 0; i < m.n[0]; i++) {
+            long long a = (i < m.n[0] / 2) ? i : i - m.n[0];
+            for (int j = 0; j < m.n[1]; j++) {
+                long long b = (j < m.n[1] / 2) ? j : j - m.n[1];
+                cxy[(size_t)(a * a + b * b)]++;
+            }
+        }
+        std::vector<unsigned long long> c3((size_t)p->nkey, 0);
+        for (size_t u = 0; u < cxy.size(); u++) {
+            if (!cxy[u]) continue;
+            for (int k = 0; k < m.n[2]; k++) {
+                long long c = (k < m.n[2] / 2) ? k : k - m.n[2];
+                c3[u + (size_t)(c * c)] += cxy[u];
+            }
+        }
+        unsigned long long mx = 0;
+        for (auto v : c3) mx = std::max(mx, v);
+        p->maxcount = (double)mx;
+    }
+    // device tables
+    std::vector<double> tabs;
+    host_axis_tables(m, tabs, p->tab_off);
+    if (alloc(ctx, p, p->tab, sizeof(double) * tabs.size())) { delete p; return nullptr; }
+    if (alloc(ctx, p, p->f2b_d, sizeof(int2) * FINE_N)) { delete p; return nullptr; }
+    if (alloc(ctx, p, p->lu_d, sizeof(double) * 2 * p->nk)) { delete p; return nullptr; }
+    if (alloc(ctx, p, p->kb_d, sizeof(double) * p->nk)) { delete p; return nullptr; }
+    std::vector<double> lu(p->lower);
+    lu.insert(lu.end(), p->upper.begin(), p->upper.end());
+    hipError_t e = hipSuccess;
+    e = (hipError_t)(e | hipMemcpy(p->tab.p, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice));
+    e = (hipError_t)(e | hipMemcpy(p->f2b_d.p, p->f2b.data(), sizeof(int2) * FINE_N, hipMemcpyHostToDevice));
+    e = (hipError_t)(e | hipMemcpy(p->lu_d.p, lu.data(), sizeof(double) * lu.size(), hipMemcpyHostToDevice));
+    e = (hipError_t)(e | hipMemcpy(p->kb_d.p, p->kbin.data(), sizeof(double) * p->nk, hipMemcpyHostToDevice));
+    if (e != hipSuccess) return fail("box plan: table upload failed");
+    int rc = cfg->precision == HM_FP64 ? plan_alloc<double>(ctx, p) : plan_alloc<float>(ctx, p);
+    if (rc) { delete p; return nullptr; }
+    if (cfg->bk) {
+        if (hipMemcpy(p->pairs_d.p, p->pairs.data(), sizeof(short2) * p->pairs.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail("box plan: pair table upload failed");
+        const double dr = m.L[0] / m.n[0];
+        const long long tot = (long long)p->nk * p->nkey;
+        if (ctx->launch("j0_table", [&] {
+                hipLaunchKernelGGL(k_j0_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   (const double*)p->kb_d.p, p->nk, p->nkey, dr, (double*)p->jtab.p);
+            }))
+        { delete p; return nullptr; }
+    }
+    // shell mode counts on the N mesh (data independent)
+    {
+        CountArgs ca;
+        for (int a = 0; a < 3; a++) {
+            ca.n[a] = m.n[a];
+            ca.s[a] = p->s[a];
+            ca.c[a] = support_count(p->s[a], m.n[a]);
+        }
+        ca.total = (long long)ca.c[0] * ca.c[1] * ca.c[2];
+        AxisTabs t = axis_tabs(p);
+        for (int a = 0; a < 3; a++) ca.kv[a] = t.kv[a];
+        ca.lower = (const double*)p->lu_d.p;
+        ca.upper = ca.lower + p->nk;
+        ca.nk = p->nk;
+        if (fill_zero(ctx, p->shell_cnt.p, sizeof(unsigned long long) * p->nk)) { delete p; return nullptr; }
+        if (ctx->launch("shell_count", [&] {
+                hipLaunchKernelGGL(k_shell_count, dim3((unsigned)((ca.total + 255) / 256)), dim3(256), 0, ctx->stream, ca,
+                                   (unsigned long long*)p->shell_cnt.p);
+            }))
+        { delete p; return nullptr; }
+        p->shell_nmode.resize(p->nk);
+        if (hipMemcpyAsync(p->shell_nmode.data(), p->shell_cnt.p, sizeof(long long) * p->nk, hipMemcpyDeviceToHost,
+                           ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return fail("box plan: shell count readback failed");
+    }
+    return p;
+}
+
+void plan_destroy(BoxPlanImpl* p) {
+    if (!p) return;
+    DevBuf* bufs[] = {&p->tab, &p->f2b_d, &p->lu_d, &p->kb_d, &p->fixed, &p->dk, &p->pk_part, &p->pk_cnt, &p->pk_red,
+                      &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->amax, &p->jtab, &p->pairs_d,
+                      &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt};
+    for (DevBuf* b : bufs) dev_free(*b);
+    delete p;
+}
+
+static ColGeom geom(long long OS, long long PS, int n_outer, int outer_full, int outer_s, int n_inner, int inner_full,
+                    int inner_s, int point_s) {
+    ColGeom g;
+    g.OS = OS;
+    g.PS = PS;
+    g.n_outer = n_outer;
+    g.outer_full = outer_full;
+    g.outer_s = outer_s;
+    g.n_inner = n_inner;
+    g.inner_full = inner_full;
+    g.inner_s = inner_s;
+    g.point_s = point_s;
+    return g;
+}
+
+// Inverse transform of one triple-product field on the M mesh:
+// x pass from the spectrum source, then y columns, then z rows.
+template <typename T>
+static int field_transform(hm_ctx* ctx, BoxPlanImpl* p, const ShellSrc<T>& src, C<T>* out, const int sup[3]) {
+    const int* M = p->M;
+    const long long M12 = (long long)M[1] * M[2];
+    ColGeom ga = geom(M[2], M12, support_count(sup[1], M[1]), M[1], sup[1] >= 0 && 2 * sup[1] + 1 < M[1] ? sup[1] : -1,
+                      support_count(sup[2], M[2]), M[2], sup[2] >= 0 && 2 * sup[2] + 1 < M[2] ? sup[2] : -1,
+                      sup[0] >= 0 && 2 * sup[0] + 1 < M[0] ? sup[0] : -1);
+    if (launch_cols<T, +1>(ctx, "tri_pass_x", M[0], src, out, ga)) return -1;
+    ColGeom gb = geom(M12, M[2], M[0], M[0], -1, support_count(sup[2], M[2]), M[2],
+                      sup[2] >= 0 && 2 * sup[2] + 1 < M[2] ? sup[2] : -1, sup[1] >= 0 && 2 * sup[1] + 1 < M[1] ? sup[1] : -1);
+    if (fft_cols<T>(ctx, "tri_pass_y", M[1], +1, out, out, gb)) return -1;
+    RowGeom rg{(long long)M[0] * M[1], M[2], M[2], sup[2] >= 0 && 2 * sup[2] + 1 < M[2] ? sup[2] : -1};
+    return fft_rows<T>(ctx, "tri_pass_z", M[2], +1, out, out, rg);
+}
+
+template <typename T>
+static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res) {
+    const Mesh& m = p->mesh;
+    const AxisTabs tabs = axis_tabs(p);
+    const int wp = wpow_of(m.assign);
+    if (np <= 0) {
+        ctx->set_error("box execute: no particles");
+        return -1;
+    }
+    // 1. assignment (field.hpp:145-184, 505-523)
+    AssignGeom ag;
+    for (int a = 0; a < 3; a++) {
+        ag.n[a] = m.n[a];
+        ag.L[a] = m.L[a];
+    }
+    ag.ng = m.ng();
+    ag.assign = m.assign;
+    const double scale = fixed_scale_for((double)np);
+    if (fill_zero(ctx, p->fixed.p, sizeof(long long) * (size_t)m.ng())) return -1;
+    if (assign_fixed(ctx, ag, d_pos, nullptr, np, scale, (long long*)p->fixed.p, nullptr)) return -1;
+    // 2. forward r2c with the fluctuation (rho - N/V) dV formed on load
+    const double dV = m.volume() / (double)m.ng();
+    FixedRealSrc<T> fsrc{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale, 1.0 / dV, (double)np / m.volume(), dV};
+    C<T>* dk = (C<T>*)p->dk.p;
+    const long long nzh = m.nzh();
+    if (launch_r2c<T>(ctx, "fft_r2c_z", m.n[2] / 2, fsrc, dk, (long long)m.n[0] * m.n[1])) return -1;
+    if (fft_cols<T>(ctx, "fft_r2c_y", m.n[1], -1, dk, dk,
+                    geom((long long)m.n[1] * nzh, nzh, m.n[0], m.n[0], -1, (int)nzh, (int)nzh, -1, -1)))
+        return -1;
+    if (fft_cols<T>(ctx, "fft_r2c_x", m.n[0], -1, dk, dk,
+                    geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
+        return -1;
+    // 3. P_l binning (field.hpp:765-852)
+    BinArgs ba;
+    for (int a = 0; a < 3; a++) ba.n[a] = m.n[a];
+    ba.nzh = (int)nzh;
+    ba.s01 = p->s01;
+    ba.nrow_y = p->nrow_y;
+    ba.nrows = p->nrows;
+    ba.nzmax = p->nzmax;
+    ba.t = tabs;
+    ba.wp = wp;
+    ba.f2b = (const int2*)p->f2b_d.p;
+    ba.nk = p->nk;
+    ba.ne = std::max(p->ne, 1);
+    for (int e = 0; e < HM_MAX_ELLS; e++) ba.ells[e] = (e < p->ne) ? p->cfg.pk_ells[e] : 0;
+    if (p->ne == 0) ba.ells[0] = 0;
+    ba.sn = (double)np;
+    const int ncols = p->nk * ba.ne;
+    if (fill_zero(ctx, p->pk_part.p, sizeof(double) * (size_t)ncols * p->nrows)) return -1;
+    if (fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows)) return -1;
+    if (ctx->launch("bin_pk", [&] {
+            hipLaunchKernelGGL(k_bin_box<T>, dim3((unsigned)((p->nrows + 255) / 256)), dim3(256), 0, ctx->stream, dk, ba,
+                               (double*)p->pk_part.p, (long long*)p->pk_cnt.p);
+        }))
+        return -1;
+    if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->pk_part.p, p->nrows, ncols, (double*)p->pk_red.p))
+        return -1;
+    if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->pk_cnt.p, p->nrows, p->nk,
+                               (long long*)p->pk_cnt_red.p))
+        return -1;
+    const int npairs = (int)p->pairs.size();
+    if (p->cfg.bk) {
+        // 4. shot-noise ijk term (field.hpp:1116-1160, bispec.hpp:1023-1056)
+        XiSrc<T> xs;
+        xs.dk = dk;
+        for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
+        xs.nzh = (int)nzh;
+        xs.t = tabs;
+        xs.wp = wp;
+        xs.sn = (double)np;
+        xs.inv_v = 1.0 / m.volume();
+        C<T>* xh = (C<T>*)p->xi_half.p;
+        if (launch_cols<T, +1>(ctx, "xi_pass_x", m.n[0], xs, xh,
+                               geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
+            return -1;
+        if (fft_cols<T>(ctx, "xi_pass_y", m.n[1], +1, xh, xh,
+                        geom((long long)m.n[1] * nzh, nzh, m.n[0], m.n[0], -1, (int)nzh, (int)nzh, -1, -1)))
+            return -1;
+        T* xr = (T*)p->xi_real.p;
+        if (fft_c2r_rows<T>(ctx, "xi_pass_z", m.n[2] / 2, xh, xr, (long long)m.n[0] * m.n[1], -1)) return -1;
+        if (fill_zero(ctx, p->amax.p, sizeof(unsigned long long))) return -1;
+        if (fill_zero(ctx, p->hist.p, sizeof(unsigned long long) * (size_t)p->nkey)) return -1;
+        const long long ng = m.ng();
+        const unsigned hb = (unsigned)std::min<long long>((ng + 255) / 256, 16384);
+        if (ctx->launch("xi_absmax", [&] {
+                hipLaunchKernelGGL(k_absmax<T>, dim3(hb), dim3(256), 0, ctx->stream, xr, ng,
+                                   (unsigned long long*)p->amax.p);
+            }))
+            return -1;
+        if (ctx->launch("xi_hist", [&] {
+                hipLaunchKernelGGL(k_hist_n2<T>, dim3(hb), dim3(256), 0, ctx->stream, xr, m.n[0], m.n[1], m.n[2],
+                                   (const unsigned long long*)p->amax.p, p->maxcount, (unsigned long long*)p->hist.p);
+            }))
+            return -1;
+        if (ctx->launch("sn_pairs", [&] {
+                hipLaunchKernelGGL(k_sn_pairs, dim3((unsigned)npairs), dim3(256), 0, ctx->stream,
+                                   (const double*)p->jtab.p, p->nkey, (const unsigned long long*)p->hist.p,
+                                   (const unsigned long long*)p->amax.p, p->maxcount, (const short2*)p->pairs_d.p, dV,
+                                   (double*)p->sn_pairs.p);
+            }))
+            return -1;
+        // 5. shell fields and delta/W on the triple mesh (field.hpp:346-401, 610-639, 328-344)
+        const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
+        C<T>* F = (C<T>*)p->fields.p;
+        ShellSrc<T> ss;
+        ss.dk = dk;
+        for (int a = 0; a < 3; a++) {
+            ss.n[a] = m.n[a];
+            ss.M[a] = p->M[a];
+        }
+        ss.nzh = (int)nzh;
+        ss.t = tabs;
+        ss.wp = wp;
+        for (int b = 0; b < p->nk; b++) {
+            ss.shell = 1;
+            ss.lower = p->lower[b];
+            ss.upper = p->upper[b];
+            if (field_transform<T>(ctx, p, ss, F + (long long)b * nvox, p->s)) return -1;
+        }
+        ss.shell = 0;
+        if (field_transform<T>(ctx, p, ss, F + (long long)p->nk * nvox, p->s3)) return -1;
+        // 6. triple sums
+        const size_t lds = sizeof(C<T>) * (GRAM_CH + 2 * (size_t)p->nk * (GRAM_CH + 1));
+        if (ctx->launch("gram", [&] {
+                hipLaunchKernelGGL(k_gram_sym<T>, dim3((unsigned)p->gram_blocks), dim3(256), lds, ctx->stream, F, nvox,
+                                   p->nk, F + (long long)p->nk * nvox, nvox, p->gram_vpb, (const short2*)p->pairs_d.p,
+                                   npairs, (double2*)p->gram_part.p);
+            }))
+            return -1;
+        if (reduce_cols<double>(ctx, "gram_reduce", (const double*)p->gram_part.p, p->gram_blocks, 2 * npairs,
+                                (double*)p->gram_red.p))
+            return -1;
+    }
+    // 7. results to host
+    std::vector<double> pk(ncols), gram(2 * (size_t)npairs), snp((size_t)npairs);
+    std::vector<long long> cnt(p->nk);
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(pk.data(), p->pk_red.p, sizeof(double) * ncols, hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), p->pk_cnt_red.p, sizeof(long long) * p->nk, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    if (p->cfg.bk) {
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(gram.data(), p->gram_red.p, sizeof(double) * 2 * npairs, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(snp.data(), p->sn_pairs.p, sizeof(double) * npairs, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+    }
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int nk = p->nk;
+    const double V = m.volume(), N = (double)np;
+    // P_l: average (field.hpp:839-845), (2l+1) and V/N^2 (powerspec.hpp:306-316)
+    std::vector<double> praw((size_t)ba.ne * nk);
+    for (int e = 0; e < ba.ne; e++)
+        for (int b = 0; b < nk; b++) praw[(size_t)e * nk + b] = cnt[b] ? pk[(size_t)b * ba.ne + e] / (double)cnt[b] : 0.0;
+    if (res) {
+        const double norm = V / N / N;
+        for (int e = 0; e < p->ne; e++)
+            for (int b = 0; b < nk; b++) {
+                const double v = praw[(size_t)e * nk + b];
+                if (res->pk_raw) {
+                    res->pk_raw[2 * ((size_t)e * nk + b)] = v;
+                    res->pk_raw[2 * ((size_t)e * nk + b) + 1] = 0.0;
+                }
+                if (res->pk) {
+                    res->pk[2 * ((size_t)e * nk + b)] = norm * (0.0 + (double)(2 * p->cfg.pk_ells[e] + 1) * v);
+                    res->pk[2 * ((size_t)e * nk + b) + 1] = 0.0;
+                }
+            }
+        if (res->pk_nmode)
+            for (int b = 0; b < nk; b++) res->pk_nmode[b] = cnt[b];
+        if (res->shell_nmode)
+            for (int b = 0; b < nk; b++) res->shell_nmode[b] = p->shell_nmode[b];
+    }
+    if (p->cfg.bk && res) {
+        // P_raw of the monopole: calcPowerSpectrum(dn_00_sn, N_LM_sn, ..., 0, 0) (bispec.hpp:946, 953)
+        int e0 = -1;
+        for (int e = 0; e < p->ne; e++)
+            if (p->cfg.pk_ells[e] == 0) e0 = e;
+        std::vector<double> p0(nk, 0.0);
+        if (e0 >= 0)
+            for (int b = 0; b < nk; b++) p0[b] = praw[(size_t)e0 * nk + b];
+        const long long M3 = (long long)p->M[0] * p->M[1] * p->M[2];
+        const double norm = (V / N / N) * (V / N);
+        for (int q = 0; q < npairs; q++) {
+            const int a = p->pairs[q].x, b = p->pairs[q].y;
+            const double den = (double)M3 * (double)p->shell_nmode[a] * (double)p->shell_nmode[b];
+            const double br = gram[2 * q] / den, bi = gram[2 * q + 1] / den;
+            const double sr = N + p0[a] + p0[b] + snp[q];
+            for (int sym = 0; sym < 2; sym++) {
+                const int r = sym ? b : a, c = sym ? a : b;
+                const size_t o = 2 * ((size_t)r * nk + c);
+                if (res->bk_sum) {
+                    res->bk_sum[o] = br;
+                    res->bk_sum[o + 1] = bi;
+                }
+                if (res->sn_sum) {
+                    res->sn_sum[o] = sr;
+                    res->sn_sum[o + 1] = 0.0;
+                }
+                if (res->bk) {
+                    res->bk[o] = norm * (br - sr);
+                    res->bk[o + 1] = norm * (bi - 0.0);
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res) {
+    return p->cfg.precision == HM_FP64 ? plan_execute<double>(ctx, p, d_pos, np, res)
+                                       : plan_execute<float>(ctx, p, d_pos, np, res);
+}
+
+size_t plan_bytes(const BoxPlanImpl* p) { return p->bytes; }
+void plan_triple_mesh(const BoxPlanImpl* p, int m[3]) {
+    for (int a = 0; a < 3; a++) m[a] = p->M[a];
+}
+
+}  // namespace hm

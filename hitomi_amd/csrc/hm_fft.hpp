@@ -1,0 +1,371 @@
+// hm_fft.hpp -- LDS-staged Stockham FFT building blocks for gfx950.
+//
+// A workgroup (256 lanes = 4 wave64) loads a tile of B lines of length L
+// into LDS, runs the Stockham autosort stages for a compile-time radix plan
+// (radices 2, 3, 4, 5, 8), and stores the tile back.  Two tile shapes:
+//   * columns: lines whose points are PS elements apart while B consecutive
+//     lines are contiguous in memory, so each point of the tile is one
+//     coalesced run of B elements (B*8 = 128 B for FP32, B*16 for FP64);
+//   * rows: B contiguous lines.
+// Points outside a "support" |n| <= s (n the signed index) are taken as zero
+// without being read: this prunes the passes of band-limited inverse
+// transforms.
+//
+// Sign convention is FFTW's (fftw_plan_dft_3d as called at field.hpp:321 and
+// field.hpp:339): DIR = -1 forward exp(-2 pi i nk/L), DIR = +1 backward,
+// unnormalised.
+#pragma once
+
+#include "hm_common.hpp"
+
+namespace hm {
+
+constexpr int FFT_NT = 256;  // lanes per workgroup
+
+// ---------------------------------------------------------------------------
+// small DFTs, y_u = sum_t a_t exp(DIR 2 pi i t u / R)
+// ---------------------------------------------------------------------------
+template <typename T, int R, int DIR> struct Dft;
+
+template <typename T, int DIR> struct Dft<T, 2, DIR> {
+    __device__ static inline void run(C<T>* a) {
+        C<T> t = a[1];
+        a[1] = csub<T>(a[0], t);
+        a[0] = cadd<T>(a[0], t);
+    }
+};
+
+template <typename T, int DIR> struct Dft<T, 4, DIR> {
+    __device__ static inline void run(C<T>* a) {
+        C<T> t0 = cadd<T>(a[0], a[2]), t1 = csub<T>(a[0], a[2]);
+        C<T> t2 = cadd<T>(a[1], a[3]), t3 = cmuli<T>(csub<T>(a[1], a[3]), DIR);
+        a[0] = cadd<T>(t0, t2);
+        a[2] = csub<T>(t0, t2);
+        a[1] = cadd<T>(t1, t3);
+        a[3] = csub<T>(t1, t3);
+    }
+};
+
+template <typename T, int DIR> struct Dft<T, 8, DIR> {
+    __device__ static inline void run(C<T>* a) {
+        const T h = (T)0.70710678118654752440;
+        C<T> e[4] = {a[0], a[2], a[4], a[6]};
+        C<T> o[4] = {a[1], a[3], a[5], a[7]};
+        Dft<T, 4, DIR>::run(e);
+        Dft<T, 4, DIR>::run(o);
+        // w^1 = h (1 + DIR i), w^2 = DIR i, w^3 = h (-1 + DIR i)
+        C<T> o1 = mk<T>(h * (o[1].x - (T)DIR * o[1].y), h * (o[1].y + (T)DIR * o[1].x));
+        C<T> o2 = cmuli<T>(o[2], DIR);
+        C<T> o3 = mk<T>(h * (-o[3].x - (T)DIR * o[3].y), h * (-o[3].y + (T)DIR * o[3].x));
+        a[0] = cadd<T>(e[0], o[0]);
+        a[4] = csub<T>(e[0], o[0]);
+        a[1] = cadd<T>(e[1], o1);
+        a[5] = csub<T>(e[1], o1);
+        a[2] = cadd<T>(e[2], o2);
+        a[6] = csub<T>(e[2], o2);
+        a[3] = cadd<T>(e[3], o3);
+        a[7] = csub<T>(e[3], o3);
+    }
+};
+
+template <typename T, int DIR> struct Dft<T, 3, DIR> {
+    __device__ static inline void run(C<T>* a) {
+        const T s3 = (T)0.86602540378443864676;
+        C<T> t = cadd<T>(a[1], a[2]);
+        C<T> d = csub<T>(a[1], a[2]);
+        C<T> m = mk<T>(a[0].x - (T)0.5 * t.x, a[0].y - (T)0.5 * t.y);
+        C<T> r = cmuli<T>(cscale<T>(d, s3), DIR);
+        a[0] = cadd<T>(a[0], t);
+        a[1] = cadd<T>(m, r);
+        a[2] = csub<T>(m, r);
+    }
+};
+
+template <typename T, int DIR> struct Dft<T, 5, DIR> {
+    __device__ static inline void run(C<T>* a) {
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+        C<T> t1 = cadd<T>(a[1], a[4]), t2 = cadd<T>(a[2], a[3]);
+        C<T> d1 = csub<T>(a[1], a[4]), d2 = csub<T>(a[2], a[3]);
+        C<T> m1 = mk<T>(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+        C<T> m2 = mk<T>(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+        C<T> r1 = cmuli<T>(mk<T>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y), DIR);
+        C<T> r2 = cmuli<T>(mk<T>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y), DIR);
+        a[0] = cadd<T>(a[0], cadd<T>(t1, t2));
+        a[1] = cadd<T>(m1, r1);
+        a[4] = csub<T>(m1, r1);
+        a[2] = cadd<T>(m2, r2);
+        a[3] = csub<T>(m2, r2);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Stockham stages over a tile in LDS: sm[b * LP + p], b < B, p < L
+// ---------------------------------------------------------------------------
+template <int L> struct LdsPad { static constexpr int LP = L + 1; };
+
+template <typename T, int DIR, int L, int B, int NS, int R, int... RS>
+__device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ tw) {
+    constexpr int LP = LdsPad<L>::LP;
+    constexpr int NBF = L / R;
+    constexpr int TOT = B * NBF;
+    constexpr int Q = (TOT + FFT_NT - 1) / FFT_NT;
+    const int tid = threadIdx.x;
+    C<T> v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int idx = tid + q * FFT_NT;
+        if ((TOT % FFT_NT == 0) || idx < TOT) {
+            const int b = idx / NBF, j = idx % NBF;
+            const int k = j % NS;
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                C<T> a = sm[b * LP + j + t * NBF];
+                if (NS > 1 && t > 0) {
+                    const int e = t * k * (L / (NS * R));
+                    C<T> w = tw[e];
+                    if (DIR > 0) w.y = -w.y;
+                    a = cmul<T>(a, w);
+                }
+                v[q][t] = a;
+            }
+            Dft<T, R, DIR>::run(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int idx = tid + q * FFT_NT;
+        if ((TOT % FFT_NT == 0) || idx < TOT) {
+            const int b = idx / NBF, j = idx % NBF;
+            const int k = j % NS;
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int t = 0; t < R; t++) sm[b * LP + base + t * NS] = v[q][t];
+        }
+    }
+    __syncthreads();
+    if constexpr (sizeof...(RS) > 0) lds_stockham<T, DIR, L, B, NS * R, RS...>(sm, tw);
+}
+
+// Radix plans of the supported line lengths.
+template <typename T, int DIR, int L, int B>
+__device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw) {
+    if constexpr (L == 8) lds_stockham<T, DIR, L, B, 1, 8>(sm, tw);
+    else if constexpr (L == 16) lds_stockham<T, DIR, L, B, 1, 4, 4>(sm, tw);
+    else if constexpr (L == 32) lds_stockham<T, DIR, L, B, 1, 8, 4>(sm, tw);
+    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, 1, 8, 2, 3>(sm, tw);
+    else if constexpr (L == 64) lds_stockham<T, DIR, L, B, 1, 8, 8>(sm, tw);
+    else if constexpr (L == 80) lds_stockham<T, DIR, L, B, 1, 8, 2, 5>(sm, tw);
+    else if constexpr (L == 96) lds_stockham<T, DIR, L, B, 1, 8, 4, 3>(sm, tw);
+    else if constexpr (L == 128) lds_stockham<T, DIR, L, B, 1, 8, 4, 4>(sm, tw);
+    else if constexpr (L == 160) lds_stockham<T, DIR, L, B, 1, 8, 4, 5>(sm, tw);
+    else if constexpr (L == 192) lds_stockham<T, DIR, L, B, 1, 8, 8, 3>(sm, tw);
+    else if constexpr (L == 256) lds_stockham<T, DIR, L, B, 1, 8, 8, 4>(sm, tw);
+    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, 1, 8, 4, 3, 3>(sm, tw);
+    else if constexpr (L == 320) lds_stockham<T, DIR, L, B, 1, 8, 8, 5>(sm, tw);
+    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, 1, 8, 8, 2, 3>(sm, tw);
+    else if constexpr (L == 480) lds_stockham<T, DIR, L, B, 1, 8, 4, 3, 5>(sm, tw);
+    else if constexpr (L == 512) lds_stockham<T, DIR, L, B, 1, 8, 8, 8>(sm, tw);
+    else if constexpr (L == 576) lds_stockham<T, DIR, L, B, 1, 8, 8, 3, 3>(sm, tw);
+    else if constexpr (L == 640) lds_stockham<T, DIR, L, B, 1, 8, 8, 2, 5>(sm, tw);
+    else if constexpr (L == 768) lds_stockham<T, DIR, L, B, 1, 8, 8, 4, 3>(sm, tw);
+    else if constexpr (L == 1024) lds_stockham<T, DIR, L, B, 1, 8, 8, 4, 4>(sm, tw);
+    else static_assert(L == 8, "unsupported FFT length");
+}
+
+// Lengths with a radix plan, ascending.
+static constexpr int kFftLengths[] = {8, 16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 288, 320, 384, 480, 512, 576, 640, 768, 1024};
+
+constexpr int pow2floor(int x) {
+    int r = 1;
+    while (2 * r <= x) r *= 2;
+    return r;
+}
+
+// Tile width (lines per workgroup) for a column pass: up to 128 B per point,
+// a power of two, within 64 KB of LDS.
+template <typename T, int L> struct ColTile {
+    static constexpr int want = (int)(128 / sizeof(C<T>));
+    static constexpr int fit = (int)(65536 / ((L + 1) * sizeof(C<T>)));
+    static constexpr int B = pow2floor(want < fit ? want : fit);
+};
+template <typename T, int L> struct RowTile {
+    static constexpr int want = (4096 / L) > 0 ? (4096 / L) : 1;
+    static constexpr int fit = (int)(65536 / ((L + 1) * sizeof(C<T>)));
+    static constexpr int B = pow2floor(want < fit ? want : fit);
+};
+
+// compact index -> actual index of a signed support |n| <= s on a length-n axis
+__host__ __device__ inline int support_map(int c, int s, int n) {
+    if (s < 0) return c;
+    return (c <= s) ? c : (n - (2 * s + 1) + c);
+}
+__host__ __device__ inline int support_count(int s, int n) { return (s < 0 || 2 * s + 1 >= n) ? n : 2 * s + 1; }
+__host__ __device__ inline bool in_support(int p, int s, int n) { return s < 0 || p <= s || p >= n - s; }
+
+// Column-pass geometry: element (o, b, p) at o*OS + b + p*PS, where o and b
+// are the actual outer/inner line indices.
+struct ColGeom {
+    long long OS, PS;
+    int n_outer, outer_full, outer_s;  // compact count, full extent, support
+    int n_inner, inner_full, inner_s;
+    int point_s;                       // support on the transformed axis
+};
+
+// Row-pass geometry: element (r, p) at r*RS + p.
+struct RowGeom {
+    long long n_rows, RS_in, RS_out;
+    int point_s;
+};
+
+// Plain global source / destination.
+template <typename T> struct GSrc {
+    const C<T>* p;
+    __device__ inline C<T> operator()(int /*o*/, int /*b*/, int /*pt*/, long long addr) const { return p[addr]; }
+};
+
+// Column FFT: grid.x = tiles of B compact inner lines, grid.y = compact outer index.
+template <typename T, int L, int DIR, class Src>
+__global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__ dst, ColGeom g,
+                                                    const C<T>* __restrict__ tw) {
+    constexpr int B = ColTile<T, L>::B;
+    constexpr int LP = LdsPad<L>::LP;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const int o = support_map(blockIdx.y, g.outer_s, g.outer_full);
+    const long long obase = (long long)o * g.OS;
+    const int tid = threadIdx.x;
+    const int bl = tid % B;
+    const int bc = blockIdx.x * B + bl;  // compact inner index
+    const bool bvalid = bc < g.n_inner;
+    const int b = bvalid ? support_map(bc, g.inner_s, g.inner_full) : 0;
+    constexpr int PSTEP = FFT_NT / B;
+    for (int p = tid / B; p < L; p += PSTEP) {
+        C<T> v = mk<T>((T)0, (T)0);
+        if (bvalid && in_support(p, g.point_s, L)) v = src(o, b, p, obase + b + (long long)p * g.PS);
+        sm[bl * LP + p] = v;
+    }
+    __syncthreads();
+    lds_fft<T, DIR, L, B>(sm, tw);
+    if (bvalid) {
+        for (int p = tid / B; p < L; p += PSTEP) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + p];
+    }
+}
+
+// Row FFT over contiguous lines (complex in, complex out).
+template <typename T, int L, int DIR>
+__global__ __launch_bounds__(FFT_NT) void k_fft_rows(const C<T>* __restrict__ src, C<T>* __restrict__ dst, RowGeom g,
+                                                    const C<T>* __restrict__ tw) {
+    constexpr int B = RowTile<T, L>::B;
+    constexpr int LP = LdsPad<L>::LP;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < g.n_rows && in_support(p, g.point_s, L)) v = src[(r0 + b) * g.RS_in + p];
+        sm[b * LP + p] = v;
+    }
+    __syncthreads();
+    lds_fft<T, DIR, L, B>(sm, tw);
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        if (r0 + b < g.n_rows) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + p];
+    }
+}
+
+// Real-to-half-complex rows: N = 2L reals per row -> L+1 complex.
+// RSrc(row, m) returns the complex pair (x[2m], x[2m+1]) of a row.
+// tw2 holds exp(-2 pi i k / (2L)), k < 2L.
+template <typename T, int L, class RSrc>
+__global__ __launch_bounds__(FFT_NT) void k_r2c_rows(RSrc src, C<T>* __restrict__ dst, long long n_rows,
+                                                    const C<T>* __restrict__ tw, const C<T>* __restrict__ tw2) {
+    constexpr int B = RowTile<T, L>::B;
+    constexpr int LP = LdsPad<L>::LP;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < n_rows) v = src(r0 + b, p);
+        sm[b * LP + p] = v;
+    }
+    __syncthreads();
+    lds_fft<T, -1, L, B>(sm, tw);
+    // X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[L-k]) / 2, O = (Z[k] - conj Z[L-k]) / (2i)
+    for (int e = threadIdx.x; e < B * (L + 1); e += FFT_NT) {
+        const int b = e / (L + 1), k = e % (L + 1);
+        if (r0 + b >= n_rows) continue;
+        const C<T> zk = sm[b * LP + (k % L)];
+        const C<T> zc = cconj<T>(sm[b * LP + ((L - k) % L)]);
+        const C<T> E = mk<T>((T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y));
+        const C<T> d = mk<T>((T)0.5 * (zk.x - zc.x), (T)0.5 * (zk.y - zc.y));
+        const C<T> O = mk<T>(d.y, -d.x);  // d / i
+        const C<T> w = tw2[k];
+        dst[(r0 + b) * (L + 1) + k] = cadd<T>(E, cmul<T>(w, O));
+    }
+}
+
+// Half-complex-to-real rows (unnormalised backward): L+1 complex -> 2L reals.
+// Z'[k] = (X[k] + conj X[L-k]) + i exp(+2 pi i k / 2L) (X[k] - conj X[L-k]).
+template <typename T, int L, class HSrc>
+__global__ __launch_bounds__(FFT_NT) void k_c2r_rows(HSrc src, T* __restrict__ dst, long long n_rows, int point_s,
+                                                    const C<T>* __restrict__ tw, const C<T>* __restrict__ tw2) {
+    constexpr int B = RowTile<T, L>::B;
+    constexpr int LP = LdsPad<L>::LP;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, k = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < n_rows) {
+            const C<T> xk = (point_s < 0 || k <= point_s) ? src(r0 + b, k) : mk<T>((T)0, (T)0);
+            const int kc = L - k;
+            const C<T> xc = (point_s < 0 || kc <= point_s) ? cconj<T>(src(r0 + b, kc)) : mk<T>((T)0, (T)0);
+            const C<T> s = cadd<T>(xk, xc);
+            const C<T> d = csub<T>(xk, xc);
+            C<T> w = tw2[k];
+            w.y = -w.y;  // exp(+2 pi i k / 2L)
+            const C<T> wd = cmul<T>(w, d);
+            v = mk<T>(s.x - wd.y, s.y + wd.x);  // s + i * wd
+        }
+        sm[b * LP + k] = v;
+    }
+    __syncthreads();
+    lds_fft<T, +1, L, B>(sm, tw);
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, m = e % L;
+        if (r0 + b < n_rows) {
+            const C<T> z = sm[b * LP + m];
+            dst[(r0 + b) * (2 * L) + 2 * m] = z.x;
+            dst[(r0 + b) * (2 * L) + 2 * m + 1] = z.y;
+        }
+    }
+}
+
+// Half-spectrum source for the c2r rows: row r holds L+1 complex at r*(L+1).
+template <typename T> struct HalfSrc {
+    const C<T>* p;
+    int L;
+    __device__ inline C<T> operator()(long long r, int k) const { return p[r * (L + 1) + k]; }
+};
+
+// Real rows source for the r2c rows: row r holds 2L reals.
+template <typename T> struct RealSrc {
+    const T* p;
+    int L;
+    __device__ inline C<T> operator()(long long r, int m) const {
+        const C<T>* q = reinterpret_cast<const C<T>*>(p + r * (2 * (long long)L));
+        return q[m];
+    }
+};
+
+// LDS bytes of the three tile shapes.
+template <typename T, int L> constexpr size_t col_lds() { return (size_t)ColTile<T, L>::B * (L + 1) * sizeof(C<T>); }
+template <typename T, int L> constexpr size_t row_lds() { return (size_t)RowTile<T, L>::B * (L + 1) * sizeof(C<T>); }
+
+}  // namespace hm

@@ -1,0 +1,126 @@
+// hm_fft_launch.hpp -- host-side dispatch of the LDS Stockham kernels over
+// the supported line lengths (compile-time radix plans).
+#pragma once
+
+#include "hm_internal.hpp"
+
+#define HM_POW2_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+#define HM_MIXED_LENGTHS(X) X(48) X(80) X(96) X(160) X(192) X(288) X(320) X(384) X(480) X(576) X(640) X(768)
+
+namespace hm {
+
+template <typename T, int L, int DIR, class Src>
+int launch_cols_L(hm_ctx* ctx, const char* name, const Src& src, C<T>* dst, const ColGeom& g) {
+    constexpr int B = ColTile<T, L>::B;
+    if (g.n_inner <= 0 || g.n_outer <= 0) return 0;
+    if (g.n_outer > 65535) {
+        ctx->set_error("fft column pass: outer extent exceeds 65535");
+        return -1;
+    }
+    dim3 grid((unsigned)((g.n_inner + B - 1) / B), (unsigned)g.n_outer);
+    const size_t lds = col_lds<T, L>();
+    const C<T>* tw = ctx->twiddle<T>(L);
+    if (!tw) return -1;
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw);
+    });
+}
+
+template <typename T, int DIR, class Src>
+int launch_cols(hm_ctx* ctx, const char* name, int L, const Src& src, C<T>* dst, const ColGeom& g) {
+    switch (L) {
+#define HM_CASE(LL) \
+    case LL: return launch_cols_L<T, LL, DIR, Src>(ctx, name, src, dst, g);
+        HM_POW2_LENGTHS(HM_CASE)
+        HM_MIXED_LENGTHS(HM_CASE)
+#undef HM_CASE
+    default: break;
+    }
+    ctx->set_error("fft: unsupported line length " + std::to_string(L));
+    return -1;
+}
+
+template <typename T, int L, int DIR>
+int launch_rows_L(hm_ctx* ctx, const char* name, const C<T>* src, C<T>* dst, const RowGeom& g) {
+    constexpr int B = RowTile<T, L>::B;
+    if (g.n_rows <= 0) return 0;
+    dim3 grid((unsigned)((g.n_rows + B - 1) / B));
+    const size_t lds = row_lds<T, L>();
+    const C<T>* tw = ctx->twiddle<T>(L);
+    if (!tw) return -1;
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_fft_rows<T, L, DIR>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw);
+    });
+}
+
+template <typename T, int DIR>
+int launch_rows(hm_ctx* ctx, const char* name, int L, const C<T>* src, C<T>* dst, const RowGeom& g) {
+    switch (L) {
+#define HM_CASE(LL) \
+    case LL: return launch_rows_L<T, LL, DIR>(ctx, name, src, dst, g);
+        HM_POW2_LENGTHS(HM_CASE)
+        HM_MIXED_LENGTHS(HM_CASE)
+#undef HM_CASE
+    default: break;
+    }
+    ctx->set_error("fft: unsupported line length " + std::to_string(L));
+    return -1;
+}
+
+// r2c rows of N = 2*L2 reals.
+template <typename T, int L2, class RSrc>
+int launch_r2c_L(hm_ctx* ctx, const char* name, const RSrc& src, C<T>* dst, long long n_rows) {
+    constexpr int B = RowTile<T, L2>::B;
+    if (n_rows <= 0) return 0;
+    dim3 grid((unsigned)((n_rows + B - 1) / B));
+    const size_t lds = row_lds<T, L2>();
+    const C<T>* tw = ctx->twiddle<T>(L2);
+    const C<T>* tw2 = ctx->twiddle<T>(2 * L2);
+    if (!tw || !tw2) return -1;
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_r2c_rows<T, L2, RSrc>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, n_rows, tw, tw2);
+    });
+}
+
+template <typename T, class RSrc>
+int launch_r2c(hm_ctx* ctx, const char* name, int L2, const RSrc& src, C<T>* dst, long long n_rows) {
+    switch (L2) {
+#define HM_CASE(LL) \
+    case LL: return launch_r2c_L<T, LL, RSrc>(ctx, name, src, dst, n_rows);
+        HM_POW2_LENGTHS(HM_CASE)
+#undef HM_CASE
+    default: break;
+    }
+    ctx->set_error("fft r2c: unsupported half length " + std::to_string(L2));
+    return -1;
+}
+
+template <typename T, int L2, class HSrc>
+int launch_c2r_L(hm_ctx* ctx, const char* name, const HSrc& src, T* dst, long long n_rows, int point_s) {
+    constexpr int B = RowTile<T, L2>::B;
+    if (n_rows <= 0) return 0;
+    dim3 grid((unsigned)((n_rows + B - 1) / B));
+    const size_t lds = row_lds<T, L2>();
+    const C<T>* tw = ctx->twiddle<T>(L2);
+    const C<T>* tw2 = ctx->twiddle<T>(2 * L2);
+    if (!tw || !tw2) return -1;
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_c2r_rows<T, L2, HSrc>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, n_rows, point_s,
+                           tw, tw2);
+    });
+}
+
+template <typename T, class HSrc>
+int launch_c2r(hm_ctx* ctx, const char* name, int L2, const HSrc& src, T* dst, long long n_rows, int point_s) {
+    switch (L2) {
+#define HM_CASE(LL) \
+    case LL: return launch_c2r_L<T, LL, HSrc>(ctx, name, src, dst, n_rows, point_s);
+        HM_POW2_LENGTHS(HM_CASE)
+#undef HM_CASE
+    default: break;
+    }
+    ctx->set_error("fft c2r: unsupported half length " + std::to_string(L2));
+    return -1;
+}
+
+}  // namespace hm

@@ -1,0 +1,64 @@
+// hm_internal.hpp -- context, workspace and launch plumbing shared by the
+// translation units of libhitomi_amd.so.  Nothing here crosses the C ABI.
+#pragma once
+
+#include "hm_common.hpp"
+#include "hm_fft.hpp"
+
+#include <functional>
+#include <memory>
+
+struct hm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // twiddle tables exp(-2 pi i e / L), e < L, keyed by (L, sizeof(T))
+    std::map<std::pair<int, int>, void*> twiddles;
+    // kernel timing (hipEvents on the launch stream), off by default
+    bool timing = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, long long>> kstats;  // name -> (ms total, launches)
+    bool capturing = false;
+
+    void set_error(const std::string& m) { err = m; }
+    hipEvent_t get_event();
+    template <typename T> const hm::C<T>* twiddle(int L);
+    // Launch wrapper: records HIP events around the launch when timing is on.
+    int launch(const char* name, const std::function<void()>& fn);
+};
+
+namespace hm {
+
+// Device allocation owned by the library.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+int dev_alloc(hm_ctx* ctx, DevBuf& b, size_t bytes);
+void dev_free(DevBuf& b);
+
+// Zero-fill with a kernel (graph-capture safe; no hipMemsetAsync).
+int fill_zero(hm_ctx* ctx, void* p, size_t bytes);
+
+// 3D transforms on row-major grids (FFTW conventions, unnormalised).
+template <typename T> int fft3d_c2c(hm_ctx* ctx, C<T>* data, const int n[3], int dir);
+template <typename T> int fft3d_r2c(hm_ctx* ctx, const T* in, C<T>* out, const int n[3]);
+template <typename T> int fft3d_c2r(hm_ctx* ctx, C<T>* in, T* out, const int n[3]);
+
+// Column / row passes with a plain global source.
+template <typename T> int fft_cols(hm_ctx* ctx, const char* name, int L, int dir, const C<T>* src, C<T>* dst,
+                                  const ColGeom& g);
+template <typename T> int fft_rows(hm_ctx* ctx, const char* name, int L, int dir, const C<T>* src, C<T>* dst,
+                                  const RowGeom& g);
+template <typename T> int fft_c2r_rows(hm_ctx* ctx, const char* name, int L2, const C<T>* src, T* dst,
+                                      long long n_rows, int point_s);
+
+bool fft_length_supported(int L);
+int fft_length_at_least(int n);  // smallest supported length >= n, or -1
+
+}  // namespace hm

@@ -1,0 +1,134 @@
+/*
+ * hitomi_amd.h -- C ABI of libhitomi_amd.so, the MI355X (gfx950) kernels of
+ * hitomi_measurement's FFT estimator path.
+ *
+ * The reference has no FFI: its estimators are header-only C++ templates
+ * called from main_box.cpp / main.cpp (reference src/hitomi_measurement/).
+ * Each entry point below replaces one of those internal seams; the citation
+ * names the reference function (file:line) whose arithmetic it performs.
+ *
+ * Conventions
+ *  - Every call returns 0 on success and -1 on failure; hm_last_error()
+ *    then describes the failure.  No C++ exception crosses this boundary.
+ *  - Pointers named d_* are device pointers (from hm_dev_alloc); all other
+ *    pointers are host memory owned by the caller and only borrowed for the
+ *    duration of the call.
+ *  - Calls are ordered on the context's stream.  Calls that return host
+ *    results (hm_box_plan_execute, hm_memcpy_d2h) synchronise that stream.
+ *  - One host thread per context.  Grids are row-major [n0][n1][n2] with
+ *    the z index fastest, as DensityFieldClass (field.hpp:364) lays them.
+ *  - FFT sign and scale follow FFTW as the reference calls it
+ *    (field.hpp:321, field.hpp:339): sign -1 forward, +1 backward,
+ *    unnormalised.
+ */
+#ifndef HITOMI_AMD_H
+#define HITOMI_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hm_ctx hm_ctx;
+typedef struct hm_box_plan hm_box_plan;
+
+enum { HM_NGP = 0, HM_CIC = 1, HM_TSC = 2 };  /* parameter.hpp:533 "assignment" */
+enum { HM_FP32 = 0, HM_FP64 = 1 };            /* precision of the grids and FFTs */
+
+/* The mesh part of ParameterClass (parameter.hpp:13-16, 533). */
+typedef struct {
+    int n_mesh[3];
+    double boxsize[3];
+    int assign; /* HM_NGP, HM_CIC or HM_TSC */
+} hm_mesh;
+
+/* ---------------------------------------------------------------- */
+/* context and device memory                                          */
+/* ---------------------------------------------------------------- */
+hm_ctx *hm_ctx_create(int device, int flags);
+void hm_ctx_destroy(hm_ctx *ctx);
+const char *hm_last_error(const hm_ctx *ctx);
+int hm_sync(hm_ctx *ctx);
+void *hm_dev_alloc(hm_ctx *ctx, size_t bytes);
+int hm_dev_free(hm_ctx *ctx, void *d_ptr);
+int hm_memcpy_h2d(hm_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+int hm_memcpy_d2h(hm_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+int hm_memset_zero(hm_ctx *ctx, void *d_ptr, size_t bytes);
+
+/* Kernel timing with HIP events recorded on the launch stream around every
+ * launch (off by default).  hm_timing_get() synchronises and reports the
+ * accumulated time and launch count of the kernels launched under `name`. */
+int hm_timing_enable(hm_ctx *ctx, int on);
+int hm_timing_get(hm_ctx *ctx, const char *name, double *total_ms, long long *launches);
+int hm_timing_reset(hm_ctx *ctx);
+
+/* ---------------------------------------------------------------- */
+/* mass assignment                                                    */
+/* ---------------------------------------------------------------- */
+
+/* DensityFieldClass::calcField (field.hpp:47-184): rho(x) = sum_p w_p W(x-x_p)/dV
+ * with NGP/CIC/TSC, the reference's unwrapped stencil and linear range check.
+ * d_pos: np x 3 FP64 positions; d_w: np complex FP64 weights (re, im) or NULL
+ * for unit weights; wabs_sum >= sum_p (|re w_p| + |im w_p|) (<= 0: np).
+ * Output: FP64 grids (d_out_im may be NULL).  Cell sums are accumulated in
+ * 64-bit fixed point, so the result is bitwise independent of particle order. */
+int hm_assign(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const double *d_w, long long np,
+              double wabs_sum, double *d_out_re, double *d_out_im);
+
+/* The 1/8/27 stencil cell indices of each particle as field.hpp:84-93,
+ * 121-133, 162-176 compute them; -1 where the reference's range check
+ * drops the contribution.  d_coords: np * (1|8|27) int64. */
+int hm_cell_coords(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, long long np, long long *d_coords);
+
+/* ---------------------------------------------------------------- */
+/* FFT (replaces fftw_plan_dft_3d/fftw_execute, field.hpp:310-344)    */
+/* ---------------------------------------------------------------- */
+/* In-place complex 3D DFT of d_data ([n0][n1][n2] complex FP32/FP64). */
+int hm_fft_c2c(hm_ctx *ctx, int precision, void *d_data, const int n[3], int sign);
+/* Real [n0][n1][n2] -> half spectrum [n0][n1][n2/2+1], forward (sign -1). */
+int hm_fft_r2c(hm_ctx *ctx, int precision, const void *d_in, void *d_out, const int n[3]);
+/* Half spectrum -> real, backward (sign +1); d_in is overwritten. */
+int hm_fft_c2r(hm_ctx *ctx, int precision, void *d_in, void *d_out, const int n[3]);
+
+/* ---------------------------------------------------------------- */
+/* periodic-box estimators (main_box.cpp:171-228, measure = pk / bk)  */
+/* ---------------------------------------------------------------- */
+#define HM_MAX_ELLS 4
+
+typedef struct {
+    hm_mesh mesh;
+    int precision;             /* HM_FP32: FP32 grids/FFTs, FP64 bin sums; HM_FP64: FP64 throughout */
+    int n_kbin;                /* k-bins of ToolClass::setKbin (tools.hpp:118-124) */
+    double kmin, kmax;
+    int n_pk_ells;             /* P_l for l in pk_ells (calcPowerSpectrumForBOX, powerspec.hpp:273-332) */
+    int pk_ells[HM_MAX_ELLS];
+    int bk;                    /* 1: B_000 for every (ith_kbin, k2) pair (calcBiSpectrumForBOX, bispec.hpp:877-1203) */
+    int regrid;                /* 1: evaluate the triple product on the smallest mesh that holds it exactly */
+    int prune;                 /* 1: skip the zero lines of band-limited transforms */
+} hm_box_config;
+
+/* Host result arrays (any may be NULL). */
+typedef struct {
+    double *pk;            /* [n_pk_ells][n_kbin][2] norm*(2l+1)*P as written to pk%d.dat */
+    double *pk_raw;        /* [n_pk_ells][n_kbin][2] bin averages of calcPowerSpectrum (field.hpp:839-845) */
+    long long *pk_nmode;   /* [n_kbin] modes per bin */
+    double *bk;            /* [n_kbin][n_kbin][2] norm*(bk - sn), row i = file bk000_%02d.dat with ith_kbin = i */
+    double *bk_sum;        /* [n_kbin][n_kbin][2] bk_save (bispec.hpp:1168) */
+    double *sn_sum;        /* [n_kbin][n_kbin][2] sn_save (bispec.hpp:941-1056) */
+    long long *shell_nmode;/* [n_kbin] n_mode of calcInverseFourierTransformForBispectrum (field.hpp:383) */
+} hm_box_result;
+
+hm_box_plan *hm_box_plan_create(hm_ctx *ctx, const hm_box_config *cfg);
+void hm_box_plan_destroy(hm_ctx *ctx, hm_box_plan *plan);
+/* One mock: d_pos (np x 3 FP64, already periodic) -> P_l and B_000. */
+int hm_box_plan_execute(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos, long long np, hm_box_result *res);
+/* Device bytes the plan holds. */
+size_t hm_box_plan_bytes(const hm_box_plan *plan);
+/* Mesh used for the triple product (n_mesh when regrid is off). */
+int hm_box_plan_triple_mesh(const hm_box_plan *plan, int m[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1119 @@
+/*
+ * hm_oracle.c -- FP64 CPU restatement of hitomi_measurement's periodic-box
+ * estimator path.  TEST INFRASTRUCTURE ONLY (see hm_oracle.h): the checker
+ * for tests/, smoke() and bench.py's cpu_baseline, never part of the
+ * product.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no -march, so that the
+ * FP64 index arithmetic is evaluated exactly as the reference's plain
+ * `g++ -O3` x86-64 build evaluates it: no fused multiply-add).
+ *
+ * Citations are reference file:line under src/hitomi_measurement/.
+ */
+#include "hm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------ */
+/* parameters                                                           */
+/* ------------------------------------------------------------------ */
+
+void or_param_init(or_param *p, int n0, int n1, int n2, double L0, double L1, double L2, int assign) {
+    memset(p, 0, sizeof(*p));
+    p->n_mesh[0] = n0;
+    p->n_mesh[1] = n1;
+    p->n_mesh[2] = n2;
+    p->boxsize[0] = L0;
+    p->boxsize[1] = L1;
+    p->boxsize[2] = L2;
+    p->volume = L0 * L1 * L2;                       /* parameter.hpp:232 */
+    p->n_mesh_tot = (long long)n0 * n1 * n2;        /* parameter.hpp:233 (int there, Q8) */
+    p->assign = assign;
+    p->n_kbin = 1;
+    p->n_rbin = 1;
+}
+
+void or_set_kbin(const or_param *p, double *kbin) {
+    double dk = (p->kmax - p->kmin) / (double)(p->n_kbin - 1);
+    for (int i = 0; i < p->n_kbin; i++) kbin[i] = p->kmin + dk * (double)i;
+}
+
+void or_set_rbin(const or_param *p, double *rbin) {
+    double dr = (p->rmax - p->rmin) / (double)(p->n_rbin - 1);
+    for (int i = 0; i < p->n_rbin; i++) rbin[i] = p->rmin + dr * (double)i;
+}
+
+long long or_coord(const or_param *p, int i, int j, int k) {
+    return ((long long)i * p->n_mesh[1] + j) * p->n_mesh[2] + k;
+}
+
+/* ------------------------------------------------------------------ */
+/* mass assignment, field.hpp:47-184                                    */
+/* ------------------------------------------------------------------ */
+
+/* Per axis: the first stencil index and the weights (field.hpp:84-86,
+ * 121-126, 162-169). */
+static int stencil_axis(int assign, double xp, int *iw0, double *w) {
+    if (assign == OR_NGP) {
+        *iw0 = (int)(xp + 0.5);
+        w[0] = 1.0;
+        return 1;
+    } else if (assign == OR_CIC) {
+        int i0 = (int)(xp);
+        double dx = xp - (double)i0;
+        w[0] = 1.0 - dx;
+        w[1] = dx;
+        *iw0 = i0;
+        return 2;
+    } else {
+        int i1 = (int)(xp + 0.5);
+        double dx = xp - (double)i1;
+        w[0] = 0.5 * (0.5 - dx) * (0.5 - dx);
+        w[1] = 0.75 - dx * dx;
+        w[2] = 0.5 * (0.5 + dx) * (0.5 + dx);
+        *iw0 = i1 - 1;
+        return 3;
+    }
+}
+
+int or_calc_field(const or_param *p, const double *pos, const double *weight, long long np, double *field) {
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        field[2 * i] = 0.0;
+        field[2 * i + 1] = 0.0;
+    }
+    double dV = p->volume / (double)(p->n_mesh_tot);
+    double factor = 1.0 / dV;
+    for (long long q = 0; q < np; q++) {
+        double w[3][3];
+        int iw0[3];
+        int s = 1;
+        for (int axes = 0; axes < 3; axes++) {
+            double xp = pos[3 * q + axes] * (double)(p->n_mesh[axes]) / p->boxsize[axes];
+            s = stencil_axis(p->assign, xp, &iw0[axes], w[axes]);
+        }
+        for (int i = 0; i < s; i++)
+            for (int j = 0; j < s; j++)
+                for (int k = 0; k < s; k++) {
+                    /* field.hpp:175: the per-axis index is never wrapped (Q1);
+                     * only the linear coord is range-checked. */
+                    long long coord = ((long long)(iw0[0] + i) * p->n_mesh[1] + (iw0[1] + j)) * p->n_mesh[2] +
+                                      (iw0[2] + k);
+                    if ((coord >= 0) && (coord < p->n_mesh_tot)) {
+                        field[2 * coord] += weight[2 * q] * factor * w[0][i] * w[1][j] * w[2][k];
+                        field[2 * coord + 1] += weight[2 * q + 1] * factor * w[0][i] * w[1][j] * w[2][k];
+                    }
+                }
+    }
+    return 0;
+}
+
+int or_cell_coords(const or_param *p, const double *pos, long long np, long long *coords) {
+    int s = (p->assign == OR_NGP) ? 1 : (p->assign == OR_CIC ? 2 : 3);
+    int s3 = s * s * s;
+    for (long long q = 0; q < np; q++) {
+        double w[3][3];
+        int iw0[3];
+        for (int axes = 0; axes < 3; axes++) {
+            double xp = pos[3 * q + axes] * (double)(p->n_mesh[axes]) / p->boxsize[axes];
+            stencil_axis(p->assign, xp, &iw0[axes], w[axes]);
+        }
+        int t = 0;
+        for (int i = 0; i < s; i++)
+            for (int j = 0; j < s; j++)
+                for (int k = 0; k < s; k++, t++) {
+                    long long coord = ((long long)(iw0[0] + i) * p->n_mesh[1] + (iw0[1] + j)) * p->n_mesh[2] +
+                                      (iw0[2] + k);
+                    coords[q * s3 + t] = ((coord >= 0) && (coord < p->n_mesh_tot)) ? coord : -1;
+                }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT (replaces fftw_plan_dft_3d + fftw_execute, FFTW_ESTIMATE)        */
+/* Mixed-radix Stockham autosort: radix 4, 2, 3, 5 and a direct DFT     */
+/* stage for any other prime factor.                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int n, sign, nf;
+    int f[64];
+    double *tw; /* tw[m] = exp(sign * 2 pi i m / n), m < n */
+} fft_plan1;
+
+static void plan1_init(fft_plan1 *pl, int n, int sign) {
+    pl->n = n;
+    pl->sign = sign;
+    pl->nf = 0;
+    int m = n;
+    while (m % 4 == 0) { pl->f[pl->nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { pl->f[pl->nf++] = 2; m /= 2; }
+    for (int r = 3; (long long)r * r <= m; r += 2)
+        while (m % r == 0) { pl->f[pl->nf++] = r; m /= r; }
+    if (m > 1) pl->f[pl->nf++] = m;
+    pl->tw = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    for (int t = 0; t < n; t++) {
+        double a = 2.0 * M_PI * (double)t / (double)n;
+        pl->tw[2 * t] = cos(a);
+        pl->tw[2 * t + 1] = (double)sign * sin(a);
+    }
+}
+
+static void plan1_free(fft_plan1 *pl) { free(pl->tw); pl->tw = NULL; }
+
+static void stockham_stage(const double *x, double *y, const fft_plan1 *pl, int r, int ns, double *a) {
+    const int n = pl->n;
+    const int m = n / r;
+    const long long tstep = n / (ns * r);
+    const double *tw = pl->tw;
+    const double s = (double)pl->sign;
+    for (int j = 0; j < m; j++) {
+        int k = j % ns;
+        for (int t = 0; t < r; t++) {
+            double xr = x[2 * (j + t * m)], xi = x[2 * (j + t * m) + 1];
+            long long e = ((long long)t * k * tstep) % n;
+            double cr = tw[2 * e], ci = tw[2 * e + 1];
+            a[2 * t] = xr * cr - xi * ci;
+            a[2 * t + 1] = xr * ci + xi * cr;
+        }
+        int base = (j / ns) * ns * r + k;
+        if (r == 2) {
+            y[2 * base] = a[0] + a[2];
+            y[2 * base + 1] = a[1] + a[3];
+            y[2 * (base + ns)] = a[0] - a[2];
+            y[2 * (base + ns) + 1] = a[1] - a[3];
+        } else if (r == 4) {
+            double t0r = a[0] + a[4], t0i = a[1] + a[5];
+            double t1r = a[0] - a[4], t1i = a[1] - a[5];
+            double t2r = a[2] + a[6], t2i = a[3] + a[7];
+            /* (a1 - a3) * (sign * i) */
+            double t3r = -s * (a[3] - a[7]), t3i = s * (a[2] - a[6]);
+            y[2 * base] = t0r + t2r;
+            y[2 * base + 1] = t0i + t2i;
+            y[2 * (base + ns)] = t1r + t3r;
+            y[2 * (base + ns) + 1] = t1i + t3i;
+            y[2 * (base + 2 * ns)] = t0r - t2r;
+            y[2 * (base + 2 * ns) + 1] = t0i - t2i;
+            y[2 * (base + 3 * ns)] = t1r - t3r;
+            y[2 * (base + 3 * ns) + 1] = t1i - t3i;
+        } else {
+            const long long ustep = n / r;
+            for (int u = 0; u < r; u++) {
+                double sr = 0.0, si = 0.0;
+                for (int t = 0; t < r; t++) {
+                    long long e = ((long long)t * u * ustep) % n;
+                    double cr = tw[2 * e], ci = tw[2 * e + 1];
+                    sr += a[2 * t] * cr - a[2 * t + 1] * ci;
+                    si += a[2 * t] * ci + a[2 * t + 1] * cr;
+                }
+                y[2 * (base + u * ns)] = sr;
+                y[2 * (base + u * ns) + 1] = si;
+            }
+        }
+    }
+}
+
+/* x: n complex values; work: 2n doubles; a: 2*maxradix doubles. */
+static void fft_line(double *x, const fft_plan1 *pl, double *work, double *a) {
+    double *src = x, *dst = work;
+    int ns = 1;
+    for (int s = 0; s < pl->nf; s++) {
+        stockham_stage(src, dst, pl, pl->f[s], ns, a);
+        ns *= pl->f[s];
+        double *t = src; src = dst; dst = t;
+    }
+    if (src != x) memcpy(x, src, sizeof(double) * 2 * (size_t)pl->n);
+}
+
+static int max_radix(const fft_plan1 *pl) {
+    int r = 4;
+    for (int s = 0; s < pl->nf; s++) if (pl->f[s] > r) r = pl->f[s];
+    return r;
+}
+
+int or_fft1d(double *x, int n, int sign) {
+    fft_plan1 pl;
+    plan1_init(&pl, n, sign);
+    double *work = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *a = (double *)malloc(sizeof(double) * 2 * (size_t)max_radix(&pl));
+    fft_line(x, &pl, work, a);
+    free(a);
+    free(work);
+    plan1_free(&pl);
+    return 0;
+}
+
+#define OR_BLK 16
+
+/* Transform along one axis of a row-major [n0][n1][n2] complex grid:
+ * lines of length len with point stride `stride`, for all (outer, inner). */
+static void fft_axis(double *f, int len, long long stride, long long n_outer, long long outer_stride,
+                     long long n_inner, int sign) {
+    fft_plan1 pl;
+    plan1_init(&pl, len, sign);
+    double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)len * OR_BLK);
+    double *work = (double *)malloc(sizeof(double) * 2 * (size_t)len);
+    double *a = (double *)malloc(sizeof(double) * 2 * (size_t)max_radix(&pl));
+    for (long long o = 0; o < n_outer; o++) {
+        for (long long i0 = 0; i0 < n_inner; i0 += OR_BLK) {
+            int nb = (int)((n_inner - i0) < OR_BLK ? (n_inner - i0) : OR_BLK);
+            double *base = f + 2 * (o * outer_stride + i0);
+            for (int t = 0; t < len; t++)
+                for (int b = 0; b < nb; b++) {
+                    buf[2 * ((size_t)b * len + t)] = base[2 * (t * stride + b)];
+                    buf[2 * ((size_t)b * len + t) + 1] = base[2 * (t * stride + b) + 1];
+                }
+            for (int b = 0; b < nb; b++) fft_line(buf + 2 * (size_t)b * len, &pl, work, a);
+            for (int t = 0; t < len; t++)
+                for (int b = 0; b < nb; b++) {
+                    base[2 * (t * stride + b)] = buf[2 * ((size_t)b * len + t)];
+                    base[2 * (t * stride + b) + 1] = buf[2 * ((size_t)b * len + t) + 1];
+                }
+        }
+    }
+    free(a);
+    free(work);
+    free(buf);
+    plan1_free(&pl);
+}
+
+int or_fft3d(const or_param *p, double *f, int sign) {
+    const long long n0 = p->n_mesh[0], n1 = p->n_mesh[1], n2 = p->n_mesh[2];
+    /* z: contiguous lines */
+    {
+        fft_plan1 pl;
+        plan1_init(&pl, (int)n2, sign);
+        double *work = (double *)malloc(sizeof(double) * 2 * (size_t)n2);
+        double *a = (double *)malloc(sizeof(double) * 2 * (size_t)max_radix(&pl));
+        for (long long l = 0; l < n0 * n1; l++) fft_line(f + 2 * l * n2, &pl, work, a);
+        free(a);
+        free(work);
+        plan1_free(&pl);
+    }
+    /* y: stride n2, for each x-plane */
+    fft_axis(f, (int)n1, n2, n0, n1 * n2, n2, sign);
+    /* x: stride n1*n2 */
+    fft_axis(f, (int)n0, n1 * n2, 1, 0, n1 * n2, sign);
+    return 0;
+}
+
+/* field.hpp:310-326 */
+void or_fourier_transform(const or_param *p, double *field) {
+    double dV = p->volume / (double)(p->n_mesh_tot);
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        field[2 * i] *= dV;
+        field[2 * i + 1] *= dV;
+    }
+    or_fft3d(p, field, -1);
+}
+
+/* field.hpp:328-344 */
+void or_inverse_fourier_transform(const or_param *p, double *field) {
+    double V = p->volume;
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        field[2 * i] /= V;
+        field[2 * i + 1] /= V;
+    }
+    or_fft3d(p, field, +1);
+}
+
+/* ------------------------------------------------------------------ */
+/* assignment / shot-noise window functions                             */
+/* ------------------------------------------------------------------ */
+
+static void kvec_of(const or_param *p, int i, int j, int k, const double *dk, double *kvec) {
+    /* field.hpp:797-799: the Nyquist index maps to the negative frequency (Q15) */
+    kvec[0] = (i < p->n_mesh[0] / 2) ? (double)i * dk[0] : (double)(i - p->n_mesh[0]) * dk[0];
+    kvec[1] = (j < p->n_mesh[1] / 2) ? (double)j * dk[1] : (double)(j - p->n_mesh[1]) * dk[1];
+    kvec[2] = (k < p->n_mesh[2] / 2) ? (double)k * dk[2] : (double)(k - p->n_mesh[2]) * dk[2];
+}
+
+/* field.hpp:641-717 (truncation toward zero of k/dk + 1e-5: n<0 -> |n|-1, Q2) */
+double or_assignment_function(const or_param *p, const double *kvec) {
+    double dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    int i = (int)(kvec[0] / dk[0] + 1.0e-5);
+    int j = (int)(kvec[1] / dk[1] + 1.0e-5);
+    int k = (int)(kvec[2] / dk[2] + 1.0e-5);
+    double x1 = M_PI * (double)i / (double)(p->n_mesh[0]);
+    double x2 = M_PI * (double)j / (double)(p->n_mesh[1]);
+    double x3 = M_PI * (double)k / (double)(p->n_mesh[2]);
+    double f1 = (i != 0) ? sin(x1) / x1 : 1.0;
+    double f2 = (j != 0) ? sin(x2) / x2 : 1.0;
+    double f3 = (k != 0) ? sin(x3) / x3 : 1.0;
+    double ret = f1 * f2 * f3;
+    if (p->assign == OR_NGP) return pow(ret, 1);
+    if (p->assign == OR_CIC) return pow(ret, 2);
+    if (p->assign == OR_TSC) return pow(ret, 3);
+    return 1.0;
+}
+
+/* field.hpp:1252-1310 */
+double or_shotnoise_function(const or_param *p, const double *kvec) {
+    if (p->assign == OR_NGP) return 1.0;
+    double dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    int i = (int)(kvec[0] / dk[0] + 1.0e-5);
+    int j = (int)(kvec[1] / dk[1] + 1.0e-5);
+    int k = (int)(kvec[2] / dk[2] + 1.0e-5);
+    double x1 = M_PI * (double)i / (double)(p->n_mesh[0]);
+    double x2 = M_PI * (double)j / (double)(p->n_mesh[1]);
+    double x3 = M_PI * (double)k / (double)(p->n_mesh[2]);
+    double f1 = (i != 0) ? sin(x1) : 0.0;
+    double f2 = (j != 0) ? sin(x2) : 0.0;
+    double f3 = (k != 0) ? sin(x3) : 0.0;
+    if (p->assign == OR_CIC) {
+        return (1.0 - (2.0 / 3.0) * f1 * f1) * (1.0 - (2.0 / 3.0) * f2 * f2) * (1.0 - (2.0 / 3.0) * f3 * f3);
+    }
+    if (p->assign == OR_TSC) {
+        return (1.0 - f1 * f1 + 2.0 * pow(f1, 4) / 15.0) * (1.0 - f2 * f2 + 2.0 * pow(f2, 4) / 15.0) *
+               (1.0 - f3 * f3 + 2.0 * pow(f3, 4) / 15.0);
+    }
+    return 0.0;
+}
+
+/* field.hpp:610-639 */
+void or_assignment_correction(const or_param *p, double *field) {
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double wf = or_assignment_function(p, kvec);
+                field[2 * coord] /= wf;
+                field[2 * coord + 1] /= wf;
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* special functions                                                    */
+/* ------------------------------------------------------------------ */
+
+static double factd(int n) {
+    double r = 1.0;
+    for (int i = 2; i <= n; i++) r *= (double)i;
+    return r;
+}
+
+/* gsl_sf_legendre_sphPlm(l, m, x) = sqrt((2l+1)/(4pi)) sqrt((l-m)!/(l+m)!) P_l^m(x),
+ * P_l^m with the Condon-Shortley phase (-1)^m, 0 <= m <= l. */
+double or_sphPlm(int l, int m, double x) {
+    if (m < 0 || m > l) return 0.0;
+    double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double pmm = 1.0;
+    double fact = 1.0;
+    for (int i = 1; i <= m; i++) {
+        pmm *= -fact * somx2;
+        fact += 2.0;
+    }
+    double plm;
+    if (l == m) {
+        plm = pmm;
+    } else {
+        double pmmp1 = x * (2 * m + 1) * pmm;
+        if (l == m + 1) {
+            plm = pmmp1;
+        } else {
+            double pll = 0.0;
+            for (int ll = m + 2; ll <= l; ll++) {
+                pll = (x * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pmm) / (double)(ll - m);
+                pmm = pmmp1;
+                pmmp1 = pll;
+            }
+            plm = pll;
+        }
+    }
+    double norm = sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * factd(l - m) / factd(l + m));
+    return norm * plm;
+}
+
+/* tools.hpp:12-68: conj(Y_lm(v)) * sqrt(4pi/(2l+1)) with the reference's conventions. */
+void or_ylm(int ell, int M, const double *pos, double *out) {
+    if ((ell == 0) && (M == 0)) {
+        out[0] = 1.0;
+        out[1] = 0.0;
+        return;
+    }
+    double xmag_xyz2 = 0.0;
+    for (int axes = 0; axes < 3; axes++) xmag_xyz2 += pos[axes] * pos[axes];
+    double xmag_xyz = sqrt(xmag_xyz2);
+    double xmag_xy = sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+    double mu = 0.0;
+    if (fabs(xmag_xyz) < 1.0e-15) {
+        out[0] = 0.0;
+        out[1] = 0.0;
+        return;
+    } else {
+        mu = pos[2] / xmag_xyz;
+    }
+    double phi = 0.0;
+    if (fabs(xmag_xy) < 1.0e-15) {
+        phi = 0.0;
+    } else {
+        double cosphi = pos[0] / xmag_xy;
+        phi = acos(cosphi);
+    }
+    if (pos[1] < 0.0) phi = -phi + 2.0 * M_PI;
+    int S = abs(M);
+    double sgn = pow(-1.0, (M - S) / 2.0);
+    double plm = or_sphPlm(ell, S, mu);
+    double c = cos((double)M * phi), s = sin((double)M * phi);
+    double yr = sgn * plm * c, yi = sgn * plm * s;
+    yi = -yi; /* conj */
+    double nrm = sqrt(4.0 * M_PI / (2.0 * (double)ell + 1.0));
+    out[0] = yr * nrm;
+    out[1] = yi * nrm;
+}
+
+/* gsl_sf_coupling_3j via the Racah formula. */
+double or_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+    if (m1 + m2 + m3 != 0) return 0.0;
+    if (j1 < 0 || j2 < 0 || j3 < 0) return 0.0;
+    if (abs(m1) > j1 || abs(m2) > j2 || abs(m3) > j3) return 0.0;
+    if (j3 < abs(j1 - j2) || j3 > j1 + j2) return 0.0;
+    double tri = factd(j1 + j2 - j3) * factd(j1 - j2 + j3) * factd(-j1 + j2 + j3) / factd(j1 + j2 + j3 + 1);
+    double pre = sqrt(tri * factd(j1 + m1) * factd(j1 - m1) * factd(j2 + m2) * factd(j2 - m2) * factd(j3 + m3) *
+                      factd(j3 - m3));
+    int kmin = 0;
+    if (j2 - j3 - m1 > kmin) kmin = j2 - j3 - m1;
+    if (j1 - j3 + m2 > kmin) kmin = j1 - j3 + m2;
+    int kmax = j1 + j2 - j3;
+    if (j1 - m1 < kmax) kmax = j1 - m1;
+    if (j2 + m2 < kmax) kmax = j2 + m2;
+    double sum = 0.0;
+    for (int k = kmin; k <= kmax; k++) {
+        double den = factd(k) * factd(j1 + j2 - j3 - k) * factd(j1 - m1 - k) * factd(j2 + m2 - k) *
+                     factd(j3 - j2 + m1 + k) * factd(j3 - j1 - m2 + k);
+        sum += ((k % 2) ? -1.0 : 1.0) / den;
+    }
+    double sgn = (((j1 - j2 - m3) % 2) != 0) ? -1.0 : 1.0;
+    return sgn * pre * sum;
+}
+
+/* Spherical Bessel j_l(x), x >= 0: power series for x < l + 2, upward
+ * recurrence from j0, j1 beyond. */
+double or_bessel_jl(int l, double x) {
+    if (x == 0.0) return (l == 0) ? 1.0 : 0.0;
+    if (x < (double)l + 2.0) {
+        double dfact = 1.0; /* (2l+1)!! */
+        for (int i = 1; i <= 2 * l + 1; i += 2) dfact *= (double)i;
+        double pre = pow(x, l) / dfact;
+        double term = 1.0, sum = 1.0, y = -0.5 * x * x;
+        for (int k = 1; k < 200; k++) {
+            term *= y / ((double)k * (double)(2 * l + 2 * k + 1));
+            sum += term;
+            if (fabs(term) < 1e-18 * fabs(sum)) break;
+        }
+        return pre * sum;
+    }
+    double s = sin(x), c = cos(x);
+    double j0 = s / x;
+    if (l == 0) return j0;
+    double j1 = s / (x * x) - c / x;
+    for (int n = 1; n < l; n++) {
+        double j2 = (2.0 * n + 1.0) / x * j1 - j0;
+        j0 = j1;
+        j1 = j2;
+    }
+    return j1;
+}
+
+/* bessel.hpp:10-37: natural cubic spline (gsl_interp_cspline) of j_l on
+ * 10^6 equidistant nodes in [0, 1e4]. */
+struct or_bessel {
+    int nn;
+    double xmin, xmax, dx;
+    double *x, *y, *c; /* c = second derivative / 2 at the nodes */
+};
+
+or_bessel *or_bessel_new(int ell) {
+    or_bessel *b = (or_bessel *)calloc(1, sizeof(or_bessel));
+    const int NN = 1000000;
+    b->nn = NN;
+    b->xmin = 0;
+    b->xmax = 10000;
+    b->dx = (b->xmax - b->xmin) / (double)(NN - 1);
+    b->x = (double *)malloc(sizeof(double) * NN);
+    b->y = (double *)malloc(sizeof(double) * NN);
+    b->c = (double *)calloc(NN, sizeof(double));
+    for (int i = 0; i < NN; i++) {
+        b->x[i] = b->xmin + b->dx * (double)i;
+        b->y[i] = or_bessel_jl(ell, b->x[i]);
+    }
+    /* natural spline: c_0 = c_{N-1} = 0; tridiagonal system for c_1..c_{N-2}
+     * h_{i-1} c_{i-1} + 2 (h_{i-1} + h_i) c_i + h_i c_{i+1} = 3 (g_i - g_{i-1}) */
+    int m = NN - 2;
+    double *diag = (double *)malloc(sizeof(double) * m);
+    double *off = (double *)malloc(sizeof(double) * m);
+    double *rhs = (double *)malloc(sizeof(double) * m);
+    for (int i = 0; i < m; i++) {
+        double h_i = b->x[i + 2] - b->x[i + 1];
+        double h_im1 = b->x[i + 1] - b->x[i];
+        double g_i = (b->y[i + 2] - b->y[i + 1]) / h_i;
+        double g_im1 = (b->y[i + 1] - b->y[i]) / h_im1;
+        diag[i] = 2.0 * (h_im1 + h_i);
+        off[i] = h_i;
+        rhs[i] = 3.0 * (g_i - g_im1);
+    }
+    /* Thomas algorithm (symmetric tridiagonal) */
+    for (int i = 1; i < m; i++) {
+        double w = off[i - 1] / diag[i - 1];
+        diag[i] -= w * off[i - 1];
+        rhs[i] -= w * rhs[i - 1];
+    }
+    b->c[m] = rhs[m - 1] / diag[m - 1];
+    for (int i = m - 2; i >= 0; i--) b->c[i + 1] = (rhs[i] - off[i] * b->c[i + 2]) / diag[i];
+    free(diag);
+    free(off);
+    free(rhs);
+    return b;
+}
+
+double or_bessel_eval(const or_bessel *b, double x) {
+    if (x < b->x[0] || x > b->x[b->nn - 1]) return NAN; /* GSL_EDOM (Q13) */
+    /* interval with x_i <= x < x_{i+1} (last interval closed) */
+    long lo = 0, hi = b->nn - 1;
+    while (hi > lo + 1) {
+        long mid = (lo + hi) / 2;
+        if (b->x[mid] > x) hi = mid; else lo = mid;
+    }
+    long i = lo;
+    double x_lo = b->x[i], x_hi = b->x[i + 1];
+    double y_lo = b->y[i], y_hi = b->y[i + 1];
+    double dx = x_hi - x_lo, delx = x - x_lo;
+    double c_i = b->c[i], c_ip1 = b->c[i + 1];
+    double bb = (y_hi - y_lo) / dx - dx * (c_ip1 + 2.0 * c_i) / 3.0;
+    double d = (c_ip1 - c_i) / (3.0 * dx);
+    return y_lo + delx * (bb + delx * (c_i + delx * d));
+}
+
+void or_bessel_free(or_bessel *b) {
+    if (!b) return;
+    free(b->x);
+    free(b->y);
+    free(b->c);
+    free(b);
+}
+
+/* ------------------------------------------------------------------ */
+/* two-point statistics                                                 */
+/* ------------------------------------------------------------------ */
+
+/* (ar + i ai)(br + i bi) as std::complex<double> evaluates it */
+#define CMUL(ar, ai, br, bi, outr, outi) \
+    do {                                  \
+        double _r = (ar) * (br) - (ai) * (bi); \
+        double _i = (ar) * (bi) + (ai) * (br); \
+        outr = _r;                        \
+        outi = _i;                        \
+    } while (0)
+
+/* field.hpp:765-852 */
+int or_calc_power_spectrum(const or_param *p, const double *f1, const double *f2, const double *kbin, double sn_re,
+                           double sn_im, int ell, int m, double *pk, int *n_mode) {
+    double dk_bin = kbin[1] - kbin[0];
+    const int n_temp = 100000;
+    const double dk_temp = 0.0001;
+    double *pk_temp = (double *)calloc(2 * (size_t)n_temp, sizeof(double));
+    int *n_mode_temp = (int *)calloc((size_t)n_temp, sizeof(int));
+    for (int i = 0; i < p->n_kbin; i++) {
+        pk[2 * i] = 0.0;
+        pk[2 * i + 1] = 0.0;
+        n_mode[i] = 0;
+    }
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double kmag = sqrt(kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2]);
+                int kindex = (int)(kmag / dk_temp + 0.5);
+                if (kindex < n_temp) {
+                    double pr, pi;
+                    CMUL(f1[2 * coord], f1[2 * coord + 1], f2[2 * coord], -f2[2 * coord + 1], pr, pi);
+                    double S = or_shotnoise_function(p, kvec);
+                    pr -= sn_re * S;
+                    pi -= sn_im * S;
+                    double wf = or_assignment_function(p, kvec);
+                    double w2 = pow(wf, 2);
+                    pr /= w2;
+                    pi /= w2;
+                    double y[2];
+                    or_ylm(ell, m, kvec, y);
+                    CMUL(pr, pi, y[0], y[1], pr, pi);
+                    pk_temp[2 * kindex] += pr;
+                    pk_temp[2 * kindex + 1] += pi;
+                    n_mode_temp[kindex]++;
+                }
+            }
+    for (int j = 0; j < p->n_kbin; j++) {
+        double lower = (kbin[j] > dk_bin * 0.5) ? (kbin[j] - dk_bin * 0.5) : 0.0;
+        double upper = kbin[j] + dk_bin * 0.5;
+        for (int i = 0; i < n_temp; i++) {
+            double kmag_temp = (double)i * dk_temp;
+            if ((kmag_temp > lower) && (kmag_temp <= upper)) {
+                pk[2 * j] += pk_temp[2 * i];
+                pk[2 * j + 1] += pk_temp[2 * i + 1];
+                n_mode[j] += n_mode_temp[i];
+            }
+        }
+    }
+    for (int i = 0; i < p->n_kbin; i++) {
+        if (n_mode[i] != 0) {
+            pk[2 * i] /= (double)n_mode[i];
+            pk[2 * i + 1] /= (double)n_mode[i];
+        } else {
+            pk[2 * i] = 0.0;
+            pk[2 * i + 1] = 0.0;
+        }
+    }
+    free(pk_temp);
+    free(n_mode_temp);
+    return 0;
+}
+
+/* field.hpp:854-977 */
+int or_calc_correlation_function(const or_param *p, const double *f1, const double *f2, const double *rbin,
+                                 double sn_re, double sn_im, int ell, int m, double *xi, int *n_mode) {
+    double *xi3d = (double *)calloc(2 * (size_t)p->n_mesh_tot, sizeof(double));
+    double factor = 1.0 / p->volume;
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double pr, pi;
+                CMUL(f1[2 * coord], f1[2 * coord + 1], f2[2 * coord], -f2[2 * coord + 1], pr, pi);
+                double S = or_shotnoise_function(p, kvec);
+                pr -= sn_re * S;
+                pi -= sn_im * S;
+                double w2 = pow(or_assignment_function(p, kvec), 2);
+                pr /= w2;
+                pi /= w2;
+                xi3d[2 * coord] = factor * pr;
+                xi3d[2 * coord + 1] = factor * pi;
+            }
+    or_fft3d(p, xi3d, +1);
+    const int n_temp = 20000;
+    const double dr_temp = 0.5;
+    double *xi_temp = (double *)calloc(2 * (size_t)n_temp, sizeof(double));
+    int *n_mode_temp = (int *)calloc((size_t)n_temp, sizeof(int));
+    for (int i = 0; i < p->n_rbin; i++) {
+        xi[2 * i] = 0.0;
+        xi[2 * i + 1] = 0.0;
+        n_mode[i] = 0;
+    }
+    double rvec[3], dr[3];
+    dr[0] = p->boxsize[0] / (double)(p->n_mesh[0]);
+    dr[1] = p->boxsize[1] / (double)(p->n_mesh[1]);
+    dr[2] = p->boxsize[2] / (double)(p->n_mesh[2]);
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dr, rvec);
+                double rmag = sqrt(rvec[0] * rvec[0] + rvec[1] * rvec[1] + rvec[2] * rvec[2]);
+                int rindex = (int)(rmag / dr_temp + 0.5);
+                if (rindex < n_temp) {
+                    double y[2], xr, xim;
+                    or_ylm(ell, m, rvec, y);
+                    CMUL(xi3d[2 * coord], xi3d[2 * coord + 1], y[0], y[1], xr, xim);
+                    xi_temp[2 * rindex] += xr;
+                    xi_temp[2 * rindex + 1] += xim;
+                    n_mode_temp[rindex]++;
+                }
+            }
+    double dr_bin = rbin[1] - rbin[0];
+    for (int j = 0; j < p->n_rbin; j++) {
+        double lower = (rbin[j] > dr_bin * 0.5) ? (rbin[j] - dr_bin * 0.5) : 0.0;
+        double upper = rbin[j] + dr_bin * 0.5;
+        for (int i = 0; i < n_temp; i++) {
+            double rmag_temp = (double)i * dr_temp;
+            if ((rmag_temp > lower) && (rmag_temp <= upper)) {
+                xi[2 * j] += xi_temp[2 * i];
+                xi[2 * j + 1] += xi_temp[2 * i + 1];
+                n_mode[j] += n_mode_temp[i];
+            }
+        }
+    }
+    for (int i = 0; i < p->n_rbin; i++) {
+        if (n_mode[i] != 0) {
+            xi[2 * i] /= (double)n_mode[i];
+            xi[2 * i + 1] /= (double)n_mode[i];
+        } else {
+            xi[2 * i] = 0.0;
+            xi[2 * i + 1] = 0.0;
+        }
+    }
+    free(xi3d);
+    free(xi_temp);
+    free(n_mode_temp);
+    return 0;
+}
+
+/* field.hpp:346-401: shell (max(k-dk/2,0), k+dk/2] on the raw |k| (Q3),
+ * value Y * delta / W, backward FFT, divided by the mode count. */
+int or_ifft_for_bispectrum(const or_param *p, const double *density, double kmag_in, double dk_in, const double *ylm,
+                           double *out) {
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        out[2 * i] = 0.0;
+        out[2 * i + 1] = 0.0;
+    }
+    int n_mode = 0;
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double kmag = sqrt(kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2]);
+                double lower = (kmag_in > dk_in * 0.5) ? (kmag_in - dk_in * 0.5) : 0.0;
+                double upper = kmag_in + dk_in * 0.5;
+                if ((kmag > lower) && (kmag <= upper)) {
+                    double wf = or_assignment_function(p, kvec);
+                    double dr = density[2 * coord] / wf, di = density[2 * coord + 1] / wf;
+                    double yr = ylm ? ylm[2 * coord] : 1.0, yi = ylm ? ylm[2 * coord + 1] : 0.0;
+                    CMUL(yr, yi, dr, di, out[2 * coord], out[2 * coord + 1]);
+                    n_mode++;
+                } else {
+                    out[2 * coord] = 0.0;
+                    out[2 * coord + 1] = 0.0;
+                }
+            }
+    or_fft3d(p, out, +1);
+    for (long long c = 0; c < p->n_mesh_tot; c++) {
+        out[2 * c] /= (double)n_mode;
+        out[2 * c + 1] /= (double)n_mode;
+    }
+    return n_mode;
+}
+
+/* field.hpp:1116-1160 */
+int or_shotnoise_bispectrum_ijk(const or_param *p, const double *f1, const double *f2, double sn_re, double sn_im,
+                                double *xi) {
+    double factor = 1.0 / p->volume;
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double pr, pi;
+                CMUL(f1[2 * coord], f1[2 * coord + 1], f2[2 * coord], -f2[2 * coord + 1], pr, pi);
+                double S = or_shotnoise_function(p, kvec);
+                pr -= sn_re * S;
+                pi -= sn_im * S;
+                double w2 = pow(or_assignment_function(p, kvec), 2);
+                pr /= w2;
+                pi /= w2;
+                xi[2 * coord] = factor * pr;
+                xi[2 * coord + 1] = factor * pi;
+            }
+    or_fft3d(p, xi, +1);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* box density fields and drivers                                       */
+/* ------------------------------------------------------------------ */
+
+static double *unit_weights(long long np) {
+    double *w = (double *)malloc(sizeof(double) * 2 * (size_t)(np > 0 ? np : 1));
+    for (long long q = 0; q < np; q++) {
+        w[2 * q] = 1.0;
+        w[2 * q + 1] = 0.0;
+    }
+    return w;
+}
+
+int or_box_density_fluctuation(const or_param *p, const double *pos, long long np, double *field) {
+    double *w = unit_weights(np);
+    or_calc_field(p, pos, w, np, field);
+    free(w);
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        field[2 * i] -= ((double)np / p->volume);
+        field[2 * i + 1] -= 0.0;
+    }
+    return 0;
+}
+
+int or_box_density(const or_param *p, const double *pos, long long np, double *field) {
+    double *w = unit_weights(np);
+    or_calc_field(p, pos, w, np, field);
+    free(w);
+    return 0;
+}
+
+static double *grid_alloc(const or_param *p) {
+    return (double *)calloc(2 * (size_t)p->n_mesh_tot, sizeof(double));
+}
+
+/* powerspec.hpp:273-332 */
+int or_box_pk(const or_param *p, const double *pos, long long np, const double *kbin, double *pk_out, int *n_mode) {
+    if (!((p->ELL == p->ell1) && (p->ell2 == 0))) return -1;
+    double *dn = grid_alloc(p);
+    or_box_density_fluctuation(p, pos, np, dn);
+    or_fourier_transform(p, dn);
+    double *pk = (double *)malloc(sizeof(double) * 2 * p->n_kbin);
+    or_calc_power_spectrum(p, dn, dn, kbin, (double)np, 0.0, p->ELL, 0, pk, n_mode);
+    double norm = p->volume / (double)np / (double)np;
+    for (int i = 0; i < p->n_kbin; i++) {
+        double sr = 0.0 + (double)(2 * p->ELL + 1) * pk[2 * i];
+        double si = 0.0 + (double)(2 * p->ELL + 1) * pk[2 * i + 1];
+        pk_out[2 * i] = norm * sr;
+        pk_out[2 * i + 1] = norm * si;
+    }
+    free(pk);
+    free(dn);
+    return 0;
+}
+
+/* powerspec.hpp:334-393 */
+int or_box_xi(const or_param *p, const double *pos, long long np, const double *rbin, double *xi_out) {
+    if (!((p->ELL == p->ell1) && (p->ell2 == 0))) return -1;
+    double *dn = grid_alloc(p);
+    or_box_density_fluctuation(p, pos, np, dn);
+    or_fourier_transform(p, dn);
+    double *xi = (double *)malloc(sizeof(double) * 2 * p->n_rbin);
+    int *nm = (int *)malloc(sizeof(int) * p->n_rbin);
+    or_calc_correlation_function(p, dn, dn, rbin, (double)np, 0.0, p->ELL, 0, xi, nm);
+    double norm = p->volume / (double)np / (double)np;
+    for (int i = 0; i < p->n_rbin; i++) xi_out[i] = norm * (0.0 + (double)(2 * p->ELL + 1) * xi[2 * i]);
+    free(nm);
+    free(xi);
+    free(dn);
+    return 0;
+}
+
+static void store_ylm_fourier(const or_param *p, int ell, int m, double *ylm) {
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                kvec_of(p, i, j, k, dk, kvec);
+                or_ylm(ell, m, kvec, ylm + 2 * or_coord(p, i, j, k));
+            }
+}
+
+static void store_ylm_config(const or_param *p, int ell, int m, double *ylm) {
+    double rvec[3], dr[3];
+    dr[0] = p->boxsize[0] / (double)(p->n_mesh[0]);
+    dr[1] = p->boxsize[1] / (double)(p->n_mesh[1]);
+    dr[2] = p->boxsize[2] / (double)(p->n_mesh[2]);
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                kvec_of(p, i, j, k, dr, rvec);
+                or_ylm(ell, m, rvec, ylm + 2 * or_coord(p, i, j, k));
+            }
+}
+
+/* i^n, exactly (the reference's pow(complex I, n) agrees to ~1e-16) */
+static void ipow(int n, double *r, double *i) {
+    switch (((n % 4) + 4) % 4) {
+    case 0: *r = 1.0; *i = 0.0; break;
+    case 1: *r = 0.0; *i = 1.0; break;
+    case 2: *r = -1.0; *i = 0.0; break;
+    default: *r = 0.0; *i = -1.0; break;
+    }
+}
+
+/* bispec.hpp:877-1203 */
+int or_box_bk(const or_param *p, const double *pos, long long np, const double *kbin, double *bk_save,
+              double *sn_save, double *out) {
+    const int l1 = p->ell1, l2 = p->ell2, L = p->ELL, nk = p->n_kbin, ith = p->ith_kbin;
+    if (fabs(or_wigner3j(l1, l2, L, 0, 0, 0)) < 1.0e-10) return -1;
+    for (int i = 0; i < nk; i++) {
+        sn_save[2 * i] = 0.0;
+        sn_save[2 * i + 1] = 0.0;
+        bk_save[2 * i] = 0.0;
+        bk_save[2 * i + 1] = 0.0;
+    }
+    const long long ng = p->n_mesh_tot;
+    double *pk = (double *)malloc(sizeof(double) * 2 * nk);
+    int *nm = (int *)malloc(sizeof(int) * nk);
+
+    /* shot noise: two-point terms (bispec.hpp:912-963) */
+    {
+        double *dn_00_sn = grid_alloc(p), *N_LM_sn = grid_alloc(p);
+        or_box_density_fluctuation(p, pos, np, dn_00_sn);
+        or_fourier_transform(p, dn_00_sn);
+        or_box_density(p, pos, np, N_LM_sn);
+        or_fourier_transform(p, N_LM_sn);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                int M = 0;
+                double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, M);
+                w *= (double)(2 * L + 1) * (double)(2 * l1 + 1) * (double)(2 * l2 + 1);
+                if (fabs(w) < 1.0e-10) continue;
+                double shotnoise = (double)np;
+                if ((l1 == 0) && (l2 == 0))
+                    for (int i = 0; i < nk; i++) sn_save[2 * i] += w * shotnoise;
+                if (l2 == 0) {
+                    or_calc_power_spectrum(p, dn_00_sn, N_LM_sn, kbin, shotnoise, 0.0, l1, m1, pk, nm);
+                    for (int i = 0; i < nk; i++) {
+                        sn_save[2 * i] += w * pk[2 * ith];
+                        sn_save[2 * i + 1] += w * pk[2 * ith + 1];
+                    }
+                }
+                if (l1 == 0) {
+                    or_calc_power_spectrum(p, dn_00_sn, N_LM_sn, kbin, shotnoise, 0.0, l2, m2, pk, nm);
+                    for (int i = 0; i < nk; i++) {
+                        sn_save[2 * i] += w * pk[2 * i];
+                        sn_save[2 * i + 1] += w * pk[2 * i + 1];
+                    }
+                }
+            }
+        free(dn_00_sn);
+        free(N_LM_sn);
+    }
+
+    /* shot noise: the ijk term (bispec.hpp:972-1073) */
+    {
+        double *N_00_sn = grid_alloc(p), *dn_LM_sn = grid_alloc(p);
+        or_box_density(p, pos, np, N_00_sn);
+        or_fourier_transform(p, N_00_sn);
+        or_box_density_fluctuation(p, pos, np, dn_LM_sn);
+        or_fourier_transform(p, dn_LM_sn);
+        or_bessel *sj1 = or_bessel_new(l1);
+        or_bessel *sj2 = or_bessel_new(l2);
+        double *Ylm1 = grid_alloc(p), *Ylm2 = grid_alloc(p), *xi = grid_alloc(p);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                int M = 0;
+                double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, M);
+                w *= (double)(2 * L + 1) * (double)(2 * l1 + 1) * (double)(2 * l2 + 1);
+                if (fabs(w) < 1.0e-10) continue;
+                store_ylm_config(p, l1, m1, Ylm1);
+                store_ylm_config(p, l2, m2, Ylm2);
+                memset(xi, 0, sizeof(double) * 2 * (size_t)ng);
+                or_shotnoise_bispectrum_ijk(p, dn_LM_sn, N_00_sn, (double)np, 0.0, xi);
+                double dr[3];
+                dr[0] = p->boxsize[0] / (double)(p->n_mesh[0]);
+                dr[1] = p->boxsize[1] / (double)(p->n_mesh[1]);
+                dr[2] = p->boxsize[2] / (double)(p->n_mesh[2]);
+                for (int ik = 0; ik < nk; ik++) {
+                    double kmag2 = kbin[ik];
+                    double kmag1 = kbin[ith];
+                    double sr = 0.0, si = 0.0;
+                    double rvec[3];
+                    for (int i = 0; i < p->n_mesh[0]; i++)
+                        for (int j = 0; j < p->n_mesh[1]; j++)
+                            for (int k = 0; k < p->n_mesh[2]; k++) {
+                                long long coord = or_coord(p, i, j, k);
+                                kvec_of(p, i, j, k, dr, rvec);
+                                double rmag = sqrt(rvec[0] * rvec[0] + rvec[1] * rvec[1] + rvec[2] * rvec[2]);
+                                double j1 = or_bessel_eval(sj1, kmag1 * rmag);
+                                double j2 = or_bessel_eval(sj2, kmag2 * rmag);
+                                double jj = j1 * j2;
+                                double tr = jj * xi[2 * coord], ti = jj * xi[2 * coord + 1];
+                                CMUL(tr, ti, Ylm1[2 * coord], Ylm1[2 * coord + 1], tr, ti);
+                                CMUL(tr, ti, Ylm2[2 * coord], Ylm2[2 * coord + 1], tr, ti);
+                                sr += tr;
+                                si += ti;
+                            }
+                    double ir, ii;
+                    ipow(l1 + l2, &ir, &ii);
+                    double fac = p->volume / (double)(p->n_mesh_tot);
+                    double cr = ir * fac, ci = ii * fac;
+                    CMUL(sr, si, cr, ci, sr, si);
+                    sn_save[2 * ik] += w * sr;
+                    sn_save[2 * ik + 1] += w * si;
+                }
+            }
+        or_bessel_free(sj1);
+        or_bessel_free(sj2);
+        free(Ylm1);
+        free(Ylm2);
+        free(xi);
+        free(N_00_sn);
+        free(dn_LM_sn);
+    }
+
+    /* bispectrum (bispec.hpp:1100-1182) */
+    {
+        double *dn_00 = grid_alloc(p), *dn_LM = grid_alloc(p);
+        or_box_density_fluctuation(p, pos, np, dn_00);
+        or_fourier_transform(p, dn_00);
+        or_box_density_fluctuation(p, pos, np, dn_LM);
+        or_fourier_transform(p, dn_LM);
+        or_assignment_correction(p, dn_LM);
+        or_inverse_fourier_transform(p, dn_LM);
+        double *Ylm1 = grid_alloc(p), *Ylm2 = grid_alloc(p);
+        double *t1 = grid_alloc(p), *t2 = grid_alloc(p);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                int M = 0;
+                double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, M);
+                w *= (double)(2 * L + 1) * (double)(2 * l1 + 1) * (double)(2 * l2 + 1);
+                if (fabs(w) < 1.0e-10) continue;
+                store_ylm_fourier(p, l1, m1, Ylm1);
+                store_ylm_fourier(p, l2, m2, Ylm2);
+                double dk = kbin[1] - kbin[0];
+                double kmag1 = kbin[ith];
+                or_ifft_for_bispectrum(p, dn_00, kmag1, dk, Ylm1, t1);
+                for (int ik = 0; ik < nk; ik++) {
+                    double kmag2 = kbin[ik];
+                    or_ifft_for_bispectrum(p, dn_00, kmag2, dk, Ylm2, t2);
+                    double sr = 0.0, si = 0.0;
+                    double fac = p->volume / (double)(p->n_mesh_tot);
+                    for (long long c = 0; c < ng; c++) {
+                        double ar = fac * t1[2 * c], ai = fac * t1[2 * c + 1];
+                        CMUL(ar, ai, t2[2 * c], t2[2 * c + 1], ar, ai);
+                        CMUL(ar, ai, dn_LM[2 * c], dn_LM[2 * c + 1], ar, ai);
+                        sr += ar;
+                        si += ai;
+                    }
+                    bk_save[2 * ik] += w * sr;
+                    bk_save[2 * ik + 1] += w * si;
+                }
+            }
+        free(Ylm1);
+        free(Ylm2);
+        free(t1);
+        free(t2);
+        free(dn_00);
+        free(dn_LM);
+    }
+
+    double norm = p->volume / (double)np / (double)np;
+    norm *= (p->volume / (double)np);
+    for (int i = 0; i < nk; i++) {
+        out[2 * i] = norm * (bk_save[2 * i] - sn_save[2 * i]);
+        out[2 * i + 1] = norm * (bk_save[2 * i + 1] - sn_save[2 * i + 1]);
+    }
+    free(pk);
+    free(nm);
+    return 0;
+}

@@ -1,0 +1,257 @@
+"""ctypes view of liboracle.so, the FP64 CPU restatement of the reference
+(TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; never by the product)."""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+NGP, CIC, TSC = 0, 1, 2
+
+
+class Param(ct.Structure):
+    _fields_ = [
+        ("n_mesh", ct.c_int * 3), ("boxsize", ct.c_double * 3), ("volume", ct.c_double),
+        ("n_mesh_tot", ct.c_longlong), ("assign", ct.c_int), ("ell1", ct.c_int), ("ell2", ct.c_int),
+        ("ELL", ct.c_int), ("n_kbin", ct.c_int), ("n_rbin", ct.c_int), ("ith_kbin", ct.c_int),
+        ("ith_rbin", ct.c_int), ("kmin", ct.c_double), ("kmax", ct.c_double), ("rmin", ct.c_double),
+        ("rmax", ct.c_double),
+    ]
+
+
+_lib = None
+_D = ct.POINTER(ct.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ct.CDLL(LIB_PATH)
+        P = ct.POINTER(Param)
+        sig = {
+            "or_param_init": (None, [P, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_int]),
+            "or_set_kbin": (None, [P, _D]),
+            "or_set_rbin": (None, [P, _D]),
+            "or_calc_field": (ct.c_int, [P, _D, _D, ct.c_longlong, _D]),
+            "or_cell_coords": (ct.c_int, [P, _D, ct.c_longlong, ct.POINTER(ct.c_longlong)]),
+            "or_fft3d": (ct.c_int, [P, _D, ct.c_int]),
+            "or_fft1d": (ct.c_int, [_D, ct.c_int, ct.c_int]),
+            "or_fourier_transform": (None, [P, _D]),
+            "or_inverse_fourier_transform": (None, [P, _D]),
+            "or_assignment_correction": (None, [P, _D]),
+            "or_assignment_function": (ct.c_double, [P, _D]),
+            "or_shotnoise_function": (ct.c_double, [P, _D]),
+            "or_ylm": (None, [ct.c_int, ct.c_int, _D, _D]),
+            "or_sphPlm": (ct.c_double, [ct.c_int, ct.c_int, ct.c_double]),
+            "or_wigner3j": (ct.c_double, [ct.c_int] * 6),
+            "or_bessel_jl": (ct.c_double, [ct.c_int, ct.c_double]),
+            "or_bessel_new": (ct.c_void_p, [ct.c_int]),
+            "or_bessel_eval": (ct.c_double, [ct.c_void_p, ct.c_double]),
+            "or_bessel_free": (None, [ct.c_void_p]),
+            "or_calc_power_spectrum": (ct.c_int, [P, _D, _D, _D, ct.c_double, ct.c_double, ct.c_int, ct.c_int, _D,
+                                                  ct.POINTER(ct.c_int)]),
+            "or_calc_correlation_function": (ct.c_int, [P, _D, _D, _D, ct.c_double, ct.c_double, ct.c_int, ct.c_int,
+                                                        _D, ct.POINTER(ct.c_int)]),
+            "or_ifft_for_bispectrum": (ct.c_int, [P, _D, ct.c_double, ct.c_double, _D, _D]),
+            "or_box_density_fluctuation": (ct.c_int, [P, _D, ct.c_longlong, _D]),
+            "or_box_density": (ct.c_int, [P, _D, ct.c_longlong, _D]),
+            "or_box_pk": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, ct.POINTER(ct.c_int)]),
+            "or_box_xi": (ct.c_int, [P, _D, ct.c_longlong, _D, _D]),
+            "or_box_bk": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, _D, _D]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def param(n, L, assign=TSC, kmin=0.01, kmax=0.2, n_kbin=20, ell1=0, ell2=0, ELL=0, ith_kbin=0,
+          rmin=30.0, rmax=150.0, n_rbin=25, ith_rbin=0) -> Param:
+    n = list(n) if np.ndim(n) else [int(n)] * 3
+    L = list(L) if np.ndim(L) else [float(L)] * 3
+    p = Param()
+    lib().or_param_init(ct.byref(p), n[0], n[1], n[2], L[0], L[1], L[2], assign)
+    p.kmin, p.kmax, p.n_kbin = kmin, kmax, n_kbin
+    p.rmin, p.rmax, p.n_rbin = rmin, rmax, n_rbin
+    p.ell1, p.ell2, p.ELL = ell1, ell2, ELL
+    p.ith_kbin, p.ith_rbin = ith_kbin, ith_rbin
+    return p
+
+
+def kbin(p: Param):
+    out = np.zeros(p.n_kbin)
+    lib().or_set_kbin(ct.byref(p), dp(out))
+    return out
+
+
+def rbin(p: Param):
+    out = np.zeros(p.n_rbin)
+    lib().or_set_rbin(ct.byref(p), dp(out))
+    return out
+
+
+def shape(p: Param):
+    return (p.n_mesh[0], p.n_mesh[1], p.n_mesh[2])
+
+
+def calc_field(p: Param, pos, weight=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    w = np.zeros((pos.shape[0], 2))
+    if weight is None:
+        w[:, 0] = 1.0
+    else:
+        w[:] = np.asarray(weight, dtype=np.float64).reshape(-1, 2)
+    f = np.zeros(shape(p) + (2,))
+    lib().or_calc_field(ct.byref(p), dp(pos), dp(w), pos.shape[0], dp(f))
+    return f[..., 0] + 1j * f[..., 1]
+
+
+def cell_coords(p: Param, pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    s = {NGP: 1, CIC: 8, TSC: 27}[p.assign]
+    out = np.zeros((pos.shape[0], s), dtype=np.int64)
+    lib().or_cell_coords(ct.byref(p), dp(pos), pos.shape[0], out.ctypes.data_as(ct.POINTER(ct.c_longlong)))
+    return out
+
+
+def _c(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a, a.view(np.float64)
+
+
+def fft3d(p: Param, a, sign):
+    a, v = _c(np.array(a, dtype=np.complex128, copy=True))
+    lib().or_fft3d(ct.byref(p), dp(v), int(sign))
+    return a
+
+
+def fft1d(a, sign):
+    a, v = _c(np.array(a, dtype=np.complex128, copy=True))
+    lib().or_fft1d(dp(v), a.shape[0], int(sign))
+    return a
+
+
+def assignment_function(p: Param, kvec):
+    k = np.ascontiguousarray(kvec, dtype=np.float64)
+    return lib().or_assignment_function(ct.byref(p), dp(k))
+
+
+def shotnoise_function(p: Param, kvec):
+    k = np.ascontiguousarray(kvec, dtype=np.float64)
+    return lib().or_shotnoise_function(ct.byref(p), dp(k))
+
+
+def ylm(ell, m, v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros(2)
+    lib().or_ylm(ell, m, dp(v), dp(out))
+    return complex(out[0], out[1])
+
+
+def sph_plm(l, m, x):
+    return lib().or_sphPlm(l, m, x)
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    return lib().or_wigner3j(j1, j2, j3, m1, m2, m3)
+
+
+def bessel_jl(l, x):
+    return lib().or_bessel_jl(l, float(x))
+
+
+class BesselSpline:
+    def __init__(self, ell):
+        self.h = lib().or_bessel_new(ell)
+
+    def __call__(self, x):
+        return lib().or_bessel_eval(self.h, float(x))
+
+    def __del__(self):
+        try:
+            lib().or_bessel_free(self.h)
+        except Exception:
+            pass
+
+
+def box_density_fluctuation(p: Param, pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    f = np.zeros(shape(p) + (2,))
+    lib().or_box_density_fluctuation(ct.byref(p), dp(pos), pos.shape[0], dp(f))
+    return f[..., 0] + 1j * f[..., 1]
+
+
+def fourier_transform(p: Param, field):
+    a, v = _c(np.array(field, dtype=np.complex128, copy=True))
+    lib().or_fourier_transform(ct.byref(p), dp(v))
+    return a
+
+
+def calc_power_spectrum(p: Param, f1, f2, kb, sn, ell, m):
+    a1, v1 = _c(f1)
+    a2, v2 = _c(f2)
+    kb = np.ascontiguousarray(kb, dtype=np.float64)
+    pk = np.zeros(2 * p.n_kbin)
+    nm = np.zeros(p.n_kbin, dtype=np.int32)
+    lib().or_calc_power_spectrum(ct.byref(p), dp(v1), dp(v2), dp(kb), float(np.real(sn)), float(np.imag(sn)), ell, m,
+                                 dp(pk), nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+    return pk[0::2] + 1j * pk[1::2], nm
+
+
+def ifft_for_bispectrum(p: Param, density, kmag, dk):
+    a, v = _c(density)
+    out = np.zeros(shape(p) + (2,))
+    n = lib().or_ifft_for_bispectrum(ct.byref(p), dp(v), float(kmag), float(dk), None, dp(out))
+    return out[..., 0] + 1j * out[..., 1], n
+
+
+def box_pk(p: Param, pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    kb = kbin(p)
+    out = np.zeros(2 * p.n_kbin)
+    nm = np.zeros(p.n_kbin, dtype=np.int32)
+    rc = lib().or_box_pk(ct.byref(p), dp(pos), pos.shape[0], dp(kb), dp(out), nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+    if rc:
+        raise ValueError("or_box_pk rejected the multipole combination")
+    return out[0::2] + 1j * out[1::2], nm
+
+
+def box_xi(p: Param, pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    rb = rbin(p)
+    out = np.zeros(p.n_rbin)
+    rc = lib().or_box_xi(ct.byref(p), dp(pos), pos.shape[0], dp(rb), dp(out))
+    if rc:
+        raise ValueError("or_box_xi rejected the multipole combination")
+    return out
+
+
+def box_bk(p: Param, pos):
+    """bispec.hpp:877-1203 for p.ith_kbin: (out, bk_save, sn_save), complex arrays."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    kb = kbin(p)
+    bk = np.zeros(2 * p.n_kbin)
+    sn = np.zeros(2 * p.n_kbin)
+    out = np.zeros(2 * p.n_kbin)
+    rc = lib().or_box_bk(ct.byref(p), dp(pos), pos.shape[0], dp(kb), dp(bk), dp(sn), dp(out))
+    if rc:
+        raise ValueError("or_box_bk rejected the multipole combination")
+    c = lambda a: a[0::2] + 1j * a[1::2]
+    return c(out), c(bk), c(sn)

@@ -1,0 +1,151 @@
+"""Periodic-box P_l and B_000 (hm_box_plan) against the oracle's restatement of
+calcPowerSpectrumForBOX (powerspec.hpp:273-332) and calcBiSpectrumForBOX
+(bispec.hpp:877-1203, one oracle job per ith_kbin).
+
+Tolerances (stated per quantity):
+  - mode counts (P bins and B shells): bit-exact;
+  - P_l: |dP_l(k)| <= tol * (|P_0(k)| + V/N), tol = 1e-10 for FP64 grids and
+    1e-5 for FP32 grids (FP64 bin sums in both);
+  - B: |dB(k1,k2)| <= tol * max|B| with tol = 1e-9 (FP64) / 2e-4 (FP32); the
+    bk and sn sums are checked separately with the same tolerances.
+"""
+import numpy as np
+import pytest
+
+import hitomi_amd as hm
+from oracle import pyoracle as po
+
+from .conftest import clustered_catalogue, uniform_catalogue
+
+pytestmark = pytest.mark.gpu
+
+PK_TOL = {hm.FP64: 1e-10, hm.FP32: 1e-5}
+BK_TOL = {hm.FP64: 1e-9, hm.FP32: 2e-4}
+
+
+def oracle_pk(n, L, pos, kmin, kmax, nk, ells=(0, 2, 4)):
+    out = []
+    for l in ells:
+        p = po.param(n, L, po.TSC, kmin=kmin, kmax=kmax, n_kbin=nk, ell1=l, ell2=0, ELL=l)
+        pk, nm = po.box_pk(p, pos)
+        out.append(pk)
+    return np.array(out), nm
+
+
+def check_pk(res, ref_pk, ref_nm, prec, V, N):
+    assert np.array_equal(res["pk_nmode"], ref_nm.astype(np.int64))
+    got = res["pk"][..., 0] + 1j * res["pk"][..., 1]
+    scale = np.abs(ref_pk[0].real) + V / N
+    assert np.all(np.abs(got - ref_pk) <= PK_TOL[prec] * scale[None, :]), np.abs(got - ref_pk) / scale
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+def test_pk_config1_catalogue(ctx, prec):
+    """128^3, 2e5 particles, L = 1000, z-RSD sigma = 5, bins 0.005..0.195 (20)."""
+    n, L, N = 128, 1000.0, 200000
+    pos = uniform_catalogue(N, L, seed=1, rsd_sigma=5.0)
+    ref_pk, ref_nm = oracle_pk(n, L, pos, 0.005, 0.195, 20)
+    plan = hm.BoxPlan(ctx, n, L, "TSC", n_kbin=20, kmin=0.005, kmax=0.195, pk_ells=(0, 2, 4), bk=False, precision=prec)
+    res = plan.execute(pos)
+    check_pk(res, ref_pk, ref_nm, prec, L ** 3, N)
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+def test_pk_low_k_pruned(ctx, prec):
+    """Bins far below Nyquist: the binning visits only the low-k cube."""
+    n, L, N = 64, 2000.0, 30000
+    pos = clustered_catalogue(N, L, seed=2)
+    ref_pk, ref_nm = oracle_pk(n, L, pos, 0.01, 0.04, 7)
+    plan = hm.BoxPlan(ctx, n, L, "TSC", n_kbin=7, kmin=0.01, kmax=0.04, pk_ells=(0, 2, 4), bk=False, precision=prec)
+    check_pk(plan.execute(pos), ref_pk, ref_nm, prec, L ** 3, N)
+
+
+_ORACLE_CACHE = {}
+
+
+def oracle_bk_all(n, L, pos, kmin, kmax, nk):
+    key = (n, L, pos.shape[0], kmin, kmax, nk, float(pos.sum()))
+    if key not in _ORACLE_CACHE:
+        _ORACLE_CACHE[key] = _oracle_bk_all(n, L, pos, kmin, kmax, nk)
+    return _ORACLE_CACHE[key]
+
+
+def _oracle_bk_all(n, L, pos, kmin, kmax, nk):
+    out = np.zeros((nk, nk), complex)
+    bks = np.zeros((nk, nk), complex)
+    sns = np.zeros((nk, nk), complex)
+    for ith in range(nk):
+        p = po.param(n, L, po.TSC, kmin=kmin, kmax=kmax, n_kbin=nk, ith_kbin=ith)
+        o, b, s = po.box_bk(p, pos)
+        out[ith], bks[ith], sns[ith] = o, b, s
+    return out, bks, sns
+
+
+def shell_counts(n, L, kmin, kmax, nk):
+    p = po.param(n, L, po.TSC, kmin=kmin, kmax=kmax, n_kbin=nk)
+    kb = po.kbin(p)
+    one = np.ones((n, n, n), complex)
+    return np.array([po.ifft_for_bispectrum(p, one, k, kb[1] - kb[0])[1] for k in kb])
+
+
+BK_CASES = [
+    # n, L, N, kmin, kmax, nk, expected triple mesh
+    (64, 2000.0, 20000, 0.01, 0.025, 4, 48),   # exact re-grid 64 -> 48
+    (32, 1000.0, 20000, 0.02, 0.08, 4, 32),    # shells pruned, no re-grid
+]
+
+
+@pytest.mark.parametrize("case", BK_CASES)
+@pytest.mark.parametrize("mode", ["fp64", "fp64_plain", "fp32"])
+def test_bk000_all_pairs(ctx, case, mode):
+    n, L, N, kmin, kmax, nk, M = case
+    pos = clustered_catalogue(N, L, seed=n + nk, nclump=30, sigma=30.0)
+    ref, ref_bk, ref_sn = oracle_bk_all(n, L, pos, kmin, kmax, nk)
+    prec = hm.FP32 if mode == "fp32" else hm.FP64
+    plain = mode == "fp64_plain"
+    plan = hm.BoxPlan(ctx, n, L, "TSC", n_kbin=nk, kmin=kmin, kmax=kmax, pk_ells=(0,), bk=True, precision=prec,
+                      regrid=not plain, prune=not plain)
+    assert plan.triple_mesh == ((n, n, n) if plain else (M, M, M))
+    res = plan.execute(pos)
+    assert np.array_equal(res["shell_nmode"], shell_counts(n, L, kmin, kmax, nk))
+    c = lambda a: a[..., 0] + 1j * a[..., 1]
+    tol = BK_TOL[prec]
+    assert np.abs(c(res["bk_sum"]) - ref_bk).max() <= tol * np.abs(ref_bk).max()
+    assert np.abs(c(res["sn_sum"]) - ref_sn).max() <= tol * np.abs(ref_sn).max()
+    assert np.abs(c(res["bk"]) - ref).max() <= tol * np.abs(ref).max()
+
+
+def test_fp32_pipeline_bitwise_deterministic(ctx):
+    n, L, N = 64, 2000.0, 40000
+    pos = clustered_catalogue(N, L, seed=5)
+    plan = hm.BoxPlan(ctx, n, L, "TSC", n_kbin=4, kmin=0.01, kmax=0.025, pk_ells=(0, 2, 4), bk=True, precision=hm.FP32)
+    a = plan.execute(pos)
+    perm = np.random.default_rng(0).permutation(N)
+    b = plan.execute(pos[perm])
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_metric_mesh_512_properties(ctx):
+    """512^3, L = 2000, 4e6 particles, 20 bins 0.015..0.205: FP32 P_l agree with the
+    FP64 plan to 1e-5 (P_0 + SN); B is symmetric in (k1, k2); repeat runs are bitwise equal."""
+    n, L, N = 512, 2000.0, 4_000_000
+    pos = clustered_catalogue(N, L, seed=42, nclump=400, sigma=15.0)
+    kw = dict(n_kbin=20, kmin=0.015, kmax=0.205, pk_ells=(0, 2, 4), bk=True)
+    p32 = hm.BoxPlan(ctx, n, L, "TSC", precision=hm.FP32, **kw)
+    dpos = ctx.array(pos)
+    r1 = p32.execute(dpos)
+    r2 = p32.execute(dpos)
+    for k in r1:
+        assert np.array_equal(r1[k], r2[k]), k
+    p32.close()
+    p64 = hm.BoxPlan(ctx, n, L, "TSC", precision=hm.FP64, **kw)
+    r64 = p64.execute(dpos)
+    p64.close()
+    assert np.array_equal(r1["pk_nmode"], r64["pk_nmode"])
+    scale = np.abs(r64["pk"][0, :, 0]) + L ** 3 / N
+    assert np.all(np.abs(r1["pk"][..., 0] - r64["pk"][..., 0]) <= 1e-5 * scale[None, :])
+    b = r1["bk"][..., 0]
+    assert np.allclose(b, b.T, rtol=0, atol=0)
+    b64 = r64["bk"][..., 0]
+    assert np.abs(b - b64).max() <= 2e-4 * np.abs(b64).max()

@@ -1,0 +1,300 @@
+"""Pins the FP64 CPU oracle (oracle/hm_oracle.c) before anything is compared
+against it.
+
+What pins what (reference file:line of the restated code in brackets):
+  - FFT [fftw_plan_dft_3d, field.hpp:321/339]: numpy.fft fixtures (golden),
+    including prime and non-power-of-two lengths; Parseval; impulse.
+  - Wigner 3j [common.hpp:29]: sympy exact values (golden).
+  - sphPlm and reduced Y_lm [tools.hpp:12-68]: sympy assoc_legendre / Ynm (golden).
+  - j_l and its spline [bessel.hpp:10-37]: mpmath (golden); spline vs closed form.
+  - mass assignment [field.hpp:145-184]: single particle on a node gives
+    (1/8, 3/4, 1/8)^3; mass conservation; hand-derived Q1 spill/drop coords;
+    an independent vectorised TSC.
+  - window / shot-noise functions [field.hpp:700-717, 1293-1309]: hand values
+    incl. the truncated-index quirk W(-n) = W(n-1).
+  - P binning [field.hpp:765-852]: exact mode counts from a vectorised count;
+    two-mode plane wave gives the closed-form bin value.
+  - B [bispec.hpp:1100-1182]: brute-force triangle sum in Fourier space equals
+    the FFT estimator; shot-noise ijk term equals a numpy/scipy evaluation.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.special import spherical_jn
+
+from oracle import pyoracle as po
+
+from .conftest import clustered_catalogue
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def special():
+    with open(os.path.join(GOLD, "special.json")) as f:
+        return json.load(f)
+
+
+def test_fft_golden():
+    fx = np.load(os.path.join(GOLD, "fft_fixtures.npz"))
+    for key in fx.files:
+        if key.startswith("in_"):
+            shape = tuple(int(s) for s in key[3:].split("x"))
+            p = po.param(shape, (100.0, 110.0, 120.0))
+            a = fx[key]
+            f = po.fft3d(p, a, -1)
+            b = po.fft3d(p, a, +1)
+            ref_f, ref_b = fx["fwd_" + key[3:]], fx["bwd_" + key[3:]]
+            assert np.abs(f - ref_f).max() <= 1e-12 * np.abs(ref_f).max()
+            assert np.abs(b - ref_b).max() <= 1e-12 * np.abs(ref_b).max()
+        if key.startswith("in1_"):
+            n = key[4:]
+            f = po.fft1d(fx[key], -1)
+            assert np.abs(f - fx["fwd1_" + n]).max() <= 1e-12 * np.abs(fx["fwd1_" + n]).max()
+
+
+def test_fft_impulse_and_parseval():
+    p = po.param((8, 12, 16), 100.0)
+    a = np.zeros((8, 12, 16), complex)
+    a[3, 5, 7] = 2.0 - 1.0j
+    f = po.fft3d(p, a, -1)
+    i, j, k = np.meshgrid(np.arange(8), np.arange(12), np.arange(16), indexing="ij")
+    expect = (2.0 - 1.0j) * np.exp(-2j * np.pi * (3 * i / 8 + 5 * j / 12 + 7 * k / 16))
+    assert np.allclose(f, expect, atol=1e-13)
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal((8, 12, 16)) + 1j * rng.standard_normal((8, 12, 16))
+    fb = po.fft3d(p, b, -1)
+    assert np.isclose(np.sum(np.abs(fb) ** 2), b.size * np.sum(np.abs(b) ** 2), rtol=1e-13)
+
+
+def test_wigner3j_golden(special):
+    for j1, j2, j3, m1, m2, m3, v in special["wigner3j"]:
+        assert po.wigner3j(j1, j2, j3, m1, m2, m3) == pytest.approx(v, abs=1e-14)
+    assert po.wigner3j(1, 1, 1, 0, 0, 0) == 0.0  # odd sum with zero m
+    assert po.wigner3j(2, 2, 2, 1, 1, 1) == 0.0  # m1 + m2 + m3 != 0
+
+
+def test_sphplm_and_ylm_golden(special):
+    for l, m, x, v in special["sphPlm"]:
+        assert po.sph_plm(l, m, x) == pytest.approx(v, rel=1e-12, abs=1e-14)
+    for l, m, v, re, im in special["ylm_reduced"]:
+        y = po.ylm(l, m, v)
+        assert y.real == pytest.approx(re, abs=1e-12) and y.imag == pytest.approx(im, abs=1e-12), (l, m, v)
+    # tools.hpp:18-20 and 32-34
+    assert po.ylm(0, 0, [0.0, 0.0, 0.0]) == 1.0
+    assert po.ylm(2, 1, [0.0, 0.0, 0.0]) == 0.0
+
+
+def test_bessel_golden(special):
+    for l, x, v in special["bessel_jl"]:
+        got = po.bessel_jl(l, x)
+        assert got == pytest.approx(v, rel=1e-11, abs=1e-15), (l, x)
+    # the spline interpolates j_l to ~1e-10 away from the natural end condition
+    for ell in (0, 1, 2):
+        sp = po.BesselSpline(ell)
+        for x in [0.0, 0.07, 0.5, 1.234, 17.5, 333.3, 9999.99]:
+            assert sp(x) == pytest.approx(po.bessel_jl(ell, x), abs=2e-10)
+        assert np.isnan(sp(10000.5))  # outside the spline domain (bessel.hpp:13)
+
+
+def test_tsc_single_particle_on_node():
+    L, n = 100.0, 16
+    p = po.param(n, L, po.TSC)
+    dx = L / n
+    pos = np.array([[5 * dx, 7 * dx, 9 * dx]])
+    f = po.calc_field(p, pos).real
+    dV = dx ** 3
+    w = np.array([0.125, 0.75, 0.125])
+    expect = np.einsum("i,j,k->ijk", w, w, w) / dV
+    assert np.allclose(f[4:7, 6:9, 8:11], expect, rtol=1e-14, atol=0)
+    assert np.isclose(f.sum() * dV, 1.0, rtol=1e-14)
+
+
+@pytest.mark.parametrize("assign", [po.NGP, po.CIC, po.TSC])
+def test_mass_conservation_interior(assign):
+    L, n = 200.0, 16
+    p = po.param(n, L, assign)
+    rng = np.random.default_rng(3)
+    pos = 2 * L / n + rng.random((500, 3)) * (L - 4 * L / n)
+    f = po.calc_field(p, pos)
+    assert np.isclose(f.real.sum() * (L / n) ** 3, 500.0, rtol=1e-13)
+
+
+def test_q1_unwrapped_stencil_coords():
+    """field.hpp:175-176: a z index of n spills into (ix, iy + 1, 0); an x index
+    of n or -1 is dropped, a y index of n spills into the next x plane."""
+    L, n = 100.0, 8
+    dx = L / n
+    p = po.param(n, L, po.TSC)
+    pos = np.array([[3.2 * dx, 4.1 * dx, (n - 0.3) * dx],    # iz1 = n -> stencil n-1, n, n+1
+                    [(n - 0.2) * dx, 2.0 * dx, 2.0 * dx],    # ix1 = n -> n-1, n, n+1
+                    [0.2 * dx, 2.0 * dx, 2.0 * dx]])         # ix1 = 0 -> -1, 0, 1
+    c = po.cell_coords(p, pos)
+    ix, iy = 3, 4
+    got = set(c[0].tolist())
+    for di in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for z in (n - 1, n, n + 1):
+                lin = ((ix + di) * n + (iy + dj)) * n + z
+                assert lin in got
+    # z = n and n + 1 spill into the next row: (3, 4, n) is the linear index of (3, 5, 0)
+    assert ((3 * n + 5) * n + 0) in got and ((3 * n + 5) * n + 1) in got
+    # x = n + 1 row: dropped (all coords >= n^3 become -1)
+    assert (c[1].reshape(3, 3, 3)[2] == -1).all()
+    assert (c[1].reshape(3, 3, 3)[1] == -1).all()   # x = n
+    assert (c[1].reshape(3, 3, 3)[0] >= 0).all()    # x = n - 1
+    assert (c[2].reshape(3, 3, 3)[0] == -1).all()   # x = -1
+
+
+def test_tsc_against_independent_vectorised():
+    L, n = 300.0, 16
+    p = po.param(n, L, po.TSC)
+    rng = np.random.default_rng(5)
+    pos = 2 * L / n + rng.random((300, 3)) * (L - 4 * L / n)
+    f = po.calc_field(p, pos).real
+    xp = pos * n / L
+    i1 = np.floor(xp + 0.5).astype(int)
+    d = xp - i1
+    ws = [0.5 * (0.5 - d) ** 2, 0.75 - d ** 2, 0.5 * (0.5 + d) ** 2]
+    g = np.zeros((n, n, n))
+    for a in range(3):
+        for b in range(3):
+            for c in range(3):
+                np.add.at(g, (i1[:, 0] + a - 1, i1[:, 1] + b - 1, i1[:, 2] + c - 1), ws[a][:, 0] * ws[b][:, 1] * ws[c][:, 2])
+    assert np.allclose(f, g / (L / n) ** 3, rtol=1e-12, atol=1e-15)
+
+
+def test_window_quirk_values():
+    L, n = 1000.0, 32
+    p = po.param(n, L, po.TSC)
+    dk = 2 * np.pi / L
+    sinc = lambda m: np.sin(np.pi * m / n) / (np.pi * m / n)
+    assert po.assignment_function(p, [dk * 1, 0, 0]) == pytest.approx(sinc(1) ** 3, rel=1e-15)
+    assert po.assignment_function(p, [-dk * 1, 0, 0]) == 1.0          # |n| - 1 = 0
+    for m in (2, 5, 15):
+        assert po.assignment_function(p, [0, -dk * m, 0]) == pytest.approx(sinc(m - 1) ** 3, rel=1e-15)
+    assert po.assignment_function(p, [0, 0, -dk * 16]) == pytest.approx(sinc(15) ** 3, rel=1e-15)  # Nyquist
+    s = np.sin(np.pi * 3 / n)
+    assert po.shotnoise_function(p, [dk * 3, 0, 0]) == pytest.approx(1 - s * s + 2 * s ** 4 / 15, rel=1e-15)
+    pc = po.param(n, L, po.CIC)
+    assert po.assignment_function(pc, [dk * 2, dk, 0]) == pytest.approx((sinc(2) * sinc(1)) ** 2, rel=1e-15)
+    assert po.shotnoise_function(pc, [0, -dk * 4, 0]) == pytest.approx(1 - 2 / 3 * np.sin(np.pi * 3 / n) ** 2,
+                                                                        rel=1e-15)
+
+
+def _mode_counts_numpy(n, L, kbin):
+    """Fine histogram (1e-4) + coarse fold of field.hpp:793-837, vectorised."""
+    dk = 2 * np.pi / L
+    idx = np.arange(n)
+    kv = np.where(idx < n // 2, idx, idx - n) * dk
+    kx, ky, kz = np.meshgrid(kv, kv, kv, indexing="ij")
+    kmag = np.sqrt(kx * kx + ky * ky + kz * kz)
+    kindex = (kmag / 0.0001 + 0.5).astype(np.int64)
+    hist = np.bincount(kindex[kindex < 100000].ravel(), minlength=100000)
+    dkb = kbin[1] - kbin[0]
+    centres = np.arange(100000) * 0.0001
+    out = []
+    for kb in kbin:
+        lo = kb - dkb * 0.5 if kb > dkb * 0.5 else 0.0
+        hi = kb + dkb * 0.5
+        out.append(int(hist[(centres > lo) & (centres <= hi)].sum()))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("n,L,kmin,kmax,nk", [(16, 500.0, 0.01, 0.2, 20), (24, 1000.0, 0.005, 0.195, 20)])
+def test_pk_mode_counts(n, L, kmin, kmax, nk):
+    p = po.param(n, L, po.TSC, kmin=kmin, kmax=kmax, n_kbin=nk)
+    kb = po.kbin(p)
+    f = np.ones((n, n, n), complex)
+    _, nm = po.calc_power_spectrum(p, f, f, kb, 0.0, 0, 0)
+    assert (nm == _mode_counts_numpy(n, L, kb)).all()
+
+
+def test_pk_plane_wave_closed_form():
+    n, L = 16, 400.0
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.2, n_kbin=10)
+    kb = po.kbin(p)
+    f = np.zeros((n, n, n), complex)
+    A = 3.0 + 4.0j
+    f[2, 1, 0] = A
+    f[-2, -1, 0] = np.conj(A)
+    pk, nm = po.calc_power_spectrum(p, f, f, kb, 0.0, 0, 0)
+    dk = 2 * np.pi / L
+    kmag = np.sqrt(5.0) * dk
+    W1 = po.assignment_function(p, [2 * dk, dk, 0.0])
+    W2 = po.assignment_function(p, [-2 * dk, -dk, 0.0])
+    j = int(np.argmin(np.abs(kb - kmag)))
+    expect = (abs(A) ** 2 / W1 ** 2 + abs(A) ** 2 / W2 ** 2) / nm[j]
+    assert pk[j].real == pytest.approx(expect, rel=1e-13)
+    others = np.delete(np.arange(len(kb)), j)
+    assert np.all(pk[others] == 0)
+
+
+def test_box_bk_bruteforce_triangles():
+    """bk_save of bispec.hpp:1159-1168 equals the direct Fourier-space triangle sum."""
+    n, L = 12, 600.0
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.05, n_kbin=4, ith_kbin=1)
+    kb = po.kbin(p)
+    pos = clustered_catalogue(3000, L, seed=7, nclump=6, sigma=40.0)
+    out, bk_save, sn_save = po.box_bk(p, pos)
+    # brute force
+    d = po.fourier_transform(p, po.box_density_fluctuation(p, pos))
+    dk = 2 * np.pi / L
+    idx = np.arange(n)
+    sgn = np.where(idx < n // 2, idx, idx - n)
+    modes = [(i, j, k) for i in range(n) for j in range(n) for k in range(n)]
+    kvec = {m: np.array([sgn[m[0]], sgn[m[1]], sgn[m[2]]], float) * dk for m in modes}
+    W = {m: po.assignment_function(p, kvec[m]) for m in modes}
+    dkb = kb[1] - kb[0]
+
+    def shell(kc):
+        lo = kc - dkb * 0.5 if kc > dkb * 0.5 else 0.0
+        hi = kc + dkb * 0.5
+        return [m for m in modes if lo < np.sqrt(np.sum(kvec[m] ** 2)) <= hi]
+
+    V, Ng = L ** 3, n ** 3
+    s1 = shell(kb[1])
+    for ik in range(len(kb)):
+        s2 = shell(kb[ik])
+        tot = 0.0
+        for q1 in s1:
+            a1 = d[q1] / W[q1]
+            for q2 in s2:
+                q3 = tuple((-(q1[a] + q2[a])) % n for a in range(3))
+                tot += a1 * (d[q2] / W[q2]) * (d[q3] / W[q3])
+        expect = (V / Ng) * Ng / (len(s1) * len(s2) * V) * tot
+        assert bk_save[ik] == pytest.approx(expect, rel=1e-10, abs=1e-12 * abs(expect) + 1e-300)
+
+
+def test_box_bk_shotnoise_ijk_numpy():
+    """sn_save of bispec.hpp:939-1056 for B_000 from numpy.fft + scipy j0."""
+    n, L, N = 12, 600.0, 2000
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.05, n_kbin=4, ith_kbin=2)
+    kb = po.kbin(p)
+    pos = clustered_catalogue(N, L, seed=9, nclump=5, sigma=50.0)
+    out, bk_save, sn_save = po.box_bk(p, pos)
+    d = po.fourier_transform(p, po.box_density_fluctuation(p, pos))
+    nn = d.copy()
+    nn[0, 0, 0] += N
+    pk, _ = po.calc_power_spectrum(p, d, nn, kb, float(N), 0, 0)
+    dk = 2 * np.pi / L
+    idx = np.arange(n)
+    sgn = np.where(idx < n // 2, idx, idx - n)
+    g = np.zeros((n, n, n), complex)
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                kv = np.array([sgn[i], sgn[j], sgn[k]], float) * dk
+                S = po.shotnoise_function(p, kv)
+                W = po.assignment_function(p, kv)
+                g[i, j, k] = (d[i, j, k] * np.conj(nn[i, j, k]) - N * S) / W ** 2 / L ** 3
+    xi = np.fft.ifftn(g) * g.size
+    dr = L / n
+    rx, ry, rz = np.meshgrid(sgn * dr, sgn * dr, sgn * dr, indexing="ij")
+    r = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+    for ik in range(len(kb)):
+        ijk = np.sum(spherical_jn(0, kb[2] * r) * spherical_jn(0, kb[ik] * r) * xi) * (L ** 3 / n ** 3)
+        expect = N + pk[2] + pk[ik] + ijk
+        assert sn_save[ik] == pytest.approx(expect, rel=1e-9, abs=1e-9 * abs(expect))
