@@ -21,6 +21,11 @@ FP32, FP64 = 0, 1
 ASSIGN = {"NGP": NGP, "CIC": CIC, "TSC": TSC}
 MAX_ELLS = 4
 
+# Launch names the box pipeline reports to hm_timing_get().
+BOX_KERNELS = ("fill_zero", "assign_fixed", "fft_r2c_z", "fft_r2c_y", "fft_r2c_x", "bin_pk", "bin_reduce",
+               "xi_pass_x", "xi_pass_y", "xi_pass_z", "xi_absmax", "xi_hist", "sn_pairs", "tri_pass_x",
+               "tri_pass_y", "tri_pass_z", "gram", "gram_reduce")
+
 
 class HitomiError(RuntimeError):
     pass
@@ -72,6 +77,7 @@ _SIGS = {
     "hm_timing_enable": (ct.c_int, [_VP, ct.c_int]),
     "hm_timing_get": (ct.c_int, [_VP, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]),
     "hm_timing_reset": (ct.c_int, [_VP]),
+    "hm_timing_select": (ct.c_int, [_VP, ct.c_char_p]),
     "hm_assign": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
     "hm_cell_coords": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_longlong, _VP]),
     "hm_fft_c2c": (ct.c_int, [_VP, ct.c_int, _VP, ct.POINTER(ct.c_int), ct.c_int]),
@@ -201,6 +207,9 @@ class Context:
         n = ct.c_longlong()
         self.check(self.lib.hm_timing_get(self.h, name.encode(), ct.byref(ms), ct.byref(n)), "hm_timing_get")
         return ms.value, n.value
+
+    def timing_select(self, name=None):
+        self.check(self.lib.hm_timing_select(self.h, name.encode() if name else None), "hm_timing_select")
 
     def timing_reset(self):
         self.check(self.lib.hm_timing_reset(self.h), "hm_timing_reset")

@@ -159,6 +159,12 @@ int hm_timing_reset(hm_ctx* ctx) {
     return 0;
 }
 
+int hm_timing_select(hm_ctx* ctx, const char* name) {
+    if (!ctx) return -1;
+    ctx->timing_only = name ? name : "";
+    return 0;
+}
+
 int hm_assign(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
               double wabs_sum, double* d_out_re, double* d_out_im) {
     if (!ctx) return -1;

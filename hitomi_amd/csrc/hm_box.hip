@@ -444,7 +444,7 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
     const Mesh& m = p->mesh;
     if (alloc(ctx, p, p->fixed, sizeof(long long) * (size_t)m.ng())) return -1;
     if (alloc(ctx, p, p->dk, sizeof(C<T>) * (size_t)m.nh())) return -1;
-    const size_t ncols = (size_t)p->nk * p->ne;
+    const size_t ncols = (size_t)p->nk * (size_t)std::max(p->ne, 1);
     if (alloc(ctx, p, p->pk_part, sizeof(double) * ncols * (size_t)p->nrows)) return -1;
     if (alloc(ctx, p, p->pk_cnt, sizeof(long long) * (size_t)p->nk * (size_t)p->nrows)) return -1;
     if (alloc(ctx, p, p->pk_red, sizeof(double) * ncols)) return -1;

@@ -16,7 +16,7 @@ hipEvent_t hm_ctx::get_event() {
 }
 
 int hm_ctx::launch(const char* name, const std::function<void()>& fn) {
-    const bool t = timing && !capturing;
+    const bool t = timing && !capturing && (timing_only.empty() || timing_only == name);
     hipEvent_t a = nullptr, b = nullptr;
     if (t) {
         a = get_event();

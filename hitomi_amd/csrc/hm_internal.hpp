@@ -16,6 +16,7 @@ struct hm_ctx {
     std::map<std::pair<int, int>, void*> twiddles;
     // kernel timing (hipEvents on the launch stream), off by default
     bool timing = false;
+    std::string timing_only;  // empty: time every launch
     struct Pending {
         std::string name;
         hipEvent_t a, b;

@@ -62,6 +62,8 @@ int hm_memset_zero(hm_ctx *ctx, void *d_ptr, size_t bytes);
 int hm_timing_enable(hm_ctx *ctx, int on);
 int hm_timing_get(hm_ctx *ctx, const char *name, double *total_ms, long long *launches);
 int hm_timing_reset(hm_ctx *ctx);
+/* Restrict the timed launches to one kernel name (NULL or "" times all). */
+int hm_timing_select(hm_ctx *ctx, const char *name);
 
 /* ---------------------------------------------------------------- */
 /* mass assignment                                                    */
