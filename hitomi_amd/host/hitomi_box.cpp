@@ -1,0 +1,206 @@
+// hitomi_box -- drop-in for the periodic-box program of hitomi_measurement
+// (main_box.cpp:13-242): `hitomi_box param.ini`.
+//
+// Reads the same .ini keys, the same Gadget-1 / Rockstar inputs, writes the
+// same input_parameters, simulation_parameters and pk%d.dat /
+// bk%d%d%d_%02d.dat files, and exits with status 1 where the reference calls
+// EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
+//
+// Supported now: measure = pk (any ELL, ell1 = ELL, ell2 = 0) and measure = bk
+// for (ell1, ell2, ELL) = (0, 0, 0), flag_recon = False.  Other measurements
+// are rejected with an error instead of being computed differently.
+#include "../../include/hitomi_amd.h"
+#include "hm_catalog.hpp"
+#include "hm_param.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+using namespace hmhost;
+
+static double g_t0 = 0.0;
+static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+static void stage(const char* what) { printf("%s: %.3f sec.\n", what, now() - g_t0); }
+
+[[noreturn]] static void EXIT() { std::exit(1); }
+
+static int assign_code(const std::string& a) { return a == "NGP" ? HM_NGP : (a == "CIC" ? HM_CIC : HM_TSC); }
+
+struct Gpu {
+    hm_ctx* ctx = nullptr;
+    double* d_pos = nullptr;
+    ~Gpu() {
+        if (ctx && d_pos) hm_dev_free(ctx, d_pos);
+        if (ctx) hm_ctx_destroy(ctx);
+    }
+};
+
+static int run_box(const Params& p, const Catalog& P_D, Gpu& g, const double* kbin, std::vector<double>& pk,
+                   std::vector<double>& bk, bool want_bk, int ell) {
+    hm_box_config cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    for (int a = 0; a < 3; a++) {
+        cfg.mesh.n_mesh[a] = p.n_mesh[a];
+        cfg.mesh.boxsize[a] = p.boxsize[a];
+    }
+    cfg.mesh.assign = assign_code(p.assign);
+    cfg.precision = p.precision == "FP64" ? HM_FP64 : HM_FP32;
+    cfg.n_kbin = p.n_kbin;
+    cfg.kmin = p.kmin;
+    cfg.kmax = p.kmax;
+    cfg.n_pk_ells = 1;
+    cfg.pk_ells[0] = want_bk ? 0 : ell;
+    cfg.bk = want_bk ? 1 : 0;
+    cfg.regrid = 1;
+    cfg.prune = 1;
+    (void)kbin;
+    hm_box_plan* plan = hm_box_plan_create(g.ctx, &cfg);
+    if (!plan) {
+        printf("hm_box_plan_create: %s\n", hm_last_error(g.ctx));
+        return -1;
+    }
+    const int nk = p.n_kbin;
+    pk.assign(2 * (size_t)nk, 0.0);
+    bk.assign(2 * (size_t)nk * nk, 0.0);
+    hm_box_result res;
+    std::memset(&res, 0, sizeof(res));
+    res.pk = pk.data();
+    if (want_bk) res.bk = bk.data();
+    int rc = hm_box_plan_execute(g.ctx, plan, g.d_pos, P_D.n_tot, &res);
+    if (rc) printf("hm_box_plan_execute: %s\n", hm_last_error(g.ctx));
+    hm_box_plan_destroy(g.ctx, plan);
+    return rc;
+}
+
+int main(int argc, char* argv[]) {
+    g_t0 = now();
+    printf("Start: %.3f sec.\n", 0.0);
+    stage("Reading input parameters");
+    if (argc != 2) {
+        printf("The input parameter file is not found.\n");
+        printf("For example, run './a.out param.ini'\n");
+        EXIT();
+    }
+    Params p;
+    if (p.read(argv[1])) {
+        printf("'param.readParameters' fails.\n");
+        EXIT();
+    }
+    if (p.make_output_directory()) {
+        printf("'param.makeOutputDirectory' fails.\n");
+        EXIT();
+    }
+    if (p.print_parameters()) {
+        printf("'param.printParameters' fails.\n");
+        EXIT();
+    }
+    std::vector<double> kbin(p.n_kbin), rbin(p.n_rbin);
+    set_kbin(p, kbin.data());
+    set_rbin(p, rbin.data());
+    stage("Done");
+
+    stage("Reading particle data");
+    Catalog P_D;
+    if (p.sim_data == "Gadget") {
+        if (P_D.read_gadget(p.data_file)) {
+            printf("'P_D.readParticlesFromGadgetSnapshot(param.data_file)' fails.\n");
+            EXIT();
+        }
+    } else if (p.sim_data == "Rockstar") {
+        if (P_D.read_rockstar(p.data_file, p.log10_Mmin, p.log10_Mmax)) {
+            printf("'P_D.readParticlesFromRockstar(param.data_file, param.log10_Mmin, param.log10_Mmax)' fails.\n");
+            EXIT();
+        }
+    } else {
+        EXIT();
+    }
+    // particleGadget.hpp:643-653: the box size comes from the catalogue
+    for (int a = 0; a < 3; a++) p.boxsize[a] = P_D.boxsize;
+    p.volume = p.boxsize[0] * p.boxsize[1] * p.boxsize[2];
+    if (p.flag_RSD == "True") {
+        if (P_D.set_rsd(p.boxsize)) {
+            printf("'P_D.setRSD(param)' fails.\n");
+            EXIT();
+        }
+    }
+    P_D.print_parameters(p.output_dir);
+    stage("Done");
+
+    if (p.flag_recon == "True") {
+        printf("flag_recon = True: the reconstruction path is not available in this build.\n");
+        EXIT();
+    }
+
+    printf("Calculating '%s': %.3f sec.\n", p.measure.c_str(), now() - g_t0);
+    Gpu g;
+    g.ctx = hm_ctx_create(0, 0);
+    if (!g.ctx) {
+        printf("no usable GPU (hm_ctx_create failed)\n");
+        EXIT();
+    }
+    g.d_pos = (double*)hm_dev_alloc(g.ctx, sizeof(double) * 3 * (size_t)P_D.n_tot);
+    if (!g.d_pos || hm_memcpy_h2d(g.ctx, g.d_pos, P_D.pos.data(), sizeof(double) * 3 * (size_t)P_D.n_tot)) {
+        printf("particle upload failed: %s\n", hm_last_error(g.ctx));
+        EXIT();
+    }
+    const int nk = p.n_kbin;
+    std::vector<double> pk, bk;
+    if (p.measure == "pk") {
+        // powerspec.hpp:280-284
+        if (!((p.ELL == p.ell1) && (p.ell2 == 0))) {
+            printf("This multipole combination is not allowed.\n");
+            printf("It should be ELL == ell1 and ell2 == 0\n");
+            printf("'calcPowerSpectrumForBOX(P_D, param, kbin)' fails.\n");
+            EXIT();
+        }
+        if (run_box(p, P_D, g, kbin.data(), pk, bk, false, p.ELL)) {
+            printf("'calcPowerSpectrumForBOX(P_D, param, kbin)' fails.\n");
+            EXIT();
+        }
+        // powerspec.hpp:318-327
+        char buf[2048];
+        snprintf(buf, sizeof(buf), "%s/pk%d.dat", p.output_dir.c_str(), p.ELL);
+        FILE* fp = fopen(buf, "w");
+        if (!fp) EXIT();
+        fprintf(fp, "## k [h/Mpc]\t pk_real [(Mpc/h)^3]\t pk_imag [(Mpc/h)^3]\n");
+        for (int i = 0; i < nk; i++) fprintf(fp, "%.5f \t %.7e \t %.7e\n", kbin[i], pk[2 * i], pk[2 * i + 1]);
+        fclose(fp);
+    } else if (p.measure == "bk") {
+        if (!(p.ell1 == 0 && p.ell2 == 0 && p.ELL == 0)) {
+            printf("measure = bk with (ell1, ell2, ELL) = (%d, %d, %d) is not available in this build; only (0, 0, 0).\n",
+                   p.ell1, p.ell2, p.ELL);
+            printf("'calcBiSpectrumForBOX(P_D, param, kbin)' fails.\n");
+            EXIT();
+        }
+        if (run_box(p, P_D, g, kbin.data(), pk, bk, true, 0)) {
+            printf("'calcBiSpectrumForBOX(P_D, param, kbin)' fails.\n");
+            EXIT();
+        }
+        // bispec.hpp:1187-1196, one file per ith_kbin (all rows with bk_all_rows = True)
+        const int r0 = p.bk_all_rows == "True" ? 0 : p.ith_kbin;
+        const int r1 = p.bk_all_rows == "True" ? nk : p.ith_kbin + 1;
+        for (int ith = r0; ith < r1; ith++) {
+            char buf[2048];
+            snprintf(buf, sizeof(buf), "%s/bk%d%d%d_%02d.dat", p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith);
+            FILE* fp = fopen(buf, "w");
+            if (!fp) EXIT();
+            fprintf(fp, "## k1 [h/Mpc] \t k2 [h/Mpc] \t bk_real \t bk_imag\n");
+            for (int i = 0; i < nk; i++) {
+                const size_t o = 2 * ((size_t)ith * nk + i);
+                fprintf(fp, "%.5f \t %.5f \t %.7e \t %.7e\n", kbin[ith], kbin[i], bk[o], bk[o + 1]);
+            }
+            fclose(fp);
+        }
+    } else {
+        printf("measure = %s is not available in this build.\n", p.measure.c_str());
+        EXIT();
+    }
+    stage("Done");
+    printf("End: %.3f sec.\n", now() - g_t0);
+    return 0;
+}

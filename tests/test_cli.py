@@ -1,0 +1,114 @@
+"""The drop-in program hitomi_box (main_box.cpp): parameter-file semantics and
+rejections on the CPU; end-to-end outputs against the oracle on the GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from hitomi_amd import io as hio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "hitomi_amd", "bin", "hitomi_box")
+
+
+def run(ini, cwd):
+    if not os.path.exists(EXE):
+        pytest.fail("hitomi_box is not built: run __graft_entry__.build()")
+    return subprocess.run([EXE, str(ini)], cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def test_no_argument_exits_1(tmp_path):
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 1 and "The input parameter file is not found." in r.stdout
+
+
+@pytest.mark.parametrize("key,value,message", [
+    ("measure", "xi", "The 'measure' is not loaded correctly."),
+    ("boxsize_y", "0", "The 'boxsize' is not loaded correctly."),
+    ("n_mesh_z", "-4", "The 'n_mesh' is not loaded correctly."),
+    ("ell2", "-1", "The 'ell1', 'ell2', or 'ELL' is not loaded correctly."),
+    ("kmin", "0", "The 'kmin' is not loaded correctly."),
+    ("kmax", "0.001", "The 'kmax' is not loaded correctly."),
+    ("n_kbin", "1", "The 'n_kbin' is not loaded correctly."),
+    ("ith_kbin", "20", "The 'ith_kbin' is not loaded correctly."),
+    ("assignment", "PCS", "The 'assign' is not loaded correctly."),
+    ("flag_recon", "Maybe", "The 'flag_recon' is not loaded correctly."),
+    ("sim_data", "Tipsy", "The 'sim_data' is not loaded correctly."),
+    ("log10_Mmax", "12.0", "The 'log10_Mmax' is not loaded correctly."),
+    ("data_mpi_file", None, "The 'data_mpi_file' is not loaded correctly."),
+])
+def test_rejected_parameters(tmp_path, key, value, message):
+    ini = tmp_path / "p.ini"
+    hio.write_ini(ini, output_dir=str(tmp_path / "out"), **{key: value})
+    r = run(ini, tmp_path)
+    assert r.returncode == 1
+    assert message in r.stdout and "'param.readParameters' fails." in r.stdout
+
+
+def test_substring_keys(tmp_path):
+    """parameter.hpp:161-207: keys match anywhere in a line.  A line whose
+    *comment* names another key also sets that key (here RG is set from a line
+    about NR), exactly as the reference parses it."""
+    ini = tmp_path / "p.ini"
+    hio.write_ini(ini, output_dir=str(tmp_path / "out"), RG=None, data_dir=str(tmp_path))
+    with open(ini, "a") as f:
+        f.write("NR = 0 %% also parsed as RG\n")
+    (tmp_path / "snap.0").write_bytes(b"")
+    r = run(ini, tmp_path)
+    # the catalogue is empty, so the run stops at the reader -- after the parameters were accepted
+    assert "'param.readParameters' fails." not in r.stdout
+    txt = (tmp_path / "out" / "input_parameters").read_text()
+    assert "RG = 0.00000\n" in txt and "NR = 0\n" in txt
+
+
+def test_input_parameters_file(tmp_path):
+    ini = tmp_path / "p.ini"
+    out = tmp_path / "out"
+    hio.write_ini(ini, output_dir=str(out), data_dir=str(tmp_path), realization=3, n_mesh_x=32, kmin=0.005)
+    run(ini, tmp_path)
+    txt = (out / "0003" / "input_parameters").read_text()
+    assert txt.startswith(f"\ndata_dir = {tmp_path}\ndata_file = {tmp_path}/snap\nrandom_file = {tmp_path}/AAA\n"
+                          f"output_dir = {out}/0003\n\nmeasure = pk\n\nrealization = 3\n")
+    assert "n_mesh_x = 32\nn_mesh_y = 64\n" in txt and "kmin = 0.00500\n" in txt
+    assert txt.endswith("NR = 0\ndata_mpi_file = mock\n\nassignment = TSC\n\n")
+
+
+def test_missing_snapshot_exits_1(tmp_path):
+    ini = tmp_path / "p.ini"
+    hio.write_ini(ini, output_dir=str(tmp_path / "out"), data_dir=str(tmp_path), data_file="nothere")
+    r = run(ini, tmp_path)
+    assert r.returncode == 1 and "can not open" in r.stdout
+
+
+@pytest.mark.gpu
+def test_end_to_end_pk_and_bk_match_oracle(tmp_path):
+    from oracle import pyoracle as po
+    from .conftest import clustered_catalogue
+
+    L, n, N = 1000.0, 32, 30000
+    pos = clustered_catalogue(N, L, seed=77, nclump=20, sigma=25.0).astype(np.float32)
+    hio.write_gadget1(str(tmp_path / "snap"), pos, boxsize=L, num_files=2)
+    posd = pos.astype(np.float64)
+    kw = dict(data_dir=str(tmp_path), boxsize_x=L, boxsize_y=L, boxsize_z=L, n_mesh_x=n, n_mesh_y=n, n_mesh_z=n,
+              kmin=0.02, kmax=0.08, n_kbin=4, precision="FP64")
+    for ell in (0, 2):
+        ini = tmp_path / f"pk{ell}.ini"
+        hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="pk", ell1=ell, ELL=ell, **kw)
+        r = run(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        got = hio.read_table(tmp_path / "out" / f"pk{ell}.dat")
+        p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.08, n_kbin=4, ell1=ell, ELL=ell)
+        ref, _ = po.box_pk(p, posd)
+        assert np.allclose(got[:, 0], po.kbin(p), atol=5e-6)
+        assert np.allclose(got[:, 1], ref.real, rtol=2e-7, atol=1e-7 * np.abs(ref.real).max())
+    ini = tmp_path / "bk.ini"
+    hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="bk", ith_kbin=2, **kw)
+    r = run(ini, tmp_path)
+    assert r.returncode == 0, r.stdout
+    got = hio.read_table(tmp_path / "out" / "bk000_02.dat")
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.08, n_kbin=4, ith_kbin=2)
+    ref, _, _ = po.box_bk(p, posd)
+    assert np.allclose(got[:, 2], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
+    assert np.allclose(got[:, 3], ref.imag, atol=1e-6 * np.abs(ref.real).max())
+    assert not (tmp_path / "out" / "bk000_01.dat").exists()
