@@ -51,14 +51,15 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8):
     w = 2 * s + 1
     W3 = 4 * s + 1 if 4 * s + 1 < M else M
     table = {
-        "assign_fixed": npart * 24 + npart * 27 * 8,  # positions + 27 cell read-modify-writes
+        "assign_gather": n ** 3 * 4 + npart * 24 + n ** 3 * 8,  # cell index in, sorted positions, int64 mesh out
+        "assign_scan": 3 * n ** 3 * 4,                            # reduce pass + read/write pass over the cell counts
+        "xi_rows": n ** 3 * (elem // 2),                          # every xi voxel once (the row sums are small)
         "fft_r2c_z": n ** 3 * 8 + half,               # int64 mesh in, half spectrum out
         "fft_r2c_y": 2 * half,
         "fft_r2c_x": 2 * half,
         "xi_pass_x": 2 * half,
         "xi_pass_y": 2 * half,
         "xi_pass_z": half + n ** 3 * (elem // 2),
-        "xi_hist": n ** 3 * (elem // 2),
         # the 21 transforms: 20 shells of support w, one delta/W of support W3 (averaged per launch)
         "tri_pass_x": ((nk * (w ** 3 + M * w * w) + (W3 ** 3 + M * W3 * W3)) * elem) / (nk + 1),
         "tri_pass_y": ((nk * (M * w * w + M * M * w) + (M * W3 * W3 + M * M * W3)) * elem) / (nk + 1),

@@ -22,8 +22,9 @@ ASSIGN = {"NGP": NGP, "CIC": CIC, "TSC": TSC}
 MAX_ELLS = 4
 
 # Launch names the box pipeline reports to hm_timing_get().
-BOX_KERNELS = ("fill_zero", "assign_fixed", "fft_r2c_z", "fft_r2c_y", "fft_r2c_x", "bin_pk", "bin_reduce",
-               "xi_pass_x", "xi_pass_y", "xi_pass_z", "xi_absmax", "xi_hist", "sn_pairs", "tri_pass_x",
+BOX_KERNELS = ("fill_zero", "assign_count", "assign_scan", "assign_scatter",
+               "assign_gather", "assign_border", "fft_r2c_z", "fft_r2c_y", "fft_r2c_x", "bin_pk", "bin_reduce",
+               "xi_pass_x", "xi_pass_y", "xi_pass_z", "xi_rows", "xi_hist", "sn_pairs", "tri_pass_x",
                "tri_pass_y", "tri_pass_z", "gram", "gram_reduce")
 
 
@@ -79,6 +80,7 @@ _SIGS = {
     "hm_timing_reset": (ct.c_int, [_VP]),
     "hm_timing_select": (ct.c_int, [_VP, ct.c_char_p]),
     "hm_assign": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
+    "hm_assign_atomic": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
     "hm_cell_coords": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_longlong, _VP]),
     "hm_fft_c2c": (ct.c_int, [_VP, ct.c_int, _VP, ct.POINTER(ct.c_int), ct.c_int]),
     "hm_fft_r2c": (ct.c_int, [_VP, ct.c_int, _VP, _VP, ct.POINTER(ct.c_int)]),
@@ -215,7 +217,7 @@ class Context:
         self.check(self.lib.hm_timing_reset(self.h), "hm_timing_reset")
 
     # a6: mass assignment
-    def assign(self, mesh: Mesh, pos, weights=None, wabs_sum=0.0):
+    def assign(self, mesh: Mesh, pos, weights=None, wabs_sum=0.0, atomic=False):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         npart = pos.shape[0]
         dpos = self.array(pos)
@@ -224,8 +226,9 @@ class Context:
         shape = (mesh.n_mesh[0], mesh.n_mesh[1], mesh.n_mesh[2])
         re = self.empty(shape, np.float64)
         im = self.empty(shape, np.float64) if weights is not None else None
+        fn = self.lib.hm_assign_atomic if atomic else self.lib.hm_assign
         self.check(
-            self.lib.hm_assign(self.h, ct.byref(mesh), dpos.ptr, dw.ptr if dw else None, npart, float(wabs_sum),
+            fn(self.h, ct.byref(mesh), dpos.ptr, dw.ptr if dw else None, npart, float(wabs_sum),
                                re.ptr, im.ptr if im else None), "hm_assign")
         out_re = re.download()
         out_im = im.download() if im is not None else np.zeros_like(out_re)

@@ -165,8 +165,8 @@ int hm_timing_select(hm_ctx* ctx, const char* name) {
     return 0;
 }
 
-int hm_assign(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
-              double wabs_sum, double* d_out_re, double* d_out_im) {
+static int assign_impl(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
+                       double wabs_sum, double* d_out_re, double* d_out_im, bool sorted) {
     if (!ctx) return -1;
     hm::AssignGeom g;
     if (!mesh_ok(ctx, mesh, g)) return -1;
@@ -178,15 +178,33 @@ int hm_assign(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const doubl
     if (hm::dev_alloc(ctx, q, bytes)) return -1;
     long long* qre = (long long*)q.p;
     long long* qim = d_out_im ? qre + g.ng : nullptr;
-    int rc = hm::fill_zero(ctx, q.p, bytes);
-    if (!rc) rc = hm::assign_fixed(ctx, g, d_pos, d_w, np, scale, qre, qim);
+    int rc = 0;
+    hm::SortWork w;
+    if (sorted) {
+        rc = hm::sort_work_alloc(ctx, w, g.ng, np, d_w != nullptr);
+        if (!rc) rc = hm::assign_sorted(ctx, g, d_pos, d_w, np, scale, qre, qim, w);
+    } else {
+        rc = hm::fill_zero(ctx, q.p, bytes);
+        if (!rc) rc = hm::assign_fixed(ctx, g, d_pos, d_w, np, scale, qre, qim);
+    }
     const double dV = (g.L[0] * g.L[1] * g.L[2]) / (double)g.ng;
     const double factor = 1.0 / dV;
     if (!rc) rc = hm::fixed_to_density(ctx, qre, g.ng, 1.0 / scale, factor, d_out_re);
     if (!rc && qim) rc = hm::fixed_to_density(ctx, qim, g.ng, 1.0 / scale, factor, d_out_im);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = bad(ctx, "hm_assign: stream error");
+    hm::sort_work_free(w);
     hm::dev_free(q);
     return rc;
+}
+
+int hm_assign(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
+              double wabs_sum, double* d_out_re, double* d_out_im) {
+    return assign_impl(ctx, mesh, d_pos, d_w, np, wabs_sum, d_out_re, d_out_im, true);
+}
+
+int hm_assign_atomic(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
+                     double wabs_sum, double* d_out_re, double* d_out_im) {
+    return assign_impl(ctx, mesh, d_pos, d_w, np, wabs_sum, d_out_re, d_out_im, false);
 }
 
 int hm_cell_coords(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, long long np, long long* d_coords) {

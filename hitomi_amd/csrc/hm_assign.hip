@@ -12,16 +12,14 @@ double fixed_scale_for(double wabs_sum) {
     return std::ldexp(1.0, 62 - e);
 }
 
-// One thread per particle: 1/8/27 stencil contributions, each rounded to the
-// nearest multiple of 2^-s and added with a 64-bit integer atomic.
+// The stencil contributions of one particle, each rounded to the nearest
+// multiple of 2^-s and added with a 64-bit integer atomic.
 template <int S>
-__global__ __launch_bounds__(256) void k_assign_fixed(const double* __restrict__ pos, const double* __restrict__ w,
-                                                     long long np, AssignGeom g, double scale,
-                                                     unsigned long long* __restrict__ gre,
-                                                     unsigned long long* __restrict__ gim) {
+__device__ __forceinline__ void scatter_particle(const double* __restrict__ pos, const double* __restrict__ w,
+                                                 long long q, const AssignGeom& g, double scale,
+                                                 unsigned long long* __restrict__ gre,
+                                                 unsigned long long* __restrict__ gim) {
 #pragma clang fp contract(off)
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
     double wt[3][3];
     int iw0[3];
 #pragma unroll
@@ -53,6 +51,16 @@ __global__ __launch_bounds__(256) void k_assign_fixed(const double* __restrict__
     }
 }
 
+template <int S>
+__global__ __launch_bounds__(256) void k_assign_fixed(const double* __restrict__ pos, const double* __restrict__ w,
+                                                     long long np, AssignGeom g, double scale,
+                                                     unsigned long long* __restrict__ gre,
+                                                     unsigned long long* __restrict__ gim) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    scatter_particle<S>(pos, w, q, g, scale, gre, gim);
+}
+
 int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
                  double scale, long long* d_re, long long* d_im) {
     if (np <= 0) return 0;
@@ -67,6 +75,299 @@ int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const do
         else
             hipLaunchKernelGGL(k_assign_fixed<3>, dim3(blocks), dim3(256), 0, ctx->stream, d_pos, d_w, np, g, scale, re, im);
     });
+}
+
+// ---------------------------------------------------------------------------
+// sort-by-cell assignment
+// ---------------------------------------------------------------------------
+template <int S>
+__device__ __forceinline__ bool first_cell(const double* __restrict__ pos, long long q, const AssignGeom& g,
+                                           long long* key) {
+#pragma clang fp contract(off)
+    double wt[3];
+    int iw0[3];
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        stencil_axis(g.assign, axis_xp(pos[3 * q + a], g.n[a], g.L[a]), &iw0[a], wt);
+        inside = inside && iw0[a] >= 0 && iw0[a] + S - 1 <= g.n[a] - 1;
+    }
+    *key = ((long long)iw0[0] * g.n[1] + iw0[1]) * g.n[2] + iw0[2];
+    return inside;
+}
+
+// Count particles per first stencil cell; list the border particles
+// (one counter atomic per wave for the border list).
+template <int S>
+__global__ __launch_bounds__(256) void k_sort_count(const double* __restrict__ pos, long long np, AssignGeom g,
+                                                   int* __restrict__ count, int* __restrict__ slot,
+                                                   int* __restrict__ border) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool is_border = false;
+    if (q < np) {
+        long long key;
+        if (first_cell<S>(pos, q, g, &key)) {
+            slot[q] = atomicAdd(count + key, 1);
+        } else {
+            slot[q] = -1;
+            is_border = true;
+        }
+    }
+    const unsigned long long mask = __ballot(is_border);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(border, (int)__popcll(mask));
+        base = __shfl(base, leader);
+        if (is_border) {
+            const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+            border[1 + base + (int)__popcll(below)] = (int)q;
+        }
+    }
+}
+
+constexpr int SCAN_NT = 256, SCAN_ITEMS = 16, SCAN_BLK = SCAN_NT * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const int* __restrict__ a, long long n,
+                                                        int* __restrict__ bsum) {
+    __shared__ int sm[SCAN_NT];
+    const long long base = (long long)blockIdx.x * SCAN_BLK;
+    int s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const long long k = base + i * SCAN_NT + threadIdx.x;
+        if (k < n) s += a[k];
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = SCAN_NT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sm[0];
+}
+
+// exclusive scan of the block sums in one workgroup
+__global__ __launch_bounds__(1024) void k_scan_bsum(int* __restrict__ b, int nb) {
+    __shared__ int sm[1024];
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += 1024) {
+        const int i = c0 + threadIdx.x;
+        const int v = (i < nb) ? b[i] : 0;
+        sm[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int t = (threadIdx.x >= off) ? sm[threadIdx.x - off] : 0;
+            __syncthreads();
+            sm[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) b[i] = carry + sm[threadIdx.x] - v;
+        carry += sm[1023];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(int* __restrict__ a, long long n,
+                                                       const int* __restrict__ bsum) {
+    __shared__ int tile[SCAN_BLK];
+    __shared__ int tsum[SCAN_NT];
+    const long long base = (long long)blockIdx.x * SCAN_BLK;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int e = i * SCAN_NT + threadIdx.x;
+        tile[e] = (base + e < n) ? a[base + e] : 0;
+    }
+    __syncthreads();
+    int run = 0;
+    int loc[SCAN_ITEMS];
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        loc[i] = run;
+        run += tile[threadIdx.x * SCAN_ITEMS + i];
+    }
+    tsum[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < SCAN_NT; off <<= 1) {
+        const int t = (threadIdx.x >= off) ? tsum[threadIdx.x - off] : 0;
+        __syncthreads();
+        tsum[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const int off = bsum[blockIdx.x] + tsum[threadIdx.x] - run;
+    for (int i = 0; i < SCAN_ITEMS; i++) tile[threadIdx.x * SCAN_ITEMS + i] = off + loc[i];
+    __syncthreads();
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int e = i * SCAN_NT + threadIdx.x;
+        if (base + e < n) a[base + e] = tile[e];
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_sort_scatter(const double* __restrict__ pos, const double* __restrict__ w,
+                                                     long long np, AssignGeom g, const int* __restrict__ start,
+                                                     const int* __restrict__ slot, double* __restrict__ spos,
+                                                     double* __restrict__ sw) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const int r = slot[q];
+    if (r < 0) return;
+    long long key;
+    first_cell<S>(pos, q, g, &key);
+    const long long d = (long long)start[key] + r;
+    spos[3 * d] = pos[3 * q];
+    spos[3 * d + 1] = pos[3 * q + 1];
+    spos[3 * d + 2] = pos[3 * q + 2];
+    if (w) {
+        sw[2 * d] = w[2 * q];
+        sw[2 * d + 1] = w[2 * q + 1];
+    }
+}
+
+// Output tile per workgroup: GX x GY x GZ cells, one cell = one thread-slot.
+constexpr int GX = 4, GY = 8, GZ = 64;
+
+// Every output cell sums the contributions of the particles whose first
+// stencil cell is one of its S^3 lower neighbours, in fixed point.
+template <int S>
+__global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ start, const double* __restrict__ spos,
+                                                    const double* __restrict__ sw, AssignGeom g, double scale,
+                                                    long long* __restrict__ out_re, long long* __restrict__ out_im) {
+#pragma clang fp contract(off)
+    constexpr int LX = GX + S - 1, LY = GY + S - 1, LZ = GZ + S;
+    __shared__ int st[LX * LY * LZ];
+    const int x0 = blockIdx.z * GX, y0 = blockIdx.y * GY, z0 = blockIdx.x * GZ;
+    for (int e = threadIdx.x; e < LX * LY * LZ; e += 256) {
+        const int dz = e % LZ, dy = (e / LZ) % LY, dx = e / (LZ * LY);
+        const int xx = x0 - (S - 1) + dx, yy = y0 - (S - 1) + dy, zz = z0 - (S - 1) + dz;
+        int v = 0;
+        if (xx >= 0 && xx < g.n[0] && yy >= 0 && yy < g.n[1]) {
+            const long long lin = ((long long)xx * g.n[1] + yy) * g.n[2] + zz;
+            if (lin >= 0 && lin <= g.ng) v = start[lin];
+        }
+        st[e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < GX * GY * GZ; e += 256) {
+        const int x = x0 + e / (GY * GZ), y = y0 + (e / GZ) % GY, z = z0 + e % GZ;
+        if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
+        long long are = 0, aim = 0;
+        for (int dx = 0; dx < S; dx++) {
+            const int sx = x - dx;
+            if (sx < 0 || sx > g.n[0] - S) continue;
+            for (int dy = 0; dy < S; dy++) {
+                const int sy = y - dy;
+                if (sy < 0 || sy > g.n[1] - S) continue;
+                for (int dz = 0; dz < S; dz++) {
+                    const int sz = z - dz;
+                    if (sz < 0 || sz > g.n[2] - S) continue;
+                    const int t = ((sx - x0 + S - 1) * LY + (sy - y0 + S - 1)) * LZ + (sz - z0 + S - 1);
+                    const int b = st[t], en = st[t + 1];
+                    for (int p = b; p < en; p++) {
+                        double wt[3][3];
+                        int iw0[3];
+                        for (int a = 0; a < 3; a++)
+                            stencil_axis(g.assign, axis_xp(spos[3 * (long long)p + a], g.n[a], g.L[a]), &iw0[a], wt[a]);
+                        const double wij = wt[0][dx] * wt[1][dy];
+                        const double wijk = wij * wt[2][dz];
+                        const double wre = sw ? sw[2 * (long long)p] : 1.0;
+                        are += __double2ll_rn(wre * wijk * scale);
+                        if (out_im) {
+                            const double wim = sw ? sw[2 * (long long)p + 1] : 0.0;
+                            aim += __double2ll_rn(wim * wijk * scale);
+                        }
+                    }
+                }
+            }
+        }
+        const long long lin = ((long long)x * g.n[1] + y) * g.n[2] + z;
+        out_re[lin] = are;
+        if (out_im) out_im[lin] = aim;
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_assign_border(const double* __restrict__ pos, const double* __restrict__ w,
+                                                      const int* __restrict__ border, AssignGeom g, double scale,
+                                                      unsigned long long* __restrict__ gre,
+                                                      unsigned long long* __restrict__ gim) {
+    const int nb = border[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x)
+        scatter_particle<S>(pos, w, border[1 + i], g, scale, gre, gim);
+}
+
+int sort_work_alloc(hm_ctx* ctx, SortWork& w, long long ng, long long np, bool weights) {
+    const long long nb = (ng + 1 + SCAN_BLK - 1) / SCAN_BLK;
+    if (dev_alloc(ctx, w.start, sizeof(int) * (size_t)(ng + 1))) return -1;
+    if (dev_alloc(ctx, w.slot, sizeof(int) * (size_t)(np > 0 ? np : 1))) return -1;
+    if (dev_alloc(ctx, w.spos, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1))) return -1;
+    if (weights && dev_alloc(ctx, w.sw, sizeof(double) * 2 * (size_t)(np > 0 ? np : 1))) return -1;
+    if (dev_alloc(ctx, w.border, sizeof(int) * (size_t)(np + 1))) return -1;
+    if (dev_alloc(ctx, w.bsum, sizeof(int) * (size_t)nb)) return -1;
+    return 0;
+}
+
+void sort_work_free(SortWork& w) {
+    dev_free(w.start);
+    dev_free(w.slot);
+    dev_free(w.spos);
+    dev_free(w.sw);
+    dev_free(w.border);
+    dev_free(w.bsum);
+}
+
+template <int S>
+static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
+                           double scale, long long* d_re, long long* d_im, SortWork& w) {
+    int* start = (int*)w.start.p;
+    int* slot = (int*)w.slot.p;
+    int* border = (int*)w.border.p;
+    int* bsum = (int*)w.bsum.p;
+    double* spos = (double*)w.spos.p;
+    double* sw = d_w ? (double*)w.sw.p : nullptr;
+    auto* re = reinterpret_cast<unsigned long long*>(d_re);
+    auto* im = reinterpret_cast<unsigned long long*>(d_im);
+    const long long n = g.ng + 1;
+    const unsigned nb = (unsigned)((n + SCAN_BLK - 1) / SCAN_BLK);
+    const unsigned pb = (unsigned)((np + 255) / 256);
+    if (fill_zero(ctx, start, sizeof(int) * (size_t)n)) return -1;
+    if (fill_zero(ctx, border, sizeof(int))) return -1;
+    if (np > 0 && ctx->launch("assign_count", [&] {
+            hipLaunchKernelGGL(k_sort_count<S>, dim3(pb), dim3(256), 0, ctx->stream, d_pos, np, g, start, slot, border);
+        }))
+        return -1;
+    if (ctx->launch("assign_scan", [&] {
+            hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, start, n, bsum);
+            hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)nb);
+            hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, start, n, bsum);
+        }))
+        return -1;
+    if (np > 0 && ctx->launch("assign_scatter", [&] {
+            hipLaunchKernelGGL(k_sort_scatter<S>, dim3(pb), dim3(256), 0, ctx->stream, d_pos, d_w, np, g, start, slot,
+                               spos, sw);
+        }))
+        return -1;
+    const dim3 gg((unsigned)((g.n[2] + GZ - 1) / GZ), (unsigned)((g.n[1] + GY - 1) / GY),
+                  (unsigned)((g.n[0] + GX - 1) / GX));
+    if (ctx->launch("assign_gather", [&] {
+            hipLaunchKernelGGL(k_sort_gather<S>, gg, dim3(256), 0, ctx->stream, start, spos, sw, g, scale, d_re, d_im);
+        }))
+        return -1;
+    if (np > 0 && ctx->launch("assign_border", [&] {
+            hipLaunchKernelGGL(k_assign_border<S>, dim3(256), dim3(256), 0, ctx->stream, d_pos, d_w, border, g, scale, re,
+                               im);
+        }))
+        return -1;
+    return 0;
+}
+
+int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
+                  double scale, long long* d_re, long long* d_im, SortWork& w) {
+    if (np > 0x7ffffff0LL || g.ng > 0x7ffffff0LL) {
+        ctx->set_error("assign: more than 2^31 particles or cells");
+        return -1;
+    }
+    if (g.assign == 0) return assign_sorted_S<1>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w);
+    if (g.assign == 1) return assign_sorted_S<2>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w);
+    return assign_sorted_S<3>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w);
 }
 
 template <int S>

@@ -69,8 +69,30 @@ template <typename T> struct FixedRealSrc {
     }
 };
 
-// Accumulate particles into a zeroed fixed-point grid (re and optional im).
+// Accumulate particles into a zeroed fixed-point grid (re and optional im)
+// with one 64-bit atomic per stencil cell.
 int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
                  double scale, long long* d_re, long long* d_im);
+
+// Sort-by-cell assignment: particles whose whole stencil lies inside the
+// mesh are counting-sorted by their first stencil cell and every output cell
+// gathers its <= 27 neighbour cells in one thread (no atomics, one plain
+// store per cell); the few particles whose stencil leaves the mesh (the Q1
+// spill/drop cases) are added afterwards with atomics.  Each contribution is
+// rounded to the same fixed-point value as in assign_fixed, so the grid is
+// bitwise identical to it.
+struct SortWork {
+    DevBuf start;   // int32 [ng + 1]: particle count, then exclusive prefix, per first-stencil cell
+    DevBuf slot;    // int32 [np]: rank within the cell, or -1 for a border particle
+    DevBuf spos;    // double [3 np] positions in cell order
+    DevBuf sw;      // double [2 np] weights in cell order
+    DevBuf border;  // int32 [np + 1]: border count, then border particle indices
+    DevBuf bsum;    // int32 scan block sums
+    size_t bytes() const { return start.bytes + slot.bytes + spos.bytes + sw.bytes + border.bytes + bsum.bytes; }
+};
+int sort_work_alloc(hm_ctx* ctx, SortWork& w, long long ng, long long np, bool weights);
+void sort_work_free(SortWork& w);
+int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
+                  double scale, long long* d_re, long long* d_im, SortWork& w);
 
 }  // namespace hm

@@ -274,50 +274,44 @@ template <typename T> struct XiSrc {
     }
 };
 
-// |x| max over a real mesh, as the bit pattern of a non-negative double
+// Shot-noise radial sums, step 1: R[a][k] = sum of xi over the rows (i, j)
+// whose minimum-image i'^2 + j'^2 equals a (rows listed per a in a fixed
+// order), one workgroup per distinct a.  FP64, no atomics.
 template <typename T>
-__global__ __launch_bounds__(256) void k_absmax(const T* __restrict__ x, long long n, unsigned long long* __restrict__ out) {
-    __shared__ double sm[256];
-    double m = 0.0;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        m = fmax(m, fabs((double)x[i]));
-    sm[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
-        __syncthreads();
+__global__ __launch_bounds__(256) void k_xi_rows(const T* __restrict__ xi, int n2, const int* __restrict__ a_start,
+                                                const int* __restrict__ rows, double* __restrict__ R) {
+    const int ai = blockIdx.x;
+    const int r0 = a_start[ai], r1 = a_start[ai + 1];
+    for (int k = threadIdx.x; k < n2; k += 256) {
+        double acc = 0.0;
+        for (int r = r0; r < r1; r++) acc += (double)xi[(long long)rows[r] * n2 + k];
+        R[(long long)ai * n2 + k] = acc;
     }
-    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
 }
 
-// 2^(62 - e) with absmax * maxcount < 2^e
-__device__ __forceinline__ double hist_scale(unsigned long long bits, double maxcount) {
-    const double amax = __longlong_as_double((long long)bits);
-    int e = 0;
-    frexp(amax * maxcount * (1.0 + 1e-9) + 1e-300, &e);
-    if (e < -900) e = -900;  // an all-zero field: any finite scale
-    return ldexp(1.0, 62 - e);
-}
-
-// H[n2] += xi(x) for the minimum-image |n|^2 of every voxel (rvec rule of
-// bispec.hpp:1040-1043), in 64-bit fixed point.
-template <typename T>
-__global__ __launch_bounds__(256) void k_hist_n2(const T* __restrict__ xi, int n0, int n1, int n2,
-                                                const unsigned long long* __restrict__ amax, double maxcount,
-                                                unsigned long long* __restrict__ H) {
-    const double sc = hist_scale(*amax, maxcount);
-    const long long ng = (long long)n0 * n1 * n2;
-    for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < ng; v += (long long)gridDim.x * 256) {
-        const int k = (int)(v % n2);
-        const long long r = v / n2;
-        const int j = (int)(r % n1), i = (int)(r / n1);
-        const long long a = (i < n0 / 2) ? i : i - n0;
-        const long long b = (j < n1 / 2) ? j : j - n1;
-        const long long c = (k < n2 / 2) ? k : k - n2;
-        const long long key = a * a + b * b + c * c;
-        const long long q = __double2ll_rn((double)xi[v] * sc);
-        if (q) atomicAdd(H + key, (unsigned long long)q);
+// Step 2: H[n] = sum over k with a = n - k'^2 a row key of R[a][k]: the sum of
+// xi over all voxels with minimum-image |n|^2 = n (rvec rule of
+// bispec.hpp:1040-1043), in a fixed order.
+__global__ __launch_bounds__(256) void k_xi_hist(const double* __restrict__ R, int n2, const int* __restrict__ a_idx,
+                                                long long amax, long long nkey, double* __restrict__ H) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nkey) return;
+    long long m = (long long)sqrt((double)n);
+    while (m * m > n) m--;
+    while ((m + 1) * (m + 1) <= n) m++;
+    if (m > n2 / 2) m = n2 / 2;
+    double acc = 0.0;
+    for (long long q = 0; q <= m; q++) {
+        const long long a = n - q * q;
+        if (a > amax) continue;
+        const int ai = a_idx[a];
+        if (ai < 0) continue;
+        const double* row = R + (long long)ai * n2;
+        if (q == 0) acc += row[0];
+        else if (q == n2 / 2) acc += row[n2 / 2];
+        else acc += row[q] + row[n2 - q];
     }
+    H[n] = acc;
 }
 
 // J[a][n2] = j0(k_a sqrt(n2) dr) (bessel.hpp spline of j0 replaced by the
@@ -333,19 +327,14 @@ __global__ void k_j0_table(const double* __restrict__ kb, int nk, long long nkey
 
 // S[a][b] = dV * sum_n2 J[a][n2] J[b][n2] H[n2], one block per pair (a <= b)
 __global__ __launch_bounds__(256) void k_sn_pairs(const double* __restrict__ J, long long nkey,
-                                                 const unsigned long long* __restrict__ H,
-                                                 const unsigned long long* __restrict__ amax, double maxcount,
-                                                 const short2* __restrict__ pairs, double dV, double* __restrict__ out) {
+                                                 const double* __restrict__ H, const short2* __restrict__ pairs,
+                                                 double dV, double* __restrict__ out) {
     __shared__ double sm[256];
     const short2 pr = pairs[blockIdx.x];
-    const double inv = 1.0 / hist_scale(*amax, maxcount);
     const double* Ja = J + (long long)pr.x * nkey;
     const double* Jb = J + (long long)pr.y * nkey;
     double acc = 0.0;
-    for (long long key = threadIdx.x; key < nkey; key += 256) {
-        const long long h = (long long)H[key];
-        if (h) acc += Ja[key] * Jb[key] * ((double)h * inv);
-    }
+    for (long long key = threadIdx.x; key < nkey; key += 256) acc += Ja[key] * Jb[key] * H[key];
     sm[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -423,13 +412,15 @@ struct BoxPlanImpl {
     int nzmax = 0, s01 = -1, nrow_y = 0;
     long long nrows = 0;
     long long nkey = 0;   // |n|^2 keys
-    double maxcount = 0;  // most voxels sharing one |n|^2
+    long long amax = 0;   // largest i'^2 + j'^2
+    int na = 0;           // distinct i'^2 + j'^2 over the rows of the mesh
+    SortWork sort;
     std::vector<short2> pairs;
     std::vector<long long> shell_nmode;
     bool counted = false;
     size_t bytes = 0;
-    DevBuf tab, f2b_d, lu_d, kb_d, fixed, dk, pk_part, pk_cnt, pk_red, pk_cnt_red, xi_half, xi_real, hist, amax, jtab,
-        pairs_d, sn_pairs, fields, gram_part, gram_red, shell_cnt;
+    DevBuf tab, f2b_d, lu_d, kb_d, fixed, dk, pk_part, pk_cnt, pk_red, pk_cnt_red, xi_half, xi_real, hist, jtab,
+        pairs_d, a_start, a_rows, a_idx, xi_R, sn_pairs, fields, gram_part, gram_red, shell_cnt;
     long long gram_blocks = 0;
     int gram_vpb = 4096;
 };
@@ -452,8 +443,8 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
     if (p->cfg.bk) {
         if (alloc(ctx, p, p->xi_half, sizeof(C<T>) * (size_t)m.nh())) return -1;
         if (alloc(ctx, p, p->xi_real, sizeof(T) * (size_t)m.ng())) return -1;
-        if (alloc(ctx, p, p->hist, sizeof(unsigned long long) * (size_t)p->nkey)) return -1;
-        if (alloc(ctx, p, p->amax, sizeof(unsigned long long))) return -1;
+        if (alloc(ctx, p, p->hist, sizeof(double) * (size_t)p->nkey)) return -1;
+        if (alloc(ctx, p, p->xi_R, sizeof(double) * (size_t)p->na * (size_t)m.n[2])) return -1;
         if (alloc(ctx, p, p->jtab, sizeof(double) * (size_t)p->nk * (size_t)p->nkey)) return -1;
         if (alloc(ctx, p, p->pairs_d, sizeof(short2) * p->pairs.size())) return -1;
         if (alloc(ctx, p, p->sn_pairs, sizeof(double) * p->pairs.size())) return -1;
@@ -464,6 +455,7 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
         if (alloc(ctx, p, p->gram_red, sizeof(double2) * p->pairs.size())) return -1;
     }
     if (alloc(ctx, p, p->shell_cnt, sizeof(unsigned long long) * (size_t)p->nk)) return -1;
+    if (sort_work_alloc(ctx, p->sort, m.ng(), 1, false)) return -1;
     return 0;
 }
 
@@ -571,26 +563,35 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
         if (!(dr0 == dr1 && dr1 == dr2)) return fail("box plan: the bispectrum needs equal cell sizes on all axes");
         const long long h0 = m.n[0] / 2, h1 = m.n[1] / 2, h2 = m.n[2] / 2;
         p->nkey = h0 * h0 + h1 * h1 + h2 * h2 + 1;
-        // largest number of voxels sharing one |n|^2
-        std::vector<unsigned int> cxy((size_t)(h0 * h0 + h1 * h1 + 1), 0);
+        // rows (i, j) grouped by the minimum-image i'^2 + j'^2
+        p->amax = h0 * h0 + h1 * h1;
+        std::vector<int> cnt((size_t)p->amax + 1, 0), key((size_t)m.n[0] * m.n[1]);
         for (int i = 0; i < m.n[0]; i++) {
             long long a = (i < m.n[0] / 2) ? i : i - m.n[0];
             for (int j = 0; j < m.n[1]; j++) {
                 long long b = (j < m.n[1] / 2) ? j : j - m.n[1];
-                cxy[(size_t)(a * a + b * b)]++;
+                key[(size_t)i * m.n[1] + j] = (int)(a * a + b * b);
+                cnt[(size_t)(a * a + b * b)]++;
             }
         }
-        std::vector<unsigned long long> c3((size_t)p->nkey, 0);
-        for (size_t u = 0; u < cxy.size(); u++) {
-            if (!cxy[u]) continue;
-            for (int k = 0; k < m.n[2]; k++) {
-                long long c = (k < m.n[2] / 2) ? k : k - m.n[2];
-                c3[u + (size_t)(c * c)] += cxy[u];
+        std::vector<int> a_idx((size_t)p->amax + 1, -1), a_start(1, 0);
+        for (long long a = 0; a <= p->amax; a++)
+            if (cnt[(size_t)a]) {
+                a_idx[(size_t)a] = (int)(a_start.size() - 1);
+                a_start.push_back(a_start.back() + cnt[(size_t)a]);
             }
+        p->na = (int)(a_start.size() - 1);
+        std::vector<int> fillp(a_start.begin(), a_start.end() - 1), rows(key.size());
+        for (size_t r = 0; r < key.size(); r++) rows[(size_t)fillp[(size_t)a_idx[(size_t)key[r]]]++] = (int)r;
+        if (alloc(ctx, p, p->a_start, sizeof(int) * a_start.size()) ||
+            alloc(ctx, p, p->a_rows, sizeof(int) * rows.size()) || alloc(ctx, p, p->a_idx, sizeof(int) * a_idx.size())) {
+            delete p;
+            return nullptr;
         }
-        unsigned long long mx = 0;
-        for (auto v : c3) mx = std::max(mx, v);
-        p->maxcount = (double)mx;
+        if (hipMemcpy(p->a_start.p, a_start.data(), sizeof(int) * a_start.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->a_rows.p, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->a_idx.p, a_idx.data(), sizeof(int) * a_idx.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail("box plan: row table upload failed");
     }
     // device tables
     std::vector<double> tabs;
@@ -652,9 +653,10 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
 void plan_destroy(BoxPlanImpl* p) {
     if (!p) return;
     DevBuf* bufs[] = {&p->tab, &p->f2b_d, &p->lu_d, &p->kb_d, &p->fixed, &p->dk, &p->pk_part, &p->pk_cnt, &p->pk_red,
-                      &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->amax, &p->jtab, &p->pairs_d,
+                      &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
                       &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt};
     for (DevBuf* b : bufs) dev_free(*b);
+    sort_work_free(p->sort);
     delete p;
 }
 
@@ -708,8 +710,12 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     ag.ng = m.ng();
     ag.assign = m.assign;
     const double scale = fixed_scale_for((double)np);
-    if (fill_zero(ctx, p->fixed.p, sizeof(long long) * (size_t)m.ng())) return -1;
-    if (assign_fixed(ctx, ag, d_pos, nullptr, np, scale, (long long*)p->fixed.p, nullptr)) return -1;
+    {
+        const size_t before = p->sort.bytes();
+        if (sort_work_alloc(ctx, p->sort, m.ng(), np, false)) return -1;
+        p->bytes += p->sort.bytes() - before;
+    }
+    if (assign_sorted(ctx, ag, d_pos, nullptr, np, scale, (long long*)p->fixed.p, nullptr, p->sort)) return -1;
     // 2. forward r2c with the fluctuation (rho - N/V) dV formed on load
     const double dV = m.volume() / (double)m.ng();
     FixedRealSrc<T> fsrc{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale, 1.0 / dV, (double)np / m.volume(), dV};
@@ -771,25 +777,21 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             return -1;
         T* xr = (T*)p->xi_real.p;
         if (fft_c2r_rows<T>(ctx, "xi_pass_z", m.n[2] / 2, xh, xr, (long long)m.n[0] * m.n[1], -1)) return -1;
-        if (fill_zero(ctx, p->amax.p, sizeof(unsigned long long))) return -1;
-        if (fill_zero(ctx, p->hist.p, sizeof(unsigned long long) * (size_t)p->nkey)) return -1;
-        const long long ng = m.ng();
-        const unsigned hb = (unsigned)std::min<long long>((ng + 255) / 256, 16384);
-        if (ctx->launch("xi_absmax", [&] {
-                hipLaunchKernelGGL(k_absmax<T>, dim3(hb), dim3(256), 0, ctx->stream, xr, ng,
-                                   (unsigned long long*)p->amax.p);
+        if (ctx->launch("xi_rows", [&] {
+                hipLaunchKernelGGL(k_xi_rows<T>, dim3((unsigned)p->na), dim3(256), 0, ctx->stream, xr, m.n[2],
+                                   (const int*)p->a_start.p, (const int*)p->a_rows.p, (double*)p->xi_R.p);
             }))
             return -1;
         if (ctx->launch("xi_hist", [&] {
-                hipLaunchKernelGGL(k_hist_n2<T>, dim3(hb), dim3(256), 0, ctx->stream, xr, m.n[0], m.n[1], m.n[2],
-                                   (const unsigned long long*)p->amax.p, p->maxcount, (unsigned long long*)p->hist.p);
+                hipLaunchKernelGGL(k_xi_hist, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   (const double*)p->xi_R.p, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey,
+                                   (double*)p->hist.p);
             }))
             return -1;
         if (ctx->launch("sn_pairs", [&] {
                 hipLaunchKernelGGL(k_sn_pairs, dim3((unsigned)npairs), dim3(256), 0, ctx->stream,
-                                   (const double*)p->jtab.p, p->nkey, (const unsigned long long*)p->hist.p,
-                                   (const unsigned long long*)p->amax.p, p->maxcount, (const short2*)p->pairs_d.p, dV,
-                                   (double*)p->sn_pairs.p);
+                                   (const double*)p->jtab.p, p->nkey, (const double*)p->hist.p,
+                                   (const short2*)p->pairs_d.p, dV, (double*)p->sn_pairs.p);
             }))
             return -1;
         // 5. shell fields and delta/W on the triple mesh (field.hpp:346-401, 610-639, 328-344)

@@ -77,6 +77,11 @@ int hm_timing_select(hm_ctx *ctx, const char *name);
  * 64-bit fixed point, so the result is bitwise independent of particle order. */
 int hm_assign(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const double *d_w, long long np,
               double wabs_sum, double *d_out_re, double *d_out_im);
+/* The same grid from the scatter form of field.hpp:172-180 (one 64-bit
+ * fixed-point atomic per stencil cell); bitwise equal to hm_assign, which
+ * sorts particles by cell and gathers.  Kept as the cross-check. */
+int hm_assign_atomic(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const double *d_w, long long np,
+                     double wabs_sum, double *d_out_re, double *d_out_im);
 
 /* The 1/8/27 stencil cell indices of each particle as field.hpp:84-93,
  * 121-133, 162-176 compute them; -1 where the reference's range check

@@ -67,6 +67,22 @@ def test_assign_order_independent_bitwise(ctx):
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("name", ["NGP", "CIC", "TSC"])
+def test_sorted_gather_equals_atomic_scatter_bitwise(ctx, name):
+    """The sort-by-cell gather and the atomic scatter round every stencil
+    contribution to the same fixed-point value: identical grids, including the
+    spilled / dropped border contributions (edge catalogue)."""
+    for n, L in [(16, 100.0), (64, 1000.0)]:
+        pos = np.concatenate([edge_catalogue(n, L), clustered_catalogue(50000, L, seed=n)])
+        a, _ = ctx.assign(hm.make_mesh(n, L, name), pos)
+        b, _ = ctx.assign(hm.make_mesh(n, L, name), pos, atomic=True)
+        assert np.array_equal(a, b), (name, n)
+    w = np.random.default_rng(3).standard_normal((pos.shape[0], 2))
+    a = ctx.assign(hm.make_mesh(n, L, name), pos, w, wabs_sum=np.abs(w).sum())
+    b = ctx.assign(hm.make_mesh(n, L, name), pos, w, wabs_sum=np.abs(w).sum(), atomic=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
 def test_assign_empty_catalogue(ctx):
     re, im = ctx.assign(hm.make_mesh(16, 100.0, "TSC"), np.zeros((0, 3)))
     assert np.all(re == 0)
