@@ -51,9 +51,8 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8):
     w = 2 * s + 1
     W3 = 4 * s + 1 if 4 * s + 1 < M else M
     table = {
-        "assign_gather": n ** 3 * 4 + npart * 24 + n ** 3 * 8,  # cell index in, sorted positions, int64 mesh out
-        "assign_scan": 3 * n ** 3 * 4,                            # reduce pass + read/write pass over the cell counts
-        "xi_rows": n ** 3 * (elem // 2),                          # every xi voxel once (the row sums are small)
+        "assign_gather": npart * 24 + n ** 3 * 8,                # sorted positions in, int64 mesh out
+        "xi_group": half,                                         # the y-transformed half spectrum once
         "fft_r2c_z": n ** 3 * 8 + half,               # int64 mesh in, half spectrum out
         "fft_r2c_y": 2 * half,
         "fft_r2c_x": 2 * half,

@@ -24,7 +24,7 @@ MAX_ELLS = 4
 # Launch names the box pipeline reports to hm_timing_get().
 BOX_KERNELS = ("fill_zero", "assign_count", "assign_scan", "assign_scatter",
                "assign_gather", "assign_border", "fft_r2c_z", "fft_r2c_y", "fft_r2c_x", "bin_pk", "bin_reduce",
-               "xi_pass_x", "xi_pass_y", "xi_pass_z", "xi_rows", "xi_hist", "sn_pairs", "tri_pass_x",
+               "xi_pass_x", "xi_pass_y", "xi_pass_z", "xi_group", "xi_hist", "sn_pairs", "tri_pass_x",
                "tri_pass_y", "tri_pass_z", "gram", "gram_reduce")
 
 

@@ -181,7 +181,7 @@ static int assign_impl(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, co
     int rc = 0;
     hm::SortWork w;
     if (sorted) {
-        rc = hm::sort_work_alloc(ctx, w, g.ng, np, d_w != nullptr);
+        rc = hm::sort_work_alloc(ctx, w, g, np, d_w != nullptr);
         if (!rc) rc = hm::assign_sorted(ctx, g, d_pos, d_w, np, scale, qre, qim, w);
     } else {
         rc = hm::fill_zero(ctx, q.p, bytes);

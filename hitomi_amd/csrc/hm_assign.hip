@@ -80,6 +80,13 @@ int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const do
 // ---------------------------------------------------------------------------
 // sort-by-cell assignment
 // ---------------------------------------------------------------------------
+// Output tile per workgroup: GX x GY x GZ cells.  Particles are sorted by
+// the (x, y, z / GZ) chunk of their first stencil cell.
+constexpr int GX = 4, GY = 8, GZ = 64;
+
+__host__ __device__ inline int chunks_z(const AssignGeom& g) { return (g.n[2] + GZ - 1) / GZ; }
+__host__ __device__ inline long long n_keys(const AssignGeom& g) { return (long long)g.n[0] * g.n[1] * chunks_z(g); }
+
 template <int S>
 __device__ __forceinline__ bool first_cell(const double* __restrict__ pos, long long q, const AssignGeom& g,
                                            long long* key) {
@@ -92,7 +99,7 @@ __device__ __forceinline__ bool first_cell(const double* __restrict__ pos, long 
         stencil_axis(g.assign, axis_xp(pos[3 * q + a], g.n[a], g.L[a]), &iw0[a], wt);
         inside = inside && iw0[a] >= 0 && iw0[a] + S - 1 <= g.n[a] - 1;
     }
-    *key = ((long long)iw0[0] * g.n[1] + iw0[1]) * g.n[2] + iw0[2];
+    *key = ((long long)iw0[0] * g.n[1] + iw0[1]) * chunks_z(g) + iw0[2] / GZ;
     return inside;
 }
 
@@ -222,65 +229,79 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const double* __restrict__
     }
 }
 
-// Output tile per workgroup: GX x GY x GZ cells, one cell = one thread-slot.
-constexpr int GX = 4, GY = 8, GZ = 64;
-
-// Every output cell sums the contributions of the particles whose first
-// stencil cell is one of its S^3 lower neighbours, in fixed point.
-template <int S>
+// One workgroup owns one output tile.  The interior particles whose first
+// stencil cell lies in the tile extended by S-1 cells towards -x, -y, -z are
+// exactly those that can reach the tile; in key order they form one
+// contiguous run per extended (x, y) row (the z chunks of the tile and the
+// one before it), read from the prefix array.  Each
+// such particle adds its in-tile stencil contributions to an LDS tile with
+// 64-bit integer atomics (order-independent), and the tile is then stored
+// once with plain writes.
+template <int S, bool CPLX>
 __global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ start, const double* __restrict__ spos,
                                                     const double* __restrict__ sw, AssignGeom g, double scale,
                                                     long long* __restrict__ out_re, long long* __restrict__ out_im) {
 #pragma clang fp contract(off)
-    constexpr int LX = GX + S - 1, LY = GY + S - 1, LZ = GZ + S;
-    __shared__ int st[LX * LY * LZ];
+    constexpr int RX = GX + S - 1, RY = GY + S - 1, NR = RX * RY, NC = GX * GY * GZ;
+    __shared__ unsigned long long acc[CPLX ? 2 * NC : NC];
+    __shared__ int rbeg[NR], roff[NR + 1];
     const int x0 = blockIdx.z * GX, y0 = blockIdx.y * GY, z0 = blockIdx.x * GZ;
-    for (int e = threadIdx.x; e < LX * LY * LZ; e += 256) {
-        const int dz = e % LZ, dy = (e / LZ) % LY, dx = e / (LZ * LY);
-        const int xx = x0 - (S - 1) + dx, yy = y0 - (S - 1) + dy, zz = z0 - (S - 1) + dz;
-        int v = 0;
-        if (xx >= 0 && xx < g.n[0] && yy >= 0 && yy < g.n[1]) {
-            const long long lin = ((long long)xx * g.n[1] + yy) * g.n[2] + zz;
-            if (lin >= 0 && lin <= g.ng) v = start[lin];
+    for (int e = threadIdx.x; e < (CPLX ? 2 * NC : NC); e += 256) acc[e] = 0ull;
+    if (threadIdx.x < NR) {
+        const int r = threadIdx.x;
+        const int xx = x0 - (S - 1) + r / RY, yy = y0 - (S - 1) + r % RY;
+        const int zlo = (z0 - (S - 1) > 0) ? z0 - (S - 1) : 0;
+        const int zhi = (z0 + GZ - 1 < g.n[2] - S) ? z0 + GZ - 1 : g.n[2] - S;
+        int b = 0, n = 0;
+        if (xx >= 0 && xx <= g.n[0] - S && yy >= 0 && yy <= g.n[1] - S && zlo <= zhi) {
+            const long long lo = ((long long)xx * g.n[1] + yy) * chunks_z(g);
+            b = start[lo + zlo / GZ];
+            n = start[lo + zhi / GZ + 1] - b;
         }
-        st[e] = v;
+        rbeg[r] = b;
+        roff[r + 1] = n;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < GX * GY * GZ; e += 256) {
-        const int x = x0 + e / (GY * GZ), y = y0 + (e / GZ) % GY, z = z0 + e % GZ;
-        if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
-        long long are = 0, aim = 0;
-        for (int dx = 0; dx < S; dx++) {
-            const int sx = x - dx;
-            if (sx < 0 || sx > g.n[0] - S) continue;
-            for (int dy = 0; dy < S; dy++) {
-                const int sy = y - dy;
-                if (sy < 0 || sy > g.n[1] - S) continue;
-                for (int dz = 0; dz < S; dz++) {
-                    const int sz = z - dz;
-                    if (sz < 0 || sz > g.n[2] - S) continue;
-                    const int t = ((sx - x0 + S - 1) * LY + (sy - y0 + S - 1)) * LZ + (sz - z0 + S - 1);
-                    const int b = st[t], en = st[t + 1];
-                    for (int p = b; p < en; p++) {
-                        double wt[3][3];
-                        int iw0[3];
-                        for (int a = 0; a < 3; a++)
-                            stencil_axis(g.assign, axis_xp(spos[3 * (long long)p + a], g.n[a], g.L[a]), &iw0[a], wt[a]);
-                        const double wij = wt[0][dx] * wt[1][dy];
-                        const double wijk = wij * wt[2][dz];
-                        const double wre = sw ? sw[2 * (long long)p] : 1.0;
-                        are += __double2ll_rn(wre * wijk * scale);
-                        if (out_im) {
-                            const double wim = sw ? sw[2 * (long long)p + 1] : 0.0;
-                            aim += __double2ll_rn(wim * wijk * scale);
-                        }
-                    }
+    if (threadIdx.x == 0) {
+        roff[0] = 0;
+        for (int r = 0; r < NR; r++) roff[r + 1] += roff[r];
+    }
+    __syncthreads();
+    const int total = roff[NR];
+    for (int j = threadIdx.x; j < total; j += 256) {
+        int r = 0;
+        while (roff[r + 1] <= j) r++;
+        const long long p = (long long)rbeg[r] + (j - roff[r]);
+        double wt[3][3];
+        int iw0[3];
+        for (int a = 0; a < 3; a++) stencil_axis(g.assign, axis_xp(spos[3 * p + a], g.n[a], g.L[a]), &iw0[a], wt[a]);
+        const double wre = sw ? sw[2 * p] : 1.0;
+        const double wim = sw ? sw[2 * p + 1] : 0.0;
+        for (int i = 0; i < S; i++) {
+            const int ox = iw0[0] + i - x0;
+            if (ox < 0 || ox >= GX) continue;
+            for (int jj = 0; jj < S; jj++) {
+                const int oy = iw0[1] + jj - y0;
+                if (oy < 0 || oy >= GY) continue;
+                const double wij = wt[0][i] * wt[1][jj];
+                for (int k = 0; k < S; k++) {
+                    const int oz = iw0[2] + k - z0;
+                    if (oz < 0 || oz >= GZ) continue;
+                    const double wijk = wij * wt[2][k];
+                    const int c = (ox * GY + oy) * GZ + oz;
+                    atomicAdd(&acc[c], (unsigned long long)__double2ll_rn(wre * wijk * scale));
+                    if (CPLX) atomicAdd(&acc[NC + c], (unsigned long long)__double2ll_rn(wim * wijk * scale));
                 }
             }
         }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NC; e += 256) {
+        const int x = x0 + e / (GY * GZ), y = y0 + (e / GZ) % GY, z = z0 + e % GZ;
+        if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
         const long long lin = ((long long)x * g.n[1] + y) * g.n[2] + z;
-        out_re[lin] = are;
-        if (out_im) out_im[lin] = aim;
+        out_re[lin] = (long long)acc[e];
+        if (CPLX) out_im[lin] = (long long)acc[NC + e];
     }
 }
 
@@ -294,9 +315,10 @@ __global__ __launch_bounds__(256) void k_assign_border(const double* __restrict_
         scatter_particle<S>(pos, w, border[1 + i], g, scale, gre, gim);
 }
 
-int sort_work_alloc(hm_ctx* ctx, SortWork& w, long long ng, long long np, bool weights) {
-    const long long nb = (ng + 1 + SCAN_BLK - 1) / SCAN_BLK;
-    if (dev_alloc(ctx, w.start, sizeof(int) * (size_t)(ng + 1))) return -1;
+int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights) {
+    const long long nk = n_keys(g);
+    const long long nb = (nk + 1 + SCAN_BLK - 1) / SCAN_BLK;
+    if (dev_alloc(ctx, w.start, sizeof(int) * (size_t)(nk + 1))) return -1;
     if (dev_alloc(ctx, w.slot, sizeof(int) * (size_t)(np > 0 ? np : 1))) return -1;
     if (dev_alloc(ctx, w.spos, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1))) return -1;
     if (weights && dev_alloc(ctx, w.sw, sizeof(double) * 2 * (size_t)(np > 0 ? np : 1))) return -1;
@@ -325,7 +347,7 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
     double* sw = d_w ? (double*)w.sw.p : nullptr;
     auto* re = reinterpret_cast<unsigned long long*>(d_re);
     auto* im = reinterpret_cast<unsigned long long*>(d_im);
-    const long long n = g.ng + 1;
+    const long long n = n_keys(g) + 1;
     const unsigned nb = (unsigned)((n + SCAN_BLK - 1) / SCAN_BLK);
     const unsigned pb = (unsigned)((np + 255) / 256);
     if (fill_zero(ctx, start, sizeof(int) * (size_t)n)) return -1;
@@ -348,7 +370,12 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
     const dim3 gg((unsigned)((g.n[2] + GZ - 1) / GZ), (unsigned)((g.n[1] + GY - 1) / GY),
                   (unsigned)((g.n[0] + GX - 1) / GX));
     if (ctx->launch("assign_gather", [&] {
-            hipLaunchKernelGGL(k_sort_gather<S>, gg, dim3(256), 0, ctx->stream, start, spos, sw, g, scale, d_re, d_im);
+            if (d_im)
+                hipLaunchKernelGGL((k_sort_gather<S, true>), gg, dim3(256), 0, ctx->stream, start, spos, sw, g, scale,
+                                   d_re, d_im);
+            else
+                hipLaunchKernelGGL((k_sort_gather<S, false>), gg, dim3(256), 0, ctx->stream, start, spos, sw, g, scale,
+                                   d_re, d_im);
         }))
         return -1;
     if (np > 0 && ctx->launch("assign_border", [&] {

@@ -75,14 +75,14 @@ int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const do
                  double scale, long long* d_re, long long* d_im);
 
 // Sort-by-cell assignment: particles whose whole stencil lies inside the
-// mesh are counting-sorted by their first stencil cell and every output cell
-// gathers its <= 27 neighbour cells in one thread (no atomics, one plain
-// store per cell); the few particles whose stencil leaves the mesh (the Q1
+// mesh are counting-sorted by the 64-cell z chunk of their first stencil cell
+// and each output tile accumulates the particles that reach it in LDS
+// (integer atomics there, one plain store per cell in HBM); the few particles whose stencil leaves the mesh (the Q1
 // spill/drop cases) are added afterwards with atomics.  Each contribution is
 // rounded to the same fixed-point value as in assign_fixed, so the grid is
 // bitwise identical to it.
 struct SortWork {
-    DevBuf start;   // int32 [ng + 1]: particle count, then exclusive prefix, per first-stencil cell
+    DevBuf start;   // int32 [keys + 1]: particle count, then exclusive prefix, per (x, y, z/64) chunk
     DevBuf slot;    // int32 [np]: rank within the cell, or -1 for a border particle
     DevBuf spos;    // double [3 np] positions in cell order
     DevBuf sw;      // double [2 np] weights in cell order
@@ -90,7 +90,7 @@ struct SortWork {
     DevBuf bsum;    // int32 scan block sums
     size_t bytes() const { return start.bytes + slot.bytes + spos.bytes + sw.bytes + border.bytes + bsum.bytes; }
 };
-int sort_work_alloc(hm_ctx* ctx, SortWork& w, long long ng, long long np, bool weights);
+int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights);
 void sort_work_free(SortWork& w);
 int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
                   double scale, long long* d_re, long long* d_im, SortWork& w);
