@@ -59,10 +59,13 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8):
         "xi_pass_x": 2 * half,
         "xi_pass_y": 2 * half,
         "xi_pass_z": half + n ** 3 * (elem // 2),
-        # the 21 transforms: 20 shells of support w, one delta/W of support W3 (averaged per launch)
-        "tri_pass_x": ((nk * (w ** 3 + M * w * w) + (W3 ** 3 + M * W3 * W3)) * elem) / (nk + 1),
-        "tri_pass_y": ((nk * (M * w * w + M * M * w) + (M * W3 * W3 + M * M * W3)) * elem) / (nk + 1),
-        "tri_pass_z": ((nk * (M * M * w + M ** 3) + (M * M * W3 + M ** 3)) * elem) / (nk + 1),
+        # triple-product fields: nk shells of support w, one delta/W of support W3 (compulsory in + out)
+        "tri_stage": W3 ** 3 * elem * 2 + w ** 3 * 2,
+        "tri_pass_x": w ** 3 * (elem + 2) + nk * M * w * w * elem,
+        "tri3_pass_x": (W3 ** 3 + M * W3 * W3) * elem,
+        "tri_pass_y": nk * (M * w * w + M * M * w) * elem,
+        "tri3_pass_y": (M * W3 * W3 + M * M * W3) * elem,
+        "tri_z_gram": (nk * M * M * w + M * M * W3) * elem,
         "gram": (nk + 1) * M ** 3 * elem,
     }
     return table.get(name)

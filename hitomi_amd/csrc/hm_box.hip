@@ -225,7 +225,7 @@ template <typename T> struct ShellSrc {
         return sgn >= 0 ? sgn : sgn + n[a];
     }
     // o = y, b = z, p = x on the M mesh
-    __device__ inline C<T> operator()(int o, int b, int p, long long) const {
+    __device__ inline C<T> operator()(int, int o, int b, int p, long long) const {
 #pragma clang fp contract(off)
         const int i = to_n(p, 0), j = to_n(o, 1), k = to_n(b, 2);
         if (shell) {
@@ -255,7 +255,7 @@ template <typename T> struct XiSrc {
     int wp;
     double sn, inv_v;
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
-    __device__ inline C<T> operator()(int, int b, int p, long long addr) const {
+    __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
 #pragma clang fp contract(off)
         const int i = p, j = b / nzh, k = b % nzh;
         const C<T> d = dk[addr];
@@ -404,6 +404,234 @@ __global__ __launch_bounds__(256) void k_gram_sym(const C<T>* __restrict__ field
 }
 
 // ---------------------------------------------------------------------------
+// triple-product fields on the M mesh
+// ---------------------------------------------------------------------------
+// support that actually prunes an axis of length M (-1: whole axis)
+static inline int sup_eff(int s, int M) { return (s >= 0 && 2 * s + 1 < M) ? s : -1; }
+
+// compact index of an in-support signed index (inverse of support_map)
+__host__ __device__ inline int support_unmap(int m, int s, int n) {
+    if (s < 0) return m;
+    return (m <= s) ? m : m - (n - (2 * s + 1));
+}
+
+// Stage: D = delta(k) / W(k) (field.hpp:378-379, 610-639) on the support cube
+// of delta/W, written in place of the first transform's input, and, for the
+// modes inside the shell support cube, the first and last k-shell that holds
+// the mode (shell rule of field.hpp:371-373 on the N-mesh |k|; two shells only
+// where rounding makes neighbouring bin edges overlap).
+template <typename T> struct TriStage {
+    const C<T>* dk;
+    int n[3], nzh, M[3];
+    int s3[3], c3[3];  // delta/W support and compact extents
+    int s[3], cs[3];   // shell support and compact extents
+    AxisTabs t;
+    int wp, nk;
+    const double* lower;
+    const double* upper;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tri_stage(TriStage<T> a, C<T>* __restrict__ D, char2* __restrict__ sid) {
+#pragma clang fp contract(off)
+    const long long tot = (long long)a.c3[0] * a.c3[1] * a.c3[2];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    const int cz = (int)(e % a.c3[2]);
+    const long long r = e / a.c3[2];
+    const int cy = (int)(r % a.c3[1]), cx = (int)(r / a.c3[1]);
+    const int x = support_map(cx, a.s3[0], a.M[0]), y = support_map(cy, a.s3[1], a.M[1]),
+              z = support_map(cz, a.s3[2], a.M[2]);
+    int idx[3];
+    const int mm[3] = {x, y, z};
+    for (int q = 0; q < 3; q++) {
+        if (a.M[q] == a.n[q]) {
+            idx[q] = mm[q];
+        } else {
+            const int sg = (mm[q] <= a.M[q] / 2) ? mm[q] : mm[q] - a.M[q];
+            idx[q] = sg >= 0 ? sg : sg + a.n[q];
+        }
+    }
+    const int i = idx[0], j = idx[1], k = idx[2];
+    C<T> d;
+    if (k < a.nzh) {
+        d = a.dk[((long long)i * a.n[1] + j) * a.nzh + k];
+    } else {
+        const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
+        d = cconj<T>(a.dk[((long long)im * a.n[1] + jm) * a.nzh + (a.n[2] - k)]);
+    }
+    const double W = wpow((a.t.wf[0][i] * a.t.wf[1][j]) * a.t.wf[2][k], a.wp);
+    D[((long long)x * a.M[1] + y) * a.M[2] + z] = mk<T>((T)((double)d.x / W), (T)((double)d.y / W));
+    if (in_support(x, a.s[0], a.M[0]) && in_support(y, a.s[1], a.M[1]) && in_support(z, a.s[2], a.M[2])) {
+        const double kmag = kmag3(a.t.kv[0][i], a.t.kv[1][j], a.t.kv[2][k]);
+        int lo = -1, hi = -2;
+        for (int b = 0; b < a.nk; b++)
+            if (kmag > a.lower[b] && kmag <= a.upper[b]) {
+                if (lo < 0) lo = b;
+                hi = b;
+            }
+        const int sx = support_unmap(x, a.s[0], a.M[0]), sy = support_unmap(y, a.s[1], a.M[1]),
+                  sz = support_unmap(z, a.s[2], a.M[2]);
+        sid[((long long)sx * a.cs[1] + sy) * a.cs[2] + sz] = make_char2((char)lo, (char)hi);
+    }
+}
+
+// First (x) pass source of shell f: D where shell f holds the mode, else 0.
+template <typename T> struct ShellMaskSrc {
+    const C<T>* D;
+    const char2* sid;
+    int M[3], s[3], cs[3];
+    __device__ inline C<T> operator()(int f, int o, int b, int p, long long) const {
+        const int sx = support_unmap(p, s[0], M[0]), sy = support_unmap(o, s[1], M[1]),
+                  sz = support_unmap(b, s[2], M[2]);
+        const char2 r = sid[((long long)sx * cs[1] + sy) * cs[2] + sz];
+        if (f < (int)r.x || f > (int)r.y) return mk<T>((T)0, (T)0);
+        return D[((long long)p * M[1] + o) * M[2] + b];
+    }
+};
+
+// Fused last (z) transform + triple sums.  A workgroup takes rows (x, y) of
+// the M mesh one at a time: the in-support z coefficients of every shell and
+// of delta/W are loaded into LDS (24 lines, delta/W on the last), transformed
+// there, and sum_z F_a F_3 F_b is accumulated for 4 x 4 blocks of shell pairs
+// (block (bi, bj), bi <= bj), each block split over z among `nsl` lanes.  The
+// fields never return to HBM.
+constexpr int TRI_B = 24;
+constexpr int TRI_RPW = 16;  // rows per workgroup
+
+template <int L> struct TriFits { static constexpr bool value = (size_t)TRI_B * (L + 1) * sizeof(float2) <= 65536; };
+
+template <typename T, int L>
+__global__ __launch_bounds__(256) void k_tri_z_gram(const C<T>* __restrict__ F, long long fstride, int nk, int s_shell,
+                                                   int s_f3, long long nrows, int ngrp, int nblk, int nsl,
+                                                   const C<T>* __restrict__ tw, double* __restrict__ part) {
+    constexpr int LP = LdsPad<L>::LP;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const int t = threadIdx.x;
+    const int blk = t / nsl, sl = t % nsl;
+    const bool active = blk < nblk;
+    int bi = 0, bj = 0;
+    if (active) {
+        int q = blk;
+        while (q >= ngrp - bi) {
+            q -= ngrp - bi;
+            bi++;
+        }
+        bj = bi + q;
+    }
+    double acc[32];
+#pragma unroll
+    for (int q = 0; q < 32; q++) acc[q] = 0.0;
+    const long long r0 = (long long)blockIdx.x * TRI_RPW;
+    const long long r1 = (r0 + TRI_RPW < nrows) ? r0 + TRI_RPW : nrows;
+    for (long long r = r0; r < r1; r++) {
+        for (int e = t; e < TRI_B * L; e += 256) {
+            const int f = e / L, z = e % L;
+            C<T> v = mk<T>((T)0, (T)0);
+            if (f < nk) {
+                if (in_support(z, s_shell, L)) v = F[(long long)f * fstride + r * L + z];
+            } else if (f == TRI_B - 1) {
+                if (in_support(z, s_f3, L)) v = F[(long long)nk * fstride + r * L + z];
+            }
+            sm[f * LP + z] = v;
+        }
+        __syncthreads();
+        lds_fft<T, +1, L, TRI_B>(sm, tw);
+        if (active) {
+            T ar[16], ai[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) ar[q] = ai[q] = (T)0;
+            const C<T>* f3 = sm + (TRI_B - 1) * LP;
+            const C<T>* fa = sm + 4 * bi * LP;
+            const C<T>* fb = sm + 4 * bj * LP;
+            for (int z = sl; z < L; z += nsl) {
+                const C<T> w = f3[z];
+                C<T> c[4], g[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    c[i] = cmul<T>(fa[i * LP + z], w);
+                    g[i] = fb[i * LP + z];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        ar[i * 4 + j] += c[i].x * g[j].x - c[i].y * g[j].y;
+                        ai[i * 4 + j] += c[i].x * g[j].y + c[i].y * g[j].x;
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                acc[q] += (double)ar[q];
+                acc[16 + q] += (double)ai[q];
+            }
+        }
+        __syncthreads();
+    }
+    // fixed-order reduction over the z slices of each block through LDS
+    double* red = reinterpret_cast<double*>(smraw);
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) red[t * 16 + q] = active ? acc[16 * h + q] : 0.0;
+        __syncthreads();
+        for (int o = t; o < nblk * 16; o += 256) {
+            const int b = o / 16, q = o % 16;
+            double sum = 0.0;
+            for (int l = 0; l < nsl; l++) sum += red[(b * nsl + l) * 16 + q];
+            part[((long long)blockIdx.x * nblk * 16 + o) * 2 + h] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int L>
+static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, int nk, int s_shell, int s_f3,
+                               long long nrows, int ngrp, int nblk, int nsl, double* part) {
+    const size_t lds = (size_t)TRI_B * LdsPad<L>::LP * sizeof(C<T>);
+    const C<T>* tw = ctx->twiddle<T>(L);
+    if (!tw) return -1;
+    const unsigned blocks = (unsigned)((nrows + TRI_RPW - 1) / TRI_RPW);
+    return ctx->launch("tri_z_gram", [&] {
+        hipLaunchKernelGGL((k_tri_z_gram<T, L>), dim3(blocks), dim3(256), lds, ctx->stream, F, fstride, nk, s_shell, s_f3,
+                           nrows, ngrp, nblk, nsl, tw, part);
+    });
+}
+
+// FP32 and a z extent whose 24-line tile fits 64 KB of LDS.
+static int tri_fused_ok(int prec, int L) {
+    if (prec != HM_FP32) return 0;
+    switch (L) {
+#define HM_CASE(LL) \
+    case LL: return TriFits<LL>::value ? 1 : 0;
+        HM_POW2_LENGTHS(HM_CASE)
+        HM_MIXED_LENGTHS(HM_CASE)
+#undef HM_CASE
+    default: return 0;
+    }
+}
+
+template <typename T>
+static int launch_tri_z_gram(hm_ctx* ctx, int L, const C<T>* F, long long fstride, int nk, int s_shell, int s_f3,
+                             long long nrows, int ngrp, int nblk, int nsl, double* part) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        switch (L) {
+#define HM_CASE(LL)                                                                                                    \
+    case LL:                                                                                                           \
+        if constexpr (TriFits<LL>::value)                                                                              \
+            return launch_tri_z_gram_L<T, LL>(ctx, F, fstride, nk, s_shell, s_f3, nrows, ngrp, nblk, nsl, part);       \
+        break;
+            HM_POW2_LENGTHS(HM_CASE)
+            HM_MIXED_LENGTHS(HM_CASE)
+#undef HM_CASE
+        default: break;
+        }
+    }
+    ctx->set_error("tri_z_gram: unsupported configuration");
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
 struct BoxPlanImpl {
@@ -430,6 +658,10 @@ struct BoxPlanImpl {
         pairs_d, a_start, a_rows, a_idx, xi_R, sn_pairs, fields, gram_part, gram_red, shell_cnt;
     long long gram_blocks = 0;
     int gram_vpb = 4096;
+    bool fused = false;  // z transform of the triple fields fused with the triple sums
+    int ngrp = 0, nblk = 0, nsl = 0;  // 4-shell groups, group pairs, z slices per pair block (fused)
+    int gram_cols = 0;                // doubles per partial row of the triple sums
+    DevBuf sid;                       // char2 first/last shell per mode of the shell support cube
 };
 
 static int alloc(hm_ctx* ctx, BoxPlanImpl* p, DevBuf& b, size_t bytes) {
@@ -458,8 +690,20 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
         const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
         if (alloc(ctx, p, p->fields, sizeof(C<T>) * (size_t)nvox * (size_t)(p->nk + 1))) return -1;
         p->gram_blocks = (nvox + p->gram_vpb - 1) / p->gram_vpb;
-        if (alloc(ctx, p, p->gram_part, sizeof(double2) * (size_t)p->gram_blocks * p->pairs.size())) return -1;
-        if (alloc(ctx, p, p->gram_red, sizeof(double2) * p->pairs.size())) return -1;
+        p->fused = tri_fused_ok(p->cfg.precision, p->M[2]) && p->nk < TRI_B;
+        p->gram_cols = 2 * (int)p->pairs.size();
+        if (p->fused) {
+            p->gram_blocks = ((long long)p->M[0] * p->M[1] + TRI_RPW - 1) / TRI_RPW;
+            p->ngrp = (p->nk + 3) / 4;
+            p->nblk = p->ngrp * (p->ngrp + 1) / 2;
+            p->nsl = 256 / p->nblk;
+            p->gram_cols = 2 * 16 * p->nblk;
+        }
+        if (alloc(ctx, p, p->gram_part, sizeof(double) * (size_t)p->gram_blocks * p->gram_cols)) return -1;
+        if (alloc(ctx, p, p->gram_red, sizeof(double) * (size_t)p->gram_cols)) return -1;
+        long long ns = 1;
+        for (int a = 0; a < 3; a++) ns *= support_count(sup_eff(p->s[a], p->M[a]), p->M[a]);
+        if (alloc(ctx, p, p->sid, sizeof(char2) * (size_t)ns)) return -1;
     }
     if (alloc(ctx, p, p->shell_cnt, sizeof(unsigned long long) * (size_t)p->nk)) return -1;
     return 0;
@@ -660,7 +904,7 @@ void plan_destroy(BoxPlanImpl* p) {
     if (!p) return;
     DevBuf* bufs[] = {&p->tab, &p->f2b_d, &p->lu_d, &p->kb_d, &p->fixed, &p->dk, &p->pk_part, &p->pk_cnt, &p->pk_red,
                       &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
-                      &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt};
+                      &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -681,10 +925,50 @@ static ColGeom geom(long long OS, long long PS, int n_outer, int outer_full, int
     return g;
 }
 
+// x pass over nf fields (stride FS) of support sup: lines (y, z) in the
+// support, points x in the support.
+static ColGeom tri_geom_x(const int M[3], const int sup[3], long long FS, int nf) {
+    ColGeom g = geom(M[2], (long long)M[1] * M[2], support_count(sup[1], M[1]), M[1], sup[1],
+                     support_count(sup[2], M[2]), M[2], sup[2], sup[0]);
+    g.FS = FS;
+    g.nf = nf;
+    return g;
+}
+// y pass: lines (x all, z in the support), points y in the support.
+static ColGeom tri_geom_y(const int M[3], const int sup[3], long long FS, int nf) {
+    ColGeom g = geom((long long)M[1] * M[2], M[2], M[0], M[0], -1, support_count(sup[2], M[2]), M[2], sup[2], sup[1]);
+    g.FS = FS;
+    g.nf = nf;
+    return g;
+}
+
+// Triple sums from the reduced partial columns to pair order (a <= b).
+static void gram_to_pairs(const BoxPlanImpl* p, const std::vector<double>& raw, std::vector<double>& out) {
+    if (!p->fused) {
+        out = raw;
+        return;
+    }
+    std::vector<int> pidx((size_t)p->nk * p->nk, -1);
+    for (size_t q = 0; q < p->pairs.size(); q++) pidx[(size_t)p->pairs[q].x * p->nk + p->pairs[q].y] = (int)q;
+    int blk = 0;
+    for (int bi = 0; bi < p->ngrp; bi++)
+        for (int bj = bi; bj < p->ngrp; bj++, blk++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    const int a = 4 * bi + i, b = 4 * bj + j;
+                    if (a >= p->nk || b >= p->nk || a > b) continue;
+                    const int q = pidx[(size_t)a * p->nk + b];
+                    const int o = blk * 16 + i * 4 + j;
+                    out[2 * (size_t)q] = raw[2 * (size_t)o];
+                    out[2 * (size_t)q + 1] = raw[2 * (size_t)o + 1];
+                }
+}
+
 // Inverse transform of one triple-product field on the M mesh:
 // x pass from the spectrum source, then y columns, then z rows.
 template <typename T>
-static int field_transform(hm_ctx* ctx, BoxPlanImpl* p, const ShellSrc<T>& src, C<T>* out, const int sup[3]) {
+static int field_transform(hm_ctx* ctx, BoxPlanImpl* p, const ShellSrc<T>& src, C<T>* out, const int sup[3],
+                           bool z_pass) {
     const int* M = p->M;
     const long long M12 = (long long)M[1] * M[2];
     ColGeom ga = geom(M[2], M12, support_count(sup[1], M[1]), M[1], sup[1] >= 0 && 2 * sup[1] + 1 < M[1] ? sup[1] : -1,
@@ -694,6 +978,7 @@ static int field_transform(hm_ctx* ctx, BoxPlanImpl* p, const ShellSrc<T>& src, 
     ColGeom gb = geom(M12, M[2], M[0], M[0], -1, support_count(sup[2], M[2]), M[2],
                       sup[2] >= 0 && 2 * sup[2] + 1 < M[2] ? sup[2] : -1, sup[1] >= 0 && 2 * sup[1] + 1 < M[1] ? sup[1] : -1);
     if (fft_cols<T>(ctx, "tri_pass_y", M[1], +1, out, out, gb)) return -1;
+    if (!z_pass) return 0;
     RowGeom rg{(long long)M[0] * M[1], M[2], M[2], sup[2] >= 0 && 2 * sup[2] + 1 < M[2] ? sup[2] : -1};
     return fft_rows<T>(ctx, "tri_pass_z", M[2], +1, out, out, rg);
 }
@@ -803,50 +1088,86 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         // 5. shell fields and delta/W on the triple mesh (field.hpp:346-401, 610-639, 328-344)
         const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
         C<T>* F = (C<T>*)p->fields.p;
-        ShellSrc<T> ss;
-        ss.dk = dk;
+        C<T>* F3 = F + (long long)p->nk * nvox;
+        TriStage<T> st;
+        st.dk = dk;
+        st.nzh = (int)nzh;
         for (int a = 0; a < 3; a++) {
-            ss.n[a] = m.n[a];
-            ss.M[a] = p->M[a];
+            st.n[a] = m.n[a];
+            st.M[a] = p->M[a];
+            st.s3[a] = sup_eff(p->s3[a], p->M[a]);
+            st.c3[a] = support_count(st.s3[a], p->M[a]);
+            st.s[a] = sup_eff(p->s[a], p->M[a]);
+            st.cs[a] = support_count(st.s[a], p->M[a]);
         }
-        ss.nzh = (int)nzh;
-        ss.t = tabs;
-        ss.wp = wp;
-        for (int b = 0; b < p->nk; b++) {
-            ss.shell = 1;
-            ss.lower = p->lower[b];
-            ss.upper = p->upper[b];
-            if (field_transform<T>(ctx, p, ss, F + (long long)b * nvox, p->s)) return -1;
-        }
-        ss.shell = 0;
-        if (field_transform<T>(ctx, p, ss, F + (long long)p->nk * nvox, p->s3)) return -1;
-        // 6. triple sums
-        const size_t lds = sizeof(C<T>) * (GRAM_CH + 2 * (size_t)p->nk * (GRAM_CH + 1));
-        if (ctx->launch("gram", [&] {
-                hipLaunchKernelGGL(k_gram_sym<T>, dim3((unsigned)p->gram_blocks), dim3(256), lds, ctx->stream, F, nvox,
-                                   p->nk, F + (long long)p->nk * nvox, nvox, p->gram_vpb, (const short2*)p->pairs_d.p,
-                                   npairs, (double2*)p->gram_part.p);
+        st.t = tabs;
+        st.wp = wp;
+        st.nk = p->nk;
+        st.lower = (const double*)p->lu_d.p;
+        st.upper = st.lower + p->nk;
+        const long long nst = (long long)st.c3[0] * st.c3[1] * st.c3[2];
+        if (ctx->launch("tri_stage", [&] {
+                hipLaunchKernelGGL(k_tri_stage<T>, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, ctx->stream, st,
+                                   F3, (char2*)p->sid.p);
             }))
             return -1;
-        if (reduce_cols<double>(ctx, "gram_reduce", (const double*)p->gram_part.p, p->gram_blocks, 2 * npairs,
+        ShellMaskSrc<T> ms;
+        ms.D = F3;
+        ms.sid = (const char2*)p->sid.p;
+        for (int a = 0; a < 3; a++) {
+            ms.M[a] = p->M[a];
+            ms.s[a] = st.s[a];
+            ms.cs[a] = st.cs[a];
+        }
+        // x pass of every shell (one launch), then of delta/W in place
+        ColGeom gx = tri_geom_x(p->M, st.s, nvox, p->nk);
+        if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
+        ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
+        if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
+        // y pass
+        ColGeom gy = tri_geom_y(p->M, st.s, nvox, p->nk);
+        if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
+        ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
+        if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;
+        // 6. z pass and triple sums
+        const long long nrows = (long long)p->M[0] * p->M[1];
+        if (p->fused) {
+            if (launch_tri_z_gram<T>(ctx, p->M[2], F, nvox, p->nk, st.s[2], st.s3[2], nrows, p->ngrp, p->nblk, p->nsl,
+                                     (double*)p->gram_part.p))
+                return -1;
+        } else {
+            RowGeom rg{nrows * p->nk, p->M[2], p->M[2], st.s[2]};
+            if (fft_rows<T>(ctx, "tri_pass_z", p->M[2], +1, F, F, rg)) return -1;
+            RowGeom rg3{nrows, p->M[2], p->M[2], st.s3[2]};
+            if (fft_rows<T>(ctx, "tri3_pass_z", p->M[2], +1, F3, F3, rg3)) return -1;
+            const size_t lds = sizeof(C<T>) * (GRAM_CH + 2 * (size_t)p->nk * (GRAM_CH + 1));
+            if (ctx->launch("gram", [&] {
+                    hipLaunchKernelGGL(k_gram_sym<T>, dim3((unsigned)p->gram_blocks), dim3(256), lds, ctx->stream, F,
+                                       nvox, p->nk, F3, nvox, p->gram_vpb, (const short2*)p->pairs_d.p, npairs,
+                                       (double2*)p->gram_part.p);
+                }))
+                return -1;
+        }
+        if (reduce_cols<double>(ctx, "gram_reduce", (const double*)p->gram_part.p, p->gram_blocks, p->gram_cols,
                                 (double*)p->gram_red.p))
             return -1;
     }
     // 7. results to host
-    std::vector<double> pk(ncols), gram(2 * (size_t)npairs), snp((size_t)npairs);
+    std::vector<double> pk(ncols), gram(2 * (size_t)npairs), graw((size_t)p->gram_cols), snp((size_t)npairs);
     std::vector<long long> cnt(p->nk);
     HM_HIP_CHECK(ctx, hipMemcpyAsync(pk.data(), p->pk_red.p, sizeof(double) * ncols, hipMemcpyDeviceToHost, ctx->stream));
     HM_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), p->pk_cnt_red.p, sizeof(long long) * p->nk, hipMemcpyDeviceToHost,
                                      ctx->stream));
     if (p->cfg.bk) {
-        HM_HIP_CHECK(ctx, hipMemcpyAsync(gram.data(), p->gram_red.p, sizeof(double) * 2 * npairs, hipMemcpyDeviceToHost,
-                                         ctx->stream));
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(graw.data(), p->gram_red.p, sizeof(double) * p->gram_cols,
+                                         hipMemcpyDeviceToHost, ctx->stream));
         HM_HIP_CHECK(ctx, hipMemcpyAsync(snp.data(), p->sn_pairs.p, sizeof(double) * npairs, hipMemcpyDeviceToHost,
                                          ctx->stream));
     }
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const int nk = p->nk;
     const double V = m.volume(), N = (double)np;
+    if (p->cfg.bk) gram_to_pairs(p, graw, gram);
     // P_l: average (field.hpp:839-845), (2l+1) and V/N^2 (powerspec.hpp:306-316)
     std::vector<double> praw((size_t)ba.ne * nk);
     for (int e = 0; e < ba.ne; e++)

@@ -204,10 +204,13 @@ __host__ __device__ inline int support_map(int c, int s, int n) {
 __host__ __device__ inline int support_count(int s, int n) { return (s < 0 || 2 * s + 1 >= n) ? n : 2 * s + 1; }
 __host__ __device__ inline bool in_support(int p, int s, int n) { return s < 0 || p <= s || p >= n - s; }
 
-// Column-pass geometry: element (o, b, p) at o*OS + b + p*PS, where o and b
-// are the actual outer/inner line indices.
+// Column-pass geometry: element (f, o, b, p) at f*FS + o*OS + b + p*PS, where
+// o and b are the actual outer/inner line indices and f < nf a batch index
+// (grid.z) over equally laid-out arrays.
 struct ColGeom {
     long long OS, PS;
+    long long FS = 0;
+    int nf = 1;
     int n_outer, outer_full, outer_s;  // compact count, full extent, support
     int n_inner, inner_full, inner_s;
     int point_s;                       // support on the transformed axis
@@ -222,7 +225,7 @@ struct RowGeom {
 // Plain global source / destination.
 template <typename T> struct GSrc {
     const C<T>* p;
-    __device__ inline C<T> operator()(int /*o*/, int /*b*/, int /*pt*/, long long addr) const { return p[addr]; }
+    __device__ inline C<T> operator()(int /*f*/, int /*o*/, int /*b*/, int /*pt*/, long long addr) const { return p[addr]; }
 };
 
 // Column FFT: grid.x = tiles of B compact inner lines, grid.y = compact outer index.
@@ -234,7 +237,8 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
     const int o = support_map(blockIdx.y, g.outer_s, g.outer_full);
-    const long long obase = (long long)o * g.OS;
+    const int f = blockIdx.z;
+    const long long obase = (long long)f * g.FS + (long long)o * g.OS;
     const int tid = threadIdx.x;
     const int bl = tid % B;
     const int bc = blockIdx.x * B + bl;  // compact inner index
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     constexpr int PSTEP = FFT_NT / B;
     for (int p = tid / B; p < L; p += PSTEP) {
         C<T> v = mk<T>((T)0, (T)0);
-        if (bvalid && in_support(p, g.point_s, L)) v = src(o, b, p, obase + b + (long long)p * g.PS);
+        if (bvalid && in_support(p, g.point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
         sm[bl * LP + p] = v;
     }
     __syncthreads();

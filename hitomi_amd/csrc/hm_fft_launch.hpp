@@ -17,7 +17,7 @@ int launch_cols_L(hm_ctx* ctx, const char* name, const Src& src, C<T>* dst, cons
         ctx->set_error("fft column pass: outer extent exceeds 65535");
         return -1;
     }
-    dim3 grid((unsigned)((g.n_inner + B - 1) / B), (unsigned)g.n_outer);
+    dim3 grid((unsigned)((g.n_inner + B - 1) / B), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
     const size_t lds = col_lds<T, L>();
     const C<T>* tw = ctx->twiddle<T>(L);
     if (!tw) return -1;
