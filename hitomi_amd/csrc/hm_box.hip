@@ -501,8 +501,10 @@ constexpr int TRI_RPW = 16;  // rows per workgroup
 
 template <int L> struct TriFits { static constexpr bool value = (size_t)TRI_B * (L + 1) * sizeof(float2) <= 65536; };
 
+constexpr int TRI_NT = 512;  // lanes per workgroup of the fused kernel
+
 template <typename T, int L>
-__global__ __launch_bounds__(256) void k_tri_z_gram(const C<T>* __restrict__ F, long long fstride, int nk, int s_shell,
+__global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ F, long long fstride, int nk, int s_shell,
                                                    int s_f3, long long nrows, int ngrp, int nblk, int nsl,
                                                    const C<T>* __restrict__ tw, double* __restrict__ part) {
     constexpr int LP = LdsPad<L>::LP;
@@ -520,28 +522,44 @@ __global__ __launch_bounds__(256) void k_tri_z_gram(const C<T>* __restrict__ F, 
         }
         bj = bi + q;
     }
-    double acc[32];
+    // FP32 sums over the workgroup's rows (TRI_RPW rows x L / nsl voxels per
+    // lane), FP64 from the slice reduction on
+    T ar[16], ai[16];
 #pragma unroll
-    for (int q = 0; q < 32; q++) acc[q] = 0.0;
+    for (int q = 0; q < 16; q++) ar[q] = ai[q] = (T)0;
     const long long r0 = (long long)blockIdx.x * TRI_RPW;
     const long long r1 = (r0 + TRI_RPW < nrows) ? r0 + TRI_RPW : nrows;
-    for (long long r = r0; r < r1; r++) {
-        for (int e = t; e < TRI_B * L; e += 256) {
+    // the next row's in-support coefficients are loaded into registers while
+    // the current row is transformed and reduced
+    constexpr int NPRE = (TRI_B * L + TRI_NT - 1) / TRI_NT;
+    C<T> pre[NPRE];
+    auto load_row = [&](long long r) {
+#pragma unroll
+        for (int q = 0; q < NPRE; q++) {
+            const int e = t + q * TRI_NT;
             const int f = e / L, z = e % L;
             C<T> v = mk<T>((T)0, (T)0);
-            if (f < nk) {
-                if (in_support(z, s_shell, L)) v = F[(long long)f * fstride + r * L + z];
-            } else if (f == TRI_B - 1) {
-                if (in_support(z, s_f3, L)) v = F[(long long)nk * fstride + r * L + z];
+            if (e < TRI_B * L) {
+                if (f < nk) {
+                    if (in_support(z, s_shell, L)) v = F[(long long)f * fstride + r * L + z];
+                } else if (f == TRI_B - 1) {
+                    if (in_support(z, s_f3, L)) v = F[(long long)nk * fstride + r * L + z];
+                }
             }
-            sm[f * LP + z] = v;
+            pre[q] = v;
+        }
+    };
+    if (r0 < r1) load_row(r0);
+    for (long long r = r0; r < r1; r++) {
+#pragma unroll
+        for (int q = 0; q < NPRE; q++) {
+            const int e = t + q * TRI_NT;
+            if (e < TRI_B * L) sm[(e / L) * LP + e % L] = pre[q];
         }
         __syncthreads();
-        lds_fft<T, +1, L, TRI_B>(sm, tw);
+        if (r + 1 < r1) load_row(r + 1);
+        lds_fft<T, +1, L, TRI_B, TRI_NT>(sm, tw);
         if (active) {
-            T ar[16], ai[16];
-#pragma unroll
-            for (int q = 0; q < 16; q++) ar[q] = ai[q] = (T)0;
             const C<T>* f3 = sm + (TRI_B - 1) * LP;
             const C<T>* fa = sm + 4 * bi * LP;
             const C<T>* fb = sm + 4 * bj * LP;
@@ -561,25 +579,25 @@ __global__ __launch_bounds__(256) void k_tri_z_gram(const C<T>* __restrict__ F, 
                         ai[i * 4 + j] += c[i].x * g[j].y + c[i].y * g[j].x;
                     }
             }
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                acc[q] += (double)ar[q];
-                acc[16 + q] += (double)ai[q];
-            }
         }
         __syncthreads();
     }
-    // fixed-order reduction over the z slices of each block through LDS
+    // fixed-order reduction over the z slices of each block through LDS, four
+    // of the 16 pair sums (re or im) per round
     double* red = reinterpret_cast<double*>(smraw);
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < 8; h++) {
+        const int q0 = (h % 4) * 4;
 #pragma unroll
-        for (int q = 0; q < 16; q++) red[t * 16 + q] = active ? acc[16 * h + q] : 0.0;
+        for (int q = 0; q < 4; q++) {
+            const T v = (h < 4) ? ar[q0 + q] : ai[q0 + q];
+            red[t * 4 + q] = active ? (double)v : 0.0;
+        }
         __syncthreads();
-        for (int o = t; o < nblk * 16; o += 256) {
-            const int b = o / 16, q = o % 16;
+        for (int o = t; o < nblk * 4; o += TRI_NT) {
+            const int b = o / 4, q = o % 4;
             double sum = 0.0;
-            for (int l = 0; l < nsl; l++) sum += red[(b * nsl + l) * 16 + q];
-            part[((long long)blockIdx.x * nblk * 16 + o) * 2 + h] = sum;
+            for (int l = 0; l < nsl; l++) sum += red[(b * nsl + l) * 4 + q];
+            part[((long long)blockIdx.x * nblk * 16 + b * 16 + q0 + q) * 2 + (h < 4 ? 0 : 1)] = sum;
         }
         __syncthreads();
     }
@@ -588,12 +606,14 @@ __global__ __launch_bounds__(256) void k_tri_z_gram(const C<T>* __restrict__ F, 
 template <typename T, int L>
 static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, int nk, int s_shell, int s_f3,
                                long long nrows, int ngrp, int nblk, int nsl, double* part) {
-    const size_t lds = (size_t)TRI_B * LdsPad<L>::LP * sizeof(C<T>);
+    const size_t fft_lds = (size_t)TRI_B * LdsPad<L>::LP * sizeof(C<T>);
+    const size_t red_lds = (size_t)TRI_NT * 4 * sizeof(double);
+    const size_t lds = fft_lds > red_lds ? fft_lds : red_lds;
     const C<T>* tw = ctx->twiddle<T>(L);
     if (!tw) return -1;
     const unsigned blocks = (unsigned)((nrows + TRI_RPW - 1) / TRI_RPW);
     return ctx->launch("tri_z_gram", [&] {
-        hipLaunchKernelGGL((k_tri_z_gram<T, L>), dim3(blocks), dim3(256), lds, ctx->stream, F, fstride, nk, s_shell, s_f3,
+        hipLaunchKernelGGL((k_tri_z_gram<T, L>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride, nk, s_shell, s_f3,
                            nrows, ngrp, nblk, nsl, tw, part);
     });
 }
@@ -696,7 +716,7 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
             p->gram_blocks = ((long long)p->M[0] * p->M[1] + TRI_RPW - 1) / TRI_RPW;
             p->ngrp = (p->nk + 3) / 4;
             p->nblk = p->ngrp * (p->ngrp + 1) / 2;
-            p->nsl = 256 / p->nblk;
+            p->nsl = TRI_NT / p->nblk;
             p->gram_cols = 2 * 16 * p->nblk;
         }
         if (alloc(ctx, p, p->gram_part, sizeof(double) * (size_t)p->gram_blocks * p->gram_cols)) return -1;

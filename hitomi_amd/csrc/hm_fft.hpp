@@ -104,18 +104,18 @@ template <typename T, int DIR> struct Dft<T, 5, DIR> {
 // ---------------------------------------------------------------------------
 template <int L> struct LdsPad { static constexpr int LP = L + 1; };
 
-template <typename T, int DIR, int L, int B, int NS, int R, int... RS>
+template <typename T, int DIR, int L, int B, int NT, int NS, int R, int... RS>
 __device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ tw) {
     constexpr int LP = LdsPad<L>::LP;
     constexpr int NBF = L / R;
     constexpr int TOT = B * NBF;
-    constexpr int Q = (TOT + FFT_NT - 1) / FFT_NT;
+    constexpr int Q = (TOT + NT - 1) / NT;
     const int tid = threadIdx.x;
     C<T> v[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const int idx = tid + q * FFT_NT;
-        if ((TOT % FFT_NT == 0) || idx < TOT) {
+        const int idx = tid + q * NT;
+        if ((TOT % NT == 0) || idx < TOT) {
             const int b = idx / NBF, j = idx % NBF;
             const int k = j % NS;
 #pragma unroll
@@ -135,8 +135,8 @@ __device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ 
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const int idx = tid + q * FFT_NT;
-        if ((TOT % FFT_NT == 0) || idx < TOT) {
+        const int idx = tid + q * NT;
+        if ((TOT % NT == 0) || idx < TOT) {
             const int b = idx / NBF, j = idx % NBF;
             const int k = j % NS;
             const int base = (j / NS) * NS * R + k;
@@ -145,32 +145,32 @@ __device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ 
         }
     }
     __syncthreads();
-    if constexpr (sizeof...(RS) > 0) lds_stockham<T, DIR, L, B, NS * R, RS...>(sm, tw);
+    if constexpr (sizeof...(RS) > 0) lds_stockham<T, DIR, L, B, NT, NS * R, RS...>(sm, tw);
 }
 
 // Radix plans of the supported line lengths.
-template <typename T, int DIR, int L, int B>
+template <typename T, int DIR, int L, int B, int NT = FFT_NT>
 __device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw) {
-    if constexpr (L == 8) lds_stockham<T, DIR, L, B, 1, 8>(sm, tw);
-    else if constexpr (L == 16) lds_stockham<T, DIR, L, B, 1, 4, 4>(sm, tw);
-    else if constexpr (L == 32) lds_stockham<T, DIR, L, B, 1, 8, 4>(sm, tw);
-    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, 1, 8, 2, 3>(sm, tw);
-    else if constexpr (L == 64) lds_stockham<T, DIR, L, B, 1, 8, 8>(sm, tw);
-    else if constexpr (L == 80) lds_stockham<T, DIR, L, B, 1, 8, 2, 5>(sm, tw);
-    else if constexpr (L == 96) lds_stockham<T, DIR, L, B, 1, 8, 4, 3>(sm, tw);
-    else if constexpr (L == 128) lds_stockham<T, DIR, L, B, 1, 8, 4, 4>(sm, tw);
-    else if constexpr (L == 160) lds_stockham<T, DIR, L, B, 1, 8, 4, 5>(sm, tw);
-    else if constexpr (L == 192) lds_stockham<T, DIR, L, B, 1, 8, 8, 3>(sm, tw);
-    else if constexpr (L == 256) lds_stockham<T, DIR, L, B, 1, 8, 8, 4>(sm, tw);
-    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, 1, 8, 4, 3, 3>(sm, tw);
-    else if constexpr (L == 320) lds_stockham<T, DIR, L, B, 1, 8, 8, 5>(sm, tw);
-    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, 1, 8, 8, 2, 3>(sm, tw);
-    else if constexpr (L == 480) lds_stockham<T, DIR, L, B, 1, 8, 4, 3, 5>(sm, tw);
-    else if constexpr (L == 512) lds_stockham<T, DIR, L, B, 1, 8, 8, 8>(sm, tw);
-    else if constexpr (L == 576) lds_stockham<T, DIR, L, B, 1, 8, 8, 3, 3>(sm, tw);
-    else if constexpr (L == 640) lds_stockham<T, DIR, L, B, 1, 8, 8, 2, 5>(sm, tw);
-    else if constexpr (L == 768) lds_stockham<T, DIR, L, B, 1, 8, 8, 4, 3>(sm, tw);
-    else if constexpr (L == 1024) lds_stockham<T, DIR, L, B, 1, 8, 8, 4, 4>(sm, tw);
+    if constexpr (L == 8) lds_stockham<T, DIR, L, B, NT, 1, 8>(sm, tw);
+    else if constexpr (L == 16) lds_stockham<T, DIR, L, B, NT, 1, 4, 4>(sm, tw);
+    else if constexpr (L == 32) lds_stockham<T, DIR, L, B, NT, 1, 8, 4>(sm, tw);
+    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3>(sm, tw);
+    else if constexpr (L == 64) lds_stockham<T, DIR, L, B, NT, 1, 8, 8>(sm, tw);
+    else if constexpr (L == 80) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 5>(sm, tw);
+    else if constexpr (L == 96) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 3>(sm, tw);
+    else if constexpr (L == 128) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 4>(sm, tw);
+    else if constexpr (L == 160) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 5>(sm, tw);
+    else if constexpr (L == 192) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 3>(sm, tw);
+    else if constexpr (L == 256) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4>(sm, tw);
+    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 3, 3>(sm, tw);
+    else if constexpr (L == 320) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 5>(sm, tw);
+    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 2, 3>(sm, tw);
+    else if constexpr (L == 480) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 3, 5>(sm, tw);
+    else if constexpr (L == 512) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 8>(sm, tw);
+    else if constexpr (L == 576) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 3, 3>(sm, tw);
+    else if constexpr (L == 640) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 2, 5>(sm, tw);
+    else if constexpr (L == 768) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4, 3>(sm, tw);
+    else if constexpr (L == 1024) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4, 4>(sm, tw);
     else static_assert(L == 8, "unsupported FFT length");
 }
 

@@ -92,6 +92,8 @@ BK_CASES = [
     # n, L, N, kmin, kmax, nk, expected triple mesh
     (64, 2000.0, 20000, 0.01, 0.025, 4, 48),   # exact re-grid 64 -> 48
     (32, 1000.0, 20000, 0.02, 0.08, 4, 32),    # shells pruned, no re-grid
+    (64, 2000.0, 20000, 0.005, 0.05, 10, 64),  # 10 shells: three 4-shell groups in the fused triple kernel
+    (64, 1500.0, 15000, 0.01, 0.04, 7, 48),    # 7 shells: a partial group; re-grid 64 -> 48
 ]
 
 
