@@ -71,6 +71,12 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8):
     return table.get(name)
 
 
+# HBM bytes per launch of the dominant kernels, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on this workload and scaled by the counters' reading
+# of a 1 GiB copy (x2.000 for FETCH_SIZE, x1.000 for WRITE_SIZE).
+MEASURED_TRAFFIC = {"tri_z_gram": 2334.7e6}
+
+
 def make_mocks(count, seed0=1000):
     from hitomi_amd import mocks
 
@@ -112,6 +118,21 @@ def cpu_baseline(pos, budget_s=30.0):
               f"3 P + 20 B jobs = {sec_per_mock:.0f} s/mock")
     return {"value": 3600.0 / sec_per_mock, "unit": "mocks/hour", "cores": 1, "kind": "port", "sample": sample,
             "sample_seconds": round(t_assign + t_pk + t_bk, 1)}
+
+
+def copy_calibration(ctx, nbytes):
+    """One device-to-device copy of nbytes with the library's copy kernel
+    (hm::k_copy16): the known-size kernel that scales the FETCH_SIZE /
+    WRITE_SIZE counters."""
+    a = ctx.empty((nbytes // 16, 2), np.float64)
+    b = ctx.empty((nbytes // 16, 2), np.float64)
+    ctx.check(ctx.lib.hm_memset_zero(ctx.h, a.ptr, a.nbytes), "memset")
+    ctx.sync()
+    b.copy_from(a)
+    ctx.sync()
+    log(f"copy calibration: {a.nbytes} bytes read, {a.nbytes} bytes written")
+    a.free()
+    b.free()
 
 
 def run(args):
@@ -172,6 +193,8 @@ def run(args):
     b_dom = algorithmic_bytes(dom_name, N_MESH, M, s, KW["n_kbin"], npart)
     achieved = b_dom / (ms_dom / n_dom) / 1e6
     plan.close()
+    if args.copy_calibration:
+        copy_calibration(ctx, args.copy_calibration)
 
     plain = None
     if args.plain_steps > 0:
@@ -207,7 +230,8 @@ def run(args):
             "k_bins": "20 bins, centres 0.015-0.205 h/Mpc", "triple_mesh": M, "parallelism": "1 GPU",
         },
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get(dom_name),
                      "algorithmic_bytes_per_launch": int(b_dom), "avg_launch_ms": round(ms_dom / n_dom, 4)},
     }
     if plain:
@@ -227,6 +251,8 @@ def main():
     ap.add_argument("--mocks", type=int, default=2, help="distinct resident mocks cycled through")
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain FP64 path (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one device copy of BYTES (scales HBM counters under rocprofv3 --pmc)")
     run(ap.parse_args())
 
 

@@ -75,6 +75,7 @@ _SIGS = {
     "hm_dev_free": (ct.c_int, [_VP, _VP]),
     "hm_memcpy_h2d": (ct.c_int, [_VP, _VP, _VP, ct.c_size_t]),
     "hm_memcpy_d2h": (ct.c_int, [_VP, _VP, _VP, ct.c_size_t]),
+    "hm_memcpy_d2d": (ct.c_int, [_VP, _VP, _VP, ct.c_size_t]),
     "hm_memset_zero": (ct.c_int, [_VP, _VP, ct.c_size_t]),
     "hm_timing_enable": (ct.c_int, [_VP, ct.c_int]),
     "hm_timing_get": (ct.c_int, [_VP, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]),
@@ -150,6 +151,13 @@ class DeviceArray:
         out = np.empty(self.shape, dtype=self.dtype)
         self.ctx.check(self.ctx.lib.hm_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), "d2h")
         return out
+
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of other's bytes into this array (hm_memcpy_d2d)."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        self.ctx.check(self.ctx.lib.hm_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.nbytes), "d2d")
+        return self
 
     def free(self):
         if self.ptr:

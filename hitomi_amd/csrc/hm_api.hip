@@ -6,6 +6,7 @@
 
 #include "../../include/hitomi_amd.h"
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -18,6 +19,18 @@ size_t plan_bytes(const BoxPlanImpl* p);
 void plan_triple_mesh(const BoxPlanImpl* p, int m[3]);
 int cell_coords(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, long long np, long long* d_out);
 int fixed_to_density(hm_ctx* ctx, const long long* d_q, long long n, double inv_scale, double factor, double* d_out);
+}  // namespace hm
+
+namespace hm {
+// Plain device copy: 16 B per lane where both pointers allow it, bytes for the rest.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ s, uint4* __restrict__ d, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = s[i];
+}
+__global__ __launch_bounds__(256) void k_copy1(const unsigned char* __restrict__ s, unsigned char* __restrict__ d,
+                                               long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = s[i];
+}
+static unsigned copy_blocks(long long n) { return (unsigned)std::min<long long>((n + 255) / 256, 1 << 16); }
 }  // namespace hm
 
 struct hm_box_plan {
@@ -106,6 +119,23 @@ int hm_memcpy_h2d(hm_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
     HM_HIP_CHECK(ctx, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
+}
+
+int hm_memcpy_d2d(hm_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!ctx) return -1;
+    if (!bytes) return 0;
+    if (!d_dst || !d_src) return bad(ctx, "memcpy_d2d: NULL pointer");
+    const bool v = (((size_t)d_dst | (size_t)d_src) & 15) == 0;
+    const long long n16 = v ? (long long)(bytes / 16) : 0;
+    const long long tail = (long long)bytes - 16 * n16;
+    return ctx->launch("copy", [&] {
+        if (n16)
+            hipLaunchKernelGGL(hm::k_copy16, dim3(hm::copy_blocks(n16)), dim3(256), 0, ctx->stream, (const uint4*)d_src,
+                               (uint4*)d_dst, n16);
+        if (tail)
+            hipLaunchKernelGGL(hm::k_copy1, dim3(hm::copy_blocks(tail)), dim3(256), 0, ctx->stream,
+                               (const unsigned char*)d_src + 16 * n16, (unsigned char*)d_dst + 16 * n16, tail);
+    });
 }
 
 int hm_memcpy_d2h(hm_ctx* ctx, void* dst, const void* d_src, size_t bytes) {

@@ -54,6 +54,10 @@ void *hm_dev_alloc(hm_ctx *ctx, size_t bytes);
 int hm_dev_free(hm_ctx *ctx, void *d_ptr);
 int hm_memcpy_h2d(hm_ctx *ctx, void *d_dst, const void *src, size_t bytes);
 int hm_memcpy_d2h(hm_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+/* Device-to-device copy with the library's own copy kernel, ordered on the
+ * context's stream (no counterpart in the reference: plumbing, and the
+ * known-size kernel that scales HBM traffic counters). */
+int hm_memcpy_d2d(hm_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 int hm_memset_zero(hm_ctx *ctx, void *d_ptr, size_t bytes);
 
 /* Kernel timing with HIP events recorded on the launch stream around every

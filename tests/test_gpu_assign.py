@@ -86,3 +86,13 @@ def test_sorted_gather_equals_atomic_scatter_bitwise(ctx, name):
 def test_assign_empty_catalogue(ctx):
     re, im = ctx.assign(hm.make_mesh(16, 100.0, "TSC"), np.zeros((0, 3)))
     assert np.all(re == 0)
+
+
+@pytest.mark.parametrize("nbytes", [16, 4096 + 8, 3 * 2**20 + 5])
+def test_device_copy(ctx, nbytes):
+    rng = np.random.default_rng(nbytes)
+    src = rng.integers(0, 256, nbytes, dtype=np.uint8)
+    a = ctx.array(src)
+    b = ctx.empty((nbytes,), np.uint8)
+    b.copy_from(a)
+    assert np.array_equal(b.download(), src)
