@@ -49,6 +49,11 @@ class BoxConfig(ct.Structure):
         ("bk", ct.c_int),
         ("regrid", ct.c_int),
         ("prune", ct.c_int),
+        ("n_rbin", ct.c_int),
+        ("rmin", ct.c_double),
+        ("rmax", ct.c_double),
+        ("n_xi_ells", ct.c_int),
+        ("xi_ells", ct.c_int * MAX_ELLS),
     ]
 
 
@@ -61,6 +66,33 @@ class BoxResult(ct.Structure):
         ("bk_sum", ct.POINTER(ct.c_double)),
         ("sn_sum", ct.POINTER(ct.c_double)),
         ("shell_nmode", ct.POINTER(ct.c_longlong)),
+        ("xi", ct.POINTER(ct.c_double)),
+        ("xi_raw", ct.POINTER(ct.c_double)),
+        ("xi_nmode", ct.POINTER(ct.c_longlong)),
+    ]
+
+
+class SurveyConfig(ct.Structure):
+    _fields_ = [
+        ("mesh", Mesh),
+        ("precision", ct.c_int),
+        ("n_kbin", ct.c_int),
+        ("kmin", ct.c_double),
+        ("kmax", ct.c_double),
+        ("ell", ct.c_int),
+        ("prune", ct.c_int),
+    ]
+
+
+class SurveyInfo(ct.Structure):
+    _fields_ = [
+        ("wsum_d", ct.c_double),
+        ("wsum_r", ct.c_double),
+        ("alpha", ct.c_double),
+        ("vsurvey", ct.c_double),
+        ("nbar", ct.c_double),
+        ("shift", ct.c_double * 3),
+        ("norm", ct.c_double),
     ]
 
 
@@ -92,6 +124,11 @@ _SIGS = {
     "hm_box_plan_execute": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.POINTER(BoxResult)]),
     "hm_box_plan_bytes": (ct.c_size_t, [_VP]),
     "hm_box_plan_triple_mesh": (ct.c_int, [_VP, ct.POINTER(ct.c_int)]),
+    "hm_survey_plan_create": (_VP, [_VP, ct.POINTER(SurveyConfig)]),
+    "hm_survey_plan_destroy": (None, [_VP, _VP]),
+    "hm_survey_plan_execute": (ct.c_int, [_VP, _VP, _VP, _VP, ct.c_longlong, _VP, _VP, ct.c_longlong,
+                                          ct.POINTER(SurveyInfo), ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]),
+    "hm_survey_plan_bytes": (ct.c_size_t, [_VP]),
 }
 
 _lib = None
@@ -282,12 +319,14 @@ class Context:
 
 
 class BoxPlan:
-    """Periodic-box P_l + B_000 plan (hm_box_plan): calcPowerSpectrumForBOX
-    (powerspec.hpp:273-332) and calcBiSpectrumForBOX (bispec.hpp:877-1203) for
+    """Periodic-box plan (hm_box_plan): calcPowerSpectrumForBOX
+    (powerspec.hpp:273-332), calcTwoPointCorrelationFunctionForBOX
+    (powerspec.hpp:334-393) and calcBiSpectrumForBOX (bispec.hpp:877-1203) for
     every ith_kbin at once."""
 
     def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2,
-                 pk_ells=(0, 2, 4), bk=True, precision=FP32, regrid=True, prune=True):
+                 pk_ells=(0, 2, 4), bk=True, precision=FP32, regrid=True, prune=True, xi_ells=(), n_rbin=25,
+                 rmin=30.0, rmax=150.0):
         self.ctx = ctx
         cfg = BoxConfig()
         cfg.mesh = make_mesh(n_mesh, boxsize, assign)
@@ -301,7 +340,15 @@ class BoxPlan:
         cfg.bk = 1 if bk else 0
         cfg.regrid = 1 if regrid else 0
         cfg.prune = 1 if prune else 0
+        cfg.n_rbin = int(n_rbin)
+        cfg.rmin = float(rmin)
+        cfg.rmax = float(rmax)
+        cfg.n_xi_ells = len(xi_ells)
+        for i, l in enumerate(xi_ells):
+            cfg.xi_ells[i] = int(l)
         self.cfg = cfg
+        self.xi_ells = tuple(xi_ells)
+        self.n_rbin = int(n_rbin)
         self.n_kbin = int(n_kbin)
         self.pk_ells = tuple(pk_ells)
         self.h = ctx.lib.hm_box_plan_create(ctx.h, ct.byref(cfg))
@@ -343,6 +390,8 @@ class BoxPlan:
             "pk": np.zeros((ne, nk, 2)), "pk_raw": np.zeros((ne, nk, 2)), "pk_nmode": np.zeros(nk, np.int64),
             "bk": np.zeros((nk, nk, 2)), "bk_sum": np.zeros((nk, nk, 2)), "sn_sum": np.zeros((nk, nk, 2)),
             "shell_nmode": np.zeros(nk, np.int64),
+            "xi": np.zeros((len(self.xi_ells), self.n_rbin)), "xi_raw": np.zeros((len(self.xi_ells), self.n_rbin)),
+            "xi_nmode": np.zeros(self.n_rbin, np.int64),
         }
         r = BoxResult()
         dp = ct.POINTER(ct.c_double)
@@ -354,6 +403,70 @@ class BoxPlan:
         r.bk_sum = out["bk_sum"].ctypes.data_as(dp)
         r.sn_sum = out["sn_sum"].ctypes.data_as(dp)
         r.shell_nmode = out["shell_nmode"].ctypes.data_as(lp)
+        r.xi = out["xi"].ctypes.data_as(dp)
+        r.xi_raw = out["xi_raw"].ctypes.data_as(dp)
+        r.xi_nmode = out["xi_nmode"].ctypes.data_as(lp)
         self.ctx.check(self.ctx.lib.hm_box_plan_execute(self.ctx.h, self.h, dpos.ptr, int(npart), ct.byref(r)),
                        "hm_box_plan_execute")
         return out
+
+
+class SurveyPlan:
+    """Survey-geometry P_l plan (hm_survey_plan): main.cpp's preparation
+    (main.cpp:109-166) and calcPowerSpectrum (powerspec.hpp:4-91)."""
+
+    def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2, ell=0,
+                 precision=FP32, prune=True):
+        self.ctx = ctx
+        cfg = SurveyConfig()
+        cfg.mesh = make_mesh(n_mesh, boxsize, assign)
+        cfg.precision = precision
+        cfg.n_kbin = int(n_kbin)
+        cfg.kmin = float(kmin)
+        cfg.kmax = float(kmax)
+        cfg.ell = int(ell)
+        cfg.prune = 1 if prune else 0
+        self.cfg = cfg
+        self.n_kbin = int(n_kbin)
+        self.h = ctx.lib.hm_survey_plan_create(ctx.h, ct.byref(cfg))
+        if not self.h:
+            ctx.raise_error("hm_survey_plan_create")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hm_survey_plan_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self):
+        return self.ctx.lib.hm_survey_plan_bytes(self.h)
+
+    def execute(self, pos_d, w_d, pos_r, w_r):
+        """Positions (N, 3) float64 in the survey frame (unshifted) and weights (N,),
+        host arrays or DeviceArrays.  Returns (pk complex [n_kbin], nmode, info dict)."""
+        def dev(a, shape):
+            if isinstance(a, DeviceArray):
+                return a
+            return self.ctx.array(np.ascontiguousarray(a, dtype=np.float64).reshape(shape))
+
+        pd = dev(pos_d, (-1, 3))
+        pr = dev(pos_r, (-1, 3))
+        nd, nr = pd.shape[0], pr.shape[0]
+        wd = dev(w_d, (nd,))
+        wr = dev(w_r, (nr,))
+        pk = np.zeros((self.n_kbin, 2))
+        nm = np.zeros(self.n_kbin, np.int64)
+        info = SurveyInfo()
+        self.ctx.check(self.ctx.lib.hm_survey_plan_execute(
+            self.ctx.h, self.h, pd.ptr, wd.ptr, nd, pr.ptr, wr.ptr, nr, ct.byref(info),
+            pk.ctypes.data_as(ct.POINTER(ct.c_double)), nm.ctypes.data_as(ct.POINTER(ct.c_longlong))),
+            "hm_survey_plan_execute")
+        d = {k: getattr(info, k) for k in ("wsum_d", "wsum_r", "alpha", "vsurvey", "nbar", "norm")}
+        d["shift"] = np.array(info.shift[:])
+        return pk[:, 0] + 1j * pk[:, 1], nm, d

@@ -18,6 +18,12 @@ int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long
 size_t plan_bytes(const BoxPlanImpl* p);
 void plan_triple_mesh(const BoxPlanImpl* p, int m[3]);
 int cell_coords(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, long long np, long long* d_out);
+struct SurveyPlanImpl;
+SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg);
+void survey_destroy(SurveyPlanImpl* p);
+int survey_execute(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const double* wD, long long nD, const double* pR,
+                   const double* wR, long long nR, hm_survey_info* info, double* pk, long long* nmode);
+size_t survey_bytes(const SurveyPlanImpl* p);
 int fixed_to_density(hm_ctx* ctx, const long long* d_q, long long n, double inv_scale, double factor, double* d_out);
 }  // namespace hm
 
@@ -35,6 +41,9 @@ static unsigned copy_blocks(long long n) { return (unsigned)std::min<long long>(
 
 struct hm_box_plan {
     hm::BoxPlanImpl* impl;
+};
+struct hm_survey_plan {
+    hm::SurveyPlanImpl* impl;
 };
 
 static int bad(hm_ctx* ctx, const char* msg) {
@@ -318,5 +327,40 @@ int hm_box_plan_triple_mesh(const hm_box_plan* plan, int m[3]) {
     hm::plan_triple_mesh(plan->impl, m);
     return 0;
 }
+
+hm_survey_plan* hm_survey_plan_create(hm_ctx* ctx, const hm_survey_config* cfg) {
+    if (!ctx) return nullptr;
+    if (!cfg) return bad(ctx, "survey plan: config is NULL"), nullptr;
+    try {
+        hm::SurveyPlanImpl* impl = hm::survey_create(ctx, cfg);
+        if (!impl) return nullptr;
+        return new hm_survey_plan{impl};
+    } catch (const std::exception& e) {
+        ctx->set_error(std::string("survey plan: ") + e.what());
+        return nullptr;
+    }
+}
+
+void hm_survey_plan_destroy(hm_ctx* ctx, hm_survey_plan* plan) {
+    if (!plan) return;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    hm::survey_destroy(plan->impl);
+    delete plan;
+}
+
+int hm_survey_plan_execute(hm_ctx* ctx, hm_survey_plan* plan, const double* d_pos_d, const double* d_w_d, long long n_d,
+                           const double* d_pos_r, const double* d_w_r, long long n_r, hm_survey_info* info, double* pk,
+                           long long* nmode) {
+    if (!ctx) return -1;
+    if (!plan || !d_pos_d || !d_w_d || !d_pos_r || !d_w_r) return bad(ctx, "survey execute: NULL argument");
+    try {
+        return hm::survey_execute(ctx, plan->impl, d_pos_d, d_w_d, n_d, d_pos_r, d_w_r, n_r, info, pk, nmode);
+    } catch (const std::exception& e) {
+        ctx->set_error(std::string("survey execute: ") + e.what());
+        return -1;
+    }
+}
+
+size_t hm_survey_plan_bytes(const hm_survey_plan* plan) { return plan ? hm::survey_bytes(plan->impl) : 0; }
 
 }  // extern "C"

@@ -17,6 +17,7 @@
 //   6. all 210 triple sums sum_x F_a F_b F_3 in one pass (FP32 products,
 //      FP64 accumulation per 64 voxels, fixed-order block reduction).
 #include "hm_assign.hpp"
+#include "hm_bins.hpp"
 #include "hm_fft_launch.hpp"
 
 #include "../../include/hitomi_amd.h"
@@ -26,67 +27,6 @@
 #include <cstring>
 
 namespace hm {
-
-constexpr int FINE_N = 100000;      // field.hpp:775
-constexpr double FINE_DK = 0.0001;  // field.hpp:776
-
-// ---------------------------------------------------------------------------
-// per-axis FP64 tables of the N-mesh: k component (field.hpp:797-799),
-// assignment-window factor and shot-noise factor with the reference's index
-// truncation (field.hpp:700-716, 1293-1309).
-// ---------------------------------------------------------------------------
-struct AxisTabs {
-    const double* kv[3];
-    const double* wf[3];
-    const double* sf[3];
-};
-
-static void host_axis_tables(const Mesh& m, std::vector<double>& out, long long off[3]) {
-    const long long ntot = (long long)m.n[0] + m.n[1] + m.n[2];
-    out.assign(3 * ntot, 0.0);
-    long long o = 0;
-    for (int a = 0; a < 3; a++) {
-        off[a] = o;
-        const int n = m.n[a];
-        const double dk = 2.0 * M_PI / m.L[a];
-        for (int i = 0; i < n; i++) {
-            double kv = (i < n / 2) ? (double)i * dk : (double)(i - n) * dk;
-            int ii = (int)(kv / dk + 1.0e-5);
-            double x = M_PI * (double)ii / (double)n;
-            double f = (ii != 0) ? std::sin(x) / x : 1.0;
-            double s = (ii != 0) ? std::sin(x) : 0.0;
-            double sfac;
-            if (m.assign == 0) sfac = 1.0;
-            else if (m.assign == 1) sfac = 1.0 - (2.0 / 3.0) * s * s;
-            else sfac = 1.0 - s * s + 2.0 * std::pow(s, 4) / 15.0;
-            out[o + i] = kv;
-            out[ntot + o + i] = f;
-            out[2 * ntot + o + i] = sfac;
-        }
-        o += n;
-    }
-}
-
-__device__ __forceinline__ double wpow(double r, int p) { return p == 1 ? r : (p == 2 ? r * r : r * r * r); }
-
-// |k| exactly as field.hpp:800 evaluates it (no contraction)
-__device__ __forceinline__ double kmag3(double a, double b, double c) {
-#pragma clang fp contract(off)
-    return sqrt(a * a + b * b + c * c);
-}
-
-// Legendre P_l(mu) by the three-term recurrence (the m = 0 reduced
-// spherical harmonic of tools.hpp:12-68 is sphPlm(l,0,mu) sqrt(4pi/(2l+1))).
-__device__ __forceinline__ double legendre(int l, double mu) {
-    if (l == 0) return 1.0;
-    double p0 = 1.0, p1 = mu;
-    for (int k = 2; k <= l; k++) {
-        double p2 = ((2.0 * k - 1.0) * mu * p1 - (k - 1.0) * p0) / (double)k;
-        p0 = p1;
-        p1 = p2;
-    }
-    return p1;
-}
 
 // ---------------------------------------------------------------------------
 // P_l binning (calcPowerSpectrum, field.hpp:765-852) on the half spectrum
@@ -144,30 +84,6 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
             }
         }
     }
-}
-
-// Sum rows of a [nrows][ncols] array per column in a fixed order.
-template <typename V>
-__global__ __launch_bounds__(256) void k_reduce_cols(const V* __restrict__ part, long long nrows, int ncols,
-                                                    V* __restrict__ out) {
-    __shared__ V sm[256];
-    const int c = blockIdx.x;
-    V acc = 0;
-    for (long long r = threadIdx.x; r < nrows; r += 256) acc += part[r * ncols + c];
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[c] = sm[0];
-}
-
-template <typename V> static int reduce_cols(hm_ctx* ctx, const char* name, const V* part, long long nrows, int ncols, V* out) {
-    if (ncols <= 0) return 0;
-    return ctx->launch(name, [&] {
-        hipLaunchKernelGGL(k_reduce_cols<V>, dim3((unsigned)ncols), dim3(256), 0, ctx->stream, part, nrows, ncols, out);
-    });
 }
 
 // ---------------------------------------------------------------------------
@@ -254,6 +170,7 @@ template <typename T> struct XiSrc {
     AxisTabs t;
     int wp;
     double sn, inv_v;
+    double dc;  // added to Re d at k = 0 in Re d conj(n): N for the shot-noise field, 0 for xi
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
     __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
 #pragma clang fp contract(off)
@@ -261,7 +178,7 @@ template <typename T> struct XiSrc {
         const C<T> d = dk[addr];
         const double re = (double)d.x, im = (double)d.y;
         double p2;
-        if (i == 0 && j == 0 && k == 0) p2 = re * (re + sn) + im * im;
+        if (i == 0 && j == 0 && k == 0) p2 = re * (re + dc) + im * im;
         else p2 = re * re + im * im;
         const int ii = (n[0] - i) % n[0], jj = (n[1] - j) % n[1], kk = (n[2] - k) % n[2];
         const double S1 = (t.sf[0][i] * t.sf[1][j]) * t.sf[2][k];
@@ -273,6 +190,72 @@ template <typename T> struct XiSrc {
         return mk<T>((T)(0.5 * (g1 + g2)), (T)0);
     }
 };
+
+// Rows of a half spectrum restricted to |x|, |y| within a support, in the
+// compact order of support_map (the z c2r of a pruned inverse transform).
+template <typename T> struct SupHalfSrc {
+    const C<T>* p;
+    int nzh, n0, n1, s0, s1, c1;
+    __device__ inline C<T> operator()(long long r, int k) const {
+        const int x = support_map((int)(r / c1), s0, n0), y = support_map((int)(r % c1), s1, n1);
+        return p[((long long)x * n1 + y) * nzh + k];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// xi_l binning (calcCorrelationFunction, field.hpp:906-945) over the cube of
+// separations that can reach an r-bin: fine 0.5 Mpc/h index, coarse fold
+// (lower, upper], Y_l0(r) = P_l(r_z / |r|), FP64 sums per compact row.
+// ---------------------------------------------------------------------------
+constexpr int FINE_NR = 20000;     // field.hpp:906
+constexpr double FINE_DR = 0.5;    // field.hpp:907
+
+struct XiBinArgs {
+    int n[3], s[3], c[3];
+    double dr[3];
+    long long nrows;
+    const int2* f2b;
+    int nr, ne;
+    int ells[HM_MAX_ELLS];
+};
+
+__device__ __forceinline__ double mesh_vec(int i, int n, double d) {
+#pragma clang fp contract(off)
+    return (i < n / 2) ? (double)i * d : (double)(i - n) * d;  // field.hpp:929-931
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_xi(const T* __restrict__ xr, XiBinArgs a, double* __restrict__ part,
+                                               long long* __restrict__ cnt) {
+#pragma clang fp contract(off)
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.nrows) return;
+    const int i = support_map((int)(row / a.c[1]), a.s[0], a.n[0]);
+    const int j = support_map((int)(row % a.c[1]), a.s[1], a.n[1]);
+    const double rx = mesh_vec(i, a.n[0], a.dr[0]), ry = mesh_vec(j, a.n[1], a.dr[1]);
+    const T* line = xr + row * (long long)a.n[2];
+    double* P = part + row * (long long)a.nr * a.ne;
+    long long* Cn = cnt + row * (long long)a.nr;
+    for (int kc = 0; kc < a.c[2]; kc++) {
+        const int k = support_map(kc, a.s[2], a.n[2]);
+        const double rz = mesh_vec(k, a.n[2], a.dr[2]);
+        const double rmag = kmag3(rx, ry, rz);
+        const int ri = (int)(rmag / FINE_DR + 0.5);
+        if (ri >= FINE_NR) continue;
+        const int2 b = a.f2b[ri];
+        if (b.x < 0) continue;
+        const double v = (double)line[k];
+        const double mu = (fabs(rmag) < 1.0e-15) ? 0.0 : rz / rmag;
+        for (int bb = b.x; bb <= b.y; bb++) {
+            for (int e = 0; e < a.ne; e++) {
+                const int l = a.ells[e];
+                const double y = (l == 0) ? 1.0 : ((fabs(rmag) < 1.0e-15) ? 0.0 : legendre(l, mu));
+                P[bb * a.ne + e] += v * y;
+            }
+            Cn[bb] += 1;
+        }
+    }
+}
 
 // Shot-noise radial sums, step 1 (before the last, z, transform, which is
 // linear): G[a][kz] = sum of the half-spectrum rows (i, j, .) whose
@@ -682,6 +665,12 @@ struct BoxPlanImpl {
     int ngrp = 0, nblk = 0, nsl = 0;  // 4-shell groups, group pairs, z slices per pair block (fused)
     int gram_cols = 0;                // doubles per partial row of the triple sums
     DevBuf sid;                       // char2 first/last shell per mode of the shell support cube
+    // xi_l
+    int nr = 0, nxe = 0;
+    std::vector<double> rbin, rlower, rupper;
+    int rs[3] = {-1, -1, -1};  // support of the separations that reach an r-bin
+    long long xi_rows = 0;
+    DevBuf f2b_r_d, xi_r, xi_part, xi_cnt, xi_red, xi_cnt_red;
 };
 
 static int alloc(hm_ctx* ctx, BoxPlanImpl* p, DevBuf& b, size_t bytes) {
@@ -726,6 +715,15 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
         if (alloc(ctx, p, p->sid, sizeof(char2) * (size_t)ns)) return -1;
     }
     if (alloc(ctx, p, p->shell_cnt, sizeof(unsigned long long) * (size_t)p->nk)) return -1;
+    if (p->nxe > 0) {
+        if (!p->cfg.bk && alloc(ctx, p, p->xi_half, sizeof(C<T>) * (size_t)m.nh())) return -1;
+        const size_t cols = (size_t)p->nr * (size_t)p->nxe;
+        if (alloc(ctx, p, p->xi_r, sizeof(T) * (size_t)p->xi_rows * (size_t)m.n[2])) return -1;
+        if (alloc(ctx, p, p->xi_part, sizeof(double) * cols * (size_t)p->xi_rows)) return -1;
+        if (alloc(ctx, p, p->xi_cnt, sizeof(long long) * (size_t)p->nr * (size_t)p->xi_rows)) return -1;
+        if (alloc(ctx, p, p->xi_red, sizeof(double) * cols)) return -1;
+        if (alloc(ctx, p, p->xi_cnt_red, sizeof(long long) * (size_t)p->nr)) return -1;
+    }
     return 0;
 }
 
@@ -741,7 +739,8 @@ static AxisTabs axis_tabs(const BoxPlanImpl* p) {
     return t;
 }
 
-static int wpow_of(int assign) { return assign == 0 ? 1 : (assign == 1 ? 2 : 3); }
+
+void plan_destroy(BoxPlanImpl* p);
 
 BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     auto* p = new BoxPlanImpl();
@@ -754,7 +753,7 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     m.assign = cfg->mesh.assign;
     auto fail = [&](const std::string& s) -> BoxPlanImpl* {
         ctx->set_error(s);
-        delete p;
+        plan_destroy(p);
         return nullptr;
     };
     for (int a = 0; a < 3; a++) {
@@ -855,7 +854,7 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
         for (size_t r = 0; r < key.size(); r++) rows[(size_t)fillp[(size_t)a_idx[(size_t)key[r]]]++] = (int)r;
         if (alloc(ctx, p, p->a_start, sizeof(int) * a_start.size()) ||
             alloc(ctx, p, p->a_rows, sizeof(int) * rows.size()) || alloc(ctx, p, p->a_idx, sizeof(int) * a_idx.size())) {
-            delete p;
+            plan_destroy(p);
             return nullptr;
         }
         if (hipMemcpy(p->a_start.p, a_start.data(), sizeof(int) * a_start.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -863,13 +862,54 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
             hipMemcpy(p->a_idx.p, a_idx.data(), sizeof(int) * a_idx.size(), hipMemcpyHostToDevice) != hipSuccess)
             return fail("box plan: row table upload failed");
     }
+    // xi_l: r bins (tools.hpp:126-132), fine -> coarse fold (field.hpp:948-960)
+    if (cfg->n_xi_ells < 0 || cfg->n_xi_ells > HM_MAX_ELLS) return fail("box plan: n_xi_ells out of range");
+    p->nxe = cfg->n_xi_ells;
+    std::vector<int2> f2b_r;
+    if (p->nxe > 0) {
+        if (cfg->n_rbin < 2 || cfg->n_rbin > 256) return fail("box plan: n_rbin must be in [2, 256]");
+        if (!(cfg->rmax > cfg->rmin)) return fail("box plan: rmax must exceed rmin");
+        p->nr = cfg->n_rbin;
+        p->rbin.resize(p->nr);
+        const double drb = (cfg->rmax - cfg->rmin) / (double)(p->nr - 1);
+        for (int i = 0; i < p->nr; i++) p->rbin[i] = cfg->rmin + drb * (double)i;
+        const double dr_bin = p->rbin[1] - p->rbin[0];
+        p->rlower.resize(p->nr);
+        p->rupper.resize(p->nr);
+        double rumax = 0.0;
+        for (int j = 0; j < p->nr; j++) {
+            p->rlower[j] = (p->rbin[j] > dr_bin * 0.5) ? (p->rbin[j] - dr_bin * 0.5) : 0.0;
+            p->rupper[j] = p->rbin[j] + dr_bin * 0.5;
+            rumax = std::max(rumax, p->rupper[j]);
+        }
+        f2b_r.assign(FINE_NR, make_int2(-1, -2));
+        for (int i = 0; i < FINE_NR; i++) {
+            const double rt = (double)i * FINE_DR;
+            int lo = -1, hi = -2;
+            for (int j = 0; j < p->nr; j++)
+                if ((rt > p->rlower[j]) && (rt <= p->rupper[j])) {
+                    if (lo < 0) lo = j;
+                    else if (j != hi + 1) return fail("box plan: non-contiguous r-bin overlap");
+                    hi = j;
+                }
+            f2b_r[i] = make_int2(lo, hi);
+        }
+        // a separation reaches a bin only if its fine centre int(|r|/0.5 + 0.5)*0.5 <= rumax,
+        // i.e. |r| < rumax + 0.25, so every |r_a| is below that too
+        for (int a = 0; a < 3; a++) {
+            const double dr = m.L[a] / (double)m.n[a];
+            const long long s = (long long)std::floor((rumax + FINE_DR) / dr) + 1;
+            p->rs[a] = (!cfg->prune || 2 * s + 1 >= m.n[a] || s >= m.n[a] / 2) ? -1 : (int)s;
+        }
+        p->xi_rows = (long long)support_count(p->rs[0], m.n[0]) * support_count(p->rs[1], m.n[1]);
+    }
     // device tables
     std::vector<double> tabs;
     host_axis_tables(m, tabs, p->tab_off);
-    if (alloc(ctx, p, p->tab, sizeof(double) * tabs.size())) { delete p; return nullptr; }
-    if (alloc(ctx, p, p->f2b_d, sizeof(int2) * FINE_N)) { delete p; return nullptr; }
-    if (alloc(ctx, p, p->lu_d, sizeof(double) * 2 * p->nk)) { delete p; return nullptr; }
-    if (alloc(ctx, p, p->kb_d, sizeof(double) * p->nk)) { delete p; return nullptr; }
+    if (alloc(ctx, p, p->tab, sizeof(double) * tabs.size())) { plan_destroy(p); return nullptr; }
+    if (alloc(ctx, p, p->f2b_d, sizeof(int2) * FINE_N)) { plan_destroy(p); return nullptr; }
+    if (alloc(ctx, p, p->lu_d, sizeof(double) * 2 * p->nk)) { plan_destroy(p); return nullptr; }
+    if (alloc(ctx, p, p->kb_d, sizeof(double) * p->nk)) { plan_destroy(p); return nullptr; }
     std::vector<double> lu(p->lower);
     lu.insert(lu.end(), p->upper.begin(), p->upper.end());
     hipError_t e = hipSuccess;
@@ -879,7 +919,12 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     e = (hipError_t)(e | hipMemcpy(p->kb_d.p, p->kbin.data(), sizeof(double) * p->nk, hipMemcpyHostToDevice));
     if (e != hipSuccess) return fail("box plan: table upload failed");
     int rc = cfg->precision == HM_FP64 ? plan_alloc<double>(ctx, p) : plan_alloc<float>(ctx, p);
-    if (rc) { delete p; return nullptr; }
+    if (rc) { plan_destroy(p); return nullptr; }
+    if (p->nxe > 0) {
+        if (alloc(ctx, p, p->f2b_r_d, sizeof(int2) * FINE_NR)) { plan_destroy(p); return nullptr; }
+        if (hipMemcpy(p->f2b_r_d.p, f2b_r.data(), sizeof(int2) * FINE_NR, hipMemcpyHostToDevice) != hipSuccess)
+            return fail("box plan: r-bin table upload failed");
+    }
     if (cfg->bk) {
         if (hipMemcpy(p->pairs_d.p, p->pairs.data(), sizeof(short2) * p->pairs.size(), hipMemcpyHostToDevice) != hipSuccess)
             return fail("box plan: pair table upload failed");
@@ -889,7 +934,7 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
                 hipLaunchKernelGGL(k_j0_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
                                    (const double*)p->kb_d.p, p->nk, p->nkey, dr, (double*)p->jtab.p);
             }))
-        { delete p; return nullptr; }
+        { plan_destroy(p); return nullptr; }
     }
     // shell mode counts on the N mesh (data independent)
     {
@@ -905,12 +950,12 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
         ca.lower = (const double*)p->lu_d.p;
         ca.upper = ca.lower + p->nk;
         ca.nk = p->nk;
-        if (fill_zero(ctx, p->shell_cnt.p, sizeof(unsigned long long) * p->nk)) { delete p; return nullptr; }
+        if (fill_zero(ctx, p->shell_cnt.p, sizeof(unsigned long long) * p->nk)) { plan_destroy(p); return nullptr; }
         if (ctx->launch("shell_count", [&] {
                 hipLaunchKernelGGL(k_shell_count, dim3((unsigned)((ca.total + 255) / 256)), dim3(256), 0, ctx->stream, ca,
                                    (unsigned long long*)p->shell_cnt.p);
             }))
-        { delete p; return nullptr; }
+        { plan_destroy(p); return nullptr; }
         p->shell_nmode.resize(p->nk);
         if (hipMemcpyAsync(p->shell_nmode.data(), p->shell_cnt.p, sizeof(long long) * p->nk, hipMemcpyDeviceToHost,
                            ctx->stream) != hipSuccess ||
@@ -924,7 +969,8 @@ void plan_destroy(BoxPlanImpl* p) {
     if (!p) return;
     DevBuf* bufs[] = {&p->tab, &p->f2b_d, &p->lu_d, &p->kb_d, &p->fixed, &p->dk, &p->pk_part, &p->pk_cnt, &p->pk_red,
                       &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
-                      &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid};
+                      &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid,
+                      &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -1056,6 +1102,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (p->ne == 0) ba.ells[0] = 0;
     ba.sn = (double)np;
     const int ncols = p->nk * ba.ne;
+    const bool do_pk = p->ne > 0 || p->cfg.bk;
+    if (do_pk) {
     if (fill_zero(ctx, p->pk_part.p, sizeof(double) * (size_t)ncols * p->nrows)) return -1;
     if (fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows)) return -1;
     if (ctx->launch("bin_pk", [&] {
@@ -1068,6 +1116,58 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->pk_cnt.p, p->nrows, p->nk,
                                (long long*)p->pk_cnt_red.p))
         return -1;
+    }
+    // 3b. xi_l (calcCorrelationFunction, field.hpp:854-977): the even part of
+    // (|d|^2 - N S)/W^2/V inverse-transformed (its real part, which is all the
+    // reference writes, powerspec.hpp:385), then binned over the separations
+    // that reach an r-bin
+    if (p->nxe > 0) {
+        XiSrc<T> xs;
+        xs.dk = dk;
+        for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
+        xs.nzh = (int)nzh;
+        xs.t = tabs;
+        xs.wp = wp;
+        xs.sn = (double)np;
+        xs.inv_v = 1.0 / m.volume();
+        xs.dc = 0.0;
+        C<T>* xh = (C<T>*)p->xi_half.p;
+        const int cx = support_count(p->rs[0], m.n[0]), cy = support_count(p->rs[1], m.n[1]);
+        if (launch_cols<T, +1>(ctx, "xi2_pass_x", m.n[0], xs, xh,
+                               geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
+            return -1;
+        if (fft_cols<T>(ctx, "xi2_pass_y", m.n[1], +1, xh, xh,
+                        geom((long long)m.n[1] * nzh, nzh, cx, m.n[0], p->rs[0], (int)nzh, (int)nzh, -1, -1)))
+            return -1;
+        SupHalfSrc<T> hs{xh, (int)nzh, m.n[0], m.n[1], p->rs[0], p->rs[1], cy};
+        T* xr = (T*)p->xi_r.p;
+        if (launch_c2r<T>(ctx, "xi2_pass_z", m.n[2] / 2, hs, xr, p->xi_rows, -1)) return -1;
+        XiBinArgs xa;
+        for (int a = 0; a < 3; a++) {
+            xa.n[a] = m.n[a];
+            xa.s[a] = p->rs[a];
+            xa.c[a] = support_count(p->rs[a], m.n[a]);
+            xa.dr[a] = m.L[a] / (double)m.n[a];
+        }
+        xa.nrows = p->xi_rows;
+        xa.f2b = (const int2*)p->f2b_r_d.p;
+        xa.nr = p->nr;
+        xa.ne = p->nxe;
+        for (int e = 0; e < HM_MAX_ELLS; e++) xa.ells[e] = (e < p->nxe) ? p->cfg.xi_ells[e] : 0;
+        const int xcols = p->nr * p->nxe;
+        if (fill_zero(ctx, p->xi_part.p, sizeof(double) * (size_t)xcols * p->xi_rows)) return -1;
+        if (fill_zero(ctx, p->xi_cnt.p, sizeof(long long) * (size_t)p->nr * p->xi_rows)) return -1;
+        if (ctx->launch("bin_xi", [&] {
+                hipLaunchKernelGGL(k_bin_xi<T>, dim3((unsigned)((p->xi_rows + 255) / 256)), dim3(256), 0, ctx->stream, xr,
+                                   xa, (double*)p->xi_part.p, (long long*)p->xi_cnt.p);
+            }))
+            return -1;
+        if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->xi_part.p, p->xi_rows, xcols, (double*)p->xi_red.p))
+            return -1;
+        if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->xi_cnt.p, p->xi_rows, p->nr,
+                                   (long long*)p->xi_cnt_red.p))
+            return -1;
+    }
     const int npairs = (int)p->pairs.size();
     if (p->cfg.bk) {
         // 4. shot-noise ijk term (field.hpp:1116-1160, bispec.hpp:1023-1056)
@@ -1079,6 +1179,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.wp = wp;
         xs.sn = (double)np;
         xs.inv_v = 1.0 / m.volume();
+        xs.dc = (double)np;
         C<T>* xh = (C<T>*)p->xi_half.p;
         if (launch_cols<T, +1>(ctx, "xi_pass_x", m.n[0], xs, xh,
                                geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
@@ -1175,9 +1276,20 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     // 7. results to host
     std::vector<double> pk(ncols), gram(2 * (size_t)npairs), graw((size_t)p->gram_cols), snp((size_t)npairs);
     std::vector<long long> cnt(p->nk);
-    HM_HIP_CHECK(ctx, hipMemcpyAsync(pk.data(), p->pk_red.p, sizeof(double) * ncols, hipMemcpyDeviceToHost, ctx->stream));
-    HM_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), p->pk_cnt_red.p, sizeof(long long) * p->nk, hipMemcpyDeviceToHost,
-                                     ctx->stream));
+    std::vector<double> xsum((size_t)p->nr * p->nxe);
+    std::vector<long long> xcnt(p->nr);
+    if (do_pk) {
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(pk.data(), p->pk_red.p, sizeof(double) * ncols, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), p->pk_cnt_red.p, sizeof(long long) * p->nk,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (p->nxe > 0) {
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(xsum.data(), p->xi_red.p, sizeof(double) * xsum.size(), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(xcnt.data(), p->xi_cnt_red.p, sizeof(long long) * p->nr,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+    }
     if (p->cfg.bk) {
         HM_HIP_CHECK(ctx, hipMemcpyAsync(graw.data(), p->gram_red.p, sizeof(double) * p->gram_cols,
                                          hipMemcpyDeviceToHost, ctx->stream));
@@ -1210,6 +1322,15 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             for (int b = 0; b < nk; b++) res->pk_nmode[b] = cnt[b];
         if (res->shell_nmode)
             for (int b = 0; b < nk; b++) res->shell_nmode[b] = p->shell_nmode[b];
+        // xi_l: average (field.hpp:961-968), (2l+1) and V/N^2, real part (powerspec.hpp:361-386)
+        for (int e = 0; e < p->nxe; e++)
+            for (int b = 0; b < p->nr; b++) {
+                const double v = xcnt[b] ? xsum[(size_t)b * p->nxe + e] / (double)xcnt[b] : 0.0;
+                if (res->xi_raw) res->xi_raw[(size_t)e * p->nr + b] = v;
+                if (res->xi) res->xi[(size_t)e * p->nr + b] = (V / N / N) * (0.0 + (double)(2 * p->cfg.xi_ells[e] + 1) * v);
+            }
+        if (res->xi_nmode)
+            for (int b = 0; b < p->nr; b++) res->xi_nmode[b] = xcnt[b];
     }
     if (p->cfg.bk && res) {
         // P_raw of the monopole: calcPowerSpectrum(dn_00_sn, N_LM_sn, ..., 0, 0) (bispec.hpp:946, 953)

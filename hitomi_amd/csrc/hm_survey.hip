@@ -1,0 +1,658 @@
+// hm_survey.hip -- survey-geometry P_l on one MI355X (main.cpp:109-166 and
+// calcPowerSpectrum, powerspec.hpp:4-91).
+//
+// One execute takes a data and a random catalogue (unshifted Cartesian
+// positions and weights) to the normalised P_l(k) the reference writes:
+//   1. weight sums, alpha, the minimum of the randoms and the FFT-box shift;
+//   2. the survey volume from the squared random density;
+//   3. the Y_lm-weighted fluctuations D - alpha R for M = 0..l (the negative
+//      M follow from F_{-M}(x) = (-1)^M conj F_M(x), real weights) and the
+//      plain one for (0, 0), each assigned in 64-bit fixed point and
+//      transformed forward;
+//   4. the shot noise sums for M = 0..l;
+//   5. one pass over the modes that can reach a k-bin that forms, per mode,
+//      sum_M w_M (F_LM conj F_00 - SN_M S)/W^2 Y_{l,-M}(k) with the
+//      reference's 3j weights, fine 1e-4 histogram and coarse fold.
+#include "hm_assign.hpp"
+#include "hm_bins.hpp"
+#include "hm_fft_launch.hpp"
+
+#include "../../include/hitomi_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hm {
+
+constexpr int SV_MAX_ELL = 8;
+
+// ---------------------------------------------------------------------------
+// Wigner 3j symbol (gsl_sf_coupling_3j, common.hpp:29) by the Racah sum with
+// long double factorials; the arguments are small integers here.
+// ---------------------------------------------------------------------------
+static long double lfact(int n) {
+    long double r = 1.0L;
+    for (int i = 2; i <= n; i++) r *= (long double)i;
+    return r;
+}
+
+double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+    if (m1 + m2 + m3 != 0 || j1 < 0 || j2 < 0 || j3 < 0) return 0.0;
+    if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.0;
+    if (j3 < std::abs(j1 - j2) || j3 > j1 + j2) return 0.0;
+    const long double delta =
+        lfact(j1 + j2 - j3) * lfact(j1 - j2 + j3) * lfact(-j1 + j2 + j3) / lfact(j1 + j2 + j3 + 1);
+    const long double pre = std::sqrt(delta * lfact(j1 + m1) * lfact(j1 - m1) * lfact(j2 + m2) * lfact(j2 - m2) *
+                                      lfact(j3 + m3) * lfact(j3 - m3));
+    const int tlo = std::max({0, j2 - j3 - m1, j1 - j3 + m2});
+    const int thi = std::min({j1 + j2 - j3, j1 - m1, j2 + m2});
+    long double sum = 0.0L;
+    for (int t = tlo; t <= thi; t++) {
+        const long double den = lfact(t) * lfact(j3 - j2 + t + m1) * lfact(j3 - j1 + t - m2) * lfact(j1 + j2 - j3 - t) *
+                                lfact(j1 - t - m1) * lfact(j2 - t + m2);
+        sum += ((t & 1) ? -1.0L : 1.0L) / den;
+    }
+    const int ph = j1 - j2 - m3;
+    return (double)(((ph % 2) != 0 ? -1.0L : 1.0L) * pre * sum);
+}
+
+// ---------------------------------------------------------------------------
+// Reduced spherical harmonic of tools.hpp:12-68:
+//   Y(l, M)(v) = conj[(-1)^((M-|M|)/2) N_l^|M|(mu) e^{i M phi}] sqrt(4 pi/(2l+1))
+// with 1 for (0, 0), 0 for |v| < 1e-15, mu = v_z/|v|, phi = acos(v_x/|v_xy|)
+// (0 when |v_xy| < 1e-15), reflected to 2 pi - phi when v_y < 0.  N_l^m is
+// the orthonormal associated Legendre function with the Condon-Shortley
+// phase, from the stable normalised recurrence
+//   N_m^m = sqrt(1/4pi) prod_{i<=m} (-sqrt(1-x^2) sqrt((2i+1)/(2i))),
+//   N_l^m = a_lm (x N_{l-1}^m - N_{l-2}^m / a_{l-1,m}),  a_lm = sqrt((4l^2-1)/(l^2-m^2)).
+// ---------------------------------------------------------------------------
+__device__ inline double norm_plm(int l, int m, double x) {
+    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double ymm = 0.28209479177387814;  // 1 / sqrt(4 pi)
+    for (int i = 1; i <= m; i++) ymm *= -somx2 * sqrt((2.0 * i + 1.0) / (2.0 * i));
+    if (l == m) return ymm;
+    double y1 = x * sqrt(2.0 * m + 3.0) * ymm;  // N_{m+1}^m
+    double y0 = ymm;
+    double aprev = sqrt(2.0 * m + 3.0);
+    for (int ll = m + 2; ll <= l; ll++) {
+        const double a = sqrt((4.0 * ll * ll - 1.0) / ((double)ll * ll - (double)m * m));
+        const double y2 = a * (x * y1 - y0 / aprev);
+        y0 = y1;
+        y1 = y2;
+        aprev = a;
+    }
+    return y1;
+}
+
+struct Cplx {
+    double re, im;
+};
+
+__device__ inline Cplx ylm_red(int l, int M, double vx, double vy, double vz) {
+#pragma clang fp contract(off)
+    if (l == 0 && M == 0) return {1.0, 0.0};
+    const double r = sqrt(vx * vx + vy * vy + vz * vz);
+    const double rxy = sqrt(vx * vx + vy * vy);
+    if (fabs(r) < 1.0e-15) return {0.0, 0.0};
+    const double mu = vz / r;
+    double phi = (fabs(rxy) < 1.0e-15) ? 0.0 : acos(vx / rxy);
+    if (vy < 0.0) phi = -phi + 2.0 * M_PI;
+    const int S = M < 0 ? -M : M;
+    const double sgn = (M < 0 && (M & 1)) ? -1.0 : 1.0;
+    const double p = sgn * norm_plm(l, S, mu);
+    double sn, cs;
+    sincos((double)M * phi, &sn, &cs);
+    const double nrm = sqrt(4.0 * M_PI / (2.0 * (double)l + 1.0));
+    return {p * cs * nrm, -(p * sn) * nrm};
+}
+
+// ---------------------------------------------------------------------------
+// particle passes
+// ---------------------------------------------------------------------------
+// per-block partials: sum w_D, sum w_D^2, sum w_R, sum |w_D|, sum |w_R|; fixed order
+constexpr int SV_NSUM = 5;
+__global__ __launch_bounds__(256) void k_survey_wsums(const double* __restrict__ wD, long long nD,
+                                                     const double* __restrict__ wR, long long nR,
+                                                     double* __restrict__ part) {
+    __shared__ double sm[SV_NSUM][256];
+    double v[SV_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const long long st = (long long)gridDim.x * 256;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nD; q += st) {
+        v[0] += wD[q];
+        v[1] += pow(wD[q], 2);
+        v[3] += fabs(wD[q]);
+    }
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nR; q += st) {
+        v[2] += wR[q];
+        v[4] += fabs(wR[q]);
+    }
+    for (int k = 0; k < SV_NSUM; k++) sm[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < SV_NSUM; k++) sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < SV_NSUM) part[blockIdx.x * SV_NSUM + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// per-block minimum of the positions per axis (order independent)
+__global__ __launch_bounds__(256) void k_survey_minpos(const double* __restrict__ pos, long long n,
+                                                      double* __restrict__ part) {
+    __shared__ double sm[3][256];
+    double m[3] = {INFINITY, INFINITY, INFINITY};
+    const long long st = (long long)gridDim.x * 256;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += st)
+        for (int a = 0; a < 3; a++) m[a] = fmin(m[a], pos[3 * q + a]);
+    for (int a = 0; a < 3; a++) sm[a][threadIdx.x] = m[a];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int a = 0; a < 3; a++) sm[a][threadIdx.x] = fmin(sm[a][threadIdx.x], sm[a][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// Shifted positions of D then R (particle.hpp:204-218) in one array.
+__global__ __launch_bounds__(256) void k_survey_shift(const double* __restrict__ pD, long long nD,
+                                                     const double* __restrict__ pR, long long nR, double dx, double dy,
+                                                     double dz, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nD + nR) return;
+    const double* s = q < nD ? pD + 3 * q : pR + 3 * (q - nD);
+    out[3 * q] = s[0] - dx;
+    out[3 * q + 1] = s[1] - dy;
+    out[3 * q + 2] = s[2] - dz;
+}
+
+// LOS of an unshifted position (particle.hpp:355-375)
+__device__ inline void los_of(const double* p, double* l) {
+#pragma clang fp contract(off)
+    const double m = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    l[0] = p[0] / m;
+    l[1] = p[1] / m;
+    l[2] = p[2] / m;
+}
+
+// Complex weights of one field over D then R: w Y(l, M)(los) for D and
+// -alpha w Y(l, M)(los) for R (field.hpp:186-228; D - alpha R is formed by
+// the fixed-point sum).  mode 1: the random catalogue alone with (w, 0)
+// (field.hpp:476-485).
+__global__ __launch_bounds__(256) void k_survey_weights(const double* __restrict__ pD, const double* __restrict__ wD,
+                                                       long long nD, const double* __restrict__ pR,
+                                                       const double* __restrict__ wR, long long nR, double alpha, int l,
+                                                       int M, int mode, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (mode == 1) {
+        if (q < nR) {
+            out[2 * q] = wR[q];
+            out[2 * q + 1] = 0.0;
+        }
+        return;
+    }
+    if (q >= nD + nR) return;
+    const bool d = q < nD;
+    const double* p = d ? pD + 3 * q : pR + 3 * (q - nD);
+    double los[3];
+    los_of(p, los);
+    const Cplx y = ylm_red(l, M, los[0], los[1], los[2]);
+    const double w = d ? wD[q] : wR[q - nD];
+    double re = y.re * w, im = y.im * w;
+    if (!d) {
+        re = -(alpha * re);
+        im = -(alpha * im);
+    }
+    out[2 * q] = re;
+    out[2 * q + 1] = im;
+}
+
+// Shot-noise sums of field.hpp:1164-1188 for M = 0..l: per block
+// [M][sum_D re, sum_D im, sum_R re, sum_R im], fixed order.
+__global__ __launch_bounds__(256) void k_survey_sn(const double* __restrict__ pD, const double* __restrict__ wD,
+                                                  long long nD, const double* __restrict__ pR,
+                                                  const double* __restrict__ wR, long long nR, int l,
+                                                  double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double sm[256];
+    const int nm = l + 1;
+    double acc[4 * (SV_MAX_ELL + 1)];
+    for (int i = 0; i < 4 * nm; i++) acc[i] = 0.0;
+    const long long st = (long long)gridDim.x * 256;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nD + nR; q += st) {
+        const bool d = q < nD;
+        const double* p = d ? pD + 3 * q : pR + 3 * (q - nD);
+        const double w = d ? wD[q] : wR[q - nD];
+        const double w2 = pow(w, 2);
+        double los[3];
+        los_of(p, los);
+        for (int M = 0; M < nm; M++) {
+            const Cplx y = ylm_red(l, M, los[0], los[1], los[2]);
+            acc[4 * M + (d ? 0 : 2)] += w2 * y.re;
+            acc[4 * M + (d ? 1 : 3)] += w2 * y.im;
+        }
+    }
+    for (int i = 0; i < 4 * nm; i++) {
+        sm[threadIdx.x] = acc[i];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(long long)blockIdx.x * 4 * nm + i] = sm[0];
+        __syncthreads();
+    }
+}
+
+// sum over cells of rho^2 dV, rho = (q 2^-s)(1/dV) (field.hpp:487-492); per-block partials
+__global__ __launch_bounds__(256) void k_survey_sqsum(const long long* __restrict__ q, long long n, double inv_scale,
+                                                     double factor, double dV, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double sm[256];
+    double a = 0.0;
+    const long long st = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += st) {
+        const double r = ((double)q[i] * inv_scale) * factor;
+        a += r * r * dV;
+    }
+    sm[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// fixed point -> complex grid times dV (calcFourierTransform, field.hpp:313-318)
+template <typename T>
+__global__ __launch_bounds__(256) void k_survey_to_complex(const long long* __restrict__ re,
+                                                          const long long* __restrict__ im, long long n,
+                                                          double inv_scale, double factor, double dV,
+                                                          C<T>* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long st = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += st) {
+        const double a = (((double)re[i] * inv_scale) * factor) * dV;
+        const double b = im ? (((double)im[i] * inv_scale) * factor) * dV : 0.0;
+        out[i] = mk<T>((T)a, (T)b);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused binning over M (calcPowerSpectrum, field.hpp:765-852, summed with the
+// weights of powerspec.hpp:44-59)
+// ---------------------------------------------------------------------------
+struct SurveyBinArgs {
+    int n[3], s[3], c[3];
+    long long nrows, ng;
+    AxisTabs t;
+    int wp;
+    const int2* f2b;
+    int nk;
+    int l;
+    double w[2 * SV_MAX_ELL + 1];       // w_M, M = -l..l
+    double sn[2 * (SV_MAX_ELL + 1)];    // SN_M (re, im), M = 0..l
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_survey(const C<T>* __restrict__ f00, const C<T>* __restrict__ fl,
+                                                   SurveyBinArgs a, double* __restrict__ part,
+                                                   long long* __restrict__ cnt) {
+#pragma clang fp contract(off)
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.nrows) return;
+    const int i = support_map((int)(row / a.c[1]), a.s[0], a.n[0]);
+    const int j = support_map((int)(row % a.c[1]), a.s[1], a.n[1]);
+    const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
+    double* P = part + row * 2 * (long long)a.nk;
+    long long* Cn = cnt + row * (long long)a.nk;
+    for (int kc = 0; kc < a.c[2]; kc++) {
+        const int k = support_map(kc, a.s[2], a.n[2]);
+        const double kx = a.t.kv[0][i], ky = a.t.kv[1][j], kz = a.t.kv[2][k];
+        const double kmag = kmag3(kx, ky, kz);
+        const int kindex = (int)(kmag / FINE_DK + 0.5);
+        if (kindex >= FINE_N) continue;
+        const int2 b = a.f2b[kindex];
+        if (b.x < 0) continue;
+        const long long lin = ((long long)i * a.n[1] + j) * a.n[2] + k;
+        const long long mir = ((long long)im * a.n[1] + jm) * a.n[2] + (a.n[2] - k) % a.n[2];
+        const double S = (a.t.sf[0][i] * a.t.sf[1][j]) * a.t.sf[2][k];
+        const double W = wpow((a.t.wf[0][i] * a.t.wf[1][j]) * a.t.wf[2][k], a.wp);
+        const double W2 = pow(W, 2);
+        const C<T> g = f00[lin];
+        const double gr = (double)g.x, gi = -(double)g.y;  // conj F_00
+        double accr = 0.0, acci = 0.0;
+        for (int M = -a.l; M <= a.l; M++) {
+            const double wM = a.w[M + a.l];
+            if (wM == 0.0) continue;
+            const int am = M < 0 ? -M : M;
+            const double sg = (M < 0 && (am & 1)) ? -1.0 : 1.0;
+            double fr, fi, sr, si;
+            if (M >= 0) {
+                const C<T> f = fl[(long long)am * a.ng + lin];
+                fr = (double)f.x;
+                fi = (double)f.y;
+                sr = a.sn[2 * am];
+                si = a.sn[2 * am + 1];
+            } else {
+                const C<T> f = fl[(long long)am * a.ng + mir];
+                fr = sg * (double)f.x;
+                fi = -(sg * (double)f.y);
+                sr = sg * a.sn[2 * am];
+                si = -(sg * a.sn[2 * am + 1]);
+            }
+            // (F conj F00 - SN S) / W^2
+            double pr = fr * gr - fi * gi, pi = fr * gi + fi * gr;
+            pr -= sr * S;
+            pi -= si * S;
+            pr /= W2;
+            pi /= W2;
+            const Cplx y = ylm_red(a.l, -M, kx, ky, kz);
+            const double vr = pr * y.re - pi * y.im, vi = pr * y.im + pi * y.re;
+            accr += wM * vr;
+            acci += wM * vi;
+        }
+        for (int bb = b.x; bb <= b.y; bb++) {
+            P[2 * bb] += accr;
+            P[2 * bb + 1] += acci;
+            Cn[bb] += 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct SurveyPlanImpl {
+    hm_survey_config cfg;
+    Mesh mesh;
+    int nk = 0, l = 0, nf = 0;  // fields: index 0 = (0, 0), 1 + M = (l, M)
+    std::vector<double> kbin, lower, upper, wM;
+    int s[3];
+    long long nrows = 0;
+    long long tab_off[3];
+    SortWork sort;
+    DevBuf tab, f2b, pos, wts, fre, fim, fields, part, cnt, red, cnt_red, scratch;
+    size_t bytes = 0;
+};
+
+static int sv_alloc(hm_ctx* ctx, SurveyPlanImpl* p, DevBuf& b, size_t bytes) {
+    const size_t before = b.bytes;
+    if (dev_alloc(ctx, b, bytes)) return -1;
+    p->bytes += b.bytes - before;
+    return 0;
+}
+
+void survey_destroy(SurveyPlanImpl* p) {
+    if (!p) return;
+    DevBuf* bufs[] = {&p->tab, &p->f2b, &p->pos, &p->wts, &p->fre, &p->fim, &p->fields,
+                      &p->part, &p->cnt, &p->red, &p->cnt_red, &p->scratch};
+    for (DevBuf* b : bufs) dev_free(*b);
+    sort_work_free(p->sort);
+    delete p;
+}
+
+SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
+    auto* p = new SurveyPlanImpl();
+    p->cfg = *cfg;
+    Mesh& m = p->mesh;
+    auto fail = [&](const std::string& s) -> SurveyPlanImpl* {
+        ctx->set_error(s);
+        survey_destroy(p);
+        return nullptr;
+    };
+    for (int a = 0; a < 3; a++) {
+        m.n[a] = cfg->mesh.n_mesh[a];
+        m.L[a] = cfg->mesh.boxsize[a];
+        if (!fft_length_supported(m.n[a])) return fail("survey plan: unsupported n_mesh " + std::to_string(m.n[a]));
+        if (!(m.L[a] > 0)) return fail("survey plan: boxsize must be > 0");
+    }
+    m.assign = cfg->mesh.assign;
+    if (m.assign < 0 || m.assign > 2) return fail("survey plan: assign must be NGP, CIC or TSC");
+    if (cfg->precision != HM_FP32 && cfg->precision != HM_FP64) return fail("survey plan: precision must be FP32 or FP64");
+    if (cfg->n_kbin < 2 || cfg->n_kbin > 256) return fail("survey plan: n_kbin must be in [2, 256]");
+    if (cfg->ell < 0 || cfg->ell > SV_MAX_ELL) return fail("survey plan: ell must be in [0, 8]");
+    p->nk = cfg->n_kbin;
+    p->l = cfg->ell;
+    p->nf = p->l == 0 ? 1 : p->l + 2;
+    bin_edges(cfg->kmin, cfg->kmax, p->nk, p->kbin, p->lower, p->upper);
+    std::vector<int2> f2b;
+    const std::string e = fine_to_coarse(p->lower, p->upper, FINE_N, FINE_DK, f2b);
+    if (!e.empty()) return fail("survey plan: " + e);
+    // weights of powerspec.hpp:44-52 with ell1 = ELL = l, ell2 = 0: only m1 = -M survives
+    p->wM.assign(2 * p->l + 1, 0.0);
+    for (int M = -p->l; M <= p->l; M++)
+        for (int m1 = -p->l; m1 <= p->l; m1++) {
+            double w = wigner_3j(p->l, 0, p->l, 0, 0, 0) * wigner_3j(p->l, 0, p->l, m1, 0, M);
+            w *= (double)(2 * p->l + 1) * (double)(2 * p->l + 1);
+            if (fabs(w) < 1.0e-10) continue;
+            if (m1 != -M) return fail("survey plan: unexpected 3j selection");
+            p->wM[M + p->l] = w;
+        }
+    double umax = 0;
+    for (double u : p->upper) umax = std::max(umax, u);
+    for (int a = 0; a < 3; a++) {
+        const double dk = 2.0 * M_PI / m.L[a];
+        const long long s = (long long)std::floor((umax + FINE_DK) / dk) + 1;
+        p->s[a] = (!cfg->prune || 2 * s + 1 >= m.n[a] || s >= m.n[a] / 2) ? -1 : (int)s;
+    }
+    p->nrows = (long long)support_count(p->s[0], m.n[0]) * support_count(p->s[1], m.n[1]);
+    std::vector<double> tabs;
+    host_axis_tables(m, tabs, p->tab_off);
+    const size_t elem = cfg->precision == HM_FP64 ? 16 : 8;
+    const long long ng = m.ng();
+    if (sv_alloc(ctx, p, p->tab, sizeof(double) * tabs.size()) || sv_alloc(ctx, p, p->f2b, sizeof(int2) * FINE_N) ||
+        sv_alloc(ctx, p, p->fre, sizeof(long long) * (size_t)ng) ||
+        sv_alloc(ctx, p, p->fim, sizeof(long long) * (size_t)ng) ||
+        sv_alloc(ctx, p, p->fields, elem * (size_t)ng * (size_t)p->nf) ||
+        sv_alloc(ctx, p, p->part, sizeof(double) * 2 * (size_t)p->nk * (size_t)p->nrows) ||
+        sv_alloc(ctx, p, p->cnt, sizeof(long long) * (size_t)p->nk * (size_t)p->nrows) ||
+        sv_alloc(ctx, p, p->red, sizeof(double) * 2 * (size_t)p->nk) ||
+        sv_alloc(ctx, p, p->cnt_red, sizeof(long long) * (size_t)p->nk))
+        return fail(ctx->err);
+    if (hipMemcpy(p->tab.p, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->f2b.p, f2b.data(), sizeof(int2) * FINE_N, hipMemcpyHostToDevice) != hipSuccess)
+        return fail("survey plan: table upload failed");
+    return p;
+}
+
+size_t survey_bytes(const SurveyPlanImpl* p) { return p ? p->bytes + p->sort.bytes() : 0; }
+
+static unsigned red_blocks(long long n) { return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 1024)); }
+
+template <typename T>
+static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const double* wD, long long nD,
+                      const double* pR, const double* wR, long long nR, hm_survey_info* info, double* pk_out,
+                      long long* nmode_out) {
+    const Mesh& m = p->mesh;
+    const long long N = nD + nR, ng = m.ng();
+    const double dV = m.volume() / (double)ng;
+    if (nD <= 0 || nR <= 0) {
+        ctx->set_error("survey execute: data and randoms must be non-empty");
+        return -1;
+    }
+    const unsigned rb = red_blocks(std::max(nD, nR));
+    const unsigned rbg = red_blocks(ng);
+    const int nsn = 4 * (p->l + 1);
+    const size_t scratch = sizeof(double) * std::max<size_t>({(size_t)rb * SV_NSUM, (size_t)rb * 3, (size_t)rb * nsn,
+                                                             (size_t)rbg, 64});
+    if (sv_alloc(ctx, p, p->scratch, scratch + sizeof(double) * 64) || sv_alloc(ctx, p, p->pos, sizeof(double) * 3 * (size_t)N) ||
+        sv_alloc(ctx, p, p->wts, sizeof(double) * 2 * (size_t)N))
+        return -1;
+    double* part = (double*)p->scratch.p;
+    double* red = part + scratch / sizeof(double);
+    // 1. weight sums (particle.hpp:334-353, 284-299) and the randoms' minimum (particle.hpp:160-200)
+    std::vector<double> hp((size_t)rb * 3);
+    if (ctx->launch("survey_wsums", [&] {
+            hipLaunchKernelGGL(k_survey_wsums, dim3(rb), dim3(256), 0, ctx->stream, wD, nD, wR, nR, part);
+        }))
+        return -1;
+    if (reduce_cols<double>(ctx, "survey_reduce", part, rb, SV_NSUM, red)) return -1;
+    double sums[SV_NSUM], mn[3];
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(sums, red, sizeof(sums), hipMemcpyDeviceToHost, ctx->stream));
+    if (ctx->launch("survey_minpos", [&] {
+            hipLaunchKernelGGL(k_survey_minpos, dim3(rb), dim3(256), 0, ctx->stream, pR, nR, part);
+        }))
+        return -1;
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * rb, hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int a = 0; a < 3; a++) {
+        mn[a] = hp[a];
+        for (unsigned b = 1; b < rb; b++) mn[a] = std::min(mn[a], hp[3 * b + a]);
+    }
+    const double swD = sums[0], sw2D = sums[1], swR = sums[2], awD = sums[3], awR = sums[4];
+    if (swD == 0.0 || swR == 0.0) {
+        ctx->set_error("survey execute: zero total weight (particle.hpp:274-276)");
+        return -1;
+    }
+    const double alpha = swD / swR;  // particle.hpp:278
+    double dP[3];
+    for (int a = 0; a < 3; a++) dP[a] = mn[a] - 1.0 * m.L[a] / (double)m.n[a];  // particle.hpp:236-239, factor 1
+    double* pos = (double*)p->pos.p;
+    double* wts = (double*)p->wts.p;
+    const unsigned pb = (unsigned)((N + 255) / 256);
+    if (ctx->launch("survey_shift", [&] {
+            hipLaunchKernelGGL(k_survey_shift, dim3(pb), dim3(256), 0, ctx->stream, pD, nD, pR, nR, dP[0], dP[1], dP[2],
+                               pos);
+        }))
+        return -1;
+    AssignGeom ag;
+    for (int a = 0; a < 3; a++) {
+        ag.n[a] = m.n[a];
+        ag.L[a] = m.L[a];
+    }
+    ag.ng = ng;
+    ag.assign = m.assign;
+    {
+        const size_t before = p->sort.bytes();
+        if (sort_work_alloc(ctx, p->sort, ag, N, true)) return -1;
+        p->bytes += p->sort.bytes() - before;
+    }
+    long long* fre = (long long*)p->fre.p;
+    long long* fim = (long long*)p->fim.p;
+    // 2. survey volume (field.hpp:449-503)
+    if (ctx->launch("survey_weights", [&] {
+            hipLaunchKernelGGL(k_survey_weights, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, ctx->stream, pD, wD,
+                               nD, pR, wR, nR, alpha, 0, 0, 1, wts + 2 * nD);
+        }))
+        return -1;
+    const double sv_scale = fixed_scale_for(awR);
+    if (assign_sorted(ctx, ag, pos + 3 * nD, wts + 2 * nD, nR, sv_scale, fre, nullptr, p->sort)) return -1;
+    if (ctx->launch("survey_sqsum", [&] {
+            hipLaunchKernelGGL(k_survey_sqsum, dim3(rbg), dim3(256), 0, ctx->stream, fre, ng, 1.0 / sv_scale, 1.0 / dV,
+                               dV, part);
+        }))
+        return -1;
+    if (reduce_cols<double>(ctx, "survey_reduce", part, rbg, 1, red)) return -1;
+    double isq = 0.0;
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(&isq, red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const double vs = swR * swR / isq;      // field.hpp:499
+    const double nbar = swD * swD / vs / sw2D;  // particle.hpp:296
+    // 3. Y_lm-weighted fluctuations and their transforms.  |Y_red| <= 1, so every
+    // particle's |re| + |im| is at most sqrt(2) |w|: the bound fixes the fixed-point scale.
+    const double wabs = 1.5 * (awD + std::fabs(alpha) * awR);
+    C<T>* fields = (C<T>*)p->fields.p;
+    const int n3[3] = {m.n[0], m.n[1], m.n[2]};
+    for (int f = 0; f < p->nf; f++) {
+        const int l = f == 0 ? 0 : p->l, M = f == 0 ? 0 : f - 1;
+        if (ctx->launch("survey_weights", [&] {
+                hipLaunchKernelGGL(k_survey_weights, dim3(pb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR, alpha,
+                                   l, M, 0, wts);
+            }))
+            return -1;
+        const double scale = fixed_scale_for(wabs);
+        const bool cplx = M != 0;
+        if (assign_sorted(ctx, ag, pos, wts, N, scale, fre, cplx ? fim : nullptr, p->sort)) return -1;
+        C<T>* out = fields + (long long)f * ng;
+        if (ctx->launch("survey_to_complex", [&] {
+                hipLaunchKernelGGL(k_survey_to_complex<T>, dim3(rbg), dim3(256), 0, ctx->stream, fre,
+                                   cplx ? fim : nullptr, ng, 1.0 / scale, 1.0 / dV, dV, out);
+            }))
+            return -1;
+        if (fft3d_c2c<T>(ctx, out, n3, -1)) return -1;
+    }
+    // 4. shot noise sums for M = 0..l
+    if (ctx->launch("survey_sn", [&] {
+            hipLaunchKernelGGL(k_survey_sn, dim3(rb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR, p->l, part);
+        }))
+        return -1;
+    if (reduce_cols<double>(ctx, "survey_reduce", part, rb, nsn, red)) return -1;
+    std::vector<double> snr(nsn);
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(snr.data(), red, sizeof(double) * nsn, hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    // 5. binning
+    SurveyBinArgs a;
+    for (int ax = 0; ax < 3; ax++) {
+        a.n[ax] = m.n[ax];
+        a.s[ax] = p->s[ax];
+        a.c[ax] = support_count(p->s[ax], m.n[ax]);
+    }
+    a.nrows = p->nrows;
+    a.ng = ng;
+    const double* base = (const double*)p->tab.p;
+    const long long ntot = (long long)m.n[0] + m.n[1] + m.n[2];
+    for (int ax = 0; ax < 3; ax++) {
+        a.t.kv[ax] = base + p->tab_off[ax];
+        a.t.wf[ax] = base + ntot + p->tab_off[ax];
+        a.t.sf[ax] = base + 2 * ntot + p->tab_off[ax];
+    }
+    a.wp = wpow_of(m.assign);
+    a.f2b = (const int2*)p->f2b.p;
+    a.nk = p->nk;
+    a.l = p->l;
+    for (int i = 0; i < 2 * SV_MAX_ELL + 1; i++) a.w[i] = i < 2 * p->l + 1 ? p->wM[i] : 0.0;
+    for (int M = 0; M <= p->l; M++) {
+        // field.hpp:1187: sum_D + alpha^2 sum_R
+        a.sn[2 * M] = snr[4 * M] + pow(alpha, 2) * snr[4 * M + 2];
+        a.sn[2 * M + 1] = snr[4 * M + 1] + pow(alpha, 2) * snr[4 * M + 3];
+    }
+    if (fill_zero(ctx, p->part.p, sizeof(double) * 2 * (size_t)p->nk * p->nrows)) return -1;
+    if (fill_zero(ctx, p->cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows)) return -1;
+    const C<T>* fl = p->l == 0 ? fields : fields + ng;
+    if (ctx->launch("bin_survey", [&] {
+            hipLaunchKernelGGL(k_bin_survey<T>, dim3((unsigned)((p->nrows + 255) / 256)), dim3(256), 0, ctx->stream,
+                               fields, fl, a, (double*)p->part.p, (long long*)p->cnt.p);
+        }))
+        return -1;
+    if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->part.p, p->nrows, 2 * p->nk, (double*)p->red.p))
+        return -1;
+    if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->cnt.p, p->nrows, p->nk, (long long*)p->cnt_red.p))
+        return -1;
+    std::vector<double> pk(2 * (size_t)p->nk);
+    std::vector<long long> cnt(p->nk);
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(pk.data(), p->red.p, sizeof(double) * pk.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), p->cnt_red.p, sizeof(long long) * p->nk, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const double norm = vs / swD / swD;  // particle.hpp:302-313
+    if (pk_out)
+        for (int b = 0; b < p->nk; b++) {
+            pk_out[2 * b] = cnt[b] ? norm * (pk[2 * b] / (double)cnt[b]) : 0.0;
+            pk_out[2 * b + 1] = cnt[b] ? norm * (pk[2 * b + 1] / (double)cnt[b]) : 0.0;
+        }
+    if (nmode_out)
+        for (int b = 0; b < p->nk; b++) nmode_out[b] = cnt[b];
+    if (info) {
+        info->wsum_d = swD;
+        info->wsum_r = swR;
+        info->alpha = alpha;
+        info->vsurvey = vs;
+        info->nbar = nbar;
+        info->norm = norm;
+        for (int ax = 0; ax < 3; ax++) info->shift[ax] = dP[ax];
+    }
+    return 0;
+}
+
+int survey_execute(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const double* wD, long long nD, const double* pR,
+                   const double* wR, long long nR, hm_survey_info* info, double* pk, long long* nmode) {
+    return p->cfg.precision == HM_FP64 ? survey_run<double>(ctx, p, pD, wD, nD, pR, wR, nR, info, pk, nmode)
+                                       : survey_run<float>(ctx, p, pD, wD, nD, pR, wR, nR, info, pk, nmode);
+}
+
+}  // namespace hm

@@ -6,9 +6,10 @@
 // bk%d%d%d_%02d.dat files, and exits with status 1 where the reference calls
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
-// Supported now: measure = pk (any ELL, ell1 = ELL, ell2 = 0) and measure = bk
-// for (ell1, ell2, ELL) = (0, 0, 0), flag_recon = False.  Other measurements
-// are rejected with an error instead of being computed differently.
+// Supported now: measure = pk and 2PCF (any ELL, ell1 = ELL, ell2 = 0) and
+// measure = bk for (ell1, ell2, ELL) = (0, 0, 0), flag_recon = False.  Other
+// measurements are rejected with an error instead of being computed
+// differently.
 #include "../../include/hitomi_amd.h"
 #include "hm_catalog.hpp"
 #include "hm_param.hpp"
@@ -40,8 +41,11 @@ struct Gpu {
     }
 };
 
-static int run_box(const Params& p, const Catalog& P_D, Gpu& g, const double* kbin, std::vector<double>& pk,
-                   std::vector<double>& bk, bool want_bk, int ell) {
+enum Measure { M_PK, M_XI, M_BK };
+
+static int run_box(const Params& p, const Catalog& P_D, Gpu& g, Measure what, int ell, std::vector<double>& pk,
+                   std::vector<double>& bk, std::vector<double>& xi) {
+    const bool want_bk = what == M_BK;
     hm_box_config cfg;
     std::memset(&cfg, 0, sizeof(cfg));
     for (int a = 0; a < 3; a++) {
@@ -53,12 +57,16 @@ static int run_box(const Params& p, const Catalog& P_D, Gpu& g, const double* kb
     cfg.n_kbin = p.n_kbin;
     cfg.kmin = p.kmin;
     cfg.kmax = p.kmax;
-    cfg.n_pk_ells = 1;
+    cfg.n_pk_ells = what == M_XI ? 0 : 1;
     cfg.pk_ells[0] = want_bk ? 0 : ell;
     cfg.bk = want_bk ? 1 : 0;
     cfg.regrid = 1;
     cfg.prune = 1;
-    (void)kbin;
+    cfg.n_rbin = p.n_rbin;
+    cfg.rmin = p.rmin;
+    cfg.rmax = p.rmax;
+    cfg.n_xi_ells = what == M_XI ? 1 : 0;
+    cfg.xi_ells[0] = ell;
     hm_box_plan* plan = hm_box_plan_create(g.ctx, &cfg);
     if (!plan) {
         printf("hm_box_plan_create: %s\n", hm_last_error(g.ctx));
@@ -67,10 +75,12 @@ static int run_box(const Params& p, const Catalog& P_D, Gpu& g, const double* kb
     const int nk = p.n_kbin;
     pk.assign(2 * (size_t)nk, 0.0);
     bk.assign(2 * (size_t)nk * nk, 0.0);
+    xi.assign((size_t)p.n_rbin, 0.0);
     hm_box_result res;
     std::memset(&res, 0, sizeof(res));
-    res.pk = pk.data();
+    if (what == M_PK) res.pk = pk.data();
     if (want_bk) res.bk = bk.data();
+    if (what == M_XI) res.xi = xi.data();
     int rc = hm_box_plan_execute(g.ctx, plan, g.d_pos, P_D.n_tot, &res);
     if (rc) printf("hm_box_plan_execute: %s\n", hm_last_error(g.ctx));
     hm_box_plan_destroy(g.ctx, plan);
@@ -149,7 +159,7 @@ int main(int argc, char* argv[]) {
         EXIT();
     }
     const int nk = p.n_kbin;
-    std::vector<double> pk, bk;
+    std::vector<double> pk, bk, xi;
     if (p.measure == "pk") {
         // powerspec.hpp:280-284
         if (!((p.ELL == p.ell1) && (p.ell2 == 0))) {
@@ -158,7 +168,7 @@ int main(int argc, char* argv[]) {
             printf("'calcPowerSpectrumForBOX(P_D, param, kbin)' fails.\n");
             EXIT();
         }
-        if (run_box(p, P_D, g, kbin.data(), pk, bk, false, p.ELL)) {
+        if (run_box(p, P_D, g, M_PK, p.ELL, pk, bk, xi)) {
             printf("'calcPowerSpectrumForBOX(P_D, param, kbin)' fails.\n");
             EXIT();
         }
@@ -170,6 +180,26 @@ int main(int argc, char* argv[]) {
         fprintf(fp, "## k [h/Mpc]\t pk_real [(Mpc/h)^3]\t pk_imag [(Mpc/h)^3]\n");
         for (int i = 0; i < nk; i++) fprintf(fp, "%.5f \t %.7e \t %.7e\n", kbin[i], pk[2 * i], pk[2 * i + 1]);
         fclose(fp);
+    } else if (p.measure == "2PCF") {
+        // powerspec.hpp:341-345
+        if (!((p.ELL == p.ell1) && (p.ell2 == 0))) {
+            printf("This multipole combination is not allowed.\n");
+            printf("It should be ELL == ell1 and ell2 == 0\n");
+            printf("'calcTwoPointCorrelationFunctionForBOX(P_D, param, rbin)' fails.\n");
+            EXIT();
+        }
+        if (run_box(p, P_D, g, M_XI, p.ELL, pk, bk, xi)) {
+            printf("'calcTwoPointCorrelationFunctionForBOX(P_D, param, rbin)' fails.\n");
+            EXIT();
+        }
+        // powerspec.hpp:380-386
+        char buf[2048];
+        snprintf(buf, sizeof(buf), "%s/xi%d.dat", p.output_dir.c_str(), p.ELL);
+        FILE* fp = fopen(buf, "w");
+        if (!fp) EXIT();
+        fprintf(fp, "## r [Mpc/h] \t xi\n");
+        for (int i = 0; i < p.n_rbin; i++) fprintf(fp, "%.5f \t %.7e\n", rbin[i], xi[i]);
+        fclose(fp);
     } else if (p.measure == "bk") {
         if (!(p.ell1 == 0 && p.ell2 == 0 && p.ELL == 0)) {
             printf("measure = bk with (ell1, ell2, ELL) = (%d, %d, %d) is not available in this build; only (0, 0, 0).\n",
@@ -177,7 +207,7 @@ int main(int argc, char* argv[]) {
             printf("'calcBiSpectrumForBOX(P_D, param, kbin)' fails.\n");
             EXIT();
         }
-        if (run_box(p, P_D, g, kbin.data(), pk, bk, true, 0)) {
+        if (run_box(p, P_D, g, M_BK, 0, pk, bk, xi)) {
             printf("'calcBiSpectrumForBOX(P_D, param, kbin)' fails.\n");
             EXIT();
         }

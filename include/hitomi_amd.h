@@ -117,6 +117,10 @@ typedef struct {
     int bk;                    /* 1: B_000 for every (ith_kbin, k2) pair (calcBiSpectrumForBOX, bispec.hpp:877-1203) */
     int regrid;                /* 1: evaluate the triple product on the smallest mesh that holds it exactly */
     int prune;                 /* 1: skip the zero lines of band-limited transforms */
+    int n_rbin;                /* r-bins of ToolClass::setRbin (tools.hpp:126-132) */
+    double rmin, rmax;
+    int n_xi_ells;             /* xi_l for l in xi_ells (calcTwoPointCorrelationFunctionForBOX, powerspec.hpp:334-393) */
+    int xi_ells[HM_MAX_ELLS];
 } hm_box_config;
 
 /* Host result arrays (any may be NULL). */
@@ -128,16 +132,54 @@ typedef struct {
     double *bk_sum;        /* [n_kbin][n_kbin][2] bk_save (bispec.hpp:1168) */
     double *sn_sum;        /* [n_kbin][n_kbin][2] sn_save (bispec.hpp:941-1056) */
     long long *shell_nmode;/* [n_kbin] n_mode of calcInverseFourierTransformForBispectrum (field.hpp:383) */
+    double *xi;            /* [n_xi_ells][n_rbin] norm*(2l+1)*Re xi as written to xi%d.dat */
+    double *xi_raw;        /* [n_xi_ells][n_rbin] real part of the bin averages of calcCorrelationFunction (field.hpp:961-968) */
+    long long *xi_nmode;   /* [n_rbin] mesh points per r-bin */
 } hm_box_result;
 
 hm_box_plan *hm_box_plan_create(hm_ctx *ctx, const hm_box_config *cfg);
 void hm_box_plan_destroy(hm_ctx *ctx, hm_box_plan *plan);
-/* One mock: d_pos (np x 3 FP64, already periodic) -> P_l and B_000. */
+/* One mock: d_pos (np x 3 FP64, already periodic) -> P_l, xi_l and B_000. */
 int hm_box_plan_execute(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos, long long np, hm_box_result *res);
 /* Device bytes the plan holds. */
 size_t hm_box_plan_bytes(const hm_box_plan *plan);
 /* Mesh used for the triple product (n_mesh when regrid is off). */
 int hm_box_plan_triple_mesh(const hm_box_plan *plan, int m[3]);
+
+/* ---------------------------------------------------------------- */
+/* survey estimators (main.cpp:109-166, measure = pk)                 */
+/* ---------------------------------------------------------------- */
+typedef struct hm_survey_plan hm_survey_plan;
+
+typedef struct {
+    hm_mesh mesh;      /* FFT box: n_mesh and boxsize from the parameter file */
+    int precision;     /* HM_FP32 or HM_FP64 grids and FFTs; FP64 sums */
+    int n_kbin;        /* tools.hpp:118-124 */
+    double kmin, kmax;
+    int ell;           /* ELL = ell1, ell2 = 0 (calcPowerSpectrum, powerspec.hpp:11-15) */
+    int prune;         /* 1: bin only the modes that can reach a k-bin */
+} hm_survey_config;
+
+/* The survey quantities main.cpp derives before measuring. */
+typedef struct {
+    double wsum_d, wsum_r; /* n_tot_weight (particle.hpp:334-353) */
+    double alpha;          /* sum w_D / sum w_R (particle.hpp:268-282) */
+    double vsurvey;        /* (sum w_R)^2 / sum_x rho_R^2 dV (field.hpp:449-503) */
+    double nbar;           /* (sum w_D)^2 / Vsurvey / sum w_D^2 (particle.hpp:284-299) */
+    double shift[3];       /* min(R) - one cell, subtracted from every position (particle.hpp:222-265) */
+    double norm;           /* Vsurvey / (sum w_D)^2 (particle.hpp:302-313) */
+} hm_survey_info;
+
+hm_survey_plan *hm_survey_plan_create(hm_ctx *ctx, const hm_survey_config *cfg);
+void hm_survey_plan_destroy(hm_ctx *ctx, hm_survey_plan *plan);
+/* calcPowerSpectrum (powerspec.hpp:4-91) after main.cpp's preparation
+ * (main.cpp:109-166): d_pos_* are the unshifted Cartesian positions (n x 3
+ * FP64; the line of sight is pos/|pos| of these), d_w_* the weights (FP64).
+ * pk: [n_kbin][2] = norm * pk_save as written to pk%d.dat; nmode: [n_kbin]. */
+int hm_survey_plan_execute(hm_ctx *ctx, hm_survey_plan *plan, const double *d_pos_d, const double *d_w_d, long long n_d,
+                           const double *d_pos_r, const double *d_w_r, long long n_r, hm_survey_info *info, double *pk,
+                           long long *nmode);
+size_t hm_survey_plan_bytes(const hm_survey_plan *plan);
 
 #ifdef __cplusplus
 }

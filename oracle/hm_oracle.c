@@ -1117,3 +1117,159 @@ int or_box_bk(const or_param *p, const double *pos, long long np, const double *
     free(nm);
     return 0;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* survey (main.cpp:109-166, particle.hpp, field.hpp:186-228, 449-503,  */
+/* 1164-1188, powerspec.hpp:4-91)                                       */
+/* ------------------------------------------------------------------ */
+
+/* particle.hpp:334-353 */
+static double weight_sum(const double *w, long long n) {
+    double s = 0.0;
+    for (long long p = 0; p < n; p++) s += w[p];
+    return s;
+}
+
+/* particle.hpp:355-375 */
+static void calc_los(const double *pos, long long n, double *los) {
+    for (long long p = 0; p < n; p++) {
+        double m = sqrt(pos[3 * p] * pos[3 * p] + pos[3 * p + 1] * pos[3 * p + 1] + pos[3 * p + 2] * pos[3 * p + 2]);
+        los[3 * p] = pos[3 * p] / m;
+        los[3 * p + 1] = pos[3 * p + 1] / m;
+        los[3 * p + 2] = pos[3 * p + 2] / m;
+    }
+}
+
+/* particle.hpp:160-200 */
+static void min_pos(const double *pos, long long n, double *mn) {
+    mn[0] = pos[0];
+    mn[1] = pos[1];
+    mn[2] = pos[2];
+    for (long long i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++)
+            if (mn[a] > pos[3 * i + a]) mn[a] = pos[3 * i + a];
+}
+
+int or_survey_prepare(const or_param *p, double *posD, const double *wD, long long nD, double *posR,
+                      const double *wR, long long nR, double *losD, double *losR, double *info) {
+    double swD = weight_sum(wD, nD), swR = weight_sum(wR, nR);
+    calc_los(posD, nD, losD);
+    calc_los(posR, nR, losR);
+    if (swR == 0 || swD == 0) return -1;               /* particle.hpp:274-276 */
+    double alpha = swD / swR;                           /* particle.hpp:278 */
+    double dP[3];                                       /* particle.hpp:222-265, factor = 1 */
+    min_pos(posR, nR, dP);
+    for (int a = 0; a < 3; a++) dP[a] -= 1.0 * p->boxsize[a] / (double)(p->n_mesh[a]);
+    for (long long q = 0; q < nD; q++)
+        for (int a = 0; a < 3; a++) posD[3 * q + a] -= dP[a];
+    for (long long q = 0; q < nR; q++)
+        for (int a = 0; a < 3; a++) posR[3 * q + a] -= dP[a];
+    /* field.hpp:449-503 */
+    double *w = (double *)calloc(2 * (size_t)nR, sizeof(double));
+    for (long long q = 0; q < nR; q++) w[2 * q] = wR[q];
+    double *f = (double *)malloc(sizeof(double) * 2 * (size_t)p->n_mesh_tot);
+    or_calc_field(p, posR, w, nR, f);
+    double dV = p->volume / (double)(p->n_mesh_tot);
+    double norm = 0.0;
+    for (long long i = 0; i < p->n_mesh_tot; i++) norm += f[2 * i] * f[2 * i] * dV;
+    double vs = weight_sum(wR, nR) * weight_sum(wR, nR) / norm;
+    free(f);
+    free(w);
+    /* particle.hpp:284-299 */
+    double sw2 = 0.0;
+    for (long long q = 0; q < nD; q++) sw2 += pow(wD[q], 2);
+    double nbar = swD * swD / vs / sw2;
+    info[0] = swD;
+    info[1] = swR;
+    info[2] = alpha;
+    info[3] = vs;
+    info[4] = nbar;
+    info[5] = dP[0];
+    info[6] = dP[1];
+    info[7] = dP[2];
+    return 0;
+}
+
+/* field.hpp:186-228: D with w Y_LM(los), minus alpha times R with w Y_LM(los) */
+int or_ylm_weighted_fluctuation(const or_param *p, const double *posD, const double *losD, const double *wD,
+                                long long nD, const double *posR, const double *losR, const double *wR, long long nR,
+                                double alpha, int ell, int M, double *field) {
+    double *w = (double *)malloc(sizeof(double) * 2 * (size_t)(nD > nR ? nD : nR));
+    double *fR = (double *)malloc(sizeof(double) * 2 * (size_t)p->n_mesh_tot);
+    double y[2];
+    for (long long q = 0; q < nD; q++) {
+        or_ylm(ell, M, losD + 3 * q, y);
+        w[2 * q] = y[0] * wD[q];
+        w[2 * q + 1] = y[1] * wD[q];
+    }
+    or_calc_field(p, posD, w, nD, field);
+    for (long long q = 0; q < nR; q++) {
+        or_ylm(ell, M, losR + 3 * q, y);
+        w[2 * q] = y[0] * wR[q];
+        w[2 * q + 1] = y[1] * wR[q];
+    }
+    or_calc_field(p, posR, w, nR, fR);
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        field[2 * i] -= alpha * fR[2 * i];
+        field[2 * i + 1] -= alpha * fR[2 * i + 1];
+    }
+    free(fR);
+    free(w);
+    return 0;
+}
+
+/* field.hpp:1164-1188 */
+void or_shotnoise_pk(const double *losD, const double *wD, long long nD, const double *losR, const double *wR,
+                     long long nR, double alpha, int ell, int M, double *out) {
+    double dr = 0.0, di = 0.0, rr = 0.0, ri = 0.0, y[2];
+    for (long long q = 0; q < nD; q++) {
+        or_ylm(ell, M, losD + 3 * q, y);
+        dr += pow(wD[q], 2) * y[0];
+        di += pow(wD[q], 2) * y[1];
+    }
+    for (long long q = 0; q < nR; q++) {
+        or_ylm(ell, M, losR + 3 * q, y);
+        rr += pow(wR[q], 2) * y[0];
+        ri += pow(wR[q], 2) * y[1];
+    }
+    out[0] = dr + pow(alpha, 2) * rr;
+    out[1] = di + pow(alpha, 2) * ri;
+}
+
+/* powerspec.hpp:4-91: pk_save (2*n_kbin, complex) times norm, as written to pk%d.dat */
+int or_survey_pk(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                 const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                 const double *kbin, double *out, int *n_mode) {
+    if (!((p->ELL == p->ell1) && (p->ell2 == 0))) return -1;
+    const int nk = p->n_kbin;
+    double *d00 = grid_alloc(p), *dLM = grid_alloc(p);
+    or_ylm_weighted_fluctuation(p, posD, losD, wD, nD, posR, losR, wR, nR, alpha, 0, 0, d00);
+    or_fourier_transform(p, d00);
+    double *pk = (double *)calloc(2 * (size_t)nk, sizeof(double));
+    double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)nk);
+    for (int i = 0; i < 2 * nk; i++) out[i] = 0.0;
+    for (int M = -p->ELL; M <= p->ELL; M++) {
+        or_ylm_weighted_fluctuation(p, posD, losD, wD, nD, posR, losR, wR, nR, alpha, p->ELL, M, dLM);
+        or_fourier_transform(p, dLM);
+        double sn[2];
+        or_shotnoise_pk(losD, wD, nD, losR, wR, nR, alpha, p->ELL, M, sn);
+        for (int m1 = -p->ell1; m1 <= p->ell1; m1++) {
+            double w = or_wigner3j(p->ell1, 0, p->ELL, 0, 0, 0) * or_wigner3j(p->ell1, 0, p->ELL, m1, 0, M);
+            w *= (double)(2 * p->ELL + 1) * (double)(2 * p->ell1 + 1);
+            if (fabs(w) < 1.0e-10) continue;
+            or_calc_power_spectrum(p, dLM, d00, kbin, sn[0], sn[1], p->ell1, m1, tmp, n_mode);
+            for (int i = 0; i < nk; i++) {
+                pk[2 * i] += w * tmp[2 * i];
+                pk[2 * i + 1] += w * tmp[2 * i + 1];
+            }
+        }
+    }
+    double norm = vs / weight_sum(wD, nD) / weight_sum(wD, nD); /* particle.hpp:302-313 */
+    for (int i = 0; i < 2 * nk; i++) out[i] = norm * pk[i];
+    free(tmp);
+    free(pk);
+    free(dLM);
+    free(d00);
+    return 0;
+}

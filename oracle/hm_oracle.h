@@ -100,6 +100,23 @@ int or_box_xi(const or_param *p, const double *pos, long long np, const double *
 int or_box_bk(const or_param *p, const double *pos, long long np, const double *kbin,
               double *bk_save, double *sn_save, double *out);
 
+/* Survey (main.cpp:109-166): weight sums, LOS of the unshifted positions, alpha,
+ * the FFT-box shift (positions are shifted in place), Vsurvey and nbar.
+ * info = {sum w_D, sum w_R, alpha, Vsurvey, nbar, dP[3]}. */
+int or_survey_prepare(const or_param *p, double *posD, const double *wD, long long nD, double *posR,
+                      const double *wR, long long nR, double *losD, double *losR, double *info);
+/* field.hpp:186-228 (complex field, 2*n_mesh_tot). */
+int or_ylm_weighted_fluctuation(const or_param *p, const double *posD, const double *losD, const double *wD,
+                                long long nD, const double *posR, const double *losR, const double *wR, long long nR,
+                                double alpha, int ell, int M, double *field);
+/* field.hpp:1164-1188 (out2 complex). */
+void or_shotnoise_pk(const double *losD, const double *wD, long long nD, const double *losR, const double *wR,
+                     long long nR, double alpha, int ell, int M, double *out2);
+/* powerspec.hpp:4-91: norm * pk_save (2*n_kbin) as written to pk%d.dat; positions already shifted. */
+int or_survey_pk(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                 const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                 const double *kbin, double *out, int *n_mode);
+
 /* Row-major index helper used by the tests. */
 long long or_coord(const or_param *p, int i, int j, int k);
 

@@ -69,6 +69,13 @@ def lib():
             "or_box_pk": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, ct.POINTER(ct.c_int)]),
             "or_box_xi": (ct.c_int, [P, _D, ct.c_longlong, _D, _D]),
             "or_box_bk": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, _D, _D]),
+            "or_survey_prepare": (ct.c_int, [P, _D, _D, ct.c_longlong, _D, _D, ct.c_longlong, _D, _D, _D]),
+            "or_ylm_weighted_fluctuation": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong,
+                                                       ct.c_double, ct.c_int, ct.c_int, _D]),
+            "or_shotnoise_pk": (None, [_D, _D, ct.c_longlong, _D, _D, ct.c_longlong, ct.c_double, ct.c_int, ct.c_int,
+                                       _D]),
+            "or_survey_pk": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
+                                        ct.c_double, _D, _D, ct.POINTER(ct.c_int)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -215,6 +222,18 @@ def calc_power_spectrum(p: Param, f1, f2, kb, sn, ell, m):
     return pk[0::2] + 1j * pk[1::2], nm
 
 
+def calc_correlation_function(p: Param, f1, f2, rb, sn, ell, m):
+    """field.hpp:854-977: (xi[n_rbin] complex, n_mode[n_rbin])."""
+    a1, v1 = _c(f1)
+    a2, v2 = _c(f2)
+    rb = np.ascontiguousarray(rb, dtype=np.float64)
+    xi = np.zeros(2 * p.n_rbin)
+    nm = np.zeros(p.n_rbin, dtype=np.int32)
+    lib().or_calc_correlation_function(ct.byref(p), dp(v1), dp(v2), dp(rb), float(np.real(sn)), float(np.imag(sn)),
+                                       ell, m, dp(xi), nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+    return xi[0::2] + 1j * xi[1::2], nm
+
+
 def ifft_for_bispectrum(p: Param, density, kmag, dk):
     a, v = _c(density)
     out = np.zeros(shape(p) + (2,))
@@ -255,3 +274,48 @@ def box_bk(p: Param, pos):
         raise ValueError("or_box_bk rejected the multipole combination")
     c = lambda a: a[0::2] + 1j * a[1::2]
     return c(out), c(bk), c(sn)
+
+
+class Survey:
+    """main.cpp:109-166 on a data and a random catalogue: positions shifted
+    into the FFT box, LOS of the unshifted positions, alpha, Vsurvey, nbar."""
+
+    def __init__(self, p: Param, posD, wD, posR, wR):
+        self.posD = np.array(posD, dtype=np.float64).reshape(-1, 3)
+        self.posR = np.array(posR, dtype=np.float64).reshape(-1, 3)
+        self.wD = np.ascontiguousarray(wD, dtype=np.float64)
+        self.wR = np.ascontiguousarray(wR, dtype=np.float64)
+        self.losD = np.zeros_like(self.posD)
+        self.losR = np.zeros_like(self.posR)
+        info = np.zeros(8)
+        rc = lib().or_survey_prepare(ct.byref(p), dp(self.posD), dp(self.wD), len(self.wD), dp(self.posR), dp(self.wR),
+                                     len(self.wR), dp(self.losD), dp(self.losR), dp(info))
+        if rc:
+            raise ValueError("or_survey_prepare: empty weights")
+        self.wsumD, self.wsumR, self.alpha, self.vsurvey, self.nbar = info[:5]
+        self.shift = info[5:8].copy()
+
+    def _args(self):
+        return (dp(self.posD), dp(self.losD), dp(self.wD), len(self.wD), dp(self.posR), dp(self.losR), dp(self.wR),
+                len(self.wR))
+
+    def field(self, p: Param, ell, M):
+        f = np.zeros(shape(p) + (2,))
+        lib().or_ylm_weighted_fluctuation(ct.byref(p), *self._args(), float(self.alpha), ell, M, dp(f))
+        return f[..., 0] + 1j * f[..., 1]
+
+    def shotnoise(self, ell, M):
+        out = np.zeros(2)
+        lib().or_shotnoise_pk(dp(self.losD), dp(self.wD), len(self.wD), dp(self.losR), dp(self.wR), len(self.wR),
+                              float(self.alpha), ell, M, dp(out))
+        return out[0] + 1j * out[1]
+
+    def pk(self, p: Param):
+        kb = kbin(p)
+        out = np.zeros(2 * p.n_kbin)
+        nm = np.zeros(p.n_kbin, dtype=np.int32)
+        rc = lib().or_survey_pk(ct.byref(p), *self._args(), float(self.alpha), float(self.vsurvey), dp(kb), dp(out),
+                                nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+        if rc:
+            raise ValueError("or_survey_pk rejected the multipole combination")
+        return out[0::2] + 1j * out[1::2], nm

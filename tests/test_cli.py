@@ -112,3 +112,14 @@ def test_end_to_end_pk_and_bk_match_oracle(tmp_path):
     assert np.allclose(got[:, 2], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
     assert np.allclose(got[:, 3], ref.imag, atol=1e-6 * np.abs(ref.real).max())
     assert not (tmp_path / "out" / "bk000_01.dat").exists()
+    for ell in (0, 2):
+        ini = tmp_path / f"xi{ell}.ini"
+        hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="2PCF", ell1=ell, ELL=ell, rmin=40.0,
+                      rmax=200.0, n_rbin=9, **kw)
+        r = run(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        got = hio.read_table(tmp_path / "out" / f"xi{ell}.dat")
+        p = po.param(n, L, po.TSC, rmin=40.0, rmax=200.0, n_rbin=9, ell1=ell, ELL=ell)
+        ref = po.box_xi(p, posd)
+        assert np.allclose(got[:, 0], po.rbin(p), atol=5e-6)
+        assert np.allclose(got[:, 1], ref, rtol=2e-7, atol=1e-7 * np.abs(ref).max())

@@ -298,3 +298,40 @@ def test_box_bk_shotnoise_ijk_numpy():
         ijk = np.sum(spherical_jn(0, kb[2] * r) * spherical_jn(0, kb[ik] * r) * xi) * (L ** 3 / n ** 3)
         expect = N + pk[2] + pk[ik] + ijk
         assert sn_save[ik] == pytest.approx(expect, rel=1e-9, abs=1e-9 * abs(expect))
+
+
+@pytest.mark.parametrize("ell", [0, 2])
+def test_xi_plane_wave_closed_form(ell):
+    """calcCorrelationFunction (field.hpp:854-977) of a two-mode field: the
+    inverse transform is (|A|^2/V)(e^{i k0.r}/W(k0)^2 + e^{-i k0.r}/W(-k0)^2)
+    in closed form; binned with the 0.5 Mpc/h fine rule and P_l(r_z/|r|)."""
+    from scipy.special import eval_legendre
+
+    n, L = 16, 400.0
+    p = po.param(n, L, po.TSC, rmin=30.0, rmax=150.0, n_rbin=7)
+    rb = po.rbin(p)
+    f = np.zeros((n, n, n), complex)
+    A = 2.0 - 1.5j
+    f[2, 1, 3] = A
+    f[-2, -1, -3] = np.conj(A)
+    xi, nm = po.calc_correlation_function(p, f, f, rb, 0.0, ell, 0)
+    dk = 2 * np.pi / L
+    k0 = np.array([2, 1, 3]) * dk
+    W1 = po.assignment_function(p, k0)
+    W2 = po.assignment_function(p, -k0)
+    idx = np.arange(n)
+    sgn = np.where(idx < n // 2, idx, idx - n) * (L / n)
+    rx, ry, rz = np.meshgrid(sgn, sgn, sgn, indexing="ij")
+    r = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+    phase = k0[0] * rx + k0[1] * ry + k0[2] * rz
+    field = abs(A) ** 2 / L ** 3 * (np.exp(1j * phase) / W1 ** 2 + np.exp(-1j * phase) / W2 ** 2)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        y = np.where(r > 0, eval_legendre(ell, np.where(r > 0, rz / r, 0.0)), 0.0) if ell else np.ones_like(r)
+    centre = (r / 0.5 + 0.5).astype(int) * 0.5
+    drb = rb[1] - rb[0]
+    for j, c in enumerate(rb):
+        lo = c - drb * 0.5 if c > drb * 0.5 else 0.0
+        sel = (centre > lo) & (centre <= c + drb * 0.5)
+        assert nm[j] == sel.sum()
+        expect = (field[sel] * y[sel]).sum() / sel.sum() if sel.sum() else 0.0
+        assert abs(xi[j] - expect) <= 1e-12 * abs(A) ** 2 / L ** 3 * 2 / min(W1, W2) ** 2
