@@ -81,7 +81,14 @@ class SurveyConfig(ct.Structure):
         ("kmax", ct.c_double),
         ("ell", ct.c_int),
         ("prune", ct.c_int),
+        ("measure", ct.c_int),
+        ("n_rbin", ct.c_int),
+        ("rmin", ct.c_double),
+        ("rmax", ct.c_double),
     ]
+
+
+SURVEY_MEASURES = {"pk": 0, "2PCF": 1, "window2PCF": 2}
 
 
 class SurveyInfo(ct.Structure):
@@ -412,13 +419,18 @@ class BoxPlan:
 
 
 class SurveyPlan:
-    """Survey-geometry P_l plan (hm_survey_plan): main.cpp's preparation
-    (main.cpp:109-166) and calcPowerSpectrum (powerspec.hpp:4-91)."""
+    """Survey-geometry plan (hm_survey_plan): main.cpp's preparation
+    (main.cpp:109-166) and calcPowerSpectrum (powerspec.hpp:4-91),
+    calcTwoPointCorrelationFunction (93-179) or its window version (182-270)."""
 
     def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2, ell=0,
-                 precision=FP32, prune=True):
+                 precision=FP32, prune=True, measure="pk", n_rbin=25, rmin=30.0, rmax=150.0):
         self.ctx = ctx
         cfg = SurveyConfig()
+        cfg.measure = SURVEY_MEASURES[measure]
+        cfg.n_rbin = int(n_rbin)
+        cfg.rmin = float(rmin)
+        cfg.rmax = float(rmax)
         cfg.mesh = make_mesh(n_mesh, boxsize, assign)
         cfg.precision = precision
         cfg.n_kbin = int(n_kbin)
@@ -427,7 +439,7 @@ class SurveyPlan:
         cfg.ell = int(ell)
         cfg.prune = 1 if prune else 0
         self.cfg = cfg
-        self.n_kbin = int(n_kbin)
+        self.n_kbin = int(n_kbin) if measure == "pk" else int(n_rbin)
         self.h = ctx.lib.hm_survey_plan_create(ctx.h, ct.byref(cfg))
         if not self.h:
             ctx.raise_error("hm_survey_plan_create")
@@ -449,7 +461,8 @@ class SurveyPlan:
 
     def execute(self, pos_d, w_d, pos_r, w_r):
         """Positions (N, 3) float64 in the survey frame (unshifted) and weights (N,),
-        host arrays or DeviceArrays.  Returns (pk complex [n_kbin], nmode, info dict)."""
+        host arrays or DeviceArrays.  Returns (norm * pk_save or norm * xi_save, complex per bin;
+        modes or mesh points per bin; info dict)."""
         def dev(a, shape):
             if isinstance(a, DeviceArray):
                 return a

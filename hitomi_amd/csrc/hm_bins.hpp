@@ -73,6 +73,14 @@ __device__ __forceinline__ double legendre(int l, double mu) {
     return p1;
 }
 
+constexpr int FINE_NR = 20000;   // field.hpp:906
+constexpr double FINE_DR = 0.5;  // field.hpp:907
+
+__device__ __forceinline__ double mesh_vec(int i, int n, double d) {
+#pragma clang fp contract(off)
+    return (i < n / 2) ? (double)i * d : (double)(i - n) * d;  // field.hpp:929-931
+}
+
 inline int wpow_of(int assign) { return assign == 0 ? 1 : (assign == 1 ? 2 : 3); }
 
 // Fine index -> [first, last] coarse bin whose (lower, upper] holds the fine

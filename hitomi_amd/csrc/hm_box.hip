@@ -207,8 +207,6 @@ template <typename T> struct SupHalfSrc {
 // separations that can reach an r-bin: fine 0.5 Mpc/h index, coarse fold
 // (lower, upper], Y_l0(r) = P_l(r_z / |r|), FP64 sums per compact row.
 // ---------------------------------------------------------------------------
-constexpr int FINE_NR = 20000;     // field.hpp:906
-constexpr double FINE_DR = 0.5;    // field.hpp:907
 
 struct XiBinArgs {
     int n[3], s[3], c[3];
@@ -219,10 +217,6 @@ struct XiBinArgs {
     int ells[HM_MAX_ELLS];
 };
 
-__device__ __forceinline__ double mesh_vec(int i, int n, double d) {
-#pragma clang fp contract(off)
-    return (i < n / 2) ? (double)i * d : (double)(i - n) * d;  // field.hpp:929-931
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_bin_xi(const T* __restrict__ xr, XiBinArgs a, double* __restrict__ part,

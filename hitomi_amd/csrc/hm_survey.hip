@@ -180,7 +180,7 @@ __device__ inline void los_of(const double* p, double* l) {
 // Complex weights of one field over D then R: w Y(l, M)(los) for D and
 // -alpha w Y(l, M)(los) for R (field.hpp:186-228; D - alpha R is formed by
 // the fixed-point sum).  mode 1: the random catalogue alone with (w, 0)
-// (field.hpp:476-485).
+// (field.hpp:476-485); mode 2: the random catalogue alone with w Y(l, M).
 __global__ __launch_bounds__(256) void k_survey_weights(const double* __restrict__ pD, const double* __restrict__ wD,
                                                        long long nD, const double* __restrict__ pR,
                                                        const double* __restrict__ wR, long long nR, double alpha, int l,
@@ -191,6 +191,16 @@ __global__ __launch_bounds__(256) void k_survey_weights(const double* __restrict
         if (q < nR) {
             out[2 * q] = wR[q];
             out[2 * q + 1] = 0.0;
+        }
+        return;
+    }
+    if (mode == 2) {  // the random catalogue alone with w Y(l, M)(los) (field.hpp:230-247)
+        if (q < nR) {
+            double los[3];
+            los_of(pR + 3 * q, los);
+            const Cplx y = ylm_red(l, M, los[0], los[1], los[2]);
+            out[2 * q] = y.re * wR[q];
+            out[2 * q + 1] = y.im * wR[q];
         }
         return;
     }
@@ -365,6 +375,90 @@ __global__ __launch_bounds__(256) void k_bin_survey(const C<T>* __restrict__ f00
 }
 
 // ---------------------------------------------------------------------------
+// correlation functions (calcCorrelationFunction, field.hpp:854-977, summed
+// with the weights of powerspec.hpp:134-147): per M, g_M(k) = (F_LM conj F_00
+// - SN_M S)/W^2 / V on the full mesh, its inverse transform restricted to the
+// separations that reach an r-bin, and the binning of w_M xi_M(r) Y_{l,-M}(r)
+// added to the partial sums of the previous M
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_survey_g(const C<T>* __restrict__ f00, const C<T>* __restrict__ fl, int n0,
+                                                 int n1, int n2, AxisTabs t, int wp, int M, double snr, double sni,
+                                                 double inv_v, C<T>* __restrict__ g) {
+#pragma clang fp contract(off)
+    const long long ng = (long long)n0 * n1 * n2;
+    const long long st = (long long)gridDim.x * 256;
+    const int am = M < 0 ? -M : M;
+    const double sg = (M < 0 && (am & 1)) ? -1.0 : 1.0;
+    for (long long lin = (long long)blockIdx.x * 256 + threadIdx.x; lin < ng; lin += st) {
+        const int k = (int)(lin % n2);
+        const long long r = lin / n2;
+        const int j = (int)(r % n1), i = (int)(r / n1);
+        double fr, fi;
+        if (M >= 0) {
+            const C<T> f = fl[(long long)am * ng + lin];
+            fr = (double)f.x;
+            fi = (double)f.y;
+        } else {
+            const long long mir = ((long long)((n0 - i) % n0) * n1 + (n1 - j) % n1) * n2 + (n2 - k) % n2;
+            const C<T> f = fl[(long long)am * ng + mir];
+            fr = sg * (double)f.x;
+            fi = -(sg * (double)f.y);
+        }
+        const C<T> h = f00[lin];
+        const double hr = (double)h.x, hi = -(double)h.y;
+        double pr = fr * hr - fi * hi, pi = fr * hi + fi * hr;
+        const double S = (t.sf[0][i] * t.sf[1][j]) * t.sf[2][k];
+        pr -= snr * S;
+        pi -= sni * S;
+        const double W = wpow((t.wf[0][i] * t.wf[1][j]) * t.wf[2][k], wp);
+        const double W2 = pow(W, 2);
+        pr /= W2;
+        pi /= W2;
+        g[lin] = mk<T>((T)(inv_v * pr), (T)(inv_v * pi));
+    }
+}
+
+struct SurveyXiArgs {
+    int n[3], s[3], c[3];
+    double dr[3];
+    long long nrows;
+    const int2* f2b;
+    int nr, l, m1, count;
+    double w;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_survey_xi(const C<T>* __restrict__ xi, SurveyXiArgs a,
+                                                      double* __restrict__ part, long long* __restrict__ cnt) {
+#pragma clang fp contract(off)
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.nrows) return;
+    const int i = support_map((int)(row / a.c[1]), a.s[0], a.n[0]);
+    const int j = support_map((int)(row % a.c[1]), a.s[1], a.n[1]);
+    const double rx = mesh_vec(i, a.n[0], a.dr[0]), ry = mesh_vec(j, a.n[1], a.dr[1]);
+    double* P = part + row * 2 * (long long)a.nr;
+    long long* Cn = cnt + row * (long long)a.nr;
+    for (int kc = 0; kc < a.c[2]; kc++) {
+        const int k = support_map(kc, a.s[2], a.n[2]);
+        const double rz = mesh_vec(k, a.n[2], a.dr[2]);
+        const double rmag = kmag3(rx, ry, rz);
+        const int ri = (int)(rmag / FINE_DR + 0.5);
+        if (ri >= FINE_NR) continue;
+        const int2 b = a.f2b[ri];
+        if (b.x < 0) continue;
+        const C<T> v = xi[((long long)i * a.n[1] + j) * a.n[2] + k];
+        const Cplx y = ylm_red(a.l, a.m1, rx, ry, rz);
+        const double vr = (double)v.x * y.re - (double)v.y * y.im, vi = (double)v.x * y.im + (double)v.y * y.re;
+        for (int bb = b.x; bb <= b.y; bb++) {
+            P[2 * bb] += a.w * vr;
+            P[2 * bb + 1] += a.w * vi;
+            if (a.count) Cn[bb] += 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
 struct SurveyPlanImpl {
@@ -375,8 +469,12 @@ struct SurveyPlanImpl {
     int s[3];
     long long nrows = 0;
     long long tab_off[3];
+    int measure = HM_SURVEY_PK;
+    int nr = 0;            // r-bins of the correlation functions
+    int rs[3] = {-1, -1, -1};
+    long long nrows_r = 0;
     SortWork sort;
-    DevBuf tab, f2b, pos, wts, fre, fim, fields, part, cnt, red, cnt_red, scratch;
+    DevBuf tab, f2b, f2b_r, pos, wts, fre, fim, fields, gwork, part, cnt, red, cnt_red, scratch;
     size_t bytes = 0;
 };
 
@@ -389,8 +487,8 @@ static int sv_alloc(hm_ctx* ctx, SurveyPlanImpl* p, DevBuf& b, size_t bytes) {
 
 void survey_destroy(SurveyPlanImpl* p) {
     if (!p) return;
-    DevBuf* bufs[] = {&p->tab, &p->f2b, &p->pos, &p->wts, &p->fre, &p->fim, &p->fields,
-                      &p->part, &p->cnt, &p->red, &p->cnt_red, &p->scratch};
+    DevBuf* bufs[] = {&p->tab, &p->f2b, &p->f2b_r, &p->pos, &p->wts, &p->fre, &p->fim, &p->fields,
+                      &p->gwork, &p->part, &p->cnt, &p->red, &p->cnt_red, &p->scratch};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -416,6 +514,8 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
     if (cfg->precision != HM_FP32 && cfg->precision != HM_FP64) return fail("survey plan: precision must be FP32 or FP64");
     if (cfg->n_kbin < 2 || cfg->n_kbin > 256) return fail("survey plan: n_kbin must be in [2, 256]");
     if (cfg->ell < 0 || cfg->ell > SV_MAX_ELL) return fail("survey plan: ell must be in [0, 8]");
+    p->measure = cfg->measure;
+    if (p->measure < HM_SURVEY_PK || p->measure > HM_SURVEY_XI_WINDOW) return fail("survey plan: unknown measure");
     p->nk = cfg->n_kbin;
     p->l = cfg->ell;
     p->nf = p->l == 0 ? 1 : p->l + 2;
@@ -441,6 +541,27 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
         p->s[a] = (!cfg->prune || 2 * s + 1 >= m.n[a] || s >= m.n[a] / 2) ? -1 : (int)s;
     }
     p->nrows = (long long)support_count(p->s[0], m.n[0]) * support_count(p->s[1], m.n[1]);
+    std::vector<int2> f2b_r;
+    if (p->measure != HM_SURVEY_PK) {
+        if (cfg->n_rbin < 2 || cfg->n_rbin > 256) return fail("survey plan: n_rbin must be in [2, 256]");
+        if (!(cfg->rmax > cfg->rmin)) return fail("survey plan: rmax must exceed rmin");
+        p->nr = cfg->n_rbin;
+        std::vector<double> rb, rlo, rup;
+        bin_edges(cfg->rmin, cfg->rmax, p->nr, rb, rlo, rup);
+        const std::string er = fine_to_coarse(rlo, rup, FINE_NR, FINE_DR, f2b_r);
+        if (!er.empty()) return fail("survey plan: " + er);
+        double rumax = 0.0;
+        for (double u : rup) rumax = std::max(rumax, u);
+        // a separation reaches a bin only if |r| < rumax + 0.25 (fine centre rule)
+        for (int a = 0; a < 3; a++) {
+            const double dr = m.L[a] / (double)m.n[a];
+            const long long sr = (long long)std::floor((rumax + FINE_DR) / dr) + 1;
+            p->rs[a] = (!cfg->prune || 2 * sr + 1 >= m.n[a] || sr >= m.n[a] / 2) ? -1 : (int)sr;
+        }
+        p->nrows_r = (long long)support_count(p->rs[0], m.n[0]) * support_count(p->rs[1], m.n[1]);
+    }
+    const int nbin = p->measure == HM_SURVEY_PK ? p->nk : p->nr;
+    const long long nrow_bins = p->measure == HM_SURVEY_PK ? p->nrows : p->nrows_r;
     std::vector<double> tabs;
     host_axis_tables(m, tabs, p->tab_off);
     const size_t elem = cfg->precision == HM_FP64 ? 16 : 8;
@@ -449,13 +570,19 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
         sv_alloc(ctx, p, p->fre, sizeof(long long) * (size_t)ng) ||
         sv_alloc(ctx, p, p->fim, sizeof(long long) * (size_t)ng) ||
         sv_alloc(ctx, p, p->fields, elem * (size_t)ng * (size_t)p->nf) ||
-        sv_alloc(ctx, p, p->part, sizeof(double) * 2 * (size_t)p->nk * (size_t)p->nrows) ||
-        sv_alloc(ctx, p, p->cnt, sizeof(long long) * (size_t)p->nk * (size_t)p->nrows) ||
-        sv_alloc(ctx, p, p->red, sizeof(double) * 2 * (size_t)p->nk) ||
-        sv_alloc(ctx, p, p->cnt_red, sizeof(long long) * (size_t)p->nk))
+        sv_alloc(ctx, p, p->part, sizeof(double) * 2 * (size_t)nbin * (size_t)nrow_bins) ||
+        sv_alloc(ctx, p, p->cnt, sizeof(long long) * (size_t)nbin * (size_t)nrow_bins) ||
+        sv_alloc(ctx, p, p->red, sizeof(double) * 2 * (size_t)nbin) ||
+        sv_alloc(ctx, p, p->cnt_red, sizeof(long long) * (size_t)nbin))
+        return fail(ctx->err);
+    if (p->measure != HM_SURVEY_PK &&
+        (sv_alloc(ctx, p, p->gwork, elem * (size_t)ng) || sv_alloc(ctx, p, p->f2b_r, sizeof(int2) * FINE_NR)))
         return fail(ctx->err);
     if (hipMemcpy(p->tab.p, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->f2b.p, f2b.data(), sizeof(int2) * FINE_N, hipMemcpyHostToDevice) != hipSuccess)
+        return fail("survey plan: table upload failed");
+    if (p->measure != HM_SURVEY_PK &&
+        hipMemcpy(p->f2b_r.p, f2b_r.data(), sizeof(int2) * FINE_NR, hipMemcpyHostToDevice) != hipSuccess)
         return fail("survey plan: table upload failed");
     return p;
 }
@@ -555,19 +682,28 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
     const double nbar = swD * swD / vs / sw2D;  // particle.hpp:296
     // 3. Y_lm-weighted fluctuations and their transforms.  |Y_red| <= 1, so every
     // particle's |re| + |im| is at most sqrt(2) |w|: the bound fixes the fixed-point scale.
-    const double wabs = 1.5 * (awD + std::fabs(alpha) * awR);
+    // The window function (field.hpp:230-247) uses the random catalogue alone.
+    const bool window = p->measure == HM_SURVEY_XI_WINDOW;
+    const double wabs = window ? 1.5 * awR : 1.5 * (awD + std::fabs(alpha) * awR);
     C<T>* fields = (C<T>*)p->fields.p;
     const int n3[3] = {m.n[0], m.n[1], m.n[2]};
     for (int f = 0; f < p->nf; f++) {
         const int l = f == 0 ? 0 : p->l, M = f == 0 ? 0 : f - 1;
         if (ctx->launch("survey_weights", [&] {
-                hipLaunchKernelGGL(k_survey_weights, dim3(pb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR, alpha,
-                                   l, M, 0, wts);
+                if (window)
+                    hipLaunchKernelGGL(k_survey_weights, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, ctx->stream,
+                                       pD, wD, nD, pR, wR, nR, alpha, l, M, 2, wts + 2 * nD);
+                else
+                    hipLaunchKernelGGL(k_survey_weights, dim3(pb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR,
+                                       alpha, l, M, 0, wts);
             }))
             return -1;
         const double scale = fixed_scale_for(wabs);
         const bool cplx = M != 0;
-        if (assign_sorted(ctx, ag, pos, wts, N, scale, fre, cplx ? fim : nullptr, p->sort)) return -1;
+        const int arc = window ? assign_sorted(ctx, ag, pos + 3 * nD, wts + 2 * nD, nR, scale, fre, cplx ? fim : nullptr,
+                                               p->sort)
+                               : assign_sorted(ctx, ag, pos, wts, N, scale, fre, cplx ? fim : nullptr, p->sort);
+        if (arc) return -1;
         C<T>* out = fields + (long long)f * ng;
         if (ctx->launch("survey_to_complex", [&] {
                 hipLaunchKernelGGL(k_survey_to_complex<T>, dim3(rbg), dim3(256), 0, ctx->stream, fre,
@@ -585,58 +721,129 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
     std::vector<double> snr(nsn);
     HM_HIP_CHECK(ctx, hipMemcpyAsync(snr.data(), red, sizeof(double) * nsn, hipMemcpyDeviceToHost, ctx->stream));
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    // 5. binning
-    SurveyBinArgs a;
-    for (int ax = 0; ax < 3; ax++) {
-        a.n[ax] = m.n[ax];
-        a.s[ax] = p->s[ax];
-        a.c[ax] = support_count(p->s[ax], m.n[ax]);
+    // shot noise per M: sum_D + alpha^2 sum_R (field.hpp:1187), or sum_R for the window (field.hpp:1202-1218)
+    std::vector<double> snM(2 * (size_t)(p->l + 1));
+    for (int M = 0; M <= p->l; M++) {
+        snM[2 * M] = window ? snr[4 * M + 2] : snr[4 * M] + pow(alpha, 2) * snr[4 * M + 2];
+        snM[2 * M + 1] = window ? snr[4 * M + 3] : snr[4 * M + 1] + pow(alpha, 2) * snr[4 * M + 3];
     }
-    a.nrows = p->nrows;
-    a.ng = ng;
+    const C<T>* fl = p->l == 0 ? fields : fields + ng;
     const double* base = (const double*)p->tab.p;
     const long long ntot = (long long)m.n[0] + m.n[1] + m.n[2];
+    AxisTabs tabs;
     for (int ax = 0; ax < 3; ax++) {
-        a.t.kv[ax] = base + p->tab_off[ax];
-        a.t.wf[ax] = base + ntot + p->tab_off[ax];
-        a.t.sf[ax] = base + 2 * ntot + p->tab_off[ax];
+        tabs.kv[ax] = base + p->tab_off[ax];
+        tabs.wf[ax] = base + ntot + p->tab_off[ax];
+        tabs.sf[ax] = base + 2 * ntot + p->tab_off[ax];
     }
-    a.wp = wpow_of(m.assign);
-    a.f2b = (const int2*)p->f2b.p;
-    a.nk = p->nk;
-    a.l = p->l;
-    for (int i = 0; i < 2 * SV_MAX_ELL + 1; i++) a.w[i] = i < 2 * p->l + 1 ? p->wM[i] : 0.0;
-    for (int M = 0; M <= p->l; M++) {
-        // field.hpp:1187: sum_D + alpha^2 sum_R
-        a.sn[2 * M] = snr[4 * M] + pow(alpha, 2) * snr[4 * M + 2];
-        a.sn[2 * M + 1] = snr[4 * M + 1] + pow(alpha, 2) * snr[4 * M + 3];
+    const int nbin = p->measure == HM_SURVEY_PK ? p->nk : p->nr;
+    const long long nrow_bins = p->measure == HM_SURVEY_PK ? p->nrows : p->nrows_r;
+    if (fill_zero(ctx, p->part.p, sizeof(double) * 2 * (size_t)nbin * nrow_bins)) return -1;
+    if (fill_zero(ctx, p->cnt.p, sizeof(long long) * (size_t)nbin * nrow_bins)) return -1;
+    if (p->measure == HM_SURVEY_PK) {
+        // 5. binning
+        SurveyBinArgs a;
+        for (int ax = 0; ax < 3; ax++) {
+            a.n[ax] = m.n[ax];
+            a.s[ax] = p->s[ax];
+            a.c[ax] = support_count(p->s[ax], m.n[ax]);
+        }
+        a.nrows = p->nrows;
+        a.ng = ng;
+        a.t = tabs;
+        a.wp = wpow_of(m.assign);
+        a.f2b = (const int2*)p->f2b.p;
+        a.nk = p->nk;
+        a.l = p->l;
+        for (int i = 0; i < 2 * SV_MAX_ELL + 1; i++) a.w[i] = i < 2 * p->l + 1 ? p->wM[i] : 0.0;
+        for (int i = 0; i < 2 * (p->l + 1); i++) a.sn[i] = snM[i];
+        if (ctx->launch("bin_survey", [&] {
+                hipLaunchKernelGGL(k_bin_survey<T>, dim3((unsigned)((p->nrows + 255) / 256)), dim3(256), 0,
+                                   ctx->stream, fields, fl, a, (double*)p->part.p, (long long*)p->cnt.p);
+            }))
+            return -1;
+    } else {
+        // 5'. per M: g_M, its inverse transform on the separations that reach a bin, binning
+        C<T>* g = (C<T>*)p->gwork.p;
+        const int cz = support_count(p->rs[2], m.n[2]), cy = support_count(p->rs[1], m.n[1]);
+        SurveyXiArgs xa;
+        for (int ax = 0; ax < 3; ax++) {
+            xa.n[ax] = m.n[ax];
+            xa.s[ax] = p->rs[ax];
+            xa.c[ax] = support_count(p->rs[ax], m.n[ax]);
+            xa.dr[ax] = m.L[ax] / (double)m.n[ax];
+        }
+        xa.nrows = p->nrows_r;
+        xa.f2b = (const int2*)p->f2b_r.p;
+        xa.nr = p->nr;
+        xa.l = p->l;
+        bool first = true;
+        for (int M = -p->l; M <= p->l; M++) {
+            const double wM = p->wM[M + p->l];
+            if (wM == 0.0) continue;
+            const int am = M < 0 ? -M : M;
+            const double sg = (M < 0 && (am & 1)) ? -1.0 : 1.0;
+            const double snre = M >= 0 ? snM[2 * am] : sg * snM[2 * am];
+            const double snim = M >= 0 ? snM[2 * am + 1] : -(sg * snM[2 * am + 1]);
+            if (ctx->launch("survey_g", [&] {
+                    hipLaunchKernelGGL(k_survey_g<T>, dim3(rbg), dim3(256), 0, ctx->stream, fields, fl, m.n[0], m.n[1],
+                                       m.n[2], tabs, wpow_of(m.assign), M, snre, snim, 1.0 / m.volume(), g);
+                }))
+                return -1;
+            RowGeom rg{(long long)m.n[0] * m.n[1], m.n[2], m.n[2], -1};
+            if (fft_rows<T>(ctx, "survey_xi_z", m.n[2], +1, g, g, rg)) return -1;
+            ColGeom gy;
+            gy.OS = (long long)m.n[1] * m.n[2];
+            gy.PS = m.n[2];
+            gy.n_outer = m.n[0];
+            gy.outer_full = m.n[0];
+            gy.outer_s = -1;
+            gy.n_inner = cz;
+            gy.inner_full = m.n[2];
+            gy.inner_s = p->rs[2];
+            gy.point_s = -1;
+            if (fft_cols<T>(ctx, "survey_xi_y", m.n[1], +1, g, g, gy)) return -1;
+            ColGeom gx;
+            gx.OS = m.n[2];
+            gx.PS = (long long)m.n[1] * m.n[2];
+            gx.n_outer = cy;
+            gx.outer_full = m.n[1];
+            gx.outer_s = p->rs[1];
+            gx.n_inner = cz;
+            gx.inner_full = m.n[2];
+            gx.inner_s = p->rs[2];
+            gx.point_s = -1;
+            if (fft_cols<T>(ctx, "survey_xi_x", m.n[0], +1, g, g, gx)) return -1;
+            xa.m1 = -M;
+            xa.w = wM;
+            xa.count = first ? 1 : 0;
+            first = false;
+            if (ctx->launch("bin_survey_xi", [&] {
+                    hipLaunchKernelGGL(k_bin_survey_xi<T>, dim3((unsigned)((p->nrows_r + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, g, xa, (double*)p->part.p, (long long*)p->cnt.p);
+                }))
+                return -1;
+        }
     }
-    if (fill_zero(ctx, p->part.p, sizeof(double) * 2 * (size_t)p->nk * p->nrows)) return -1;
-    if (fill_zero(ctx, p->cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows)) return -1;
-    const C<T>* fl = p->l == 0 ? fields : fields + ng;
-    if (ctx->launch("bin_survey", [&] {
-            hipLaunchKernelGGL(k_bin_survey<T>, dim3((unsigned)((p->nrows + 255) / 256)), dim3(256), 0, ctx->stream,
-                               fields, fl, a, (double*)p->part.p, (long long*)p->cnt.p);
-        }))
+    if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->part.p, nrow_bins, 2 * nbin, (double*)p->red.p))
         return -1;
-    if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->part.p, p->nrows, 2 * p->nk, (double*)p->red.p))
+    if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->cnt.p, nrow_bins, nbin, (long long*)p->cnt_red.p))
         return -1;
-    if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->cnt.p, p->nrows, p->nk, (long long*)p->cnt_red.p))
-        return -1;
-    std::vector<double> pk(2 * (size_t)p->nk);
-    std::vector<long long> cnt(p->nk);
+    std::vector<double> pk(2 * (size_t)nbin);
+    std::vector<long long> cnt(nbin);
     HM_HIP_CHECK(ctx, hipMemcpyAsync(pk.data(), p->red.p, sizeof(double) * pk.size(), hipMemcpyDeviceToHost, ctx->stream));
-    HM_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), p->cnt_red.p, sizeof(long long) * p->nk, hipMemcpyDeviceToHost,
+    HM_HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), p->cnt_red.p, sizeof(long long) * nbin, hipMemcpyDeviceToHost,
                                      ctx->stream));
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const double norm = vs / swD / swD;  // particle.hpp:302-313
+    // particle.hpp:302-313 on P_D, or on P_R for the window (powerspec.hpp:249)
+    const double norm = window ? vs / swR / swR : vs / swD / swD;
     if (pk_out)
-        for (int b = 0; b < p->nk; b++) {
+        for (int b = 0; b < nbin; b++) {
             pk_out[2 * b] = cnt[b] ? norm * (pk[2 * b] / (double)cnt[b]) : 0.0;
             pk_out[2 * b + 1] = cnt[b] ? norm * (pk[2 * b + 1] / (double)cnt[b]) : 0.0;
         }
     if (nmode_out)
-        for (int b = 0; b < p->nk; b++) nmode_out[b] = cnt[b];
+        for (int b = 0; b < nbin; b++) nmode_out[b] = cnt[b];
     if (info) {
         info->wsum_d = swD;
         info->wsum_r = swR;

@@ -151,13 +151,22 @@ int hm_box_plan_triple_mesh(const hm_box_plan *plan, int m[3]);
 /* ---------------------------------------------------------------- */
 typedef struct hm_survey_plan hm_survey_plan;
 
+enum {
+    HM_SURVEY_PK = 0,        /* measure = pk: calcPowerSpectrum (powerspec.hpp:4-91) */
+    HM_SURVEY_XI = 1,        /* measure = 2PCF: calcTwoPointCorrelationFunction (powerspec.hpp:93-179) */
+    HM_SURVEY_XI_WINDOW = 2  /* measure = window2PCF: calcTwoPointWindowCorrelationFunction (powerspec.hpp:182-270) */
+};
+
 typedef struct {
     hm_mesh mesh;      /* FFT box: n_mesh and boxsize from the parameter file */
     int precision;     /* HM_FP32 or HM_FP64 grids and FFTs; FP64 sums */
     int n_kbin;        /* tools.hpp:118-124 */
     double kmin, kmax;
     int ell;           /* ELL = ell1, ell2 = 0 (calcPowerSpectrum, powerspec.hpp:11-15) */
-    int prune;         /* 1: bin only the modes that can reach a k-bin */
+    int prune;         /* 1: bin only the modes (separations) that can reach a bin */
+    int measure;       /* HM_SURVEY_PK, HM_SURVEY_XI or HM_SURVEY_XI_WINDOW */
+    int n_rbin;        /* tools.hpp:126-132, for the correlation functions */
+    double rmin, rmax;
 } hm_survey_config;
 
 /* The survey quantities main.cpp derives before measuring. */
@@ -167,17 +176,19 @@ typedef struct {
     double vsurvey;        /* (sum w_R)^2 / sum_x rho_R^2 dV (field.hpp:449-503) */
     double nbar;           /* (sum w_D)^2 / Vsurvey / sum w_D^2 (particle.hpp:284-299) */
     double shift[3];       /* min(R) - one cell, subtracted from every position (particle.hpp:222-265) */
-    double norm;           /* Vsurvey / (sum w_D)^2 (particle.hpp:302-313) */
+    double norm;           /* Vsurvey / (sum w)^2 of D, or of R for the window (particle.hpp:302-313) */
 } hm_survey_info;
 
 hm_survey_plan *hm_survey_plan_create(hm_ctx *ctx, const hm_survey_config *cfg);
 void hm_survey_plan_destroy(hm_ctx *ctx, hm_survey_plan *plan);
-/* calcPowerSpectrum (powerspec.hpp:4-91) after main.cpp's preparation
- * (main.cpp:109-166): d_pos_* are the unshifted Cartesian positions (n x 3
- * FP64; the line of sight is pos/|pos| of these), d_w_* the weights (FP64).
- * pk: [n_kbin][2] = norm * pk_save as written to pk%d.dat; nmode: [n_kbin]. */
+/* The configured measurement after main.cpp's preparation (main.cpp:109-166):
+ * d_pos_* are the unshifted Cartesian positions (n x 3 FP64; the line of
+ * sight is pos/|pos| of these), d_w_* the weights (FP64).
+ * out: [n_bins][2] = norm * pk_save (pk%d.dat) or norm * xi_save (whose real
+ * part xi%d.dat and xi%d_window.dat hold); nmode: [n_bins] modes or mesh
+ * points per bin. */
 int hm_survey_plan_execute(hm_ctx *ctx, hm_survey_plan *plan, const double *d_pos_d, const double *d_w_d, long long n_d,
-                           const double *d_pos_r, const double *d_w_r, long long n_r, hm_survey_info *info, double *pk,
+                           const double *d_pos_r, const double *d_w_r, long long n_r, hm_survey_info *info, double *out,
                            long long *nmode);
 size_t hm_survey_plan_bytes(const hm_survey_plan *plan);
 

@@ -1273,3 +1273,69 @@ int or_survey_pk(const or_param *p, const double *posD, const double *losD, cons
     free(d00);
     return 0;
 }
+
+
+/* field.hpp:230-247: R alone with w Y_LM(los) */
+static void ylm_weighted_density(const or_param *p, const double *posR, const double *losR, const double *wR,
+                                 long long nR, int ell, int M, double *field) {
+    double *w = (double *)malloc(sizeof(double) * 2 * (size_t)(nR > 0 ? nR : 1));
+    double y[2];
+    for (long long q = 0; q < nR; q++) {
+        or_ylm(ell, M, losR + 3 * q, y);
+        w[2 * q] = y[0] * wR[q];
+        w[2 * q + 1] = y[1] * wR[q];
+    }
+    or_calc_field(p, posR, w, nR, field);
+    free(w);
+}
+
+/* powerspec.hpp:93-179 (window = 0) and 182-270 (window = 1): norm * xi_save (2*n_rbin, complex);
+ * the files hold the real part. */
+int or_survey_xi(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                 const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                 const double *rbin, int window, double *out, int *n_mode) {
+    if (!((p->ELL == p->ell1) && (p->ell2 == 0))) return -1;
+    const int nr = p->n_rbin;
+    double *d00 = grid_alloc(p), *dLM = grid_alloc(p);
+    if (window) ylm_weighted_density(p, posR, losR, wR, nR, 0, 0, d00);
+    else or_ylm_weighted_fluctuation(p, posD, losD, wD, nD, posR, losR, wR, nR, alpha, 0, 0, d00);
+    or_fourier_transform(p, d00);
+    double *xs = (double *)calloc(2 * (size_t)nr, sizeof(double));
+    double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)nr);
+    for (int M = -p->ELL; M <= p->ELL; M++) {
+        double sn[2];
+        if (window) {
+            ylm_weighted_density(p, posR, losR, wR, nR, p->ELL, M, dLM);
+            /* field.hpp:1202-1218 */
+            double y[2];
+            sn[0] = sn[1] = 0.0;
+            for (long long q = 0; q < nR; q++) {
+                or_ylm(p->ELL, M, losR + 3 * q, y);
+                sn[0] += pow(wR[q], 2) * y[0];
+                sn[1] += pow(wR[q], 2) * y[1];
+            }
+        } else {
+            or_ylm_weighted_fluctuation(p, posD, losD, wD, nD, posR, losR, wR, nR, alpha, p->ELL, M, dLM);
+            or_shotnoise_pk(losD, wD, nD, losR, wR, nR, alpha, p->ELL, M, sn);
+        }
+        or_fourier_transform(p, dLM);
+        for (int m1 = -p->ell1; m1 <= p->ell1; m1++) {
+            double w = or_wigner3j(p->ell1, 0, p->ELL, 0, 0, 0) * or_wigner3j(p->ell1, 0, p->ELL, m1, 0, M);
+            w *= (double)(2 * p->ELL + 1) * (double)(2 * p->ell1 + 1);
+            if (fabs(w) < 1.0e-10) continue;
+            or_calc_correlation_function(p, dLM, d00, rbin, sn[0], sn[1], p->ell1, m1, tmp, n_mode);
+            for (int i = 0; i < nr; i++) {
+                xs[2 * i] += w * tmp[2 * i];
+                xs[2 * i + 1] += w * tmp[2 * i + 1];
+            }
+        }
+    }
+    const double sw = window ? weight_sum(wR, nR) : weight_sum(wD, nD);
+    const double norm = vs / sw / sw; /* particle.hpp:302-313 on P_D, or on P_R for the window */
+    for (int i = 0; i < 2 * nr; i++) out[i] = norm * xs[i];
+    free(tmp);
+    free(xs);
+    free(dLM);
+    free(d00);
+    return 0;
+}

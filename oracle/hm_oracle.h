@@ -117,6 +117,11 @@ int or_survey_pk(const or_param *p, const double *posD, const double *losD, cons
                  const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
                  const double *kbin, double *out, int *n_mode);
 
+/* powerspec.hpp:93-179 (window 0) / 182-270 (window 1): norm * xi_save (2*n_rbin). */
+int or_survey_xi(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                 const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                 const double *rbin, int window, double *out, int *n_mode);
+
 /* Row-major index helper used by the tests. */
 long long or_coord(const or_param *p, int i, int j, int k);
 

@@ -76,6 +76,8 @@ def lib():
                                        _D]),
             "or_survey_pk": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, _D, ct.POINTER(ct.c_int)]),
+            "or_survey_xi": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
+                                        ct.c_double, _D, ct.c_int, _D, ct.POINTER(ct.c_int)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -318,4 +320,14 @@ class Survey:
                                 nm.ctypes.data_as(ct.POINTER(ct.c_int)))
         if rc:
             raise ValueError("or_survey_pk rejected the multipole combination")
+        return out[0::2] + 1j * out[1::2], nm
+
+    def xi(self, p: Param, window=False):
+        rb = rbin(p)
+        out = np.zeros(2 * p.n_rbin)
+        nm = np.zeros(p.n_rbin, dtype=np.int32)
+        rc = lib().or_survey_xi(ct.byref(p), *self._args(), float(self.alpha), float(self.vsurvey), dp(rb),
+                                1 if window else 0, dp(out), nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+        if rc:
+            raise ValueError("or_survey_xi rejected the multipole combination")
         return out[0::2] + 1j * out[1::2], nm

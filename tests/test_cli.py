@@ -123,3 +123,88 @@ def test_end_to_end_pk_and_bk_match_oracle(tmp_path):
         ref = po.box_xi(p, posd)
         assert np.allclose(got[:, 0], po.rbin(p), atol=5e-6)
         assert np.allclose(got[:, 1], ref, rtol=2e-7, atol=1e-7 * np.abs(ref).max())
+
+
+SURVEY = os.path.join(ROOT, "hitomi_amd", "bin", "hitomi_survey")
+
+
+def run_survey(ini, cwd):
+    if not os.path.exists(SURVEY):
+        pytest.fail("hitomi_survey is not built: run __graft_entry__.build()")
+    return subprocess.run([SURVEY, str(ini)], cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def write_xyzw(path, pos, w):
+    with open(path, "w") as f:
+        f.write("# x y z w\n")
+        for (x, y, z), ww in zip(pos, w):
+            f.write(f"{x:.10e} {y:.10e} {z:.10e} {ww:.10e}\n")
+
+
+def test_survey_missing_randoms_exits_1(tmp_path):
+    write_xyzw(tmp_path / "data.dat", np.ones((3, 3)) * 500.0, np.ones(3))
+    ini = tmp_path / "p.ini"
+    hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="nothere.dat",
+                  output_dir=str(tmp_path / "out"))
+    r = run_survey(ini, tmp_path)
+    assert r.returncode == 1 and "Cannot open the file" in r.stdout and "'P_R.readParticleBOSS' fails." in r.stdout
+
+
+def test_survey_unsupported_measure_exits_1(tmp_path):
+    write_xyzw(tmp_path / "data.dat", np.ones((3, 3)) * 500.0, np.ones(3))
+    ini = tmp_path / "p.ini"
+    hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="data.dat",
+                  output_dir=str(tmp_path / "out"), measure="3PCF")
+    r = run_survey(ini, tmp_path)
+    assert r.returncode == 1 and "not available" in r.stdout
+
+
+@pytest.mark.gpu
+def test_survey_end_to_end_matches_oracle(tmp_path):
+    from oracle import pyoracle as po
+
+    rng = np.random.default_rng(5)
+    lo, size = np.array([600.0, 900.0, 300.0]), 500.0
+    posD = lo + rng.random((2500, 3)) * size
+    posR = lo + rng.random((10000, 3)) * size
+    wD = 1.0 / (1.0 + rng.random(2500))
+    wR = 1.0 / (1.0 + rng.random(10000))
+    write_xyzw(tmp_path / "data.dat", posD, wD)
+    write_xyzw(tmp_path / "rand.dat", posR, wR)
+    # what the files hold, as the reader parses them
+    rd = lambda f: np.loadtxt(f, comments="#")
+    D, R = rd(tmp_path / "data.dat"), rd(tmp_path / "rand.dat")
+    n, L = 32, 700.0
+    for ell in (0, 2):
+        ini = tmp_path / f"pk{ell}.ini"
+        hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="rand.dat",
+                      output_dir=str(tmp_path / "out"), measure="pk", ell1=ell, ELL=ell, boxsize_x=L, boxsize_y=L,
+                      boxsize_z=L, n_mesh_x=n, n_mesh_y=n, n_mesh_z=n, kmin=0.02, kmax=0.12, n_kbin=6,
+                      precision="FP64")
+        r = run_survey(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        got = hio.read_table(tmp_path / "out" / f"pk{ell}.dat")
+        p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.12, n_kbin=6, ell1=ell, ELL=ell)
+        s = po.Survey(p, D[:, :3], D[:, 3], R[:, :3], R[:, 3])
+        ref, _ = s.pk(p)
+        assert np.allclose(got[:, 0], po.kbin(p), atol=5e-6)
+        assert np.allclose(got[:, 1], ref.real, rtol=2e-7, atol=1e-7 * np.abs(ref.real).max())
+        assert np.allclose(got[:, 2], ref.imag, rtol=2e-7, atol=1e-6 * np.abs(ref.real).max())
+    for measure, fname in (("2PCF", "xi2.dat"), ("window2PCF", "xi2_window.dat")):
+        ini = tmp_path / f"{measure}.ini"
+        hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="rand.dat",
+                      output_dir=str(tmp_path / "out"), measure=measure, ell1=2, ELL=2, boxsize_x=L, boxsize_y=L,
+                      boxsize_z=L, n_mesh_x=n, n_mesh_y=n, n_mesh_z=n, rmin=40.0, rmax=200.0, n_rbin=5,
+                      precision="FP64")
+        r = run_survey(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        got = hio.read_table(tmp_path / "out" / fname)
+        p = po.param(n, L, po.TSC, rmin=40.0, rmax=200.0, n_rbin=5, ell1=2, ELL=2)
+        s = po.Survey(p, D[:, :3], D[:, 3], R[:, :3], R[:, 3])
+        ref, _ = s.xi(p, window=measure == "window2PCF")
+        assert np.allclose(got[:, 0], po.rbin(p), atol=5e-6)
+        assert np.allclose(got[:, 1], ref.real, rtol=2e-7, atol=1e-7 * np.abs(ref.real).max())
+    txt = (tmp_path / "out" / "survey_parameters").read_text()
+    assert f"P_D.n_tot = {len(D)}" in txt and f"P_R.n_tot = {len(R)}" in txt
+    assert f"alpha = {s.alpha:.5f}" in txt
+    assert f"nbar / 1.0e-4 [(Mpc/h)^{{-3}}] = {s.nbar / 1e-4:.5f}" in txt

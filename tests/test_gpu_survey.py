@@ -89,3 +89,45 @@ def test_survey_pk(ctx, case, ell, prec):
     scale = np.abs(ref0) + sn0
     err = np.abs(got - ref)
     assert np.all(err <= TOL[prec] * scale), (err / scale).max()
+
+
+XI_CASES = [
+    # name, n, L, assign, rmin, rmax, nr, prune
+    ("patch", 32, 800.0, "TSC", 40.0, 240.0, 9, True),
+    ("west", 32, 700.0, "CIC", 30.0, 450.0, 8, True),   # bins reach past half the box: full mesh
+    ("patch", 48, 800.0, "TSC", 40.0, 200.0, 5, False),
+]
+
+_XCACHE = {}
+
+
+def oracle_xi(n, L, assign, rmin, rmax, nr, ell, window, name):
+    key = (n, L, assign, rmin, rmax, nr, ell, window, name)
+    if key not in _XCACHE:
+        posD, wD, posR, wR = get_cat(name)
+        p = po.param(n, L, assign, rmin=rmin, rmax=rmax, n_rbin=nr, ell1=ell, ELL=ell)
+        s = po.Survey(p, posD, wD, posR, wR)
+        xi, nm = s.xi(p, window=window)
+        p0 = po.param(n, L, assign, rmin=rmin, rmax=rmax, n_rbin=nr)
+        xi0, _ = s.xi(p0, window=window)
+        _XCACHE[key] = (s, xi, nm, xi0)
+    return _XCACHE[key]
+
+
+@pytest.mark.parametrize("case", XI_CASES)
+@pytest.mark.parametrize("ell", [0, 1, 2])
+@pytest.mark.parametrize("window", [False, True])
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+def test_survey_xi(ctx, case, ell, window, prec):
+    """xi_l: |dxi_l(r)| <= tol * max_r |xi_0(r)| (tol 1e-10 FP64, 2e-5 FP32); points per bin bit-exact."""
+    name, n, L, assign, rmin, rmax, nr, prune = case
+    code = {"NGP": po.NGP, "CIC": po.CIC, "TSC": po.TSC}[assign]
+    s, ref, nm, ref0 = oracle_xi(n, L, code, rmin, rmax, nr, ell, window, name)
+    plan = hm.SurveyPlan(ctx, n, L, assign, ell=ell, precision=prec, prune=prune,
+                         measure="window2PCF" if window else "2PCF", n_rbin=nr, rmin=rmin, rmax=rmax)
+    got, gnm, info = plan.execute(*get_cat(name))
+    plan.close()
+    assert np.array_equal(gnm, nm.astype(np.int64))
+    scale = np.abs(ref0).max()
+    err = np.abs(got - ref).max()
+    assert err <= TOL[prec] * scale, (err, scale)

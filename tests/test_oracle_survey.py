@@ -110,3 +110,67 @@ def test_survey_pk_equals_direct_yamamoto_sum(ell):
         assert nm[j] == sel.sum()
         expect = norm * val[sel].sum() / sel.sum()
         assert abs(got[j] - expect) <= 1e-10 * abs(norm * val[sel]).mean() + 1e-300
+
+
+@pytest.mark.parametrize("ell,window", [(0, False), (2, False), (1, False), (2, True)])
+def test_survey_xi_equals_direct_sum(ell, window):
+    """powerspec.hpp:93-270: sum_M w_M Y_{l,-M}(r) IFFT[(F_LM conj F_00 - SN_M S)/W^2/V]
+    equals, by the addition theorem, sum_k e^{ik.r} [(2l+1) A_l(r, k) conj F_00(k)
+    - (2l+1) SN_l(r) S(k)] / W(k)^2 / V with A_l(r, k) = sum_p w_p L_l(r.los_p) S_p(k),
+    evaluated here point by point (window: the random catalogue alone)."""
+    n, L = 10, 500.0
+    p = po.param(n, L, po.TSC, rmin=40.0, rmax=240.0, n_rbin=6, ell1=ell, ELL=ell)
+    posD, wD, posR, wR = survey_catalogue(100, 400, np.array([300.0, -700.0, 500.0]), 330.0, seed=11 + ell)
+    s = po.Survey(p, posD, wD, posR, wR)
+    got, nm = s.xi(p, window=window)
+
+    idx = np.arange(n)
+    sgn = np.where(idx < n // 2, idx, idx - n)
+    grid = np.stack(np.meshgrid(sgn, sgn, sgn, indexing="ij"), -1).reshape(-1, 3)
+    K = grid * (2 * np.pi / L)
+    Rv = grid * (L / n)
+    rmag = np.linalg.norm(Rv, axis=1)
+    rhat = np.where(rmag[:, None] > 0, Rv / np.where(rmag > 0, rmag, 1)[:, None], 0.0)
+    dx = L / n
+
+    def stencil_phase(pos):
+        cells, wts = tsc_stencil(p, pos)
+        assert np.all((cells >= 0) & (cells < n ** 3))
+        ci = np.stack(np.unravel_index(cells, (n, n, n)), -1) * dx
+        ph = np.exp(-1j * np.einsum("kc,pqc->pqk", K, ci))
+        return np.einsum("pq,pqk->pk", wts, ph)
+
+    def leg(los):
+        out = eval_legendre(ell, rhat @ los.T) if ell else np.ones((len(Rv), len(los)))
+        if ell:
+            out[rmag == 0] = 0.0
+        return out  # (N_r, N_p)
+
+    SR, LR = stencil_phase(s.posR), leg(s.losR)
+    if window:
+        A = (LR * wR[None, :]) @ SR
+        F00 = wR @ SR
+        SN = (LR * (wR ** 2)[None, :]).sum(1)
+        sw = wR.sum()
+    else:
+        SD, LD = stencil_phase(s.posD), leg(s.losD)
+        A = (LD * wD[None, :]) @ SD - s.alpha * ((LR * wR[None, :]) @ SR)
+        F00 = wD @ SD - s.alpha * (wR @ SR)
+        SN = (LD * (wD ** 2)[None, :]).sum(1) + s.alpha ** 2 * (LR * (wR ** 2)[None, :]).sum(1)
+        sw = wD.sum()
+    W = np.array([po.assignment_function(p, k) for k in K])
+    S = np.array([po.shotnoise_function(p, k) for k in K])
+    g = (2 * ell + 1) * (A * np.conj(F00)[None, :] - SN[:, None] * S[None, :]) / W[None, :] ** 2 / L ** 3
+    xi = (g * np.exp(1j * (Rv @ K.T))).sum(1)
+    centre = (rmag / 0.5 + 0.5).astype(int) * 0.5
+    rb = po.rbin(p)
+    drb = rb[1] - rb[0]
+    norm = s.vsurvey / sw ** 2
+    for j, c in enumerate(rb):
+        lo = c - drb * 0.5 if c > drb * 0.5 else 0.0
+        sel = (centre > lo) & (centre <= c + drb * 0.5)
+        assert nm[j] == sel.sum()
+        if sel.sum() == 0:
+            continue
+        expect = norm * xi[sel].sum() / sel.sum()
+        assert abs(got[j] - expect) <= 1e-10 * norm * np.abs(xi[sel]).mean()
