@@ -103,6 +103,15 @@ class SurveyInfo(ct.Structure):
     ]
 
 
+class ReconConfig(ct.Structure):
+    _fields_ = [
+        ("mesh", Mesh),
+        ("precision", ct.c_int),
+        ("b1", ct.c_double),
+        ("RG", ct.c_double),
+    ]
+
+
 # Every symbol include/hitomi_amd.h declares, with (restype, argtypes).
 _VP = ct.c_void_p
 _SIGS = {
@@ -129,8 +138,13 @@ _SIGS = {
     "hm_box_plan_create": (_VP, [_VP, ct.POINTER(BoxConfig)]),
     "hm_box_plan_destroy": (None, [_VP, _VP]),
     "hm_box_plan_execute": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.POINTER(BoxResult)]),
+    "hm_box_plan_execute_recon": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, _VP, ct.c_longlong, ct.c_double,
+                                             ct.POINTER(BoxResult)]),
     "hm_box_plan_bytes": (ct.c_size_t, [_VP]),
     "hm_box_plan_triple_mesh": (ct.c_int, [_VP, ct.POINTER(ct.c_int)]),
+    "hm_recon_displace": (ct.c_int, [_VP, ct.POINTER(ReconConfig), _VP, _VP, ct.c_longlong, _VP, _VP, ct.c_longlong,
+                                     ct.c_double, ct.c_double]),
+    "hm_periodic_wrap": (ct.c_int, [_VP, _VP, ct.c_longlong, ct.POINTER(ct.c_double)]),
     "hm_survey_plan_create": (_VP, [_VP, ct.POINTER(SurveyConfig)]),
     "hm_survey_plan_destroy": (None, [_VP, _VP]),
     "hm_survey_plan_execute": (ct.c_int, [_VP, _VP, _VP, _VP, ct.c_longlong, _VP, _VP, ct.c_longlong,
@@ -288,6 +302,30 @@ class Context:
         assert out_re.size == ng
         return out_re, out_im
 
+    # a28: reconstruction
+    def recon_displace(self, n_mesh, boxsize, pos_d, pos_r, alpha, norm, b1, RG, w_d=None, w_r=None, assign="TSC",
+                       precision=FP64):
+        """particle_reconstruction.hpp: (displaced D, displaced R) as host arrays."""
+        cfg = ReconConfig()
+        cfg.mesh = make_mesh(n_mesh, boxsize, assign)
+        cfg.precision = precision
+        cfg.b1 = float(b1)
+        cfg.RG = float(RG)
+        dD = self.array(np.ascontiguousarray(pos_d, dtype=np.float64).reshape(-1, 3))
+        dR = self.array(np.ascontiguousarray(pos_r, dtype=np.float64).reshape(-1, 3))
+        wd = self.array(np.ascontiguousarray(w_d, dtype=np.float64)) if w_d is not None else None
+        wr = self.array(np.ascontiguousarray(w_r, dtype=np.float64)) if w_r is not None else None
+        self.check(self.lib.hm_recon_displace(self.h, ct.byref(cfg), dD.ptr, wd.ptr if wd else None, dD.shape[0], dR.ptr,
+                                              wr.ptr if wr else None, dR.shape[0], float(alpha), float(norm)),
+                   "hm_recon_displace")
+        return dD.download(), dR.download()
+
+    def periodic_wrap(self, pos, boxsize):
+        d = self.array(np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3))
+        L = (ct.c_double * 3)(*(list(boxsize) if np.ndim(boxsize) else [float(boxsize)] * 3))
+        self.check(self.lib.hm_periodic_wrap(self.h, d.ptr, d.shape[0], L), "hm_periodic_wrap")
+        return d.download()
+
     def cell_coords(self, mesh: Mesh, pos):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         s = {NGP: 1, CIC: 8, TSC: 27}[mesh.assign]
@@ -383,7 +421,7 @@ class BoxPlan:
         self.ctx.lib.hm_box_plan_triple_mesh(self.h, m)
         return tuple(m)
 
-    def execute(self, pos, npart=None):
+    def execute(self, pos, npart=None, recon=None):
         """pos: host (N,3) float64 array or a DeviceArray of it."""
         if isinstance(pos, DeviceArray):
             dpos = pos
@@ -413,9 +451,21 @@ class BoxPlan:
         r.xi = out["xi"].ctypes.data_as(dp)
         r.xi_raw = out["xi_raw"].ctypes.data_as(dp)
         r.xi_nmode = out["xi_nmode"].ctypes.data_as(lp)
-        self.ctx.check(self.ctx.lib.hm_box_plan_execute(self.ctx.h, self.h, dpos.ptr, int(npart), ct.byref(r)),
-                       "hm_box_plan_execute")
+        if recon is None:
+            self.ctx.check(self.ctx.lib.hm_box_plan_execute(self.ctx.h, self.h, dpos.ptr, int(npart), ct.byref(r)),
+                           "hm_box_plan_execute")
+        else:
+            dr, alpha = recon
+            self.ctx.check(self.ctx.lib.hm_box_plan_execute_recon(self.ctx.h, self.h, dpos.ptr, int(npart), dr.ptr,
+                                                                  dr.shape[0], float(alpha), ct.byref(r)),
+                           "hm_box_plan_execute_recon")
         return out
+
+    def execute_recon(self, pos_d, pos_r, alpha):
+        """The reconstructed box: D - alpha R (powerspec.hpp:396-519)."""
+        dr = pos_r if isinstance(pos_r, DeviceArray) else self.ctx.array(
+            np.ascontiguousarray(pos_r, dtype=np.float64).reshape(-1, 3))
+        return self.execute(pos_d, recon=(dr, alpha))
 
 
 class SurveyPlan:

@@ -3,6 +3,7 @@
 // reports them through hm_last_error().
 #include "hm_assign.hpp"
 #include "hm_internal.hpp"
+#include "hm_recon.hpp"
 
 #include "../../include/hitomi_amd.h"
 
@@ -14,7 +15,8 @@ namespace hm {
 struct BoxPlanImpl;
 BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg);
 void plan_destroy(BoxPlanImpl* p);
-int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res);
+int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res,
+                     const double* d_pos_r, long long nr, double alpha);
 size_t plan_bytes(const BoxPlanImpl* p);
 void plan_triple_mesh(const BoxPlanImpl* p, int m[3]);
 int cell_coords(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, long long np, long long* d_out);
@@ -313,9 +315,21 @@ int hm_box_plan_execute(hm_ctx* ctx, hm_box_plan* plan, const double* d_pos, lon
     if (!ctx) return -1;
     if (!plan || !d_pos) return bad(ctx, "hm_box_plan_execute: bad arguments");
     try {
-        return hm::plan_execute_any(ctx, plan->impl, d_pos, np, res);
+        return hm::plan_execute_any(ctx, plan->impl, d_pos, np, res, nullptr, 0, 0.0);
     } catch (const std::exception& e) {
         ctx->set_error(std::string("hm_box_plan_execute: ") + e.what());
+        return -1;
+    }
+}
+
+int hm_box_plan_execute_recon(hm_ctx* ctx, hm_box_plan* plan, const double* d_pos_d, long long n_d,
+                              const double* d_pos_r, long long n_r, double alpha, hm_box_result* res) {
+    if (!ctx) return -1;
+    if (!plan || !d_pos_d || !d_pos_r || n_r <= 0) return bad(ctx, "hm_box_plan_execute_recon: bad arguments");
+    try {
+        return hm::plan_execute_any(ctx, plan->impl, d_pos_d, n_d, res, d_pos_r, n_r, alpha);
+    } catch (const std::exception& e) {
+        ctx->set_error(std::string("hm_box_plan_execute_recon: ") + e.what());
         return -1;
     }
 }
@@ -362,5 +376,44 @@ int hm_survey_plan_execute(hm_ctx* ctx, hm_survey_plan* plan, const double* d_po
 }
 
 size_t hm_survey_plan_bytes(const hm_survey_plan* plan) { return plan ? hm::survey_bytes(plan->impl) : 0; }
+
+int hm_recon_displace(hm_ctx* ctx, const hm_recon_config* cfg, double* d_pos_d, const double* d_w_d, long long n_d,
+                      double* d_pos_r, const double* d_w_r, long long n_r, double alpha, double norm) {
+    if (!ctx) return -1;
+    if (!cfg) return bad(ctx, "reconstruction: config is NULL");
+    if ((n_d > 0 && !d_pos_d) || (n_r > 0 && !d_pos_r) || n_d < 0 || n_r < 0)
+        return bad(ctx, "reconstruction: bad particle arrays");
+    hm::AssignGeom g;
+    if (!mesh_ok(ctx, &cfg->mesh, g)) return -1;
+    for (int a = 0; a < 3; a++)
+        if (!hm::fft_length_supported(g.n[a])) return bad(ctx, "reconstruction: unsupported n_mesh_recon");
+    hm::Mesh m;
+    for (int a = 0; a < 3; a++) {
+        m.n[a] = g.n[a];
+        m.L[a] = g.L[a];
+    }
+    m.assign = g.assign;
+    hm::ReconWork w;
+    int rc;
+    try {
+        rc = cfg->precision == HM_FP64
+                 ? hm::recon_displace<double>(ctx, m, cfg->b1, cfg->RG, d_pos_d, d_w_d, n_d, d_pos_r, d_w_r, n_r, alpha,
+                                              norm, w)
+                 : hm::recon_displace<float>(ctx, m, cfg->b1, cfg->RG, d_pos_d, d_w_d, n_d, d_pos_r, d_w_r, n_r, alpha,
+                                             norm, w);
+        if (rc == 0 && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = bad(ctx, "reconstruction: stream error");
+    } catch (const std::exception& e) {
+        rc = bad(ctx, e.what());
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    hm::recon_work_free(w);
+    return rc;
+}
+
+int hm_periodic_wrap(hm_ctx* ctx, double* d_pos, long long n, const double boxsize[3]) {
+    if (!ctx) return -1;
+    if (!boxsize || (n > 0 && !d_pos)) return bad(ctx, "periodic wrap: NULL argument");
+    return hm::periodic_wrap(ctx, d_pos, n, boxsize);
+}
 
 }  // extern "C"

@@ -69,6 +69,21 @@ template <typename T> struct FixedRealSrc {
     }
 };
 
+// fixed point -> complex grid times dV (calcFourierTransform, field.hpp:313-318)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fixed_to_complex(const long long* __restrict__ re,
+                                                          const long long* __restrict__ im, long long n,
+                                                          double inv_scale, double factor, double dV,
+                                                          C<T>* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long st = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += st) {
+        const double a = (((double)re[i] * inv_scale) * factor) * dV;
+        const double b = im ? (((double)im[i] * inv_scale) * factor) * dV : 0.0;
+        out[i] = mk<T>((T)a, (T)b);
+    }
+}
+
 // Accumulate particles into a zeroed fixed-point grid (re and optional im)
 // with one 64-bit atomic per stencil cell.
 int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,

@@ -19,6 +19,7 @@
 #include "hm_assign.hpp"
 #include "hm_bins.hpp"
 #include "hm_fft_launch.hpp"
+#include "hm_recon.hpp"
 
 #include "../../include/hitomi_amd.h"
 
@@ -659,6 +660,7 @@ struct BoxPlanImpl {
     int ngrp = 0, nblk = 0, nsl = 0;  // 4-shell groups, group pairs, z slices per pair block (fused)
     int gram_cols = 0;                // doubles per partial row of the triple sums
     DevBuf sid;                       // char2 first/last shell per mode of the shell support cube
+    DevBuf rpos, rw;                  // D then R positions and weights of a reconstructed box
     // xi_l
     int nr = 0, nxe = 0;
     std::vector<double> rbin, rlower, rupper;
@@ -964,7 +966,7 @@ void plan_destroy(BoxPlanImpl* p) {
     DevBuf* bufs[] = {&p->tab, &p->f2b_d, &p->lu_d, &p->kb_d, &p->fixed, &p->dk, &p->pk_part, &p->pk_cnt, &p->pk_red,
                       &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
                       &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid,
-                      &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red};
+                      &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red, &p->rpos, &p->rw};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -1044,7 +1046,11 @@ static int field_transform(hm_ctx* ctx, BoxPlanImpl* p, const ShellSrc<T>& src, 
 }
 
 template <typename T>
-static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res) {
+// d_pos_r / nr / alpha: the reconstructed box (main_box.cpp:199-228): the
+// density is D - alpha R with unit weights (field.hpp:528-558) instead of
+// rho - nbar, the two-point shot noise N_D + alpha^2 N_R (field.hpp:1190-1197).
+static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res,
+                        const double* d_pos_r = nullptr, long long nr = 0, double alpha = 0.0) {
     const Mesh& m = p->mesh;
     const AxisTabs tabs = axis_tabs(p);
     const int wp = wpow_of(m.assign);
@@ -1060,16 +1066,35 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     }
     ag.ng = m.ng();
     ag.assign = m.assign;
-    const double scale = fixed_scale_for((double)np);
-    {
-        const size_t before = p->sort.bytes();
-        if (sort_work_alloc(ctx, p->sort, ag, np, false)) return -1;
-        p->bytes += p->sort.bytes() - before;
+    const bool recon = nr > 0;
+    if (recon && p->cfg.bk) {
+        ctx->set_error("box execute: the bispectrum after reconstruction is not available");
+        return -1;
     }
-    if (assign_sorted(ctx, ag, d_pos, nullptr, np, scale, (long long*)p->fixed.p, nullptr, p->sort)) return -1;
+    const double scale = fixed_scale_for(recon ? (double)np + std::fabs(alpha) * (double)nr : (double)np);
+    {
+        const size_t before = p->sort.bytes() + p->rpos.bytes + p->rw.bytes;
+        if (sort_work_alloc(ctx, p->sort, ag, np + nr, recon)) return -1;
+        if (recon && (dev_alloc(ctx, p->rpos, sizeof(double) * 3 * (size_t)(np + nr)) ||
+                      dev_alloc(ctx, p->rw, sizeof(double) * 2 * (size_t)(np + nr))))
+            return -1;
+        p->bytes += p->sort.bytes() + p->rpos.bytes + p->rw.bytes - before;
+    }
+    if (recon) {
+        if (pack_two(ctx, d_pos, nullptr, np, d_pos_r, nullptr, nr, alpha, (double*)p->rpos.p, (double*)p->rw.p))
+            return -1;
+        if (assign_sorted(ctx, ag, (const double*)p->rpos.p, (const double*)p->rw.p, np + nr, scale,
+                          (long long*)p->fixed.p, nullptr, p->sort))
+            return -1;
+    } else if (assign_sorted(ctx, ag, d_pos, nullptr, np, scale, (long long*)p->fixed.p, nullptr, p->sort)) {
+        return -1;
+    }
+    // two-point shot noise: N (field.hpp:523) or N_D + alpha^2 N_R (field.hpp:1190-1197)
+    const double sn2 = recon ? (double)np + pow(alpha, 2) * (double)nr : (double)np;
     // 2. forward r2c with the fluctuation (rho - N/V) dV formed on load
     const double dV = m.volume() / (double)m.ng();
-    FixedRealSrc<T> fsrc{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale, 1.0 / dV, (double)np / m.volume(), dV};
+    FixedRealSrc<T> fsrc{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale, 1.0 / dV,
+                         recon ? 0.0 : (double)np / m.volume(), dV};
     C<T>* dk = (C<T>*)p->dk.p;
     const long long nzh = m.nzh();
     if (launch_r2c<T>(ctx, "fft_r2c_z", m.n[2] / 2, fsrc, dk, (long long)m.n[0] * m.n[1])) return -1;
@@ -1094,7 +1119,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     ba.ne = std::max(p->ne, 1);
     for (int e = 0; e < HM_MAX_ELLS; e++) ba.ells[e] = (e < p->ne) ? p->cfg.pk_ells[e] : 0;
     if (p->ne == 0) ba.ells[0] = 0;
-    ba.sn = (double)np;
+    ba.sn = sn2;
     const int ncols = p->nk * ba.ne;
     const bool do_pk = p->ne > 0 || p->cfg.bk;
     if (do_pk) {
@@ -1122,7 +1147,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.nzh = (int)nzh;
         xs.t = tabs;
         xs.wp = wp;
-        xs.sn = (double)np;
+        xs.sn = sn2;
         xs.inv_v = 1.0 / m.volume();
         xs.dc = 0.0;
         C<T>* xh = (C<T>*)p->xi_half.p;
@@ -1362,9 +1387,10 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     return 0;
 }
 
-int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res) {
-    return p->cfg.precision == HM_FP64 ? plan_execute<double>(ctx, p, d_pos, np, res)
-                                       : plan_execute<float>(ctx, p, d_pos, np, res);
+int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res,
+                     const double* d_pos_r, long long nr, double alpha) {
+    return p->cfg.precision == HM_FP64 ? plan_execute<double>(ctx, p, d_pos, np, res, d_pos_r, nr, alpha)
+                                       : plan_execute<float>(ctx, p, d_pos, np, res, d_pos_r, nr, alpha);
 }
 
 size_t plan_bytes(const BoxPlanImpl* p) { return p->bytes; }

@@ -277,21 +277,6 @@ __global__ __launch_bounds__(256) void k_survey_sqsum(const long long* __restric
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-// fixed point -> complex grid times dV (calcFourierTransform, field.hpp:313-318)
-template <typename T>
-__global__ __launch_bounds__(256) void k_survey_to_complex(const long long* __restrict__ re,
-                                                          const long long* __restrict__ im, long long n,
-                                                          double inv_scale, double factor, double dV,
-                                                          C<T>* __restrict__ out) {
-#pragma clang fp contract(off)
-    const long long st = (long long)gridDim.x * 256;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += st) {
-        const double a = (((double)re[i] * inv_scale) * factor) * dV;
-        const double b = im ? (((double)im[i] * inv_scale) * factor) * dV : 0.0;
-        out[i] = mk<T>((T)a, (T)b);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // fused binning over M (calcPowerSpectrum, field.hpp:765-852, summed with the
 // weights of powerspec.hpp:44-59)
@@ -706,7 +691,7 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
         if (arc) return -1;
         C<T>* out = fields + (long long)f * ng;
         if (ctx->launch("survey_to_complex", [&] {
-                hipLaunchKernelGGL(k_survey_to_complex<T>, dim3(rbg), dim3(256), 0, ctx->stream, fre,
+                hipLaunchKernelGGL(k_fixed_to_complex<T>, dim3(rbg), dim3(256), 0, ctx->stream, fre,
                                    cplx ? fim : nullptr, ng, 1.0 / scale, 1.0 / dV, dV, out);
             }))
             return -1;

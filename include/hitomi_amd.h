@@ -141,10 +141,41 @@ hm_box_plan *hm_box_plan_create(hm_ctx *ctx, const hm_box_config *cfg);
 void hm_box_plan_destroy(hm_ctx *ctx, hm_box_plan *plan);
 /* One mock: d_pos (np x 3 FP64, already periodic) -> P_l, xi_l and B_000. */
 int hm_box_plan_execute(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos, long long np, hm_box_result *res);
+/* The reconstructed box (main_box.cpp:199-228, flag_recon = True): P_l and
+ * xi_l of D - alpha R with unit weights (field.hpp:528-558), shot noise
+ * N_D + alpha^2 N_R (field.hpp:1190-1197), norm V / N_D^2 (powerspec.hpp:396-519).
+ * d_pos_d / d_pos_r: displaced and wrapped positions (n x 3 FP64).  A plan
+ * with bk = 1 is rejected (the reconstructed bispectrum is not built). */
+int hm_box_plan_execute_recon(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos_d, long long n_d,
+                              const double *d_pos_r, long long n_r, double alpha, hm_box_result *res);
 /* Device bytes the plan holds. */
 size_t hm_box_plan_bytes(const hm_box_plan *plan);
 /* Mesh used for the triple product (n_mesh when regrid is off). */
 int hm_box_plan_triple_mesh(const hm_box_plan *plan, int m[3]);
+
+/* ---------------------------------------------------------------- */
+/* reconstruction (particle_reconstruction.hpp)                       */
+/* ---------------------------------------------------------------- */
+typedef struct {
+    hm_mesh mesh;      /* n_mesh_recon and boxsize; assign is the density's scheme (the gather is TSC) */
+    int precision;     /* HM_FP32 or HM_FP64 grids and FFTs */
+    double b1;         /* b1_fid */
+    double RG;         /* Gaussian smoothing radius RG [Mpc/h] */
+} hm_recon_config;
+
+/* calcReconstructionParticles (particle_reconstruction.hpp:12-208) and
+ * calcReconstructionParticlesForBOX (210-402): the field of D - alpha R with
+ * weights d_w_* (NULL: 1), transformed and divided by the assignment window,
+ * Psi(k) = -i (G/b1) k/k^2 delta/W / V (0 at k = 0, where the reference leaves
+ * the grid unwritten), transformed back, scaled by norm and TSC-gathered
+ * (real part) onto every particle of both catalogues, in place, without a
+ * wrap.  The box caller passes alpha = double(N_D / N_R) with the reference's
+ * integer division and norm = V / N_D; the survey caller alpha = sum w_D /
+ * sum w_R and norm = Vsurvey / sum w_D. */
+int hm_recon_displace(hm_ctx *ctx, const hm_recon_config *cfg, double *d_pos_d, const double *d_w_d, long long n_d,
+                      double *d_pos_r, const double *d_w_r, long long n_r, double alpha, double norm);
+/* calcPeriodicBoundary (particleGadget.hpp:565-591): one step into [0, L) per axis. */
+int hm_periodic_wrap(hm_ctx *ctx, double *d_pos, long long n, const double boxsize[3]);
 
 /* ---------------------------------------------------------------- */
 /* survey estimators (main.cpp:109-166, measure = pk)                 */

@@ -1339,3 +1339,131 @@ int or_survey_xi(const or_param *p, const double *posD, const double *losD, cons
     free(d00);
     return 0;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* reconstruction (particle_reconstruction.hpp)                         */
+/* ------------------------------------------------------------------ */
+
+/* particle_reconstruction.hpp:56-112, 252-302: Psi_a(x) for a corrected
+ * spectrum dk (delta/W, complex, n_mesh_tot): Psi(k) = -i (G/b) k_a/k^2 dk,
+ * G = exp(-(k RG)^2/2), 0 for |k| < 1e-10 (left unwritten there, Q5), / V,
+ * backward FFT.  psi: 3 complex grids. */
+void or_recon_psi(const or_param *p, const double *dk, double b1, double RG, double *psi) {
+    const long long ng = p->n_mesh_tot;
+    double kvec[3], dkv[3];
+    dkv[0] = 2.0 * M_PI / p->boxsize[0];
+    dkv[1] = 2.0 * M_PI / p->boxsize[1];
+    dkv[2] = 2.0 * M_PI / p->boxsize[2];
+    for (long long i = 0; i < 6 * ng; i++) psi[i] = 0.0;
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dkv, kvec);
+                double kmag = sqrt(kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2]);
+                if (kmag < 1.0e-10) continue;
+                double kmag2 = kmag * kmag;
+                double W = exp(-pow(kmag * RG, 2) / 2.0);
+                double fr = dk[2 * coord], fi = dk[2 * coord + 1];
+                for (int a = 0; a < 3; a++) {
+                    /* (-i) (W/b) (k_a/k^2) f: (c f_im, -c f_re) with c = (W/b)(k_a/k^2) */
+                    double c = (W / b1) * (kvec[a] / kmag2);
+                    psi[2 * (a * ng + coord)] = c * fi;
+                    psi[2 * (a * ng + coord) + 1] = -(c * fr);
+                }
+            }
+    for (long long i = 0; i < 6 * ng; i++) psi[i] /= p->volume;
+    for (int a = 0; a < 3; a++) or_fft3d(p, psi + 2 * a * ng, +1);
+}
+
+/* particle_reconstruction.hpp:141-205, 334-388: TSC gather of Re Psi * norm
+ * added to the positions, linear index range check only (Q1), no wrap. */
+void or_recon_gather(const or_param *p, const double *psi, double norm, double *pos, long long np) {
+    const long long ng = p->n_mesh_tot;
+    for (long long q = 0; q < np; q++) {
+        double w[3][3];
+        int iw[3][3];
+        for (int a = 0; a < 3; a++) {
+            double xp = pos[3 * q + a] * (double)(p->n_mesh[a]) / p->boxsize[a];
+            iw[1][a] = (int)(xp + 0.5);
+            double dx = xp - (double)iw[1][a];
+            w[0][a] = 0.5 * (0.5 - dx) * (0.5 - dx);
+            w[1][a] = 0.75 - dx * dx;
+            w[2][a] = 0.5 * (0.5 + dx) * (0.5 + dx);
+            iw[0][a] = iw[1][a] - 1;
+            iw[2][a] = iw[1][a] + 1;
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) {
+                    long long coord = ((long long)iw[i][0] * p->n_mesh[1] + iw[j][1]) * p->n_mesh[2] + iw[k][2];
+                    if (coord >= 0 && coord < ng) {
+                        double ww = w[i][0] * w[j][1] * w[k][2];
+                        for (int a = 0; a < 3; a++) pos[3 * q + a] += ww * (psi[2 * (a * ng + coord)] * norm);
+                    }
+                }
+    }
+}
+
+/* particle_reconstruction.hpp:12-208 (weights given) and 210-402 (weights NULL: 1):
+ * D - alpha R on p's mesh (the recon mesh), FFT, / W, Psi, gather into both catalogues. */
+int or_recon_displace(const or_param *p, double *posD, const double *wD, long long nD, double *posR,
+                      const double *wR, long long nR, double alpha, double norm, double b1, double RG) {
+    const long long ng = p->n_mesh_tot;
+    double *f = grid_alloc(p), *fR = grid_alloc(p);
+    double *w = (double *)calloc(2 * (size_t)((nD > nR ? nD : nR) + 1), sizeof(double));
+    for (long long q = 0; q < nD; q++) w[2 * q] = wD ? wD[q] : 1.0;
+    or_calc_field(p, posD, w, nD, f);
+    for (long long q = 0; q < nR; q++) w[2 * q] = wR ? wR[q] : 1.0;
+    or_calc_field(p, posR, w, nR, fR);
+    for (long long i = 0; i < ng; i++) {
+        f[2 * i] -= alpha * fR[2 * i];
+        f[2 * i + 1] -= alpha * fR[2 * i + 1];
+    }
+    or_fourier_transform(p, f);
+    or_assignment_correction(p, f);
+    double *psi = (double *)malloc(sizeof(double) * 6 * (size_t)ng);
+    or_recon_psi(p, f, b1, RG, psi);
+    or_recon_gather(p, psi, norm, posD, nD);
+    or_recon_gather(p, psi, norm, posR, nR);
+    free(psi);
+    free(w);
+    free(fR);
+    free(f);
+    return 0;
+}
+
+/* powerspec.hpp:396-456 (xi = 0) and 459-519 (xi = 1): D - alpha R with unit
+ * weights, SN = N_D + alpha^2 N_R (field.hpp:1190-1197), norm V / N_D^2.
+ * out: 2*n_kbin (P) or 2*n_rbin (xi, real part written). */
+int or_box_recon_2pt(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
+                     double alpha, const double *bins, int xi, double *out, int *n_mode) {
+    if (!((p->ELL == p->ell1) && (p->ell2 == 0))) return -1;
+    double *f = grid_alloc(p), *fR = grid_alloc(p);
+    double *w = (double *)calloc(2 * (size_t)((nD > nR ? nD : nR) + 1), sizeof(double));
+    for (long long q = 0; q < nD; q++) w[2 * q] = 1.0;
+    or_calc_field(p, posD, w, nD, f);
+    for (long long q = 0; q < nR; q++) w[2 * q] = 1.0;
+    or_calc_field(p, posR, w, nR, fR);
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        f[2 * i] -= alpha * fR[2 * i];
+        f[2 * i + 1] -= alpha * fR[2 * i + 1];
+    }
+    or_fourier_transform(p, f);
+    double sn = (double)nD + pow(alpha, 2) * (double)nR;
+    int nb = xi ? p->n_rbin : p->n_kbin;
+    double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)nb);
+    if (xi) or_calc_correlation_function(p, f, f, bins, sn, 0.0, p->ELL, 0, tmp, n_mode);
+    else or_calc_power_spectrum(p, f, f, bins, sn, 0.0, p->ELL, 0, tmp, n_mode);
+    double norm = p->volume / (double)nD / (double)nD;
+    for (int i = 0; i < nb; i++) {
+        out[2 * i] = norm * (0.0 + (double)(2 * p->ELL + 1) * tmp[2 * i]);
+        out[2 * i + 1] = norm * (0.0 + (double)(2 * p->ELL + 1) * tmp[2 * i + 1]);
+    }
+    free(tmp);
+    free(w);
+    free(fR);
+    free(f);
+    return 0;
+}

@@ -122,6 +122,15 @@ int or_survey_xi(const or_param *p, const double *posD, const double *losD, cons
                  const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
                  const double *rbin, int window, double *out, int *n_mode);
 
+/* Reconstruction (particle_reconstruction.hpp).  psi: 3 complex grids. */
+void or_recon_psi(const or_param *p, const double *dk, double b1, double RG, double *psi);
+void or_recon_gather(const or_param *p, const double *psi, double norm, double *pos, long long np);
+int or_recon_displace(const or_param *p, double *posD, const double *wD, long long nD, double *posR,
+                      const double *wR, long long nR, double alpha, double norm, double b1, double RG);
+/* powerspec.hpp:396-519: box P_l / xi_l after reconstruction (out complex, times norm). */
+int or_box_recon_2pt(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
+                     double alpha, const double *bins, int xi, double *out, int *n_mode);
+
 /* Row-major index helper used by the tests. */
 long long or_coord(const or_param *p, int i, int j, int k);
 

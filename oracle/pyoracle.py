@@ -76,6 +76,12 @@ def lib():
                                        _D]),
             "or_survey_pk": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, _D, ct.POINTER(ct.c_int)]),
+            "or_recon_psi": (None, [P, _D, ct.c_double, ct.c_double, _D]),
+            "or_recon_gather": (None, [P, _D, ct.c_double, _D, ct.c_longlong]),
+            "or_recon_displace": (ct.c_int, [P, _D, _D, ct.c_longlong, _D, _D, ct.c_longlong, ct.c_double, ct.c_double,
+                                             ct.c_double, ct.c_double]),
+            "or_box_recon_2pt": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, ct.c_int, _D,
+                                            ct.POINTER(ct.c_int)]),
             "or_survey_xi": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, ct.c_int, _D, ct.POINTER(ct.c_int)]),
         }
@@ -331,3 +337,45 @@ class Survey:
         if rc:
             raise ValueError("or_survey_xi rejected the multipole combination")
         return out[0::2] + 1j * out[1::2], nm
+
+
+def recon_psi(p: Param, dk, b1, RG):
+    """particle_reconstruction.hpp:56-112: (3, n0, n1, n2) complex Psi(x)."""
+    a, v = _c(dk)
+    out = np.zeros((3,) + shape(p) + (2,))
+    lib().or_recon_psi(ct.byref(p), dp(v), float(b1), float(RG), dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def recon_gather(p: Param, psi, norm, pos):
+    psi = np.ascontiguousarray(np.stack([psi.real, psi.imag], -1), dtype=np.float64)
+    pos = np.array(pos, dtype=np.float64).reshape(-1, 3)
+    lib().or_recon_gather(ct.byref(p), dp(psi), float(norm), dp(pos), pos.shape[0])
+    return pos
+
+
+def recon_displace(p: Param, posD, wD, posR, wR, alpha, norm, b1, RG):
+    """particle_reconstruction.hpp:12-402 on p's mesh: (displaced D, displaced R)."""
+    posD = np.array(posD, dtype=np.float64).reshape(-1, 3)
+    posR = np.array(posR, dtype=np.float64).reshape(-1, 3)
+    wd = None if wD is None else np.ascontiguousarray(wD, dtype=np.float64)
+    wr = None if wR is None else np.ascontiguousarray(wR, dtype=np.float64)
+    lib().or_recon_displace(ct.byref(p), dp(posD), dp(wd) if wd is not None else None, posD.shape[0], dp(posR),
+                            dp(wr) if wr is not None else None, posR.shape[0], float(alpha), float(norm), float(b1),
+                            float(RG))
+    return posD, posR
+
+
+def box_recon_2pt(p: Param, posD, posR, alpha, xi=False):
+    """powerspec.hpp:396-519: norm * (2l+1) * P_l (or xi_l), complex, and modes per bin."""
+    posD = np.ascontiguousarray(posD, dtype=np.float64).reshape(-1, 3)
+    posR = np.ascontiguousarray(posR, dtype=np.float64).reshape(-1, 3)
+    bins = rbin(p) if xi else kbin(p)
+    nb = len(bins)
+    out = np.zeros(2 * nb)
+    nm = np.zeros(nb, dtype=np.int32)
+    rc = lib().or_box_recon_2pt(ct.byref(p), dp(posD), posD.shape[0], dp(posR), posR.shape[0], float(alpha), dp(bins),
+                                1 if xi else 0, dp(out), nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+    if rc:
+        raise ValueError("or_box_recon_2pt rejected the multipole combination")
+    return out[0::2] + 1j * out[1::2], nm
