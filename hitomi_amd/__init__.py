@@ -85,6 +85,10 @@ class SurveyConfig(ct.Structure):
         ("n_rbin", ct.c_int),
         ("rmin", ct.c_double),
         ("rmax", ct.c_double),
+        ("recon", ct.c_int),
+        ("n_mesh_recon", ct.c_int * 3),
+        ("b1", ct.c_double),
+        ("RG", ct.c_double),
     ]
 
 
@@ -474,9 +478,18 @@ class SurveyPlan:
     calcTwoPointCorrelationFunction (93-179) or its window version (182-270)."""
 
     def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2, ell=0,
-                 precision=FP32, prune=True, measure="pk", n_rbin=25, rmin=30.0, rmax=150.0):
+                 precision=FP32, prune=True, measure="pk", n_rbin=25, rmin=30.0, rmax=150.0, recon=None):
+        """recon: None, or (n_mesh_recon, b1_fid, RG) to reconstruct before measuring."""
         self.ctx = ctx
         cfg = SurveyConfig()
+        if recon is not None:
+            nm, b1, RG = recon
+            nm = list(nm) if np.ndim(nm) else [int(nm)] * 3
+            cfg.recon = 1
+            for a in range(3):
+                cfg.n_mesh_recon[a] = int(nm[a])
+            cfg.b1 = float(b1)
+            cfg.RG = float(RG)
         cfg.measure = SURVEY_MEASURES[measure]
         cfg.n_rbin = int(n_rbin)
         cfg.rmin = float(rmin)

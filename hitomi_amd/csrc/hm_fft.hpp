@@ -171,11 +171,26 @@ __device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw) {
     else if constexpr (L == 640) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 2, 5>(sm, tw);
     else if constexpr (L == 768) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4, 3>(sm, tw);
     else if constexpr (L == 1024) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4, 4>(sm, tw);
+    else if constexpr (L == 24) lds_stockham<T, DIR, L, B, NT, 1, 8, 3>(sm, tw);
+    else if constexpr (L == 40) lds_stockham<T, DIR, L, B, NT, 1, 8, 5>(sm, tw);
+    else if constexpr (L == 72) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 3>(sm, tw);
+    else if constexpr (L == 120) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 5>(sm, tw);
+    else if constexpr (L == 144) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 3>(sm, tw);
+    else if constexpr (L == 200) lds_stockham<T, DIR, L, B, NT, 1, 8, 5, 5>(sm, tw);
+    else if constexpr (L == 216) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 3, 3>(sm, tw);
+    else if constexpr (L == 240) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 5>(sm, tw);
+    else if constexpr (L == 360) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 3, 5>(sm, tw);
+    else if constexpr (L == 400) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 5, 5>(sm, tw);
+    else if constexpr (L == 432) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 3, 3>(sm, tw);
+    else if constexpr (L == 600) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 5, 5>(sm, tw);
+    else if constexpr (L == 720) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 3, 5>(sm, tw);
+    else if constexpr (L == 800) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 5, 5>(sm, tw);
+    else if constexpr (L == 960) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 3, 5>(sm, tw);
     else static_assert(L == 8, "unsupported FFT length");
 }
 
 // Lengths with a radix plan, ascending.
-static constexpr int kFftLengths[] = {8, 16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 288, 320, 384, 480, 512, 576, 640, 768, 1024};
+static constexpr int kFftLengths[] = {8, 16, 24, 32, 40, 48, 64, 72, 80, 96, 120, 128, 144, 160, 192, 200, 216, 240, 256, 288, 320, 360, 384, 400, 432, 480, 512, 576, 600, 640, 720, 768, 800, 960, 1024};
 
 constexpr int pow2floor(int x) {
     int r = 1;

@@ -5,7 +5,7 @@
 #include "hm_internal.hpp"
 
 #define HM_POW2_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
-#define HM_MIXED_LENGTHS(X) X(48) X(80) X(96) X(160) X(192) X(288) X(320) X(384) X(480) X(576) X(640) X(768)
+#define HM_MIXED_LENGTHS(X) X(24) X(40) X(48) X(72) X(80) X(96) X(120) X(144) X(160) X(192) X(200) X(216) X(240) X(288) X(320) X(360) X(384) X(400) X(432) X(480) X(576) X(600) X(640) X(720) X(768) X(800) X(960)
 
 namespace hm {
 

@@ -16,6 +16,7 @@
 #include "hm_assign.hpp"
 #include "hm_bins.hpp"
 #include "hm_fft_launch.hpp"
+#include "hm_recon.hpp"
 
 #include "../../include/hitomi_amd.h"
 
@@ -168,7 +169,19 @@ __global__ __launch_bounds__(256) void k_survey_shift(const double* __restrict__
     out[3 * q + 2] = s[2] - dz;
 }
 
-// LOS of an unshifted position (particle.hpp:355-375)
+__global__ __launch_bounds__(256) void k_shift_inplace(double* __restrict__ pos, long long n, double dx, double dy,
+                                                      double dz) {
+#pragma clang fp contract(off)
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    pos[3 * q] -= dx;
+    pos[3 * q + 1] -= dy;
+    pos[3 * q + 2] -= dz;
+}
+
+// LOS of an unshifted position (particle.hpp:355-375); the lines of sight are
+// always taken from the caller's input positions, which nothing modifies, so
+// they stay those of the unreconstructed catalogues (main.cpp:156-177)
 __device__ inline void los_of(const double* p, double* l) {
 #pragma clang fp contract(off)
     const double m = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
@@ -459,6 +472,7 @@ struct SurveyPlanImpl {
     int rs[3] = {-1, -1, -1};
     long long nrows_r = 0;
     SortWork sort;
+    ReconWork rw;
     DevBuf tab, f2b, f2b_r, pos, wts, fre, fim, fields, gwork, part, cnt, red, cnt_red, scratch;
     size_t bytes = 0;
 };
@@ -476,6 +490,7 @@ void survey_destroy(SurveyPlanImpl* p) {
                       &p->gwork, &p->part, &p->cnt, &p->red, &p->cnt_red, &p->scratch};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
+    recon_work_free(p->rw);
     delete p;
 }
 
@@ -499,6 +514,11 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
     if (cfg->precision != HM_FP32 && cfg->precision != HM_FP64) return fail("survey plan: precision must be FP32 or FP64");
     if (cfg->n_kbin < 2 || cfg->n_kbin > 256) return fail("survey plan: n_kbin must be in [2, 256]");
     if (cfg->ell < 0 || cfg->ell > SV_MAX_ELL) return fail("survey plan: ell must be in [0, 8]");
+    if (cfg->recon) {
+        for (int a = 0; a < 3; a++)
+            if (!fft_length_supported(cfg->n_mesh_recon[a])) return fail("survey plan: unsupported n_mesh_recon");
+        if (!(cfg->b1 != 0.0)) return fail("survey plan: b1_fid must be non-zero");
+    }
     p->measure = cfg->measure;
     if (p->measure < HM_SURVEY_PK || p->measure > HM_SURVEY_XI_WINDOW) return fail("survey plan: unknown measure");
     p->nk = cfg->n_kbin;
@@ -572,7 +592,7 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
     return p;
 }
 
-size_t survey_bytes(const SurveyPlanImpl* p) { return p ? p->bytes + p->sort.bytes() : 0; }
+size_t survey_bytes(const SurveyPlanImpl* p) { return p ? p->bytes + p->sort.bytes() + recon_work_bytes(p->rw) : 0; }
 
 static unsigned red_blocks(long long n) { return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 1024)); }
 
@@ -665,6 +685,33 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const double vs = swR * swR / isq;      // field.hpp:499
     const double nbar = swD * swD / vs / sw2D;  // particle.hpp:296
+    if (p->cfg.recon) {
+        // main.cpp:172-183: reconstruct on the n_mesh_recon mesh, then shift into the FFT box again
+        Mesh mr;
+        for (int a = 0; a < 3; a++) {
+            mr.n[a] = p->cfg.n_mesh_recon[a];
+            mr.L[a] = m.L[a];
+        }
+        mr.assign = m.assign;
+        if (recon_displace<T>(ctx, mr, p->cfg.b1, p->cfg.RG, pos, wD, nD, pos + 3 * nD, wR, nR, alpha, vs / swD, p->rw))
+            return -1;
+        if (ctx->launch("survey_minpos", [&] {
+                hipLaunchKernelGGL(k_survey_minpos, dim3(rb), dim3(256), 0, ctx->stream, pos + 3 * nD, nR, part);
+            }))
+            return -1;
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * rb, hipMemcpyDeviceToHost, ctx->stream));
+        HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        double d2[3];
+        for (int a = 0; a < 3; a++) {
+            double v = hp[a];
+            for (unsigned b = 1; b < rb; b++) v = std::min(v, hp[3 * b + a]);
+            d2[a] = v - 1.0 * m.L[a] / (double)m.n[a];
+        }
+        if (ctx->launch("survey_shift", [&] {
+                hipLaunchKernelGGL(k_shift_inplace, dim3(pb), dim3(256), 0, ctx->stream, pos, N, d2[0], d2[1], d2[2]);
+            }))
+            return -1;
+    }
     // 3. Y_lm-weighted fluctuations and their transforms.  |Y_red| <= 1, so every
     // particle's |re| + |im| is at most sqrt(2) |w|: the bound fixes the fixed-point scale.
     // The window function (field.hpp:230-247) uses the random catalogue alone.

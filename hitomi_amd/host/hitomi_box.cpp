@@ -6,10 +6,10 @@
 // bk%d%d%d_%02d.dat files, and exits with status 1 where the reference calls
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
-// Supported now: measure = pk and 2PCF (any ELL, ell1 = ELL, ell2 = 0) and
-// measure = bk for (ell1, ell2, ELL) = (0, 0, 0), flag_recon = False.  Other
-// measurements are rejected with an error instead of being computed
-// differently.
+// Supported now: measure = pk and 2PCF (any ELL, ell1 = ELL, ell2 = 0), with
+// or without reconstruction (flag_recon), and measure = bk for
+// (ell1, ell2, ELL) = (0, 0, 0) without it.  Other measurements are rejected
+// with an error instead of being computed differently.
 #include "../../include/hitomi_amd.h"
 #include "hm_catalog.hpp"
 #include "hm_param.hpp"
@@ -35,11 +35,22 @@ static int assign_code(const std::string& a) { return a == "NGP" ? HM_NGP : (a =
 struct Gpu {
     hm_ctx* ctx = nullptr;
     double* d_pos = nullptr;
+    double* d_pos_r = nullptr;  // randoms of a reconstructed box
+    long long n_r = 0;
+    double alpha = 0.0;
     ~Gpu() {
         if (ctx && d_pos) hm_dev_free(ctx, d_pos);
+        if (ctx && d_pos_r) hm_dev_free(ctx, d_pos_r);
         if (ctx) hm_ctx_destroy(ctx);
     }
 };
+
+static double* upload(Gpu& g, const std::vector<double>& v) {
+    double* d = (double*)hm_dev_alloc(g.ctx, sizeof(double) * (v.empty() ? 1 : v.size()));
+    if (!d) return nullptr;
+    if (!v.empty() && hm_memcpy_h2d(g.ctx, d, v.data(), sizeof(double) * v.size())) return nullptr;
+    return d;
+}
 
 enum Measure { M_PK, M_XI, M_BK };
 
@@ -81,7 +92,8 @@ static int run_box(const Params& p, const Catalog& P_D, Gpu& g, Measure what, in
     if (what == M_PK) res.pk = pk.data();
     if (want_bk) res.bk = bk.data();
     if (what == M_XI) res.xi = xi.data();
-    int rc = hm_box_plan_execute(g.ctx, plan, g.d_pos, P_D.n_tot, &res);
+    int rc = g.d_pos_r ? hm_box_plan_execute_recon(g.ctx, plan, g.d_pos, P_D.n_tot, g.d_pos_r, g.n_r, g.alpha, &res)
+                       : hm_box_plan_execute(g.ctx, plan, g.d_pos, P_D.n_tot, &res);
     if (rc) printf("hm_box_plan_execute: %s\n", hm_last_error(g.ctx));
     hm_box_plan_destroy(g.ctx, plan);
     return rc;
@@ -141,23 +153,63 @@ int main(int argc, char* argv[]) {
     P_D.print_parameters(p.output_dir);
     stage("Done");
 
-    if (p.flag_recon == "True") {
-        printf("flag_recon = True: the reconstruction path is not available in this build.\n");
-        EXIT();
-    }
-
-    printf("Calculating '%s': %.3f sec.\n", p.measure.c_str(), now() - g_t0);
     Gpu g;
     g.ctx = hm_ctx_create(0, 0);
     if (!g.ctx) {
         printf("no usable GPU (hm_ctx_create failed)\n");
         EXIT();
     }
-    g.d_pos = (double*)hm_dev_alloc(g.ctx, sizeof(double) * 3 * (size_t)P_D.n_tot);
-    if (!g.d_pos || hm_memcpy_h2d(g.ctx, g.d_pos, P_D.pos.data(), sizeof(double) * 3 * (size_t)P_D.n_tot)) {
+    g.d_pos = upload(g, P_D.pos);
+    if (!g.d_pos) {
         printf("particle upload failed: %s\n", hm_last_error(g.ctx));
         EXIT();
     }
+    const bool recon = p.flag_recon == "True";
+    const char* sfx = recon ? "_recon" : "";
+    if (recon) {
+        // main_box.cpp:141-159
+        printf("Calculating reconstruction: %.3f sec.\n", now() - g_t0);
+        Catalog P_R;
+        if (P_R.read_randoms(p.random_file)) {
+            // the reference prints this and carries on with no randoms (then divides by N_R = 0)
+            printf("'P_R.readRandomParticles(param.random_file)' fails.\n");
+            EXIT();
+        }
+        g.d_pos_r = upload(g, P_R.pos);
+        g.n_r = P_R.n_tot;
+        if (!g.d_pos_r) {
+            printf("particle upload failed: %s\n", hm_last_error(g.ctx));
+            EXIT();
+        }
+        hm_recon_config rcfg;
+        std::memset(&rcfg, 0, sizeof(rcfg));
+        for (int a = 0; a < 3; a++) {
+            rcfg.mesh.n_mesh[a] = p.n_mesh_recon[a];
+            rcfg.mesh.boxsize[a] = p.boxsize[a];
+        }
+        rcfg.mesh.assign = assign_code(p.assign);
+        rcfg.precision = p.precision == "FP64" ? HM_FP64 : HM_FP32;
+        rcfg.b1 = p.b1_fid;
+        rcfg.RG = p.RG;
+        // particle_reconstruction.hpp:221-222: alpha by integer division (Q4), Vsurvey = V
+        const double alpha_int = double(P_D.n_tot / P_R.n_tot);
+        if (hm_recon_displace(g.ctx, &rcfg, g.d_pos, nullptr, P_D.n_tot, g.d_pos_r, nullptr, P_R.n_tot, alpha_int,
+                              p.volume / double(P_D.n_tot)) ||
+            hm_periodic_wrap(g.ctx, g.d_pos, P_D.n_tot, p.boxsize) ||
+            hm_periodic_wrap(g.ctx, g.d_pos_r, P_R.n_tot, p.boxsize) ||
+            hm_memcpy_d2h(g.ctx, P_D.pos.data(), g.d_pos, sizeof(double) * 3 * (size_t)P_D.n_tot) ||
+            hm_memcpy_d2h(g.ctx, P_R.pos.data(), g.d_pos_r, sizeof(double) * 3 * (size_t)P_R.n_tot)) {
+            printf("reconstruction failed: %s\n", hm_last_error(g.ctx));
+            EXIT();
+        }
+        P_D.calc_min_max();
+        P_R.calc_min_max();
+        g.alpha = double(P_D.n_tot) / double(P_R.n_tot);  // main_box.cpp:153
+        if (Catalog::print_parameters_recon(p.output_dir, P_D, P_R, g.alpha)) EXIT();
+        stage("Done");
+    }
+
+    printf("Calculating '%s': %.3f sec.\n", p.measure.c_str(), now() - g_t0);
     const int nk = p.n_kbin;
     std::vector<double> pk, bk, xi;
     if (p.measure == "pk") {
@@ -174,7 +226,7 @@ int main(int argc, char* argv[]) {
         }
         // powerspec.hpp:318-327
         char buf[2048];
-        snprintf(buf, sizeof(buf), "%s/pk%d.dat", p.output_dir.c_str(), p.ELL);
+        snprintf(buf, sizeof(buf), "%s/pk%d%s.dat", p.output_dir.c_str(), p.ELL, sfx);
         FILE* fp = fopen(buf, "w");
         if (!fp) EXIT();
         fprintf(fp, "## k [h/Mpc]\t pk_real [(Mpc/h)^3]\t pk_imag [(Mpc/h)^3]\n");
@@ -194,13 +246,17 @@ int main(int argc, char* argv[]) {
         }
         // powerspec.hpp:380-386
         char buf[2048];
-        snprintf(buf, sizeof(buf), "%s/xi%d.dat", p.output_dir.c_str(), p.ELL);
+        snprintf(buf, sizeof(buf), "%s/xi%d%s.dat", p.output_dir.c_str(), p.ELL, sfx);
         FILE* fp = fopen(buf, "w");
         if (!fp) EXIT();
         fprintf(fp, "## r [Mpc/h] \t xi\n");
         for (int i = 0; i < p.n_rbin; i++) fprintf(fp, "%.5f \t %.7e\n", rbin[i], xi[i]);
         fclose(fp);
     } else if (p.measure == "bk") {
+        if (recon) {
+            printf("measure = bk with flag_recon = True is not available in this build.\n");
+            EXIT();
+        }
         if (!(p.ell1 == 0 && p.ell2 == 0 && p.ELL == 0)) {
             printf("measure = bk with (ell1, ell2, ELL) = (%d, %d, %d) is not available in this build; only (0, 0, 0).\n",
                    p.ell1, p.ell2, p.ELL);

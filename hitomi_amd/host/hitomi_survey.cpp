@@ -7,8 +7,8 @@
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
 // Supported now: measure = pk, 2PCF and window2PCF (ell1 = ELL, ell2 = 0),
-// flag_recon = False.  Other measurements are rejected with an error instead
-// of being computed differently.
+// with or without reconstruction (flag_recon).  Other measurements are
+// rejected with an error instead of being computed differently.
 #include "../../include/hitomi_amd.h"
 #include "hm_param.hpp"
 #include "hm_survey_catalog.hpp"
@@ -122,10 +122,7 @@ int main(int argc, char* argv[]) {
         printf("'P_R.readParticleBOSS' fails.\n");
         EXIT();
     }
-    if (p.flag_recon == "True") {
-        printf("flag_recon = True: the reconstruction path is not available in this build.\n");
-        EXIT();
-    }
+    const bool recon = p.flag_recon == "True";
     int measure = -1;
     std::string fails;
     if (p.measure == "pk") {
@@ -179,6 +176,10 @@ int main(int argc, char* argv[]) {
     cfg.n_rbin = p.n_rbin;
     cfg.rmin = p.rmin;
     cfg.rmax = p.rmax;
+    cfg.recon = recon ? 1 : 0;
+    for (int a = 0; a < 3; a++) cfg.n_mesh_recon[a] = p.n_mesh_recon[a];
+    cfg.b1 = p.b1_fid;
+    cfg.RG = p.RG;
     hm_survey_plan* plan = hm_survey_plan_create(g.ctx, &cfg);
     if (!plan) {
         printf("hm_survey_plan_create: %s\n", hm_last_error(g.ctx));
@@ -204,7 +205,7 @@ int main(int argc, char* argv[]) {
     char buf[2048];
     if (measure == HM_SURVEY_PK) {
         // powerspec.hpp:72-87
-        snprintf(buf, sizeof(buf), "%s/pk%d.dat", p.output_dir.c_str(), p.ELL);
+        snprintf(buf, sizeof(buf), recon ? "%s/pk%d_recon.dat" : "%s/pk%d.dat", p.output_dir.c_str(), p.ELL);
         FILE* fp = fopen(buf, "w");
         if (!fp) EXIT();
         fprintf(fp, "## k [h/Mpc]\t pk_real [(Mpc/h)^3]\t pk_imag [(Mpc/h)^3]\n");
@@ -212,8 +213,9 @@ int main(int argc, char* argv[]) {
         fclose(fp);
     } else {
         // powerspec.hpp:161-175, 253-266
-        snprintf(buf, sizeof(buf), measure == HM_SURVEY_XI ? "%s/xi%d.dat" : "%s/xi%d_window.dat", p.output_dir.c_str(),
-                 p.ELL);
+        const char* fmt = measure == HM_SURVEY_XI ? (recon ? "%s/xi%d_recon.dat" : "%s/xi%d.dat")
+                                                  : (recon ? "%s/xi%d_recon_window.dat" : "%s/xi%d_window.dat");
+        snprintf(buf, sizeof(buf), fmt, p.output_dir.c_str(), p.ELL);
         FILE* fp = fopen(buf, "w");
         if (!fp) EXIT();
         fprintf(fp, "## r [Mpc/h] \t xi\n");

@@ -222,6 +222,66 @@ struct Catalog {
     }
 
     // particleGadget.hpp:450-489
+    // particleGadget.hpp:390-448: "x y z" per line, lines that do not parse skipped
+    int read_randoms(const std::string& fname) {
+        std::ifstream fin(fname.c_str());
+        if (fin.fail()) {
+            printf("can not open file '%s'...\n", fname.c_str());
+            return -1;
+        }
+        std::string str;
+        long long n = 0;
+        double x, y, z;
+        while (std::getline(fin, str))
+            if (sscanf(str.c_str(), "%lf %lf %lf", &x, &y, &z) == 3) n++;
+        if (init(n)) return -1;
+        fin.clear();
+        fin.seekg(0);
+        long long i = 0;
+        while (std::getline(fin, str) && i < n) {
+            if (sscanf(str.c_str(), "%lf %lf %lf", &x, &y, &z) != 3) continue;
+            pos[3 * i] = x;
+            pos[3 * i + 1] = y;
+            pos[3 * i + 2] = z;
+            i++;
+        }
+        calc_min_max();
+        return 0;
+    }
+
+    // particleGadget.hpp:487-530
+    static int print_parameters_recon(const std::string& outdir, const Catalog& D, const Catalog& R, double alpha) {
+        std::string path = outdir + "/simulation_parameters";
+        FILE* fp = fopen(path.c_str(), "w");
+        if (!fp) {
+            printf("output directry '%s' does not exist\n", outdir.c_str());
+            return -1;
+        }
+        fprintf(fp, "\n");
+        fprintf(fp, "P_D.n_tot = %lld\n", D.n_tot);
+        fprintf(fp, "P_R.n_tot = %lld\n", R.n_tot);
+        fprintf(fp, "\n");
+        fprintf(fp, "P_D.redshift = %.5f\n", D.redshift);
+        fprintf(fp, "P_D.scaleFactor = %.5f\n", D.scaleFactor);
+        fprintf(fp, "P_D.boxsize [Mpc/h] = %.5f\n", D.boxsize);
+        fprintf(fp, "P_D.omegaM0 = %.5f\n", D.omegaM0);
+        fprintf(fp, "P_D.omegaLambda = %.5f\n", D.omegaLambda);
+        fprintf(fp, "P_D.hubble = %.5f\n", D.hubble);
+        fprintf(fp, "P_D.mass [10^10 Msun/h] = %.10f\n", D.mass);
+        fprintf(fp, "\n");
+        fprintf(fp, "alpha = %.5f\n", alpha);
+        fprintf(fp, "nbar / 1.0e-4 [(Mpc/h)^{-3}] = %.5f\n", double(D.n_tot) / std::pow(D.boxsize, 3) / 1.0e-4);
+        fprintf(fp, "\n");
+        for (int a = 0; a < 3; a++) fprintf(fp, "P_D.pos_min[%d] [Mpc/h] = %.5f\n", a, D.pos_min[a]);
+        for (int a = 0; a < 3; a++) fprintf(fp, "P_R.pos_min[%d] [Mpc/h] = %.5f\n", a, R.pos_min[a]);
+        fprintf(fp, "\n");
+        for (int a = 0; a < 3; a++) fprintf(fp, "P_D.pos_max[%d] [Mpc/h] = %.5f\n", a, D.pos_max[a]);
+        for (int a = 0; a < 3; a++) fprintf(fp, "P_R.pos_max[%d] [Mpc/h] = %.5f\n", a, R.pos_max[a]);
+        fprintf(fp, "\n");
+        fclose(fp);
+        return 0;
+    }
+
     int print_parameters(const std::string& outdir) const {
         std::string path = outdir + "/simulation_parameters";
         FILE* fp = fopen(path.c_str(), "w");

@@ -198,6 +198,9 @@ typedef struct {
     int measure;       /* HM_SURVEY_PK, HM_SURVEY_XI or HM_SURVEY_XI_WINDOW */
     int n_rbin;        /* tools.hpp:126-132, for the correlation functions */
     double rmin, rmax;
+    int recon;         /* 1: reconstruct first (main.cpp:172-183, flag_recon = True) */
+    int n_mesh_recon[3];
+    double b1, RG;     /* b1_fid, RG */
 } hm_survey_config;
 
 /* The survey quantities main.cpp derives before measuring. */
@@ -212,7 +215,11 @@ typedef struct {
 
 hm_survey_plan *hm_survey_plan_create(hm_ctx *ctx, const hm_survey_config *cfg);
 void hm_survey_plan_destroy(hm_ctx *ctx, hm_survey_plan *plan);
-/* The configured measurement after main.cpp's preparation (main.cpp:109-166):
+/* The configured measurement after main.cpp's preparation (main.cpp:109-166)
+ * and, with recon = 1, the reconstruction (calcReconstructionParticles on the
+ * n_mesh_recon mesh with alpha and norm = Vsurvey / sum w_D, then the FFT-box
+ * shift again, main.cpp:172-183; the lines of sight and Vsurvey stay those of
+ * the unreconstructed catalogues, as there):
  * d_pos_* are the unshifted Cartesian positions (n x 3 FP64; the line of
  * sight is pos/|pos| of these), d_w_* the weights (FP64).
  * out: [n_bins][2] = norm * pk_save (pk%d.dat) or norm * xi_save (whose real

@@ -208,3 +208,74 @@ def test_survey_end_to_end_matches_oracle(tmp_path):
     assert f"P_D.n_tot = {len(D)}" in txt and f"P_R.n_tot = {len(R)}" in txt
     assert f"alpha = {s.alpha:.5f}" in txt
     assert f"nbar / 1.0e-4 [(Mpc/h)^{{-3}}] = {s.nbar / 1e-4:.5f}" in txt
+
+
+@pytest.mark.gpu
+def test_box_reconstruction_end_to_end(tmp_path):
+    """main_box.cpp with flag_recon = True: randoms read, reconstruction on n_mesh_recon,
+    wrap, P_l / xi_l of D - alpha R written to pk%d_recon.dat / xi%d_recon.dat."""
+    from oracle import pyoracle as po
+    from .conftest import clustered_catalogue, uniform_catalogue
+
+    L, n, nrm, N, NR = 1000.0, 32, 24, 20000, 50000
+    pos = clustered_catalogue(N, L, seed=81, nclump=20, sigma=25.0).astype(np.float32)
+    hio.write_gadget1(str(tmp_path / "snap"), pos, boxsize=L, num_files=1)
+    R = uniform_catalogue(NR, L, seed=82)
+    with open(tmp_path / "rand.dat", "w") as f:
+        for x, y, z in R:
+            f.write(f"{x:.10e} {y:.10e} {z:.10e}\n")
+    Rf = np.loadtxt(tmp_path / "rand.dat")
+    b1, RG = 2.0, 20.0
+    kw = dict(data_dir=str(tmp_path), random_file="rand.dat", boxsize_x=L, boxsize_y=L, boxsize_z=L, n_mesh_x=n,
+              n_mesh_y=n, n_mesh_z=n, n_mesh_recon_x=nrm, n_mesh_recon_y=nrm, n_mesh_recon_z=nrm, flag_recon="True",
+              b1_fid=b1, RG=RG, kmin=0.01, kmax=0.15, n_kbin=8, rmin=30.0, rmax=200.0, n_rbin=10, precision="FP64")
+    dD, dR = po.recon_displace(po.param(nrm, L, po.TSC), pos.astype(np.float64), None, Rf, None, float(N // NR),
+                               L ** 3 / N, b1, RG)
+    for a in (dD, dR):
+        for ax in range(3):
+            a[a[:, ax] >= L, ax] -= L
+            a[a[:, ax] < 0.0, ax] += L
+    alpha = N / NR
+    for measure, fname, xi in (("pk", "pk2_recon.dat", False), ("2PCF", "xi2_recon.dat", True)):
+        ini = tmp_path / f"{measure}.ini"
+        hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure=measure, ell1=2, ELL=2, **kw)
+        r = run(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        got = hio.read_table(tmp_path / "out" / fname)
+        p = po.param(n, L, po.TSC, ell1=2, ELL=2, kmin=0.01, kmax=0.15, n_kbin=8, rmin=30.0, rmax=200.0, n_rbin=10)
+        ref, _ = po.box_recon_2pt(p, dD, dR, alpha, xi=xi)
+        assert np.allclose(got[:, 1], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
+    txt = (tmp_path / "out" / "simulation_parameters").read_text()
+    assert f"P_R.n_tot = {NR}" in txt and f"alpha = {alpha:.5f}" in txt
+
+
+@pytest.mark.gpu
+def test_survey_reconstruction_end_to_end(tmp_path):
+    from oracle import pyoracle as po
+
+    rng = np.random.default_rng(9)
+    lo, size = np.array([600.0, 900.0, 300.0]), 500.0
+    posD = lo + rng.random((3000, 3)) * size
+    posR = lo + rng.random((12000, 3)) * size
+    wD = 1.0 / (1.0 + rng.random(3000))
+    wR = 1.0 / (1.0 + rng.random(12000))
+    write_xyzw(tmp_path / "data.dat", posD, wD)
+    write_xyzw(tmp_path / "rand.dat", posR, wR)
+    D, R = np.loadtxt(tmp_path / "data.dat", comments="#"), np.loadtxt(tmp_path / "rand.dat", comments="#")
+    n, L, nrm, b1, RG = 32, 700.0, 24, 2.0, 20.0
+    ini = tmp_path / "p.ini"
+    hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="rand.dat",
+                  output_dir=str(tmp_path / "out"), measure="pk", ell1=0, ELL=0, boxsize_x=L, boxsize_y=L,
+                  boxsize_z=L, n_mesh_x=n, n_mesh_y=n, n_mesh_z=n, kmin=0.02, kmax=0.12, n_kbin=6, precision="FP64",
+                  flag_recon="True", n_mesh_recon_x=nrm, n_mesh_recon_y=nrm, n_mesh_recon_z=nrm, b1_fid=b1, RG=RG)
+    r = run_survey(ini, tmp_path)
+    assert r.returncode == 0, r.stdout
+    got = hio.read_table(tmp_path / "out" / "pk0_recon.dat")
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.12, n_kbin=6)
+    s = po.Survey(p, D[:, :3], D[:, 3], R[:, :3], R[:, 3])
+    dD, dR = po.recon_displace(po.param(nrm, L, po.TSC), s.posD, D[:, 3], s.posR, R[:, 3], s.alpha,
+                               s.vsurvey / D[:, 3].sum(), b1, RG)
+    d2 = dR.min(0) - L / n
+    s.posD, s.posR = dD - d2, dR - d2
+    ref, _ = s.pk(p)
+    assert np.allclose(got[:, 1], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())

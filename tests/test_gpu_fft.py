@@ -8,7 +8,8 @@ import hitomi_amd as hm
 pytestmark = pytest.mark.gpu
 
 POW2 = [8, 16, 32, 64, 128, 256, 512, 1024]
-MIXED = [48, 80, 96, 160, 192, 288, 320, 384, 480, 576, 640, 768]
+MIXED = [24, 40, 48, 72, 80, 96, 120, 144, 160, 192, 200, 216, 240, 288, 320, 360, 384, 400, 432, 480, 576, 600, 640,
+         720, 768, 800, 960]
 TOL = {hm.FP64: 1e-12, hm.FP32: 2e-6}
 
 

@@ -131,3 +131,30 @@ def test_survey_xi(ctx, case, ell, window, prec):
     scale = np.abs(ref0).max()
     err = np.abs(got - ref).max()
     assert err <= TOL[prec] * scale, (err, scale)
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+@pytest.mark.parametrize("measure", ["pk", "2PCF"])
+def test_survey_after_reconstruction(ctx, prec, measure):
+    """main.cpp:172-183: reconstruct (n_mesh_recon, b1_fid, RG), shift into the box again,
+    measure with the original lines of sight and Vsurvey.  Tolerance 1e-9 (FP64) / 1e-4
+    (FP32) of the P_0 (or max xi_0) scale: the displaced positions carry the FP32 error."""
+    n, L, nr_mesh, b1, RG, ell = 32, 800.0, 24, 2.0, 25.0, 2
+    posD, wD, posR, wR = get_cat("patch")
+    kw = dict(kmin=0.02, kmax=0.14, n_kbin=7) if measure == "pk" else dict(rmin=40.0, rmax=240.0, n_rbin=9)
+    p = po.param(n, L, po.TSC, ell1=ell, ELL=ell, **kw)
+    p0 = po.param(n, L, po.TSC, **kw)
+    s = po.Survey(p, posD, wD, posR, wR)
+    dD, dR = po.recon_displace(po.param(nr_mesh, L, po.TSC), s.posD, wD, s.posR, wR, s.alpha,
+                               s.vsurvey / wD.sum(), b1, RG)
+    d2 = dR.min(0) - L / n
+    s.posD, s.posR = dD - d2, dR - d2
+    ref, nm = s.pk(p) if measure == "pk" else s.xi(p)
+    ref0, _ = s.pk(p0) if measure == "pk" else s.xi(p0)
+    plan = hm.SurveyPlan(ctx, n, L, "TSC", ell=ell, precision=prec, measure=measure, recon=(nr_mesh, b1, RG), **kw)
+    got, gnm, info = plan.execute(posD, wD, posR, wR)
+    plan.close()
+    assert np.array_equal(gnm, nm.astype(np.int64))
+    assert info["vsurvey"] == pytest.approx(s.vsurvey, rel=1e-12)
+    tol = 1e-9 if prec == hm.FP64 else 1e-4
+    assert np.abs(got - ref).max() <= tol * np.abs(ref0).max()
