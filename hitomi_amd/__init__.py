@@ -54,6 +54,7 @@ class BoxConfig(ct.Structure):
         ("rmax", ct.c_double),
         ("n_xi_ells", ct.c_int),
         ("xi_ells", ct.c_int * MAX_ELLS),
+        ("bk_ell", ct.c_int * 3),
     ]
 
 
@@ -148,6 +149,7 @@ _SIGS = {
     "hm_box_plan_triple_mesh": (ct.c_int, [_VP, ct.POINTER(ct.c_int)]),
     "hm_recon_displace": (ct.c_int, [_VP, ct.POINTER(ReconConfig), _VP, _VP, ct.c_longlong, _VP, _VP, ct.c_longlong,
                                      ct.c_double, ct.c_double]),
+    "hm_sph_bessel_jl": (ct.c_double, [ct.c_int, ct.c_double]),
     "hm_periodic_wrap": (ct.c_int, [_VP, _VP, ct.c_longlong, ct.POINTER(ct.c_double)]),
     "hm_survey_plan_create": (_VP, [_VP, ct.POINTER(SurveyConfig)]),
     "hm_survey_plan_destroy": (None, [_VP, _VP]),
@@ -375,7 +377,7 @@ class BoxPlan:
 
     def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2,
                  pk_ells=(0, 2, 4), bk=True, precision=FP32, regrid=True, prune=True, xi_ells=(), n_rbin=25,
-                 rmin=30.0, rmax=150.0):
+                 rmin=30.0, rmax=150.0, bk_ells=(0, 0, 0)):
         self.ctx = ctx
         cfg = BoxConfig()
         cfg.mesh = make_mesh(n_mesh, boxsize, assign)
@@ -392,6 +394,8 @@ class BoxPlan:
         cfg.n_rbin = int(n_rbin)
         cfg.rmin = float(rmin)
         cfg.rmax = float(rmax)
+        for i in range(3):
+            cfg.bk_ell[i] = int(bk_ells[i])
         cfg.n_xi_ells = len(xi_ells)
         for i, l in enumerate(xi_ells):
             cfg.xi_ells[i] = int(l)

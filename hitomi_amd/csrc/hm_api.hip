@@ -3,6 +3,7 @@
 // reports them through hm_last_error().
 #include "hm_assign.hpp"
 #include "hm_internal.hpp"
+#include "hm_bessel.hpp"
 #include "hm_recon.hpp"
 
 #include "../../include/hitomi_amd.h"
@@ -415,5 +416,7 @@ int hm_periodic_wrap(hm_ctx* ctx, double* d_pos, long long n, const double boxsi
     if (!boxsize || (n > 0 && !d_pos)) return bad(ctx, "periodic wrap: NULL argument");
     return hm::periodic_wrap(ctx, d_pos, n, boxsize);
 }
+
+double hm_sph_bessel_jl(int l, double x) { return hm::sph_jl(l, x); }
 
 }  // extern "C"

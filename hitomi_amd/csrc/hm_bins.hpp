@@ -81,6 +81,59 @@ __device__ __forceinline__ double mesh_vec(int i, int n, double d) {
     return (i < n / 2) ? (double)i * d : (double)(i - n) * d;  // field.hpp:929-931
 }
 
+// ---------------------------------------------------------------------------
+// Reduced spherical harmonic of tools.hpp:12-68:
+//   Y(l, M)(v) = conj[(-1)^((M-|M|)/2) N_l^|M|(mu) e^{i M phi}] sqrt(4 pi/(2l+1))
+// with 1 for (0, 0), 0 for |v| < 1e-15, mu = v_z/|v|, phi = acos(v_x/|v_xy|)
+// (0 when |v_xy| < 1e-15), reflected to 2 pi - phi when v_y < 0.  N_l^m is
+// the orthonormal associated Legendre function with the Condon-Shortley
+// phase, from the stable normalised recurrence
+//   N_m^m = sqrt(1/4pi) prod_{i<=m} (-sqrt(1-x^2) sqrt((2i+1)/(2i))),
+//   N_l^m = a_lm (x N_{l-1}^m - N_{l-2}^m / a_{l-1,m}),  a_lm = sqrt((4l^2-1)/(l^2-m^2)).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double norm_plm(int l, int m, double x) {
+    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double ymm = 0.28209479177387814;  // 1 / sqrt(4 pi)
+    for (int i = 1; i <= m; i++) ymm *= -somx2 * sqrt((2.0 * i + 1.0) / (2.0 * i));
+    if (l == m) return ymm;
+    double y1 = x * sqrt(2.0 * m + 3.0) * ymm;  // N_{m+1}^m
+    double y0 = ymm;
+    double aprev = sqrt(2.0 * m + 3.0);
+    for (int ll = m + 2; ll <= l; ll++) {
+        const double a = sqrt((4.0 * ll * ll - 1.0) / ((double)ll * ll - (double)m * m));
+        const double y2 = a * (x * y1 - y0 / aprev);
+        y0 = y1;
+        y1 = y2;
+        aprev = a;
+    }
+    return y1;
+}
+
+struct Cplx {
+    double re, im;
+};
+
+__host__ __device__ inline Cplx ylm_red(int l, int M, double vx, double vy, double vz) {
+#pragma clang fp contract(off)
+    if (l == 0 && M == 0) return {1.0, 0.0};
+    const double r = sqrt(vx * vx + vy * vy + vz * vz);
+    const double rxy = sqrt(vx * vx + vy * vy);
+    if (fabs(r) < 1.0e-15) return {0.0, 0.0};
+    const double mu = vz / r;
+    double phi = (fabs(rxy) < 1.0e-15) ? 0.0 : acos(vx / rxy);
+    if (vy < 0.0) phi = -phi + 2.0 * M_PI;
+    const int S = M < 0 ? -M : M;
+    const double sgn = (M < 0 && (M & 1)) ? -1.0 : 1.0;
+    const double p = sgn * norm_plm(l, S, mu);
+    double sn, cs;
+    sincos((double)M * phi, &sn, &cs);
+    const double nrm = sqrt(4.0 * M_PI / (2.0 * (double)l + 1.0));
+    return {p * cs * nrm, -(p * sn) * nrm};
+}
+
+// gsl_sf_coupling_3j (common.hpp:29) for small integer arguments
+double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3);
+
 inline int wpow_of(int assign) { return assign == 0 ? 1 : (assign == 1 ? 2 : 3); }
 
 // Fine index -> [first, last] coarse bin whose (lower, upper] holds the fine

@@ -17,6 +17,7 @@
 //   6. all 210 triple sums sum_x F_a F_b F_3 in one pass (FP32 products,
 //      FP64 accumulation per 64 voxels, fixed-order block reduction).
 #include "hm_assign.hpp"
+#include "hm_bessel.hpp"
 #include "hm_bins.hpp"
 #include "hm_fft_launch.hpp"
 #include "hm_recon.hpp"
@@ -32,6 +33,8 @@ namespace hm {
 // ---------------------------------------------------------------------------
 // P_l binning (calcPowerSpectrum, field.hpp:765-852) on the half spectrum
 // ---------------------------------------------------------------------------
+constexpr int BIN_MAX_ELLS = 8;  // user multipoles plus those the bispectrum shot noise needs
+
 struct BinArgs {
     int n[3], nzh;
     int s01;          // support on x, y (-1: all)
@@ -42,7 +45,7 @@ struct BinArgs {
     int wp;           // window power
     const int2* f2b;  // fine index -> [lo, hi] coarse bins
     int nk, ne;
-    int ells[HM_MAX_ELLS];
+    int ells[BIN_MAX_ELLS];
     double sn;
 };
 
@@ -172,6 +175,7 @@ template <typename T> struct XiSrc {
     int wp;
     double sn, inv_v;
     double dc;  // added to Re d at k = 0 in Re d conj(n): N for the shot-noise field, 0 for xi
+    int odd = 0;  // 1: -i times the odd part of g instead of its even part (its transform is Im xi)
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
     __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
 #pragma clang fp contract(off)
@@ -188,6 +192,7 @@ template <typename T> struct XiSrc {
         const double W2 = wpow((t.wf[0][ii] * t.wf[1][jj]) * t.wf[2][kk], wp);
         const double g1 = inv_v * ((p2 - sn * S1) / (W1 * W1));
         const double g2 = inv_v * ((p2 - sn * S2) / (W2 * W2));
+        if (odd) return mk<T>((T)0, (T)(-(0.5 * (g1 - g2))));
         return mk<T>((T)(0.5 * (g1 + g2)), (T)0);
     }
 };
@@ -299,6 +304,69 @@ __global__ __launch_bounds__(256) void k_xi_hist(const T* __restrict__ R, int n2
     H[n] = acc;
 }
 
+// As k_xi_hist for a general multipole: each voxel's xi is weighted by
+// A(mu) = sum_m w_m Y(l1, m)(r) Y(l2, -m)(r), which depends on mu = r_z/|r|
+// alone (the e^{i m phi} factors cancel) and is a polynomial of degree
+// l1 + l2 in mu, evaluated from its Chebyshev coefficients (Clenshaw); A0 is
+// its value at r = 0, where the reference's Y vanish for l > 0.
+constexpr int ANG_MAX = 17;
+struct AngPoly {
+    int deg;
+    double c[ANG_MAX];
+    double a0;
+};
+
+__device__ __forceinline__ double ang_eval(const AngPoly& A, double mu) {
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = A.deg; k >= 1; k--) {
+        const double b0 = 2.0 * mu * b1 - b2 + A.c[k];
+        b2 = b1;
+        b1 = b0;
+    }
+    return mu * b1 - b2 + A.c[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_xi_hist_ang(const T* __restrict__ R, int n2, const int* __restrict__ a_idx,
+                                                    long long amax, long long nkey, AngPoly A,
+                                                    double* __restrict__ H) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nkey) return;
+    long long m = (long long)sqrt((double)n);
+    while (m * m > n) m--;
+    while ((m + 1) * (m + 1) <= n) m++;
+    if (m > n2 / 2) m = n2 / 2;
+    const double rn = sqrt((double)n);
+    double acc = 0.0;
+    for (long long q = 0; q <= m; q++) {
+        const long long a = n - q * q;
+        if (a > amax) continue;
+        const int ai = a_idx[a];
+        if (ai < 0) continue;
+        const T* row = R + (long long)ai * n2;
+        if (n == 0) {
+            acc += A.a0 * (double)row[0];
+        } else if (q == 0) {
+            acc += ang_eval(A, 0.0) * (double)row[0];
+        } else if (q == n2 / 2) {  // the Nyquist plane is the negative z (field.hpp:1042)
+            acc += ang_eval(A, -(double)q / rn) * (double)row[n2 / 2];
+        } else {
+            acc += ang_eval(A, (double)q / rn) * (double)row[q] + ang_eval(A, -(double)q / rn) * (double)row[n2 - q];
+        }
+    }
+    H[n] = acc;
+}
+
+// J[a][n2] = j_l(k_a sqrt(n2) dr) from the spline of bessel.hpp:12-25
+__global__ void k_jl_table(const double* __restrict__ sy, const double* __restrict__ sc, const double* __restrict__ kb,
+                           int nk, long long nkey, double dr, double* __restrict__ J) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nk * nkey) return;
+    const int a = (int)(t / nkey);
+    const long long key = t % nkey;
+    J[t] = jl_spline_eval(sy, sc, kb[a] * (sqrt((double)key) * dr));
+}
+
 // J[a][n2] = j0(k_a sqrt(n2) dr) (bessel.hpp spline of j0 replaced by the
 // closed form sin x / x)
 __global__ void k_j0_table(const double* __restrict__ kb, int nk, long long nkey, double dr, double* __restrict__ J) {
@@ -310,14 +378,14 @@ __global__ void k_j0_table(const double* __restrict__ kb, int nk, long long nkey
     J[t] = (x == 0.0) ? 1.0 : sin(x) / x;
 }
 
-// S[a][b] = dV * sum_n2 J[a][n2] J[b][n2] H[n2], one block per pair (a <= b)
-__global__ __launch_bounds__(256) void k_sn_pairs(const double* __restrict__ J, long long nkey,
-                                                 const double* __restrict__ H, const short2* __restrict__ pairs,
-                                                 double dV, double* __restrict__ out) {
+// S[a][b] = dV * sum_n2 J1[a][n2] J2[b][n2] H[n2], one block per pair
+__global__ __launch_bounds__(256) void k_sn_pairs(const double* __restrict__ J1, const double* __restrict__ J2,
+                                                 long long nkey, const double* __restrict__ H,
+                                                 const short2* __restrict__ pairs, double dV, double* __restrict__ out) {
     __shared__ double sm[256];
     const short2 pr = pairs[blockIdx.x];
-    const double* Ja = J + (long long)pr.x * nkey;
-    const double* Jb = J + (long long)pr.y * nkey;
+    const double* Ja = J1 + (long long)pr.x * nkey;
+    const double* Jb = J2 + (long long)pr.y * nkey;
     double acc = 0.0;
     for (long long key = threadIdx.x; key < nkey; key += 256) acc += Ja[key] * Jb[key] * H[key];
     sm[threadIdx.x] = acc;
@@ -334,41 +402,41 @@ __global__ __launch_bounds__(256) void k_sn_pairs(const double* __restrict__ J, 
 // ---------------------------------------------------------------------------
 constexpr int GRAM_CH = 64;
 
-template <typename T>
+template <typename T, int CH = GRAM_CH>
 __global__ __launch_bounds__(256) void k_gram_sym(const C<T>* __restrict__ fields, long long fstride, int nf,
                                                  const C<T>* __restrict__ f3, long long nvox, int vpb,
                                                  const short2* __restrict__ pairs, int npairs,
                                                  double2* __restrict__ part) {
     extern __shared__ unsigned char smraw[];
     C<T>* f3s = reinterpret_cast<C<T>*>(smraw);
-    C<T>* cs = f3s + GRAM_CH;                   // [nf][GRAM_CH + 1]: F_a F_3
-    C<T>* gs = cs + (size_t)nf * (GRAM_CH + 1);  // [nf][GRAM_CH + 1]: F_b
+    C<T>* cs = f3s + CH;                   // [nf][CH + 1]: F_a F_3
+    C<T>* gs = cs + (size_t)nf * (CH + 1);  // [nf][CH + 1]: F_b
     const int t = threadIdx.x;
     const long long v0 = (long long)blockIdx.x * vpb;
     const long long v1 = (v0 + vpb < nvox) ? v0 + vpb : nvox;
     short2 pr = make_short2(0, 0);
     if (t < npairs) pr = pairs[t];
     double accr = 0.0, acci = 0.0;
-    for (long long c0 = v0; c0 < v1; c0 += GRAM_CH) {
-        if (t < GRAM_CH) {
+    for (long long c0 = v0; c0 < v1; c0 += CH) {
+        if (t < CH) {
             const long long v = c0 + t;
             f3s[t] = (v < v1) ? f3[v] : mk<T>((T)0, (T)0);
         }
         __syncthreads();
-        for (int e = t; e < nf * GRAM_CH; e += 256) {
-            const int f = e / GRAM_CH, x = e % GRAM_CH;
+        for (int e = t; e < nf * CH; e += 256) {
+            const int f = e / CH, x = e % CH;
             const long long v = c0 + x;
             const C<T> g = (v < v1) ? fields[(long long)f * fstride + v] : mk<T>((T)0, (T)0);
-            gs[f * (GRAM_CH + 1) + x] = g;
-            cs[f * (GRAM_CH + 1) + x] = cmul<T>(g, f3s[x]);
+            gs[f * (CH + 1) + x] = g;
+            cs[f * (CH + 1) + x] = cmul<T>(g, f3s[x]);
         }
         __syncthreads();
         if (t < npairs) {
             T ar = 0, ai = 0;
-            const C<T>* ca = cs + pr.x * (GRAM_CH + 1);
-            const C<T>* gb = gs + pr.y * (GRAM_CH + 1);
+            const C<T>* ca = cs + pr.x * (CH + 1);
+            const C<T>* gb = gs + pr.y * (CH + 1);
 #pragma unroll 8
-            for (int x = 0; x < GRAM_CH; x++) {
+            for (int x = 0; x < CH; x++) {
                 const C<T> c = ca[x], g = gb[x];
                 ar += c.x * g.x - c.y * g.y;
                 ai += c.x * g.y + c.y * g.x;
@@ -455,16 +523,35 @@ __global__ __launch_bounds__(256) void k_tri_stage(TriStage<T> a, C<T>* __restri
 }
 
 // First (x) pass source of shell f: D where shell f holds the mode, else 0.
+// With ylm set, field f < nk is shell f weighted by Y(l1, m1)(k) and field
+// nk + f shell f weighted by Y(l2, m2)(k) (calcInverseFourierTransformForBispectrum
+// with a Y table, field.hpp:346-401).
 template <typename T> struct ShellMaskSrc {
     const C<T>* D;
     const char2* sid;
     int M[3], s[3], cs[3];
+    int ylm = 0, nk = 0, l1 = 0, m1 = 0, l2 = 0, m2 = 0;
+    int n[3];
+    AxisTabs t;
+    __device__ inline int to_n(int m, int a) const {
+        if (M[a] == n[a]) return m;
+        const int sg = (m <= M[a] / 2) ? m : m - M[a];
+        return sg >= 0 ? sg : sg + n[a];
+    }
     __device__ inline C<T> operator()(int f, int o, int b, int p, long long) const {
+#pragma clang fp contract(off)
         const int sx = support_unmap(p, s[0], M[0]), sy = support_unmap(o, s[1], M[1]),
                   sz = support_unmap(b, s[2], M[2]);
         const char2 r = sid[((long long)sx * cs[1] + sy) * cs[2] + sz];
-        if (f < (int)r.x || f > (int)r.y) return mk<T>((T)0, (T)0);
-        return D[((long long)p * M[1] + o) * M[2] + b];
+        const int sh = ylm ? f % nk : f;
+        if (sh < (int)r.x || sh > (int)r.y) return mk<T>((T)0, (T)0);
+        const C<T> d = D[((long long)p * M[1] + o) * M[2] + b];
+        if (!ylm) return d;
+        const bool second = f >= nk;
+        const Cplx y = ylm_red(second ? l2 : l1, second ? m2 : m1, t.kv[0][to_n(p, 0)], t.kv[1][to_n(o, 1)],
+                               t.kv[2][to_n(b, 2)]);
+        const double dr = (double)d.x, di = (double)d.y;
+        return mk<T>((T)(y.re * dr - y.im * di), (T)(y.re * di + y.im * dr));
     }
 };
 
@@ -667,6 +754,17 @@ struct BoxPlanImpl {
     int rs[3] = {-1, -1, -1};  // support of the separations that reach an r-bin
     long long xi_rows = 0;
     DevBuf f2b_r_d, xi_r, xi_part, xi_cnt, xi_red, xi_cnt_red;
+    // bispectrum multipole (l1, l2, L); bk_gen: anything but 000
+    int l1 = 0, l2 = 0, Lb = 0;
+    bool bk_gen = false;
+    std::vector<int> mlist;           // m1 with a non-zero 3j weight (m2 = -m1, M = 0)
+    std::vector<double> wlist;
+    double w_const = 0, w_row = 0, w_col = 0;  // weights of N, P_l1(k1) (l2 = 0) and P_l2(k2) (l1 = 0)
+    std::vector<int> bin_ells;        // multipoles binned: the requested ones plus those the shot noise needs
+    int e_row = -1, e_col = -1;       // their indices in bin_ells
+    AngPoly ang;
+    bool sn_odd = false;              // odd l1 + l2: the odd part of xi enters the Bessel sweep too
+    DevBuf jtab2, gpairs_d, hist2, sn_pairs2;           // (a, nk + b): the cross sums of the two Y-weighted shell sets
 };
 
 static int alloc(hm_ctx* ctx, BoxPlanImpl* p, DevBuf& b, size_t bytes) {
@@ -690,13 +788,22 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
         if (alloc(ctx, p, p->hist, sizeof(double) * (size_t)p->nkey)) return -1;
         if (alloc(ctx, p, p->xi_R, sizeof(T) * (size_t)p->na * (size_t)m.n[2])) return -1;
         if (alloc(ctx, p, p->jtab, sizeof(double) * (size_t)p->nk * (size_t)p->nkey)) return -1;
+        if (p->bk_gen && p->l2 != p->l1 &&
+            alloc(ctx, p, p->jtab2, sizeof(double) * (size_t)p->nk * (size_t)p->nkey))
+            return -1;
+        if (p->sn_odd && (alloc(ctx, p, p->hist2, sizeof(double) * (size_t)p->nkey) ||
+                          alloc(ctx, p, p->sn_pairs2, sizeof(double) * p->pairs.size())))
+            return -1;
         if (alloc(ctx, p, p->pairs_d, sizeof(short2) * p->pairs.size())) return -1;
         if (alloc(ctx, p, p->sn_pairs, sizeof(double) * p->pairs.size())) return -1;
         const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
-        if (alloc(ctx, p, p->fields, sizeof(C<T>) * (size_t)nvox * (size_t)(p->nk + 1))) return -1;
+        // general multipole: two shell sets, delta/W transformed, and delta/W kept untransformed for the
+        // shell loader of every m
+        const int nfield = p->bk_gen ? 2 * p->nk + 2 : p->nk + 1;
+        if (alloc(ctx, p, p->fields, sizeof(C<T>) * (size_t)nvox * (size_t)nfield)) return -1;
         p->gram_blocks = (nvox + p->gram_vpb - 1) / p->gram_vpb;
-        p->fused = tri_fused_ok(p->cfg.precision, p->M[2]) && p->nk < TRI_B;
-        p->gram_cols = 2 * (int)p->pairs.size();
+        p->fused = !p->bk_gen && tri_fused_ok(p->cfg.precision, p->M[2]) && p->nk < TRI_B;
+        p->gram_cols = 2 * std::min((int)p->pairs.size(), 256);
         if (p->fused) {
             p->gram_blocks = ((long long)p->M[0] * p->M[1] + TRI_RPW - 1) / TRI_RPW;
             p->ngrp = (p->nk + 3) / 4;
@@ -762,7 +869,62 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     if (cfg->n_kbin < 2 || cfg->n_kbin > 64) return fail("box plan: n_kbin must be in [2, 64]");
     if (cfg->n_pk_ells < 0 || cfg->n_pk_ells > HM_MAX_ELLS) return fail("box plan: n_pk_ells out of range");
     p->nk = cfg->n_kbin;
-    p->ne = cfg->n_pk_ells;
+    p->bin_ells.assign(cfg->pk_ells, cfg->pk_ells + cfg->n_pk_ells);
+    if (cfg->bk) {
+        // calcBiSpectrumForBOX (bispec.hpp:885-891, 926-963): the multipole and its weights
+        p->l1 = cfg->bk_ell[0];
+        p->l2 = cfg->bk_ell[1];
+        p->Lb = cfg->bk_ell[2];
+        if (p->l1 < 0 || p->l2 < 0 || p->Lb < 0 || p->l1 > 8 || p->l2 > 8 || p->Lb > 16)
+            return fail("box plan: bispectrum multipole out of range");
+        if (std::fabs(wigner_3j(p->l1, p->l2, p->Lb, 0, 0, 0)) < 1.0e-10)
+            return fail("box plan: this multipole combination is not allowed (wigner_3j(ell1, ell2, ELL, 0,0,0) == 0)");
+        p->bk_gen = !(p->l1 == 0 && p->l2 == 0 && p->Lb == 0);
+        p->sn_odd = ((p->l1 + p->l2) & 1) != 0;
+        for (int m1 = -p->l1; m1 <= p->l1; m1++)
+            for (int m2 = -p->l2; m2 <= p->l2; m2++) {
+                double w = wigner_3j(p->l1, p->l2, p->Lb, 0, 0, 0) * wigner_3j(p->l1, p->l2, p->Lb, m1, m2, 0);
+                w *= (double)(2 * p->Lb + 1) * (double)(2 * p->l1 + 1) * (double)(2 * p->l2 + 1);
+                if (std::fabs(w) < 1.0e-10) continue;
+                p->mlist.push_back(m1);
+                p->wlist.push_back(w);
+                if (p->l1 == 0 && p->l2 == 0) p->w_const += w;
+                if (p->l2 == 0) p->w_row += w;
+                if (p->l1 == 0) p->w_col += w;
+            }
+        auto need = [&](int l) -> int {
+            for (size_t e = 0; e < p->bin_ells.size(); e++)
+                if (p->bin_ells[e] == l) return (int)e;
+            p->bin_ells.push_back(l);
+            return (int)p->bin_ells.size() - 1;
+        };
+        if (p->l2 == 0) p->e_row = need(p->l1);
+        if (p->l1 == 0) p->e_col = need(p->l2);
+        if ((int)p->bin_ells.size() > BIN_MAX_ELLS) return fail("box plan: too many multipoles");
+        // A(mu) = sum_m w_m Y(l1, m) Y(l2, -m) as Chebyshev coefficients (exact: degree l1 + l2)
+        const int deg = p->l1 + p->l2;
+        p->ang.deg = deg;
+        std::vector<double> fv(deg + 1);
+        for (int j = 0; j <= deg; j++) {
+            const double mu = std::cos(M_PI * (j + 0.5) / (deg + 1));
+            const double sx = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+            double acc = 0.0;
+            for (size_t q = 0; q < p->mlist.size(); q++) {
+                const Cplx y1 = ylm_red(p->l1, p->mlist[q], sx, 0.0, mu);
+                const Cplx y2 = ylm_red(p->l2, -p->mlist[q], sx, 0.0, mu);
+                acc += p->wlist[q] * (y1.re * y2.re - y1.im * y2.im);
+            }
+            fv[j] = acc;
+        }
+        for (int k = 0; k <= deg; k++) {
+            double c = 0.0;
+            for (int j = 0; j <= deg; j++) c += fv[j] * std::cos(M_PI * k * (j + 0.5) / (deg + 1));
+            p->ang.c[k] = (k == 0 ? 1.0 : 2.0) * c / (deg + 1);
+        }
+        for (int k = deg + 1; k < ANG_MAX; k++) p->ang.c[k] = 0.0;
+        p->ang.a0 = p->w_const;
+    }
+    p->ne = (int)p->bin_ells.size();
     // k bins, ToolClass::setKbin (tools.hpp:118-124)
     p->kbin.resize(p->nk);
     const double dkb = (cfg->kmax - cfg->kmin) / (double)(p->nk - 1);
@@ -817,12 +979,12 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     // pairs a <= b
     for (int a = 0; a < p->nk; a++)
         for (int b = a; b < p->nk; b++) p->pairs.push_back(make_short2((short)a, (short)b));
-    if (cfg->bk && p->pairs.size() > 256) return fail("box plan: at most 22 k-bins for the bispectrum");
-    if (cfg->bk) {
-        bool has0 = false;
-        for (int e = 0; e < p->ne; e++) has0 = has0 || (cfg->pk_ells[e] == 0);
-        if (!has0) return fail("box plan: the bispectrum shot noise needs l = 0 in pk_ells");
+    if (p->bk_gen) {  // every ordered pair: B_{l1 l2 L}(k1, k2) is not symmetric
+        p->pairs.clear();
+        for (int a = 0; a < p->nk; a++)
+            for (int b = 0; b < p->nk; b++) p->pairs.push_back(make_short2((short)a, (short)b));
     }
+    if (cfg->bk && !p->bk_gen && p->pairs.size() > 256) return fail("box plan: at most 22 k-bins for the bispectrum");
     if (cfg->bk) {
         const double dr0 = m.L[0] / m.n[0], dr1 = m.L[1] / m.n[1], dr2 = m.L[2] / m.n[2];
         if (!(dr0 == dr1 && dr1 == dr2)) return fail("box plan: the bispectrum needs equal cell sizes on all axes");
@@ -924,13 +1086,46 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     if (cfg->bk) {
         if (hipMemcpy(p->pairs_d.p, p->pairs.data(), sizeof(short2) * p->pairs.size(), hipMemcpyHostToDevice) != hipSuccess)
             return fail("box plan: pair table upload failed");
+        if (p->bk_gen) {
+            std::vector<short2> gp(p->pairs);
+            for (auto& q : gp) q.y = (short)(q.y + p->nk);
+            if (alloc(ctx, p, p->gpairs_d, sizeof(short2) * gp.size()) ||
+                hipMemcpy(p->gpairs_d.p, gp.data(), sizeof(short2) * gp.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return fail("box plan: pair table upload failed");
+        }
         const double dr = m.L[0] / m.n[0];
         const long long tot = (long long)p->nk * p->nkey;
-        if (ctx->launch("j0_table", [&] {
-                hipLaunchKernelGGL(k_j0_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
-                                   (const double*)p->kb_d.p, p->nk, p->nkey, dr, (double*)p->jtab.p);
-            }))
-        { plan_destroy(p); return nullptr; }
+        // j_l(k r) tables: the closed form for l = 0, else the spline of bessel.hpp:12-25
+        auto jtable = [&](int l, DevBuf& out) -> int {
+            if (l == 0)
+                return ctx->launch("j0_table", [&] {
+                    hipLaunchKernelGGL(k_j0_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                                       (const double*)p->kb_d.p, p->nk, p->nkey, dr, (double*)out.p);
+                });
+            std::vector<double> sy, sc;
+            jl_spline(l, sy, sc);
+            DevBuf dy, dc;
+            int rc = (dev_alloc(ctx, dy, sizeof(double) * sy.size()) || dev_alloc(ctx, dc, sizeof(double) * sc.size()))
+                         ? -1
+                         : 0;
+            if (!rc && (hipMemcpy(dy.p, sy.data(), sizeof(double) * sy.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(dc.p, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+                ctx->set_error("box plan: spline upload failed");
+                rc = -1;
+            }
+            if (!rc)
+                rc = ctx->launch("jl_table", [&] {
+                    hipLaunchKernelGGL(k_jl_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                                       (const double*)dy.p, (const double*)dc.p, (const double*)p->kb_d.p, p->nk,
+                                       p->nkey, dr, (double*)out.p);
+                });
+            if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = -1;
+            dev_free(dy);
+            dev_free(dc);
+            return rc;
+        };
+        if (jtable(p->l1, p->jtab)) { plan_destroy(p); return nullptr; }
+        if (p->bk_gen && p->l2 != p->l1 && jtable(p->l2, p->jtab2)) { plan_destroy(p); return nullptr; }
     }
     // shell mode counts on the N mesh (data independent)
     {
@@ -966,7 +1161,8 @@ void plan_destroy(BoxPlanImpl* p) {
     DevBuf* bufs[] = {&p->tab, &p->f2b_d, &p->lu_d, &p->kb_d, &p->fixed, &p->dk, &p->pk_part, &p->pk_cnt, &p->pk_red,
                       &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
                       &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid,
-                      &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red, &p->rpos, &p->rw};
+                      &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red, &p->rpos, &p->rw,
+                      &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -1117,11 +1313,10 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     ba.f2b = (const int2*)p->f2b_d.p;
     ba.nk = p->nk;
     ba.ne = std::max(p->ne, 1);
-    for (int e = 0; e < HM_MAX_ELLS; e++) ba.ells[e] = (e < p->ne) ? p->cfg.pk_ells[e] : 0;
-    if (p->ne == 0) ba.ells[0] = 0;
+    for (int e = 0; e < BIN_MAX_ELLS; e++) ba.ells[e] = (e < p->ne) ? p->bin_ells[e] : 0;
     ba.sn = sn2;
     const int ncols = p->nk * ba.ne;
-    const bool do_pk = p->ne > 0 || p->cfg.bk;
+    const bool do_pk = p->ne > 0;
     if (do_pk) {
     if (fill_zero(ctx, p->pk_part.p, sizeof(double) * (size_t)ncols * p->nrows)) return -1;
     if (fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows)) return -1;
@@ -1188,6 +1383,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             return -1;
     }
     const int npairs = (int)p->pairs.size();
+    std::vector<double> gacc;  // weighted cross sums of a general multipole, per ordered pair
     if (p->cfg.bk) {
         // 4. shot-noise ijk term (field.hpp:1116-1160, bispec.hpp:1023-1056)
         XiSrc<T> xs;
@@ -1199,6 +1395,12 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.sn = (double)np;
         xs.inv_v = 1.0 / m.volume();
         xs.dc = (double)np;
+        // even part of g; for an odd l1 + l2 also -i times its odd part (field.hpp:1116-1160 keeps both:
+        // an odd angular weight sees the odd part of xi, and the even part on the z Nyquist plane)
+        for (int par = 0; par < (p->sn_odd ? 2 : 1); par++) {
+        xs.odd = par;
+        void* hbuf = par ? p->hist2.p : p->hist.p;
+        void* sbuf = par ? p->sn_pairs2.p : p->sn_pairs.p;
         C<T>* xh = (C<T>*)p->xi_half.p;
         if (launch_cols<T, +1>(ctx, "xi_pass_x", m.n[0], xs, xh,
                                geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
@@ -1215,20 +1417,30 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             return -1;
         if (fft_c2r_rows<T>(ctx, "xi_pass_z", m.n[2] / 2, xg, xr, (long long)p->na, -1)) return -1;
         if (ctx->launch("xi_hist", [&] {
-                hipLaunchKernelGGL(k_xi_hist<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0, ctx->stream,
-                                   xr, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey, (double*)p->hist.p);
+                if (p->bk_gen)
+                    hipLaunchKernelGGL(k_xi_hist_ang<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey, p->ang,
+                                       (double*)hbuf);
+                else
+                    hipLaunchKernelGGL(k_xi_hist<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey,
+                                       (double*)hbuf);
             }))
             return -1;
         if (ctx->launch("sn_pairs", [&] {
                 hipLaunchKernelGGL(k_sn_pairs, dim3((unsigned)npairs), dim3(256), 0, ctx->stream,
-                                   (const double*)p->jtab.p, p->nkey, (const double*)p->hist.p,
-                                   (const short2*)p->pairs_d.p, dV, (double*)p->sn_pairs.p);
+                                   (const double*)p->jtab.p,
+                                   (const double*)(p->bk_gen && p->l2 != p->l1 ? p->jtab2.p : p->jtab.p), p->nkey, (const double*)hbuf,
+                                   (const short2*)p->pairs_d.p, dV, (double*)sbuf);
             }))
             return -1;
+        }
         // 5. shell fields and delta/W on the triple mesh (field.hpp:346-401, 610-639, 328-344)
         const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
         C<T>* F = (C<T>*)p->fields.p;
-        C<T>* F3 = F + (long long)p->nk * nvox;
+        const int nsh = p->bk_gen ? 2 * p->nk : p->nk;  // shell fields: one set, or two Y-weighted sets
+        C<T>* F3 = F + (long long)nsh * nvox;
+        C<T>* Draw = p->bk_gen ? F3 + nvox : F3;  // delta/W on the triple mesh before any transform
         TriStage<T> st;
         st.dk = dk;
         st.nzh = (int)nzh;
@@ -1248,52 +1460,114 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         const long long nst = (long long)st.c3[0] * st.c3[1] * st.c3[2];
         if (ctx->launch("tri_stage", [&] {
                 hipLaunchKernelGGL(k_tri_stage<T>, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, ctx->stream, st,
-                                   F3, (char2*)p->sid.p);
+                                   Draw, (char2*)p->sid.p);
             }))
             return -1;
+        if (p->bk_gen)
+            HM_HIP_CHECK(ctx, hipMemcpyAsync(F3, Draw, sizeof(C<T>) * (size_t)nvox, hipMemcpyDeviceToDevice, ctx->stream));
         ShellMaskSrc<T> ms;
-        ms.D = F3;
+        ms.D = Draw;
         ms.sid = (const char2*)p->sid.p;
         for (int a = 0; a < 3; a++) {
             ms.M[a] = p->M[a];
             ms.s[a] = st.s[a];
             ms.cs[a] = st.cs[a];
         }
-        // x pass of every shell (one launch), then of delta/W in place
-        ColGeom gx = tri_geom_x(p->M, st.s, nvox, p->nk);
-        if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
-        ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
-        if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
-        // y pass
-        ColGeom gy = tri_geom_y(p->M, st.s, nvox, p->nk);
-        if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
-        ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
-        if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;
-        // 6. z pass and triple sums
-        const long long nrows = (long long)p->M[0] * p->M[1];
-        if (p->fused) {
-            if (launch_tri_z_gram<T>(ctx, p->M[2], F, nvox, p->nk, st.s[2], st.s3[2], nrows, p->ngrp, p->nblk, p->nsl,
-                                     (double*)p->gram_part.p))
-                return -1;
-        } else {
-            RowGeom rg{nrows * p->nk, p->M[2], p->M[2], st.s[2]};
-            if (fft_rows<T>(ctx, "tri_pass_z", p->M[2], +1, F, F, rg)) return -1;
+        if (p->bk_gen) {
+            // delta/W on the triple mesh, all three passes once
+            ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
+            if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
+            ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
+            if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;
+            const long long nrows = (long long)p->M[0] * p->M[1];
             RowGeom rg3{nrows, p->M[2], p->M[2], st.s3[2]};
             if (fft_rows<T>(ctx, "tri3_pass_z", p->M[2], +1, F3, F3, rg3)) return -1;
-            const size_t lds = sizeof(C<T>) * (GRAM_CH + 2 * (size_t)p->nk * (GRAM_CH + 1));
-            if (ctx->launch("gram", [&] {
-                    hipLaunchKernelGGL(k_gram_sym<T>, dim3((unsigned)p->gram_blocks), dim3(256), lds, ctx->stream, F,
-                                       nvox, p->nk, F3, nvox, p->gram_vpb, (const short2*)p->pairs_d.p, npairs,
-                                       (double2*)p->gram_part.p);
-                }))
+            // per m1 (bispec.hpp:1107-1180): shells weighted by Y(l1, m1) and by Y(l2, -m1), all cross sums
+            ms.ylm = 1;
+            ms.nk = p->nk;
+            ms.l1 = p->l1;
+            ms.l2 = p->l2;
+            ms.t = tabs;
+            for (int a = 0; a < 3; a++) ms.n[a] = m.n[a];
+            gacc.assign(2 * (size_t)npairs, 0.0);
+            for (size_t q = 0; q < p->mlist.size(); q++) {
+                ms.m1 = p->mlist[q];
+                ms.m2 = -p->mlist[q];
+                ColGeom gx = tri_geom_x(p->M, st.s, nvox, nsh);
+                if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
+                ColGeom gy = tri_geom_y(p->M, st.s, nvox, nsh);
+                if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
+                RowGeom rg{nrows * nsh, p->M[2], p->M[2], st.s[2]};
+                if (fft_rows<T>(ctx, "tri_pass_z", p->M[2], +1, F, F, rg)) return -1;
+                for (int c0 = 0; c0 < npairs; c0 += 256) {
+                    const int cnt = std::min(256, npairs - c0);
+                    const short2* gp = (const short2*)p->gpairs_d.p + c0;
+                    size_t lds = sizeof(C<T>) * (GRAM_CH + 2 * (size_t)nsh * (GRAM_CH + 1));
+                    const bool narrow = lds > 65536;
+                    if (narrow) lds = sizeof(C<T>) * (32 + 2 * (size_t)nsh * 33);
+                    if (lds > 65536) {
+                        ctx->set_error("box plan: too many k-bins for the general bispectrum multipole");
+                        return -1;
+                    }
+                    if (ctx->launch("gram", [&] {
+                            if (narrow)
+                                hipLaunchKernelGGL((k_gram_sym<T, 32>), dim3((unsigned)p->gram_blocks), dim3(256), lds,
+                                                   ctx->stream, F, nvox, nsh, F3, nvox, p->gram_vpb, gp, cnt,
+                                                   (double2*)p->gram_part.p);
+                            else
+                                hipLaunchKernelGGL((k_gram_sym<T>), dim3((unsigned)p->gram_blocks), dim3(256), lds,
+                                                   ctx->stream, F, nvox, nsh, F3, nvox, p->gram_vpb, gp, cnt,
+                                                   (double2*)p->gram_part.p);
+                        }))
+                        return -1;
+                    if (reduce_cols<double>(ctx, "gram_reduce", (const double*)p->gram_part.p, p->gram_blocks,
+                                            2 * cnt, (double*)p->gram_red.p))
+                        return -1;
+                    std::vector<double> part(2 * (size_t)cnt);
+                    HM_HIP_CHECK(ctx, hipMemcpyAsync(part.data(), p->gram_red.p, sizeof(double) * part.size(),
+                                                     hipMemcpyDeviceToHost, ctx->stream));
+                    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+                    for (int e = 0; e < 2 * cnt; e++) gacc[2 * (size_t)c0 + e] += p->wlist[q] * part[e];
+                }
+            }
+        } else {
+            // x pass of every shell (one launch), then of delta/W in place
+            ColGeom gx = tri_geom_x(p->M, st.s, nvox, p->nk);
+            if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
+            ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
+            if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
+            // y pass
+            ColGeom gy = tri_geom_y(p->M, st.s, nvox, p->nk);
+            if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
+            ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
+            if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;
+            // 6. z pass and triple sums
+            const long long nrows = (long long)p->M[0] * p->M[1];
+            if (p->fused) {
+                if (launch_tri_z_gram<T>(ctx, p->M[2], F, nvox, p->nk, st.s[2], st.s3[2], nrows, p->ngrp, p->nblk, p->nsl,
+                                         (double*)p->gram_part.p))
+                    return -1;
+            } else {
+                RowGeom rg{nrows * p->nk, p->M[2], p->M[2], st.s[2]};
+                if (fft_rows<T>(ctx, "tri_pass_z", p->M[2], +1, F, F, rg)) return -1;
+                RowGeom rg3{nrows, p->M[2], p->M[2], st.s3[2]};
+                if (fft_rows<T>(ctx, "tri3_pass_z", p->M[2], +1, F3, F3, rg3)) return -1;
+                const size_t lds = sizeof(C<T>) * (GRAM_CH + 2 * (size_t)p->nk * (GRAM_CH + 1));
+                if (ctx->launch("gram", [&] {
+                        hipLaunchKernelGGL(k_gram_sym<T>, dim3((unsigned)p->gram_blocks), dim3(256), lds, ctx->stream, F,
+                                           nvox, p->nk, F3, nvox, p->gram_vpb, (const short2*)p->pairs_d.p, npairs,
+                                           (double2*)p->gram_part.p);
+                    }))
+                    return -1;
+            }
+            if (reduce_cols<double>(ctx, "gram_reduce", (const double*)p->gram_part.p, p->gram_blocks, p->gram_cols,
+                                    (double*)p->gram_red.p))
                 return -1;
         }
-        if (reduce_cols<double>(ctx, "gram_reduce", (const double*)p->gram_part.p, p->gram_blocks, p->gram_cols,
-                                (double*)p->gram_red.p))
-            return -1;
     }
     // 7. results to host
-    std::vector<double> pk(ncols), gram(2 * (size_t)npairs), graw((size_t)p->gram_cols), snp((size_t)npairs);
+    std::vector<double> pk(ncols), gram(2 * (size_t)npairs), graw((size_t)p->gram_cols), snp((size_t)npairs),
+        snp2((size_t)npairs, 0.0);
     std::vector<long long> cnt(p->nk);
     std::vector<double> xsum((size_t)p->nr * p->nxe);
     std::vector<long long> xcnt(p->nr);
@@ -1310,22 +1584,27 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                          hipMemcpyDeviceToHost, ctx->stream));
     }
     if (p->cfg.bk) {
-        HM_HIP_CHECK(ctx, hipMemcpyAsync(graw.data(), p->gram_red.p, sizeof(double) * p->gram_cols,
-                                         hipMemcpyDeviceToHost, ctx->stream));
+        if (!p->bk_gen)
+            HM_HIP_CHECK(ctx, hipMemcpyAsync(graw.data(), p->gram_red.p, sizeof(double) * p->gram_cols,
+                                             hipMemcpyDeviceToHost, ctx->stream));
         HM_HIP_CHECK(ctx, hipMemcpyAsync(snp.data(), p->sn_pairs.p, sizeof(double) * npairs, hipMemcpyDeviceToHost,
                                          ctx->stream));
+        if (p->sn_odd)
+            HM_HIP_CHECK(ctx, hipMemcpyAsync(snp2.data(), p->sn_pairs2.p, sizeof(double) * npairs,
+                                             hipMemcpyDeviceToHost, ctx->stream));
     }
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const int nk = p->nk;
     const double V = m.volume(), N = (double)np;
-    if (p->cfg.bk) gram_to_pairs(p, graw, gram);
+    if (p->cfg.bk && !p->bk_gen) gram_to_pairs(p, graw, gram);
+    if (p->bk_gen) gram = gacc;
     // P_l: average (field.hpp:839-845), (2l+1) and V/N^2 (powerspec.hpp:306-316)
     std::vector<double> praw((size_t)ba.ne * nk);
     for (int e = 0; e < ba.ne; e++)
         for (int b = 0; b < nk; b++) praw[(size_t)e * nk + b] = cnt[b] ? pk[(size_t)b * ba.ne + e] / (double)cnt[b] : 0.0;
     if (res) {
         const double norm = V / N / N;
-        for (int e = 0; e < p->ne; e++)
+        for (int e = 0; e < p->cfg.n_pk_ells; e++)  // the requested multipoles lead bin_ells
             for (int b = 0; b < nk; b++) {
                 const double v = praw[(size_t)e * nk + b];
                 if (res->pk_raw) {
@@ -1352,21 +1631,21 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             for (int b = 0; b < p->nr; b++) res->xi_nmode[b] = xcnt[b];
     }
     if (p->cfg.bk && res) {
-        // P_raw of the monopole: calcPowerSpectrum(dn_00_sn, N_LM_sn, ..., 0, 0) (bispec.hpp:946, 953)
-        int e0 = -1;
-        for (int e = 0; e < p->ne; e++)
-            if (p->cfg.pk_ells[e] == 0) e0 = e;
-        std::vector<double> p0(nk, 0.0);
-        if (e0 >= 0)
-            for (int b = 0; b < nk; b++) p0[b] = praw[(size_t)e0 * nk + b];
+        // shot noise (bispec.hpp:926-1073): w N for l1 = l2 = 0, w P_l1(k1) for l2 = 0 (the row bin),
+        // w P_l2(k2) for l1 = 0 (the column bin), i^(l1+l2) times the Bessel sweep
+        auto praw_of = [&](int e, int b) { return e >= 0 ? praw[(size_t)e * nk + b] : 0.0; };
+        const int ip = ((p->l1 + p->l2) % 4 + 4) % 4;
+        const double ir = ip == 0 ? 1.0 : (ip == 2 ? -1.0 : 0.0), ii = ip == 1 ? 1.0 : (ip == 3 ? -1.0 : 0.0);
         const long long M3 = (long long)p->M[0] * p->M[1] * p->M[2];
         const double norm = (V / N / N) * (V / N);
         for (int q = 0; q < npairs; q++) {
             const int a = p->pairs[q].x, b = p->pairs[q].y;
             const double den = (double)M3 * (double)p->shell_nmode[a] * (double)p->shell_nmode[b];
             const double br = gram[2 * q] / den, bi = gram[2 * q + 1] / den;
-            const double sr = N + p0[a] + p0[b] + snp[q];
-            for (int sym = 0; sym < 2; sym++) {
+            const double base = p->w_const * N + p->w_row * praw_of(p->e_row, a) + p->w_col * praw_of(p->e_col, b);
+            // i^(l1+l2) (S_even + i S_odd)
+            const double sr = base + (ir * snp[q] - ii * snp2[q]), si = ir * snp2[q] + ii * snp[q];
+            for (int sym = 0; sym < (p->bk_gen ? 1 : 2); sym++) {
                 const int r = sym ? b : a, c = sym ? a : b;
                 const size_t o = 2 * ((size_t)r * nk + c);
                 if (res->bk_sum) {
@@ -1375,11 +1654,11 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                 }
                 if (res->sn_sum) {
                     res->sn_sum[o] = sr;
-                    res->sn_sum[o + 1] = 0.0;
+                    res->sn_sum[o + 1] = si;
                 }
                 if (res->bk) {
                     res->bk[o] = norm * (br - sr);
-                    res->bk[o + 1] = norm * (bi - 0.0);
+                    res->bk[o + 1] = norm * (bi - si);
                 }
             }
         }

@@ -7,9 +7,10 @@
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
 // Supported now: measure = pk and 2PCF (any ELL, ell1 = ELL, ell2 = 0), with
-// or without reconstruction (flag_recon), and measure = bk for
-// (ell1, ell2, ELL) = (0, 0, 0) without it.  Other measurements are rejected
-// with an error instead of being computed differently.
+// or without reconstruction (flag_recon), and measure = bk for any
+// (ell1, ell2, ELL) with wigner_3j(ell1, ell2, ELL, 0, 0, 0) != 0 without it.
+// Other measurements are rejected with an error instead of being computed
+// differently.
 #include "../../include/hitomi_amd.h"
 #include "hm_catalog.hpp"
 #include "hm_param.hpp"
@@ -68,8 +69,11 @@ static int run_box(const Params& p, const Catalog& P_D, Gpu& g, Measure what, in
     cfg.n_kbin = p.n_kbin;
     cfg.kmin = p.kmin;
     cfg.kmax = p.kmax;
-    cfg.n_pk_ells = what == M_XI ? 0 : 1;
-    cfg.pk_ells[0] = want_bk ? 0 : ell;
+    cfg.n_pk_ells = what == M_PK ? 1 : 0;
+    cfg.pk_ells[0] = ell;
+    cfg.bk_ell[0] = p.ell1;
+    cfg.bk_ell[1] = p.ell2;
+    cfg.bk_ell[2] = p.ELL;
     cfg.bk = want_bk ? 1 : 0;
     cfg.regrid = 1;
     cfg.prune = 1;
@@ -255,12 +259,6 @@ int main(int argc, char* argv[]) {
     } else if (p.measure == "bk") {
         if (recon) {
             printf("measure = bk with flag_recon = True is not available in this build.\n");
-            EXIT();
-        }
-        if (!(p.ell1 == 0 && p.ell2 == 0 && p.ELL == 0)) {
-            printf("measure = bk with (ell1, ell2, ELL) = (%d, %d, %d) is not available in this build; only (0, 0, 0).\n",
-                   p.ell1, p.ell2, p.ELL);
-            printf("'calcBiSpectrumForBOX(P_D, param, kbin)' fails.\n");
             EXIT();
         }
         if (run_box(p, P_D, g, M_BK, 0, pk, bk, xi)) {

@@ -114,13 +114,14 @@ typedef struct {
     double kmin, kmax;
     int n_pk_ells;             /* P_l for l in pk_ells (calcPowerSpectrumForBOX, powerspec.hpp:273-332) */
     int pk_ells[HM_MAX_ELLS];
-    int bk;                    /* 1: B_000 for every (ith_kbin, k2) pair (calcBiSpectrumForBOX, bispec.hpp:877-1203) */
+    int bk;                    /* 1: B_{ell1 ell2 ELL} for every (ith_kbin, k2) pair (calcBiSpectrumForBOX, bispec.hpp:877-1203) */
     int regrid;                /* 1: evaluate the triple product on the smallest mesh that holds it exactly */
     int prune;                 /* 1: skip the zero lines of band-limited transforms */
     int n_rbin;                /* r-bins of ToolClass::setRbin (tools.hpp:126-132) */
     double rmin, rmax;
     int n_xi_ells;             /* xi_l for l in xi_ells (calcTwoPointCorrelationFunctionForBOX, powerspec.hpp:334-393) */
     int xi_ells[HM_MAX_ELLS];
+    int bk_ell[3];             /* (ell1, ell2, ELL) of the bispectrum multipole; {0, 0, 0}: B_000 */
 } hm_box_config;
 
 /* Host result arrays (any may be NULL). */
@@ -128,7 +129,7 @@ typedef struct {
     double *pk;            /* [n_pk_ells][n_kbin][2] norm*(2l+1)*P as written to pk%d.dat */
     double *pk_raw;        /* [n_pk_ells][n_kbin][2] bin averages of calcPowerSpectrum (field.hpp:839-845) */
     long long *pk_nmode;   /* [n_kbin] modes per bin */
-    double *bk;            /* [n_kbin][n_kbin][2] norm*(bk - sn), row i = file bk000_%02d.dat with ith_kbin = i */
+    double *bk;            /* [n_kbin][n_kbin][2] norm*(bk - sn), row i = file bk%d%d%d_%02d.dat with ith_kbin = i */
     double *bk_sum;        /* [n_kbin][n_kbin][2] bk_save (bispec.hpp:1168) */
     double *sn_sum;        /* [n_kbin][n_kbin][2] sn_save (bispec.hpp:941-1056) */
     long long *shell_nmode;/* [n_kbin] n_mode of calcInverseFourierTransformForBispectrum (field.hpp:383) */
@@ -152,6 +153,14 @@ int hm_box_plan_execute_recon(hm_ctx *ctx, hm_box_plan *plan, const double *d_po
 size_t hm_box_plan_bytes(const hm_box_plan *plan);
 /* Mesh used for the triple product (n_mesh when regrid is off). */
 int hm_box_plan_triple_mesh(const hm_box_plan *plan, int m[3]);
+
+/* ---------------------------------------------------------------- */
+/* special functions                                                  */
+/* ---------------------------------------------------------------- */
+/* gsl_sf_bessel_jl (bessel.hpp:20), the node values of the j_l spline the
+ * bispectrum shot noise and the three-point function sample; host code,
+ * no GPU needed.  NaN for l < 0 or x < 0. */
+double hm_sph_bessel_jl(int l, double x);
 
 /* ---------------------------------------------------------------- */
 /* reconstruction (particle_reconstruction.hpp)                       */

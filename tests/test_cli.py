@@ -112,6 +112,14 @@ def test_end_to_end_pk_and_bk_match_oracle(tmp_path):
     assert np.allclose(got[:, 2], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
     assert np.allclose(got[:, 3], ref.imag, atol=1e-6 * np.abs(ref.real).max())
     assert not (tmp_path / "out" / "bk000_01.dat").exists()
+    ini = tmp_path / "bk110.ini"
+    hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="bk", ith_kbin=1, ell1=1, ell2=1, ELL=0, **kw)
+    r = run(ini, tmp_path)
+    assert r.returncode == 0, r.stdout
+    got = hio.read_table(tmp_path / "out" / "bk110_01.dat")
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.08, n_kbin=4, ith_kbin=1, ell1=1, ell2=1, ELL=0)
+    ref, _, _ = po.box_bk(p, posd)
+    assert np.allclose(got[:, 2], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
     for ell in (0, 2):
         ini = tmp_path / f"xi{ell}.ini"
         hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="2PCF", ell1=ell, ELL=ell, rmin=40.0,

@@ -90,7 +90,7 @@ def shell_counts(n, L, kmin, kmax, nk):
 
 BK_CASES = [
     # n, L, N, kmin, kmax, nk, expected triple mesh
-    (64, 2000.0, 20000, 0.01, 0.025, 4, 48),   # exact re-grid 64 -> 48
+    (64, 2000.0, 20000, 0.01, 0.025, 4, 40),   # exact re-grid 64 -> 40 (4 s + 1 = 37)
     (32, 1000.0, 20000, 0.02, 0.08, 4, 32),    # shells pruned, no re-grid
     (64, 2000.0, 20000, 0.005, 0.05, 10, 64),  # 10 shells: three 4-shell groups in the fused triple kernel
     (64, 1500.0, 15000, 0.01, 0.04, 7, 48),    # 7 shells: a partial group; re-grid 64 -> 48
@@ -151,3 +151,51 @@ def test_metric_mesh_512_properties(ctx):
     assert np.allclose(b, b.T, rtol=0, atol=0)
     b64 = r64["bk"][..., 0]
     assert np.abs(b - b64).max() <= 2e-4 * np.abs(b64).max()
+
+
+_GEN_CACHE = {}
+
+
+def oracle_bk_general(n, L, pos, kmin, kmax, nk, ells):
+    key = (n, L, pos.shape[0], kmin, kmax, nk, ells, float(pos.sum()))
+    if key not in _GEN_CACHE:
+        out = np.zeros((nk, nk), complex)
+        bks = np.zeros((nk, nk), complex)
+        sns = np.zeros((nk, nk), complex)
+        for ith in range(nk):
+            p = po.param(n, L, po.TSC, kmin=kmin, kmax=kmax, n_kbin=nk, ith_kbin=ith, ell1=ells[0], ell2=ells[1],
+                         ELL=ells[2])
+            out[ith], bks[ith], sns[ith] = po.box_bk(p, pos)
+        _GEN_CACHE[key] = (out, bks, sns)
+    return _GEN_CACHE[key]
+
+
+GEN_CASES = [
+    # n, L, N, kmin, kmax, nk, ells, expected triple mesh
+    (32, 1000.0, 20000, 0.02, 0.08, 4, (1, 1, 0), 32),
+    (32, 1000.0, 20000, 0.02, 0.08, 4, (2, 0, 2), 32),
+    (32, 1000.0, 20000, 0.02, 0.08, 4, (0, 2, 2), 32),
+    (32, 1000.0, 20000, 0.02, 0.08, 4, (1, 0, 1), 32),   # odd l1 + l2: imaginary shot noise
+    (64, 2000.0, 20000, 0.01, 0.025, 4, (1, 1, 2), 40),  # re-grid 64 -> 40
+]
+
+
+@pytest.mark.parametrize("case", GEN_CASES)
+@pytest.mark.parametrize("mode", ["fp64", "fp64_plain", "fp32"])
+def test_bk_general_multipole(ctx, case, mode):
+    """B_{l1 l2 L}(k1, k2) for every ordered pair: bk and sn sums and the normalised
+    output against the oracle (tolerances as for B_000)."""
+    n, L, N, kmin, kmax, nk, ells, M = case
+    pos = clustered_catalogue(N, L, seed=n + nk + sum(ells), nclump=30, sigma=30.0)
+    ref, ref_bk, ref_sn = oracle_bk_general(n, L, pos, kmin, kmax, nk, ells)
+    prec = hm.FP32 if mode == "fp32" else hm.FP64
+    plain = mode == "fp64_plain"
+    plan = hm.BoxPlan(ctx, n, L, "TSC", n_kbin=nk, kmin=kmin, kmax=kmax, pk_ells=(0,), bk=True, precision=prec,
+                      regrid=not plain, prune=not plain, bk_ells=ells)
+    assert plan.triple_mesh == ((n, n, n) if plain else (M, M, M))
+    res = plan.execute(pos)
+    c = lambda a: a[..., 0] + 1j * a[..., 1]
+    tol = BK_TOL[prec]
+    assert np.abs(c(res["bk_sum"]) - ref_bk).max() <= tol * np.abs(ref_bk).max()
+    assert np.abs(c(res["sn_sum"]) - ref_sn).max() <= tol * np.abs(ref_sn).max()
+    assert np.abs(c(res["bk"]) - ref).max() <= tol * np.abs(ref).max()

@@ -335,3 +335,113 @@ def test_xi_plane_wave_closed_form(ell):
         assert nm[j] == sel.sum()
         expect = (field[sel] * y[sel]).sum() / sel.sum() if sel.sum() else 0.0
         assert abs(xi[j] - expect) <= 1e-12 * abs(A) ** 2 / L ** 3 * 2 / min(W1, W2) ** 2
+
+
+@pytest.mark.parametrize("ells", [(1, 1, 0), (2, 0, 2), (1, 1, 2)])
+def test_box_bk_general_ell_bruteforce(ells):
+    """bk_save of bispec.hpp:1100-1182 for (l1, l2, L): the direct triangle sum
+    sum_m w_m sum_{q1 in S1, q2 in S2} Y(l1, m)(q1) Y(l2, -m)(q2) d(q1) d(q2) d(-q1-q2) / W^3."""
+    l1, l2, L_ = ells
+    n, L = 10, 600.0
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.05, n_kbin=4, ith_kbin=1, ell1=l1, ell2=l2, ELL=L_)
+    kb = po.kbin(p)
+    pos = clustered_catalogue(2500, L, seed=17, nclump=6, sigma=40.0)
+    out, bk_save, sn_save = po.box_bk(p, pos)
+    d = po.fourier_transform(p, po.box_density_fluctuation(p, pos))
+    dk = 2 * np.pi / L
+    idx = np.arange(n)
+    sgn = np.where(idx < n // 2, idx, idx - n)
+    modes = [(i, j, k) for i in range(n) for j in range(n) for k in range(n)]
+    kvec = {m: np.array([sgn[m[0]], sgn[m[1]], sgn[m[2]]], float) * dk for m in modes}
+    a = {m: d[m] / po.assignment_function(p, kvec[m]) for m in modes}
+    dkb = kb[1] - kb[0]
+
+    def shell(kc):
+        lo = kc - dkb * 0.5 if kc > dkb * 0.5 else 0.0
+        return [m for m in modes if lo < np.sqrt(np.sum(kvec[m] ** 2)) <= kc + dkb * 0.5]
+
+    def ylm(l, m, v):
+        out = np.zeros(2)
+        po.lib().or_ylm(l, m, po.dp(np.ascontiguousarray(v, dtype=np.float64)), po.dp(out))
+        return out[0] + 1j * out[1]
+
+    weights = []
+    for m1 in range(-l1, l1 + 1):
+        w = po.wigner3j(l1, l2, L_, 0, 0, 0) * po.wigner3j(l1, l2, L_, m1, -m1, 0)
+        w *= (2 * L_ + 1) * (2 * l1 + 1) * (2 * l2 + 1)
+        if abs(w) >= 1e-10 and abs(m1) <= l2:
+            weights.append((m1, w))
+    s1 = shell(kb[1])
+    for ik in range(len(kb)):
+        s2 = shell(kb[ik])
+        tot = 0.0
+        for m1, w in weights:
+            y1 = {q: ylm(l1, m1, kvec[q]) for q in s1}
+            y2 = {q: ylm(l2, -m1, kvec[q]) for q in s2}
+            for q1 in s1:
+                for q2 in s2:
+                    q3 = tuple((-(q1[c] + q2[c])) % n for c in range(3))
+                    tot += w * y1[q1] * y2[q2] * a[q1] * a[q2] * a[q3]
+        expect = tot / (len(s1) * len(s2) * L ** 3) * (L ** 3 / n ** 3) * n ** 3
+        assert bk_save[ik] == pytest.approx(expect, rel=1e-9, abs=1e-10 * abs(expect) + 1e-300)
+
+
+@pytest.mark.parametrize("ells", [(1, 1, 0), (2, 0, 2), (0, 2, 2)])
+def test_box_bk_general_ell_shotnoise(ells):
+    """sn_save of bispec.hpp:912-1073 for (l1, l2, L) from numpy.fft, scipy j_l and the
+    pinned Y_lm: the P terms enter only with l2 = 0 (row bin) or l1 = 0 (column bin)."""
+    from scipy.special import spherical_jn
+
+    l1, l2, L_ = ells
+    n, L, N = 10, 600.0, 2000
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.05, n_kbin=4, ith_kbin=2, ell1=l1, ell2=l2, ELL=L_)
+    kb = po.kbin(p)
+    pos = clustered_catalogue(N, L, seed=19, nclump=5, sigma=50.0)
+    _, _, sn_save = po.box_bk(p, pos)
+    d = po.fourier_transform(p, po.box_density_fluctuation(p, pos))
+    nn = d.copy()
+    nn[0, 0, 0] += N
+    dk = 2 * np.pi / L
+    idx = np.arange(n)
+    sgn = np.where(idx < n // 2, idx, idx - n)
+    g = np.zeros((n, n, n), complex)
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                kv = np.array([sgn[i], sgn[j], sgn[k]], float) * dk
+                g[i, j, k] = (d[i, j, k] * np.conj(nn[i, j, k]) - N * po.shotnoise_function(p, kv)) \
+                    / po.assignment_function(p, kv) ** 2 / L ** 3
+    xi = np.fft.ifftn(g) * g.size
+    dr = L / n
+    R = np.stack(np.meshgrid(sgn * dr, sgn * dr, sgn * dr, indexing="ij"), -1).reshape(-1, 3)
+    r = np.linalg.norm(R, axis=1)
+
+    def ylm_grid(l, m):
+        out = np.zeros(len(R), complex)
+        for q, v in enumerate(R):
+            o = np.zeros(2)
+            po.lib().or_ylm(l, m, po.dp(np.ascontiguousarray(v)), po.dp(o))
+            out[q] = o[0] + 1j * o[1]
+        return out
+
+    expect = np.zeros(len(kb), complex)
+    for m1 in range(-l1, l1 + 1):
+        m2 = -m1
+        if abs(m2) > l2:
+            continue
+        w = po.wigner3j(l1, l2, L_, 0, 0, 0) * po.wigner3j(l1, l2, L_, m1, m2, 0)
+        w *= (2 * L_ + 1) * (2 * l1 + 1) * (2 * l2 + 1)
+        if abs(w) < 1e-10:
+            continue
+        if l2 == 0:
+            pk, _ = po.calc_power_spectrum(p, d, nn, kb, float(N), l1, m1)
+            expect += w * pk[2]
+        if l1 == 0:
+            pk, _ = po.calc_power_spectrum(p, d, nn, kb, float(N), l2, m2)
+            expect += w * pk
+        Y = ylm_grid(l1, m1) * ylm_grid(l2, m2)
+        for ik in range(len(kb)):
+            s = np.sum(spherical_jn(l1, kb[2] * r) * spherical_jn(l2, kb[ik] * r) * xi.ravel() * Y)
+            expect[ik] += w * (1j ** (l1 + l2)) * s * (L ** 3 / n ** 3)
+    scale = np.abs(expect).max()
+    assert np.abs(sn_save - expect).max() <= 1e-8 * scale
