@@ -55,6 +55,8 @@ class BoxConfig(ct.Structure):
         ("n_xi_ells", ct.c_int),
         ("xi_ells", ct.c_int * MAX_ELLS),
         ("bk_ell", ct.c_int * 3),
+        ("zeta", ct.c_int),
+        ("zeta_ell", ct.c_int * 3),
     ]
 
 
@@ -70,6 +72,9 @@ class BoxResult(ct.Structure):
         ("xi", ct.POINTER(ct.c_double)),
         ("xi_raw", ct.POINTER(ct.c_double)),
         ("xi_nmode", ct.POINTER(ct.c_longlong)),
+        ("zeta", ct.POINTER(ct.c_double)),
+        ("zeta_sum", ct.POINTER(ct.c_double)),
+        ("zeta_sn", ct.POINTER(ct.c_double)),
     ]
 
 
@@ -377,7 +382,7 @@ class BoxPlan:
 
     def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2,
                  pk_ells=(0, 2, 4), bk=True, precision=FP32, regrid=True, prune=True, xi_ells=(), n_rbin=25,
-                 rmin=30.0, rmax=150.0, bk_ells=(0, 0, 0)):
+                 rmin=30.0, rmax=150.0, bk_ells=(0, 0, 0), zeta=False, zeta_ells=(0, 0, 0)):
         self.ctx = ctx
         cfg = BoxConfig()
         cfg.mesh = make_mesh(n_mesh, boxsize, assign)
@@ -396,6 +401,8 @@ class BoxPlan:
         cfg.rmax = float(rmax)
         for i in range(3):
             cfg.bk_ell[i] = int(bk_ells[i])
+            cfg.zeta_ell[i] = int(zeta_ells[i])
+        cfg.zeta = 1 if zeta else 0
         cfg.n_xi_ells = len(xi_ells)
         for i, l in enumerate(xi_ells):
             cfg.xi_ells[i] = int(l)
@@ -445,6 +452,8 @@ class BoxPlan:
             "shell_nmode": np.zeros(nk, np.int64),
             "xi": np.zeros((len(self.xi_ells), self.n_rbin)), "xi_raw": np.zeros((len(self.xi_ells), self.n_rbin)),
             "xi_nmode": np.zeros(self.n_rbin, np.int64),
+            "zeta": np.zeros((self.n_rbin, self.n_rbin, 2)), "zeta_sum": np.zeros((self.n_rbin, self.n_rbin, 2)),
+            "zeta_sn": np.zeros((self.n_rbin, self.n_rbin, 2)),
         }
         r = BoxResult()
         dp = ct.POINTER(ct.c_double)
@@ -459,6 +468,9 @@ class BoxPlan:
         r.xi = out["xi"].ctypes.data_as(dp)
         r.xi_raw = out["xi_raw"].ctypes.data_as(dp)
         r.xi_nmode = out["xi_nmode"].ctypes.data_as(lp)
+        r.zeta = out["zeta"].ctypes.data_as(dp)
+        r.zeta_sum = out["zeta_sum"].ctypes.data_as(dp)
+        r.zeta_sn = out["zeta_sn"].ctypes.data_as(dp)
         if recon is None:
             self.ctx.check(self.ctx.lib.hm_box_plan_execute(self.ctx.h, self.h, dpos.ptr, int(npart), ct.byref(r)),
                            "hm_box_plan_execute")

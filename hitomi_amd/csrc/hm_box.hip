@@ -717,6 +717,117 @@ static int launch_tri_z_gram(hm_ctx* ctx, int L, const C<T>* F, long long fstrid
 }
 
 // ---------------------------------------------------------------------------
+// three-point function (calcThreePointCorrelationFunctionForBOX,
+// bispec.hpp:1205-1424): every r-bin field j_l(|k| r) Y(l, m)(k) (d/W) / V
+// (field.hpp:403-445) formed on the first pass of a full inverse transform
+// ---------------------------------------------------------------------------
+template <typename T> struct BesselSrc {
+    const C<T>* dk;
+    int n[3], nzh;
+    AxisTabs t;
+    int wp, nr;
+    double inv_v;
+    const double* rb;
+    int l[2], m[2];
+    const double* sy[2];
+    const double* sc[2];
+    int plain = 0;  // 1: (d/W) / V alone (calcInverseFourierTransform after the window correction)
+    // x pass of a full complex mesh: b = y * n2 + z, p = x, f = field (set f / nr, r-bin f % nr)
+    __device__ inline C<T> operator()(int f, int, int b, int p, long long) const {
+#pragma clang fp contract(off)
+        const int i = p, j = b / n[2], k = b % n[2];
+        C<T> d;
+        if (k < nzh) {
+            d = dk[((long long)i * n[1] + j) * nzh + k];
+        } else {
+            const int im = (n[0] - i) % n[0], jm = (n[1] - j) % n[1];
+            d = cconj<T>(dk[((long long)im * n[1] + jm) * nzh + (n[2] - k)]);
+        }
+        const double kx = t.kv[0][i], ky = t.kv[1][j], kz = t.kv[2][k];
+        const double kmag = kmag3(kx, ky, kz);
+        const double wf = wpow((t.wf[0][i] * t.wf[1][j]) * t.wf[2][k], wp);
+        const double dr = (double)d.x / wf, di = (double)d.y / wf;
+        if (plain) return mk<T>((T)(dr * inv_v), (T)(di * inv_v));
+        const int set = f / nr, a = f % nr;
+        const Cplx y = ylm_red(l[set], m[set], kx, ky, kz);
+        const double tr = y.re * dr - y.im * di, ti = y.re * di + y.im * dr;
+        const double jl = jl_spline_eval(sy[set], sc[set], kmag * rb[a]);
+        return mk<T>((T)(jl * tr * inv_v), (T)(jl * ti * inv_v));
+    }
+};
+
+// Shot noise of the three-point function (field.hpp:979-1113): xi(r) weighted
+// by Y(l1, m1)(r) Y(l2, m2)(r) summed over m (the polynomial A(mu)), binned
+// with the 0.5 Mpc/h fine index below 10^4 fine bins and the 3PCF edges;
+// the even part of xi adds to the real sums (and counts), the odd part (imag)
+// to the imaginary ones.
+constexpr int FINE_NZ = 10000;  // field.hpp:1027
+struct ZetaSnArgs {
+    int n[3], s[3], c[3];
+    double dr[3];
+    long long nrows;
+    const int2* f2b;
+    int nr;
+    AngPoly A;
+    int imag, count;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_xi_ang(const T* __restrict__ xr, ZetaSnArgs a, double* __restrict__ part,
+                                                   long long* __restrict__ cnt) {
+#pragma clang fp contract(off)
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.nrows) return;
+    const int i = support_map((int)(row / a.c[1]), a.s[0], a.n[0]);
+    const int j = support_map((int)(row % a.c[1]), a.s[1], a.n[1]);
+    const double rx = mesh_vec(i, a.n[0], a.dr[0]), ry = mesh_vec(j, a.n[1], a.dr[1]);
+    const T* line = xr + row * (long long)a.n[2];
+    double* P = part + row * 2 * (long long)a.nr;
+    long long* Cn = cnt + row * (long long)a.nr;
+    for (int kc = 0; kc < a.c[2]; kc++) {
+        const int k = support_map(kc, a.s[2], a.n[2]);
+        const double rz = mesh_vec(k, a.n[2], a.dr[2]);
+        const double rmag = kmag3(rx, ry, rz);
+        const int ri = (int)(rmag / FINE_DR + 0.5);
+        if (ri >= FINE_NZ) continue;
+        const int2 b = a.f2b[ri];
+        if (b.x < 0) continue;
+        const double w = (fabs(rmag) < 1.0e-15) ? a.A.a0 : ang_eval(a.A, rz / rmag);
+        const double v = w * (double)line[k];
+        for (int bb = b.x; bb <= b.y; bb++) {
+            P[2 * bb + a.imag] += v;
+            if (a.count) Cn[bb] += 1;
+        }
+    }
+}
+
+// A(mu) = sum_m w_m Y(l1, m)(r) Y(l2, -m)(r) as Chebyshev coefficients (exact:
+// a polynomial of degree l1 + l2 in mu), and its value a0 at r = 0.
+static void make_ang(int l1, int l2, const std::vector<int>& ml, const std::vector<double>& wl, double a0, AngPoly& A) {
+    const int deg = l1 + l2;
+    A.deg = deg;
+    std::vector<double> fv(deg + 1);
+    for (int j = 0; j <= deg; j++) {
+        const double mu = std::cos(M_PI * (j + 0.5) / (deg + 1));
+        const double sx = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+        double acc = 0.0;
+        for (size_t q = 0; q < ml.size(); q++) {
+            const Cplx y1 = ylm_red(l1, ml[q], sx, 0.0, mu);
+            const Cplx y2 = ylm_red(l2, -ml[q], sx, 0.0, mu);
+            acc += wl[q] * (y1.re * y2.re - y1.im * y2.im);
+        }
+        fv[j] = acc;
+    }
+    for (int k = 0; k < ANG_MAX; k++) A.c[k] = 0.0;
+    for (int k = 0; k <= deg; k++) {
+        double c = 0.0;
+        for (int j = 0; j <= deg; j++) c += fv[j] * std::cos(M_PI * k * (j + 0.5) / (deg + 1));
+        A.c[k] = (k == 0 ? 1.0 : 2.0) * c / (deg + 1);
+    }
+    A.a0 = a0;
+}
+
+// ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
 struct BoxPlanImpl {
@@ -764,7 +875,18 @@ struct BoxPlanImpl {
     int e_row = -1, e_col = -1;       // their indices in bin_ells
     AngPoly ang;
     bool sn_odd = false;              // odd l1 + l2: the odd part of xi enters the Bessel sweep too
-    DevBuf jtab2, gpairs_d, hist2, sn_pairs2;           // (a, nk + b): the cross sums of the two Y-weighted shell sets
+    DevBuf jtab2, hist2, sn_pairs2;
+    DevBuf gpairs_d;  // (a, nk + b): the cross sums of the two Y-weighted shell sets
+    // three-point function
+    bool zeta = false, z_gen = false, z_odd = false;
+    int zl[3] = {0, 0, 0};
+    std::vector<int> zmlist;
+    std::vector<double> zwlist;
+    AngPoly zang;
+    std::vector<short2> zpairs;       // 000: a <= b on one field set; else (a, nr + b)
+    int zrs[3] = {-1, -1, -1};
+    long long z_rows = 0;
+    DevBuf zf2b_d, zrb_d, zsy[2], zsc[2], zfields, zr, zpart, zcnt, zred, zcnt_red, zpairs_d;
 };
 
 static int alloc(hm_ctx* ctx, BoxPlanImpl* p, DevBuf& b, size_t bytes) {
@@ -818,8 +940,21 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
         if (alloc(ctx, p, p->sid, sizeof(char2) * (size_t)ns)) return -1;
     }
     if (alloc(ctx, p, p->shell_cnt, sizeof(unsigned long long) * (size_t)p->nk)) return -1;
+    if (p->zeta) {
+        const int nr = p->cfg.n_rbin;
+        const int nf = p->z_gen ? 2 * nr : nr;
+        if (!p->xi_half.p && alloc(ctx, p, p->xi_half, sizeof(C<T>) * (size_t)m.nh())) return -1;
+        if (alloc(ctx, p, p->zfields, sizeof(C<T>) * (size_t)m.ng() * (size_t)(nf + 1)) ||
+            alloc(ctx, p, p->zr, sizeof(T) * (size_t)p->z_rows * (size_t)m.n[2]) ||
+            alloc(ctx, p, p->zpart, sizeof(double) * 2 * (size_t)nr * (size_t)std::max(p->z_rows, (long long)1) +
+                                        sizeof(double) * 2 * 256 * (size_t)((m.ng() + p->gram_vpb - 1) / p->gram_vpb)) ||
+            alloc(ctx, p, p->zcnt, sizeof(long long) * (size_t)nr * (size_t)p->z_rows) ||
+            alloc(ctx, p, p->zred, sizeof(double) * 2 * 256) || alloc(ctx, p, p->zcnt_red, sizeof(long long) * nr) ||
+            alloc(ctx, p, p->zpairs_d, sizeof(short2) * p->zpairs.size()))
+            return -1;
+    }
     if (p->nxe > 0) {
-        if (!p->cfg.bk && alloc(ctx, p, p->xi_half, sizeof(C<T>) * (size_t)m.nh())) return -1;
+        if (!p->cfg.bk && !p->xi_half.p && alloc(ctx, p, p->xi_half, sizeof(C<T>) * (size_t)m.nh())) return -1;
         const size_t cols = (size_t)p->nr * (size_t)p->nxe;
         if (alloc(ctx, p, p->xi_r, sizeof(T) * (size_t)p->xi_rows * (size_t)m.n[2])) return -1;
         if (alloc(ctx, p, p->xi_part, sizeof(double) * cols * (size_t)p->xi_rows)) return -1;
@@ -901,28 +1036,7 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
         if (p->l2 == 0) p->e_row = need(p->l1);
         if (p->l1 == 0) p->e_col = need(p->l2);
         if ((int)p->bin_ells.size() > BIN_MAX_ELLS) return fail("box plan: too many multipoles");
-        // A(mu) = sum_m w_m Y(l1, m) Y(l2, -m) as Chebyshev coefficients (exact: degree l1 + l2)
-        const int deg = p->l1 + p->l2;
-        p->ang.deg = deg;
-        std::vector<double> fv(deg + 1);
-        for (int j = 0; j <= deg; j++) {
-            const double mu = std::cos(M_PI * (j + 0.5) / (deg + 1));
-            const double sx = std::sqrt(std::max(0.0, 1.0 - mu * mu));
-            double acc = 0.0;
-            for (size_t q = 0; q < p->mlist.size(); q++) {
-                const Cplx y1 = ylm_red(p->l1, p->mlist[q], sx, 0.0, mu);
-                const Cplx y2 = ylm_red(p->l2, -p->mlist[q], sx, 0.0, mu);
-                acc += p->wlist[q] * (y1.re * y2.re - y1.im * y2.im);
-            }
-            fv[j] = acc;
-        }
-        for (int k = 0; k <= deg; k++) {
-            double c = 0.0;
-            for (int j = 0; j <= deg; j++) c += fv[j] * std::cos(M_PI * k * (j + 0.5) / (deg + 1));
-            p->ang.c[k] = (k == 0 ? 1.0 : 2.0) * c / (deg + 1);
-        }
-        for (int k = deg + 1; k < ANG_MAX; k++) p->ang.c[k] = 0.0;
-        p->ang.a0 = p->w_const;
+        make_ang(p->l1, p->l2, p->mlist, p->wlist, p->w_const, p->ang);
     }
     p->ne = (int)p->bin_ells.size();
     // k bins, ToolClass::setKbin (tools.hpp:118-124)
@@ -1061,6 +1175,62 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
         }
         p->xi_rows = (long long)support_count(p->rs[0], m.n[0]) * support_count(p->rs[1], m.n[1]);
     }
+    // three-point function: multipole weights, r-bins with the 3PCF edges (field.hpp:1066-1095)
+    std::vector<int2> zf2b;
+    std::vector<double> zrb;
+    if (cfg->zeta) {
+        p->zeta = true;
+        for (int a = 0; a < 3; a++) p->zl[a] = cfg->zeta_ell[a];
+        if (p->zl[0] < 0 || p->zl[1] < 0 || p->zl[2] < 0 || p->zl[0] > 8 || p->zl[1] > 8 || p->zl[2] > 16)
+            return fail("box plan: three-point multipole out of range");
+        if (std::fabs(wigner_3j(p->zl[0], p->zl[1], p->zl[2], 0, 0, 0)) < 1.0e-10)
+            return fail("box plan: this multipole combination is not allowed (wigner_3j(ell1, ell2, ELL, 0,0,0) == 0)");
+        if (cfg->n_rbin < 2 || cfg->n_rbin > 64 || !(cfg->rmax > cfg->rmin))
+            return fail("box plan: the three-point function needs n_rbin in [2, 64] and rmax > rmin");
+        const double dr0 = m.L[0] / m.n[0], dr1 = m.L[1] / m.n[1], dr2 = m.L[2] / m.n[2];
+        (void)dr0, (void)dr1, (void)dr2;
+        p->z_gen = !(p->zl[0] == 0 && p->zl[1] == 0 && p->zl[2] == 0);
+        p->z_odd = ((p->zl[0] + p->zl[1]) & 1) != 0;
+        double wconst = 0.0;
+        for (int m1 = -p->zl[0]; m1 <= p->zl[0]; m1++)
+            for (int m2 = -p->zl[1]; m2 <= p->zl[1]; m2++) {
+                double w = wigner_3j(p->zl[0], p->zl[1], p->zl[2], 0, 0, 0) *
+                           wigner_3j(p->zl[0], p->zl[1], p->zl[2], m1, m2, 0);
+                w *= (double)(2 * p->zl[2] + 1) * (double)(2 * p->zl[0] + 1) * (double)(2 * p->zl[1] + 1);
+                if (std::fabs(w) < 1.0e-10) continue;
+                p->zmlist.push_back(m1);
+                p->zwlist.push_back(w);
+                if (p->zl[0] == 0 && p->zl[1] == 0) wconst += w;
+            }
+        make_ang(p->zl[0], p->zl[1], p->zmlist, p->zwlist, wconst, p->zang);
+        const int nr = cfg->n_rbin;
+        zrb.resize(nr);
+        const double drr = (cfg->rmax - cfg->rmin) / (double)(nr - 1);
+        for (int i = 0; i < nr; i++) zrb[i] = cfg->rmin + drr * (double)i;  // tools.hpp:126-132
+        std::vector<double> drb(nr - 1), lo(nr), up(nr);
+        for (int j = 0; j < nr - 1; j++) drb[j] = zrb[j + 1] - zrb[j];
+        double rumax = 0.0;
+        for (int j = 0; j < nr; j++) {
+            lo[j] = (j == 0) ? ((zrb[j] > drb[j] * 0.5) ? (zrb[j] - drb[j] * 0.5) : 0.0) : zrb[j] - drb[j - 1] * 0.5;
+            up[j] = (j == nr - 1) ? (zrb[j] + drb[j - 1] * 0.5) : (zrb[j] + drb[j] * 0.5);
+            rumax = std::max(rumax, up[j]);
+        }
+        const std::string ez = fine_to_coarse(lo, up, FINE_NZ, FINE_DR, zf2b);
+        if (!ez.empty()) return fail("box plan: " + ez);
+        for (int a = 0; a < 3; a++) {
+            const double dr = m.L[a] / (double)m.n[a];
+            const long long sr = (long long)std::floor((rumax + FINE_DR) / dr) + 1;
+            p->zrs[a] = (!cfg->prune || 2 * sr + 1 >= m.n[a] || sr >= m.n[a] / 2) ? -1 : (int)sr;
+        }
+        p->z_rows = (long long)support_count(p->zrs[0], m.n[0]) * support_count(p->zrs[1], m.n[1]);
+        if (p->z_gen) {
+            for (int a = 0; a < nr; a++)
+                for (int b = 0; b < nr; b++) p->zpairs.push_back(make_short2((short)a, (short)(nr + b)));
+        } else {
+            for (int a = 0; a < nr; a++)
+                for (int b = a; b < nr; b++) p->zpairs.push_back(make_short2((short)a, (short)b));
+        }
+    }
     // device tables
     std::vector<double> tabs;
     host_axis_tables(m, tabs, p->tab_off);
@@ -1078,6 +1248,24 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     if (e != hipSuccess) return fail("box plan: table upload failed");
     int rc = cfg->precision == HM_FP64 ? plan_alloc<double>(ctx, p) : plan_alloc<float>(ctx, p);
     if (rc) { plan_destroy(p); return nullptr; }
+    if (p->zeta) {
+        if (alloc(ctx, p, p->zf2b_d, sizeof(int2) * FINE_NZ) || alloc(ctx, p, p->zrb_d, sizeof(double) * zrb.size()) ||
+            hipMemcpy(p->zf2b_d.p, zf2b.data(), sizeof(int2) * FINE_NZ, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->zrb_d.p, zrb.data(), sizeof(double) * zrb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->zpairs_d.p, p->zpairs.data(), sizeof(short2) * p->zpairs.size(), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return fail("box plan: three-point table upload failed");
+        for (int s2 = 0; s2 < 2; s2++) {
+            if (s2 == 1 && p->zl[1] == p->zl[0]) break;
+            std::vector<double> sy, sc;
+            jl_spline(p->zl[s2], sy, sc);
+            if (alloc(ctx, p, p->zsy[s2], sizeof(double) * sy.size()) ||
+                alloc(ctx, p, p->zsc[s2], sizeof(double) * sc.size()) ||
+                hipMemcpy(p->zsy[s2].p, sy.data(), sizeof(double) * sy.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(p->zsc[s2].p, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return fail("box plan: spline upload failed");
+        }
+    }
     if (p->nxe > 0) {
         if (alloc(ctx, p, p->f2b_r_d, sizeof(int2) * FINE_NR)) { plan_destroy(p); return nullptr; }
         if (hipMemcpy(p->f2b_r_d.p, f2b_r.data(), sizeof(int2) * FINE_NR, hipMemcpyHostToDevice) != hipSuccess)
@@ -1162,7 +1350,9 @@ void plan_destroy(BoxPlanImpl* p) {
                       &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
                       &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid,
                       &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red, &p->rpos, &p->rw,
-                      &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2};
+                      &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
+                      &p->zsy[1], &p->zsc[0], &p->zsc[1], &p->zfields, &p->zr, &p->zpart, &p->zcnt, &p->zred,
+                      &p->zcnt_red, &p->zpairs_d};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -1263,8 +1453,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     ag.ng = m.ng();
     ag.assign = m.assign;
     const bool recon = nr > 0;
-    if (recon && p->cfg.bk) {
-        ctx->set_error("box execute: the bispectrum after reconstruction is not available");
+    if (recon && (p->cfg.bk || p->zeta)) {
+        ctx->set_error("box execute: the bispectrum and three-point function after reconstruction are not available");
         return -1;
     }
     const double scale = fixed_scale_for(recon ? (double)np + std::fabs(alpha) * (double)nr : (double)np);
@@ -1565,6 +1755,140 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                 return -1;
         }
     }
+    // 6'. three-point function (bispec.hpp:1205-1424)
+    std::vector<double> zacc, zsum, zcntv;
+    if (p->zeta) {
+        const int nr = p->cfg.n_rbin;
+        const int nf = p->z_gen ? 2 * nr : nr;
+        const long long ng = m.ng();
+        C<T>* ZF = (C<T>*)p->zfields.p;
+        C<T>* Z3 = ZF + (long long)nf * ng;
+        BesselSrc<T> bs;
+        bs.dk = dk;
+        for (int a = 0; a < 3; a++) bs.n[a] = m.n[a];
+        bs.nzh = (int)nzh;
+        bs.t = tabs;
+        bs.wp = wp;
+        bs.nr = nr;
+        bs.inv_v = 1.0 / m.volume();
+        bs.rb = (const double*)p->zrb_d.p;
+        bs.l[0] = p->zl[0];
+        bs.l[1] = p->zl[1];
+        bs.sy[0] = (const double*)p->zsy[0].p;
+        bs.sc[0] = (const double*)p->zsc[0].p;
+        bs.sy[1] = (const double*)(p->zl[1] == p->zl[0] ? p->zsy[0].p : p->zsy[1].p);
+        bs.sc[1] = (const double*)(p->zl[1] == p->zl[0] ? p->zsc[0].p : p->zsc[1].p);
+        const long long plane = (long long)m.n[1] * m.n[2];
+        auto full_inverse = [&](const char* nx, const char* ny, const char* nz, C<T>* out, int nfield) -> int {
+            ColGeom gx = geom(0, plane, 1, 1, -1, (int)plane, (int)plane, -1, -1);
+            gx.FS = ng;
+            gx.nf = nfield;
+            if (launch_cols<T, +1>(ctx, nx, m.n[0], bs, out, gx)) return -1;
+            ColGeom gy = geom(plane, m.n[2], m.n[0], m.n[0], -1, m.n[2], m.n[2], -1, -1);
+            gy.FS = ng;
+            gy.nf = nfield;
+            if (fft_cols<T>(ctx, ny, m.n[1], +1, out, out, gy)) return -1;
+            RowGeom rg{(long long)m.n[0] * m.n[1] * nfield, m.n[2], m.n[2], -1};
+            return fft_rows<T>(ctx, nz, m.n[2], +1, out, out, rg);
+        };
+        // (d/W)/V transformed back once (bispec.hpp:1308-1317)
+        bs.plain = 1;
+        if (full_inverse("zeta3_pass_x", "zeta3_pass_y", "zeta3_pass_z", Z3, 1)) return -1;
+        bs.plain = 0;
+        const int nzp = (int)p->zpairs.size();
+        const long long gblocks = (ng + p->gram_vpb - 1) / p->gram_vpb;
+        double* gpart = (double*)p->zpart.p + 2 * (size_t)nr * (size_t)std::max(p->z_rows, (long long)1);
+        zacc.assign(2 * (size_t)nzp, 0.0);
+        for (size_t q = 0; q < p->zmlist.size(); q++) {
+            bs.m[0] = p->zmlist[q];
+            bs.m[1] = -p->zmlist[q];
+            if (full_inverse("zeta_pass_x", "zeta_pass_y", "zeta_pass_z", ZF, nf)) return -1;
+            for (int c0 = 0; c0 < nzp; c0 += 256) {
+                const int cnt = std::min(256, nzp - c0);
+                const short2* gp = (const short2*)p->zpairs_d.p + c0;
+                size_t lds = sizeof(C<T>) * (GRAM_CH + 2 * (size_t)nf * (GRAM_CH + 1));
+                const bool narrow = lds > 65536;
+                if (narrow) lds = sizeof(C<T>) * (32 + 2 * (size_t)nf * 33);
+                if (lds > 65536) {
+                    ctx->set_error("box plan: too many r-bins for the three-point function");
+                    return -1;
+                }
+                if (ctx->launch("zeta_gram", [&] {
+                        if (narrow)
+                            hipLaunchKernelGGL((k_gram_sym<T, 32>), dim3((unsigned)gblocks), dim3(256), lds, ctx->stream,
+                                               ZF, ng, nf, Z3, ng, p->gram_vpb, gp, cnt, (double2*)gpart);
+                        else
+                            hipLaunchKernelGGL((k_gram_sym<T>), dim3((unsigned)gblocks), dim3(256), lds, ctx->stream,
+                                               ZF, ng, nf, Z3, ng, p->gram_vpb, gp, cnt, (double2*)gpart);
+                    }))
+                    return -1;
+                if (reduce_cols<double>(ctx, "gram_reduce", gpart, gblocks, 2 * cnt, (double*)p->zred.p)) return -1;
+                std::vector<double> part(2 * (size_t)cnt);
+                HM_HIP_CHECK(ctx, hipMemcpyAsync(part.data(), p->zred.p, sizeof(double) * part.size(),
+                                                 hipMemcpyDeviceToHost, ctx->stream));
+                HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+                for (int e = 0; e < 2 * cnt; e++) zacc[2 * (size_t)c0 + e] += p->zwlist[q] * part[e];
+            }
+        }
+        // shot noise: xi of (d conj(n) - N S)/W^2/V, weighted by A(mu), binned (field.hpp:979-1113)
+        XiSrc<T> xs;
+        xs.dk = dk;
+        for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
+        xs.nzh = (int)nzh;
+        xs.t = tabs;
+        xs.wp = wp;
+        xs.sn = (double)np;
+        xs.inv_v = 1.0 / m.volume();
+        xs.dc = (double)np;
+        C<T>* xh = (C<T>*)p->xi_half.p;
+        const int cx = support_count(p->zrs[0], m.n[0]), cy = support_count(p->zrs[1], m.n[1]);
+        ZetaSnArgs za;
+        for (int a = 0; a < 3; a++) {
+            za.n[a] = m.n[a];
+            za.s[a] = p->zrs[a];
+            za.c[a] = support_count(p->zrs[a], m.n[a]);
+            za.dr[a] = m.L[a] / (double)m.n[a];
+        }
+        za.nrows = p->z_rows;
+        za.f2b = (const int2*)p->zf2b_d.p;
+        za.nr = nr;
+        za.A = p->zang;
+        if (fill_zero(ctx, p->zpart.p, sizeof(double) * 2 * (size_t)nr * p->z_rows)) return -1;
+        if (fill_zero(ctx, p->zcnt.p, sizeof(long long) * (size_t)nr * p->z_rows)) return -1;
+        for (int par = 0; par < (p->z_odd ? 2 : 1); par++) {
+            xs.odd = par;
+            if (launch_cols<T, +1>(ctx, "zsn_pass_x", m.n[0], xs, xh,
+                                   geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh),
+                                        -1, -1)))
+                return -1;
+            if (fft_cols<T>(ctx, "zsn_pass_y", m.n[1], +1, xh, xh,
+                            geom((long long)m.n[1] * nzh, nzh, cx, m.n[0], p->zrs[0], (int)nzh, (int)nzh, -1, -1)))
+                return -1;
+            SupHalfSrc<T> hs{xh, (int)nzh, m.n[0], m.n[1], p->zrs[0], p->zrs[1], cy};
+            if (launch_c2r<T>(ctx, "zsn_pass_z", m.n[2] / 2, hs, (T*)p->zr.p, p->z_rows, -1)) return -1;
+            za.imag = par;
+            za.count = par == 0;
+            if (ctx->launch("zsn_bin", [&] {
+                    hipLaunchKernelGGL(k_bin_xi_ang<T>, dim3((unsigned)((p->z_rows + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, (const T*)p->zr.p, za, (double*)p->zpart.p,
+                                       (long long*)p->zcnt.p);
+                }))
+                return -1;
+        }
+        if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->zpart.p, p->z_rows, 2 * nr, (double*)p->zred.p))
+            return -1;
+        if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->zcnt.p, p->z_rows, nr,
+                                   (long long*)p->zcnt_red.p))
+            return -1;
+        zsum.resize(2 * (size_t)nr);
+        std::vector<long long> zc(nr);
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(zsum.data(), p->zred.p, sizeof(double) * zsum.size(), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(zc.data(), p->zcnt_red.p, sizeof(long long) * nr, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        zcntv.assign(zc.begin(), zc.end());
+    }
     // 7. results to host
     std::vector<double> pk(ncols), gram(2 * (size_t)npairs), graw((size_t)p->gram_cols), snp((size_t)npairs),
         snp2((size_t)npairs, 0.0);
@@ -1660,6 +1984,38 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                     res->bk[o] = norm * (br - sr);
                     res->bk[o + 1] = norm * (bi - si);
                 }
+            }
+        }
+    }
+    if (p->zeta && res) {
+        const int nr = p->cfg.n_rbin;
+        const double dV = V / (double)m.ng();
+        const int lz = p->zl[0] + p->zl[1];
+        const int ip = lz % 4;
+        const double ir = ip == 0 ? 1.0 : (ip == 2 ? -1.0 : 0.0), ii = ip == 1 ? 1.0 : (ip == 3 ? -1.0 : 0.0);
+        const double sg = (lz & 1) ? -1.0 : 1.0;  // (-1)^(l1+l2) (field.hpp:1104)
+        const double norm = (V / N / N) * (V / N);
+        auto put = [&](double* dst, int r, int c, double re, double im) {
+            if (!dst) return;
+            dst[2 * ((size_t)r * nr + c)] = re;
+            dst[2 * ((size_t)r * nr + c) + 1] = im;
+        };
+        for (size_t q = 0; q < p->zpairs.size(); q++) {
+            const int a = p->zpairs[q].x, b = p->zpairs[q].y % nr;
+            // i^(l1+l2) dV sum_x F1 F2 F3 (bispec.hpp:1381-1387)
+            const double gr = zacc[2 * q] * dV, gi = zacc[2 * q + 1] * dV;
+            const double zr = ir * gr - ii * gi, zi = ir * gi + ii * gr;
+            for (int sym = 0; sym < (p->z_gen ? 1 : 2); sym++) {
+                const int r = sym ? b : a, c = sym ? a : b;
+                double sr = 0.0, si = 0.0;
+                if (r == c && zcntv[r] != 0) {
+                    const double f = sg / (double)zcntv[r] / (double)zcntv[r] / dV;
+                    sr = zsum[2 * r] * f;
+                    si = zsum[2 * r + 1] * f;
+                }
+                put(res->zeta_sum, r, c, zr, zi);
+                put(res->zeta_sn, r, c, sr, si);
+                put(res->zeta, r, c, norm * (zr - sr), norm * (zi - si));
             }
         }
     }

@@ -7,7 +7,7 @@
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
 // Supported now: measure = pk and 2PCF (any ELL, ell1 = ELL, ell2 = 0), with
-// or without reconstruction (flag_recon), and measure = bk for any
+// or without reconstruction (flag_recon), and measure = bk and 3PCF for any
 // (ell1, ell2, ELL) with wigner_3j(ell1, ell2, ELL, 0, 0, 0) != 0 without it.
 // Other measurements are rejected with an error instead of being computed
 // differently.
@@ -53,10 +53,10 @@ static double* upload(Gpu& g, const std::vector<double>& v) {
     return d;
 }
 
-enum Measure { M_PK, M_XI, M_BK };
+enum Measure { M_PK, M_XI, M_BK, M_ZETA };
 
 static int run_box(const Params& p, const Catalog& P_D, Gpu& g, Measure what, int ell, std::vector<double>& pk,
-                   std::vector<double>& bk, std::vector<double>& xi) {
+                   std::vector<double>& bk, std::vector<double>& xi, std::vector<double>* zeta = nullptr) {
     const bool want_bk = what == M_BK;
     hm_box_config cfg;
     std::memset(&cfg, 0, sizeof(cfg));
@@ -82,6 +82,12 @@ static int run_box(const Params& p, const Catalog& P_D, Gpu& g, Measure what, in
     cfg.rmax = p.rmax;
     cfg.n_xi_ells = what == M_XI ? 1 : 0;
     cfg.xi_ells[0] = ell;
+    if (what == M_ZETA) {
+        cfg.zeta = 1;
+        cfg.zeta_ell[0] = p.ell1;
+        cfg.zeta_ell[1] = p.ell2;
+        cfg.zeta_ell[2] = p.ELL;
+    }
     hm_box_plan* plan = hm_box_plan_create(g.ctx, &cfg);
     if (!plan) {
         printf("hm_box_plan_create: %s\n", hm_last_error(g.ctx));
@@ -96,6 +102,10 @@ static int run_box(const Params& p, const Catalog& P_D, Gpu& g, Measure what, in
     if (what == M_PK) res.pk = pk.data();
     if (want_bk) res.bk = bk.data();
     if (what == M_XI) res.xi = xi.data();
+    if (what == M_ZETA && zeta) {
+        zeta->assign(2 * (size_t)p.n_rbin * p.n_rbin, 0.0);
+        res.zeta = zeta->data();
+    }
     int rc = g.d_pos_r ? hm_box_plan_execute_recon(g.ctx, plan, g.d_pos, P_D.n_tot, g.d_pos_r, g.n_r, g.alpha, &res)
                        : hm_box_plan_execute(g.ctx, plan, g.d_pos, P_D.n_tot, &res);
     if (rc) printf("hm_box_plan_execute: %s\n", hm_last_error(g.ctx));
@@ -278,6 +288,30 @@ int main(int argc, char* argv[]) {
                 const size_t o = 2 * ((size_t)ith * nk + i);
                 fprintf(fp, "%.5f \t %.5f \t %.7e \t %.7e\n", kbin[ith], kbin[i], bk[o], bk[o + 1]);
             }
+            fclose(fp);
+        }
+    } else if (p.measure == "3PCF") {
+        if (recon) {
+            printf("measure = 3PCF with flag_recon = True is not available in this build.\n");
+            EXIT();
+        }
+        std::vector<double> zeta;
+        if (run_box(p, P_D, g, M_ZETA, 0, pk, bk, xi, &zeta)) {
+            printf("'calcThreePointCorrelationFunctionForBOX(P_D, param, rbin)' fails.\n");
+            EXIT();
+        }
+        // bispec.hpp:1409-1417, one file per ith_rbin (all rows with bk_all_rows = True)
+        const int nr = p.n_rbin;
+        const int r0 = p.bk_all_rows == "True" ? 0 : p.ith_rbin;
+        const int r1 = p.bk_all_rows == "True" ? nr : p.ith_rbin + 1;
+        for (int ith = r0; ith < r1; ith++) {
+            char buf[2048];
+            snprintf(buf, sizeof(buf), "%s/zeta%d%d%d_%02d.dat", p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith);
+            FILE* fp = fopen(buf, "w");
+            if (!fp) EXIT();
+            fprintf(fp, "## r1 [Mpc/h] \t r2 [Mpc/h] \t zeta \n");
+            for (int i = 0; i < nr; i++)
+                fprintf(fp, "%.5f \t %.5f \t %.7e\n", rbin[ith], rbin[i], zeta[2 * ((size_t)ith * nr + i)]);
             fclose(fp);
         }
     } else {

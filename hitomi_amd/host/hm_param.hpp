@@ -9,7 +9,8 @@
 // data_dir + "/".  Two keys are added, chosen so that no existing key is a
 // substring of them and they are no substring of an existing key:
 //   precision   = FP32 | FP64   (grids and FFTs; bin sums are FP64 in both)
-//   bk_all_rows = True | False  (write bk*_%02d.dat for every ith_kbin)
+//   bk_all_rows = True | False  (write bk*_%02d.dat for every ith_kbin and
+//                                zeta*_%02d.dat for every ith_rbin)
 #pragma once
 
 #include <sys/stat.h>

@@ -122,6 +122,9 @@ typedef struct {
     int n_xi_ells;             /* xi_l for l in xi_ells (calcTwoPointCorrelationFunctionForBOX, powerspec.hpp:334-393) */
     int xi_ells[HM_MAX_ELLS];
     int bk_ell[3];             /* (ell1, ell2, ELL) of the bispectrum multipole; {0, 0, 0}: B_000 */
+    int zeta;                  /* 1: zeta_{ell1 ell2 ELL}(r1, r2) for every (ith_rbin, r2) pair on the r-bins
+                                  (calcThreePointCorrelationFunctionForBOX, bispec.hpp:1205-1424) */
+    int zeta_ell[3];
 } hm_box_config;
 
 /* Host result arrays (any may be NULL). */
@@ -136,6 +139,10 @@ typedef struct {
     double *xi;            /* [n_xi_ells][n_rbin] norm*(2l+1)*Re xi as written to xi%d.dat */
     double *xi_raw;        /* [n_xi_ells][n_rbin] real part of the bin averages of calcCorrelationFunction (field.hpp:961-968) */
     long long *xi_nmode;   /* [n_rbin] mesh points per r-bin */
+    double *zeta;          /* [n_rbin][n_rbin][2] norm*(zeta - sn); row i = file zeta%d%d%d_%02d.dat (real part)
+                              with ith_rbin = i */
+    double *zeta_sum;      /* [n_rbin][n_rbin][2] zeta_save (bispec.hpp:1384-1389) */
+    double *zeta_sn;       /* [n_rbin][n_rbin][2] sn_save (bispec.hpp:1279-1285; diagonal only) */
 } hm_box_result;
 
 hm_box_plan *hm_box_plan_create(hm_ctx *ctx, const hm_box_config *cfg);

@@ -1467,3 +1467,210 @@ int or_box_recon_2pt(const or_param *p, const double *posD, long long nD, const 
     free(f);
     return 0;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* three-point correlation function (box)                               */
+/* ------------------------------------------------------------------ */
+
+/* field.hpp:403-445: IFFT of j_l(|k| r) Y(k) (d/W) / V, Y a complex grid (or NULL: 1) */
+static void ifft_for_3pcf(const or_param *p, const double *density, double rmag, const double *ylm,
+                          const or_bessel *sj, double *out) {
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double kmag = sqrt(kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2]);
+                double wf = or_assignment_function(p, kvec);
+                double dr = density[2 * coord] / wf, di = density[2 * coord + 1] / wf;
+                double yr = ylm ? ylm[2 * coord] : 1.0, yi = ylm ? ylm[2 * coord + 1] : 0.0;
+                double tr, ti;
+                CMUL(yr, yi, dr, di, tr, ti);
+                double jl = or_bessel_eval(sj, kmag * rmag);
+                out[2 * coord] = jl * tr / p->volume;
+                out[2 * coord + 1] = jl * ti / p->volume;
+            }
+    or_fft3d(p, out, +1);
+}
+
+/* field.hpp:979-1113 with Y1 Y2 tables: returns xi[i] (complex) for the
+ * 3PCF bin rule and the (-1)^(l1+l2) / n_mode^2 / dV scaling (Q14) */
+static void corr_for_3pcf(const or_param *p, const double *f1, const double *f2, const double *rbin, double sn,
+                          const double *ylm1, const double *ylm2, double *xi, int *n_mode) {
+    const long long ng = p->n_mesh_tot;
+    double *x3 = (double *)calloc(2 * (size_t)ng, sizeof(double));
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double pr, pi;
+                CMUL(f1[2 * coord], f1[2 * coord + 1], f2[2 * coord], -f2[2 * coord + 1], pr, pi);
+                double S = or_shotnoise_function(p, kvec);
+                pr -= sn * S;
+                double w2 = pow(or_assignment_function(p, kvec), 2);
+                pr /= w2;
+                pi /= w2;
+                x3[2 * coord] = pr / p->volume;
+                x3[2 * coord + 1] = pi / p->volume;
+            }
+    or_fft3d(p, x3, +1);
+    const int n_temp = 10000;
+    const double dr_temp = 0.5;
+    double *xt = (double *)calloc(2 * (size_t)n_temp, sizeof(double));
+    int *nt = (int *)calloc((size_t)n_temp, sizeof(int));
+    double rvec[3], dr[3];
+    dr[0] = p->boxsize[0] / (double)(p->n_mesh[0]);
+    dr[1] = p->boxsize[1] / (double)(p->n_mesh[1]);
+    dr[2] = p->boxsize[2] / (double)(p->n_mesh[2]);
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dr, rvec);
+                double rmag = sqrt(rvec[0] * rvec[0] + rvec[1] * rvec[1] + rvec[2] * rvec[2]);
+                int rindex = (int)(rmag / dr_temp + 0.5);
+                if (rindex < n_temp) {
+                    double yr, yi, tr, ti;
+                    CMUL(ylm1[2 * coord], ylm1[2 * coord + 1], ylm2[2 * coord], ylm2[2 * coord + 1], yr, yi);
+                    CMUL(x3[2 * coord], x3[2 * coord + 1], yr, yi, tr, ti);
+                    xt[2 * rindex] += tr;
+                    xt[2 * rindex + 1] += ti;
+                    nt[rindex]++;
+                }
+            }
+    const int nr = p->n_rbin;
+    double *drb = (double *)malloc(sizeof(double) * (size_t)(nr > 1 ? nr - 1 : 1));
+    for (int j = 0; j < nr - 1; j++) drb[j] = rbin[j + 1] - rbin[j];
+    for (int j = 0; j < nr; j++) {
+        double lower = (j == 0) ? ((rbin[j] > drb[j] * 0.5) ? (rbin[j] - drb[j] * 0.5) : 0.0) : rbin[j] - drb[j - 1] * 0.5;
+        double upper = (j == nr - 1) ? (rbin[j] + drb[j - 1] * 0.5) : (rbin[j] + drb[j] * 0.5);
+        xi[2 * j] = 0.0;
+        xi[2 * j + 1] = 0.0;
+        n_mode[j] = 0;
+        for (int i = 0; i < n_temp; i++) {
+            double rt = (double)i * dr_temp;
+            if ((rt > lower) && (rt <= upper)) {
+                xi[2 * j] += xt[2 * i];
+                xi[2 * j + 1] += xt[2 * i + 1];
+                n_mode[j] += nt[i];
+            }
+        }
+    }
+    double dV = p->volume / (double)ng;
+    for (int j = 0; j < nr; j++) {
+        if (n_mode[j] != 0) {
+            double f = pow(-1.0, p->ell1 + p->ell2) / (double)n_mode[j] / (double)n_mode[j] / dV;
+            xi[2 * j] *= f;
+            xi[2 * j + 1] *= f;
+        } else {
+            xi[2 * j] = 0.0;
+            xi[2 * j + 1] = 0.0;
+        }
+    }
+    free(drb);
+    free(xt);
+    free(nt);
+    free(x3);
+}
+
+/* bispec.hpp:1205-1424 for p->ith_rbin: zeta_save, sn_save (2*n_rbin) and
+ * out = norm * (zeta - sn) (real part written to zeta%d%d%d_%02d.dat). */
+int or_box_zeta(const or_param *p, const double *pos, long long np, const double *rbin, double *zeta_save,
+                double *sn_save, double *out) {
+    const int l1 = p->ell1, l2 = p->ell2, L = p->ELL, nr = p->n_rbin, ith = p->ith_rbin;
+    if (fabs(or_wigner3j(l1, l2, L, 0, 0, 0)) < 1.0e-10) return -1;
+    const long long ng = p->n_mesh_tot;
+    for (int i = 0; i < 2 * nr; i++) {
+        zeta_save[i] = 0.0;
+        sn_save[i] = 0.0;
+    }
+    double *Y1 = grid_alloc(p), *Y2 = grid_alloc(p);
+    int *nm = (int *)malloc(sizeof(int) * nr);
+    double *tmp = (double *)malloc(sizeof(double) * 2 * nr);
+    /* shot noise (bispec.hpp:1229-1290) */
+    {
+        double *N00 = grid_alloc(p), *dnLM = grid_alloc(p);
+        or_box_density(p, pos, np, N00);
+        or_fourier_transform(p, N00);
+        or_box_density_fluctuation(p, pos, np, dnLM);
+        or_fourier_transform(p, dnLM);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, 0);
+                w *= (double)(2 * L + 1) * (double)(2 * l1 + 1) * (double)(2 * l2 + 1);
+                if (fabs(w) < 1.0e-10) continue;
+                store_ylm_config(p, l1, m1, Y1);
+                store_ylm_config(p, l2, m2, Y2);
+                corr_for_3pcf(p, dnLM, N00, rbin, (double)np, Y1, Y2, tmp, nm);
+                sn_save[2 * ith] += w * tmp[2 * ith];
+                sn_save[2 * ith + 1] += w * tmp[2 * ith + 1];
+            }
+        free(N00);
+        free(dnLM);
+    }
+    /* three-point function (bispec.hpp:1300-1400) */
+    {
+        double *dn00 = grid_alloc(p), *dnLM = grid_alloc(p), *t1 = grid_alloc(p), *t2 = grid_alloc(p);
+        or_box_density_fluctuation(p, pos, np, dn00);
+        or_fourier_transform(p, dn00);
+        or_box_density_fluctuation(p, pos, np, dnLM);
+        or_fourier_transform(p, dnLM);
+        or_assignment_correction(p, dnLM);
+        or_inverse_fourier_transform(p, dnLM);
+        or_bessel *sj1 = or_bessel_new(l1), *sj2 = or_bessel_new(l2);
+        const double fac = p->volume / (double)ng;
+        double ir, ii;
+        ipow(l1 + l2, &ir, &ii);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, 0);
+                w *= (double)(2 * L + 1) * (double)(2 * l1 + 1) * (double)(2 * l2 + 1);
+                if (fabs(w) < 1.0e-10) continue;
+                store_ylm_fourier(p, l1, m1, Y1);
+                store_ylm_fourier(p, l2, m2, Y2);
+                ifft_for_3pcf(p, dn00, rbin[ith], Y1, sj1, t1);
+                for (int ir2 = 0; ir2 < nr; ir2++) {
+                    ifft_for_3pcf(p, dn00, rbin[ir2], Y2, sj2, t2);
+                    double sr = 0.0, si = 0.0;
+                    for (long long c = 0; c < ng; c++) {
+                        /* pow(I, l1+l2) * fac * f1 * f2 * f3, left to right */
+                        double ar = ir * fac, ai = ii * fac;
+                        CMUL(ar, ai, t1[2 * c], t1[2 * c + 1], ar, ai);
+                        CMUL(ar, ai, t2[2 * c], t2[2 * c + 1], ar, ai);
+                        CMUL(ar, ai, dnLM[2 * c], dnLM[2 * c + 1], ar, ai);
+                        sr += ar;
+                        si += ai;
+                    }
+                    zeta_save[2 * ir2] += w * sr;
+                    zeta_save[2 * ir2 + 1] += w * si;
+                }
+            }
+        or_bessel_free(sj1);
+        or_bessel_free(sj2);
+        free(dn00);
+        free(dnLM);
+        free(t1);
+        free(t2);
+    }
+    double norm = p->volume / (double)np / (double)np;
+    norm *= (p->volume / (double)np);
+    for (int i = 0; i < nr; i++) {
+        out[2 * i] = norm * (zeta_save[2 * i] - sn_save[2 * i]);
+        out[2 * i + 1] = norm * (zeta_save[2 * i + 1] - sn_save[2 * i + 1]);
+    }
+    free(Y1);
+    free(Y2);
+    free(nm);
+    free(tmp);
+    return 0;
+}

@@ -131,6 +131,10 @@ int or_recon_displace(const or_param *p, double *posD, const double *wD, long lo
 int or_box_recon_2pt(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
                      double alpha, const double *bins, int xi, double *out, int *n_mode);
 
+/* bispec.hpp:1205-1424 for p->ith_rbin: zeta_save, sn_save, out = norm * (zeta - sn) (2*n_rbin each). */
+int or_box_zeta(const or_param *p, const double *pos, long long np, const double *rbin, double *zeta_save,
+                double *sn_save, double *out);
+
 /* Row-major index helper used by the tests. */
 long long or_coord(const or_param *p, int i, int j, int k);
 

@@ -82,6 +82,7 @@ def lib():
                                              ct.c_double, ct.c_double]),
             "or_box_recon_2pt": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, ct.c_int, _D,
                                             ct.POINTER(ct.c_int)]),
+            "or_box_zeta": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, _D, _D]),
             "or_survey_xi": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, ct.c_int, _D, ct.POINTER(ct.c_int)]),
         }
@@ -268,6 +269,20 @@ def box_xi(p: Param, pos):
     if rc:
         raise ValueError("or_box_xi rejected the multipole combination")
     return out
+
+
+def box_zeta(p: Param, pos):
+    """bispec.hpp:1205-1424 for p.ith_rbin: (out, zeta_save, sn_save), complex arrays."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    rb = rbin(p)
+    z = np.zeros(2 * p.n_rbin)
+    sn = np.zeros(2 * p.n_rbin)
+    out = np.zeros(2 * p.n_rbin)
+    rc = lib().or_box_zeta(ct.byref(p), dp(pos), pos.shape[0], dp(rb), dp(z), dp(sn), dp(out))
+    if rc:
+        raise ValueError("or_box_zeta rejected the multipole combination")
+    c = lambda a: a[0::2] + 1j * a[1::2]
+    return c(out), c(z), c(sn)
 
 
 def box_bk(p: Param, pos):

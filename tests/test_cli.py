@@ -131,9 +131,20 @@ def test_end_to_end_pk_and_bk_match_oracle(tmp_path):
         ref = po.box_xi(p, posd)
         assert np.allclose(got[:, 0], po.rbin(p), atol=5e-6)
         assert np.allclose(got[:, 1], ref, rtol=2e-7, atol=1e-7 * np.abs(ref).max())
+    ini = tmp_path / "zeta.ini"
+    hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="3PCF", ith_rbin=3, ell1=1, ell2=1, ELL=0,
+                  rmin=40.0, rmax=200.0, n_rbin=6, **kw)
+    r = run(ini, tmp_path)
+    assert r.returncode == 0, r.stdout
+    got = hio.read_table(tmp_path / "out" / "zeta110_03.dat")
+    p = po.param(n, L, po.TSC, rmin=40.0, rmax=200.0, n_rbin=6, ith_rbin=3, ell1=1, ell2=1, ELL=0)
+    ref, _, _ = po.box_zeta(p, posd)
+    rb = po.rbin(p)
+    assert np.allclose(got[:, 0], rb[3], atol=5e-6) and np.allclose(got[:, 1], rb, atol=5e-6)
+    assert np.allclose(got[:, 2], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
 
 
-SURVEY = os.path.join(ROOT, "hitomi_amd", "bin", "hitomi_survey")
+SURVEY =os.path.join(ROOT, "hitomi_amd", "bin", "hitomi_survey")
 
 
 def run_survey(ini, cwd):
