@@ -24,11 +24,13 @@ struct AxisTabs {
     const double* kv[3];
     const double* wf[3];
     const double* sf[3];
+    const double* iw[3];  // 1 / wf^(2p), p = 1, 2, 3 for NGP, CIC, TSC
 };
 
 inline void host_axis_tables(const Mesh& m, std::vector<double>& out, long long off[3]) {
     const long long ntot = (long long)m.n[0] + m.n[1] + m.n[2];
-    out.assign(3 * ntot, 0.0);
+    out.assign(4 * ntot, 0.0);
+    const int p = m.assign == 0 ? 1 : (m.assign == 1 ? 2 : 3);
     long long o = 0;
     for (int a = 0; a < 3; a++) {
         off[a] = o;
@@ -47,12 +49,23 @@ inline void host_axis_tables(const Mesh& m, std::vector<double>& out, long long 
             out[o + i] = kv;
             out[ntot + o + i] = f;
             out[2 * ntot + o + i] = sfac;
+            out[3 * ntot + o + i] = 1.0 / std::pow(f, 2 * p);
         }
         o += n;
     }
 }
 
 __device__ __forceinline__ double wpow(double r, int p) { return p == 1 ? r : (p == 2 ? r * r : r * r * r); }
+
+// 1/x for a normal positive x: the hardware estimate and two Newton steps
+// (within an ulp or two of the correctly rounded quotient, without the
+// scale/fix-up sequence of a full double division)
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
 
 // |k| exactly as field.hpp:800 evaluates it (no contraction)
 __device__ __forceinline__ double kmag3(double a, double b, double c) {

@@ -185,13 +185,15 @@ template <typename T> struct XiSrc {
         double p2;
         if (i == 0 && j == 0 && k == 0) p2 = re * (re + dc) + im * im;
         else p2 = re * re + im * im;
-        const int ii = (n[0] - i) % n[0], jj = (n[1] - j) % n[1], kk = (n[2] - k) % n[2];
-        const double S1 = (t.sf[0][i] * t.sf[1][j]) * t.sf[2][k];
-        const double W1 = wpow((t.wf[0][i] * t.wf[1][j]) * t.wf[2][k], wp);
-        const double S2 = (t.sf[0][ii] * t.sf[1][jj]) * t.sf[2][kk];
-        const double W2 = wpow((t.wf[0][ii] * t.wf[1][jj]) * t.wf[2][kk], wp);
-        const double g1 = inv_v * ((p2 - sn * S1) / (W1 * W1));
-        const double g2 = inv_v * ((p2 - sn * S2) / (W2 * W2));
+        // S and 1/W^2 of k and of -k from per-axis factors (the y, z ones are
+        // the same along a column and leave the loop)
+        const int ii = i ? n[0] - i : 0, jj = j ? n[1] - j : 0, kk = k ? n[2] - k : 0;
+        const double S1 = t.sf[0][i] * (t.sf[1][j] * t.sf[2][k]);
+        const double Q1 = t.iw[0][i] * (t.iw[1][j] * t.iw[2][k]);
+        const double S2 = t.sf[0][ii] * (t.sf[1][jj] * t.sf[2][kk]);
+        const double Q2 = t.iw[0][ii] * (t.iw[1][jj] * t.iw[2][kk]);
+        const double g1 = inv_v * ((p2 - sn * S1) * Q1);
+        const double g2 = inv_v * ((p2 - sn * S2) * Q2);
         if (odd) return mk<T>((T)0, (T)(-(0.5 * (g1 - g2))));
         return mk<T>((T)(0.5 * (g1 + g2)), (T)0);
     }
@@ -378,23 +380,64 @@ __global__ void k_j0_table(const double* __restrict__ kb, int nk, long long nkey
     J[t] = (x == 0.0) ? 1.0 : sin(x) / x;
 }
 
-// S[a][b] = dV * sum_n2 J1[a][n2] J2[b][n2] H[n2], one block per pair
-__global__ __launch_bounds__(256) void k_sn_pairs(const double* __restrict__ J1, const double* __restrict__ J2,
-                                                 long long nkey, const double* __restrict__ H,
-                                                 const short2* __restrict__ pairs, double dV, double* __restrict__ out) {
-    __shared__ double sm[256];
-    const short2 pr = pairs[blockIdx.x];
-    const double* Ja = J1 + (long long)pr.x * nkey;
-    const double* Jb = J2 + (long long)pr.y * nkey;
-    double acc = 0.0;
-    for (long long key = threadIdx.x; key < nkey; key += 256) acc += Ja[key] * Jb[key] * H[key];
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+// S[a][b] = dV * sum_n2 J1[a][n2] J2[b][n2] H[n2] for all pairs: each workgroup takes SNP_KPB keys,
+// stages J1[a][key] H[key] dV and J2[b][key] for every bin in LDS, SNP_CH keys
+// at a time, and every lane accumulates up to SNP_NPT pairs; part[blk][pair]
+// is summed over workgroups afterwards.  Reads each table once instead of once
+// per pair.
+constexpr int SNP_CH = 32;
+constexpr int SNP_CHP = SNP_CH + 1;  // row pitch in LDS (bank spread)
+constexpr int SNP_KPB = 512;
+constexpr int SNP_NPT = 16;
+__global__ __launch_bounds__(256) void k_sn_pairs_blk(const double* __restrict__ J1, const double* __restrict__ J2,
+                                                     int nk, long long nkey, const double* __restrict__ H,
+                                                     const short2* __restrict__ pairs, int npairs, double dV,
+                                                     double* __restrict__ part) {
+    extern __shared__ double snsm[];
+    double* ja = snsm;
+    double* jb = snsm + (size_t)nk * SNP_CHP;
+    const int t = threadIdx.x;
+    const long long k0 = (long long)blockIdx.x * SNP_KPB;
+    const long long k1 = (k0 + SNP_KPB < nkey) ? k0 + SNP_KPB : nkey;
+    double acc[SNP_NPT];
+    short2 pr[SNP_NPT];
+#pragma unroll
+    for (int i = 0; i < SNP_NPT; i++) {
+        acc[i] = 0.0;
+        const int q = t + 256 * i;
+        pr[i] = q < npairs ? pairs[q] : make_short2(0, 0);
+    }
+    for (long long c0 = k0; c0 < k1; c0 += SNP_CH) {
+        for (int e = t; e < nk * SNP_CH; e += 256) {
+            const int a = e / SNP_CH, c = e % SNP_CH;
+            const long long key = c0 + c;
+            double va = 0.0, vb = 0.0;
+            if (key < k1) {
+                va = J1[(long long)a * nkey + key] * (H[key] * dV);
+                vb = J2[(long long)a * nkey + key];
+            }
+            ja[a * SNP_CHP + c] = va;
+            jb[a * SNP_CHP + c] = vb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SNP_NPT; i++) {
+            if (t + 256 * i < npairs) {
+                const double* xa = ja + pr[i].x * SNP_CHP;
+                const double* xb = jb + pr[i].y * SNP_CHP;
+                double s = acc[i];
+#pragma unroll 8
+                for (int c = 0; c < SNP_CH; c++) s += xa[c] * xb[c];
+                acc[i] = s;
+            }
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = sm[0] * dV;
+#pragma unroll
+    for (int i = 0; i < SNP_NPT; i++) {
+        const int q = t + 256 * i;
+        if (q < npairs) part[(long long)blockIdx.x * npairs + q] = acc[i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -569,9 +612,10 @@ template <int L> struct TriFits { static constexpr bool value = (size_t)TRI_B * 
 constexpr int TRI_NT = 512;  // lanes per workgroup of the fused kernel
 
 template <typename T, int L>
-__global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ F, long long fstride, int nk, int s_shell,
-                                                   int s_f3, long long nrows, int ngrp, int nblk, int nsl,
-                                                   const C<T>* __restrict__ tw, double* __restrict__ part) {
+__global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ F, long long fstride, int nk,
+                                                   const int* __restrict__ zsup, int s_f3, long long nrows, int ngrp,
+                                                   int nblk, int nsl, const C<T>* __restrict__ tw,
+                                                   double* __restrict__ part) {
     constexpr int LP = LdsPad<L>::LP;
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
@@ -597,20 +641,29 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
     // the next row's in-support coefficients are loaded into registers while
     // the current row is transformed and reduced
     constexpr int NPRE = (TRI_B * L + TRI_NT - 1) / TRI_NT;
+    static_assert(NPRE <= 32, "one mask bit per staged element");
     C<T> pre[NPRE];
+    // which of this lane's staged elements lie in their field's z support
+    // (the same for every row): shell f in zsup[f], delta/W (last line) in s_f3
+    unsigned need = 0;
+#pragma unroll
+    for (int q = 0; q < NPRE; q++) {
+        const int e = t + q * TRI_NT;
+        const int f = e / L, z = e % L;
+        bool in = false;
+        if (e < TRI_B * L) {
+            if (f < nk) in = in_support(z, zsup[f], L);
+            else if (f == TRI_B - 1) in = in_support(z, s_f3, L);
+        }
+        need |= (unsigned)in << q;
+    }
     auto load_row = [&](long long r) {
 #pragma unroll
         for (int q = 0; q < NPRE; q++) {
             const int e = t + q * TRI_NT;
             const int f = e / L, z = e % L;
             C<T> v = mk<T>((T)0, (T)0);
-            if (e < TRI_B * L) {
-                if (f < nk) {
-                    if (in_support(z, s_shell, L)) v = F[(long long)f * fstride + r * L + z];
-                } else if (f == TRI_B - 1) {
-                    if (in_support(z, s_f3, L)) v = F[(long long)nk * fstride + r * L + z];
-                }
-            }
+            if ((need >> q) & 1u) v = F[(long long)(f < nk ? f : nk) * fstride + r * L + z];
             pre[q] = v;
         }
     };
@@ -669,7 +722,7 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
 }
 
 template <typename T, int L>
-static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, int nk, int s_shell, int s_f3,
+static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, int nk, const int* zsup, int s_f3,
                                long long nrows, int ngrp, int nblk, int nsl, double* part) {
     const size_t fft_lds = (size_t)TRI_B * LdsPad<L>::LP * sizeof(C<T>);
     const size_t red_lds = (size_t)TRI_NT * 4 * sizeof(double);
@@ -678,7 +731,7 @@ static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, in
     if (!tw) return -1;
     const unsigned blocks = (unsigned)((nrows + TRI_RPW - 1) / TRI_RPW);
     return ctx->launch("tri_z_gram", [&] {
-        hipLaunchKernelGGL((k_tri_z_gram<T, L>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride, nk, s_shell, s_f3,
+        hipLaunchKernelGGL((k_tri_z_gram<T, L>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride, nk, zsup, s_f3,
                            nrows, ngrp, nblk, nsl, tw, part);
     });
 }
@@ -697,14 +750,14 @@ static int tri_fused_ok(int prec, int L) {
 }
 
 template <typename T>
-static int launch_tri_z_gram(hm_ctx* ctx, int L, const C<T>* F, long long fstride, int nk, int s_shell, int s_f3,
+static int launch_tri_z_gram(hm_ctx* ctx, int L, const C<T>* F, long long fstride, int nk, const int* zsup, int s_f3,
                              long long nrows, int ngrp, int nblk, int nsl, double* part) {
     if constexpr (sizeof(T) == sizeof(float)) {
         switch (L) {
 #define HM_CASE(LL)                                                                                                    \
     case LL:                                                                                                           \
         if constexpr (TriFits<LL>::value)                                                                              \
-            return launch_tri_z_gram_L<T, LL>(ctx, F, fstride, nk, s_shell, s_f3, nrows, ngrp, nblk, nsl, part);       \
+            return launch_tri_z_gram_L<T, LL>(ctx, F, fstride, nk, zsup, s_f3, nrows, ngrp, nblk, nsl, part);          \
         break;
             HM_POW2_LENGTHS(HM_CASE)
             HM_MIXED_LENGTHS(HM_CASE)
@@ -875,7 +928,12 @@ struct BoxPlanImpl {
     int e_row = -1, e_col = -1;       // their indices in bin_ells
     AngPoly ang;
     bool sn_odd = false;              // odd l1 + l2: the odd part of xi enters the Bessel sweep too
-    DevBuf jtab2, hist2, sn_pairs2;
+    DevBuf jtab2, hist2, sn_pairs2, snp_part;
+    // per shell field: its own supports on the triple mesh (x, y, z; -1 full)
+    // and the z one alone
+    std::vector<int4> fsup;
+    std::vector<int> fsz;
+    DevBuf fsup_d, fsz_d;
     DevBuf gpairs_d;  // (a, nk + b): the cross sums of the two Y-weighted shell sets
     // three-point function
     bool zeta = false, z_gen = false, z_odd = false;
@@ -918,6 +976,9 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
             return -1;
         if (alloc(ctx, p, p->pairs_d, sizeof(short2) * p->pairs.size())) return -1;
         if (alloc(ctx, p, p->sn_pairs, sizeof(double) * p->pairs.size())) return -1;
+        if (alloc(ctx, p, p->snp_part,
+                  sizeof(double) * p->pairs.size() * (size_t)((p->nkey + SNP_KPB - 1) / SNP_KPB)))
+            return -1;
         const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
         // general multipole: two shell sets, delta/W transformed, and delta/W kept untransformed for the
         // shell loader of every m
@@ -973,6 +1034,7 @@ static AxisTabs axis_tabs(const BoxPlanImpl* p) {
         t.kv[a] = base + p->tab_off[a];
         t.wf[a] = base + ntot + p->tab_off[a];
         t.sf[a] = base + 2 * ntot + p->tab_off[a];
+        t.iw[a] = base + 3 * ntot + p->tab_off[a];
     }
     return t;
 }
@@ -1274,6 +1336,30 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     if (cfg->bk) {
         if (hipMemcpy(p->pairs_d.p, p->pairs.data(), sizeof(short2) * p->pairs.size(), hipMemcpyHostToDevice) != hipSuccess)
             return fail("box plan: pair table upload failed");
+        // every mode of shell j has |k| <= upper[j] (field.hpp:371-373), so |n_a| <= upper[j] / dk_a; one
+        // index of margin for the rounding of |k|
+        const int nsh = p->bk_gen ? 2 * p->nk : p->nk;
+        p->fsup.resize(nsh);
+        p->fsz.resize(nsh);
+        for (int f = 0; f < nsh; f++) {
+            const int j = f % p->nk;
+            int sv[3];
+            for (int a = 0; a < 3; a++) {
+                if (p->s[a] < 0) {
+                    sv[a] = sup_eff(-1, p->M[a]);
+                } else {
+                    const double dka = 2.0 * M_PI / m.L[a];
+                    const long long sj = (long long)std::floor(p->upper[j] / dka) + 1;
+                    sv[a] = sup_eff((int)std::min<long long>(sj, p->s[a]), p->M[a]);
+                }
+            }
+            p->fsup[f] = make_int4(sv[0], sv[1], sv[2], 0);
+            p->fsz[f] = sv[2];
+        }
+        if (alloc(ctx, p, p->fsup_d, sizeof(int4) * nsh) || alloc(ctx, p, p->fsz_d, sizeof(int) * nsh) ||
+            hipMemcpy(p->fsup_d.p, p->fsup.data(), sizeof(int4) * nsh, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->fsz_d.p, p->fsz.data(), sizeof(int) * nsh, hipMemcpyHostToDevice) != hipSuccess)
+            return fail("box plan: shell support upload failed");
         if (p->bk_gen) {
             std::vector<short2> gp(p->pairs);
             for (auto& q : gp) q.y = (short)(q.y + p->nk);
@@ -1350,7 +1436,7 @@ void plan_destroy(BoxPlanImpl* p) {
                       &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
                       &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid,
                       &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red, &p->rpos, &p->rw,
-                      &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
+                      &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->snp_part, &p->fsup_d, &p->fsz_d, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
                       &p->zsy[1], &p->zsc[0], &p->zsc[1], &p->zfields, &p->zr, &p->zpart, &p->zcnt, &p->zred,
                       &p->zcnt_red, &p->zpairs_d};
     for (DevBuf* b : bufs) dev_free(*b);
@@ -1375,18 +1461,29 @@ static ColGeom geom(long long OS, long long PS, int n_outer, int outer_full, int
 
 // x pass over nf fields (stride FS) of support sup: lines (y, z) in the
 // support, points x in the support.
-static ColGeom tri_geom_x(const int M[3], const int sup[3], long long FS, int nf) {
+static ColGeom tri_geom_x(const int M[3], const int sup[3], long long FS, int nf, const int4* fsup = nullptr) {
     ColGeom g = geom(M[2], (long long)M[1] * M[2], support_count(sup[1], M[1]), M[1], sup[1],
                      support_count(sup[2], M[2]), M[2], sup[2], sup[0]);
     g.FS = FS;
     g.nf = nf;
+    if (fsup) {
+        g.fsup = fsup;
+        g.role_outer = 1;
+        g.role_inner = 2;
+        g.role_point = 0;
+    }
     return g;
 }
 // y pass: lines (x all, z in the support), points y in the support.
-static ColGeom tri_geom_y(const int M[3], const int sup[3], long long FS, int nf) {
+static ColGeom tri_geom_y(const int M[3], const int sup[3], long long FS, int nf, const int4* fsup = nullptr) {
     ColGeom g = geom((long long)M[1] * M[2], M[2], M[0], M[0], -1, support_count(sup[2], M[2]), M[2], sup[2], sup[1]);
     g.FS = FS;
     g.nf = nf;
+    if (fsup) {
+        g.fsup = fsup;
+        g.role_inner = 2;
+        g.role_point = 1;
+    }
     return g;
 }
 
@@ -1617,12 +1714,17 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                        (double*)hbuf);
             }))
             return -1;
+        const long long snb = (p->nkey + SNP_KPB - 1) / SNP_KPB;
         if (ctx->launch("sn_pairs", [&] {
-                hipLaunchKernelGGL(k_sn_pairs, dim3((unsigned)npairs), dim3(256), 0, ctx->stream,
-                                   (const double*)p->jtab.p,
-                                   (const double*)(p->bk_gen && p->l2 != p->l1 ? p->jtab2.p : p->jtab.p), p->nkey, (const double*)hbuf,
-                                   (const short2*)p->pairs_d.p, dV, (double*)sbuf);
+                hipLaunchKernelGGL(k_sn_pairs_blk, dim3((unsigned)snb), dim3(256),
+                                   sizeof(double) * 2 * (size_t)p->nk * SNP_CHP, ctx->stream, (const double*)p->jtab.p,
+                                   (const double*)(p->bk_gen && p->l2 != p->l1 ? p->jtab2.p : p->jtab.p), p->nk,
+                                   p->nkey, (const double*)hbuf, (const short2*)p->pairs_d.p, (int)npairs, dV,
+                                   (double*)p->snp_part.p);
             }))
+            return -1;
+        if (reduce_cols<double>(ctx, "sn_pairs_reduce", (const double*)p->snp_part.p, snb, (int)npairs,
+                                (double*)sbuf))
             return -1;
         }
         // 5. shell fields and delta/W on the triple mesh (field.hpp:346-401, 610-639, 328-344)
@@ -1655,6 +1757,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             return -1;
         if (p->bk_gen)
             HM_HIP_CHECK(ctx, hipMemcpyAsync(F3, Draw, sizeof(C<T>) * (size_t)nvox, hipMemcpyDeviceToDevice, ctx->stream));
+        const int4* fsup = (const int4*)p->fsup_d.p;
+        const int* fsz = (const int*)p->fsz_d.p;
         ShellMaskSrc<T> ms;
         ms.D = Draw;
         ms.sid = (const char2*)p->sid.p;
@@ -1683,11 +1787,11 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             for (size_t q = 0; q < p->mlist.size(); q++) {
                 ms.m1 = p->mlist[q];
                 ms.m2 = -p->mlist[q];
-                ColGeom gx = tri_geom_x(p->M, st.s, nvox, nsh);
+                ColGeom gx = tri_geom_x(p->M, st.s, nvox, nsh, fsup);
                 if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
-                ColGeom gy = tri_geom_y(p->M, st.s, nvox, nsh);
+                ColGeom gy = tri_geom_y(p->M, st.s, nvox, nsh, fsup);
                 if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
-                RowGeom rg{nrows * nsh, p->M[2], p->M[2], st.s[2]};
+                RowGeom rg{nrows * nsh, p->M[2], p->M[2], st.s[2], fsz, nrows};
                 if (fft_rows<T>(ctx, "tri_pass_z", p->M[2], +1, F, F, rg)) return -1;
                 for (int c0 = 0; c0 < npairs; c0 += 256) {
                     const int cnt = std::min(256, npairs - c0);
@@ -1722,23 +1826,23 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             }
         } else {
             // x pass of every shell (one launch), then of delta/W in place
-            ColGeom gx = tri_geom_x(p->M, st.s, nvox, p->nk);
+            ColGeom gx = tri_geom_x(p->M, st.s, nvox, p->nk, fsup);
             if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
             ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
             if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
             // y pass
-            ColGeom gy = tri_geom_y(p->M, st.s, nvox, p->nk);
+            ColGeom gy = tri_geom_y(p->M, st.s, nvox, p->nk, fsup);
             if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
             ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
             if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;
             // 6. z pass and triple sums
             const long long nrows = (long long)p->M[0] * p->M[1];
             if (p->fused) {
-                if (launch_tri_z_gram<T>(ctx, p->M[2], F, nvox, p->nk, st.s[2], st.s3[2], nrows, p->ngrp, p->nblk, p->nsl,
+                if (launch_tri_z_gram<T>(ctx, p->M[2], F, nvox, p->nk, fsz, st.s3[2], nrows, p->ngrp, p->nblk, p->nsl,
                                          (double*)p->gram_part.p))
                     return -1;
             } else {
-                RowGeom rg{nrows * p->nk, p->M[2], p->M[2], st.s[2]};
+                RowGeom rg{nrows * p->nk, p->M[2], p->M[2], st.s[2], fsz, nrows};
                 if (fft_rows<T>(ctx, "tri_pass_z", p->M[2], +1, F, F, rg)) return -1;
                 RowGeom rg3{nrows, p->M[2], p->M[2], st.s3[2]};
                 if (fft_rows<T>(ctx, "tri3_pass_z", p->M[2], +1, F3, F3, rg3)) return -1;

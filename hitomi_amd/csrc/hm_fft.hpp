@@ -229,19 +229,51 @@ struct ColGeom {
     int n_outer, outer_full, outer_s;  // compact count, full extent, support
     int n_inner, inner_full, inner_s;
     int point_s;                       // support on the transformed axis
+    // Per-batch supports: when set, batch f takes the supports of its outer,
+    // inner and transformed axes from fsup[f] (x, y, z; component role_* of
+    // it, -1: keep the one above).  The grid is sized for the largest; lines
+    // outside a batch's own support are neither read nor written.
+    const int4* fsup = nullptr;
+    int role_outer = -1, role_inner = -1, role_point = -1;
 };
 
-// Row-pass geometry: element (r, p) at r*RS + p.
+// Row-pass geometry: element (r, p) at r*RS + p.  With fps set, row r takes
+// the support fps[r / rpf] on its points instead of point_s.
 struct RowGeom {
     long long n_rows, RS_in, RS_out;
     int point_s;
+    const int* fps = nullptr;
+    long long rpf = 1;
 };
+
+__device__ __forceinline__ int int4_get(const int4& q, int i) { return i == 0 ? q.x : (i == 1 ? q.y : q.z); }
 
 // Plain global source / destination.
 template <typename T> struct GSrc {
     const C<T>* p;
     __device__ inline C<T> operator()(int /*f*/, int /*o*/, int /*b*/, int /*pt*/, long long addr) const { return p[addr]; }
 };
+
+// Workgroups are handed to the 8 XCDs round-robin by linear id, and each XCD
+// has its own L2.  Renumbering the grid so that XCD x runs the x-th contiguous
+// slice of linear ids keeps tiles that share 128-byte lines (neighbouring
+// column tiles of 64 or 128 bytes) in one L2 instead of fetching every line
+// once per XCD.  A bijection on [0, gx*gy*gz) whatever the grid.
+constexpr int HM_NXCD = 8;
+// (Grids here hold fewer than 2^32 workgroups.)
+__device__ __forceinline__ uint3 xcd_block() {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned tot = gx * gy * gridDim.z;
+    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned q = tot / HM_NXCD, r = tot % HM_NXCD;
+    const unsigned x = lin % HM_NXCD, i = lin / HM_NXCD;
+    const unsigned t = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+    uint3 b;
+    b.x = t % gx;
+    b.y = (t / gx) % gy;
+    b.z = t / (gx * gy);
+    return b;
+}
 
 // Column FFT: grid.x = tiles of B compact inner lines, grid.y = compact outer index.
 template <typename T, int L, int DIR, class Src>
@@ -251,18 +283,33 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     constexpr int LP = LdsPad<L>::LP;
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
-    const int o = support_map(blockIdx.y, g.outer_s, g.outer_full);
-    const int f = blockIdx.z;
+    const uint3 wg = xcd_block();
+    const int f = (int)wg.z;
+    int outer_s = g.outer_s, inner_s = g.inner_s, point_s = g.point_s, n_inner = g.n_inner;
+    if (g.fsup) {
+        const int4 q = g.fsup[f];
+        if (g.role_outer >= 0) {
+            outer_s = int4_get(q, g.role_outer);
+            if ((int)wg.y >= support_count(outer_s, g.outer_full)) return;
+        }
+        if (g.role_inner >= 0) {
+            inner_s = int4_get(q, g.role_inner);
+            n_inner = support_count(inner_s, g.inner_full);
+            if ((int)wg.x * B >= n_inner) return;
+        }
+        if (g.role_point >= 0) point_s = int4_get(q, g.role_point);
+    }
+    const int o = support_map((int)wg.y, outer_s, g.outer_full);
     const long long obase = (long long)f * g.FS + (long long)o * g.OS;
     const int tid = threadIdx.x;
     const int bl = tid % B;
-    const int bc = blockIdx.x * B + bl;  // compact inner index
-    const bool bvalid = bc < g.n_inner;
-    const int b = bvalid ? support_map(bc, g.inner_s, g.inner_full) : 0;
+    const int bc = (int)wg.x * B + bl;  // compact inner index
+    const bool bvalid = bc < n_inner;
+    const int b = bvalid ? support_map(bc, inner_s, g.inner_full) : 0;
     constexpr int PSTEP = FFT_NT / B;
     for (int p = tid / B; p < L; p += PSTEP) {
         C<T> v = mk<T>((T)0, (T)0);
-        if (bvalid && in_support(p, g.point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
+        if (bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
         sm[bl * LP + p] = v;
     }
     __syncthreads();
@@ -284,7 +331,8 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows(const C<T>* __restrict__ sr
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, p = e % L;
         C<T> v = mk<T>((T)0, (T)0);
-        if (r0 + b < g.n_rows && in_support(p, g.point_s, L)) v = src[(r0 + b) * g.RS_in + p];
+        if (r0 + b < g.n_rows && in_support(p, g.fps ? g.fps[(r0 + b) / g.rpf] : g.point_s, L))
+            v = src[(r0 + b) * g.RS_in + p];
         sm[b * LP + p] = v;
     }
     __syncthreads();

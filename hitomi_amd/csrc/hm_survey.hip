@@ -717,6 +717,7 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
         tabs.kv[ax] = base + p->tab_off[ax];
         tabs.wf[ax] = base + ntot + p->tab_off[ax];
         tabs.sf[ax] = base + 2 * ntot + p->tab_off[ax];
+        tabs.iw[ax] = base + 3 * ntot + p->tab_off[ax];
     }
     const int nbin = p->measure == HM_SURVEY_PK ? p->nk : p->nr;
     const long long nrow_bins = p->measure == HM_SURVEY_PK ? p->nrows : p->nrows_r;
