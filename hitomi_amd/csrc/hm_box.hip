@@ -607,7 +607,7 @@ template <typename T> struct ShellMaskSrc {
 constexpr int TRI_B = 24;
 constexpr int TRI_RPW = 16;  // rows per workgroup
 
-template <int L> struct TriFits { static constexpr bool value = (size_t)TRI_B * (L + 1) * sizeof(float2) <= 65536; };
+template <int L> struct TriFits { static constexpr bool value = (size_t)TRI_B * LdsPad<L, false>::LP * sizeof(float2) <= 65536; };
 
 constexpr int TRI_NT = 512;  // lanes per workgroup of the fused kernel
 
@@ -616,7 +616,7 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
                                                    const int* __restrict__ zsup, int s_f3, long long nrows, int ngrp,
                                                    int nblk, int nsl, const C<T>* __restrict__ tw,
                                                    double* __restrict__ part) {
-    constexpr int LP = LdsPad<L>::LP;
+    constexpr int LP = LdsPad<L, false>::LP;
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
     const int t = threadIdx.x;
@@ -676,7 +676,7 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
         }
         __syncthreads();
         if (r + 1 < r1) load_row(r + 1);
-        lds_fft<T, +1, L, TRI_B, TRI_NT>(sm, tw);
+        lds_fft<T, +1, L, TRI_B, TRI_NT, false>(sm, tw);
         if (active) {
             const C<T>* f3 = sm + (TRI_B - 1) * LP;
             const C<T>* fa = sm + 4 * bi * LP;
@@ -692,9 +692,11 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
 #pragma unroll
                 for (int i = 0; i < 4; i++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        ar[i * 4 + j] += c[i].x * g[j].x - c[i].y * g[j].y;
-                        ai[i * 4 + j] += c[i].x * g[j].y + c[i].y * g[j].x;
+                    for (int j = 0; j < 4; j++) {  // two fused multiply-adds per part into the running sum
+                        ar[i * 4 + j] = fma(c[i].x, g[j].x, ar[i * 4 + j]);
+                        ar[i * 4 + j] = fma(-c[i].y, g[j].y, ar[i * 4 + j]);
+                        ai[i * 4 + j] = fma(c[i].x, g[j].y, ai[i * 4 + j]);
+                        ai[i * 4 + j] = fma(c[i].y, g[j].x, ai[i * 4 + j]);
                     }
             }
         }
@@ -724,7 +726,7 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
 template <typename T, int L>
 static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, int nk, const int* zsup, int s_f3,
                                long long nrows, int ngrp, int nblk, int nsl, double* part) {
-    const size_t fft_lds = (size_t)TRI_B * LdsPad<L>::LP * sizeof(C<T>);
+    const size_t fft_lds = (size_t)TRI_B * LdsPad<L, false>::LP * sizeof(C<T>);
     const size_t red_lds = (size_t)TRI_NT * 4 * sizeof(double);
     const size_t lds = fft_lds > red_lds ? fft_lds : red_lds;
     const C<T>* tw = ctx->twiddle<T>(L);

@@ -102,15 +102,21 @@ template <typename T, int DIR> struct Dft<T, 5, DIR> {
 // ---------------------------------------------------------------------------
 // Stockham stages over a tile in LDS: sm[b * LP + p], b < B, p < L
 // ---------------------------------------------------------------------------
-template <int L> struct LdsPad { static constexpr int LP = L + 1; };
+// A line of L points occupies LP slots: point p sits at slot lpos(p) =
+// p + p / 32, one spare slot after every 32 points (plus one at the end).
+// Eight-byte points 8 apart (the first Stockham stage writes j * 8 + t) then
+// fall into 32 different bank pairs across a wavefront instead of 4.
+// SW = false keeps the plain layout (slot p, LP = L + 1), which suits the
+// fused triple-sum kernel's access pattern better.
+template <int L, bool SW = true> struct LdsPad { static constexpr int LP = SW ? L + L / 32 + 1 : L + 1; };
+template <bool SW = true> __device__ __forceinline__ int lpos(int p) { return SW ? p + (p >> 5) : p; }
 
-template <typename T, int DIR, int L, int B, int NT, int NS, int R, int... RS>
-__device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ tw) {
-    constexpr int LP = LdsPad<L>::LP;
+template <typename T, int DIR, int L, int B, int NT, bool SW, int NS, int R, int... RS>
+__device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ tw, int tid) {
+    constexpr int LP = LdsPad<L, SW>::LP;
     constexpr int NBF = L / R;
     constexpr int TOT = B * NBF;
     constexpr int Q = (TOT + NT - 1) / NT;
-    const int tid = threadIdx.x;
     C<T> v[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -120,7 +126,7 @@ __device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ 
             const int k = j % NS;
 #pragma unroll
             for (int t = 0; t < R; t++) {
-                C<T> a = sm[b * LP + j + t * NBF];
+                C<T> a = sm[b * LP + lpos<SW>(j + t * NBF)];
                 if (NS > 1 && t > 0) {
                     const int e = t * k * (L / (NS * R));
                     C<T> w = tw[e];
@@ -141,51 +147,53 @@ __device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ 
             const int k = j % NS;
             const int base = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int t = 0; t < R; t++) sm[b * LP + base + t * NS] = v[q][t];
+            for (int t = 0; t < R; t++) sm[b * LP + lpos<SW>(base + t * NS)] = v[q][t];
         }
     }
     __syncthreads();
-    if constexpr (sizeof...(RS) > 0) lds_stockham<T, DIR, L, B, NT, NS * R, RS...>(sm, tw);
+    if constexpr (sizeof...(RS) > 0) lds_stockham<T, DIR, L, B, NT, SW, NS * R, RS...>(sm, tw, tid);
 }
 
-// Radix plans of the supported line lengths.
-template <typename T, int DIR, int L, int B, int NT = FFT_NT>
-__device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw) {
-    if constexpr (L == 8) lds_stockham<T, DIR, L, B, NT, 1, 8>(sm, tw);
-    else if constexpr (L == 16) lds_stockham<T, DIR, L, B, NT, 1, 4, 4>(sm, tw);
-    else if constexpr (L == 32) lds_stockham<T, DIR, L, B, NT, 1, 8, 4>(sm, tw);
-    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3>(sm, tw);
-    else if constexpr (L == 64) lds_stockham<T, DIR, L, B, NT, 1, 8, 8>(sm, tw);
-    else if constexpr (L == 80) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 5>(sm, tw);
-    else if constexpr (L == 96) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 3>(sm, tw);
-    else if constexpr (L == 128) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 4>(sm, tw);
-    else if constexpr (L == 160) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 5>(sm, tw);
-    else if constexpr (L == 192) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 3>(sm, tw);
-    else if constexpr (L == 256) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4>(sm, tw);
-    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 3, 3>(sm, tw);
-    else if constexpr (L == 320) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 5>(sm, tw);
-    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 2, 3>(sm, tw);
-    else if constexpr (L == 480) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 3, 5>(sm, tw);
-    else if constexpr (L == 512) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 8>(sm, tw);
-    else if constexpr (L == 576) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 3, 3>(sm, tw);
-    else if constexpr (L == 640) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 2, 5>(sm, tw);
-    else if constexpr (L == 768) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4, 3>(sm, tw);
-    else if constexpr (L == 1024) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 4, 4>(sm, tw);
-    else if constexpr (L == 24) lds_stockham<T, DIR, L, B, NT, 1, 8, 3>(sm, tw);
-    else if constexpr (L == 40) lds_stockham<T, DIR, L, B, NT, 1, 8, 5>(sm, tw);
-    else if constexpr (L == 72) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 3>(sm, tw);
-    else if constexpr (L == 120) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 5>(sm, tw);
-    else if constexpr (L == 144) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 3>(sm, tw);
-    else if constexpr (L == 200) lds_stockham<T, DIR, L, B, NT, 1, 8, 5, 5>(sm, tw);
-    else if constexpr (L == 216) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 3, 3>(sm, tw);
-    else if constexpr (L == 240) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 5>(sm, tw);
-    else if constexpr (L == 360) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 3, 5>(sm, tw);
-    else if constexpr (L == 400) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 5, 5>(sm, tw);
-    else if constexpr (L == 432) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 3, 3>(sm, tw);
-    else if constexpr (L == 600) lds_stockham<T, DIR, L, B, NT, 1, 8, 3, 5, 5>(sm, tw);
-    else if constexpr (L == 720) lds_stockham<T, DIR, L, B, NT, 1, 8, 2, 3, 3, 5>(sm, tw);
-    else if constexpr (L == 800) lds_stockham<T, DIR, L, B, NT, 1, 8, 4, 5, 5>(sm, tw);
-    else if constexpr (L == 960) lds_stockham<T, DIR, L, B, NT, 1, 8, 8, 3, 5>(sm, tw);
+// Radix plans of the supported line lengths.  tid: the lane's index in the
+// workgroup (threadIdx.x unless the caller passes its own copy).
+template <typename T, int DIR, int L, int B, int NT = FFT_NT, bool SW = true>
+__device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw, int tid = -1) {
+    if (tid < 0) tid = threadIdx.x;
+    if constexpr (L == 8) lds_stockham<T, DIR, L, B, NT, SW, 1, 8>(sm, tw, tid);
+    else if constexpr (L == 16) lds_stockham<T, DIR, L, B, NT, SW, 1, 4, 4>(sm, tw, tid);
+    else if constexpr (L == 32) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4>(sm, tw, tid);
+    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3>(sm, tw, tid);
+    else if constexpr (L == 64) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8>(sm, tw, tid);
+    else if constexpr (L == 80) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 5>(sm, tw, tid);
+    else if constexpr (L == 96) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3>(sm, tw, tid);
+    else if constexpr (L == 128) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 4>(sm, tw, tid);
+    else if constexpr (L == 160) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 5>(sm, tw, tid);
+    else if constexpr (L == 192) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3>(sm, tw, tid);
+    else if constexpr (L == 256) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 4>(sm, tw, tid);
+    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 320) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 5>(sm, tw, tid);
+    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 2, 3>(sm, tw, tid);
+    else if constexpr (L == 480) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 512) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 8>(sm, tw, tid);
+    else if constexpr (L == 576) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 640) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 2, 5>(sm, tw, tid);
+    else if constexpr (L == 768) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 4, 3>(sm, tw, tid);
+    else if constexpr (L == 1024) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 4, 4>(sm, tw, tid);
+    else if constexpr (L == 24) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3>(sm, tw, tid);
+    else if constexpr (L == 40) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 5>(sm, tw, tid);
+    else if constexpr (L == 72) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 120) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 144) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 200) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 5, 5>(sm, tw, tid);
+    else if constexpr (L == 216) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 240) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 360) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 400) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 5, 5>(sm, tw, tid);
+    else if constexpr (L == 432) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 600) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 5, 5>(sm, tw, tid);
+    else if constexpr (L == 720) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 800) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 5, 5>(sm, tw, tid);
+    else if constexpr (L == 960) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3, 5>(sm, tw, tid);
     else static_assert(L == 8, "unsupported FFT length");
 }
 
@@ -202,12 +210,12 @@ constexpr int pow2floor(int x) {
 // a power of two, within 64 KB of LDS.
 template <typename T, int L> struct ColTile {
     static constexpr int want = (int)(128 / sizeof(C<T>));
-    static constexpr int fit = (int)(65536 / ((L + 1) * sizeof(C<T>)));
+    static constexpr int fit = (int)(65536 / (LdsPad<L>::LP * sizeof(C<T>)));
     static constexpr int B = pow2floor(want < fit ? want : fit);
 };
 template <typename T, int L> struct RowTile {
     static constexpr int want = (4096 / L) > 0 ? (4096 / L) : 1;
-    static constexpr int fit = (int)(65536 / ((L + 1) * sizeof(C<T>)));
+    static constexpr int fit = (int)(65536 / (LdsPad<L>::LP * sizeof(C<T>)));
     static constexpr int B = pow2floor(want < fit ? want : fit);
 };
 
@@ -310,12 +318,12 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     for (int p = tid / B; p < L; p += PSTEP) {
         C<T> v = mk<T>((T)0, (T)0);
         if (bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
-        sm[bl * LP + p] = v;
+        sm[bl * LP + lpos(p)] = v;
     }
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, tw);
     if (bvalid) {
-        for (int p = tid / B; p < L; p += PSTEP) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + p];
+        for (int p = tid / B; p < L; p += PSTEP) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + lpos(p)];
     }
 }
 
@@ -333,13 +341,13 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows(const C<T>* __restrict__ sr
         C<T> v = mk<T>((T)0, (T)0);
         if (r0 + b < g.n_rows && in_support(p, g.fps ? g.fps[(r0 + b) / g.rpf] : g.point_s, L))
             v = src[(r0 + b) * g.RS_in + p];
-        sm[b * LP + p] = v;
+        sm[b * LP + lpos(p)] = v;
     }
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, tw);
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, p = e % L;
-        if (r0 + b < g.n_rows) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + p];
+        if (r0 + b < g.n_rows) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
     }
 }
 
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(FFT_NT) void k_r2c_rows(RSrc src, C<T>* __restrict_
         const int b = e / L, p = e % L;
         C<T> v = mk<T>((T)0, (T)0);
         if (r0 + b < n_rows) v = src(r0 + b, p);
-        sm[b * LP + p] = v;
+        sm[b * LP + lpos(p)] = v;
     }
     __syncthreads();
     lds_fft<T, -1, L, B>(sm, tw);
@@ -366,8 +374,8 @@ __global__ __launch_bounds__(FFT_NT) void k_r2c_rows(RSrc src, C<T>* __restrict_
     for (int e = threadIdx.x; e < B * (L + 1); e += FFT_NT) {
         const int b = e / (L + 1), k = e % (L + 1);
         if (r0 + b >= n_rows) continue;
-        const C<T> zk = sm[b * LP + (k % L)];
-        const C<T> zc = cconj<T>(sm[b * LP + ((L - k) % L)]);
+        const C<T> zk = sm[b * LP + lpos(k % L)];
+        const C<T> zc = cconj<T>(sm[b * LP + lpos((L - k) % L)]);
         const C<T> E = mk<T>((T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y));
         const C<T> d = mk<T>((T)0.5 * (zk.x - zc.x), (T)0.5 * (zk.y - zc.y));
         const C<T> O = mk<T>(d.y, -d.x);  // d / i
@@ -400,14 +408,14 @@ __global__ __launch_bounds__(FFT_NT) void k_c2r_rows(HSrc src, T* __restrict__ d
             const C<T> wd = cmul<T>(w, d);
             v = mk<T>(s.x - wd.y, s.y + wd.x);  // s + i * wd
         }
-        sm[b * LP + k] = v;
+        sm[b * LP + lpos(k)] = v;
     }
     __syncthreads();
     lds_fft<T, +1, L, B>(sm, tw);
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, m = e % L;
         if (r0 + b < n_rows) {
-            const C<T> z = sm[b * LP + m];
+            const C<T> z = sm[b * LP + lpos(m)];
             dst[(r0 + b) * (2 * L) + 2 * m] = z.x;
             dst[(r0 + b) * (2 * L) + 2 * m + 1] = z.y;
         }
@@ -432,7 +440,7 @@ template <typename T> struct RealSrc {
 };
 
 // LDS bytes of the three tile shapes.
-template <typename T, int L> constexpr size_t col_lds() { return (size_t)ColTile<T, L>::B * (L + 1) * sizeof(C<T>); }
-template <typename T, int L> constexpr size_t row_lds() { return (size_t)RowTile<T, L>::B * (L + 1) * sizeof(C<T>); }
+template <typename T, int L> constexpr size_t col_lds() { return (size_t)ColTile<T, L>::B * LdsPad<L>::LP * sizeof(C<T>); }
+template <typename T, int L> constexpr size_t row_lds() { return (size_t)RowTile<T, L>::B * LdsPad<L>::LP * sizeof(C<T>); }
 
 }  // namespace hm
