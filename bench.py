@@ -45,10 +45,27 @@ def support(n, L, kmin, kmax, nk):
     return -1 if (2 * s + 1 >= n or s >= n // 2) else s
 
 
-def algorithmic_bytes(name, n, M, s, nk, npart, elem=8):
-    """Compulsory HBM bytes of one launch of `name` (FP32 complex = 8 B)."""
+def shell_widths(L, kmin, kmax, nk, s):
+    """Per-shell cube widths 2 s_j + 1 on the triple mesh: every mode of shell j
+    has |k| <= upper_j, so |n_a| <= upper_j / dk (+1 for rounding), capped at s."""
+    kb = kmin + (kmax - kmin) / (nk - 1) * np.arange(nk)
+    dkb = kb[1] - kb[0]
+    dk = 2 * np.pi / L
+    return [2 * min(int(np.floor((k + 0.5 * dkb) / dk)) + 1, s) + 1 for k in kb]
+
+
+def n_groups(n):
+    """Distinct i'^2 + j'^2 over minimum-image row indices: the rows the shot-noise xi keeps."""
+    h = np.minimum(np.arange(n), n - np.arange(n)) ** 2
+    return int(np.unique(h[:, None] + h[None, :]).size)
+
+
+def algorithmic_bytes(name, n, M, s, nk, npart, elem=8, ws=None):
+    """Compulsory HBM bytes of one launch of `name` (FP32 complex = 8 B);
+    ws: the shells' own cube widths (default: all 2 s + 1)."""
     half = n * n * (n // 2 + 1) * elem
     w = 2 * s + 1
+    ws = ws or [w] * nk
     W3 = 4 * s + 1 if 4 * s + 1 < M else M
     table = {
         "assign_gather": npart * 24 + n ** 3 * 8,                # sorted positions in, int64 mesh out
@@ -58,14 +75,15 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8):
         "fft_r2c_x": 2 * half,
         "xi_pass_x": 2 * half,
         "xi_pass_y": 2 * half,
-        "xi_pass_z": half + n ** 3 * (elem // 2),
-        # triple-product fields: nk shells of support w, one delta/W of support W3 (compulsory in + out)
+        "xi_pass_z": n_groups(n) * ((n // 2 + 1) * elem + n * (elem // 2)),  # grouped rows, c2r
+        # triple-product fields: shell j on its own cube of width ws[j], one delta/W of support W3
+        # (compulsory in + out)
         "tri_stage": W3 ** 3 * elem * 2 + w ** 3 * 2,
-        "tri_pass_x": w ** 3 * (elem + 2) + nk * M * w * w * elem,
+        "tri_pass_x": sum(v ** 3 * (elem + 2) + M * v * v * elem for v in ws),
         "tri3_pass_x": (W3 ** 3 + M * W3 * W3) * elem,
-        "tri_pass_y": nk * (M * w * w + M * M * w) * elem,
+        "tri_pass_y": sum((M * v * v + M * M * v) * elem for v in ws),
         "tri3_pass_y": (M * W3 * W3 + M * M * W3) * elem,
-        "tri_z_gram": (nk * M * M * w + M * M * W3) * elem,
+        "tri_z_gram": (sum(M * M * v for v in ws) + M * M * W3) * elem,
         "gram": (nk + 1) * M ** 3 * elem,
     }
     return table.get(name)
@@ -74,7 +92,7 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8):
 # HBM bytes per launch of the dominant kernels, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE on this workload and scaled by the counters' reading
 # of a 1 GiB copy (x2.000 for FETCH_SIZE, x1.000 for WRITE_SIZE).
-MEASURED_TRAFFIC = {"tri_z_gram": 2334.7e6}
+MEASURED_TRAFFIC = {"tri_z_gram": 1400.6e6}
 
 
 def make_mocks(count, seed0=1000):
@@ -149,6 +167,7 @@ def run(args):
     plan = hm.BoxPlan(ctx, N_MESH, BOX, "TSC", precision=hm.FP32, **KW)
     M = plan.triple_mesh[0]
     s = support(N_MESH, BOX, KW["kmin"], KW["kmax"], KW["n_kbin"])
+    ws = shell_widths(BOX, KW["kmin"], KW["kmax"], KW["n_kbin"], s)
     log(f"plan: triple mesh {plan.triple_mesh}, support {s}, device bytes {plan.device_bytes / 2**30:.2f} GiB")
 
     # per-kernel breakdown of one (untimed) mock
@@ -166,10 +185,10 @@ def run(args):
     tot = sum(r[0] for r in rows)
     log(f"kernel breakdown of one mock ({tot:.2f} ms of kernel time):")
     for ms, name, nl in rows:
-        b = algorithmic_bytes(name, N_MESH, M, s, KW["n_kbin"], npart)
+        b = algorithmic_bytes(name, N_MESH, M, s, KW["n_kbin"], npart, ws=ws)
         bw = f"{b / (ms / nl) / 1e6:8.0f} GB/s" if b else ""
         log(f"  {name:14s} {ms:8.3f} ms  {nl:3d} launches  {bw}")
-    dom = next(r for r in rows if algorithmic_bytes(r[1], N_MESH, M, s, KW["n_kbin"], npart))
+    dom = next(r for r in rows if algorithmic_bytes(r[1], N_MESH, M, s, KW["n_kbin"], npart, ws=ws))
     dom_name = dom[1]
     ctx.timing(False)
 
@@ -190,7 +209,7 @@ def run(args):
     ctx.timing(False)
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = 3600.0 * 1000.0 / ms_per_step
-    b_dom = algorithmic_bytes(dom_name, N_MESH, M, s, KW["n_kbin"], npart)
+    b_dom = algorithmic_bytes(dom_name, N_MESH, M, s, KW["n_kbin"], npart, ws=ws)
     achieved = b_dom / (ms_dom / n_dom) / 1e6
     plan.close()
     if args.copy_calibration:

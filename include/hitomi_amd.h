@@ -103,7 +103,7 @@ int hm_fft_r2c(hm_ctx *ctx, int precision, const void *d_in, void *d_out, const 
 int hm_fft_c2r(hm_ctx *ctx, int precision, void *d_in, void *d_out, const int n[3]);
 
 /* ---------------------------------------------------------------- */
-/* periodic-box estimators (main_box.cpp:171-228, measure = pk / bk)  */
+/* periodic-box estimators (main_box.cpp:171-228: pk, 2PCF, bk, 3PCF) */
 /* ---------------------------------------------------------------- */
 #define HM_MAX_ELLS 4
 
@@ -147,7 +147,8 @@ typedef struct {
 
 hm_box_plan *hm_box_plan_create(hm_ctx *ctx, const hm_box_config *cfg);
 void hm_box_plan_destroy(hm_ctx *ctx, hm_box_plan *plan);
-/* One mock: d_pos (np x 3 FP64, already periodic) -> P_l, xi_l and B_000. */
+/* One mock: d_pos (np x 3 FP64, already periodic) -> the configured P_l,
+ * xi_l, B_{ell1 ell2 ELL} and zeta_{ell1 ell2 ELL}. */
 int hm_box_plan_execute(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos, long long np, hm_box_result *res);
 /* The reconstructed box (main_box.cpp:199-228, flag_recon = True): P_l and
  * xi_l of D - alpha R with unit weights (field.hpp:528-558), shot noise
@@ -194,7 +195,7 @@ int hm_recon_displace(hm_ctx *ctx, const hm_recon_config *cfg, double *d_pos_d, 
 int hm_periodic_wrap(hm_ctx *ctx, double *d_pos, long long n, const double boxsize[3]);
 
 /* ---------------------------------------------------------------- */
-/* survey estimators (main.cpp:109-166, measure = pk)                 */
+/* survey estimators (main.cpp:109-183: pk, 2PCF, window2PCF)        */
 /* ---------------------------------------------------------------- */
 typedef struct hm_survey_plan hm_survey_plan;
 
