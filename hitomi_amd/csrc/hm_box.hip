@@ -50,21 +50,31 @@ struct BinArgs {
 };
 
 template <typename T>
+// dk2 set: the cross spectrum dk conj(dk2) (complex; the mirrored mode takes
+// its conjugate) with the real parts into part and the imaginary ones into
+// part_im.
 __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, BinArgs a, double* __restrict__ part,
-                                                long long* __restrict__ cnt) {
+                                                long long* __restrict__ cnt, const C<T>* __restrict__ dk2 = nullptr,
+                                                double* __restrict__ part_im = nullptr) {
 #pragma clang fp contract(off)
     const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= a.nrows) return;
     const int i = support_map((int)(row / a.nrow_y), a.s01, a.n[0]);
     const int j = support_map((int)(row % a.nrow_y), a.s01, a.n[1]);
     double* P = part + row * (long long)a.nk * a.ne;
+    double* Pi = part_im ? part_im + row * (long long)a.nk * a.ne : nullptr;
     long long* Cn = cnt + row * (long long)a.nk;
     const long long rbase = ((long long)i * a.n[1] + j) * a.nzh;
     const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
     for (int kz = 0; kz <= a.nzmax; kz++) {
         const C<T> d = dk[rbase + kz];
         const double re = (double)d.x, imv = (double)d.y;
-        const double p2 = re * re + imv * imv;  // Re f1 conj(f2), f1 = f2
+        double p2 = re * re + imv * imv, pim = 0.0;  // f1 conj(f2), f1 = f2
+        if (dk2) {
+            const C<T> e = dk2[rbase + kz];
+            p2 = re * (double)e.x + imv * (double)e.y;
+            pim = imv * (double)e.x - re * (double)e.y;
+        }
         const int nmir = (kz > 0 && kz < a.n[2] - kz) ? 2 : 1;
         for (int m = 0; m < nmir; m++) {
             const int ii = m ? im : i, jj = m ? jm : j, kk = m ? (a.n[2] - kz) : kz;
@@ -77,12 +87,14 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
             const double S = (a.t.sf[0][ii] * a.t.sf[1][jj]) * a.t.sf[2][kk];
             const double W = wpow((a.t.wf[0][ii] * a.t.wf[1][jj]) * a.t.wf[2][kk], a.wp);
             const double val = (p2 - a.sn * S) / (W * W);
+            const double vim = (m ? -pim : pim) / (W * W);
             const double mu = (fabs(kmag) < 1.0e-15) ? 0.0 : kzv / kmag;
             for (int bb = b.x; bb <= b.y; bb++) {
                 for (int e = 0; e < a.ne; e++) {
                     const int l = a.ells[e];
                     const double y = (l == 0) ? 1.0 : ((fabs(kmag) < 1.0e-15) ? 0.0 : legendre(l, mu));
                     P[bb * a.ne + e] += val * y;
+                    if (Pi) Pi[bb * a.ne + e] += vim * y;
                 }
                 Cn[bb] += 1;
             }
@@ -175,16 +187,24 @@ template <typename T> struct XiSrc {
     int wp;
     double sn, inv_v;
     double dc;  // added to Re d at k = 0 in Re d conj(n): N for the shot-noise field, 0 for xi
-    int odd = 0;  // 1: -i times the odd part of g instead of its even part (its transform is Im xi)
+    int odd = 0;  // 1: the spectrum whose transform is Im xi instead of Re xi
+    const C<T>* dk2 = nullptr;  // set: g from the cross spectrum d conj(d2) (no dc term)
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
     __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
 #pragma clang fp contract(off)
         const int i = p, j = b / nzh, k = b % nzh;
         const C<T> d = dk[addr];
         const double re = (double)d.x, im = (double)d.y;
-        double p2;
-        if (i == 0 && j == 0 && k == 0) p2 = re * (re + dc) + im * im;
-        else p2 = re * re + im * im;
+        double p2, pim = 0.0;
+        if (dk2) {
+            const C<T> e = dk2[addr];
+            p2 = re * (double)e.x + im * (double)e.y;
+            pim = im * (double)e.x - re * (double)e.y;
+        } else if (i == 0 && j == 0 && k == 0) {
+            p2 = re * (re + dc) + im * im;
+        } else {
+            p2 = re * re + im * im;
+        }
         // S and 1/W^2 of k and of -k from per-axis factors (the y, z ones are
         // the same along a column and leave the loop)
         const int ii = i ? n[0] - i : 0, jj = j ? n[1] - j : 0, kk = k ? n[2] - k : 0;
@@ -192,10 +212,13 @@ template <typename T> struct XiSrc {
         const double Q1 = t.iw[0][i] * (t.iw[1][j] * t.iw[2][k]);
         const double S2 = t.sf[0][ii] * (t.sf[1][jj] * t.sf[2][kk]);
         const double Q2 = t.iw[0][ii] * (t.iw[1][jj] * t.iw[2][kk]);
+        // g(k) = a + i b at k and at -k (whose spectrum is the conjugate):
+        // Re xi transforms (g(k) + conj g(-k)) / 2, Im xi (g(k) - conj g(-k)) / 2i
         const double g1 = inv_v * ((p2 - sn * S1) * Q1);
         const double g2 = inv_v * ((p2 - sn * S2) * Q2);
-        if (odd) return mk<T>((T)0, (T)(-(0.5 * (g1 - g2))));
-        return mk<T>((T)(0.5 * (g1 + g2)), (T)0);
+        const double b1 = inv_v * (pim * Q1), b2 = -(inv_v * (pim * Q2));
+        if (odd) return mk<T>((T)(0.5 * (b1 + b2)), (T)(-(0.5 * (g1 - g2))));
+        return mk<T>((T)(0.5 * (g1 + g2)), (T)(0.5 * (b1 - b2)));
     }
 };
 
@@ -931,6 +954,9 @@ struct BoxPlanImpl {
     AngPoly ang;
     bool sn_odd = false;              // odd l1 + l2: the odd part of xi enters the Bessel sweep too
     DevBuf jtab2, hist2, sn_pairs2, snp_part;
+    // reconstructed box, bispectrum / three-point function: the spectrum of D + alpha^2 R and the binned
+    // cross spectrum with D - alpha R (re, im)
+    DevBuf dk2, pkx_part, pkx_part_im, pkx_red;
     // per shell field: its own supports on the triple mesh (x, y, z; -1 full)
     // and the z one alone
     std::vector<int4> fsup;
@@ -973,8 +999,9 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
         if (p->bk_gen && p->l2 != p->l1 &&
             alloc(ctx, p, p->jtab2, sizeof(double) * (size_t)p->nk * (size_t)p->nkey))
             return -1;
-        if (p->sn_odd && (alloc(ctx, p, p->hist2, sizeof(double) * (size_t)p->nkey) ||
-                          alloc(ctx, p, p->sn_pairs2, sizeof(double) * p->pairs.size())))
+        // (the second, odd-part sweep also serves the cross spectrum of a reconstructed box)
+        if ((alloc(ctx, p, p->hist2, sizeof(double) * (size_t)p->nkey) ||
+             alloc(ctx, p, p->sn_pairs2, sizeof(double) * p->pairs.size())))
             return -1;
         if (alloc(ctx, p, p->pairs_d, sizeof(short2) * p->pairs.size())) return -1;
         if (alloc(ctx, p, p->sn_pairs, sizeof(double) * p->pairs.size())) return -1;
@@ -1438,7 +1465,8 @@ void plan_destroy(BoxPlanImpl* p) {
                       &p->pk_cnt_red, &p->xi_half, &p->xi_real, &p->hist, &p->jtab, &p->pairs_d, &p->a_start, &p->a_rows, &p->a_idx, &p->xi_R,
                       &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid,
                       &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red, &p->rpos, &p->rw,
-                      &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->snp_part, &p->fsup_d, &p->fsz_d, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
+                      &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->snp_part, &p->fsup_d, &p->fsz_d, &p->dk2, &p->pkx_part,
+                      &p->pkx_part_im, &p->pkx_red, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
                       &p->zsy[1], &p->zsc[0], &p->zsc[1], &p->zfields, &p->zr, &p->zpart, &p->zcnt, &p->zred,
                       &p->zcnt_red, &p->zpairs_d};
     for (DevBuf* b : bufs) dev_free(*b);
@@ -1552,10 +1580,6 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     ag.ng = m.ng();
     ag.assign = m.assign;
     const bool recon = nr > 0;
-    if (recon && (p->cfg.bk || p->zeta)) {
-        ctx->set_error("box execute: the bispectrum and three-point function after reconstruction are not available");
-        return -1;
-    }
     const double scale = fixed_scale_for(recon ? (double)np + std::fabs(alpha) * (double)nr : (double)np);
     {
         const size_t before = p->sort.bytes() + p->rpos.bytes + p->rw.bytes;
@@ -1574,8 +1598,10 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     } else if (assign_sorted(ctx, ag, d_pos, nullptr, np, scale, (long long*)p->fixed.p, nullptr, p->sort)) {
         return -1;
     }
-    // two-point shot noise: N (field.hpp:523) or N_D + alpha^2 N_R (field.hpp:1190-1197)
+    // two-point shot noise: N (field.hpp:523) or N_D + alpha^2 N_R (field.hpp:1190-1197); that of the
+    // bispectrum and three-point function: N or N_D - alpha^3 N_R (field.hpp:1242-1250)
     const double sn2 = recon ? (double)np + pow(alpha, 2) * (double)nr : (double)np;
+    const double sn3 = recon ? (double)np - pow(alpha, 3) * (double)nr : (double)np;
     // 2. forward r2c with the fluctuation (rho - N/V) dV formed on load
     const double dV = m.volume() / (double)m.ng();
     FixedRealSrc<T> fsrc{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale, 1.0 / dV,
@@ -1589,6 +1615,29 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (fft_cols<T>(ctx, "fft_r2c_x", m.n[0], -1, dk, dk,
                     geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
         return -1;
+    // 2'. reconstructed box with B or zeta: the shot-noise density D + alpha^2 R
+    // (calcNormalDensityForBispectrumShotnoiseForBOXForReconstruction, field.hpp:574-605), transformed
+    C<T>* dk2 = nullptr;
+    if (recon && (p->cfg.bk || p->zeta)) {
+        if (!p->dk2.p && alloc(ctx, p, p->dk2, sizeof(C<T>) * (size_t)m.nh())) return -1;
+        dk2 = (C<T>*)p->dk2.p;
+        if (pack_two(ctx, d_pos, nullptr, np, d_pos_r, nullptr, nr, -(alpha * alpha), (double*)p->rpos.p,
+                     (double*)p->rw.p))
+            return -1;
+        const double scale2 = fixed_scale_for((double)np + alpha * alpha * (double)nr);
+        if (assign_sorted(ctx, ag, (const double*)p->rpos.p, (const double*)p->rw.p, np + nr, scale2,
+                          (long long*)p->fixed.p, nullptr, p->sort))
+            return -1;
+        FixedRealSrc<T> fsrc2{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale2, 1.0 / dV, 0.0, dV};
+        if (launch_r2c<T>(ctx, "fft_r2c_z", m.n[2] / 2, fsrc2, dk2, (long long)m.n[0] * m.n[1])) return -1;
+        if (fft_cols<T>(ctx, "fft_r2c_y", m.n[1], -1, dk2, dk2,
+                        geom((long long)m.n[1] * nzh, nzh, m.n[0], m.n[0], -1, (int)nzh, (int)nzh, -1, -1)))
+            return -1;
+        if (fft_cols<T>(ctx, "fft_r2c_x", m.n[0], -1, dk2, dk2,
+                        geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1,
+                             -1)))
+            return -1;
+    }
     // 3. P_l binning (field.hpp:765-852)
     BinArgs ba;
     for (int a = 0; a < 3; a++) ba.n[a] = m.n[a];
@@ -1619,6 +1668,31 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->pk_cnt.p, p->nrows, p->nk,
                                (long long*)p->pk_cnt_red.p))
         return -1;
+    }
+    // 3'. reconstructed box with B: the two-point shot-noise terms bin the cross spectrum of D - alpha R
+    // and D + alpha^2 R with SN N_D - alpha^3 N_R (bispec.hpp:1455-1508), complex
+    const bool cross_pk = recon && p->cfg.bk && do_pk;
+    if (cross_pk) {
+        const size_t nb = sizeof(double) * (size_t)ncols * p->nrows;
+        if (!p->pkx_part.p && (alloc(ctx, p, p->pkx_part, nb) || alloc(ctx, p, p->pkx_part_im, nb) ||
+                               alloc(ctx, p, p->pkx_red, sizeof(double) * 2 * (size_t)ncols)))
+            return -1;
+        BinArgs bx = ba;
+        bx.sn = sn3;
+        if (fill_zero(ctx, p->pkx_part.p, nb) || fill_zero(ctx, p->pkx_part_im.p, nb) ||
+            fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows))
+            return -1;
+        if (ctx->launch("bin_pk_cross", [&] {
+                hipLaunchKernelGGL(k_bin_box<T>, dim3((unsigned)((p->nrows + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   dk, bx, (double*)p->pkx_part.p, (long long*)p->pk_cnt.p, dk2,
+                                   (double*)p->pkx_part_im.p);
+            }))
+            return -1;
+        if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->pkx_part.p, p->nrows, ncols,
+                                (double*)p->pkx_red.p) ||
+            reduce_cols<double>(ctx, "bin_reduce", (const double*)p->pkx_part_im.p, p->nrows, ncols,
+                                (double*)p->pkx_red.p + ncols))
+            return -1;
     }
     // 3b. xi_l (calcCorrelationFunction, field.hpp:854-977): the even part of
     // (|d|^2 - N S)/W^2/V inverse-transformed (its real part, which is all the
@@ -1681,12 +1755,14 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.nzh = (int)nzh;
         xs.t = tabs;
         xs.wp = wp;
-        xs.sn = (double)np;
+        xs.sn = sn3;
         xs.inv_v = 1.0 / m.volume();
-        xs.dc = (double)np;
-        // even part of g; for an odd l1 + l2 also -i times its odd part (field.hpp:1116-1160 keeps both:
-        // an odd angular weight sees the odd part of xi, and the even part on the z Nyquist plane)
-        for (int par = 0; par < (p->sn_odd ? 2 : 1); par++) {
+        xs.dc = recon ? 0.0 : (double)np;
+        xs.dk2 = dk2;
+        // Re xi; for an odd l1 + l2 also Im xi (field.hpp:1116-1160 keeps both: an odd angular weight sees
+        // the odd part of xi, and the even part on the z Nyquist plane), and always for the complex cross
+        // spectrum of a reconstructed box
+        for (int par = 0; par < ((p->sn_odd || recon) ? 2 : 1); par++) {
         xs.odd = par;
         void* hbuf = par ? p->hist2.p : p->hist.p;
         void* sbuf = par ? p->sn_pairs2.p : p->sn_pairs.p;
@@ -1943,9 +2019,10 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.nzh = (int)nzh;
         xs.t = tabs;
         xs.wp = wp;
-        xs.sn = (double)np;
+        xs.sn = sn3;
         xs.inv_v = 1.0 / m.volume();
-        xs.dc = (double)np;
+        xs.dc = recon ? 0.0 : (double)np;
+        xs.dk2 = dk2;
         C<T>* xh = (C<T>*)p->xi_half.p;
         const int cx = support_count(p->zrs[0], m.n[0]), cy = support_count(p->zrs[1], m.n[1]);
         ZetaSnArgs za;
@@ -1961,7 +2038,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         za.A = p->zang;
         if (fill_zero(ctx, p->zpart.p, sizeof(double) * 2 * (size_t)nr * p->z_rows)) return -1;
         if (fill_zero(ctx, p->zcnt.p, sizeof(long long) * (size_t)nr * p->z_rows)) return -1;
-        for (int par = 0; par < (p->z_odd ? 2 : 1); par++) {
+        for (int par = 0; par < ((p->z_odd || recon) ? 2 : 1); par++) {
             xs.odd = par;
             if (launch_cols<T, +1>(ctx, "zsn_pass_x", m.n[0], xs, xh,
                                    geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh),
@@ -2019,10 +2096,14 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                              hipMemcpyDeviceToHost, ctx->stream));
         HM_HIP_CHECK(ctx, hipMemcpyAsync(snp.data(), p->sn_pairs.p, sizeof(double) * npairs, hipMemcpyDeviceToHost,
                                          ctx->stream));
-        if (p->sn_odd)
+        if (p->sn_odd || recon)
             HM_HIP_CHECK(ctx, hipMemcpyAsync(snp2.data(), p->sn_pairs2.p, sizeof(double) * npairs,
                                              hipMemcpyDeviceToHost, ctx->stream));
     }
+    std::vector<double> pkx(cross_pk ? 2 * (size_t)ncols : 0);
+    if (cross_pk)
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(pkx.data(), p->pkx_red.p, sizeof(double) * pkx.size(), hipMemcpyDeviceToHost,
+                                         ctx->stream));
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     const int nk = p->nk;
     const double V = m.volume(), N = (double)np;
@@ -2063,7 +2144,17 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (p->cfg.bk && res) {
         // shot noise (bispec.hpp:926-1073): w N for l1 = l2 = 0, w P_l1(k1) for l2 = 0 (the row bin),
         // w P_l2(k2) for l1 = 0 (the column bin), i^(l1+l2) times the Bessel sweep
-        auto praw_of = [&](int e, int b) { return e >= 0 ? praw[(size_t)e * nk + b] : 0.0; };
+        // bin averages of the shot-noise spectrum: |d|^2 - N S, or the cross spectrum (re, im) after
+        // reconstruction
+        auto praw_of = [&](int e, int b) {
+            if (e < 0) return 0.0;
+            if (!cross_pk) return praw[(size_t)e * nk + b];
+            return cnt[b] ? pkx[(size_t)b * ba.ne + e] / (double)cnt[b] : 0.0;
+        };
+        auto praw_im = [&](int e, int b) {
+            if (e < 0 || !cross_pk) return 0.0;
+            return cnt[b] ? pkx[(size_t)ncols + (size_t)b * ba.ne + e] / (double)cnt[b] : 0.0;
+        };
         const int ip = ((p->l1 + p->l2) % 4 + 4) % 4;
         const double ir = ip == 0 ? 1.0 : (ip == 2 ? -1.0 : 0.0), ii = ip == 1 ? 1.0 : (ip == 3 ? -1.0 : 0.0);
         const long long M3 = (long long)p->M[0] * p->M[1] * p->M[2];
@@ -2072,9 +2163,10 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             const int a = p->pairs[q].x, b = p->pairs[q].y;
             const double den = (double)M3 * (double)p->shell_nmode[a] * (double)p->shell_nmode[b];
             const double br = gram[2 * q] / den, bi = gram[2 * q + 1] / den;
-            const double base = p->w_const * N + p->w_row * praw_of(p->e_row, a) + p->w_col * praw_of(p->e_col, b);
-            // i^(l1+l2) (S_even + i S_odd)
-            const double sr = base + (ir * snp[q] - ii * snp2[q]), si = ir * snp2[q] + ii * snp[q];
+            const double base = p->w_const * sn3 + p->w_row * praw_of(p->e_row, a) + p->w_col * praw_of(p->e_col, b);
+            const double base_i = p->w_row * praw_im(p->e_row, a) + p->w_col * praw_im(p->e_col, b);
+            // i^(l1+l2) (S_re + i S_im)
+            const double sr = base + (ir * snp[q] - ii * snp2[q]), si = base_i + (ir * snp2[q] + ii * snp[q]);
             for (int sym = 0; sym < (p->bk_gen ? 1 : 2); sym++) {
                 const int r = sym ? b : a, c = sym ? a : b;
                 const size_t o = 2 * ((size_t)r * nk + c);

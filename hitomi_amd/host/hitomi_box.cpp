@@ -6,9 +6,10 @@
 // bk%d%d%d_%02d.dat files, and exits with status 1 where the reference calls
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
-// Supported now: measure = pk and 2PCF (any ELL, ell1 = ELL, ell2 = 0), with
-// or without reconstruction (flag_recon), and measure = bk and 3PCF for any
-// (ell1, ell2, ELL) with wigner_3j(ell1, ell2, ELL, 0, 0, 0) != 0 without it.
+// Supported: measure = pk and 2PCF (any ELL, ell1 = ELL, ell2 = 0) and
+// measure = bk and 3PCF for any (ell1, ell2, ELL) with
+// wigner_3j(ell1, ell2, ELL, 0, 0, 0) != 0, each with or without
+// reconstruction (flag_recon).
 // Other measurements are rejected with an error instead of being computed
 // differently.
 #include "../../include/hitomi_amd.h"
@@ -267,12 +268,9 @@ int main(int argc, char* argv[]) {
         for (int i = 0; i < p.n_rbin; i++) fprintf(fp, "%.5f \t %.7e\n", rbin[i], xi[i]);
         fclose(fp);
     } else if (p.measure == "bk") {
-        if (recon) {
-            printf("measure = bk with flag_recon = True is not available in this build.\n");
-            EXIT();
-        }
         if (run_box(p, P_D, g, M_BK, 0, pk, bk, xi)) {
-            printf("'calcBiSpectrumForBOX(P_D, param, kbin)' fails.\n");
+            printf(recon ? "'calcBiSpectrumForBOXForReconstruction(P_D, P_R, param, alpha, kbin)' fails.\n"
+                         : "'calcBiSpectrumForBOX(P_D, param, kbin)' fails.\n");
             EXIT();
         }
         // bispec.hpp:1187-1196, one file per ith_kbin (all rows with bk_all_rows = True)
@@ -280,7 +278,7 @@ int main(int argc, char* argv[]) {
         const int r1 = p.bk_all_rows == "True" ? nk : p.ith_kbin + 1;
         for (int ith = r0; ith < r1; ith++) {
             char buf[2048];
-            snprintf(buf, sizeof(buf), "%s/bk%d%d%d_%02d.dat", p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith);
+            snprintf(buf, sizeof(buf), "%s/bk%d%d%d_%02d%s.dat", p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith, sfx);
             FILE* fp = fopen(buf, "w");
             if (!fp) EXIT();
             fprintf(fp, "## k1 [h/Mpc] \t k2 [h/Mpc] \t bk_real \t bk_imag\n");
@@ -291,13 +289,10 @@ int main(int argc, char* argv[]) {
             fclose(fp);
         }
     } else if (p.measure == "3PCF") {
-        if (recon) {
-            printf("measure = 3PCF with flag_recon = True is not available in this build.\n");
-            EXIT();
-        }
         std::vector<double> zeta;
         if (run_box(p, P_D, g, M_ZETA, 0, pk, bk, xi, &zeta)) {
-            printf("'calcThreePointCorrelationFunctionForBOX(P_D, param, rbin)' fails.\n");
+            printf(recon ? "'calcThreePointCorrelationFunctionForBOXForReconstruction(P_D, P_R, param, alpha, rbin)' fails.\n"
+                         : "'calcThreePointCorrelationFunctionForBOX(P_D, param, rbin)' fails.\n");
             EXIT();
         }
         // bispec.hpp:1409-1417, one file per ith_rbin (all rows with bk_all_rows = True)
@@ -306,7 +301,8 @@ int main(int argc, char* argv[]) {
         const int r1 = p.bk_all_rows == "True" ? nr : p.ith_rbin + 1;
         for (int ith = r0; ith < r1; ith++) {
             char buf[2048];
-            snprintf(buf, sizeof(buf), "%s/zeta%d%d%d_%02d.dat", p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith);
+            snprintf(buf, sizeof(buf), "%s/zeta%d%d%d_%02d%s.dat", p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith,
+                     sfx);
             FILE* fp = fopen(buf, "w");
             if (!fp) EXIT();
             fprintf(fp, "## r1 [Mpc/h] \t r2 [Mpc/h] \t zeta \n");

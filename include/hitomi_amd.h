@@ -152,9 +152,11 @@ void hm_box_plan_destroy(hm_ctx *ctx, hm_box_plan *plan);
 int hm_box_plan_execute(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos, long long np, hm_box_result *res);
 /* The reconstructed box (main_box.cpp:199-228, flag_recon = True): P_l and
  * xi_l of D - alpha R with unit weights (field.hpp:528-558), shot noise
- * N_D + alpha^2 N_R (field.hpp:1190-1197), norm V / N_D^2 (powerspec.hpp:396-519).
- * d_pos_d / d_pos_r: displaced and wrapped positions (n x 3 FP64).  A plan
- * with bk = 1 is rejected (the reconstructed bispectrum is not built). */
+ * N_D + alpha^2 N_R (field.hpp:1190-1197), norm V / N_D^2 (powerspec.hpp:396-519);
+ * B and zeta of the same field with the shot-noise density D + alpha^2 R
+ * (field.hpp:574-605) and shot noise N_D - alpha^3 N_R (field.hpp:1242-1250),
+ * norm V^2 / N_D^3 (bispec.hpp:1426-1975).
+ * d_pos_d / d_pos_r: displaced and wrapped positions (n x 3 FP64). */
 int hm_box_plan_execute_recon(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos_d, long long n_d,
                               const double *d_pos_r, long long n_r, double alpha, hm_box_result *res);
 /* Device bytes the plan holds. */

@@ -949,9 +949,37 @@ static void ipow(int n, double *r, double *i) {
     }
 }
 
-/* bispec.hpp:877-1203 */
-int or_box_bk(const or_param *p, const double *pos, long long np, const double *kbin, double *bk_save,
-              double *sn_save, double *out) {
+/* The transformed fluctuation dn_k and density N_k the box estimators start
+ * from: rho - nbar and rho (field.hpp:505-523, 560-572), or, for the
+ * reconstructed box, D - alpha R and D + alpha^2 R with unit weights
+ * (field.hpp:528-558, 574-605). */
+static void box_fields(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
+                       double alpha, int recon, double *dn_k, double *N_k) {
+    if (!recon) {
+        or_box_density_fluctuation(p, posD, nD, dn_k);
+        or_box_density(p, posD, nD, N_k);
+    } else {
+        double *fR = grid_alloc(p);
+        or_box_density(p, posD, nD, dn_k);
+        or_box_density(p, posD, nD, N_k);
+        or_box_density(p, posR, nR, fR);
+        for (long long i = 0; i < p->n_mesh_tot; i++) {
+            dn_k[2 * i] -= alpha * fR[2 * i];
+            dn_k[2 * i + 1] -= alpha * fR[2 * i + 1];
+            N_k[2 * i] += alpha * alpha * fR[2 * i];
+            N_k[2 * i + 1] += alpha * alpha * fR[2 * i + 1];
+        }
+        free(fR);
+    }
+    or_fourier_transform(p, dn_k);
+    or_fourier_transform(p, N_k);
+}
+
+/* bispec.hpp:877-1203 (box) and 1426-1753 (reconstructed box) from the
+ * transformed fluctuation dn_k and density N_k: shotnoise is N or
+ * N_D - alpha^3 N_R (field.hpp:1242-1250), n_norm N or N_D. */
+static int box_bk_core(const or_param *p, const double *dn_k, const double *N_k, double shotnoise, double n_norm,
+                       const double *kbin, double *bk_save, double *sn_save, double *out) {
     const int l1 = p->ell1, l2 = p->ell2, L = p->ELL, nk = p->n_kbin, ith = p->ith_kbin;
     if (fabs(or_wigner3j(l1, l2, L, 0, 0, 0)) < 1.0e-10) return -1;
     for (int i = 0; i < nk; i++) {
@@ -966,18 +994,13 @@ int or_box_bk(const or_param *p, const double *pos, long long np, const double *
 
     /* shot noise: two-point terms (bispec.hpp:912-963) */
     {
-        double *dn_00_sn = grid_alloc(p), *N_LM_sn = grid_alloc(p);
-        or_box_density_fluctuation(p, pos, np, dn_00_sn);
-        or_fourier_transform(p, dn_00_sn);
-        or_box_density(p, pos, np, N_LM_sn);
-        or_fourier_transform(p, N_LM_sn);
+        const double *dn_00_sn = dn_k, *N_LM_sn = N_k;
         for (int m1 = -l1; m1 <= l1; m1++)
             for (int m2 = -l2; m2 <= l2; m2++) {
                 int M = 0;
                 double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, M);
                 w *= (double)(2 * L + 1) * (double)(2 * l1 + 1) * (double)(2 * l2 + 1);
                 if (fabs(w) < 1.0e-10) continue;
-                double shotnoise = (double)np;
                 if ((l1 == 0) && (l2 == 0))
                     for (int i = 0; i < nk; i++) sn_save[2 * i] += w * shotnoise;
                 if (l2 == 0) {
@@ -995,17 +1018,11 @@ int or_box_bk(const or_param *p, const double *pos, long long np, const double *
                     }
                 }
             }
-        free(dn_00_sn);
-        free(N_LM_sn);
     }
 
     /* shot noise: the ijk term (bispec.hpp:972-1073) */
     {
-        double *N_00_sn = grid_alloc(p), *dn_LM_sn = grid_alloc(p);
-        or_box_density(p, pos, np, N_00_sn);
-        or_fourier_transform(p, N_00_sn);
-        or_box_density_fluctuation(p, pos, np, dn_LM_sn);
-        or_fourier_transform(p, dn_LM_sn);
+        const double *N_00_sn = N_k, *dn_LM_sn = dn_k;
         or_bessel *sj1 = or_bessel_new(l1);
         or_bessel *sj2 = or_bessel_new(l2);
         double *Ylm1 = grid_alloc(p), *Ylm2 = grid_alloc(p), *xi = grid_alloc(p);
@@ -1018,7 +1035,7 @@ int or_box_bk(const or_param *p, const double *pos, long long np, const double *
                 store_ylm_config(p, l1, m1, Ylm1);
                 store_ylm_config(p, l2, m2, Ylm2);
                 memset(xi, 0, sizeof(double) * 2 * (size_t)ng);
-                or_shotnoise_bispectrum_ijk(p, dn_LM_sn, N_00_sn, (double)np, 0.0, xi);
+                or_shotnoise_bispectrum_ijk(p, dn_LM_sn, N_00_sn, shotnoise, 0.0, xi);
                 double dr[3];
                 dr[0] = p->boxsize[0] / (double)(p->n_mesh[0]);
                 dr[1] = p->boxsize[1] / (double)(p->n_mesh[1]);
@@ -1057,17 +1074,13 @@ int or_box_bk(const or_param *p, const double *pos, long long np, const double *
         free(Ylm1);
         free(Ylm2);
         free(xi);
-        free(N_00_sn);
-        free(dn_LM_sn);
     }
 
     /* bispectrum (bispec.hpp:1100-1182) */
     {
-        double *dn_00 = grid_alloc(p), *dn_LM = grid_alloc(p);
-        or_box_density_fluctuation(p, pos, np, dn_00);
-        or_fourier_transform(p, dn_00);
-        or_box_density_fluctuation(p, pos, np, dn_LM);
-        or_fourier_transform(p, dn_LM);
+        const double *dn_00 = dn_k;
+        double *dn_LM = grid_alloc(p);
+        memcpy(dn_LM, dn_k, sizeof(double) * 2 * (size_t)ng);
         or_assignment_correction(p, dn_LM);
         or_inverse_fourier_transform(p, dn_LM);
         double *Ylm1 = grid_alloc(p), *Ylm2 = grid_alloc(p);
@@ -1103,12 +1116,11 @@ int or_box_bk(const or_param *p, const double *pos, long long np, const double *
         free(Ylm2);
         free(t1);
         free(t2);
-        free(dn_00);
         free(dn_LM);
     }
 
-    double norm = p->volume / (double)np / (double)np;
-    norm *= (p->volume / (double)np);
+    double norm = p->volume / n_norm / n_norm;
+    norm *= (p->volume / n_norm);
     for (int i = 0; i < nk; i++) {
         out[2 * i] = norm * (bk_save[2 * i] - sn_save[2 * i]);
         out[2 * i + 1] = norm * (bk_save[2 * i + 1] - sn_save[2 * i + 1]);
@@ -1116,6 +1128,32 @@ int or_box_bk(const or_param *p, const double *pos, long long np, const double *
     free(pk);
     free(nm);
     return 0;
+}
+
+/* bispec.hpp:877-1203 */
+int or_box_bk(const or_param *p, const double *pos, long long np, const double *kbin, double *bk_save,
+              double *sn_save, double *out) {
+    if (fabs(or_wigner3j(p->ell1, p->ell2, p->ELL, 0, 0, 0)) < 1.0e-10) return -1;
+    double *dn_k = grid_alloc(p), *N_k = grid_alloc(p);
+    box_fields(p, pos, np, NULL, 0, 0.0, 0, dn_k, N_k);
+    int rc = box_bk_core(p, dn_k, N_k, (double)np, (double)np, kbin, bk_save, sn_save, out);
+    free(dn_k);
+    free(N_k);
+    return rc;
+}
+
+/* bispec.hpp:1426-1753: posD, posR displaced and wrapped; alpha = N_D / N_R
+ * as main_box.cpp:153 computes it. */
+int or_box_recon_bk(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
+                    double alpha, const double *kbin, double *bk_save, double *sn_save, double *out) {
+    if (fabs(or_wigner3j(p->ell1, p->ell2, p->ELL, 0, 0, 0)) < 1.0e-10) return -1;
+    double *dn_k = grid_alloc(p), *N_k = grid_alloc(p);
+    box_fields(p, posD, nD, posR, nR, alpha, 1, dn_k, N_k);
+    int rc = box_bk_core(p, dn_k, N_k, (double)nD - pow(alpha, 3) * (double)nR, (double)nD, kbin, bk_save,
+                         sn_save, out);
+    free(dn_k);
+    free(N_k);
+    return rc;
 }
 
 
@@ -1583,10 +1621,12 @@ static void corr_for_3pcf(const or_param *p, const double *f1, const double *f2,
     free(x3);
 }
 
-/* bispec.hpp:1205-1424 for p->ith_rbin: zeta_save, sn_save (2*n_rbin) and
- * out = norm * (zeta - sn) (real part written to zeta%d%d%d_%02d.dat). */
-int or_box_zeta(const or_param *p, const double *pos, long long np, const double *rbin, double *zeta_save,
-                double *sn_save, double *out) {
+/* bispec.hpp:1205-1424 (box) and 1755-1975 (reconstructed box) for
+ * p->ith_rbin from the transformed fluctuation dn_k and density N_k:
+ * zeta_save, sn_save (2*n_rbin) and out = norm * (zeta - sn) (real part
+ * written to zeta%d%d%d_%02d[_recon].dat). */
+static int box_zeta_core(const or_param *p, const double *dn_k, const double *N_k, double shotnoise, double n_norm,
+                         const double *rbin, double *zeta_save, double *sn_save, double *out) {
     const int l1 = p->ell1, l2 = p->ell2, L = p->ELL, nr = p->n_rbin, ith = p->ith_rbin;
     if (fabs(or_wigner3j(l1, l2, L, 0, 0, 0)) < 1.0e-10) return -1;
     const long long ng = p->n_mesh_tot;
@@ -1599,11 +1639,7 @@ int or_box_zeta(const or_param *p, const double *pos, long long np, const double
     double *tmp = (double *)malloc(sizeof(double) * 2 * nr);
     /* shot noise (bispec.hpp:1229-1290) */
     {
-        double *N00 = grid_alloc(p), *dnLM = grid_alloc(p);
-        or_box_density(p, pos, np, N00);
-        or_fourier_transform(p, N00);
-        or_box_density_fluctuation(p, pos, np, dnLM);
-        or_fourier_transform(p, dnLM);
+        const double *N00 = N_k, *dnLM = dn_k;
         for (int m1 = -l1; m1 <= l1; m1++)
             for (int m2 = -l2; m2 <= l2; m2++) {
                 double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, 0);
@@ -1611,20 +1647,16 @@ int or_box_zeta(const or_param *p, const double *pos, long long np, const double
                 if (fabs(w) < 1.0e-10) continue;
                 store_ylm_config(p, l1, m1, Y1);
                 store_ylm_config(p, l2, m2, Y2);
-                corr_for_3pcf(p, dnLM, N00, rbin, (double)np, Y1, Y2, tmp, nm);
+                corr_for_3pcf(p, dnLM, N00, rbin, shotnoise, Y1, Y2, tmp, nm);
                 sn_save[2 * ith] += w * tmp[2 * ith];
                 sn_save[2 * ith + 1] += w * tmp[2 * ith + 1];
             }
-        free(N00);
-        free(dnLM);
     }
     /* three-point function (bispec.hpp:1300-1400) */
     {
-        double *dn00 = grid_alloc(p), *dnLM = grid_alloc(p), *t1 = grid_alloc(p), *t2 = grid_alloc(p);
-        or_box_density_fluctuation(p, pos, np, dn00);
-        or_fourier_transform(p, dn00);
-        or_box_density_fluctuation(p, pos, np, dnLM);
-        or_fourier_transform(p, dnLM);
+        const double *dn00 = dn_k;
+        double *dnLM = grid_alloc(p), *t1 = grid_alloc(p), *t2 = grid_alloc(p);
+        memcpy(dnLM, dn_k, sizeof(double) * 2 * (size_t)ng);
         or_assignment_correction(p, dnLM);
         or_inverse_fourier_transform(p, dnLM);
         or_bessel *sj1 = or_bessel_new(l1), *sj2 = or_bessel_new(l2);
@@ -1657,13 +1689,12 @@ int or_box_zeta(const or_param *p, const double *pos, long long np, const double
             }
         or_bessel_free(sj1);
         or_bessel_free(sj2);
-        free(dn00);
         free(dnLM);
         free(t1);
         free(t2);
     }
-    double norm = p->volume / (double)np / (double)np;
-    norm *= (p->volume / (double)np);
+    double norm = p->volume / n_norm / n_norm;
+    norm *= (p->volume / n_norm);
     for (int i = 0; i < nr; i++) {
         out[2 * i] = norm * (zeta_save[2 * i] - sn_save[2 * i]);
         out[2 * i + 1] = norm * (zeta_save[2 * i + 1] - sn_save[2 * i + 1]);
@@ -1673,4 +1704,29 @@ int or_box_zeta(const or_param *p, const double *pos, long long np, const double
     free(nm);
     free(tmp);
     return 0;
+}
+
+/* bispec.hpp:1205-1424 */
+int or_box_zeta(const or_param *p, const double *pos, long long np, const double *rbin, double *zeta_save,
+                double *sn_save, double *out) {
+    if (fabs(or_wigner3j(p->ell1, p->ell2, p->ELL, 0, 0, 0)) < 1.0e-10) return -1;
+    double *dn_k = grid_alloc(p), *N_k = grid_alloc(p);
+    box_fields(p, pos, np, NULL, 0, 0.0, 0, dn_k, N_k);
+    int rc = box_zeta_core(p, dn_k, N_k, (double)np, (double)np, rbin, zeta_save, sn_save, out);
+    free(dn_k);
+    free(N_k);
+    return rc;
+}
+
+/* bispec.hpp:1755-1975: posD, posR displaced and wrapped; alpha = N_D / N_R. */
+int or_box_recon_zeta(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
+                      double alpha, const double *rbin, double *zeta_save, double *sn_save, double *out) {
+    if (fabs(or_wigner3j(p->ell1, p->ell2, p->ELL, 0, 0, 0)) < 1.0e-10) return -1;
+    double *dn_k = grid_alloc(p), *N_k = grid_alloc(p);
+    box_fields(p, posD, nD, posR, nR, alpha, 1, dn_k, N_k);
+    int rc = box_zeta_core(p, dn_k, N_k, (double)nD - pow(alpha, 3) * (double)nR, (double)nD, rbin, zeta_save,
+                           sn_save, out);
+    free(dn_k);
+    free(N_k);
+    return rc;
 }

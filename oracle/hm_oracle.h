@@ -134,6 +134,12 @@ int or_box_recon_2pt(const or_param *p, const double *posD, long long nD, const 
 /* bispec.hpp:1205-1424 for p->ith_rbin: zeta_save, sn_save, out = norm * (zeta - sn) (2*n_rbin each). */
 int or_box_zeta(const or_param *p, const double *pos, long long np, const double *rbin, double *zeta_save,
                 double *sn_save, double *out);
+/* bispec.hpp:1426-1753 and 1755-1975: the reconstructed box (D - alpha R,
+ * shot-noise density D + alpha^2 R, N_D - alpha^3 N_R, norm from N_D). */
+int or_box_recon_bk(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
+                    double alpha, const double *kbin, double *bk_save, double *sn_save, double *out);
+int or_box_recon_zeta(const or_param *p, const double *posD, long long nD, const double *posR, long long nR,
+                      double alpha, const double *rbin, double *zeta_save, double *sn_save, double *out);
 
 /* Row-major index helper used by the tests. */
 long long or_coord(const or_param *p, int i, int j, int k);

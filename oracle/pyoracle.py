@@ -83,6 +83,8 @@ def lib():
             "or_box_recon_2pt": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, ct.c_int, _D,
                                             ct.POINTER(ct.c_int)]),
             "or_box_zeta": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, _D, _D]),
+            "or_box_recon_bk": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, _D, _D, _D]),
+            "or_box_recon_zeta": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, _D, _D, _D]),
             "or_survey_xi": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, ct.c_int, _D, ct.POINTER(ct.c_int)]),
         }
@@ -297,6 +299,34 @@ def box_bk(p: Param, pos):
         raise ValueError("or_box_bk rejected the multipole combination")
     c = lambda a: a[0::2] + 1j * a[1::2]
     return c(out), c(bk), c(sn)
+
+
+def box_recon_bk(p: Param, posD, posR, alpha):
+    """bispec.hpp:1426-1753 for p.ith_kbin on displaced, wrapped catalogues: (out, bk_save, sn_save)."""
+    posD = np.ascontiguousarray(posD, dtype=np.float64).reshape(-1, 3)
+    posR = np.ascontiguousarray(posR, dtype=np.float64).reshape(-1, 3)
+    kb = kbin(p)
+    bk, sn, out = np.zeros(2 * p.n_kbin), np.zeros(2 * p.n_kbin), np.zeros(2 * p.n_kbin)
+    rc = lib().or_box_recon_bk(ct.byref(p), dp(posD), posD.shape[0], dp(posR), posR.shape[0], float(alpha), dp(kb),
+                               dp(bk), dp(sn), dp(out))
+    if rc:
+        raise ValueError("or_box_recon_bk rejected the multipole combination")
+    c = lambda a: a[0::2] + 1j * a[1::2]
+    return c(out), c(bk), c(sn)
+
+
+def box_recon_zeta(p: Param, posD, posR, alpha):
+    """bispec.hpp:1755-1975 for p.ith_rbin on displaced, wrapped catalogues: (out, zeta_save, sn_save)."""
+    posD = np.ascontiguousarray(posD, dtype=np.float64).reshape(-1, 3)
+    posR = np.ascontiguousarray(posR, dtype=np.float64).reshape(-1, 3)
+    rb = rbin(p)
+    z, sn, out = np.zeros(2 * p.n_rbin), np.zeros(2 * p.n_rbin), np.zeros(2 * p.n_rbin)
+    rc = lib().or_box_recon_zeta(ct.byref(p), dp(posD), posD.shape[0], dp(posR), posR.shape[0], float(alpha), dp(rb),
+                                 dp(z), dp(sn), dp(out))
+    if rc:
+        raise ValueError("or_box_recon_zeta rejected the multipole combination")
+    c = lambda a: a[0::2] + 1j * a[1::2]
+    return c(out), c(z), c(sn)
 
 
 class Survey:

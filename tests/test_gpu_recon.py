@@ -101,3 +101,62 @@ def test_box_recon_pk_xi(ctx, prec):
         assert np.array_equal(res["xi_nmode"], nm.astype(np.int64))
         tol = 1e-10 if prec == hm.FP64 else 2e-5
         assert np.abs(res["xi"][e] - ref.real).max() <= tol * np.abs(xref0.real).max()
+
+
+def _recon_catalogues(n, L, nD, nR, seed, b1=2.0, RG=20.0):
+    D = clustered_catalogue(nD, L, seed=seed)
+    R = uniform_catalogue(nR, L, seed=seed + 1)
+    p = po.param(n, L, po.TSC)
+    dD, dR = po.recon_displace(p, D, None, R, None, float(len(D) // len(R)), L ** 3 / len(D), b1, RG)
+    for a in (dD, dR):
+        for ax in range(3):
+            a[a[:, ax] >= L, ax] -= L
+            a[a[:, ax] < 0.0, ax] += L
+    return dD, dR, len(D) / len(R)
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+@pytest.mark.parametrize("ells", [(0, 0, 0), (1, 1, 0), (2, 0, 2)])
+def test_box_recon_bispectrum(ctx, prec, ells):
+    """bispec.hpp:1426-1753 for every ith_kbin: |dB| <= tol * max|B| (1e-9 FP64, 2e-4 FP32), the bk and sn
+    sums separately with the same bound; shell mode counts bit-exact."""
+    n, L = 32, 1000.0
+    dD, dR, alpha = _recon_catalogues(n, L, 12000, 30000, seed=61)
+    kw = dict(kmin=0.02, kmax=0.1, n_kbin=5)
+    plan = hm.BoxPlan(ctx, n, L, "TSC", pk_ells=(0,), bk=True, bk_ells=ells, precision=prec, **kw)
+    res = plan.execute_recon(dD, dR, alpha)
+    plan.close()
+    nk = kw["n_kbin"]
+    ref = np.zeros((nk, nk), complex)
+    ref_bk = np.zeros((nk, nk), complex)
+    ref_sn = np.zeros((nk, nk), complex)
+    for ith in range(nk):
+        p = po.param(n, L, po.TSC, ell1=ells[0], ell2=ells[1], ELL=ells[2], ith_kbin=ith, **kw)
+        ref[ith], ref_bk[ith], ref_sn[ith] = po.box_recon_bk(p, dD, dR, alpha)
+    tol = 1e-9 if prec == hm.FP64 else 2e-4
+    c = lambda a: a[..., 0] + 1j * a[..., 1]
+    for got, want in ((c(res["bk"]), ref), (c(res["bk_sum"]), ref_bk), (c(res["sn_sum"]), ref_sn)):
+        assert np.abs(got - want).max() <= tol * np.abs(want).max(), np.abs(got - want).max() / np.abs(want).max()
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+@pytest.mark.parametrize("ells", [(0, 0, 0), (1, 1, 0)])
+def test_box_recon_zeta(ctx, prec, ells):
+    """bispec.hpp:1755-1975 for every ith_rbin: zeta_save within 1e-9 (FP64) / 1e-4 (FP32) of its maximum,
+    the shot noise within 1e-10 / 2e-5 of its maximum."""
+    n, L = 16, 500.0
+    dD, dR, alpha = _recon_catalogues(n, L, 4000, 10000, seed=67, RG=25.0)
+    rw = dict(rmin=40.0, rmax=150.0, n_rbin=4)
+    plan = hm.BoxPlan(ctx, n, L, "TSC", pk_ells=(), bk=False, zeta=True, zeta_ells=ells, precision=prec, **rw)
+    res = plan.execute_recon(dD, dR, alpha)
+    plan.close()
+    nr = rw["n_rbin"]
+    ref_z = np.zeros((nr, nr), complex)
+    ref_sn = np.zeros((nr, nr), complex)
+    for ith in range(nr):
+        p = po.param(n, L, po.TSC, ell1=ells[0], ell2=ells[1], ELL=ells[2], ith_rbin=ith, **rw)
+        _, ref_z[ith], ref_sn[ith] = po.box_recon_zeta(p, dD, dR, alpha)
+    c = lambda a: a[..., 0] + 1j * a[..., 1]
+    zt, st = (1e-9, 1e-10) if prec == hm.FP64 else (1e-4, 2e-5)
+    assert np.abs(c(res["zeta_sum"]) - ref_z).max() <= zt * np.abs(ref_z).max()
+    assert np.abs(c(res["zeta_sn"]) - ref_sn).max() <= st * np.abs(ref_sn).max()
