@@ -1539,6 +1539,7 @@ static void ifft_for_3pcf(const or_param *p, const double *density, double rmag,
 /* field.hpp:979-1113 with Y1 Y2 tables: returns xi[i] (complex) for the
  * 3PCF bin rule and the (-1)^(l1+l2) / n_mode^2 / dV scaling (Q14) */
 static void corr_for_3pcf(const or_param *p, const double *f1, const double *f2, const double *rbin, double sn,
+                          double sn_im,
                           const double *ylm1, const double *ylm2, double *xi, int *n_mode) {
     const long long ng = p->n_mesh_tot;
     double *x3 = (double *)calloc(2 * (size_t)ng, sizeof(double));
@@ -1555,6 +1556,7 @@ static void corr_for_3pcf(const or_param *p, const double *f1, const double *f2,
                 CMUL(f1[2 * coord], f1[2 * coord + 1], f2[2 * coord], -f2[2 * coord + 1], pr, pi);
                 double S = or_shotnoise_function(p, kvec);
                 pr -= sn * S;
+                pi -= sn_im * S;
                 double w2 = pow(or_assignment_function(p, kvec), 2);
                 pr /= w2;
                 pi /= w2;
@@ -1647,7 +1649,7 @@ static int box_zeta_core(const or_param *p, const double *dn_k, const double *N_
                 if (fabs(w) < 1.0e-10) continue;
                 store_ylm_config(p, l1, m1, Y1);
                 store_ylm_config(p, l2, m2, Y2);
-                corr_for_3pcf(p, dnLM, N00, rbin, shotnoise, Y1, Y2, tmp, nm);
+                corr_for_3pcf(p, dnLM, N00, rbin, shotnoise, 0.0, Y1, Y2, tmp, nm);
                 sn_save[2 * ith] += w * tmp[2 * ith];
                 sn_save[2 * ith + 1] += w * tmp[2 * ith + 1];
             }
@@ -1729,4 +1731,337 @@ int or_box_recon_zeta(const or_param *p, const double *posD, long long nD, const
     free(dn_k);
     free(N_k);
     return rc;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* survey bispectrum and three-point functions (bispec.hpp:4-874)       */
+/* ------------------------------------------------------------------ */
+
+/* The catalogues of a survey estimator: D and R (shifted positions, lines
+ * of sight, weights) with alpha, or R alone for the window functions. */
+typedef struct {
+    const double *posD, *losD, *wD, *posR, *losR, *wR;
+    long long nD, nR;
+    double alpha;
+    int window;
+} svy_src;
+
+/* Y_LM(los) w^k-weighted field of one catalogue, Y conjugated when conj. */
+static void svy_weighted(const or_param *p, const double *pos, const double *los, const double *w, long long n,
+                         int L, int M, int k, int conj, double *field) {
+    double *wt = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+    double y[2];
+    for (long long q = 0; q < n; q++) {
+        or_ylm(L, M, los + 3 * q, y);
+        if (conj) y[1] = -y[1];
+        wt[2 * q] = y[0] * pow(w[q], k);
+        wt[2 * q + 1] = y[1] * pow(w[q], k);
+    }
+    or_calc_field(p, pos, wt, n, field);
+    free(wt);
+}
+
+/* field.hpp:186-247: the Y_LM-weighted fluctuation D - alpha R, or R alone */
+static void svy_fluct(const or_param *p, const svy_src *c, int L, int M, double *field) {
+    if (c->window) svy_weighted(p, c->posR, c->losR, c->wR, c->nR, L, M, 1, 0, field);
+    else or_ylm_weighted_fluctuation(p, c->posD, c->losD, c->wD, c->nD, c->posR, c->losR, c->wR, c->nR, c->alpha, L, M,
+                                     field);
+}
+
+/* field.hpp:249-306: conj(Y_LM) w^2 of D plus alpha^2 that of R, or of R alone */
+static void svy_sn_density(const or_param *p, const svy_src *c, int L, int M, double *field) {
+    if (c->window) {
+        svy_weighted(p, c->posR, c->losR, c->wR, c->nR, L, M, 2, 1, field);
+        return;
+    }
+    double *fR = grid_alloc(p);
+    svy_weighted(p, c->posD, c->losD, c->wD, c->nD, L, M, 2, 1, field);
+    svy_weighted(p, c->posR, c->losR, c->wR, c->nR, L, M, 2, 1, fR);
+    for (long long i = 0; i < p->n_mesh_tot; i++) {
+        field[2 * i] += c->alpha * c->alpha * fR[2 * i];
+        field[2 * i + 1] += c->alpha * c->alpha * fR[2 * i + 1];
+    }
+    free(fR);
+}
+
+/* field.hpp:1202-1240: sum_D w^3 Y - alpha^3 sum_R w^3 Y, or sum_R w^2 Y for the window */
+static void svy_shotnoise(const svy_src *c, int L, int M, double *out) {
+    double y[2];
+    out[0] = out[1] = 0.0;
+    if (c->window) {
+        for (long long q = 0; q < c->nR; q++) {
+            or_ylm(L, M, c->losR + 3 * q, y);
+            out[0] += pow(c->wR[q], 2) * y[0];
+            out[1] += pow(c->wR[q], 2) * y[1];
+        }
+        return;
+    }
+    double dr = 0.0, di = 0.0, rr = 0.0, ri = 0.0;
+    for (long long q = 0; q < c->nD; q++) {
+        or_ylm(L, M, c->losD + 3 * q, y);
+        dr += pow(c->wD[q], 3) * y[0];
+        di += pow(c->wD[q], 3) * y[1];
+    }
+    for (long long q = 0; q < c->nR; q++) {
+        or_ylm(L, M, c->losR + 3 * q, y);
+        rr += pow(c->wR[q], 3) * y[0];
+        ri += pow(c->wR[q], 3) * y[1];
+    }
+    out[0] = dr - pow(c->alpha, 3) * rr;
+    out[1] = di - pow(c->alpha, 3) * ri;
+}
+
+static double w3j_weight(int l1, int l2, int L, int m1, int m2, int M) {
+    double w = or_wigner3j(l1, l2, L, 0, 0, 0) * or_wigner3j(l1, l2, L, m1, m2, M);
+    return w * (double)(2 * L + 1) * (double)(2 * l1 + 1) * (double)(2 * l2 + 1);
+}
+
+/* bispec.hpp:4-353 for p->ith_kbin: bk_save, sn_save (2*n_kbin complex) and
+ * out = norm * (bk - sn), norm = Vsurvey^2 / (sum w_D)^3 (particle.hpp:319-332). */
+int or_survey_bk(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                 const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                 const double *kbin, double *bk_save, double *sn_save, double *out) {
+    const int l1 = p->ell1, l2 = p->ell2, L = p->ELL, nk = p->n_kbin, ith = p->ith_kbin;
+    if (fabs(or_wigner3j(l1, l2, L, 0, 0, 0)) < 1.0e-10) return -1;
+    svy_src c = {posD, losD, wD, posR, losR, wR, nD, nR, alpha, 0};
+    const long long ng = p->n_mesh_tot;
+    for (int i = 0; i < 2 * nk; i++) bk_save[i] = sn_save[i] = 0.0;
+    double *pk = (double *)malloc(sizeof(double) * 2 * nk);
+    int *nm = (int *)malloc(sizeof(int) * nk);
+    double sn[2];
+    /* two-point shot-noise terms (bispec.hpp:33-93) */
+    {
+        double *dn_00_sn = grid_alloc(p), *N_LM_sn = grid_alloc(p);
+        svy_fluct(p, &c, 0, 0, dn_00_sn);
+        or_fourier_transform(p, dn_00_sn);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++)
+                for (int M = -L; M <= L; M++) {
+                    double w = w3j_weight(l1, l2, L, m1, m2, M);
+                    if (fabs(w) < 1.0e-10) continue;
+                    svy_sn_density(p, &c, L, M, N_LM_sn);
+                    or_fourier_transform(p, N_LM_sn);
+                    svy_shotnoise(&c, L, M, sn);
+                    if ((l1 == 0) && (l2 == 0))
+                        for (int i = 0; i < nk; i++) {
+                            sn_save[2 * i] += w * sn[0];
+                            sn_save[2 * i + 1] += w * sn[1];
+                        }
+                    if (l2 == 0) {
+                        or_calc_power_spectrum(p, dn_00_sn, N_LM_sn, kbin, sn[0], sn[1], l1, m1, pk, nm);
+                        for (int i = 0; i < nk; i++) {
+                            sn_save[2 * i] += w * pk[2 * ith];
+                            sn_save[2 * i + 1] += w * pk[2 * ith + 1];
+                        }
+                    }
+                    if (l1 == 0) {
+                        or_calc_power_spectrum(p, dn_00_sn, N_LM_sn, kbin, sn[0], sn[1], l2, m2, pk, nm);
+                        for (int i = 0; i < nk; i++) {
+                            sn_save[2 * i] += w * pk[2 * i];
+                            sn_save[2 * i + 1] += w * pk[2 * i + 1];
+                        }
+                    }
+                }
+        free(dn_00_sn);
+        free(N_LM_sn);
+    }
+    /* the ijk term (bispec.hpp:95-211) */
+    {
+        double *N_00_sn = grid_alloc(p), *dn_LM_sn = grid_alloc(p), *xi = grid_alloc(p);
+        double *Y1 = grid_alloc(p), *Y2 = grid_alloc(p);
+        svy_sn_density(p, &c, 0, 0, N_00_sn);
+        or_fourier_transform(p, N_00_sn);
+        or_bessel *sj1 = or_bessel_new(l1), *sj2 = or_bessel_new(l2);
+        double dr[3];
+        for (int a = 0; a < 3; a++) dr[a] = p->boxsize[a] / (double)(p->n_mesh[a]);
+        double ir, ii;
+        ipow(l1 + l2, &ir, &ii);
+        const double fac = p->volume / (double)ng;
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                store_ylm_config(p, l1, m1, Y1);
+                store_ylm_config(p, l2, m2, Y2);
+                for (int M = -L; M <= L; M++) {
+                    double w = w3j_weight(l1, l2, L, m1, m2, M);
+                    if (fabs(w) < 1.0e-10) continue;
+                    svy_fluct(p, &c, L, M, dn_LM_sn);
+                    or_fourier_transform(p, dn_LM_sn);
+                    svy_shotnoise(&c, L, M, sn);
+                    or_shotnoise_bispectrum_ijk(p, dn_LM_sn, N_00_sn, sn[0], sn[1], xi);
+                    for (int ik = 0; ik < nk; ik++) {
+                        double sr = 0.0, si = 0.0, rvec[3];
+                        for (int i = 0; i < p->n_mesh[0]; i++)
+                            for (int j = 0; j < p->n_mesh[1]; j++)
+                                for (int k = 0; k < p->n_mesh[2]; k++) {
+                                    long long coord = or_coord(p, i, j, k);
+                                    kvec_of(p, i, j, k, dr, rvec);
+                                    double rmag = sqrt(rvec[0] * rvec[0] + rvec[1] * rvec[1] + rvec[2] * rvec[2]);
+                                    double jj = or_bessel_eval(sj1, kbin[ith] * rmag) * or_bessel_eval(sj2, kbin[ik] * rmag);
+                                    double tr = jj * xi[2 * coord], ti = jj * xi[2 * coord + 1];
+                                    CMUL(tr, ti, Y1[2 * coord], Y1[2 * coord + 1], tr, ti);
+                                    CMUL(tr, ti, Y2[2 * coord], Y2[2 * coord + 1], tr, ti);
+                                    sr += tr;
+                                    si += ti;
+                                }
+                        double cr = ir * fac, ci = ii * fac;
+                        CMUL(sr, si, cr, ci, sr, si);
+                        sn_save[2 * ik] += w * sr;
+                        sn_save[2 * ik + 1] += w * si;
+                    }
+                }
+            }
+        or_bessel_free(sj1);
+        or_bessel_free(sj2);
+        free(Y1);
+        free(Y2);
+        free(xi);
+        free(dn_LM_sn);
+        free(N_00_sn);
+    }
+    /* the bispectrum (bispec.hpp:216-323) */
+    {
+        double *dn_00 = grid_alloc(p), *dn_LM = grid_alloc(p), *Y1 = grid_alloc(p), *Y2 = grid_alloc(p);
+        double *t1 = grid_alloc(p), *t2 = grid_alloc(p);
+        svy_fluct(p, &c, 0, 0, dn_00);
+        or_fourier_transform(p, dn_00);
+        const double dk = kbin[1] - kbin[0], fac = p->volume / (double)ng;
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                store_ylm_fourier(p, l1, m1, Y1);
+                store_ylm_fourier(p, l2, m2, Y2);
+                for (int M = -L; M <= L; M++) {
+                    double w = w3j_weight(l1, l2, L, m1, m2, M);
+                    if (fabs(w) < 1.0e-10) continue;
+                    svy_fluct(p, &c, L, M, dn_LM);
+                    or_fourier_transform(p, dn_LM);
+                    or_assignment_correction(p, dn_LM);
+                    or_inverse_fourier_transform(p, dn_LM);
+                    or_ifft_for_bispectrum(p, dn_00, kbin[ith], dk, Y1, t1);
+                    for (int ik = 0; ik < nk; ik++) {
+                        or_ifft_for_bispectrum(p, dn_00, kbin[ik], dk, Y2, t2);
+                        double sr = 0.0, si = 0.0;
+                        for (long long q = 0; q < ng; q++) {
+                            double ar = fac * t1[2 * q], ai = fac * t1[2 * q + 1];
+                            CMUL(ar, ai, t2[2 * q], t2[2 * q + 1], ar, ai);
+                            CMUL(ar, ai, dn_LM[2 * q], dn_LM[2 * q + 1], ar, ai);
+                            sr += ar;
+                            si += ai;
+                        }
+                        bk_save[2 * ik] += w * sr;
+                        bk_save[2 * ik + 1] += w * si;
+                    }
+                }
+            }
+        free(dn_00);
+        free(dn_LM);
+        free(Y1);
+        free(Y2);
+        free(t1);
+        free(t2);
+    }
+    const double sw = weight_sum(wD, nD);
+    const double norm = vs / sw / sw * (vs / sw);
+    for (int i = 0; i < nk; i++) {
+        out[2 * i] = norm * (bk_save[2 * i] - sn_save[2 * i]);
+        out[2 * i + 1] = norm * (bk_save[2 * i + 1] - sn_save[2 * i + 1]);
+    }
+    free(pk);
+    free(nm);
+    return 0;
+}
+
+/* bispec.hpp:355-614 (window = 0) and 617-874 (window = 1) for p->ith_rbin:
+ * zeta_save, sn_save and out = norm * (zeta - sn), norm = Vsurvey^2 / (sum w)^3
+ * of D, or of R for the window. */
+int or_survey_zeta(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                   const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                   const double *rbin, int window, double *zeta_save, double *sn_save, double *out) {
+    const int l1 = p->ell1, l2 = p->ell2, L = p->ELL, nr = p->n_rbin, ith = p->ith_rbin;
+    if (fabs(or_wigner3j(l1, l2, L, 0, 0, 0)) < 1.0e-10) return -1;
+    svy_src c = {posD, losD, wD, posR, losR, wR, nD, nR, alpha, window};
+    const long long ng = p->n_mesh_tot;
+    for (int i = 0; i < 2 * nr; i++) zeta_save[i] = sn_save[i] = 0.0;
+    double *Y1 = grid_alloc(p), *Y2 = grid_alloc(p);
+    int *nm = (int *)malloc(sizeof(int) * nr);
+    double *tmp = (double *)malloc(sizeof(double) * 2 * nr);
+    double sn[2];
+    /* shot noise (bispec.hpp:388-460, 644-720) */
+    {
+        double *N00 = grid_alloc(p), *dnLM = grid_alloc(p);
+        svy_sn_density(p, &c, 0, 0, N00);
+        or_fourier_transform(p, N00);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                store_ylm_config(p, l1, m1, Y1);
+                store_ylm_config(p, l2, m2, Y2);
+                for (int M = -L; M <= L; M++) {
+                    double w = w3j_weight(l1, l2, L, m1, m2, M);
+                    if (fabs(w) < 1.0e-10) continue;
+                    svy_fluct(p, &c, L, M, dnLM);
+                    or_fourier_transform(p, dnLM);
+                    svy_shotnoise(&c, L, M, sn);
+                    corr_for_3pcf(p, dnLM, N00, rbin, sn[0], sn[1], Y1, Y2, tmp, nm);
+                    sn_save[2 * ith] += w * tmp[2 * ith];
+                    sn_save[2 * ith + 1] += w * tmp[2 * ith + 1];
+                }
+            }
+        free(N00);
+        free(dnLM);
+    }
+    /* the three-point function (bispec.hpp:470-590, 726-850) */
+    {
+        double *dn00 = grid_alloc(p), *dnLM = grid_alloc(p), *t1 = grid_alloc(p), *t2 = grid_alloc(p);
+        svy_fluct(p, &c, 0, 0, dn00);
+        or_fourier_transform(p, dn00);
+        or_bessel *sj1 = or_bessel_new(l1), *sj2 = or_bessel_new(l2);
+        const double fac = p->volume / (double)ng;
+        double ir, ii;
+        ipow(l1 + l2, &ir, &ii);
+        for (int m1 = -l1; m1 <= l1; m1++)
+            for (int m2 = -l2; m2 <= l2; m2++) {
+                store_ylm_fourier(p, l1, m1, Y1);
+                store_ylm_fourier(p, l2, m2, Y2);
+                for (int M = -L; M <= L; M++) {
+                    double w = w3j_weight(l1, l2, L, m1, m2, M);
+                    if (fabs(w) < 1.0e-10) continue;
+                    svy_fluct(p, &c, L, M, dnLM);
+                    or_fourier_transform(p, dnLM);
+                    or_assignment_correction(p, dnLM);
+                    or_inverse_fourier_transform(p, dnLM);
+                    ifft_for_3pcf(p, dn00, rbin[ith], Y1, sj1, t1);
+                    for (int ir2 = 0; ir2 < nr; ir2++) {
+                        ifft_for_3pcf(p, dn00, rbin[ir2], Y2, sj2, t2);
+                        double sr = 0.0, si = 0.0;
+                        for (long long q = 0; q < ng; q++) {
+                            double ar = ir * fac, ai = ii * fac;
+                            CMUL(ar, ai, t1[2 * q], t1[2 * q + 1], ar, ai);
+                            CMUL(ar, ai, t2[2 * q], t2[2 * q + 1], ar, ai);
+                            CMUL(ar, ai, dnLM[2 * q], dnLM[2 * q + 1], ar, ai);
+                            sr += ar;
+                            si += ai;
+                        }
+                        zeta_save[2 * ir2] += w * sr;
+                        zeta_save[2 * ir2 + 1] += w * si;
+                    }
+                }
+            }
+        or_bessel_free(sj1);
+        or_bessel_free(sj2);
+        free(dn00);
+        free(dnLM);
+        free(t1);
+        free(t2);
+    }
+    const double sw = window ? weight_sum(wR, nR) : weight_sum(wD, nD);
+    const double norm = vs / sw / sw * (vs / sw);
+    for (int i = 0; i < nr; i++) {
+        out[2 * i] = norm * (zeta_save[2 * i] - sn_save[2 * i]);
+        out[2 * i + 1] = norm * (zeta_save[2 * i + 1] - sn_save[2 * i + 1]);
+    }
+    free(Y1);
+    free(Y2);
+    free(nm);
+    free(tmp);
+    return 0;
 }

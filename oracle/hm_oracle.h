@@ -147,4 +147,13 @@ long long or_coord(const or_param *p, int i, int j, int k);
 #ifdef __cplusplus
 }
 #endif
+/* bispec.hpp:4-353 (survey B) and 355-874 (survey and window zeta) on the
+ * prepared catalogues (shifted positions, lines of sight, weights). */
+int or_survey_bk(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                 const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                 const double *kbin, double *bk_save, double *sn_save, double *out);
+int or_survey_zeta(const or_param *p, const double *posD, const double *losD, const double *wD, long long nD,
+                   const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
+                   const double *rbin, int window, double *zeta_save, double *sn_save, double *out);
+
 #endif

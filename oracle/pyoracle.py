@@ -85,6 +85,10 @@ def lib():
             "or_box_zeta": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, _D, _D]),
             "or_box_recon_bk": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, _D, _D, _D]),
             "or_box_recon_zeta": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, _D, _D, _D]),
+            "or_survey_bk": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
+                                        ct.c_double, _D, _D, _D, _D]),
+            "or_survey_zeta": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
+                                          ct.c_double, _D, ct.c_int, _D, _D, _D]),
             "or_survey_xi": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, ct.c_int, _D, ct.POINTER(ct.c_int)]),
         }
@@ -382,6 +386,28 @@ class Survey:
         if rc:
             raise ValueError("or_survey_xi rejected the multipole combination")
         return out[0::2] + 1j * out[1::2], nm
+
+    def bk(self, p: Param):
+        """bispec.hpp:4-353 for p.ith_kbin: (out, bk_save, sn_save), complex."""
+        kb = kbin(p)
+        bk, sn, out = np.zeros(2 * p.n_kbin), np.zeros(2 * p.n_kbin), np.zeros(2 * p.n_kbin)
+        rc = lib().or_survey_bk(ct.byref(p), *self._args(), float(self.alpha), float(self.vsurvey), dp(kb), dp(bk),
+                                dp(sn), dp(out))
+        if rc:
+            raise ValueError("or_survey_bk rejected the multipole combination")
+        c = lambda a: a[0::2] + 1j * a[1::2]
+        return c(out), c(bk), c(sn)
+
+    def zeta(self, p: Param, window=False):
+        """bispec.hpp:355-614 (window: 617-874) for p.ith_rbin: (out, zeta_save, sn_save), complex."""
+        rb = rbin(p)
+        z, sn, out = np.zeros(2 * p.n_rbin), np.zeros(2 * p.n_rbin), np.zeros(2 * p.n_rbin)
+        rc = lib().or_survey_zeta(ct.byref(p), *self._args(), float(self.alpha), float(self.vsurvey), dp(rb),
+                                  1 if window else 0, dp(z), dp(sn), dp(out))
+        if rc:
+            raise ValueError("or_survey_zeta rejected the multipole combination")
+        c = lambda a: a[0::2] + 1j * a[1::2]
+        return c(out), c(z), c(sn)
 
 
 def recon_psi(p: Param, dk, b1, RG):
