@@ -95,10 +95,12 @@ class SurveyConfig(ct.Structure):
         ("n_mesh_recon", ct.c_int * 3),
         ("b1", ct.c_double),
         ("RG", ct.c_double),
+        ("ell1", ct.c_int),
+        ("ell2", ct.c_int),
     ]
 
 
-SURVEY_MEASURES = {"pk": 0, "2PCF": 1, "window2PCF": 2}
+SURVEY_MEASURES = {"pk": 0, "2PCF": 1, "window2PCF": 2, "bk": 3, "3PCF": 4, "window3PCF": 5}
 
 
 class SurveyInfo(ct.Structure):
@@ -491,11 +493,14 @@ class BoxPlan:
 class SurveyPlan:
     """Survey-geometry plan (hm_survey_plan): main.cpp's preparation
     (main.cpp:109-166) and calcPowerSpectrum (powerspec.hpp:4-91),
-    calcTwoPointCorrelationFunction (93-179) or its window version (182-270)."""
+    calcTwoPointCorrelationFunction (93-179) or its window version (182-270),
+    calcBiSpectrum (bispec.hpp:4-353) or the three-point functions (355-874)."""
 
     def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2, ell=0,
-                 precision=FP32, prune=True, measure="pk", n_rbin=25, rmin=30.0, rmax=150.0, recon=None):
-        """recon: None, or (n_mesh_recon, b1_fid, RG) to reconstruct before measuring."""
+                 precision=FP32, prune=True, measure="pk", n_rbin=25, rmin=30.0, rmax=150.0, recon=None,
+                 ell1=0, ell2=0):
+        """recon: None, or (n_mesh_recon, b1_fid, RG) to reconstruct before measuring.
+        bk / 3PCF / window3PCF: the multipole (ell1, ell2, ELL = ell)."""
         self.ctx = ctx
         cfg = SurveyConfig()
         if recon is not None:
@@ -516,9 +521,13 @@ class SurveyPlan:
         cfg.kmin = float(kmin)
         cfg.kmax = float(kmax)
         cfg.ell = int(ell)
+        cfg.ell1 = int(ell1)
+        cfg.ell2 = int(ell2)
         cfg.prune = 1 if prune else 0
         self.cfg = cfg
-        self.n_kbin = int(n_kbin) if measure == "pk" else int(n_rbin)
+        self.measure = measure
+        self.n_kbin = int(n_kbin) if measure in ("pk", "bk") else int(n_rbin)
+        self.square = measure in ("bk", "3PCF", "window3PCF")
         self.h = ctx.lib.hm_survey_plan_create(ctx.h, ct.byref(cfg))
         if not self.h:
             ctx.raise_error("hm_survey_plan_create")
@@ -552,7 +561,7 @@ class SurveyPlan:
         nd, nr = pd.shape[0], pr.shape[0]
         wd = dev(w_d, (nd,))
         wr = dev(w_r, (nr,))
-        pk = np.zeros((self.n_kbin, 2))
+        pk = np.zeros((self.n_kbin, self.n_kbin, 2) if self.square else (self.n_kbin, 2))
         nm = np.zeros(self.n_kbin, np.int64)
         info = SurveyInfo()
         self.ctx.check(self.ctx.lib.hm_survey_plan_execute(
@@ -561,4 +570,4 @@ class SurveyPlan:
             "hm_survey_plan_execute")
         d = {k: getattr(info, k) for k in ("wsum_d", "wsum_r", "alpha", "vsurvey", "nbar", "norm")}
         d["shift"] = np.array(info.shift[:])
-        return pk[:, 0] + 1j * pk[:, 1], nm, d
+        return pk[..., 0] + 1j * pk[..., 1], nm, d

@@ -13,12 +13,8 @@
 #include <new>
 
 namespace hm {
-struct BoxPlanImpl;
-BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg);
-void plan_destroy(BoxPlanImpl* p);
 int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res,
                      const double* d_pos_r, long long nr, double alpha);
-size_t plan_bytes(const BoxPlanImpl* p);
 void plan_triple_mesh(const BoxPlanImpl* p, int m[3]);
 int cell_coords(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, long long np, long long* d_out);
 struct SurveyPlanImpl;

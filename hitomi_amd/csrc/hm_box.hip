@@ -1558,12 +1558,48 @@ static int field_transform(hm_ctx* ctx, BoxPlanImpl* p, const ShellSrc<T>& src, 
     return fft_rows<T>(ctx, "tri_pass_z", M[2], +1, out, out, rg);
 }
 
+// D then R into one array with weights wD^pw (1 when wD is null) and
+// cR * wR^pw: the fields D - alpha R (pw = 1, cR = -alpha) and
+// D + alpha^2 R (pw = 2, cR = alpha^2) of the reconstructed box and the survey.
+__global__ __launch_bounds__(256) void k_pack_pow(const double* __restrict__ pD, const double* __restrict__ wD,
+                                                 long long nD, const double* __restrict__ pR,
+                                                 const double* __restrict__ wR, long long nR, double cR, int pw,
+                                                 double* __restrict__ pos, double* __restrict__ w) {
+#pragma clang fp contract(off)
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nD + nR) return;
+    const bool d = q < nD;
+    const double* src = d ? pD + 3 * q : pR + 3 * (q - nD);
+    pos[3 * q] = src[0];
+    pos[3 * q + 1] = src[1];
+    pos[3 * q + 2] = src[2];
+    const double* wt = d ? wD : wR;
+    double v = wt ? wt[d ? q : q - nD] : 1.0;
+    if (pw == 2) v = v * v;
+    if (!d) v = cR * v;
+    w[2 * q] = v;
+    w[2 * q + 1] = 0.0;
+}
+
+static int pack_pow(hm_ctx* ctx, const double* pD, const double* wD, long long nD, const double* pR, const double* wR,
+                    long long nR, double cR, int pw, double* pos, double* w) {
+    if (nD + nR <= 0) return 0;
+    return ctx->launch("pack_two", [&] {
+        hipLaunchKernelGGL(k_pack_pow, dim3((unsigned)((nD + nR + 255) / 256)), dim3(256), 0, ctx->stream, pD, wD, nD, pR,
+                           wR, nR, cR, pw, pos, w);
+    });
+}
+
 template <typename T>
 // d_pos_r / nr / alpha: the reconstructed box (main_box.cpp:199-228): the
 // density is D - alpha R with unit weights (field.hpp:528-558) instead of
 // rho - nbar, the two-point shot noise N_D + alpha^2 N_R (field.hpp:1190-1197).
+// wt: survey-geometry catalogues for B and zeta with ELL = 0 (bispec.hpp:4-874):
+// the fields w (D - alpha R) and w^2 (D + alpha^2 R), or R alone for the
+// window, with the shot noise and normalisation of the survey estimators.
 static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res,
-                        const double* d_pos_r = nullptr, long long nr = 0, double alpha = 0.0) {
+                        const double* d_pos_r = nullptr, long long nr = 0, double alpha = 0.0,
+                        const BoxWeighted* wt = nullptr) {
     const Mesh& m = p->mesh;
     const AxisTabs tabs = axis_tabs(p);
     const int wp = wpow_of(m.assign);
@@ -1579,8 +1615,9 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     }
     ag.ng = m.ng();
     ag.assign = m.assign;
-    const bool recon = nr > 0;
-    const double scale = fixed_scale_for(recon ? (double)np + std::fabs(alpha) * (double)nr : (double)np);
+    const bool weighted = wt != nullptr;
+    const bool recon = weighted || nr > 0;  // two catalogues, no mean subtraction, cross-spectrum shot noise
+    const double scale = fixed_scale_for(weighted ? wt->abs1 : (recon ? (double)np + std::fabs(alpha) * (double)nr : (double)np));
     {
         const size_t before = p->sort.bytes() + p->rpos.bytes + p->rw.bytes;
         if (sort_work_alloc(ctx, p->sort, ag, np + nr, recon)) return -1;
@@ -1590,7 +1627,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         p->bytes += p->sort.bytes() + p->rpos.bytes + p->rw.bytes - before;
     }
     if (recon) {
-        if (pack_two(ctx, d_pos, nullptr, np, d_pos_r, nullptr, nr, alpha, (double*)p->rpos.p, (double*)p->rw.p))
+        if (pack_pow(ctx, d_pos, weighted ? wt->wD : nullptr, np, d_pos_r, weighted ? wt->wR : nullptr, nr,
+                     weighted ? wt->c1 : -alpha, 1, (double*)p->rpos.p, (double*)p->rw.p))
             return -1;
         if (assign_sorted(ctx, ag, (const double*)p->rpos.p, (const double*)p->rw.p, np + nr, scale,
                           (long long*)p->fixed.p, nullptr, p->sort))
@@ -1601,7 +1639,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     // two-point shot noise: N (field.hpp:523) or N_D + alpha^2 N_R (field.hpp:1190-1197); that of the
     // bispectrum and three-point function: N or N_D - alpha^3 N_R (field.hpp:1242-1250)
     const double sn2 = recon ? (double)np + pow(alpha, 2) * (double)nr : (double)np;
-    const double sn3 = recon ? (double)np - pow(alpha, 3) * (double)nr : (double)np;
+    const double sn3 = weighted ? wt->sn3 : (recon ? (double)np - pow(alpha, 3) * (double)nr : (double)np);
     // 2. forward r2c with the fluctuation (rho - N/V) dV formed on load
     const double dV = m.volume() / (double)m.ng();
     FixedRealSrc<T> fsrc{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale, 1.0 / dV,
@@ -1621,10 +1659,10 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (recon && (p->cfg.bk || p->zeta)) {
         if (!p->dk2.p && alloc(ctx, p, p->dk2, sizeof(C<T>) * (size_t)m.nh())) return -1;
         dk2 = (C<T>*)p->dk2.p;
-        if (pack_two(ctx, d_pos, nullptr, np, d_pos_r, nullptr, nr, -(alpha * alpha), (double*)p->rpos.p,
-                     (double*)p->rw.p))
+        if (pack_pow(ctx, d_pos, weighted ? wt->wD : nullptr, np, d_pos_r, weighted ? wt->wR : nullptr, nr,
+                     weighted ? wt->c2 : alpha * alpha, 2, (double*)p->rpos.p, (double*)p->rw.p))
             return -1;
-        const double scale2 = fixed_scale_for((double)np + alpha * alpha * (double)nr);
+        const double scale2 = fixed_scale_for(weighted ? wt->abs2 : (double)np + alpha * alpha * (double)nr);
         if (assign_sorted(ctx, ag, (const double*)p->rpos.p, (const double*)p->rw.p, np + nr, scale2,
                           (long long*)p->fixed.p, nullptr, p->sort))
             return -1;
@@ -2158,7 +2196,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         const int ip = ((p->l1 + p->l2) % 4 + 4) % 4;
         const double ir = ip == 0 ? 1.0 : (ip == 2 ? -1.0 : 0.0), ii = ip == 1 ? 1.0 : (ip == 3 ? -1.0 : 0.0);
         const long long M3 = (long long)p->M[0] * p->M[1] * p->M[2];
-        const double norm = (V / N / N) * (V / N);
+        const double norm = weighted ? wt->norm : (V / N / N) * (V / N);
         for (int q = 0; q < npairs; q++) {
             const int a = p->pairs[q].x, b = p->pairs[q].y;
             const double den = (double)M3 * (double)p->shell_nmode[a] * (double)p->shell_nmode[b];
@@ -2192,7 +2230,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         const int ip = lz % 4;
         const double ir = ip == 0 ? 1.0 : (ip == 2 ? -1.0 : 0.0), ii = ip == 1 ? 1.0 : (ip == 3 ? -1.0 : 0.0);
         const double sg = (lz & 1) ? -1.0 : 1.0;  // (-1)^(l1+l2) (field.hpp:1104)
-        const double norm = (V / N / N) * (V / N);
+        const double norm = weighted ? wt->norm : (V / N / N) * (V / N);
         auto put = [&](double* dst, int r, int c, double re, double im) {
             if (!dst) return;
             dst[2 * ((size_t)r * nr + c)] = re;
@@ -2224,6 +2262,12 @@ int plan_execute_any(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long
                      const double* d_pos_r, long long nr, double alpha) {
     return p->cfg.precision == HM_FP64 ? plan_execute<double>(ctx, p, d_pos, np, res, d_pos_r, nr, alpha)
                                        : plan_execute<float>(ctx, p, d_pos, np, res, d_pos_r, nr, alpha);
+}
+
+int plan_execute_weighted(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos_d, long long n_d, const double* d_pos_r,
+                          long long n_r, const BoxWeighted* wt, hm_box_result* res) {
+    return p->cfg.precision == HM_FP64 ? plan_execute<double>(ctx, p, d_pos_d, n_d, res, d_pos_r, n_r, 0.0, wt)
+                                       : plan_execute<float>(ctx, p, d_pos_d, n_d, res, d_pos_r, n_r, 0.0, wt);
 }
 
 size_t plan_bytes(const BoxPlanImpl* p) { return p->bytes; }

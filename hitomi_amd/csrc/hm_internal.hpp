@@ -4,6 +4,7 @@
 
 #include "hm_common.hpp"
 #include "hm_fft.hpp"
+#include "../../include/hitomi_amd.h"
 
 #include <functional>
 #include <memory>
@@ -61,5 +62,22 @@ template <typename T> int fft_c2r_rows(hm_ctx* ctx, const char* name, int L2, co
 
 bool fft_length_supported(int L);
 int fft_length_at_least(int n);  // smallest supported length >= n, or -1
+
+// Survey-geometry catalogues handed to a box plan for B and zeta with ELL = 0:
+// fields with weights wD, c1 * wR and wD^2, c2 * wR^2 (null weights: 1); abs1
+// and abs2 bound sum |weight| of the two fields (fixed-point scale); sn3 the
+// shot noise (sum_D w^3 - alpha^3 sum_R w^3, or sum_R w^2 for the window);
+// norm the estimator normalisation Vsurvey^2 / (sum w)^3.
+struct BoxWeighted {
+    const double* wD;
+    const double* wR;
+    double c1, c2, abs1, abs2, sn3, norm;
+};
+struct BoxPlanImpl;
+BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg);
+void plan_destroy(BoxPlanImpl* p);
+size_t plan_bytes(const BoxPlanImpl* p);
+int plan_execute_weighted(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos_d, long long n_d, const double* d_pos_r,
+                          long long n_r, const BoxWeighted* wt, hm_box_result* res);
 
 }  // namespace hm

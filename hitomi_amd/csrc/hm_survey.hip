@@ -62,21 +62,24 @@ double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3) {
 // particle passes
 // ---------------------------------------------------------------------------
 // per-block partials: sum w_D, sum w_D^2, sum w_R, sum |w_D|, sum |w_R|; fixed order
-constexpr int SV_NSUM = 5;
+constexpr int SV_NSUM = 8;  // sum w_D, w_D^2, w_R, |w_D|, |w_R|, w_R^2, w_D^3, w_R^3
 __global__ __launch_bounds__(256) void k_survey_wsums(const double* __restrict__ wD, long long nD,
                                                      const double* __restrict__ wR, long long nR,
                                                      double* __restrict__ part) {
     __shared__ double sm[SV_NSUM][256];
-    double v[SV_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double v[SV_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const long long st = (long long)gridDim.x * 256;
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nD; q += st) {
         v[0] += wD[q];
         v[1] += pow(wD[q], 2);
         v[3] += fabs(wD[q]);
+        v[6] += pow(wD[q], 3);
     }
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nR; q += st) {
         v[2] += wR[q];
         v[4] += fabs(wR[q]);
+        v[5] += pow(wR[q], 2);
+        v[7] += pow(wR[q], 3);
     }
     for (int k = 0; k < SV_NSUM; k++) sm[k][threadIdx.x] = v[k];
     __syncthreads();
@@ -425,7 +428,12 @@ struct SurveyPlanImpl {
     ReconWork rw;
     DevBuf tab, f2b, f2b_r, pos, wts, fre, fim, fields, gwork, part, cnt, red, cnt_red, scratch;
     size_t bytes = 0;
+    BoxPlanImpl* box = nullptr;  // bk / 3PCF / window3PCF: the triple-product machinery on weighted catalogues
 };
+
+static bool delegated(int measure) {
+    return measure == HM_SURVEY_BK || measure == HM_SURVEY_ZETA || measure == HM_SURVEY_ZETA_WINDOW;
+}
 
 static int sv_alloc(hm_ctx* ctx, SurveyPlanImpl* p, DevBuf& b, size_t bytes) {
     const size_t before = b.bytes;
@@ -441,6 +449,7 @@ void survey_destroy(SurveyPlanImpl* p) {
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     recon_work_free(p->rw);
+    if (p->box) plan_destroy(p->box);
     delete p;
 }
 
@@ -470,7 +479,31 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
         if (!(cfg->b1 != 0.0)) return fail("survey plan: b1_fid must be non-zero");
     }
     p->measure = cfg->measure;
-    if (p->measure < HM_SURVEY_PK || p->measure > HM_SURVEY_XI_WINDOW) return fail("survey plan: unknown measure");
+    if (p->measure < HM_SURVEY_PK || p->measure > HM_SURVEY_ZETA_WINDOW) return fail("survey plan: unknown measure");
+    if (delegated(p->measure)) {
+        // calcBiSpectrum / calcThreePoint(Window)CorrelationFunction with ELL = 0: no Y_LM(los) weight, so
+        // the estimators are those of the box on the weighted catalogues (bispec.hpp:4-874)
+        if (cfg->ell != 0)
+            return fail("survey plan: the survey bispectrum and three-point functions need ELL = 0 in this build");
+        hm_box_config bc;
+        std::memset(&bc, 0, sizeof(bc));
+        bc.mesh = cfg->mesh;
+        bc.precision = cfg->precision;
+        bc.n_kbin = cfg->n_kbin;
+        bc.kmin = cfg->kmin;
+        bc.kmax = cfg->kmax;
+        bc.regrid = 1;
+        bc.prune = cfg->prune;
+        bc.n_rbin = cfg->n_rbin;
+        bc.rmin = cfg->rmin;
+        bc.rmax = cfg->rmax;
+        const int ells[3] = {cfg->ell1, cfg->ell2, 0};
+        for (int a = 0; a < 3; a++) bc.bk_ell[a] = bc.zeta_ell[a] = ells[a];
+        bc.bk = p->measure == HM_SURVEY_BK ? 1 : 0;
+        bc.zeta = p->measure == HM_SURVEY_BK ? 0 : 1;
+        p->box = plan_create(ctx, &bc);
+        if (!p->box) return fail(std::string(ctx->err));
+    }
     p->nk = cfg->n_kbin;
     p->l = cfg->ell;
     p->nf = p->l == 0 ? 1 : p->l + 2;
@@ -521,6 +554,12 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
     host_axis_tables(m, tabs, p->tab_off);
     const size_t elem = cfg->precision == HM_FP64 ? 16 : 8;
     const long long ng = m.ng();
+    if (delegated(p->measure)) {  // the survey volume grid only
+        if (sv_alloc(ctx, p, p->tab, sizeof(double) * tabs.size()) || sv_alloc(ctx, p, p->f2b, sizeof(int2) * FINE_N) ||
+            sv_alloc(ctx, p, p->fre, sizeof(long long) * (size_t)ng))
+            return fail(ctx->err);
+        return p;
+    }
     if (sv_alloc(ctx, p, p->tab, sizeof(double) * tabs.size()) || sv_alloc(ctx, p, p->f2b, sizeof(int2) * FINE_N) ||
         sv_alloc(ctx, p, p->fre, sizeof(long long) * (size_t)ng) ||
         sv_alloc(ctx, p, p->fim, sizeof(long long) * (size_t)ng) ||
@@ -542,7 +581,9 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
     return p;
 }
 
-size_t survey_bytes(const SurveyPlanImpl* p) { return p ? p->bytes + p->sort.bytes() + recon_work_bytes(p->rw) : 0; }
+size_t survey_bytes(const SurveyPlanImpl* p) {
+    return p ? p->bytes + p->sort.bytes() + recon_work_bytes(p->rw) + (p->box ? plan_bytes(p->box) : 0) : 0;
+}
 
 static unsigned red_blocks(long long n) { return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 1024)); }
 
@@ -586,7 +627,8 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
         mn[a] = hp[a];
         for (unsigned b = 1; b < rb; b++) mn[a] = std::min(mn[a], hp[3 * b + a]);
     }
-    const double swD = sums[0], sw2D = sums[1], swR = sums[2], awD = sums[3], awR = sums[4];
+    const double swD = sums[0], sw2D = sums[1], swR = sums[2], awD = sums[3], awR = sums[4], sw2R = sums[5],
+                 sw3D = sums[6], sw3R = sums[7];
     if (swD == 0.0 || swR == 0.0) {
         ctx->set_error("survey execute: zero total weight (particle.hpp:274-276)");
         return -1;
@@ -661,6 +703,52 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
                 hipLaunchKernelGGL(k_shift_inplace, dim3(pb), dim3(256), 0, ctx->stream, pos, N, d2[0], d2[1], d2[2]);
             }))
             return -1;
+    }
+    if (delegated(p->measure)) {
+        // 3'. bispectrum / three-point function with ELL = 0 (bispec.hpp:4-874): w (D - alpha R) and
+        // w^2 (D + alpha^2 R), shot noise sum_D w^3 - alpha^3 sum_R w^3, or R alone with sum_R w^2 for the
+        // window; norm Vsurvey^2 / (sum w)^3 (particle.hpp:319-332)
+        const bool win = p->measure == HM_SURVEY_ZETA_WINDOW;
+        BoxWeighted bw;
+        const double sw = win ? swR : swD;
+        bw.norm = (vs / sw / sw) * (vs / sw);
+        if (win) {
+            bw.wD = wR;
+            bw.wR = nullptr;
+            bw.c1 = bw.c2 = 0.0;
+            bw.abs1 = awR;
+            bw.abs2 = sw2R;
+            bw.sn3 = sw2R;
+        } else {
+            bw.wD = wD;
+            bw.wR = wR;
+            bw.c1 = -alpha;
+            bw.c2 = alpha * alpha;
+            bw.abs1 = awD + std::fabs(alpha) * awR;
+            bw.abs2 = sw2D + alpha * alpha * sw2R;
+            bw.sn3 = sw3D - pow(alpha, 3) * sw3R;
+        }
+        hm_box_result res;
+        std::memset(&res, 0, sizeof(res));
+        if (p->measure == HM_SURVEY_BK) {
+            res.bk = pk_out;
+            res.shell_nmode = nmode_out;
+        } else {
+            res.zeta = pk_out;
+        }
+        const int rc = win ? plan_execute_weighted(ctx, p->box, pos + 3 * nD, nR, nullptr, 0, &bw, &res)
+                           : plan_execute_weighted(ctx, p->box, pos, nD, pos + 3 * nD, nR, &bw, &res);
+        if (rc) return -1;
+        if (info) {
+            info->wsum_d = swD;
+            info->wsum_r = swR;
+            info->alpha = alpha;
+            info->vsurvey = vs;
+            info->nbar = nbar;
+            info->norm = bw.norm;
+            for (int ax = 0; ax < 3; ax++) info->shift[ax] = dP[ax];
+        }
+        return 0;
     }
     // 3. Y_lm-weighted fluctuations and their transforms.  |Y_red| <= 1, so every
     // particle's |re| + |im| is at most sqrt(2) |w|: the bound fixes the fixed-point scale.

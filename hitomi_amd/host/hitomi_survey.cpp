@@ -6,9 +6,11 @@
 // reference's formats, and exits with status 1 where the reference calls
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
-// Supported now: measure = pk, 2PCF and window2PCF (ell1 = ELL, ell2 = 0),
-// with or without reconstruction (flag_recon).  Other measurements are
-// rejected with an error instead of being computed differently.
+// Supported: measure = pk, 2PCF and window2PCF (ell1 = ELL, ell2 = 0), and
+// bk, 3PCF and window3PCF with ELL = 0 (any ell1, ell2 that wigner_3j
+// allows), each with or without reconstruction (flag_recon).  The
+// bispectrum and three-point functions with ELL > 0 are rejected with an
+// error instead of being computed differently.
 #include "../../include/hitomi_amd.h"
 #include "hm_param.hpp"
 #include "hm_survey_catalog.hpp"
@@ -134,12 +136,27 @@ int main(int argc, char* argv[]) {
     } else if (p.measure == "window2PCF") {
         measure = HM_SURVEY_XI_WINDOW;
         fails = "'calcTwoPointWindowCorrelationFunction(P_R, param, rbin, Vsurvey)' fails.";
+    } else if (p.measure == "bk") {
+        measure = HM_SURVEY_BK;
+        fails = "'calcBiSpectrum(P_D, P_R, param, alpha, kbin, Vsurvey)' fails.";
+    } else if (p.measure == "3PCF") {
+        measure = HM_SURVEY_ZETA;
+        fails = "'calcThreePointCorrelationFunction(P_D, P_R, param, alpha, kbin, Vsurvey)' fails.";
+    } else if (p.measure == "window3PCF") {
+        measure = HM_SURVEY_ZETA_WINDOW;
+        fails = "'calcThreePointWindowCorrelationFunction(P_R, param, rbin, Vsurvey)' fails.";
     } else {
         printf("measure = %s is not available in this build.\n", p.measure.c_str());
         EXIT();
     }
+    const bool three = measure >= HM_SURVEY_BK;
+    if (three && p.ELL != 0) {
+        printf("measure = %s with ELL = %d is not available in this build (ELL = 0 only).\n", p.measure.c_str(), p.ELL);
+        printf("%s\n", fails.c_str());
+        EXIT();
+    }
     // powerspec.hpp:11-15, 101-105, 188-192
-    if (!((p.ELL == p.ell1) && (p.ell2 == 0))) {
+    if (!three && !((p.ELL == p.ell1) && (p.ell2 == 0))) {
         printf("This multipole combination is not allowed.\n");
         printf("It should be ELL == ell1 and ell2 == 0\n");
         printf("%s\n", fails.c_str());
@@ -180,13 +197,15 @@ int main(int argc, char* argv[]) {
     for (int a = 0; a < 3; a++) cfg.n_mesh_recon[a] = p.n_mesh_recon[a];
     cfg.b1 = p.b1_fid;
     cfg.RG = p.RG;
+    cfg.ell1 = p.ell1;
+    cfg.ell2 = p.ell2;
     hm_survey_plan* plan = hm_survey_plan_create(g.ctx, &cfg);
     if (!plan) {
         printf("hm_survey_plan_create: %s\n", hm_last_error(g.ctx));
         EXIT();
     }
-    const int nbin = measure == HM_SURVEY_PK ? p.n_kbin : p.n_rbin;
-    std::vector<double> pk(2 * (size_t)nbin);
+    const int nbin = (measure == HM_SURVEY_PK || measure == HM_SURVEY_BK) ? p.n_kbin : p.n_rbin;
+    std::vector<double> pk(2 * (size_t)nbin * (three ? nbin : 1));
     std::vector<long long> nm(nbin);
     hm_survey_info info;
     printf("Calculating '%s': %.3f sec.\n", p.measure.c_str(), now() - g_t0);
@@ -211,6 +230,38 @@ int main(int argc, char* argv[]) {
         fprintf(fp, "## k [h/Mpc]\t pk_real [(Mpc/h)^3]\t pk_imag [(Mpc/h)^3]\n");
         for (int i = 0; i < p.n_kbin; i++) fprintf(fp, "%.5f \t %.7e \t %.7e\n", kbin[i], pk[2 * i], pk[2 * i + 1]);
         fclose(fp);
+    } else if (measure == HM_SURVEY_BK) {
+        // bispec.hpp:329-346, one file per ith_kbin (all rows with bk_all_rows = True)
+        const int r0 = p.bk_all_rows == "True" ? 0 : p.ith_kbin;
+        const int r1 = p.bk_all_rows == "True" ? nbin : p.ith_kbin + 1;
+        for (int ith = r0; ith < r1; ith++) {
+            snprintf(buf, sizeof(buf), recon ? "%s/bk%d%d%d_%02d_recon.dat" : "%s/bk%d%d%d_%02d.dat",
+                     p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith);
+            FILE* fp = fopen(buf, "w");
+            if (!fp) EXIT();
+            fprintf(fp, "## k1 [h/Mpc] \t k2 [h/Mpc] \t bk_real \t bk_imag\n");
+            for (int i = 0; i < nbin; i++) {
+                const size_t o = 2 * ((size_t)ith * nbin + i);
+                fprintf(fp, "%.5f \t %.5f \t %.7e \t %.7e\n", kbin[ith], kbin[i], pk[o], pk[o + 1]);
+            }
+            fclose(fp);
+        }
+    } else if (three) {
+        // bispec.hpp:591-606, 851-866: real part, one file per ith_rbin
+        const bool win = measure == HM_SURVEY_ZETA_WINDOW;
+        const int r0 = p.bk_all_rows == "True" ? 0 : p.ith_rbin;
+        const int r1 = p.bk_all_rows == "True" ? nbin : p.ith_rbin + 1;
+        for (int ith = r0; ith < r1; ith++) {
+            const char* fmt = win ? (recon ? "%s/zeta%d%d%d_%02d_recon_window.dat" : "%s/zeta%d%d%d_%02d_window.dat")
+                                  : (recon ? "%s/zeta%d%d%d_%02d_recon.dat" : "%s/zeta%d%d%d_%02d.dat");
+            snprintf(buf, sizeof(buf), fmt, p.output_dir.c_str(), p.ell1, p.ell2, p.ELL, ith);
+            FILE* fp = fopen(buf, "w");
+            if (!fp) EXIT();
+            fprintf(fp, "## r1 [Mpc/h] \t r2 [Mpc/h] \t zeta \n");
+            for (int i = 0; i < nbin; i++)
+                fprintf(fp, "%.5f \t %.5f \t %.7e\n", rbin[ith], rbin[i], pk[2 * ((size_t)ith * nbin + i)]);
+            fclose(fp);
+        }
     } else {
         // powerspec.hpp:161-175, 253-266
         const char* fmt = measure == HM_SURVEY_XI ? (recon ? "%s/xi%d_recon.dat" : "%s/xi%d.dat")

@@ -197,14 +197,18 @@ int hm_recon_displace(hm_ctx *ctx, const hm_recon_config *cfg, double *d_pos_d, 
 int hm_periodic_wrap(hm_ctx *ctx, double *d_pos, long long n, const double boxsize[3]);
 
 /* ---------------------------------------------------------------- */
-/* survey estimators (main.cpp:109-183: pk, 2PCF, window2PCF)        */
+/* survey estimators (main.cpp:109-274: pk, 2PCF, window2PCF, bk,   */
+/* 3PCF, window3PCF)                                                  */
 /* ---------------------------------------------------------------- */
 typedef struct hm_survey_plan hm_survey_plan;
 
 enum {
     HM_SURVEY_PK = 0,        /* measure = pk: calcPowerSpectrum (powerspec.hpp:4-91) */
     HM_SURVEY_XI = 1,        /* measure = 2PCF: calcTwoPointCorrelationFunction (powerspec.hpp:93-179) */
-    HM_SURVEY_XI_WINDOW = 2  /* measure = window2PCF: calcTwoPointWindowCorrelationFunction (powerspec.hpp:182-270) */
+    HM_SURVEY_XI_WINDOW = 2, /* measure = window2PCF: calcTwoPointWindowCorrelationFunction (powerspec.hpp:182-270) */
+    HM_SURVEY_BK = 3,        /* measure = bk: calcBiSpectrum (bispec.hpp:4-353), ELL = 0 */
+    HM_SURVEY_ZETA = 4,      /* measure = 3PCF: calcThreePointCorrelationFunction (bispec.hpp:355-614), ELL = 0 */
+    HM_SURVEY_ZETA_WINDOW = 5 /* measure = window3PCF: calcThreePointWindowCorrelationFunction (bispec.hpp:617-874), ELL = 0 */
 };
 
 typedef struct {
@@ -212,7 +216,7 @@ typedef struct {
     int precision;     /* HM_FP32 or HM_FP64 grids and FFTs; FP64 sums */
     int n_kbin;        /* tools.hpp:118-124 */
     double kmin, kmax;
-    int ell;           /* ELL = ell1, ell2 = 0 (calcPowerSpectrum, powerspec.hpp:11-15) */
+    int ell;           /* ELL; for pk and the 2PCFs ell1 = ELL, ell2 = 0 (calcPowerSpectrum, powerspec.hpp:11-15) */
     int prune;         /* 1: bin only the modes (separations) that can reach a bin */
     int measure;       /* HM_SURVEY_PK, HM_SURVEY_XI or HM_SURVEY_XI_WINDOW */
     int n_rbin;        /* tools.hpp:126-132, for the correlation functions */
@@ -220,6 +224,7 @@ typedef struct {
     int recon;         /* 1: reconstruct first (main.cpp:172-183, flag_recon = True) */
     int n_mesh_recon[3];
     double b1, RG;     /* b1_fid, RG */
+    int ell1, ell2;    /* the bispectrum / three-point multipole (ell1, ell2, ELL) */
 } hm_survey_config;
 
 /* The survey quantities main.cpp derives before measuring. */
@@ -243,7 +248,11 @@ void hm_survey_plan_destroy(hm_ctx *ctx, hm_survey_plan *plan);
  * sight is pos/|pos| of these), d_w_* the weights (FP64).
  * out: [n_bins][2] = norm * pk_save (pk%d.dat) or norm * xi_save (whose real
  * part xi%d.dat and xi%d_window.dat hold); nmode: [n_bins] modes or mesh
- * points per bin. */
+ * points per bin.  For bk: out [n_kbin][n_kbin][2] = norm * (bk - sn), row i
+ * the file bk%d%d%d_%02d.dat of ith_kbin = i, nmode the shell n_mode; for
+ * 3PCF / window3PCF: out [n_rbin][n_rbin][2] = norm * (zeta - sn), whose real
+ * part row i of zeta%d%d%d_%02d[_window].dat holds, nmode unused (may be NULL).
+ * norm (info) = Vsurvey^2 / (sum w)^3 for these (particle.hpp:319-332). */
 int hm_survey_plan_execute(hm_ctx *ctx, hm_survey_plan *plan, const double *d_pos_d, const double *d_w_d, long long n_d,
                            const double *d_pos_r, const double *d_w_r, long long n_r, hm_survey_info *info, double *out,
                            long long *nmode);

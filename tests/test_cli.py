@@ -144,7 +144,7 @@ def test_end_to_end_pk_and_bk_match_oracle(tmp_path):
     assert np.allclose(got[:, 2], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
 
 
-SURVEY =os.path.join(ROOT, "hitomi_amd", "bin", "hitomi_survey")
+SURVEY = os.path.join(ROOT, "hitomi_amd", "bin", "hitomi_survey")
 
 
 def run_survey(ini, cwd):
@@ -169,11 +169,13 @@ def test_survey_missing_randoms_exits_1(tmp_path):
     assert r.returncode == 1 and "Cannot open the file" in r.stdout and "'P_R.readParticleBOSS' fails." in r.stdout
 
 
-def test_survey_unsupported_measure_exits_1(tmp_path):
+@pytest.mark.parametrize("measure", ["bk", "3PCF", "window3PCF"])
+def test_survey_unsupported_measure_exits_1(tmp_path, measure):
+    """The survey bispectrum and three-point functions with ELL > 0 are rejected, not computed differently."""
     write_xyzw(tmp_path / "data.dat", np.ones((3, 3)) * 500.0, np.ones(3))
     ini = tmp_path / "p.ini"
     hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="data.dat",
-                  output_dir=str(tmp_path / "out"), measure="3PCF")
+                  output_dir=str(tmp_path / "out"), measure=measure, ell1=2, ell2=0, ELL=2)
     r = run_survey(ini, tmp_path)
     assert r.returncode == 1 and "not available" in r.stdout
 

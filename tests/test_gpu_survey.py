@@ -158,3 +158,53 @@ def test_survey_after_reconstruction(ctx, prec, measure):
     assert info["vsurvey"] == pytest.approx(s.vsurvey, rel=1e-12)
     tol = 1e-9 if prec == hm.FP64 else 1e-4
     assert np.abs(got - ref).max() <= tol * np.abs(ref0).max()
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+@pytest.mark.parametrize("ells", [(0, 0, 0), (1, 1, 0), (2, 2, 0)])
+def test_survey_bispectrum(ctx, prec, ells):
+    """calcBiSpectrum (bispec.hpp:4-353) with ELL = 0 for every ith_kbin: |dB| <= tol * max|B| with tol = 1e-9
+    (FP64) / 2e-4 (FP32); shell mode counts bit-exact; norm = Vsurvey^2 / (sum w_D)^3 to 1e-12."""
+    l1, l2, L_ = ells
+    n, L = 32, 800.0
+    kw = dict(kmin=0.02, kmax=0.1, n_kbin=5)
+    cat = get_cat("patch")
+    plan = hm.SurveyPlan(ctx, n, L, "TSC", ell=L_, ell1=l1, ell2=l2, precision=prec, measure="bk", **kw)
+    got, nm, info = plan.execute(*cat)
+    plan.close()
+    ref = np.zeros((kw["n_kbin"], kw["n_kbin"]), complex)
+    for ith in range(kw["n_kbin"]):
+        p = po.param(n, L, po.TSC, ell1=l1, ell2=l2, ELL=L_, ith_kbin=ith, **kw)
+        s = po.Survey(p, *cat)
+        ref[ith], _, _ = s.bk(p)
+    assert info["norm"] == pytest.approx(s.vsurvey ** 2 / s.wsumD ** 3, rel=1e-12)
+    tol = 1e-9 if prec == hm.FP64 else 2e-4
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    assert np.all(nm > 0)
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+@pytest.mark.parametrize("measure", ["3PCF", "window3PCF"])
+@pytest.mark.parametrize("ells", [(0, 0, 0), (1, 1, 0)])
+def test_survey_zeta(ctx, prec, measure, ells):
+    """calcThreePoint(Window)CorrelationFunction (bispec.hpp:355-874) with ELL = 0 for every ith_rbin:
+    |d norm (zeta - sn)| <= tol * max of it, tol = 1e-9 (FP64) / 1e-4 (FP32)."""
+    l1, l2, L_ = ells
+    n, L = 32, 800.0
+    rw = dict(rmin=60.0, rmax=240.0, n_rbin=4)
+    cat = get_cat("patch")
+    plan = hm.SurveyPlan(ctx, n, L, "TSC", ell=L_, ell1=l1, ell2=l2, precision=prec, measure=measure, **rw)
+    got, _, info = plan.execute(*cat)
+    plan.close()
+    ref = np.zeros((rw["n_rbin"], rw["n_rbin"]), complex)
+    for ith in range(rw["n_rbin"]):
+        p = po.param(n, L, po.TSC, ell1=l1, ell2=l2, ELL=L_, ith_rbin=ith, **rw)
+        s = po.Survey(p, *cat)
+        ref[ith], _, _ = s.zeta(p, window=measure == "window3PCF")
+    tol = 1e-9 if prec == hm.FP64 else 1e-4
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+
+
+def test_survey_bispectrum_rejects_ell_nonzero(ctx):
+    with pytest.raises(RuntimeError):
+        hm.SurveyPlan(ctx, 32, 800.0, "TSC", ell=2, ell1=2, ell2=0, measure="bk", kmin=0.02, kmax=0.1, n_kbin=5)
