@@ -57,6 +57,7 @@ class BoxConfig(ct.Structure):
         ("bk_ell", ct.c_int * 3),
         ("zeta", ct.c_int),
         ("zeta_ell", ct.c_int * 3),
+        ("interlace", ct.c_int),
     ]
 
 
@@ -384,7 +385,7 @@ class BoxPlan:
 
     def __init__(self, ctx: Context, n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2,
                  pk_ells=(0, 2, 4), bk=True, precision=FP32, regrid=True, prune=True, xi_ells=(), n_rbin=25,
-                 rmin=30.0, rmax=150.0, bk_ells=(0, 0, 0), zeta=False, zeta_ells=(0, 0, 0)):
+                 rmin=30.0, rmax=150.0, bk_ells=(0, 0, 0), zeta=False, zeta_ells=(0, 0, 0), interlace=False):
         self.ctx = ctx
         cfg = BoxConfig()
         cfg.mesh = make_mesh(n_mesh, boxsize, assign)
@@ -405,6 +406,7 @@ class BoxPlan:
             cfg.bk_ell[i] = int(bk_ells[i])
             cfg.zeta_ell[i] = int(zeta_ells[i])
         cfg.zeta = 1 if zeta else 0
+        cfg.interlace = 1 if interlace else 0
         cfg.n_xi_ells = len(xi_ells)
         for i, l in enumerate(xi_ells):
             cfg.xi_ells[i] = int(l)

@@ -47,6 +47,7 @@ struct BinArgs {
     int nk, ne;
     int ells[BIN_MAX_ELLS];
     double sn;
+    int sn_w2 = 0;  // 1: shot-noise shape W(k)^2 (interlaced field) instead of the aliased S(k)
 };
 
 template <typename T>
@@ -84,8 +85,8 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
             if (kindex >= FINE_N) continue;
             const int2 b = a.f2b[kindex];
             if (b.x < 0) continue;
-            const double S = (a.t.sf[0][ii] * a.t.sf[1][jj]) * a.t.sf[2][kk];
             const double W = wpow((a.t.wf[0][ii] * a.t.wf[1][jj]) * a.t.wf[2][kk], a.wp);
+            const double S = a.sn_w2 ? W * W : (a.t.sf[0][ii] * a.t.sf[1][jj]) * a.t.sf[2][kk];
             const double val = (p2 - a.sn * S) / (W * W);
             const double vim = (m ? -pim : pim) / (W * W);
             const double mu = (fabs(kmag) < 1.0e-15) ? 0.0 : kzv / kmag;
@@ -189,6 +190,7 @@ template <typename T> struct XiSrc {
     double dc;  // added to Re d at k = 0 in Re d conj(n): N for the shot-noise field, 0 for xi
     int odd = 0;  // 1: the spectrum whose transform is Im xi instead of Re xi
     const C<T>* dk2 = nullptr;  // set: g from the cross spectrum d conj(d2) (no dc term)
+    int sn_w2 = 0;              // 1: shot-noise shape W(k)^2 (interlaced field)
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
     __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
 #pragma clang fp contract(off)
@@ -208,15 +210,32 @@ template <typename T> struct XiSrc {
         // S and 1/W^2 of k and of -k from per-axis factors (the y, z ones are
         // the same along a column and leave the loop)
         const int ii = i ? n[0] - i : 0, jj = j ? n[1] - j : 0, kk = k ? n[2] - k : 0;
-        const double S1 = t.sf[0][i] * (t.sf[1][j] * t.sf[2][k]);
+        // the spectrum at -k: the conjugate for k_z inside (0, n_z/2); on the
+        // self-conjugate planes k_z = 0, n_z/2 the stored mirror element (an
+        // interlaced field need not be Hermitian there)
+        double p2m = p2, pimm = -pim;
+        if ((k == 0 || 2 * k == n[2]) && !(ii == i && jj == j)) {
+            const long long am = ((long long)ii * n[1] + jj) * nzh + k;
+            const C<T> dm = dk[am];
+            const double mr = (double)dm.x, mi = (double)dm.y;
+            if (dk2) {
+                const C<T> em = dk2[am];
+                p2m = mr * (double)em.x + mi * (double)em.y;
+                pimm = mi * (double)em.x - mr * (double)em.y;
+            } else {
+                p2m = mr * mr + mi * mi;
+                pimm = 0.0;
+            }
+        }
         const double Q1 = t.iw[0][i] * (t.iw[1][j] * t.iw[2][k]);
-        const double S2 = t.sf[0][ii] * (t.sf[1][jj] * t.sf[2][kk]);
         const double Q2 = t.iw[0][ii] * (t.iw[1][jj] * t.iw[2][kk]);
+        const double S1 = sn_w2 ? 1.0 / Q1 : t.sf[0][i] * (t.sf[1][j] * t.sf[2][k]);
+        const double S2 = sn_w2 ? 1.0 / Q2 : t.sf[0][ii] * (t.sf[1][jj] * t.sf[2][kk]);
         // g(k) = a + i b at k and at -k (whose spectrum is the conjugate):
         // Re xi transforms (g(k) + conj g(-k)) / 2, Im xi (g(k) - conj g(-k)) / 2i
         const double g1 = inv_v * ((p2 - sn * S1) * Q1);
-        const double g2 = inv_v * ((p2 - sn * S2) * Q2);
-        const double b1 = inv_v * (pim * Q1), b2 = -(inv_v * (pim * Q2));
+        const double g2 = inv_v * ((p2m - sn * S2) * Q2);
+        const double b1 = inv_v * (pim * Q1), b2 = inv_v * (pimm * Q2);
         if (odd) return mk<T>((T)(0.5 * (b1 + b2)), (T)(-(0.5 * (g1 - g2))));
         return mk<T>((T)(0.5 * (g1 + g2)), (T)(0.5 * (b1 - b2)));
     }
@@ -957,6 +976,7 @@ struct BoxPlanImpl {
     // reconstructed box, bispectrum / three-point function: the spectrum of D + alpha^2 R and the binned
     // cross spectrum with D - alpha R (re, im)
     DevBuf dk2, pkx_part, pkx_part_im, pkx_red;
+    DevBuf dki, spos;  // interlacing: the shifted grid's spectrum and positions
     // per shell field: its own supports on the triple mesh (x, y, z; -1 full)
     // and the z one alone
     std::vector<int4> fsup;
@@ -1269,6 +1289,9 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     // three-point function: multipole weights, r-bins with the 3PCF edges (field.hpp:1066-1095)
     std::vector<int2> zf2b;
     std::vector<double> zrb;
+    if (cfg->interlace && (cfg->bk || cfg->zeta))
+        return fail("box plan: interlacing applies to P_l and xi_l (the bispectrum and three-point function use the "
+                    "reference's single grid)");
     if (cfg->zeta) {
         p->zeta = true;
         for (int a = 0; a < 3; a++) p->zl[a] = cfg->zeta_ell[a];
@@ -1466,7 +1489,7 @@ void plan_destroy(BoxPlanImpl* p) {
                       &p->sn_pairs, &p->fields, &p->gram_part, &p->gram_red, &p->shell_cnt, &p->sid,
                       &p->f2b_r_d, &p->xi_r, &p->xi_part, &p->xi_cnt, &p->xi_red, &p->xi_cnt_red, &p->rpos, &p->rw,
                       &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->snp_part, &p->fsup_d, &p->fsz_d, &p->dk2, &p->pkx_part,
-                      &p->pkx_part_im, &p->pkx_red, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
+                      &p->pkx_part_im, &p->pkx_red, &p->dki, &p->spos, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
                       &p->zsy[1], &p->zsc[0], &p->zsc[1], &p->zfields, &p->zr, &p->zpart, &p->zcnt, &p->zred,
                       &p->zcnt_red, &p->zpairs_d};
     for (DevBuf* b : bufs) dev_free(*b);
@@ -1556,6 +1579,40 @@ static int field_transform(hm_ctx* ctx, BoxPlanImpl* p, const ShellSrc<T>& src, 
     if (!z_pass) return 0;
     RowGeom rg{(long long)M[0] * M[1], M[2], M[2], sup[2] >= 0 && 2 * sup[2] + 1 < M[2] ? sup[2] : -1};
     return fft_rows<T>(ctx, "tri_pass_z", M[2], +1, out, out, rg);
+}
+
+// Interlacing: the positions shifted by h (half a cell per axis), wrapped
+// into [0, L), and the combination d = (d_a + exp(i k.h) d_b) / 2 on the
+// half spectrum, k with the Nyquist index negative.
+__global__ __launch_bounds__(256) void k_shift_wrap(const double* __restrict__ pos, long long n, double h0, double h1,
+                                                   double h2, double L0, double L1, double L2,
+                                                   double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const double h[3] = {h0, h1, h2}, L[3] = {L0, L1, L2};
+    for (int a = 0; a < 3; a++) {
+        double x = pos[3 * q + a] + h[a];
+        if (x >= L[a]) x -= L[a];
+        out[3 * q + a] = x;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interlace(C<T>* __restrict__ d, const C<T>* __restrict__ db, int n1, int nzh,
+                                                  long long nh, AxisTabs t, double h0, double h1, double h2) {
+#pragma clang fp contract(off)
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nh) return;
+    const int k = (int)(e % nzh);
+    const long long r = e / nzh;
+    const int j = (int)(r % n1), i = (int)(r / n1);
+    const double ph = (t.kv[0][i] * h0 + t.kv[1][j] * h1) + t.kv[2][k] * h2;
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const C<T> a = d[e], b = db[e];
+    const double br = cs * (double)b.x - sn * (double)b.y, bi = cs * (double)b.y + sn * (double)b.x;
+    d[e] = mk<T>((T)(0.5 * ((double)a.x + br)), (T)(0.5 * ((double)a.y + bi)));
 }
 
 // D then R into one array with weights wD^pw (1 when wD is null) and
@@ -1653,6 +1710,41 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (fft_cols<T>(ctx, "fft_r2c_x", m.n[0], -1, dk, dk,
                     geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
         return -1;
+    // 2''. interlacing: the half-cell-shifted grid, transformed and combined (no counterpart in the reference)
+    const bool inter = p->cfg.interlace != 0;
+    if (inter) {
+        if (recon) {
+            ctx->set_error("box execute: interlacing is not available for two-catalogue fields");
+            return -1;
+        }
+        if ((!p->dki.p && alloc(ctx, p, p->dki, sizeof(C<T>) * (size_t)m.nh())) ||
+            dev_alloc(ctx, p->spos, sizeof(double) * 3 * (size_t)np))
+            return -1;
+        const double h[3] = {0.5 * m.L[0] / m.n[0], 0.5 * m.L[1] / m.n[1], 0.5 * m.L[2] / m.n[2]};
+        if (ctx->launch("interlace_shift", [&] {
+                hipLaunchKernelGGL(k_shift_wrap, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, d_pos,
+                                   np, h[0], h[1], h[2], m.L[0], m.L[1], m.L[2], (double*)p->spos.p);
+            }))
+            return -1;
+        if (assign_sorted(ctx, ag, (const double*)p->spos.p, nullptr, np, scale, (long long*)p->fixed.p, nullptr,
+                          p->sort))
+            return -1;
+        C<T>* dki = (C<T>*)p->dki.p;
+        if (launch_r2c<T>(ctx, "fft_r2c_z", m.n[2] / 2, fsrc, dki, (long long)m.n[0] * m.n[1])) return -1;
+        if (fft_cols<T>(ctx, "fft_r2c_y", m.n[1], -1, dki, dki,
+                        geom((long long)m.n[1] * nzh, nzh, m.n[0], m.n[0], -1, (int)nzh, (int)nzh, -1, -1)))
+            return -1;
+        if (fft_cols<T>(ctx, "fft_r2c_x", m.n[0], -1, dki, dki,
+                        geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1,
+                             -1)))
+            return -1;
+        const long long nh = m.nh();
+        if (ctx->launch("interlace", [&] {
+                hipLaunchKernelGGL(k_interlace<T>, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, ctx->stream, dk,
+                                   dki, m.n[1], (int)nzh, nh, tabs, h[0], h[1], h[2]);
+            }))
+            return -1;
+    }
     // 2'. reconstructed box with B or zeta: the shot-noise density D + alpha^2 R
     // (calcNormalDensityForBispectrumShotnoiseForBOXForReconstruction, field.hpp:574-605), transformed
     C<T>* dk2 = nullptr;
@@ -1691,6 +1783,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     ba.ne = std::max(p->ne, 1);
     for (int e = 0; e < BIN_MAX_ELLS; e++) ba.ells[e] = (e < p->ne) ? p->bin_ells[e] : 0;
     ba.sn = sn2;
+    ba.sn_w2 = inter ? 1 : 0;
     const int ncols = p->nk * ba.ne;
     const bool do_pk = p->ne > 0;
     if (do_pk) {
@@ -1746,6 +1839,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.sn = sn2;
         xs.inv_v = 1.0 / m.volume();
         xs.dc = 0.0;
+        xs.sn_w2 = inter ? 1 : 0;
         C<T>* xh = (C<T>*)p->xi_half.p;
         const int cx = support_count(p->rs[0], m.n[0]), cy = support_count(p->rs[1], m.n[1]);
         if (launch_cols<T, +1>(ctx, "xi2_pass_x", m.n[0], xs, xh,

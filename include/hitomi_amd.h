@@ -125,6 +125,9 @@ typedef struct {
     int zeta;                  /* 1: zeta_{ell1 ell2 ELL}(r1, r2) for every (ith_rbin, r2) pair on the r-bins
                                   (calcThreePointCorrelationFunctionForBOX, bispec.hpp:1205-1424) */
     int zeta_ell[3];
+    int interlace;             /* 1: combine the density with a second grid of the positions shifted by half a cell,
+                                  d = (d_a + exp(i k.h) d_b) / 2, shot noise N W(k)^2 (no counterpart in the
+                                  reference; P_l and xi_l only; 0 gives the reference's estimator) */
 } hm_box_config;
 
 /* Host result arrays (any may be NULL). */

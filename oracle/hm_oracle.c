@@ -623,6 +623,14 @@ void or_bessel_free(or_bessel *b) {
     } while (0)
 
 /* field.hpp:765-852 */
+/* Shot-noise shape of the binned estimators: the reference's
+ * ShotnoiseFunction (0), or W(k)^2 for an interlaced field, whose odd-order
+ * aliases cancel (1).  Set only by the interlaced drivers below. */
+static int g_sn_shape_w2 = 0;
+static double sn_shape(const or_param *p, const double *kvec) {
+    return g_sn_shape_w2 ? pow(or_assignment_function(p, kvec), 2) : or_shotnoise_function(p, kvec);
+}
+
 int or_calc_power_spectrum(const or_param *p, const double *f1, const double *f2, const double *kbin, double sn_re,
                            double sn_im, int ell, int m, double *pk, int *n_mode) {
     double dk_bin = kbin[1] - kbin[0];
@@ -649,7 +657,7 @@ int or_calc_power_spectrum(const or_param *p, const double *f1, const double *f2
                 if (kindex < n_temp) {
                     double pr, pi;
                     CMUL(f1[2 * coord], f1[2 * coord + 1], f2[2 * coord], -f2[2 * coord + 1], pr, pi);
-                    double S = or_shotnoise_function(p, kvec);
+                    double S = sn_shape(p, kvec);
                     pr -= sn_re * S;
                     pi -= sn_im * S;
                     double wf = or_assignment_function(p, kvec);
@@ -706,7 +714,7 @@ int or_calc_correlation_function(const or_param *p, const double *f1, const doub
                 kvec_of(p, i, j, k, dk, kvec);
                 double pr, pi;
                 CMUL(f1[2 * coord], f1[2 * coord + 1], f2[2 * coord], -f2[2 * coord + 1], pr, pi);
-                double S = or_shotnoise_function(p, kvec);
+                double S = sn_shape(p, kvec);
                 pr -= sn_re * S;
                 pi -= sn_im * S;
                 double w2 = pow(or_assignment_function(p, kvec), 2);
@@ -2063,5 +2071,83 @@ int or_survey_zeta(const or_param *p, const double *posD, const double *losD, co
     free(Y2);
     free(nm);
     free(tmp);
+    return 0;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* interlacing (no counterpart in the reference; off by default)        */
+/* ------------------------------------------------------------------ */
+
+/* The transformed box fluctuation of the positions and of the positions
+ * shifted by half a cell on every axis (wrapped into the box), combined as
+ * (d_a(k) + exp(i k.h) d_b(k)) / 2, h = (L_a / 2 n_a), k with the
+ * Nyquist index negative as everywhere else, for k_z index <= n_z / 2; the
+ * rest by conjugate symmetry. */
+int or_box_interlaced_k(const or_param *p, const double *pos, long long np, double *out) {
+    double *fb = grid_alloc(p);
+    double *ps = (double *)malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
+    double h[3], dk[3], kvec[3];
+    for (int a = 0; a < 3; a++) {
+        h[a] = 0.5 * p->boxsize[a] / (double)p->n_mesh[a];
+        dk[a] = 2.0 * M_PI / p->boxsize[a];
+    }
+    for (long long q = 0; q < np; q++)
+        for (int a = 0; a < 3; a++) {
+            double x = pos[3 * q + a] + h[a];
+            if (x >= p->boxsize[a]) x -= p->boxsize[a];
+            ps[3 * q + a] = x;
+        }
+    or_box_density_fluctuation(p, pos, np, out);
+    or_fourier_transform(p, out);
+    or_box_density_fluctuation(p, ps, np, fb);
+    or_fourier_transform(p, fb);
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long c = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dk, kvec);
+                double ph = kvec[0] * h[0] + kvec[1] * h[1] + kvec[2] * h[2];
+                double cr = cos(ph), ci = sin(ph), br, bi;
+                CMUL(cr, ci, fb[2 * c], fb[2 * c + 1], br, bi);
+                out[2 * c] = 0.5 * (out[2 * c] + br);
+                out[2 * c + 1] = 0.5 * (out[2 * c + 1] + bi);
+            }
+    /* the interlaced field is real: modes with k_z index above n_z / 2 are the
+     * conjugates of their mirrors, which fixes the phase of the Nyquist-index
+     * modes the way a half-spectrum (real-to-complex) layout does */
+    const int n0 = p->n_mesh[0], n1 = p->n_mesh[1], n2 = p->n_mesh[2];
+    for (int i = 0; i < n0; i++)
+        for (int j = 0; j < n1; j++)
+            for (int k = n2 / 2 + 1; k < n2; k++) {
+                long long c = or_coord(p, i, j, k), cm = or_coord(p, (n0 - i) % n0, (n1 - j) % n1, n2 - k);
+                out[2 * c] = out[2 * cm];
+                out[2 * c + 1] = -out[2 * cm + 1];
+            }
+    free(ps);
+    free(fb);
+    return 0;
+}
+
+/* powerspec.hpp:273-393 on the interlaced fluctuation with the shot noise
+ * N W(k)^2: P (xi = 0, 2*n_kbin) or xi (xi = 1, 2*n_rbin) as written to
+ * pk%d.dat / xi%d.dat. */
+int or_box_interlaced_2pt(const or_param *p, const double *pos, long long np, const double *bins, int xi,
+                          double *out, int *n_mode) {
+    double *f = grid_alloc(p);
+    or_box_interlaced_k(p, pos, np, f);
+    int nb = xi ? p->n_rbin : p->n_kbin;
+    double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)nb);
+    g_sn_shape_w2 = 1;
+    if (xi) or_calc_correlation_function(p, f, f, bins, (double)np, 0.0, p->ELL, 0, tmp, n_mode);
+    else or_calc_power_spectrum(p, f, f, bins, (double)np, 0.0, p->ELL, 0, tmp, n_mode);
+    g_sn_shape_w2 = 0;
+    double norm = p->volume / (double)np / (double)np;
+    for (int i = 0; i < nb; i++) {
+        out[2 * i] = norm * (0.0 + (double)(2 * p->ELL + 1) * tmp[2 * i]);
+        out[2 * i + 1] = norm * (0.0 + (double)(2 * p->ELL + 1) * tmp[2 * i + 1]);
+    }
+    free(tmp);
+    free(f);
     return 0;
 }

@@ -156,4 +156,11 @@ int or_survey_zeta(const or_param *p, const double *posD, const double *losD, co
                    const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
                    const double *rbin, int window, double *zeta_save, double *sn_save, double *out);
 
+/* Interlacing (no counterpart in the reference): the half-cell-shifted
+ * second grid combined in Fourier space, and P_l / xi_l from it with the
+ * shot noise N W(k)^2. */
+int or_box_interlaced_k(const or_param *p, const double *pos, long long np, double *out);
+int or_box_interlaced_2pt(const or_param *p, const double *pos, long long np, const double *bins, int xi,
+                          double *out, int *n_mode);
+
 #endif

@@ -85,6 +85,8 @@ def lib():
             "or_box_zeta": (ct.c_int, [P, _D, ct.c_longlong, _D, _D, _D, _D]),
             "or_box_recon_bk": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, _D, _D, _D]),
             "or_box_recon_zeta": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_longlong, ct.c_double, _D, _D, _D, _D]),
+            "or_box_interlaced_k": (ct.c_int, [P, _D, ct.c_longlong, _D]),
+            "or_box_interlaced_2pt": (ct.c_int, [P, _D, ct.c_longlong, _D, ct.c_int, _D, ct.POINTER(ct.c_int)]),
             "or_survey_bk": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, _D, _D, _D]),
             "or_survey_zeta": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
@@ -331,6 +333,25 @@ def box_recon_zeta(p: Param, posD, posR, alpha):
         raise ValueError("or_box_recon_zeta rejected the multipole combination")
     c = lambda a: a[0::2] + 1j * a[1::2]
     return c(out), c(z), c(sn)
+
+
+def box_interlaced_k(p: Param, pos):
+    """The interlaced transformed box fluctuation (complex grid)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    f = np.zeros(shape(p) + (2,))
+    lib().or_box_interlaced_k(ct.byref(p), dp(pos), pos.shape[0], dp(f))
+    return f[..., 0] + 1j * f[..., 1]
+
+
+def box_interlaced_2pt(p: Param, pos, xi=False):
+    """P_l (or xi_l) of the interlaced fluctuation: (values complex, n_mode)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    bins = rbin(p) if xi else kbin(p)
+    out = np.zeros(2 * len(bins))
+    nm = np.zeros(len(bins), dtype=np.int32)
+    lib().or_box_interlaced_2pt(ct.byref(p), dp(pos), pos.shape[0], dp(bins), 1 if xi else 0, dp(out),
+                                nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+    return out[0::2] + 1j * out[1::2], nm
 
 
 class Survey:

@@ -199,3 +199,39 @@ def test_bk_general_multipole(ctx, case, mode):
     assert np.abs(c(res["bk_sum"]) - ref_bk).max() <= tol * np.abs(ref_bk).max()
     assert np.abs(c(res["sn_sum"]) - ref_sn).max() <= tol * np.abs(ref_sn).max()
     assert np.abs(c(res["bk"]) - ref).max() <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+def test_interlaced_pk_xi(ctx, prec):
+    """Interlacing (no counterpart in the reference): P_l and xi_l of (d_a + exp(i k.h) d_b)/2 with the shot
+    noise N W^2 against the oracle's restatement; tolerances as the plain box (P: 1e-10 / 1e-5 of
+    |P_0| + V/N, xi: 1e-10 / 2e-5 of max|xi_0|), mode and point counts bit-exact."""
+    n, L, N = 32, 1000.0, 30000
+    pos = clustered_catalogue(N, L, seed=91, nclump=25, sigma=25.0)
+    kw = dict(kmin=0.02, kmax=0.3, n_kbin=14)
+    rw = dict(rmin=40.0, rmax=200.0, n_rbin=9)
+    plan = hm.BoxPlan(ctx, n, L, "TSC", pk_ells=(0, 2), bk=False, xi_ells=(0, 2), precision=prec, interlace=True,
+                      **kw, **rw)
+    res = plan.execute(pos)
+    plan.close()
+    tol = PK_TOL[prec]
+    ref0 = None
+    for e, l in enumerate((0, 2)):
+        p = po.param(n, L, po.TSC, ell1=l, ELL=l, **kw)
+        ref, nm = po.box_interlaced_2pt(p, pos)
+        ref0 = ref if l == 0 else ref0
+        assert np.array_equal(res["pk_nmode"], nm.astype(np.int64))
+        assert np.all(np.abs(res["pk"][e, :, 0] - ref.real) <= tol * (np.abs(ref0.real) + L ** 3 / N))
+    xtol = 1e-10 if prec == hm.FP64 else 2e-5
+    x0 = None
+    for e, l in enumerate((0, 2)):
+        p = po.param(n, L, po.TSC, ell1=l, ELL=l, **rw)
+        ref, nm = po.box_interlaced_2pt(p, pos, xi=True)
+        x0 = ref if l == 0 else x0
+        assert np.array_equal(res["xi_nmode"], nm.astype(np.int64))
+        assert np.abs(res["xi"][e] - ref.real).max() <= xtol * np.abs(x0.real).max()
+
+
+def test_interlace_rejected_for_bispectrum(ctx):
+    with pytest.raises(RuntimeError):
+        hm.BoxPlan(ctx, 32, 1000.0, "TSC", bk=True, interlace=True, kmin=0.02, kmax=0.1, n_kbin=5)
