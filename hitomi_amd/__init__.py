@@ -61,6 +61,16 @@ class BoxConfig(ct.Structure):
     ]
 
 
+class BatchConfig(ct.Structure):
+    _fields_ = [("box", BoxConfig), ("max_mocks", ct.c_longlong), ("cov_with_bk", ct.c_int), ("use_graph", ct.c_int)]
+
+
+class MockConfig(ct.Structure):
+    _fields_ = [("n_side", ct.c_int), ("boxsize", ct.c_double), ("amp", ct.c_double), ("k0", ct.c_double),
+                ("slope", ct.c_double), ("kcut", ct.c_double), ("bao_amp", ct.c_double), ("bao_period", ct.c_double),
+                ("f_growth", ct.c_double)]
+
+
 class BoxResult(ct.Structure):
     _fields_ = [
         ("pk", ct.POINTER(ct.c_double)),
@@ -164,6 +174,16 @@ _SIGS = {
     "hm_survey_plan_execute": (ct.c_int, [_VP, _VP, _VP, _VP, ct.c_longlong, _VP, _VP, ct.c_longlong,
                                           ct.POINTER(SurveyInfo), ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]),
     "hm_survey_plan_bytes": (ct.c_size_t, [_VP]),
+    "hm_batch_create": (_VP, [_VP, ct.POINTER(BatchConfig)]),
+    "hm_batch_destroy": (None, [_VP, _VP]),
+    "hm_batch_dims": (ct.c_int, [_VP, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
+    "hm_batch_push": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong]),
+    "hm_batch_count": (ct.c_longlong, [_VP]),
+    "hm_batch_vectors": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong]),
+    "hm_batch_covariance": (ct.c_int, [_VP, _VP, _VP, _VP]),
+    "hm_batch_reset": (ct.c_int, [_VP, _VP]),
+    "hm_mock_white_noise": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, _VP]),
+    "hm_mock_zeldovich": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, ct.c_longlong, _VP]),
 }
 
 _lib = None
@@ -573,3 +593,106 @@ class SurveyPlan:
         d = {k: getattr(info, k) for k in ("wsum_d", "wsum_r", "alpha", "vsurvey", "nbar", "norm")}
         d["shift"] = np.array(info.shift[:])
         return pk[..., 0] + 1j * pk[..., 1], nm, d
+
+
+def box_config(n_mesh, boxsize, assign="TSC", n_kbin=20, kmin=0.01, kmax=0.2, pk_ells=(0, 2, 4), bk=True,
+               precision=FP32, regrid=True, prune=True) -> BoxConfig:
+    """The hm_box_config of a P_l (+ B_000) plan."""
+    cfg = BoxConfig()
+    cfg.mesh = make_mesh(n_mesh, boxsize, assign)
+    cfg.precision = precision
+    cfg.n_kbin = int(n_kbin)
+    cfg.kmin = float(kmin)
+    cfg.kmax = float(kmax)
+    cfg.n_pk_ells = len(pk_ells)
+    for i, l in enumerate(pk_ells):
+        cfg.pk_ells[i] = int(l)
+    cfg.bk = 1 if bk else 0
+    cfg.regrid = 1 if regrid else 0
+    cfg.prune = 1 if prune else 0
+    cfg.n_rbin = 25
+    cfg.rmin = 30.0
+    cfg.rmax = 150.0
+    return cfg
+
+
+class Batch:
+    """Batched P_l (+ B_000) pipeline with device-side covariance sums
+    (hm_batch).  No counterpart in the reference: it runs one mock per process
+    and forms the covariance with numpy.cov (calc_data_and_cov.py:284-307)."""
+
+    def __init__(self, ctx: Context, n_mesh, boxsize, max_mocks=1024, cov_with_bk=False, use_graph=True, **kw):
+        self.ctx = ctx
+        cfg = BatchConfig()
+        cfg.box = box_config(n_mesh, boxsize, **kw)
+        cfg.max_mocks = int(max_mocks)
+        cfg.cov_with_bk = 1 if cov_with_bk else 0
+        cfg.use_graph = 1 if use_graph else 0
+        self.cfg = cfg
+        self.h = ctx.lib.hm_batch_create(ctx.h, ct.byref(cfg))
+        if not self.h:
+            ctx.raise_error("hm_batch_create")
+        v, c = ct.c_int(), ct.c_int()
+        ctx.lib.hm_batch_dims(self.h, ct.byref(v), ct.byref(c))
+        self.vec_len, self.cov_len = v.value, c.value
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hm_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def push(self, pos: DeviceArray, npart=None):
+        n = pos.shape[0] if npart is None else int(npart)
+        self.ctx.check(self.ctx.lib.hm_batch_push(self.ctx.h, self.h, pos.ptr, n), "hm_batch_push")
+
+    @property
+    def count(self):
+        return self.ctx.lib.hm_batch_count(self.h)
+
+    def vectors(self):
+        rows = min(self.count, self.cfg.max_mocks)
+        out = np.zeros((max(rows, 0), self.vec_len))
+        if rows > 0:
+            self.ctx.check(self.ctx.lib.hm_batch_vectors(self.ctx.h, self.h, out.ctypes.data, rows),
+                           "hm_batch_vectors")
+        return out
+
+    def covariance(self):
+        mean = np.zeros(self.cov_len)
+        cov = np.zeros((self.cov_len, self.cov_len))
+        self.ctx.check(self.ctx.lib.hm_batch_covariance(self.ctx.h, self.h, mean.ctypes.data, cov.ctypes.data),
+                       "hm_batch_covariance")
+        return mean, cov
+
+    def reset(self):
+        self.ctx.check(self.ctx.lib.hm_batch_reset(self.ctx.h, self.h), "hm_batch_reset")
+
+
+def mock_config(n_side, boxsize, amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5, bao_amp=0.0, bao_period=0.0628,
+                f_growth=0.0) -> MockConfig:
+    c = MockConfig()
+    c.n_side = int(n_side)
+    c.boxsize = float(boxsize)
+    c.amp, c.k0, c.slope, c.kcut = float(amp), float(k0), float(slope), float(kcut)
+    c.bao_amp, c.bao_period, c.f_growth = float(bao_amp), float(bao_period), float(f_growth)
+    return c
+
+
+def mock_white_noise(ctx: Context, cfg: MockConfig, seed) -> DeviceArray:
+    out = ctx.empty((cfg.n_side,) * 3, np.float64)
+    ctx.check(ctx.lib.hm_mock_white_noise(ctx.h, ct.byref(cfg), int(seed), out.ptr), "hm_mock_white_noise")
+    return out
+
+
+def mock_zeldovich(ctx: Context, cfg: MockConfig, seed, npart, out: DeviceArray = None) -> DeviceArray:
+    """Seeded Zel'dovich mock generated on the device: (npart, 3) FP64 positions."""
+    if out is None:
+        out = ctx.empty((int(npart), 3), np.float64)
+    ctx.check(ctx.lib.hm_mock_zeldovich(ctx.h, ct.byref(cfg), int(seed), int(npart), out.ptr), "hm_mock_zeldovich")
+    return out

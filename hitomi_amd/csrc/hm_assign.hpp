@@ -60,11 +60,13 @@ template <typename T> struct FixedRealSrc {
     const long long* g;
     int L;  // half row length (complex pairs per row)
     double inv_scale, factor, nbar, dV;
+    const double* dyn = nullptr;  // set: inv_scale and nbar read from device memory (dyn[0], dyn[1])
     __device__ inline C<T> operator()(long long r, int m) const {
 #pragma clang fp contract(off)
         const long long* q = g + r * (2 * (long long)L) + 2 * m;
-        const double a = (((double)q[0] * inv_scale) * factor - nbar) * dV;
-        const double b = (((double)q[1] * inv_scale) * factor - nbar) * dV;
+        const double is = dyn ? dyn[0] : inv_scale, nb = dyn ? dyn[1] : nbar;
+        const double a = (((double)q[0] * is) * factor - nb) * dV;
+        const double b = (((double)q[1] * is) * factor - nb) * dV;
         return mk<T>((T)a, (T)b);
     }
 };

@@ -48,6 +48,7 @@ struct BinArgs {
     int ells[BIN_MAX_ELLS];
     double sn;
     int sn_w2 = 0;  // 1: shot-noise shape W(k)^2 (interlaced field) instead of the aliased S(k)
+    const double* sn_ptr = nullptr;  // set: the shot noise read from device memory instead of sn
 };
 
 template <typename T>
@@ -60,6 +61,7 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
 #pragma clang fp contract(off)
     const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= a.nrows) return;
+    const double sn = a.sn_ptr ? *a.sn_ptr : a.sn;
     const int i = support_map((int)(row / a.nrow_y), a.s01, a.n[0]);
     const int j = support_map((int)(row % a.nrow_y), a.s01, a.n[1]);
     double* P = part + row * (long long)a.nk * a.ne;
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
             if (b.x < 0) continue;
             const double W = wpow((a.t.wf[0][ii] * a.t.wf[1][jj]) * a.t.wf[2][kk], a.wp);
             const double S = a.sn_w2 ? W * W : (a.t.sf[0][ii] * a.t.sf[1][jj]) * a.t.sf[2][kk];
-            const double val = (p2 - a.sn * S) / (W * W);
+            const double val = (p2 - sn * S) / (W * W);
             const double vim = (m ? -pim : pim) / (W * W);
             const double mu = (fabs(kmag) < 1.0e-15) ? 0.0 : kzv / kmag;
             for (int bb = b.x; bb <= b.y; bb++) {
@@ -191,6 +193,8 @@ template <typename T> struct XiSrc {
     int odd = 0;  // 1: the spectrum whose transform is Im xi instead of Re xi
     const C<T>* dk2 = nullptr;  // set: g from the cross spectrum d conj(d2) (no dc term)
     int sn_w2 = 0;              // 1: shot-noise shape W(k)^2 (interlaced field)
+    const double* sn_ptr = nullptr;  // set: sn and dc read from device memory
+    const double* dc_ptr = nullptr;
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
     __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
 #pragma clang fp contract(off)
@@ -203,7 +207,7 @@ template <typename T> struct XiSrc {
             p2 = re * (double)e.x + im * (double)e.y;
             pim = im * (double)e.x - re * (double)e.y;
         } else if (i == 0 && j == 0 && k == 0) {
-            p2 = re * (re + dc) + im * im;
+            p2 = re * (re + (dc_ptr ? *dc_ptr : dc)) + im * im;
         } else {
             p2 = re * re + im * im;
         }
@@ -233,8 +237,9 @@ template <typename T> struct XiSrc {
         const double S2 = sn_w2 ? 1.0 / Q2 : t.sf[0][ii] * (t.sf[1][jj] * t.sf[2][kk]);
         // g(k) = a + i b at k and at -k (whose spectrum is the conjugate):
         // Re xi transforms (g(k) + conj g(-k)) / 2, Im xi (g(k) - conj g(-k)) / 2i
-        const double g1 = inv_v * ((p2 - sn * S1) * Q1);
-        const double g2 = inv_v * ((p2m - sn * S2) * Q2);
+        const double snv = sn_ptr ? *sn_ptr : sn;
+        const double g1 = inv_v * ((p2 - snv * S1) * Q1);
+        const double g2 = inv_v * ((p2m - snv * S2) * Q2);
         const double b1 = inv_v * (pim * Q1), b2 = inv_v * (pimm * Q2);
         if (odd) return mk<T>((T)(0.5 * (b1 + b2)), (T)(-(0.5 * (g1 - g2))));
         return mk<T>((T)(0.5 * (g1 + g2)), (T)(0.5 * (b1 - b2)));
@@ -993,6 +998,7 @@ struct BoxPlanImpl {
     int zrs[3] = {-1, -1, -1};
     long long z_rows = 0;
     DevBuf zf2b_d, zrb_d, zsy[2], zsc[2], zfields, zr, zpart, zcnt, zred, zcnt_red, zpairs_d;
+    DevBuf qo_d;  // batched mode: raw triple-sum column of each bin pair
 };
 
 static int alloc(hm_ctx* ctx, BoxPlanImpl* p, DevBuf& b, size_t bytes) {
@@ -1491,7 +1497,7 @@ void plan_destroy(BoxPlanImpl* p) {
                       &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->snp_part, &p->fsup_d, &p->fsz_d, &p->dk2, &p->pkx_part,
                       &p->pkx_part_im, &p->pkx_red, &p->dki, &p->spos, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
                       &p->zsy[1], &p->zsc[0], &p->zsc[1], &p->zfields, &p->zr, &p->zpart, &p->zcnt, &p->zred,
-                      &p->zcnt_red, &p->zpairs_d};
+                      &p->zcnt_red, &p->zpairs_d, &p->qo_d};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -1647,6 +1653,19 @@ static int pack_pow(hm_ctx* ctx, const double* pD, const double* wD, long long n
     });
 }
 
+// The per-catalogue scalars of an execution into device memory (batched mode).
+__global__ void k_set_dyn(double* __restrict__ dyn, double inv_scale, double nbar, double sn2, double sn3, double dc,
+                          double n) {
+    if (threadIdx.x == 0) {
+        dyn[DYN_INV_SCALE] = inv_scale;
+        dyn[DYN_NBAR] = nbar;
+        dyn[DYN_SN2] = sn2;
+        dyn[DYN_SN3] = sn3;
+        dyn[DYN_DC] = dc;
+        dyn[DYN_N] = n;
+    }
+}
+
 template <typename T>
 // d_pos_r / nr / alpha: the reconstructed box (main_box.cpp:199-228): the
 // density is D - alpha R with unit weights (field.hpp:528-558) instead of
@@ -1656,8 +1675,10 @@ template <typename T>
 // window, with the shot noise and normalisation of the survey estimators.
 static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, hm_box_result* res,
                         const double* d_pos_r = nullptr, long long nr = 0, double alpha = 0.0,
-                        const BoxWeighted* wt = nullptr) {
+                        const BoxWeighted* wt = nullptr, const ExecOpts* xo = nullptr) {
     const Mesh& m = p->mesh;
+    const double* dyn = xo ? xo->dyn : nullptr;
+    const bool run_assign = !xo || xo->assign;
     const AxisTabs tabs = axis_tabs(p);
     const int wp = wpow_of(m.assign);
     if (np <= 0) {
@@ -1675,7 +1696,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     const bool weighted = wt != nullptr;
     const bool recon = weighted || nr > 0;  // two catalogues, no mean subtraction, cross-spectrum shot noise
     const double scale = fixed_scale_for(weighted ? wt->abs1 : (recon ? (double)np + std::fabs(alpha) * (double)nr : (double)np));
-    {
+    if (run_assign) {
         const size_t before = p->sort.bytes() + p->rpos.bytes + p->rw.bytes;
         if (sort_work_alloc(ctx, p->sort, ag, np + nr, recon)) return -1;
         if (recon && (dev_alloc(ctx, p->rpos, sizeof(double) * 3 * (size_t)(np + nr)) ||
@@ -1683,7 +1704,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             return -1;
         p->bytes += p->sort.bytes() + p->rpos.bytes + p->rw.bytes - before;
     }
-    if (recon) {
+    if (!run_assign) {
+    } else if (recon) {
         if (pack_pow(ctx, d_pos, weighted ? wt->wD : nullptr, np, d_pos_r, weighted ? wt->wR : nullptr, nr,
                      weighted ? wt->c1 : -alpha, 1, (double*)p->rpos.p, (double*)p->rw.p))
             return -1;
@@ -1697,10 +1719,18 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     // bispectrum and three-point function: N or N_D - alpha^3 N_R (field.hpp:1242-1250)
     const double sn2 = recon ? (double)np + pow(alpha, 2) * (double)nr : (double)np;
     const double sn3 = weighted ? wt->sn3 : (recon ? (double)np - pow(alpha, 3) * (double)nr : (double)np);
+    if (dyn && run_assign &&
+        ctx->launch("set_dyn", [&] {
+            hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(64), 0, ctx->stream, (double*)dyn, 1.0 / scale,
+                               recon ? 0.0 : (double)np / m.volume(), sn2, sn3, recon ? 0.0 : (double)np, (double)np);
+        }))
+        return -1;
+    if (xo && !xo->mesh) return 0;
     // 2. forward r2c with the fluctuation (rho - N/V) dV formed on load
     const double dV = m.volume() / (double)m.ng();
     FixedRealSrc<T> fsrc{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale, 1.0 / dV,
                          recon ? 0.0 : (double)np / m.volume(), dV};
+    fsrc.dyn = dyn;
     C<T>* dk = (C<T>*)p->dk.p;
     const long long nzh = m.nzh();
     if (launch_r2c<T>(ctx, "fft_r2c_z", m.n[2] / 2, fsrc, dk, (long long)m.n[0] * m.n[1])) return -1;
@@ -1783,6 +1813,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     ba.ne = std::max(p->ne, 1);
     for (int e = 0; e < BIN_MAX_ELLS; e++) ba.ells[e] = (e < p->ne) ? p->bin_ells[e] : 0;
     ba.sn = sn2;
+    ba.sn_ptr = dyn ? dyn + DYN_SN2 : nullptr;
     ba.sn_w2 = inter ? 1 : 0;
     const int ncols = p->nk * ba.ne;
     const bool do_pk = p->ne > 0;
@@ -1810,6 +1841,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             return -1;
         BinArgs bx = ba;
         bx.sn = sn3;
+        bx.sn_ptr = dyn ? dyn + DYN_SN3 : nullptr;
         if (fill_zero(ctx, p->pkx_part.p, nb) || fill_zero(ctx, p->pkx_part_im.p, nb) ||
             fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows))
             return -1;
@@ -1837,6 +1869,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.t = tabs;
         xs.wp = wp;
         xs.sn = sn2;
+        xs.sn_ptr = dyn ? dyn + DYN_SN2 : nullptr;
         xs.inv_v = 1.0 / m.volume();
         xs.dc = 0.0;
         xs.sn_w2 = inter ? 1 : 0;
@@ -1890,6 +1923,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.sn = sn3;
         xs.inv_v = 1.0 / m.volume();
         xs.dc = recon ? 0.0 : (double)np;
+        xs.sn_ptr = dyn ? dyn + DYN_SN3 : nullptr;
+        xs.dc_ptr = dyn ? dyn + DYN_DC : nullptr;
         xs.dk2 = dk2;
         // Re xi; for an odd l1 + l2 also Im xi (field.hpp:1116-1160 keeps both: an odd angular weight sees
         // the odd part of xi, and the even part on the z Nyquist plane), and always for the complex cross
@@ -2154,6 +2189,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.sn = sn3;
         xs.inv_v = 1.0 / m.volume();
         xs.dc = recon ? 0.0 : (double)np;
+        xs.sn_ptr = dyn ? dyn + DYN_SN3 : nullptr;
+        xs.dc_ptr = dyn ? dyn + DYN_DC : nullptr;
         xs.dk2 = dk2;
         C<T>* xh = (C<T>*)p->xi_half.p;
         const int cx = support_count(p->zrs[0], m.n[0]), cy = support_count(p->zrs[1], m.n[1]);
@@ -2204,6 +2241,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         zcntv.assign(zc.begin(), zc.end());
     }
+    if (xo && !xo->readback) return 0;
     // 7. results to host
     std::vector<double> pk(ncols), gram(2 * (size_t)npairs), graw((size_t)p->gram_cols), snp((size_t)npairs),
         snp2((size_t)npairs, 0.0);
@@ -2362,6 +2400,126 @@ int plan_execute_weighted(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos_d, lo
                           long long n_r, const BoxWeighted* wt, hm_box_result* res) {
     return p->cfg.precision == HM_FP64 ? plan_execute<double>(ctx, p, d_pos_d, n_d, res, d_pos_r, n_r, 0.0, wt)
                                        : plan_execute<float>(ctx, p, d_pos_d, n_d, res, d_pos_r, n_r, 0.0, wt);
+}
+
+int plan_execute_opts(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, const ExecOpts& xo) {
+    return p->cfg.precision == HM_FP64 ? plan_execute<double>(ctx, p, d_pos, np, nullptr, nullptr, 0, 0.0, nullptr, &xo)
+                                       : plan_execute<float>(ctx, p, d_pos, np, nullptr, nullptr, 0, 0.0, nullptr, &xo);
+}
+
+std::string plan_batch_check(const BoxPlanImpl* p) {
+    if (p->cfg.interlace) return "batched execution: interlacing is not supported";
+    if (p->nxe > 0 || p->zeta) return "batched execution: P_l and B only (no xi, no three-point function)";
+    if (p->cfg.bk && p->bk_gen) return "batched execution: B_000 only";
+    if (p->cfg.n_pk_ells <= 0 && !p->cfg.bk) return "batched execution: nothing to measure";
+    return "";
+}
+
+void plan_vector_dims(const BoxPlanImpl* p, int* n_pk, int* n_bk) {
+    *n_pk = p->cfg.n_pk_ells * p->nk;
+    *n_bk = p->cfg.bk ? (int)p->pairs.size() : 0;
+}
+
+int plan_batch_prepare(hm_ctx* ctx, BoxPlanImpl* p) {
+    if (!p->cfg.bk || p->qo_d.p) return 0;
+    // the raw column of each (a <= b) pair, as gram_to_pairs reads it
+    const int npairs = (int)p->pairs.size();
+    std::vector<int> qo((size_t)npairs);
+    if (!p->fused) {
+        for (int q = 0; q < npairs; q++) qo[q] = q;
+    } else {
+        std::vector<int> pidx((size_t)p->nk * p->nk, -1);
+        for (int q = 0; q < npairs; q++) pidx[(size_t)p->pairs[q].x * p->nk + p->pairs[q].y] = q;
+        int blk = 0;
+        for (int bi = 0; bi < p->ngrp; bi++)
+            for (int bj = bi; bj < p->ngrp; bj++, blk++)
+                for (int i = 0; i < 4; i++)
+                    for (int j = 0; j < 4; j++) {
+                        const int a = 4 * bi + i, b = 4 * bj + j;
+                        if (a >= p->nk || b >= p->nk || a > b) continue;
+                        qo[pidx[(size_t)a * p->nk + b]] = blk * 16 + i * 4 + j;
+                    }
+    }
+    if (alloc(ctx, p, p->qo_d, sizeof(int) * qo.size())) return -1;
+    HM_HIP_CHECK(ctx, hipMemcpy(p->qo_d.p, qo.data(), sizeof(int) * qo.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// The data vector of one mock, formed on the device with the operations (and
+// their order) of the host finalisation of plan_execute: P_l (field.hpp:839-845,
+// powerspec.hpp:306-316) for the requested multipoles, then the real part of
+// B_000 - SN (bispec.hpp:926-1073, 1175-1183) for each bin pair a <= b.
+struct FinArgs {
+    const double* pk;          // [nk][ne] mode sums
+    const long long* cnt;      // [nk] mode counts
+    int nk, ne, npk_ells;
+    int ells[BIN_MAX_ELLS];
+    const double* gram;        // raw triple sums (re, im) per column
+    const int* qo;             // raw column of each pair
+    const short2* pairs;       // (a, b), a <= b
+    int npairs;
+    const long long* shell;    // shell mode counts
+    const double* snp;         // Bessel sweep per pair
+    double M3, V, w_const, w_row, w_col;
+    int e_row, e_col;
+    const double* dyn;         // N and the three-point shot noise
+};
+
+__global__ __launch_bounds__(256) void k_batch_finalize(FinArgs a, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const double N = a.dyn[DYN_N], sn3 = a.dyn[DYN_SN3];
+    const double V = a.V;
+    const int npk = a.npk_ells * a.nk;
+    for (int t = threadIdx.x; t < npk + a.npairs; t += blockDim.x) {
+        if (t < npk) {
+            const int e = t / a.nk, b = t % a.nk;
+            const double v = a.cnt[b] ? a.pk[(size_t)b * a.ne + e] / (double)a.cnt[b] : 0.0;
+            const double norm = V / N / N;
+            out[t] = norm * (0.0 + (double)(2 * a.ells[e] + 1) * v);
+        } else {
+            const int q = t - npk;
+            const int pa = a.pairs[q].x, pb = a.pairs[q].y;
+            const double den = a.M3 * (double)a.shell[pa] * (double)a.shell[pb];
+            const double br = a.gram[2 * (size_t)a.qo[q]] / den;
+            const double pr = a.e_row < 0 ? 0.0 : (a.cnt[pa] ? a.pk[(size_t)pa * a.ne + a.e_row] / (double)a.cnt[pa] : 0.0);
+            const double pc = a.e_col < 0 ? 0.0 : (a.cnt[pb] ? a.pk[(size_t)pb * a.ne + a.e_col] / (double)a.cnt[pb] : 0.0);
+            const double base = a.w_const * sn3 + a.w_row * pr + a.w_col * pc;
+            const double sr = base + a.snp[q];  // i^(l1 + l2) = 1
+            const double norm = (V / N / N) * (V / N);
+            out[t] = norm * (br - sr);
+        }
+    }
+}
+
+int plan_finalize_device(hm_ctx* ctx, BoxPlanImpl* p, const double* dyn, double* out) {
+    FinArgs a{};
+    a.pk = (const double*)p->pk_red.p;
+    a.cnt = (const long long*)p->pk_cnt_red.p;
+    a.nk = p->nk;
+    a.ne = std::max(p->ne, 1);
+    a.npk_ells = p->cfg.n_pk_ells;
+    for (int e = 0; e < BIN_MAX_ELLS; e++) a.ells[e] = (e < p->ne) ? p->bin_ells[e] : 0;
+    a.gram = (const double*)p->gram_red.p;
+    a.qo = (const int*)p->qo_d.p;
+    a.pairs = (const short2*)p->pairs_d.p;
+    a.npairs = p->cfg.bk ? (int)p->pairs.size() : 0;
+    a.shell = (const long long*)p->shell_cnt.p;
+    a.snp = (const double*)p->sn_pairs.p;
+    a.M3 = (double)((long long)p->M[0] * p->M[1] * p->M[2]);
+    a.V = p->mesh.volume();
+    a.w_const = p->w_const;
+    a.w_row = p->w_row;
+    a.w_col = p->w_col;
+    a.e_row = p->e_row;
+    a.e_col = p->e_col;
+    a.dyn = dyn;
+    if (a.npairs > 0 && !a.qo) {
+        ctx->set_error("batched execution: plan not prepared");
+        return -1;
+    }
+    return ctx->launch("batch_finalize", [&] {
+        hipLaunchKernelGGL(k_batch_finalize, dim3(1), dim3(256), 0, ctx->stream, a, out);
+    });
 }
 
 size_t plan_bytes(const BoxPlanImpl* p) { return p->bytes; }

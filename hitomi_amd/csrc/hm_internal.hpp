@@ -80,4 +80,22 @@ size_t plan_bytes(const BoxPlanImpl* p);
 int plan_execute_weighted(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos_d, long long n_d, const double* d_pos_r,
                           long long n_r, const BoxWeighted* wt, hm_box_result* res);
 
+// Batched execution.  The per-catalogue scalars live in device memory
+// (DYN_LEN doubles), written by the assignment stage and read by the mesh
+// stage, so that the mesh stage issues the same launches for every catalogue
+// size and can be captured once into a graph and replayed.
+enum { DYN_INV_SCALE = 0, DYN_NBAR, DYN_SN2, DYN_SN3, DYN_DC, DYN_N, DYN_LEN = 8 };
+struct ExecOpts {
+    const double* dyn = nullptr;  // device scalars
+    bool assign = true;           // run the assignment (and write dyn)
+    bool mesh = true;             // run the transforms, binning and triple sums
+    bool readback = true;         // copy the results to the host (plan_execute_opts ignores res)
+};
+int plan_execute_opts(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, const ExecOpts& xo);
+std::string plan_batch_check(const BoxPlanImpl* p);  // empty: the plan can run batched
+void plan_vector_dims(const BoxPlanImpl* p, int* n_pk, int* n_bk);
+int plan_batch_prepare(hm_ctx* ctx, BoxPlanImpl* p);  // device tables of the finalisation (no capture)
+// the data vector [P_l(k_b)], then [B_000(k_a, k_b) - SN], a <= b, into out (device)
+int plan_finalize_device(hm_ctx* ctx, BoxPlanImpl* p, const double* dyn, double* out);
+
 }  // namespace hm

@@ -261,6 +261,63 @@ int hm_survey_plan_execute(hm_ctx *ctx, hm_survey_plan *plan, const double *d_po
                            long long *nmode);
 size_t hm_survey_plan_bytes(const hm_survey_plan *plan);
 
+/* ---------------------------------------------------------------- */
+/* Batched periodic-box pipeline and covariance.  No counterpart in the
+ * reference, which runs one mock per process and forms the covariance
+ * downstream with numpy.cov over the written files
+ * (struct/covariance/calc_data_and_cov.py:284-307).  Each mock's data vector
+ * is [P_l(k_b)] for the configured multipoles, then, with bk, the real part
+ * of B_000(k_a, k_b) - SN for the bin pairs a <= b (row-major upper
+ * triangle): the values hm_box_plan_execute returns in pk and bk, bit for
+ * bit.  Mocks are pushed without host synchronisation; the vectors and the
+ * FP64 sums of the covariance stay on the device. */
+typedef struct hm_batch hm_batch;
+typedef struct {
+    hm_box_config box;     /* P_l and, with bk, B_000 (no xi, zeta, interlacing) */
+    long long max_mocks;   /* data vectors kept on the device (further mocks still enter the covariance) */
+    int cov_with_bk;       /* 0: covariance of the P_l entries; 1: of the whole vector */
+    int use_graph;         /* 1: the mesh stage is captured once into a hipGraph and replayed */
+} hm_batch_config;
+
+hm_batch *hm_batch_create(hm_ctx *ctx, const hm_batch_config *cfg);
+void hm_batch_destroy(hm_ctx *ctx, hm_batch *b);
+/* vec_len: entries per data vector; cov_len: entries the covariance covers. */
+int hm_batch_dims(const hm_batch *b, int *vec_len, int *cov_len);
+/* One mock (d_pos: n x 3 FP64, already periodic; any n > 0).  Asynchronous:
+ * d_pos must stay valid until the next hm_sync or result call. */
+int hm_batch_push(hm_ctx *ctx, hm_batch *b, const double *d_pos, long long n);
+/* Mocks pushed since creation or the last reset. */
+long long hm_batch_count(const hm_batch *b);
+/* The first min(count, max_mocks, max_rows) data vectors, [rows][vec_len]. */
+int hm_batch_vectors(hm_ctx *ctx, hm_batch *b, double *out, long long max_rows);
+/* mean [cov_len] and the unbiased (N - 1) sample covariance [cov_len][cov_len]
+ * of numpy.cov (calc_data_and_cov.py:287), from device sums of the vectors
+ * minus the first one.  Needs two or more mocks. */
+int hm_batch_covariance(hm_ctx *ctx, hm_batch *b, double *mean, double *cov);
+int hm_batch_reset(hm_ctx *ctx, hm_batch *b);
+
+/* ---------------------------------------------------------------- */
+/* Seeded Zel'dovich mocks generated on the device.  No counterpart in the
+ * reference (its only synthetic input is an unseeded uniform catalogue,
+ * struct/measurement/nbody/pk/calc_random.py:38-49).  White noise from the
+ * counter-based Philox4x32-10 generator (Salmon et al. 2011) with FP64
+ * Box-Muller; the field delta(k) = FFT(noise) sqrt(P(k) / V_cell) with every
+ * mode on a Nyquist plane set to zero; displacement Psi(k) = i k / k^2
+ * delta(k), times 1 + f_growth along z; n particles on the lattice sites
+ * chosen by a seeded permutation, moved by Psi and wrapped into [0, L). */
+typedef struct {
+    int n_side;            /* lattice and field mesh per axis: a power of two in [16, 1024] */
+    double boxsize;        /* L */
+    double amp, k0, slope, kcut;     /* P(k) = amp (k/k0)^slope exp(-(k/kcut)^2), P(0) = 0 */
+    double bao_amp, bao_period;      /* times 1 + bao_amp sin(2 pi k / bao_period) */
+    double f_growth;       /* linear redshift-space factor along z (0: real space) */
+} hm_mock_config;
+
+/* Standard normal noise of the mock with this seed, [n_side^3] FP64. */
+int hm_mock_white_noise(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long seed, double *d_out);
+/* n <= n_side^3 particles, d_pos [n][3] FP64. */
+int hm_mock_zeldovich(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long seed, long long n, double *d_pos);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2151,3 +2151,103 @@ int or_box_interlaced_2pt(const or_param *p, const double *pos, long long np, co
     free(f);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Mock generator (no counterpart in the reference).
+ *
+ * Philox4x32-10 as published by Salmon, Moraes, Dror and Shaw, "Parallel
+ * random numbers: as easy as 1, 2, 3" (SC 2011), section 3.3: ten rounds of
+ *   (L0, R0, L1, R1) -> (hi(R1 * M1) ^ L0' ^ k0, lo(R1 * M1), hi(R0 * M0) ^ L1' ^ k1, lo(R0 * M0))
+ * in the Random123 word order, the key bumped by the Weyl constants between
+ * rounds.  Pinned by the Random123 known-answer vectors (tests).
+ * ------------------------------------------------------------------------ */
+#include <stdint.h>
+
+void or_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]) {
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; round++) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t a = (uint64_t)0xD2511F53u * (uint64_t)x0;
+        const uint64_t b = (uint64_t)0xCD9E8D57u * (uint64_t)x2;
+        const uint32_t y0 = (uint32_t)(b >> 32) ^ x1 ^ k0;
+        const uint32_t y1 = (uint32_t)b;
+        const uint32_t y2 = (uint32_t)(a >> 32) ^ x3 ^ k1;
+        const uint32_t y3 = (uint32_t)a;
+        x0 = y0;
+        x1 = y1;
+        x2 = y2;
+        x3 = y3;
+    }
+    out[0] = x0;
+    out[1] = x1;
+    out[2] = x2;
+    out[3] = x3;
+}
+
+/* Standard normal of cell i: counter (i lo, i hi, 0, 0), key (seed lo, seed
+ * hi); u1 = (w0 >> 5, w1 >> 6) as a 53-bit fraction plus 2^-54 (never 0),
+ * u2 likewise from w2, w3 (no offset); sqrt(-2 ln u1) cos(2 pi u2). */
+void or_mock_white_noise(long long n, unsigned long long seed, double *out) {
+    const unsigned int key[2] = {(unsigned int)(seed & 0xFFFFFFFFu), (unsigned int)(seed >> 32)};
+    for (long long i = 0; i < n; i++) {
+        const unsigned long long u = (unsigned long long)i;
+        const unsigned int ctr[4] = {(unsigned int)(u & 0xFFFFFFFFu), (unsigned int)(u >> 32), 0u, 0u};
+        unsigned int w[4];
+        or_philox4x32_10(ctr, key, w);
+        const double f1 = (double)(((uint64_t)(w[0] >> 5) << 26) | (uint64_t)(w[1] >> 6));
+        const double f2 = (double)(((uint64_t)(w[2] >> 5) << 26) | (uint64_t)(w[3] >> 6));
+        const double u1 = f1 / 9007199254740992.0 + 1.0 / 18014398509481984.0;
+        const double u2 = f2 / 9007199254740992.0;
+        out[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+}
+
+/* splitmix64 (Steele, Lea and Flood, OOPSLA 2014; Vigna's constants): the
+ * state advances by the golden-ratio increment and is then mixed. */
+static uint64_t or_splitmix(uint64_t *state) {
+    *state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = *state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static uint64_t or_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Sites of particles 0..np-1: all sites in order when np == sites, else a
+ * balanced Feistel network on 2h bits (2^(2h) >= sites, h minimal), four
+ * rounds (l, r) -> (r, l ^ (mix(r ^ key_s) & mask)) with keys from
+ * splitmix64 seeded by seed ^ 0x5DEECE66D, cycle-walked until < sites. */
+void or_mock_sites(long long np, long long sites, unsigned long long seed, long long *out) {
+    if (np == sites) {
+        for (long long i = 0; i < np; i++) out[i] = i;
+        return;
+    }
+    int bits = 0;
+    while ((1ull << bits) < (unsigned long long)sites) bits++;
+    const int h = (bits + 1) / 2;
+    const uint64_t mask = (1ull << h) - 1ull;
+    uint64_t st = (uint64_t)seed ^ 0x5DEECE66Dull, key[4];
+    for (int s = 0; s < 4; s++) key[s] = or_splitmix(&st);
+    for (long long i = 0; i < np; i++) {
+        uint64_t y = (uint64_t)i;
+        do {
+            uint64_t l = y >> h, r = y & mask;
+            for (int s = 0; s < 4; s++) {
+                const uint64_t t = l ^ (or_mix(r ^ key[s]) & mask);
+                l = r;
+                r = t;
+            }
+            y = (l << h) | r;
+        } while (y >= (uint64_t)sites);
+        out[i] = (long long)y;
+    }
+}

@@ -163,4 +163,13 @@ int or_box_interlaced_k(const or_param *p, const double *pos, long long np, doub
 int or_box_interlaced_2pt(const or_param *p, const double *pos, long long np, const double *bins, int xi,
                           double *out, int *n_mode);
 
+/* Mock generator (no counterpart in the reference): Philox4x32-10 of Salmon
+ * et al. (SC 2011), the per-cell standard normal (FP64 Box-Muller of two
+ * 53-bit uniforms) and the seeded choice of lattice sites (a four-round
+ * Feistel network with the splitmix64 finaliser as round function,
+ * cycle-walked into [0, sites)).  The FFT part is numpy in pyoracle.py. */
+void or_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]);
+void or_mock_white_noise(long long n, unsigned long long seed, double *out);
+void or_mock_sites(long long np, long long sites, unsigned long long seed, long long *out);
+
 #endif

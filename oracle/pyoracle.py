@@ -93,6 +93,9 @@ def lib():
                                           ct.c_double, _D, ct.c_int, _D, _D, _D]),
             "or_survey_xi": (ct.c_int, [P, _D, _D, _D, ct.c_longlong, _D, _D, _D, ct.c_longlong, ct.c_double,
                                         ct.c_double, _D, ct.c_int, _D, ct.POINTER(ct.c_int)]),
+            "or_philox4x32_10": (None, [ct.POINTER(ct.c_uint), ct.POINTER(ct.c_uint), ct.POINTER(ct.c_uint)]),
+            "or_mock_white_noise": (None, [ct.c_longlong, ct.c_ulonglong, _D]),
+            "or_mock_sites": (None, [ct.c_longlong, ct.c_longlong, ct.c_ulonglong, ct.POINTER(ct.c_longlong)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -471,3 +474,88 @@ def box_recon_2pt(p: Param, posD, posR, alpha, xi=False):
     if rc:
         raise ValueError("or_box_recon_2pt rejected the multipole combination")
     return out[0::2] + 1j * out[1::2], nm
+
+
+# ---------------------------------------------------------------------------
+# Mock generator (no counterpart in the reference): Philox noise and the site
+# choice from the C restatement, the transforms in numpy (FP64).
+# ---------------------------------------------------------------------------
+def philox4x32_10(ctr, key):
+    c = (ct.c_uint * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (ct.c_uint * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (ct.c_uint * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return tuple(int(v) for v in o)
+
+
+def mock_white_noise(n_side, seed):
+    out = np.empty(int(n_side) ** 3)
+    lib().or_mock_white_noise(out.size, int(seed), dp(out))
+    return out.reshape((n_side,) * 3)
+
+
+def mock_sites(np_, sites, seed):
+    out = np.empty(int(np_), dtype=np.int64)
+    lib().or_mock_sites(int(np_), int(sites), int(seed), out.ctypes.data_as(ct.POINTER(ct.c_longlong)))
+    return out
+
+
+def mock_pk(k, amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5, bao_amp=0.0, bao_period=0.0628):
+    """P(k) = amp (k/k0)^slope exp(-(k/kcut)^2) (1 + bao_amp sin(2 pi k / bao_period)), P(0) = 0."""
+    k = np.asarray(k, dtype=np.float64)
+    out = np.zeros_like(k)
+    m = k > 0
+    p = amp * (k[m] / k0) ** slope * np.exp(-((k[m] / kcut) ** 2))
+    if bao_amp != 0.0:
+        p = p * (1.0 + bao_amp * np.sin(2.0 * np.pi * k[m] / bao_period))
+    out[m] = p
+    return out
+
+
+def mock_kgrid(n, L):
+    """Wave vectors of the half spectrum (numpy rfftn layout) and the Nyquist mask."""
+    f = np.fft.fftfreq(n, d=1.0 / n)
+    fz = np.arange(n // 2 + 1)
+    kf = 2.0 * np.pi / L
+    kx = (kf * f)[:, None, None]
+    ky = (kf * f)[None, :, None]
+    kz = (kf * fz)[None, None, :]
+    nyq = (np.abs(f)[:, None, None] == n // 2) | (np.abs(f)[None, :, None] == n // 2) | (fz[None, None, :] == n // 2)
+    return kx, ky, kz, nyq
+
+
+def zeldovich_displacement(delta_k, n, L, f_growth=0.0):
+    """Psi_a(x) = IFFT(i k_a / k^2 delta(k)) (numpy normalisation 1/n^3); modes on
+    a Nyquist plane and k = 0 give nothing; Psi_z times 1 + f_growth."""
+    kx, ky, kz, nyq = mock_kgrid(n, L)
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    keep = (~nyq) & (k2 > 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = np.where(keep, 1j * delta_k / k2, 0.0)
+    comps = [kx, ky, kz * (1.0 + f_growth)]
+    return [np.fft.irfftn(c * g, s=(n,) * 3, axes=(0, 1, 2)) for c in comps]
+
+
+def mock_delta_k(n, L, seed, pk_kw):
+    """delta(k) = FFT(noise) sqrt(P(k) / V_cell) on the half spectrum."""
+    w = mock_white_noise(n, seed)
+    wk = np.fft.rfftn(w)
+    kx, ky, kz, _ = mock_kgrid(n, L)
+    kmag = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    return wk * np.sqrt(mock_pk(kmag, **pk_kw) / (L / n) ** 3)
+
+
+def mock_zeldovich(n, L, seed, np_, f_growth=0.0, **pk_kw):
+    """(np_, 3) positions: lattice sites (mock_sites) moved by Psi, wrapped as
+    x - L floor(x / L), then minus L where that reaches L."""
+    psi = zeldovich_displacement(mock_delta_k(n, L, seed, pk_kw), n, L, f_growth)
+    sites = mock_sites(np_, n ** 3, seed)
+    ix, iy, iz = sites // (n * n), (sites // n) % n, sites % n
+    cell = L / n
+    pos = np.empty((np_, 3))
+    for a, (idx, ps) in enumerate(zip((ix, iy, iz), psi)):
+        x = (idx + 0.5) * cell + ps.ravel()[sites]
+        v = x - L * np.floor(x / L)
+        v[v >= L] -= L
+        pos[:, a] = v
+    return pos

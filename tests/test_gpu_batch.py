@@ -1,0 +1,101 @@
+"""Batched pipeline: each mock's data vector equals what the per-mock plan
+returns (P_l real parts, then Re(B_000 - SN) on a <= b), bit for bit, for
+mocks of uneven size, eagerly and replayed from the captured graph (with
+eager assignment work between replays); the covariance equals numpy.cov of
+the vectors (the N - 1 estimator, calc_data_and_cov.py:287)."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+import hitomi_amd as hm
+
+pytestmark = pytest.mark.gpu
+
+N_SIDE, L = 32, 600.0
+PLAN = dict(n_kbin=6, kmin=0.02, kmax=0.14, pk_ells=(0, 2, 4), bk=True)
+SIZES = [32768, 21000, 26500, 17000, 30000, 24000]
+
+
+def mocks(ctx):
+    cfg = hm.mock_config(N_SIDE, L, amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5, bao_amp=0.05)
+    return [hm.mock_zeldovich(ctx, cfg, 100 + i, n) for i, n in enumerate(SIZES)]
+
+
+def per_mock_vector(res, nk, npk):
+    v = [res["pk"][e, :, 0] for e in range(npk)]
+    iu = np.triu_indices(nk)
+    v.append(res["bk"][..., 0][iu])
+    return np.concatenate(v)
+
+
+@pytest.mark.parametrize("prec,graph", [(hm.FP32, True), (hm.FP32, False), (hm.FP64, True)])
+def test_batch_equals_per_mock_bitwise(ctx, prec, graph):
+    plain = prec == hm.FP64
+    kw = dict(PLAN, precision=prec, regrid=not plain, prune=not plain)
+    pos = mocks(ctx)
+    plan = hm.BoxPlan(ctx, 64, L, "TSC", **kw)
+    ref = np.array([per_mock_vector(plan.execute(p), PLAN["n_kbin"], 3) for p in pos])
+    plan.close()
+    b = hm.Batch(ctx, 64, L, max_mocks=16, use_graph=graph, **kw)
+    assert b.vec_len == 3 * 6 + 21 and b.cov_len == 18
+    for p in pos:
+        b.push(p)
+    assert b.count == len(pos)
+    got = b.vectors()
+    assert np.array_equal(got, ref)
+    # after a reset the replayed graph gives the same vectors again
+    b.reset()
+    for p in pos[:3]:
+        b.push(p)
+    assert np.array_equal(b.vectors(), ref[:3])
+    b.close()
+
+
+@pytest.mark.parametrize("with_bk", [False, True])
+def test_batch_covariance_matches_numpy(ctx, with_bk):
+    pos = mocks(ctx)
+    b = hm.Batch(ctx, 64, L, max_mocks=4, cov_with_bk=with_bk, **PLAN)
+    for rep in range(2):
+        for p in pos:
+            b.push(p)
+    for p in pos[:2]:
+        b.push(p)
+    assert b.count == 2 * len(pos) + 2
+    stored = b.vectors()
+    assert stored.shape == (4, b.vec_len)  # only max_mocks vectors are kept
+    # the same vectors again from a fresh plan, for numpy
+    plan = hm.BoxPlan(ctx, 64, L, "TSC", **PLAN)
+    one = [per_mock_vector(plan.execute(p), 6, 3) for p in pos]
+    plan.close()
+    vecs = np.array(one * 2 + one[:2])
+    assert np.array_equal(stored, vecs[:4])
+    x = vecs[:, : b.cov_len]
+    mean, cov = b.covariance()
+    ref = np.cov(x.T)
+    sd = np.sqrt(np.diag(ref))
+    assert np.abs(mean - x.mean(0)).max() <= 1e-12 * np.abs(x).max()
+    assert np.abs(cov - ref).max() <= 1e-10 * sd.max() ** 2
+    assert np.allclose(cov, cov.T, rtol=0, atol=1e-12 * sd.max() ** 2)
+    b.close()
+
+
+def test_batch_needs_two_mocks_and_rejects_unsupported(ctx):
+    b = hm.Batch(ctx, 64, L, max_mocks=2, **PLAN)
+    b.push(mocks(ctx)[0])
+    with pytest.raises(hm.HitomiError):
+        b.covariance()
+    b.close()
+    with pytest.raises(hm.HitomiError):
+        _interlaced(ctx)
+
+
+def _interlaced(ctx):
+    cfg = hm.BatchConfig()
+    cfg.box = hm.box_config(64, L, **dict(PLAN, bk=False))
+    cfg.box.interlace = 1
+    cfg.max_mocks = 2
+    h = ctx.lib.hm_batch_create(ctx.h, ct.byref(cfg))
+    if not h:
+        ctx.raise_error("hm_batch_create")
+    ctx.lib.hm_batch_destroy(ctx.h, h)
