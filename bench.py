@@ -4,11 +4,20 @@
 One step = one mock through the whole measurement the reference spreads over
 3 + 20 jobs (main_box.cpp with measure = pk for l = 0, 2, 4 and measure = bk
 for every ith_kbin): P_0, P_2, P_4 and all 210 B_000(k1, k2) pairs with their
-shot noise, on a 512^3 TSC mesh, L = 2000 Mpc/h, 4,096,000 Zel'dovich
-particles (160^3 lattice) from a seeded Gaussian field with
+shot noise, on a 512^3 TSC mesh, L = 2000 Mpc/h, 4,000,000 Zel'dovich
+particles (a 256^3 lattice, seeded subset) from a seeded Gaussian field with
 P(k) = 2e4 (k/0.05)^-1.5 exp(-(k/0.5)^2) [1 + 0.05 sin(2 pi k / 0.0628)],
-20 k-bins with centres 0.015 .. 0.205 h/Mpc.  Particle positions are resident
-in HBM before the timed region; each step ends with the results on the host.
+20 k-bins with centres 0.015 .. 0.205 h/Mpc.  The mocks are generated on the
+device and resident in HBM before the timed region; the steps run through
+the batched pipeline (each mock's data vector formed on the device, no host
+synchronisation between mocks).
+
+The covariance run (--cov-mocks, default 512) is the batched covariance
+workload: mocks of uneven size (N ~ Poisson(lambda), lambda ~ Gamma with
+mean 1.5e6 and 25% scatter, truncated to [0.8e6, 2.5e6]), L = 1500 Mpc/h, 256^3 mesh, each generated on the device
+with its own seed and measured (P_0, P_2, P_4 and B_000) through the
+graph-replayed pipeline, ending with the 60 x 60 FP64 covariance of
+[P_0, P_2, P_4] x 20 bins.
 
 Prints one JSON line (see the repository's bench contract).
 """
@@ -28,8 +37,10 @@ sys.path.insert(0, ROOT)
 METRIC = "mocks/hour (P_ℓ+B_000, 512³) batched FP32 vs plain FP64 GPU; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
-N_MESH, BOX, N_SIDE = 512, 2000.0, 160
+N_MESH, BOX, N_SIDE, N_PART = 512, 2000.0, 256, 4_000_000
 KW = dict(n_kbin=20, kmin=0.015, kmax=0.205, pk_ells=(0, 2, 4), bk=True)
+PK = dict(amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5, bao_amp=0.05, bao_period=0.0628)
+COV_MESH, COV_BOX, COV_SIDE = 256, 1500.0, 256
 
 
 def log(*a):
@@ -95,14 +106,64 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8, ws=None):
 MEASURED_TRAFFIC = {"tri_z_gram": 1400.6e6}
 
 
-def make_mocks(count, seed0=1000):
-    from hitomi_amd import mocks
+def make_mocks(ctx, count, seed0=1000):
+    import hitomi_amd as hm
 
+    cfg = hm.mock_config(N_SIDE, BOX, **PK)
     out = []
     for i in range(count):
         t = time.time()
-        out.append(mocks.zeldovich(N_SIDE, BOX, seed0 + i, pk=mocks.pk_bao))
-        log(f"mock {i}: {out[-1].shape[0]} particles in {time.time() - t:.1f}s")
+        out.append(hm.mock_zeldovich(ctx, cfg, seed0 + i, N_PART))
+        ctx.sync()
+        log(f"mock {i}: {N_PART} particles generated on the device in {time.time() - t:.2f}s")
+    return out
+
+
+def covariance_run(ctx, n_mocks, seed0=5000):
+    """The batched covariance workload; returns its summary."""
+    import hitomi_amd as hm
+    from hitomi_amd import mocks
+
+    sizes = mocks.poisson_sizes(n_mocks, 1_500_000, 800_000, 2_500_000, seed=seed0, scatter=0.25)
+    cfg = hm.mock_config(COV_SIDE, COV_BOX, **PK)
+    buf = ctx.empty((int(sizes.max()), 3), np.float64)
+    # the generator alone, for its share of the time
+    hm.mock_zeldovich(ctx, cfg, seed0, int(sizes[0]), out=buf)
+    ctx.sync()
+    t = time.perf_counter()
+    for i in range(8):
+        hm.mock_zeldovich(ctx, cfg, seed0 + i, int(sizes[i]), out=buf)
+    ctx.sync()
+    gen_ms = (time.perf_counter() - t) * 1000.0 / 8
+    out = {"mocks": int(n_mocks), "n_mesh": COV_MESH, "boxsize": COV_BOX, "lattice": COV_SIDE,
+           "particles_min": int(sizes.min()), "particles_max": int(sizes.max()),
+           "particles_mean": float(sizes.mean()), "generator_ms_per_mock": round(gen_ms, 3)}
+    for graph in (True, False):
+        b = hm.Batch(ctx, COV_MESH, COV_BOX, max_mocks=0, use_graph=graph, precision=hm.FP32, **KW)
+        hm.mock_zeldovich(ctx, cfg, seed0, int(sizes[0]), out=buf)
+        b.push(buf, int(sizes[0]))  # warm-up (and, with the graph, the capture on the next push)
+        hm.mock_zeldovich(ctx, cfg, seed0 + 1, int(sizes[1]), out=buf)
+        b.push(buf, int(sizes[1]))
+        b.reset()
+        ctx.sync()
+        t = time.perf_counter()
+        for i, n in enumerate(sizes):
+            hm.mock_zeldovich(ctx, cfg, seed0 + i, int(n), out=buf)
+            b.push(buf, int(n))
+        mean, cov = b.covariance()
+        ms = (time.perf_counter() - t) * 1000.0 / n_mocks
+        key = "graph" if graph else "eager"
+        out[f"{key}_ms_per_mock"] = round(ms, 3)
+        if graph:
+            out["mocks_per_hour"] = round(3600e3 / ms, 1)
+            out["vector"] = f"{b.vec_len} entries ([P_0, P_2, P_4] x 20, then B_000 on 210 bin pairs)"
+            out["covariance"] = f"{b.cov_len} x {b.cov_len} FP64 ([P_0, P_2, P_4] x 20), N - 1 estimator"
+            sd = np.sqrt(np.diag(cov))
+            out["p0_rel_sigma_k1"] = round(float(sd[0] / abs(mean[0])), 5)
+            out["finite"] = bool(np.isfinite(cov).all())
+        b.close()
+        log(f"covariance run ({key}): {ms:.3f} ms per mock incl. generation ({gen_ms:.3f} ms)")
+    buf.free()
     return out
 
 
@@ -132,7 +193,7 @@ def cpu_baseline(pos, budget_s=30.0):
     bk_job = 6 * t_assign + scale * max(t_bk - 6 * t_assign, 0.0)
     sec_per_mock = 3 * pk_job + 20 * bk_job
     sample = (f"oracle FP64, 1 core: assign {t_assign:.2f}s, P_0 job {t_pk:.2f}s, B_000 job (ith_kbin=0) "
-              f"{t_bk:.2f}s at 128^3 on the 4.1e6-particle mock; scaled x{scale:.0f} (mesh part) to 512^3 and to "
+              f"{t_bk:.2f}s at 128^3 on the 4.0e6-particle mock; scaled x{scale:.0f} (mesh part) to 512^3 and to "
               f"3 P + 20 B jobs = {sec_per_mock:.0f} s/mock")
     return {"value": 3600.0 / sec_per_mock, "unit": "mocks/hour", "cores": 1, "kind": "port", "sample": sample,
             "sample_seconds": round(t_assign + t_pk + t_bk, 1)}
@@ -160,10 +221,9 @@ def run(args):
         print(json.dumps({"error": "this build measures one GPU: collectives, sharding and multi-GPU runs are out "
                                    "of scope; run with --gpus 1"}))
         sys.exit(2)
-    pos_list = make_mocks(args.mocks)
-    npart = pos_list[0].shape[0]
     ctx = hm.Context(0)
-    dpos = [ctx.array(p) for p in pos_list]
+    dpos = make_mocks(ctx, args.mocks)
+    npart = N_PART
     plan = hm.BoxPlan(ctx, N_MESH, BOX, "TSC", precision=hm.FP32, **KW)
     M = plan.triple_mesh[0]
     s = support(N_MESH, BOX, KW["kmin"], KW["kmax"], KW["n_kbin"])
@@ -191,27 +251,47 @@ def run(args):
     dom = next(r for r in rows if algorithmic_bytes(r[1], N_MESH, M, s, KW["n_kbin"], npart, ws=ws))
     dom_name = dom[1]
     ctx.timing(False)
+    r = plan.execute(dpos[0])
+    ref_vec = np.concatenate([r["pk"][e, :, 0] for e in range(3)] + [r["bk"][..., 0][np.triu_indices(KW["n_kbin"])]])
+    plan.close()
 
+    # timed region: the batched pipeline (eager launches, so that HIP events can
+    # bracket the dominant kernel's launches)
+    batch = hm.Batch(ctx, N_MESH, BOX, max_mocks=1, use_graph=False, precision=hm.FP32, **KW)
     for i in range(args.warmup):
-        plan.execute(dpos[i % len(dpos)])
-    ctx.sync()
-    # timed region: HIP events around the dominant kernel's launches only
+        batch.push(dpos[i % len(dpos)])
+    batch.reset()
     ctx.timing(True)
     ctx.timing_select(dom_name)
     ctx.timing_reset()
     ctx.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        plan.execute(dpos[i % len(dpos)])
+        batch.push(dpos[i % len(dpos)])
     ctx.sync()
     t1 = time.perf_counter()
     ms_dom, n_dom = ctx.kernel_time(dom_name)
     ctx.timing(False)
+    same = bool(np.array_equal(batch.vectors()[0], ref_vec))
+    batch.close()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = 3600.0 * 1000.0 / ms_per_step
     b_dom = algorithmic_bytes(dom_name, N_MESH, M, s, KW["n_kbin"], npart, ws=ws)
     achieved = b_dom / (ms_dom / n_dom) / 1e6
-    plan.close()
+    log(f"batched: {ms_per_step:.3f} ms per mock; first vector equals the per-mock plan's: {same}")
+
+    # the same steps replayed from the captured graph
+    gbatch = hm.Batch(ctx, N_MESH, BOX, max_mocks=1, use_graph=True, precision=hm.FP32, **KW)
+    for i in range(max(args.warmup, 2)):
+        gbatch.push(dpos[i % len(dpos)])
+    ctx.sync()
+    t = time.perf_counter()
+    for i in range(args.steps):
+        gbatch.push(dpos[i % len(dpos)])
+    ctx.sync()
+    ms_graph = (time.perf_counter() - t) * 1000.0 / args.steps
+    gbatch.close()
+    log(f"graph replay: {ms_graph:.3f} ms per mock")
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
 
@@ -245,9 +325,13 @@ def run(args):
         "config": {
             "workload": "periodic-box P_l (l=0,2,4) + B_000 all 210 (k1,k2) pairs with shot noise, one mock per step",
             "n_mesh": N_MESH, "boxsize": BOX, "assignment": "TSC", "particles": npart,
-            "catalogue": "Zel'dovich, 160^3 lattice, seeded Gaussian field with BAO wiggle",
+            "catalogue": "Zel'dovich, seeded 4,000,000-site subset of a 256^3 lattice, seeded Gaussian field with "
+                         "BAO wiggle, generated on the device",
             "k_bins": "20 bins, centres 0.015-0.205 h/Mpc", "triple_mesh": M, "parallelism": "1 GPU",
+            "pipeline": "batched (device-side data vectors, no host synchronisation between mocks)",
         },
+        "batched_vector_equals_per_mock": same,
+        "graph_replay_ms_per_step": round(ms_graph, 3),
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dom_name),
@@ -256,8 +340,10 @@ def run(args):
     if plain:
         out["plain_fp64"] = plain
         out["speedup_vs_plain_fp64"] = round(value / plain["mocks_per_hour"], 2)
+    if args.cov_mocks > 0:
+        out["covariance_run"] = covariance_run(ctx, args.cov_mocks)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(pos_list[0])
+        out["cpu_baseline"] = cpu_baseline(dpos[0].download())
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -270,6 +356,7 @@ def main():
     ap.add_argument("--mocks", type=int, default=2, help="distinct resident mocks cycled through")
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain FP64 path (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cov-mocks", type=int, default=512, help="mocks of the covariance run (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device copy of BYTES (scales HBM counters under rocprofv3 --pmc)")
     run(ap.parse_args())

@@ -86,6 +86,7 @@ void hm_ctx_destroy(hm_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+    if (ctx->mock_ws) (void)hipFree(ctx->mock_ws);
     for (auto& pe : ctx->pending) {
         (void)hipEventDestroy(pe.a);
         (void)hipEventDestroy(pe.b);

@@ -26,6 +26,9 @@ struct hm_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, long long>> kstats;  // name -> (ms total, launches)
     bool capturing = false;
+    // device workspace of the mock generator, kept between calls
+    void* mock_ws = nullptr;
+    size_t mock_ws_bytes = 0;
 
     void set_error(const std::string& m) { err = m; }
     hipEvent_t get_event();

@@ -209,18 +209,32 @@ int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long
     const long long n = cfg->n_side, total = n * n * n, half = n * n * (n / 2 + 1);
     if (np <= 0 || np > total) return bad(ctx, "mock: need 0 < n <= n_side^3 particles");
     if (!d_pos) return bad(ctx, "mock: no output");
-    // noise, its spectrum, three displacement spectra; the real displacements reuse the noise and spectrum space
-    DevBuf noise, spec, s3;
+    // workspace (kept on the context): noise / x displacement, its spectrum / y displacement, three
+    // displacement spectra, z displacement
+    const size_t need = sizeof(double) * (size_t)total * 2 + sizeof(C<double>) * (size_t)half * 4;
+    if (ctx->mock_ws_bytes < need) {
+        HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->mock_ws) (void)hipFree(ctx->mock_ws);
+        ctx->mock_ws = nullptr;
+        ctx->mock_ws_bytes = 0;
+        if (hipMalloc(&ctx->mock_ws, need) != hipSuccess) {
+            ctx->mock_ws = nullptr;
+            return bad(ctx, "mock: out of device memory");
+        }
+        ctx->mock_ws_bytes = need;
+    }
+    char* ws = (char*)ctx->mock_ws;
+    double* noise = (double*)ws;
+    C<double>* spec = (C<double>*)(ws + sizeof(double) * (size_t)total);
+    C<double>* s3 = spec + half;
+    double* extra = (double*)(s3 + 3 * half);
     int rc = 0;
-    if (dev_alloc(ctx, noise, sizeof(double) * (size_t)total) || dev_alloc(ctx, spec, sizeof(C<double>) * (size_t)half) ||
-        dev_alloc(ctx, s3, 3 * sizeof(C<double>) * (size_t)half))
-        rc = -1;
     const int dims[3] = {(int)n, (int)n, (int)n};
-    C<double>* px = (C<double>*)s3.p;
+    C<double>* px = s3;
     C<double>* py = px + half;
     C<double>* pz = py + half;
-    if (!rc) rc = launch_noise(ctx, total, seed, (double*)noise.p);
-    if (!rc) rc = fft3d_r2c<double>(ctx, (const double*)noise.p, (C<double>*)spec.p, dims);
+    if (!rc) rc = launch_noise(ctx, total, seed, noise);
+    if (!rc) rc = fft3d_r2c<double>(ctx, noise, spec, dims);
     PkArgs a;
     a.n = (int)n;
     a.L = cfg->boxsize;
@@ -237,15 +251,13 @@ int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long
         rc = ctx->launch("mock_spec", [&] {
             const long long blocks = std::min((half + 255) / 256, 65536LL);
             hipLaunchKernelGGL(k_zeldovich_spec, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                               (const C<double>*)spec.p, a, px, py, pz);
+                               (const C<double>*)spec, a, px, py, pz);
         });
     // real displacements: x into the noise buffer, y into the noise spectrum's buffer (n^2 (n + 2) doubles
     // hold one real grid), z into a third grid
-    double* dx = (double*)noise.p;
-    double* dy = (double*)spec.p;
-    DevBuf extra;
-    if (!rc) rc = dev_alloc(ctx, extra, sizeof(double) * (size_t)total);
-    double* dz = (double*)extra.p;
+    double* dx = noise;
+    double* dy = (double*)spec;
+    double* dz = extra;
     if (!rc) rc = fft3d_c2r<double>(ctx, px, dx, dims);
     if (!rc) rc = fft3d_c2r<double>(ctx, py, dy, dims);
     if (!rc) rc = fft3d_c2r<double>(ctx, pz, dz, dims);
@@ -263,11 +275,6 @@ int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long
             hipLaunchKernelGGL(k_place, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, np, (int)n, cfg->boxsize, pm,
                                dx, dy, dz, inv_n3, d_pos);
         });
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = bad(ctx, "mock: stream error");
-    dev_free(noise);
-    dev_free(spec);
-    dev_free(s3);
-    dev_free(extra);
     return rc;
 }
 

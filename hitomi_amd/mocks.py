@@ -63,12 +63,21 @@ def zeldovich(n_side, L, seed, pk=pk_powerlaw, f_growth=0.0, bias_shift=0.0):
     return np.ascontiguousarray(pos)
 
 
-def poisson_sizes(n_mocks, mean, lo, hi, seed):
-    """Per-mock particle counts ~ Poisson(mean) truncated to [lo, hi] by redrawing."""
+def poisson_sizes(n_mocks, mean, lo, hi, seed, scatter=0.0):
+    """Per-mock particle counts ~ Poisson(lambda) truncated to [lo, hi] by
+    redrawing; lambda = mean, or with scatter > 0 lambda ~ Gamma with that
+    mean and relative width (a Gamma-Poisson mixture: uneven sizes)."""
     rng = np.random.default_rng(seed)
-    out = rng.poisson(mean, n_mocks)
+
+    def draw(k):
+        if scatter > 0.0:
+            shape = 1.0 / scatter ** 2
+            return rng.poisson(rng.gamma(shape, mean / shape, k))
+        return rng.poisson(mean, k)
+
+    out = draw(n_mocks)
     bad = (out < lo) | (out > hi)
     while bad.any():
-        out[bad] = rng.poisson(mean, int(bad.sum()))
+        out[bad] = draw(int(bad.sum()))
         bad = (out < lo) | (out > hi)
     return out

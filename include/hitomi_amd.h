@@ -315,7 +315,9 @@ typedef struct {
 
 /* Standard normal noise of the mock with this seed, [n_side^3] FP64. */
 int hm_mock_white_noise(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long seed, double *d_out);
-/* n <= n_side^3 particles, d_pos [n][3] FP64. */
+/* n <= n_side^3 particles, d_pos [n][3] FP64.  Asynchronous on the
+ * context's stream; the context keeps a workspace of 2 n_side^3 doubles and
+ * four half spectra (0.8 GB at n_side = 256) until it is destroyed. */
 int hm_mock_zeldovich(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long seed, long long n, double *d_pos);
 
 #ifdef __cplusplus
