@@ -71,6 +71,11 @@ class MockConfig(ct.Structure):
                 ("f_growth", ct.c_double)]
 
 
+class LReconConfig(ct.Structure):
+    _fields_ = [("n_mesh", ct.c_int), ("boxsize", ct.c_double), ("b1", ct.c_double), ("RG", ct.c_double),
+                ("psi_scale", ct.c_double), ("lr", ct.c_double), ("init_seed", ct.c_ulonglong), ("precision", ct.c_int)]
+
+
 class BoxResult(ct.Structure):
     _fields_ = [
         ("pk", ct.POINTER(ct.c_double)),
@@ -184,6 +189,21 @@ _SIGS = {
     "hm_batch_reset": (ct.c_int, [_VP, _VP]),
     "hm_mock_white_noise": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, _VP]),
     "hm_mock_zeldovich": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, ct.c_longlong, _VP]),
+    "hm_mock_zeldovich_biased": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, ct.c_double, ct.c_double,
+                                            ct.c_longlong, _VP, _VP, ct.POINTER(ct.c_longlong)]),
+    "hm_lrecon_create": (_VP, [_VP, ct.POINTER(LReconConfig)]),
+    "hm_lrecon_destroy": (None, [_VP, _VP]),
+    "hm_lrecon_n_params": (ct.c_longlong, [_VP]),
+    "hm_lrecon_get_params": (ct.c_int, [_VP, _VP, _VP]),
+    "hm_lrecon_set_params": (ct.c_int, [_VP, _VP, _VP]),
+    "hm_lrecon_features": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, _VP]),
+    "hm_lrecon_net_forward": (ct.c_int, [_VP, _VP, _VP, _VP]),
+    "hm_lrecon_net_backward": (ct.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "hm_lrecon_predict": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, _VP, _VP]),
+    "hm_lrecon_gradient": (ct.c_int, [_VP, _VP, _VP, _VP, ct.c_longlong, ct.POINTER(ct.c_double),
+                                      ct.POINTER(ct.c_double), _VP, _VP]),
+    "hm_lrecon_train_step": (ct.c_int, [_VP, _VP, _VP, _VP, ct.c_longlong, ct.POINTER(ct.c_double),
+                                        ct.POINTER(ct.c_double)]),
 }
 
 _lib = None
@@ -696,3 +716,101 @@ def mock_zeldovich(ctx: Context, cfg: MockConfig, seed, npart, out: DeviceArray 
         out = ctx.empty((int(npart), 3), np.float64)
     ctx.check(ctx.lib.hm_mock_zeldovich(ctx.h, ct.byref(cfg), int(seed), int(npart), out.ptr), "hm_mock_zeldovich")
     return out
+
+
+def mock_zeldovich_biased(ctx: Context, cfg: MockConfig, seed, mean_n, b_lag, capacity=None):
+    """Lagrangian-biased Zel'dovich tracers generated on the device:
+    (positions DeviceArray (n, 3), true shifts DeviceArray (n, 3), n)."""
+    cap = int(capacity if capacity is not None else 1.1 * mean_n + 10 * mean_n ** 0.5 + 100)
+    pos = ctx.empty((cap, 3), np.float64)
+    shift = ctx.empty((cap, 3), np.float64)
+    n = ct.c_longlong()
+    ctx.check(ctx.lib.hm_mock_zeldovich_biased(ctx.h, ct.byref(cfg), int(seed), float(mean_n), float(b_lag), cap,
+                                               pos.ptr, shift.ptr, ct.byref(n)), "hm_mock_zeldovich_biased")
+    pos.shape = (n.value, 3)
+    shift.shape = (n.value, 3)
+    pos.nbytes = shift.nbytes = n.value * 24
+    return pos, shift, n.value
+
+
+class LRecon:
+    """Learned correction of the plain reconstruction (hm_lrecon)."""
+
+    def __init__(self, ctx: Context, n_mesh, boxsize, b1=2.0, RG=15.0, psi_scale=10.0, lr=1e-3, init_seed=1,
+                 precision=FP32):
+        self.ctx = ctx
+        cfg = LReconConfig()
+        cfg.n_mesh, cfg.boxsize, cfg.b1, cfg.RG = int(n_mesh), float(boxsize), float(b1), float(RG)
+        cfg.psi_scale, cfg.lr, cfg.init_seed, cfg.precision = float(psi_scale), float(lr), int(init_seed), precision
+        self.cfg = cfg
+        self.n = int(n_mesh)
+        self.h = ctx.lib.hm_lrecon_create(ctx.h, ct.byref(cfg))
+        if not self.h:
+            ctx.raise_error("hm_lrecon_create")
+        self.n_params = ctx.lib.hm_lrecon_n_params(self.h)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hm_lrecon_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx.lib.hm_lrecon_get_params(self.ctx.h, self.h, out.ctypes.data), "hm_lrecon_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self.ctx.check(self.ctx.lib.hm_lrecon_set_params(self.ctx.h, self.h, p.ctypes.data), "hm_lrecon_set_params")
+
+    def features(self, pos: DeviceArray, npart=None):
+        n = pos.shape[0] if npart is None else int(npart)
+        out = self.ctx.empty((self.n,) * 3 + (4,), np.float32)
+        self.ctx.check(self.ctx.lib.hm_lrecon_features(self.ctx.h, self.h, pos.ptr, n, out.ptr), "hm_lrecon_features")
+        return out
+
+    def net_forward(self, x):
+        dx = self.ctx.array(np.ascontiguousarray(x, dtype=np.float32))
+        out = self.ctx.empty((self.n,) * 3 + (4,), np.float32)
+        self.ctx.check(self.ctx.lib.hm_lrecon_net_forward(self.ctx.h, self.h, dx.ptr, out.ptr), "hm_lrecon_net_forward")
+        return out.download()
+
+    def net_backward(self, x, gout):
+        dx = self.ctx.array(np.ascontiguousarray(x, dtype=np.float32))
+        dg = self.ctx.array(np.ascontiguousarray(gout, dtype=np.float32))
+        grad = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx.lib.hm_lrecon_net_backward(self.ctx.h, self.h, dx.ptr, dg.ptr, grad.ctypes.data),
+                       "hm_lrecon_net_backward")
+        return grad
+
+    def predict(self, pos: DeviceArray, npart=None):
+        n = pos.shape[0] if npart is None else int(npart)
+        sp = self.ctx.empty((n, 3), np.float64)
+        sq = self.ctx.empty((n, 3), np.float64)
+        self.ctx.check(self.ctx.lib.hm_lrecon_predict(self.ctx.h, self.h, pos.ptr, n, sp.ptr, sq.ptr),
+                       "hm_lrecon_predict")
+        return sp, sq
+
+    def gradient(self, pos: DeviceArray, target: DeviceArray, npart=None, want_gout=False):
+        n = pos.shape[0] if npart is None else int(npart)
+        l0, l1 = ct.c_double(), ct.c_double()
+        grad = np.zeros(self.n_params, np.float32)
+        gout = self.ctx.empty((self.n,) * 3 + (4,), np.float32) if want_gout else None
+        self.ctx.check(self.ctx.lib.hm_lrecon_gradient(self.ctx.h, self.h, pos.ptr, target.ptr, n, ct.byref(l0),
+                                                       ct.byref(l1), grad.ctypes.data, gout.ptr if gout else None),
+                       "hm_lrecon_gradient")
+        return l0.value, l1.value, grad, gout
+
+    def train_step(self, pos: DeviceArray, target: DeviceArray, npart=None):
+        n = pos.shape[0] if npart is None else int(npart)
+        l0, l1 = ct.c_double(), ct.c_double()
+        self.ctx.check(self.ctx.lib.hm_lrecon_train_step(self.ctx.h, self.h, pos.ptr, target.ptr, n, ct.byref(l0),
+                                                         ct.byref(l1)), "hm_lrecon_train_step")
+        return l0.value, l1.value

@@ -80,7 +80,7 @@ __device__ inline double pk_model(const PkArgs& a, double k) {
 }
 
 // Half spectrum of the noise -> the three displacement spectra.
-__global__ __launch_bounds__(256) void k_zeldovich_spec(const C<double>* __restrict__ w, PkArgs a,
+__global__ __launch_bounds__(256) void k_zeldovich_spec(C<double>* __restrict__ w, PkArgs a, int keep_delta,
                                                         C<double>* __restrict__ px, C<double>* __restrict__ py,
                                                         C<double>* __restrict__ pz) {
 #pragma clang fp contract(off)
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_zeldovich_spec(const C<double>* __restr
         const int kz = (int)(e % nzh);
         const long long r = e / nzh;
         const int j = (int)(r % n), i = (int)(r / n);
-        C<double> zx = mk<double>(0.0, 0.0), zy = zx, zz = zx;
+        C<double> zx = mk<double>(0.0, 0.0), zy = zx, zz = zx, dd = zx;
         if (2 * i != n && 2 * j != n && 2 * kz != n && (i | j | kz) != 0) {
             const double kx = kf * (double)(i < n / 2 ? i : i - n);
             const double ky = kf * (double)(j < n / 2 ? j : j - n);
@@ -105,7 +105,9 @@ __global__ __launch_bounds__(256) void k_zeldovich_spec(const C<double>* __restr
             zx = mk<double>(kx * gr, kx * gi);
             zy = mk<double>(ky * gr, ky * gi);
             zz = mk<double>((kzv * a.fz) * gr, (kzv * a.fz) * gi);
+            dd = mk<double>(amp * d.x, amp * d.y);
         }
+        if (keep_delta) w[e] = dd;
         px[e] = zx;
         py[e] = zy;
         pz[e] = zz;
@@ -164,6 +166,130 @@ __global__ __launch_bounds__(256) void k_place(long long np, int n, double L, Pe
     }
 }
 
+// Lagrangian-biased selection: site s is kept when its uniform (Philox
+// counter (s, 1, 0, 0)) falls below p = rate (1 + b delta_L(s)), clipped to
+// [0, 1].  Kept sites are numbered in site order: per-block counts, one
+// exclusive scan of them, then each block writes its kept sites.
+constexpr int SEL_B = 1024;  // sites per block (256 threads x 4)
+
+// per-block sums of max(0, 1 + b delta_L) over SEL_B sites (fixed order)
+__global__ __launch_bounds__(256) void k_sel_wsum(long long total, const double* __restrict__ dl, double inv_n3, double b,
+                                                  double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double sm[256];
+    const long long base = (long long)blockIdx.x * SEL_B;
+    double acc = 0.0;
+    for (int j = 0; j < 4; j++) {
+        const long long s = base + 4 * threadIdx.x + j;
+        if (s < total) acc += fmax(0.0, 1.0 + b * (dl[s] * inv_n3));
+    }
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__device__ inline bool keep_site(uint64_t s, uint64_t seed, const double* dl, double inv_n3, double rate, double b) {
+#pragma clang fp contract(off)
+    uint32_t c[4] = {(uint32_t)s, (uint32_t)(s >> 32), 1u, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u = u53(c[0], c[1]);
+    const double p = rate * fmax(0.0, 1.0 + b * (dl[s] * inv_n3));
+    return u < p;
+}
+
+__global__ __launch_bounds__(256) void k_sel_count(long long total, unsigned long long seed, const double* __restrict__ dl,
+                                                   double inv_n3, double rate, double b, long long* __restrict__ cnt) {
+    __shared__ int sm[256];
+    const long long base = (long long)blockIdx.x * SEL_B;
+    int c = 0;
+    for (int j = 0; j < 4; j++) {
+        const long long s = base + 4 * threadIdx.x + j;
+        if (s < total && keep_site((uint64_t)s, seed, dl, inv_n3, rate, b)) c++;
+    }
+    sm[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[blockIdx.x] = sm[0];
+}
+
+// exclusive scan of nb block counts in one workgroup (fixed order); off[nb] = total
+__global__ __launch_bounds__(256) void k_sel_scan(const long long* __restrict__ cnt, long long nb, long long* __restrict__ off) {
+    __shared__ long long sm[256];
+    const long long per = (nb + 255) / 256;
+    const long long lo = (long long)threadIdx.x * per, hi = (lo + per < nb) ? lo + per : nb;
+    long long s = 0;
+    for (long long i = lo; i < hi; i++) s += cnt[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long run = 0;
+        for (int t = 0; t < 256; t++) {
+            const long long v = sm[t];
+            sm[t] = run;
+            run += v;
+        }
+        off[nb] = run;
+    }
+    __syncthreads();
+    long long run = sm[threadIdx.x];
+    for (long long i = lo; i < hi; i++) {
+        off[i] = run;
+        run += cnt[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sel_place(long long total, int n, double L, unsigned long long seed,
+                                                   const double* __restrict__ dl, double inv_n3, double rate, double b,
+                                                   const long long* __restrict__ off, long long capacity,
+                                                   const double* __restrict__ sx, const double* __restrict__ sy,
+                                                   const double* __restrict__ sz, double* __restrict__ pos,
+                                                   double* __restrict__ shift) {
+#pragma clang fp contract(off)
+    __shared__ int sm[256];
+    const long long base = (long long)blockIdx.x * SEL_B;
+    int f[4], c = 0;
+    for (int j = 0; j < 4; j++) {
+        const long long s = base + 4 * threadIdx.x + j;
+        f[j] = (s < total && keep_site((uint64_t)s, seed, dl, inv_n3, rate, b)) ? 1 : 0;
+        c += f[j];
+    }
+    // exclusive prefix of c over the workgroup (Hillis-Steele in LDS)
+    sm[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 1; w < 256; w <<= 1) {
+        const int v = (int)threadIdx.x >= w ? sm[threadIdx.x - w] : 0;
+        __syncthreads();
+        sm[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long idx = off[blockIdx.x] + sm[threadIdx.x] - c;
+    const double cell = L / (double)n;
+    for (int j = 0; j < 4; j++) {
+        if (!f[j]) continue;
+        const long long s = base + 4 * threadIdx.x + j;
+        if (idx < capacity) {
+            const int iz = (int)(s % n), iy = (int)((s / n) % n), ix = (int)(s / ((long long)n * n));
+            const double q[3] = {((double)ix + 0.5) * cell, ((double)iy + 0.5) * cell, ((double)iz + 0.5) * cell};
+            const double d[3] = {sx[s] * inv_n3, sy[s] * inv_n3, sz[s] * inv_n3};
+            for (int a = 0; a < 3; a++) {
+                const double x = q[a] + d[a];
+                double v = x - L * floor(x / L);
+                if (v >= L) v -= L;
+                pos[3 * idx + a] = v;
+                if (shift) shift[3 * idx + a] = -d[a];
+            }
+        }
+        idx++;
+    }
+}
+
 uint64_t splitmix_next(uint64_t& s) {
     s += 0x9E3779B97F4A7C15ull;
     uint64_t z = s;
@@ -203,14 +329,15 @@ int hm_mock_white_noise(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long lo
     return 0;
 }
 
-int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long seed, long long np, double* d_pos) {
-    if (!ctx) return -1;
-    if (check_cfg(ctx, cfg)) return -1;
+}  // extern "C"
+
+namespace {
+
+// Noise -> spectrum -> the three displacement grids (x into g[0], y into g[1],
+// z into g[2], each n^3 doubles, unnormalised) and, with delta, the linear
+// density grid into g[3].  Workspace: noise | spectrum | 3 spectra | 1 grid.
+int displacement_grids(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long seed, bool delta, double* g[4]) {
     const long long n = cfg->n_side, total = n * n * n, half = n * n * (n / 2 + 1);
-    if (np <= 0 || np > total) return bad(ctx, "mock: need 0 < n <= n_side^3 particles");
-    if (!d_pos) return bad(ctx, "mock: no output");
-    // workspace (kept on the context): noise / x displacement, its spectrum / y displacement, three
-    // displacement spectra, z displacement
     const size_t need = sizeof(double) * (size_t)total * 2 + sizeof(C<double>) * (size_t)half * 4;
     if (ctx->mock_ws_bytes < need) {
         HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -226,15 +353,13 @@ int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long
     char* ws = (char*)ctx->mock_ws;
     double* noise = (double*)ws;
     C<double>* spec = (C<double>*)(ws + sizeof(double) * (size_t)total);
-    C<double>* s3 = spec + half;
-    double* extra = (double*)(s3 + 3 * half);
-    int rc = 0;
-    const int dims[3] = {(int)n, (int)n, (int)n};
-    C<double>* px = s3;
+    C<double>* px = spec + half;
     C<double>* py = px + half;
     C<double>* pz = py + half;
-    if (!rc) rc = launch_noise(ctx, total, seed, noise);
-    if (!rc) rc = fft3d_r2c<double>(ctx, noise, spec, dims);
+    double* extra = (double*)(pz + half);
+    const int dims[3] = {(int)n, (int)n, (int)n};
+    if (launch_noise(ctx, total, seed, noise)) return -1;
+    if (fft3d_r2c<double>(ctx, noise, spec, dims)) return -1;
     PkArgs a;
     a.n = (int)n;
     a.L = cfg->boxsize;
@@ -247,20 +372,36 @@ int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long
     a.fz = 1.0 + cfg->f_growth;
     const double cell = cfg->boxsize / (double)n;
     a.inv_vcell = 1.0 / ((cell * cell) * cell);
-    if (!rc)
-        rc = ctx->launch("mock_spec", [&] {
+    if (ctx->launch("mock_spec", [&] {
             const long long blocks = std::min((half + 255) / 256, 65536LL);
-            hipLaunchKernelGGL(k_zeldovich_spec, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                               (const C<double>*)spec, a, px, py, pz);
-        });
-    // real displacements: x into the noise buffer, y into the noise spectrum's buffer (n^2 (n + 2) doubles
-    // hold one real grid), z into a third grid
-    double* dx = noise;
-    double* dy = (double*)spec;
-    double* dz = extra;
-    if (!rc) rc = fft3d_c2r<double>(ctx, px, dx, dims);
-    if (!rc) rc = fft3d_c2r<double>(ctx, py, dy, dims);
-    if (!rc) rc = fft3d_c2r<double>(ctx, pz, dz, dims);
+            hipLaunchKernelGGL(k_zeldovich_spec, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, spec, a,
+                               delta ? 1 : 0, px, py, pz);
+        }))
+        return -1;
+    // a half spectrum (n^2 (n + 2) doubles) holds one real grid once transformed
+    g[0] = noise;
+    g[1] = (double*)px;
+    g[2] = extra;
+    g[3] = delta ? (double*)py : nullptr;
+    if (fft3d_c2r<double>(ctx, px, g[0], dims)) return -1;
+    if (fft3d_c2r<double>(ctx, py, g[1], dims)) return -1;
+    if (fft3d_c2r<double>(ctx, pz, g[2], dims)) return -1;
+    if (delta && fft3d_c2r<double>(ctx, spec, g[3], dims)) return -1;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long seed, long long np, double* d_pos) {
+    if (!ctx) return -1;
+    if (check_cfg(ctx, cfg)) return -1;
+    const long long n = cfg->n_side, total = n * n * n;
+    if (np <= 0 || np > total) return bad(ctx, "mock: need 0 < n <= n_side^3 particles");
+    if (!d_pos) return bad(ctx, "mock: no output");
+    double* g[4];
+    if (displacement_grids(ctx, cfg, seed, false, g)) return -1;
     Perm pm;
     pm.sites = (uint64_t)total;
     int bits = 0;
@@ -269,13 +410,66 @@ int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long
     uint64_t sm = (uint64_t)seed ^ 0x5DEECE66Dull;
     for (int s = 0; s < 4; s++) pm.key[s] = splitmix_next(sm);
     const double inv_n3 = 1.0 / (double)total;
+    return ctx->launch("mock_place", [&] {
+        const long long blocks = std::min((np + 255) / 256, 65536LL);
+        hipLaunchKernelGGL(k_place, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, np, (int)n, cfg->boxsize, pm,
+                           g[0], g[1], g[2], inv_n3, d_pos);
+    });
+}
+
+int hm_mock_zeldovich_biased(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long seed, double mean_n,
+                             double b_lag, long long capacity, double* d_pos, double* d_shift, long long* n_out) {
+    if (!ctx) return -1;
+    if (check_cfg(ctx, cfg)) return -1;
+    const long long n = cfg->n_side, total = n * n * n;
+    if (!(mean_n > 0.0) || mean_n > (double)total) return bad(ctx, "mock: need 0 < mean_n <= n_side^3");
+    if (!d_pos || !n_out || capacity <= 0) return bad(ctx, "mock: no output");
+    double* g[4];
+    if (displacement_grids(ctx, cfg, seed, true, g)) return -1;
+    const long long nb = (total + SEL_B - 1) / SEL_B;
+    DevBuf cnt;
+    if (dev_alloc(ctx, cnt, sizeof(long long) * (size_t)(2 * nb + 1))) return -1;
+    long long* c = (long long*)cnt.p;
+    long long* off = c + nb;
+    const double inv_n3 = 1.0 / (double)total;
+    // rate = mean_n / sum_s max(0, 1 + b delta_L(s)): the expected count is mean_n
+    std::vector<double> ws((size_t)nb);
+    int rc = ctx->launch("mock_sel_wsum", [&] {
+        hipLaunchKernelGGL(k_sel_wsum, dim3((unsigned)nb), dim3(256), 0, ctx->stream, total, (const double*)g[3], inv_n3,
+                           b_lag, (double*)c);
+    });
+    if (!rc && (hipMemcpyAsync(ws.data(), c, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, ctx->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess))
+        rc = bad(ctx, "mock: stream error");
+    double wsum = 0.0;
+    for (double v : ws) wsum += v;
+    if (!rc && !(wsum > 0.0)) rc = bad(ctx, "mock: the bias leaves no site with positive weight");
+    const double rate = rc ? 0.0 : mean_n / wsum;
     if (!rc)
-        rc = ctx->launch("mock_place", [&] {
-            const long long blocks = std::min((np + 255) / 256, 65536LL);
-            hipLaunchKernelGGL(k_place, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, np, (int)n, cfg->boxsize, pm,
-                               dx, dy, dz, inv_n3, d_pos);
+        rc = ctx->launch("mock_sel_count", [&] {
+            hipLaunchKernelGGL(k_sel_count, dim3((unsigned)nb), dim3(256), 0, ctx->stream, total, seed,
+                               (const double*)g[3], inv_n3, rate, b_lag, c);
         });
-    return rc;
+    if (!rc)
+        rc = ctx->launch("mock_sel_scan", [&] {
+            hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(256), 0, ctx->stream, (const long long*)c, nb, off);
+        });
+    if (!rc)
+        rc = ctx->launch("mock_sel_place", [&] {
+            hipLaunchKernelGGL(k_sel_place, dim3((unsigned)nb), dim3(256), 0, ctx->stream, total, (int)n, cfg->boxsize,
+                               seed, (const double*)g[3], inv_n3, rate, b_lag, (const long long*)off, capacity, g[0],
+                               g[1], g[2], d_pos, d_shift);
+        });
+    long long count = 0;
+    if (!rc && (hipMemcpyAsync(&count, off + nb, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess))
+        rc = bad(ctx, "mock: stream error");
+    dev_free(cnt);
+    if (rc) return rc;
+    *n_out = count;
+    if (count > capacity) return bad(ctx, "mock: more particles selected than capacity (" + std::to_string(count) + ")");
+    return 0;
 }
 
 }  // extern "C"

@@ -161,11 +161,13 @@ size_t recon_work_bytes(const ReconWork& w) {
            w.sort.bytes();
 }
 
+// Psi_a (complex, 3 grids in w.psi) of the field D - alpha R with weights;
+// returns the fixed-point scale in *scale_out (w.fre keeps the fixed-point mesh).
 template <typename T>
-int recon_displace(hm_ctx* ctx, const Mesh& m, double b1, double RG, double* pD, const double* wD, long long nD,
-                   double* pR, const double* wR, long long nR, double alpha, double norm, ReconWork& w) {
+static int recon_psi_stage(hm_ctx* ctx, const Mesh& m, double b1, double RG, const double* pD, const double* wD,
+                           long long nD, const double* pR, const double* wR, long long nR, double alpha, ReconWork& w,
+                           double* scale_out) {
     const long long N = nD + nR, ng = m.ng();
-    if (N <= 0) return 0;
     if (!(b1 != 0.0)) {
         ctx->set_error("reconstruction: b1_fid must be non-zero");
         return -1;
@@ -214,6 +216,7 @@ int recon_displace(hm_ctx* ctx, const Mesh& m, double b1, double RG, double* pD,
     ag.assign = m.assign;
     if (sort_work_alloc(ctx, w.sort, ag, N, true)) return -1;
     const double scale = fixed_scale_for(1.01 * wabs + 1.0);
+    *scale_out = scale;
     long long* fre = (long long*)w.fre.p;
     if (assign_sorted(ctx, ag, pos, wts, N, scale, fre, nullptr, w.sort)) return -1;
     C<T>* grid = (C<T>*)w.grid.p;
@@ -241,7 +244,26 @@ int recon_displace(hm_ctx* ctx, const Mesh& m, double b1, double RG, double* pD,
         return -1;
     for (int a = 0; a < 3; a++)
         if (fft3d_c2c<T>(ctx, psi + (long long)a * ng, n3, +1)) return -1;
-    AssignGeom tsc = ag;
+    return 0;
+}
+
+template <typename T>
+int recon_displace(hm_ctx* ctx, const Mesh& m, double b1, double RG, double* pD, const double* wD, long long nD,
+                   double* pR, const double* wR, long long nR, double alpha, double norm, ReconWork& w) {
+    const long long N = nD + nR;
+    if (N <= 0) return 0;
+    double scale = 0.0;
+    if (recon_psi_stage<T>(ctx, m, b1, RG, pD, wD, nD, pR, wR, nR, alpha, w, &scale)) return -1;
+    AssignGeom tsc;
+    for (int a = 0; a < 3; a++) {
+        tsc.n[a] = m.n[a];
+        tsc.L[a] = m.L[a];
+    }
+    tsc.ng = m.ng();
+    tsc.assign = m.assign;
+    double* pos = (double*)w.pos.p;
+    const unsigned pb = (unsigned)((N + 255) / 256);
+    C<T>* psi = (C<T>*)w.psi.p;
     if (ctx->launch("recon_gather", [&] {
             hipLaunchKernelGGL(k_recon_gather<T>, dim3(pb), dim3(256), 0, ctx->stream, psi, tsc, norm, pos, N);
         }))
@@ -251,6 +273,41 @@ int recon_displace(hm_ctx* ctx, const Mesh& m, double b1, double RG, double* pD,
     });
 }
 
+// features [ng][4] FP32: norm * Re Psi_a (a = 0, 1, 2) and rho / rho_bar - 1
+template <typename T>
+__global__ __launch_bounds__(256) void k_recon_features(const C<T>* __restrict__ psi, const long long* __restrict__ fre,
+                                                       long long ng, double norm, double inv_scale, double inv_mean,
+                                                       float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long st = (long long)gridDim.x * 256;
+    for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < ng; v += st) {
+        for (int a = 0; a < 3; a++) out[4 * v + a] = (float)((double)psi[a * ng + v].x * norm);
+        out[4 * v + 3] = (float)(((double)fre[v] * inv_scale) * inv_mean - 1.0);
+    }
+}
+
+template <typename T>
+int recon_features(hm_ctx* ctx, const Mesh& m, double b1, double RG, const double* pD, long long nD, double norm,
+                   float* out, ReconWork& w) {
+    if (nD <= 0) {
+        ctx->set_error("reconstruction features: no particles");
+        return -1;
+    }
+    double scale = 0.0;
+    if (recon_psi_stage<T>(ctx, m, b1, RG, pD, nullptr, nD, nullptr, nullptr, 0, 0.0, w, &scale)) return -1;
+    const long long ng = m.ng();
+    const unsigned gb = (unsigned)std::max<long long>(1, std::min<long long>((ng + 255) / 256, 16384));
+    const double inv_mean = (double)ng / (double)nD;  // 1 / (mean particles per cell)
+    return ctx->launch("recon_features", [&] {
+        hipLaunchKernelGGL(k_recon_features<T>, dim3(gb), dim3(256), 0, ctx->stream, (const C<T>*)w.psi.p,
+                           (const long long*)w.fre.p, ng, norm, 1.0 / scale, inv_mean, out);
+    });
+}
+
+template int recon_features<float>(hm_ctx*, const Mesh&, double, double, const double*, long long, double, float*,
+                                   ReconWork&);
+template int recon_features<double>(hm_ctx*, const Mesh&, double, double, const double*, long long, double, float*,
+                                    ReconWork&);
 template int recon_displace<float>(hm_ctx*, const Mesh&, double, double, double*, const double*, long long, double*,
                                    const double*, long long, double, double, ReconWork&);
 template int recon_displace<double>(hm_ctx*, const Mesh&, double, double, double*, const double*, long long, double*,

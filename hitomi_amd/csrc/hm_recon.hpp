@@ -19,6 +19,13 @@ template <typename T>
 int recon_displace(hm_ctx* ctx, const Mesh& m, double b1, double RG, double* pD, const double* wD, long long nD,
                    double* pR, const double* wR, long long nR, double alpha, double norm, ReconWork& w);
 
+// The plain reconstruction's field on mesh m for a catalogue with unit
+// weights: out [ng][4] FP32 = norm * Re Psi_a (a = 0, 1, 2) and the assigned
+// density contrast rho / rho_bar - 1.
+template <typename T>
+int recon_features(hm_ctx* ctx, const Mesh& m, double b1, double RG, const double* pD, long long nD, double norm,
+                   float* out, ReconWork& w);
+
 // D then R positions in one array with weights w (1 when NULL) for D and
 // -alpha w for R (complex, imaginary part 0).
 int pack_two(hm_ctx* ctx, const double* pD, const double* wD, long long nD, const double* pR, const double* wR,

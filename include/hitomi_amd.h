@@ -319,6 +319,67 @@ int hm_mock_white_noise(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long lo
  * context's stream; the context keeps a workspace of 2 n_side^3 doubles and
  * four half spectra (0.8 GB at n_side = 256) until it is destroyed. */
 int hm_mock_zeldovich(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long seed, long long n, double *d_pos);
+/* Biased tracers: site s is kept when its uniform (Philox counter (s, 1, 0, 0),
+ * a 53-bit fraction) is below rate max(0, 1 + b_lag delta_L(s)), delta_L the
+ * linear field on the lattice and rate = mean_n / sum_s max(0, 1 + b_lag
+ * delta_L(s)) (so mean_n is the expected count while rate (1 + b_lag delta_L)
+ * stays below 1); kept sites in site order, moved
+ * and wrapped as above.  d_shift (may be NULL) [n][3] receives the shift that
+ * returns each particle to its site, -Psi (with the z factor), unwrapped.
+ * *n_out = the number kept; more than capacity is an error (the first
+ * capacity are written).  Synchronises the context's stream. */
+int hm_mock_zeldovich_biased(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long seed, double mean_n,
+                             double b_lag, long long capacity, double *d_pos, double *d_shift, long long *n_out);
+
+/* ---------------------------------------------------------------- */
+/* Learned reconstruction.  No counterpart in the reference, whose
+ * reconstruction is the plain one (particle_reconstruction.hpp:210-402).  A
+ * residual network corrects the plain reconstruction's shift field: features
+ * [v][4] = (norm Psi_a / psi_scale, a = 0..2; (rho / rho_bar - 1) / 8) of the
+ * plain reconstruction (Psi as hm_recon_displace forms it, b1 and RG, norm =
+ * V / N, no randoms) on an n_mesh^3 TSC mesh; four 3^3 convolutions with
+ * periodic padding (4 -> 16 -> 16 -> 16 -> 3 channels, ReLU after the first
+ * three), BF16 operands and FP32 accumulation on the matrix cores; shift
+ * S_i = norm Psi(x_i) + psi_scale out(x_i), TSC-gathered with periodic wrap;
+ * loss mean_i |S_i - T_i|^2.  Parameters, FP32, in the order W1 b1 W2 b2 W3
+ * b3 W4 b4, W_l [27][cin][cout] with tap t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1)
+ * (input voxel v + (dx, dy, dz)); the last layer starts at zero, so a new
+ * model predicts the plain shifts. */
+typedef struct hm_lrecon hm_lrecon;
+typedef struct {
+    int n_mesh;                   /* per axis: a multiple of 16 and an FFT length of this library */
+    double boxsize;
+    double b1, RG;                /* the plain reconstruction's bias and Gaussian smoothing (Mpc/h) */
+    double psi_scale;             /* unit of the features and the output (Mpc/h) */
+    double lr;                    /* Adam step size */
+    unsigned long long init_seed; /* He-normal initialisation of the first three layers */
+    int precision;                /* of the plain reconstruction's transforms: HM_FP32 or HM_FP64 */
+} hm_lrecon_config;
+
+hm_lrecon *hm_lrecon_create(hm_ctx *ctx, const hm_lrecon_config *cfg);
+void hm_lrecon_destroy(hm_ctx *ctx, hm_lrecon *model);
+long long hm_lrecon_n_params(const hm_lrecon *model);
+int hm_lrecon_get_params(hm_ctx *ctx, hm_lrecon *model, float *out);
+/* Also resets the Adam moments. */
+int hm_lrecon_set_params(hm_ctx *ctx, hm_lrecon *model, const float *in);
+/* The features of a catalogue (d_pos n x 3, in [0, L)), d_feat [n_mesh^3][4]. */
+int hm_lrecon_features(hm_ctx *ctx, hm_lrecon *model, const double *d_pos, long long n, float *d_feat);
+/* The network alone: d_x [n_mesh^3][4] network input (FP32, rounded to BF16),
+ * d_out [n_mesh^3][4] (channels 0..2).  Backward: d_gout = d loss / d out,
+ * grad (host) [n_params]. */
+int hm_lrecon_net_forward(hm_ctx *ctx, hm_lrecon *model, const float *d_x, float *d_out);
+int hm_lrecon_net_backward(hm_ctx *ctx, hm_lrecon *model, const float *d_x, const float *d_gout, float *grad);
+/* Per-particle shifts [n][3]: plain and corrected (either may be NULL). */
+int hm_lrecon_predict(hm_ctx *ctx, hm_lrecon *model, const double *d_pos, long long n, double *d_shift_plain,
+                      double *d_shift_pred);
+/* Losses of the plain and corrected shifts against d_target [n][3] and the
+ * parameter gradient of the corrected one (grad host, may be NULL; d_gout
+ * device [n_mesh^3][4], may be NULL: d loss / d out). */
+int hm_lrecon_gradient(hm_ctx *ctx, hm_lrecon *model, const double *d_pos, const double *d_target, long long n,
+                       double *loss_plain, double *loss_pred, float *grad, float *d_gout);
+/* hm_lrecon_gradient, then one Adam step. */
+int hm_lrecon_train_step(hm_ctx *ctx, hm_lrecon *model, const double *d_pos, const double *d_target, long long n,
+                         double *loss_plain, double *loss_pred);
 
 #ifdef __cplusplus
 }

@@ -2251,3 +2251,16 @@ void or_mock_sites(long long np, long long sites, unsigned long long seed, long 
         out[i] = (long long)y;
     }
 }
+
+/* Uniforms of the biased-tracer selection: counter (s lo, s hi, 1, 0), key
+ * (seed lo, seed hi), (w0 >> 5, w1 >> 6) as a 53-bit fraction. */
+void or_mock_select_uniform(long long n, unsigned long long seed, double *out) {
+    const unsigned int key[2] = {(unsigned int)(seed & 0xFFFFFFFFu), (unsigned int)(seed >> 32)};
+    for (long long i = 0; i < n; i++) {
+        const unsigned long long u = (unsigned long long)i;
+        const unsigned int ctr[4] = {(unsigned int)(u & 0xFFFFFFFFu), (unsigned int)(u >> 32), 1u, 0u};
+        unsigned int w[4];
+        or_philox4x32_10(ctr, key, w);
+        out[i] = (double)(((uint64_t)(w[0] >> 5) << 26) | (uint64_t)(w[1] >> 6)) / 9007199254740992.0;
+    }
+}

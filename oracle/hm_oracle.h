@@ -171,5 +171,6 @@ int or_box_interlaced_2pt(const or_param *p, const double *pos, long long np, co
 void or_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]);
 void or_mock_white_noise(long long n, unsigned long long seed, double *out);
 void or_mock_sites(long long np, long long sites, unsigned long long seed, long long *out);
+void or_mock_select_uniform(long long n, unsigned long long seed, double *out);
 
 #endif

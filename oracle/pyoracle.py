@@ -96,6 +96,7 @@ def lib():
             "or_philox4x32_10": (None, [ct.POINTER(ct.c_uint), ct.POINTER(ct.c_uint), ct.POINTER(ct.c_uint)]),
             "or_mock_white_noise": (None, [ct.c_longlong, ct.c_ulonglong, _D]),
             "or_mock_sites": (None, [ct.c_longlong, ct.c_longlong, ct.c_ulonglong, ct.POINTER(ct.c_longlong)]),
+            "or_mock_select_uniform": (None, [ct.c_longlong, ct.c_ulonglong, _D]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -228,6 +229,13 @@ def box_density_fluctuation(p: Param, pos):
 def fourier_transform(p: Param, field):
     a, v = _c(np.array(field, dtype=np.complex128, copy=True))
     lib().or_fourier_transform(ct.byref(p), dp(v))
+    return a
+
+
+def assignment_correction(p: Param, fk):
+    """field.hpp:610-639: the transformed field divided by the assignment window."""
+    a, v = _c(np.array(fk, dtype=np.complex128, copy=True))
+    lib().or_assignment_correction(ct.byref(p), dp(v))
     return a
 
 
@@ -559,3 +567,133 @@ def mock_zeldovich(n, L, seed, np_, f_growth=0.0, **pk_kw):
         v[v >= L] -= L
         pos[:, a] = v
     return pos
+
+
+def mock_zeldovich_biased(n, L, seed, mean_n, b_lag, f_growth=0.0, **pk_kw):
+    """Biased tracers: site s kept when u_s < rate max(0, 1 + b_lag delta_L(s)),
+    rate = mean_n / sum of those weights;
+    returns (positions, shifts -Psi) of the kept sites in site order."""
+    dk = mock_delta_k(n, L, seed, pk_kw)
+    kx, ky, kz, nyq = mock_kgrid(n, L)
+    dk0 = np.where(nyq, 0.0, dk)
+    dk0[0, 0, 0] = 0.0
+    dl = np.fft.irfftn(dk0, s=(n,) * 3, axes=(0, 1, 2)).ravel()
+    u = np.empty(n ** 3)
+    lib().or_mock_select_uniform(u.size, int(seed), dp(u))
+    wgt = np.maximum(0.0, 1.0 + b_lag * dl)
+    keep = np.nonzero(u < (mean_n / wgt.sum()) * wgt)[0]
+    psi = zeldovich_displacement(dk, n, L, f_growth)
+    ix, iy, iz = keep // (n * n), (keep // n) % n, keep % n
+    cell = L / n
+    pos = np.empty((keep.size, 3))
+    shift = np.empty((keep.size, 3))
+    for a, (idx, ps) in enumerate(zip((ix, iy, iz), psi)):
+        d = ps.ravel()[keep]
+        x = (idx + 0.5) * cell + d
+        v = x - L * np.floor(x / L)
+        v[v >= L] -= L
+        pos[:, a] = v
+        shift[:, a] = -d
+    return pos, shift, keep
+
+
+# ---------------------------------------------------------------------------
+# Learned reconstruction network (no counterpart in the reference): periodic
+# 3^3 convolutions, 4 -> 16 -> 16 -> 16 -> 3 channels, ReLU after the first
+# three.  Tap t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1) reads voxel v + (dx, dy, dz).
+# With bf16=True every operand the device rounds to BF16 (input, weights,
+# stored activations, the incoming gradient and the stored layer gradients)
+# is rounded the same way (round to nearest even) and every stored FP32
+# accumulator is rounded to FP32 first; sums are FP64.
+# ---------------------------------------------------------------------------
+LRECON_CIN = (4, 16, 16, 16)
+LRECON_COUT = (16, 16, 16, 3)
+
+
+def bf16_round(x):
+    """Round to the nearest BF16 (ties to even), as float32 values."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def lrecon_unpack(params):
+    """Flat parameters -> [(W [27][cin][cout], b [cout])] * 4."""
+    out, o = [], 0
+    for ci, co in zip(LRECON_CIN, LRECON_COUT):
+        W = np.asarray(params[o:o + 27 * ci * co], dtype=np.float64).reshape(27, ci, co)
+        o += 27 * ci * co
+        b = np.asarray(params[o:o + co], dtype=np.float64)
+        o += co
+        out.append((W, b))
+    return out
+
+
+def lrecon_pack(layers):
+    return np.concatenate([np.concatenate([W.ravel(), b.ravel()]) for W, b in layers])
+
+
+def _taps():
+    return [(t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1) for t in range(27)]
+
+
+def conv3_periodic(x, W, b=None):
+    """y[v][co] = b[co] + sum_t sum_ci W[t][ci][co] x[v + off_t][ci] (FP64)."""
+    y = np.zeros(x.shape[:3] + (W.shape[2],))
+    for t, (dx, dy, dz) in enumerate(_taps()):
+        y += np.roll(x, shift=(-dx, -dy, -dz), axis=(0, 1, 2)) @ W[t]
+    return y if b is None else y + b
+
+
+def conv3_input_grad(dy, W):
+    """d/dx of sum(conv3_periodic(x, W) * dy): sum_t roll(dy @ W[t]^T, +off_t)."""
+    dx = np.zeros(dy.shape[:3] + (W.shape[1],))
+    for t, (ox, oy, oz) in enumerate(_taps()):
+        dx += np.roll(dy @ W[t].T, shift=(ox, oy, oz), axis=(0, 1, 2))
+    return dx
+
+
+def conv3_weight_grad(x, dy):
+    """d/dW[t][ci][co] = sum_v x[v + off_t][ci] dy[v][co]; d/db = sum_v dy."""
+    ci, co = x.shape[3], dy.shape[3]
+    dW = np.zeros((27, ci, co))
+    X = dy.reshape(-1, co)
+    for t, (dx_, dy_, dz_) in enumerate(_taps()):
+        dW[t] = np.roll(x, shift=(-dx_, -dy_, -dz_), axis=(0, 1, 2)).reshape(-1, ci).T @ X
+    return dW, X.sum(0)
+
+
+def lrecon_net(x, params, bf16=True):
+    """Forward pass: x [n,n,n,4] -> (out [n,n,n,3], inputs of each layer)."""
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    f32 = (lambda a: np.asarray(a, dtype=np.float32)) if bf16 else (lambda a: a)
+    layers = lrecon_unpack(params)
+    a = r(x).astype(np.float64)
+    ins = []
+    for l, (W, b) in enumerate(layers):
+        ins.append(a)
+        z = conv3_periodic(a, r(W).astype(np.float64), b)
+        if l < 3:
+            a = r(f32(np.maximum(z, 0.0))).astype(np.float64)
+        else:
+            a = f32(z).astype(np.float64)
+    return a, ins
+
+
+def lrecon_net_grad(x, params, gout, bf16=True):
+    """Parameter gradient of sum(out * gout) (gout [n,n,n,3])."""
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    f32 = (lambda a: np.asarray(a, dtype=np.float32)) if bf16 else (lambda a: a)
+    layers = lrecon_unpack(params)
+    _, ins = lrecon_net(x, params, bf16)
+    dz = r(gout).astype(np.float64)
+    grads = [None] * 4
+    for l in range(3, -1, -1):
+        W = r(layers[l][0]).astype(np.float64)
+        dW, db = conv3_weight_grad(ins[l], dz)
+        grads[l] = (dW, db)
+        if l == 0:
+            break
+        da = conv3_input_grad(dz, W)
+        dz = r(f32(np.where(ins[l] > 0.0, da, 0.0))).astype(np.float64)
+    return lrecon_pack(grads)
