@@ -141,6 +141,8 @@ def covariance_run(ctx, n_mocks, seed0=5000):
     for graph in (True, False):
         b = hm.Batch(ctx, COV_MESH, COV_BOX, max_mocks=0, use_graph=graph, precision=hm.FP32, **KW)
         hm.mock_zeldovich(ctx, cfg, seed0, int(sizes[0]), out=buf)
+        out[f"{'graph' if graph else 'eager'}_autotune"] = b.autotune(buf, reps=3, npart=int(sizes[0])).split(
+            "chosen: ")[-1]
         b.push(buf, int(sizes[0]))  # warm-up (and, with the graph, the capture on the next push)
         hm.mock_zeldovich(ctx, cfg, seed0 + 1, int(sizes[1]), out=buf)
         b.push(buf, int(sizes[1]))
@@ -256,8 +258,10 @@ def run(args):
     plan.close()
 
     # timed region: the batched pipeline (eager launches, so that HIP events can
-    # bracket the dominant kernel's launches)
+    # bracket the dominant kernel's launches), its launch shapes autotuned first
     batch = hm.Batch(ctx, N_MESH, BOX, max_mocks=1, use_graph=False, precision=hm.FP32, **KW)
+    tune = batch.autotune(dpos[0], reps=2) if args.autotune else "off"
+    log(f"autotune: {tune}")
     for i in range(args.warmup):
         batch.push(dpos[i % len(dpos)])
     batch.reset()
@@ -282,6 +286,8 @@ def run(args):
 
     # the same steps replayed from the captured graph
     gbatch = hm.Batch(ctx, N_MESH, BOX, max_mocks=1, use_graph=True, precision=hm.FP32, **KW)
+    if args.autotune:
+        gbatch.autotune(dpos[0], reps=2)
     for i in range(max(args.warmup, 2)):
         gbatch.push(dpos[i % len(dpos)])
     ctx.sync()
@@ -331,6 +337,7 @@ def run(args):
             "pipeline": "batched (device-side data vectors, no host synchronisation between mocks)",
         },
         "batched_vector_equals_per_mock": same,
+        "autotune": tune.split("chosen: ")[-1] if "chosen" in tune else tune,
         "graph_replay_ms_per_step": round(ms_graph, 3),
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -357,6 +364,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain FP64 path (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cov-mocks", type=int, default=512, help="mocks of the covariance run (0: skip)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false", help="keep the default launch shapes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device copy of BYTES (scales HBM counters under rocprofv3 --pmc)")
     run(ap.parse_args())

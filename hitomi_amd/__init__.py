@@ -170,6 +170,9 @@ _SIGS = {
                                              ct.POINTER(BoxResult)]),
     "hm_box_plan_bytes": (ct.c_size_t, [_VP]),
     "hm_box_plan_triple_mesh": (ct.c_int, [_VP, ct.POINTER(ct.c_int)]),
+    "hm_box_plan_autotune": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.c_int, ct.c_char_p, ct.c_size_t]),
+    "hm_box_plan_get_shapes": (ct.c_int, [_VP, ct.POINTER(ct.c_uint), ct.POINTER(ct.c_int)]),
+    "hm_box_plan_set_shapes": (ct.c_int, [_VP, ct.c_uint, ct.c_int]),
     "hm_recon_displace": (ct.c_int, [_VP, ct.POINTER(ReconConfig), _VP, _VP, ct.c_longlong, _VP, _VP, ct.c_longlong,
                                      ct.c_double, ct.c_double]),
     "hm_sph_bessel_jl": (ct.c_double, [ct.c_int, ct.c_double]),
@@ -187,6 +190,7 @@ _SIGS = {
     "hm_batch_vectors": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong]),
     "hm_batch_covariance": (ct.c_int, [_VP, _VP, _VP, _VP]),
     "hm_batch_reset": (ct.c_int, [_VP, _VP]),
+    "hm_batch_autotune": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.c_int, ct.c_char_p, ct.c_size_t]),
     "hm_mock_white_noise": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, _VP]),
     "hm_mock_zeldovich": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, ct.c_longlong, _VP]),
     "hm_mock_zeldovich_biased": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, ct.c_double, ct.c_double,
@@ -480,6 +484,24 @@ class BoxPlan:
         self.ctx.lib.hm_box_plan_triple_mesh(self.h, m)
         return tuple(m)
 
+    def autotune(self, pos: "DeviceArray", reps=2, npart=None):
+        """Choose the launch shapes by timing (results do not depend on them); returns the report."""
+        n = pos.shape[0] if npart is None else int(npart)
+        buf = ct.create_string_buffer(4096)
+        self.ctx.check(self.ctx.lib.hm_box_plan_autotune(self.ctx.h, self.h, pos.ptr, n, int(reps), buf, 4096),
+                       "hm_box_plan_autotune")
+        return buf.value.decode()
+
+    @property
+    def shapes(self):
+        w, b = ct.c_uint(), ct.c_int()
+        self.ctx.lib.hm_box_plan_get_shapes(self.h, ct.byref(w), ct.byref(b))
+        return w.value, b.value
+
+    @shapes.setter
+    def shapes(self, wb):
+        self.ctx.lib.hm_box_plan_set_shapes(self.h, int(wb[0]), int(wb[1]))
+
     def execute(self, pos, npart=None, recon=None):
         """pos: host (N,3) float64 array or a DeviceArray of it."""
         if isinstance(pos, DeviceArray):
@@ -692,6 +714,13 @@ class Batch:
 
     def reset(self):
         self.ctx.check(self.ctx.lib.hm_batch_reset(self.ctx.h, self.h), "hm_batch_reset")
+
+    def autotune(self, pos: DeviceArray, reps=2, npart=None):
+        n = pos.shape[0] if npart is None else int(npart)
+        buf = ct.create_string_buffer(4096)
+        self.ctx.check(self.ctx.lib.hm_batch_autotune(self.ctx.h, self.h, pos.ptr, n, int(reps), buf, 4096),
+                       "hm_batch_autotune")
+        return buf.value.decode()
 
 
 def mock_config(n_side, boxsize, amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5, bao_amp=0.0, bao_period=0.0628,

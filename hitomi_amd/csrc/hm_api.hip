@@ -74,6 +74,9 @@ hm_ctx* hm_ctx_create(int device, int flags) {
     hm_ctx* ctx = new (std::nothrow) hm_ctx();
     if (!ctx) return nullptr;
     ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
+        ctx->lds_max = (size_t)prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return nullptr;
@@ -333,6 +336,32 @@ int hm_box_plan_execute_recon(hm_ctx* ctx, hm_box_plan* plan, const double* d_po
 }
 
 size_t hm_box_plan_bytes(const hm_box_plan* plan) { return plan ? hm::plan_bytes(plan->impl) : 0; }
+
+int hm_box_plan_autotune(hm_ctx* ctx, hm_box_plan* plan, const double* d_pos, long long np, int reps, char* report,
+                         size_t report_len) {
+    if (!ctx || !plan) return -1;
+    if (!d_pos || np <= 0) return bad(ctx, "autotune: needs a catalogue");
+    std::string r;
+    const int rc = hm::plan_autotune(ctx, plan->impl, d_pos, np, reps, r);
+    if (report && report_len > 0) {
+        const size_t n = std::min(report_len - 1, r.size());
+        memcpy(report, r.data(), n);
+        report[n] = 0;
+    }
+    return rc;
+}
+
+int hm_box_plan_get_shapes(const hm_box_plan* plan, unsigned* wide_mask, int* bin_block) {
+    if (!plan || !wide_mask || !bin_block) return -1;
+    hm::plan_tune_get(plan->impl, wide_mask, bin_block);
+    return 0;
+}
+
+int hm_box_plan_set_shapes(hm_box_plan* plan, unsigned wide_mask, int bin_block) {
+    if (!plan) return -1;
+    hm::plan_tune_set(plan->impl, wide_mask, bin_block);
+    return 0;
+}
 
 int hm_box_plan_triple_mesh(const hm_box_plan* plan, int m[3]) {
     if (!plan || !m) return -1;

@@ -16,6 +16,8 @@
 // (struct/covariance/calc_data_and_cov.py:284-307).
 #include "hm_internal.hpp"
 
+#include <cstring>
+
 using namespace hm;
 
 struct hm_batch {
@@ -195,6 +197,25 @@ int hm_batch_push(hm_ctx* ctx, hm_batch* b, const double* d_pos, long long n) {
     }
     b->pushed++;
     return 0;
+}
+
+int hm_batch_autotune(hm_ctx* ctx, hm_batch* b, const double* d_pos, long long n, int reps, char* report,
+                      size_t report_len) {
+    if (!ctx || !b) return -1;
+    if (!d_pos || n <= 0) return bad(ctx, "autotune: needs a catalogue");
+    std::string r;
+    const int rc = plan_autotune(ctx, b->plan, d_pos, n, reps, r);
+    if (report && report_len > 0) {
+        const size_t m = std::min(report_len - 1, r.size());
+        memcpy(report, r.data(), m);
+        report[m] = 0;
+    }
+    // a captured graph holds the old shapes: capture again at the next push
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    return rc;
 }
 
 long long hm_batch_count(const hm_batch* b) { return b ? b->pushed : -1; }

@@ -25,6 +25,7 @@
 #include "../../include/hitomi_amd.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -934,6 +935,9 @@ static void make_ang(int l1, int l2, const std::vector<int>& ml, const std::vect
 // ---------------------------------------------------------------------------
 struct BoxPlanImpl {
     hm_box_config cfg;
+    // launch shapes (the autotuner's choice; results do not depend on them)
+    unsigned tune_wide = 0;   // wide column tiles, bit per tunable length (256, 288, 512)
+    int tune_bin = 256;       // threads per block of the P_l binning
     Mesh mesh;
     int nk = 0, ne = 0;
     std::vector<double> kbin, lower, upper;
@@ -1677,6 +1681,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                         const double* d_pos_r = nullptr, long long nr = 0, double alpha = 0.0,
                         const BoxWeighted* wt = nullptr, const ExecOpts* xo = nullptr) {
     const Mesh& m = p->mesh;
+    ctx->wide_mask = p->tune_wide;
     const double* dyn = xo ? xo->dyn : nullptr;
     const bool run_assign = !xo || xo->assign;
     const AxisTabs tabs = axis_tabs(p);
@@ -1821,8 +1826,9 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (fill_zero(ctx, p->pk_part.p, sizeof(double) * (size_t)ncols * p->nrows)) return -1;
     if (fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows)) return -1;
     if (ctx->launch("bin_pk", [&] {
-            hipLaunchKernelGGL(k_bin_box<T>, dim3((unsigned)((p->nrows + 255) / 256)), dim3(256), 0, ctx->stream, dk, ba,
-                               (double*)p->pk_part.p, (long long*)p->pk_cnt.p);
+            const int bb = p->tune_bin;
+            hipLaunchKernelGGL(k_bin_box<T>, dim3((unsigned)((p->nrows + bb - 1) / bb)), dim3(bb), 0, ctx->stream, dk,
+                               ba, (double*)p->pk_part.p, (long long*)p->pk_cnt.p);
         }))
         return -1;
     if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->pk_part.p, p->nrows, ncols, (double*)p->pk_red.p))
@@ -2520,6 +2526,74 @@ int plan_finalize_device(hm_ctx* ctx, BoxPlanImpl* p, const double* dyn, double*
     return ctx->launch("batch_finalize", [&] {
         hipLaunchKernelGGL(k_batch_finalize, dim3(1), dim3(256), 0, ctx->stream, a, out);
     });
+}
+
+// Autotuner: every combination of the wide column tile for the tunable
+// lengths this plan transforms and of the binning block size, each timed over
+// reps executions of the given catalogue; the fastest is kept.  The results
+// of a plan do not depend on these shapes (the same arithmetic per line and
+// per mode; the tests compare them bit for bit).
+int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, int reps, std::string& report) {
+    if (reps < 1) reps = 1;
+    const int lens[3] = {256, 288, 512};
+    unsigned avail = 0;
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++)
+            if (p->mesh.n[a] == lens[b] || p->M[a] == lens[b]) avail |= 1u << b;
+    if (p->cfg.precision == HM_FP64) avail = 0;  // the wide tiles exist for FP32 only
+    const int bins[3] = {256, 128, 64};
+    const unsigned keep_wide = p->tune_wide;
+    const int keep_bin = p->tune_bin;
+    double best = 1e300;
+    unsigned best_wide = 0;
+    int best_bin = 256;
+    char buf[128];
+    report.clear();
+    snprintf(buf, sizeof(buf), "lds per workgroup %zu B; ", ctx->lds_max);
+    report += buf;
+    for (unsigned w = 0; w < 8; w++) {
+        if (w & ~avail) continue;
+        for (int bb : bins) {
+            p->tune_wide = w;
+            p->tune_bin = bb;
+            if (plan_execute_any(ctx, p, d_pos, np, nullptr, nullptr, 0, 0.0)) {  // warm-up, lazy allocations
+                p->tune_wide = keep_wide;
+                p->tune_bin = keep_bin;
+                return -1;
+            }
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int r = 0; r < reps; r++)
+                if (plan_execute_any(ctx, p, d_pos, np, nullptr, nullptr, 0, 0.0)) {
+                    p->tune_wide = keep_wide;
+                    p->tune_bin = keep_bin;
+                    return -1;
+                }
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+            snprintf(buf, sizeof(buf), "wide=%u%u%u bin=%d: %.3f ms; ", w & 1u, (w >> 1) & 1u, (w >> 2) & 1u, bb, ms);
+            report += buf;
+            if (ms < best) {
+                best = ms;
+                best_wide = w;
+                best_bin = bb;
+            }
+        }
+    }
+    p->tune_wide = best_wide;
+    p->tune_bin = best_bin;
+    snprintf(buf, sizeof(buf), "chosen: wide=%u%u%u (256, 288, 512) bin=%d, %.3f ms", best_wide & 1u,
+             (best_wide >> 1) & 1u, (best_wide >> 2) & 1u, best_bin, best);
+    report += buf;
+    return 0;
+}
+
+void plan_tune_get(const BoxPlanImpl* p, unsigned* wide, int* bin) {
+    *wide = p->tune_wide;
+    *bin = p->tune_bin;
+}
+void plan_tune_set(BoxPlanImpl* p, unsigned wide, int bin) {
+    p->tune_wide = wide & 7u;
+    p->tune_bin = (bin == 64 || bin == 128) ? bin : 256;
 }
 
 size_t plan_bytes(const BoxPlanImpl* p) { return p->bytes; }

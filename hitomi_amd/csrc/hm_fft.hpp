@@ -284,10 +284,12 @@ __device__ __forceinline__ uint3 xcd_block() {
 }
 
 // Column FFT: grid.x = tiles of B compact inner lines, grid.y = compact outer index.
-template <typename T, int L, int DIR, class Src>
+// BW: columns per tile (0: ColTile's width).  The transform of each column is
+// the same arithmetic for any width; only the tiling of the grid changes.
+template <typename T, int L, int DIR, class Src, int BW = 0>
 __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__ dst, ColGeom g,
                                                     const C<T>* __restrict__ tw) {
-    constexpr int B = ColTile<T, L>::B;
+    constexpr int B = BW ? BW : ColTile<T, L>::B;
     constexpr int LP = LdsPad<L>::LP;
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
@@ -441,6 +443,11 @@ template <typename T> struct RealSrc {
 
 // LDS bytes of the three tile shapes.
 template <typename T, int L> constexpr size_t col_lds() { return (size_t)ColTile<T, L>::B * LdsPad<L>::LP * sizeof(C<T>); }
+// the wide column tile (twice ColTile's width) of the FP32 lengths the box
+// plans use most; the autotuner chooses per length whether to use it
+template <typename T, int L> constexpr bool has_wide_cols() {
+    return sizeof(T) == 4 && (L == 256 || L == 288 || L == 512);
+}
 template <typename T, int L> constexpr size_t row_lds() { return (size_t)RowTile<T, L>::B * LdsPad<L>::LP * sizeof(C<T>); }
 
 }  // namespace hm

@@ -11,16 +11,27 @@ namespace hm {
 
 template <typename T, int L, int DIR, class Src>
 int launch_cols_L(hm_ctx* ctx, const char* name, const Src& src, C<T>* dst, const ColGeom& g) {
-    constexpr int B = ColTile<T, L>::B;
+    constexpr int B0 = ColTile<T, L>::B;
     if (g.n_inner <= 0 || g.n_outer <= 0) return 0;
     if (g.n_outer > 65535) {
         ctx->set_error("fft column pass: outer extent exceeds 65535");
         return -1;
     }
-    dim3 grid((unsigned)((g.n_inner + B - 1) / B), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
-    const size_t lds = col_lds<T, L>();
     const C<T>* tw = ctx->twiddle<T>(L);
     if (!tw) return -1;
+    if constexpr (has_wide_cols<T, L>()) {
+        constexpr int BW = 2 * B0;
+        const size_t lds = (size_t)BW * LdsPad<L>::LP * sizeof(C<T>);
+        if (ctx->wide_cols(L) && lds <= ctx->lds_max) {
+            dim3 grid((unsigned)((g.n_inner + BW - 1) / BW), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
+            return ctx->launch(name, [&] {
+                hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src, BW>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g,
+                                   tw);
+            });
+        }
+    }
+    dim3 grid((unsigned)((g.n_inner + B0 - 1) / B0), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
+    const size_t lds = col_lds<T, L>();
     return ctx->launch(name, [&] {
         hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw);
     });

@@ -26,6 +26,14 @@ struct hm_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, long long>> kstats;  // name -> (ms total, launches)
     bool capturing = false;
+    // launch shapes chosen by an autotuner: bit b of wide_mask set = the wide
+    // column tile for the b-th tunable length (256, 288, 512)
+    unsigned wide_mask = 0;
+    size_t lds_max = 65536;  // dynamic LDS per workgroup the device allows
+    bool wide_cols(int L) const {
+        const int b = L == 256 ? 0 : (L == 288 ? 1 : (L == 512 ? 2 : -1));
+        return b >= 0 && ((wide_mask >> b) & 1u);
+    }
     // device workspace of the mock generator, kept between calls
     void* mock_ws = nullptr;
     size_t mock_ws_bytes = 0;
@@ -100,5 +108,10 @@ void plan_vector_dims(const BoxPlanImpl* p, int* n_pk, int* n_bk);
 int plan_batch_prepare(hm_ctx* ctx, BoxPlanImpl* p);  // device tables of the finalisation (no capture)
 // the data vector [P_l(k_b)], then [B_000(k_a, k_b) - SN], a <= b, into out (device)
 int plan_finalize_device(hm_ctx* ctx, BoxPlanImpl* p, const double* dyn, double* out);
+
+// Autotuner of the launch shapes (wide column tiles, binning block size).
+int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, int reps, std::string& report);
+void plan_tune_get(const BoxPlanImpl* p, unsigned* wide, int* bin);
+void plan_tune_set(BoxPlanImpl* p, unsigned wide, int bin);
 
 }  // namespace hm

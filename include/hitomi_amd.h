@@ -166,6 +166,17 @@ int hm_box_plan_execute_recon(hm_ctx *ctx, hm_box_plan *plan, const double *d_po
 size_t hm_box_plan_bytes(const hm_box_plan *plan);
 /* Mesh used for the triple product (n_mesh when regrid is off). */
 int hm_box_plan_triple_mesh(const hm_box_plan *plan, int m[3]);
+/* Autotuner (no counterpart in the reference): times every combination of
+ * the launch shapes it can vary -- the wide (2x) column tile of the FP32
+ * column passes of length 256, 288 and 512 (bit 0, 1, 2 of wide_mask, used
+ * when the device allows the LDS) and 64, 128 or 256 threads per block of the
+ * P_l binning -- over reps executions of d_pos, and keeps the fastest for
+ * this plan.  The results do not depend on these shapes.  report (may be
+ * NULL) receives the timings.  Get / set read and impose them. */
+int hm_box_plan_autotune(hm_ctx *ctx, hm_box_plan *plan, const double *d_pos, long long np, int reps, char *report,
+                         size_t report_len);
+int hm_box_plan_get_shapes(const hm_box_plan *plan, unsigned *wide_mask, int *bin_block);
+int hm_box_plan_set_shapes(hm_box_plan *plan, unsigned wide_mask, int bin_block);
 
 /* ---------------------------------------------------------------- */
 /* special functions                                                  */
@@ -295,6 +306,10 @@ int hm_batch_vectors(hm_ctx *ctx, hm_batch *b, double *out, long long max_rows);
  * minus the first one.  Needs two or more mocks. */
 int hm_batch_covariance(hm_ctx *ctx, hm_batch *b, double *mean, double *cov);
 int hm_batch_reset(hm_ctx *ctx, hm_batch *b);
+/* hm_box_plan_autotune on the batch's plan; a captured graph is dropped and
+ * captured again at the next push. */
+int hm_batch_autotune(hm_ctx *ctx, hm_batch *b, const double *d_pos, long long n, int reps, char *report,
+                      size_t report_len);
 
 /* ---------------------------------------------------------------- */
 /* Seeded Zel'dovich mocks generated on the device.  No counterpart in the
