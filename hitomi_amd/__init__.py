@@ -62,7 +62,8 @@ class BoxConfig(ct.Structure):
 
 
 class BatchConfig(ct.Structure):
-    _fields_ = [("box", BoxConfig), ("max_mocks", ct.c_longlong), ("cov_with_bk", ct.c_int), ("use_graph", ct.c_int)]
+    _fields_ = [("box", BoxConfig), ("max_mocks", ct.c_longlong), ("cov_with_bk", ct.c_int), ("use_graph", ct.c_int),
+                ("lanes", ct.c_int)]
 
 
 class MockConfig(ct.Structure):
@@ -187,6 +188,7 @@ _SIGS = {
     "hm_batch_dims": (ct.c_int, [_VP, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
     "hm_batch_push": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong]),
     "hm_batch_count": (ct.c_longlong, [_VP]),
+    "hm_batch_collect_timing": (ct.c_int, [_VP, _VP]),
     "hm_batch_vectors": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong]),
     "hm_batch_covariance": (ct.c_int, [_VP, _VP, _VP, _VP]),
     "hm_batch_reset": (ct.c_int, [_VP, _VP]),
@@ -663,13 +665,15 @@ class Batch:
     (hm_batch).  No counterpart in the reference: it runs one mock per process
     and forms the covariance with numpy.cov (calc_data_and_cov.py:284-307)."""
 
-    def __init__(self, ctx: Context, n_mesh, boxsize, max_mocks=1024, cov_with_bk=False, use_graph=True, **kw):
+    def __init__(self, ctx: Context, n_mesh, boxsize, max_mocks=1024, cov_with_bk=False, use_graph=True, lanes=1,
+                 **kw):
         self.ctx = ctx
         cfg = BatchConfig()
         cfg.box = box_config(n_mesh, boxsize, **kw)
         cfg.max_mocks = int(max_mocks)
         cfg.cov_with_bk = 1 if cov_with_bk else 0
         cfg.use_graph = 1 if use_graph else 0
+        cfg.lanes = int(lanes)
         self.cfg = cfg
         self.h = ctx.lib.hm_batch_create(ctx.h, ct.byref(cfg))
         if not self.h:
@@ -696,6 +700,9 @@ class Batch:
     @property
     def count(self):
         return self.ctx.lib.hm_batch_count(self.h)
+
+    def collect_timing(self):
+        self.ctx.check(self.ctx.lib.hm_batch_collect_timing(self.ctx.h, self.h), "hm_batch_collect_timing")
 
     def vectors(self):
         rows = min(self.count, self.cfg.max_mocks)

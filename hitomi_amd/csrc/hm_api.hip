@@ -185,6 +185,24 @@ static int timing_collect(hm_ctx* ctx) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace hm {
+int timing_merge(hm_ctx* dst, hm_ctx* src) {
+    if (!dst || !src || dst == src) return 0;
+    if (timing_collect(src)) return -1;
+    for (auto& kv : src->kstats) {
+        auto& s = dst->kstats[kv.first];
+        s.first += kv.second.first;
+        s.second += kv.second.second;
+    }
+    src->kstats.clear();
+    return 0;
+}
+}  // namespace hm
+
+extern "C" {
+
 int hm_timing_get(hm_ctx* ctx, const char* name, double* total_ms, long long* launches) {
     if (!ctx || !name) return -1;
     if (timing_collect(ctx)) return -1;

@@ -1,15 +1,21 @@
 // hm_batch.hip -- batched periodic-box pipeline: many mocks of uneven size
-// through one box plan, each mock's data vector formed on the device and the
+// through box plans, each mock's data vector formed on the device and the
 // covariance sums accumulated there in FP64, with no host synchronisation
 // between mocks.
 //
 // Per mock: the assignment stage runs eagerly (its grids depend on the
 // particle count) and writes the per-catalogue scalars to device memory; the
-// mesh stage (transforms, binning, shot noise, triple sums), the finalisation
-// and the covariance update read them from there, so they issue identical
-// launches for every mock.  With use_graph they are captured into a hipGraph
-// at the second mock (the first runs eagerly and performs every lazy
-// allocation) and replayed from then on.
+// mesh stage (transforms, binning, shot noise, triple sums) and the
+// finalisation read them from there, so they issue identical launches for
+// every mock.  With use_graph they are captured into a hipGraph at a lane's
+// second mock (the first runs eagerly and performs every lazy allocation) and
+// replayed from then on.
+//
+// Lanes: independent plans on streams of their own take mocks in turn, so that
+// the latency- and LDS-bound kernels of one mock overlap the bandwidth-bound
+// ones of another.  The covariance update of mock i waits for that of mock
+// i - 1 (an event), so the sums are accumulated in push order whatever the
+// number of lanes.
 //
 // The reference runs one mock per process (README.md:167, main_box.cpp:64-76)
 // and forms the covariance downstream with numpy.cov
@@ -20,14 +26,29 @@
 
 using namespace hm;
 
-struct hm_batch {
+namespace {
+
+struct Lane {
+    hm_ctx* ctx = nullptr;  // the caller's context for lane 0, an own one otherwise
+    bool own = false;
     BoxPlanImpl* plan = nullptr;
-    hm_batch_config cfg{};
-    int vec_len = 0, cov_len = 0;
-    long long capacity = 0, pushed = 0;
-    DevBuf dyn, vec, store, shift, s1, s2, count;
+    DevBuf dyn, vec;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    long long pushed = 0;
+    hipEvent_t ev_read = nullptr, ev_cov = nullptr;
+};
+
+}  // namespace
+
+struct hm_batch {
+    hm_batch_config cfg{};
+    std::vector<Lane> lanes;
+    int vec_len = 0, cov_len = 0;
+    long long capacity = 0, pushed = 0;
+    DevBuf store, shift, s1, s2, count;
+    hipEvent_t ev_in = nullptr;      // positions ready on the caller's stream
+    hipEvent_t last_cov = nullptr;   // the previous mock's covariance update
 };
 
 namespace {
@@ -61,33 +82,29 @@ __global__ void k_count_inc(long long* count) {
     if (threadIdx.x == 0) *count += 1;
 }
 
-// Mesh stage, finalisation and covariance update of the mock whose scalars
-// are in b->dyn.
-int mesh_tail(hm_ctx* ctx, hm_batch* b) {
+void drop_graph(Lane& l) {
+    if (l.exec) (void)hipGraphExecDestroy(l.exec);
+    if (l.graph) (void)hipGraphDestroy(l.graph);
+    l.exec = nullptr;
+    l.graph = nullptr;
+}
+
+// Mesh stage and finalisation of the lane's current mock (scalars in l.dyn).
+int mesh_tail(Lane& l) {
     ExecOpts xo;
-    xo.dyn = (const double*)b->dyn.p;
+    xo.dyn = (const double*)l.dyn.p;
     xo.assign = false;
     xo.readback = false;
     // the mesh stage reads the catalogue only through the fixed-point mesh and dyn
-    if (plan_execute_opts(ctx, b->plan, nullptr, 1, xo)) return -1;
-    if (plan_finalize_device(ctx, b->plan, (const double*)b->dyn.p, (double*)b->vec.p)) return -1;
-    const long long nt = std::max((long long)b->cov_len * b->cov_len, (long long)b->vec_len);
-    if (ctx->launch("cov_acc", [&] {
-            hipLaunchKernelGGL(k_cov_acc, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, ctx->stream,
-                               (const double*)b->vec.p, b->vec_len, b->cov_len, (const long long*)b->count.p,
-                               (double*)b->shift.p, (double*)b->s1.p, (double*)b->s2.p, (double*)b->store.p,
-                               b->capacity);
-        }))
-        return -1;
-    return ctx->launch("count_inc", [&] {
-        hipLaunchKernelGGL(k_count_inc, dim3(1), dim3(64), 0, ctx->stream, (long long*)b->count.p);
-    });
+    if (plan_execute_opts(l.ctx, l.plan, nullptr, 1, xo)) return -1;
+    return plan_finalize_device(l.ctx, l.plan, (const double*)l.dyn.p, (double*)l.vec.p);
 }
 
-int capture(hm_ctx* ctx, hm_batch* b) {
+int capture(Lane& l) {
+    hm_ctx* ctx = l.ctx;
     HM_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     ctx->capturing = true;
-    const int rc = mesh_tail(ctx, b);
+    const int rc = mesh_tail(l);
     ctx->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
@@ -101,17 +118,51 @@ int capture(hm_ctx* ctx, hm_batch* b) {
         (void)hipGraphDestroy(g);
         return bad(ctx, std::string("batch: graph instantiate: ") + hipGetErrorString(ei));
     }
-    b->graph = g;
-    b->exec = x;
+    l.graph = g;
+    l.exec = x;
+    return 0;
+}
+
+// the covariance update of the lane's vector, after the previous mock's
+int cov_tail(hm_batch* b, Lane& l) {
+    hm_ctx* ctx = l.ctx;
+    if (b->last_cov) HM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->last_cov, 0));
+    const long long nt = std::max((long long)b->cov_len * b->cov_len, (long long)b->vec_len);
+    if (ctx->launch("cov_acc", [&] {
+            hipLaunchKernelGGL(k_cov_acc, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, ctx->stream,
+                               (const double*)l.vec.p, b->vec_len, b->cov_len, (const long long*)b->count.p,
+                               (double*)b->shift.p, (double*)b->s1.p, (double*)b->s2.p, (double*)b->store.p,
+                               b->capacity);
+        }))
+        return -1;
+    if (ctx->launch("count_inc", [&] {
+            hipLaunchKernelGGL(k_count_inc, dim3(1), dim3(64), 0, ctx->stream, (long long*)b->count.p);
+        }))
+        return -1;
+    HM_HIP_CHECK(ctx, hipEventRecord(l.ev_cov, ctx->stream));
+    b->last_cov = l.ev_cov;
+    return 0;
+}
+
+int sync_all(hm_ctx* ctx, hm_batch* b) {
+    for (Lane& l : b->lanes) HM_HIP_CHECK(ctx, hipStreamSynchronize(l.ctx->stream));
     return 0;
 }
 
 void batch_free(hm_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    DevBuf* bufs[] = {&b->dyn, &b->vec, &b->store, &b->shift, &b->s1, &b->s2, &b->count};
+    for (Lane& l : b->lanes) {
+        if (l.ctx) (void)hipStreamSynchronize(l.ctx->stream);
+        drop_graph(l);
+        dev_free(l.dyn);
+        dev_free(l.vec);
+        plan_destroy(l.plan);
+        if (l.ev_read) (void)hipEventDestroy(l.ev_read);
+        if (l.ev_cov) (void)hipEventDestroy(l.ev_cov);
+        if (l.own) hm_ctx_destroy(l.ctx);
+    }
+    DevBuf* bufs[] = {&b->store, &b->shift, &b->s1, &b->s2, &b->count};
     for (DevBuf* d : bufs) dev_free(*d);
-    plan_destroy(b->plan);
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     delete b;
 }
 
@@ -122,29 +173,60 @@ extern "C" {
 hm_batch* hm_batch_create(hm_ctx* ctx, const hm_batch_config* cfg) {
     if (!ctx || !cfg) return nullptr;
     if (cfg->max_mocks < 0) return bad(ctx, "batch: max_mocks < 0"), nullptr;
+    const int nl = cfg->lanes < 1 ? 1 : cfg->lanes;
+    if (nl > 8) return bad(ctx, "batch: at most 8 lanes"), nullptr;
     (void)hipSetDevice(ctx->device);
     auto* b = new (std::nothrow) hm_batch();
     if (!b) return bad(ctx, "batch: out of host memory"), nullptr;
     b->cfg = *cfg;
-    b->plan = plan_create(ctx, &cfg->box);
-    if (!b->plan) {
-        delete b;
-        return nullptr;
+    b->lanes.resize(nl);
+    for (int i = 0; i < nl; i++) {
+        Lane& l = b->lanes[i];
+        if (i == 0) {
+            l.ctx = ctx;
+        } else {
+            l.ctx = hm_ctx_create(ctx->device, 0);
+            l.own = true;
+            if (!l.ctx) {
+                batch_free(b);
+                return bad(ctx, "batch: cannot create a lane's stream"), nullptr;
+            }
+        }
+        l.plan = plan_create(l.ctx, &cfg->box);
+        if (!l.plan) {
+            if (l.own) ctx->set_error(l.ctx->err);
+            batch_free(b);
+            return nullptr;
+        }
+        if (hipEventCreateWithFlags(&l.ev_read, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&l.ev_cov, hipEventDisableTiming) != hipSuccess) {
+            batch_free(b);
+            return bad(ctx, "batch: cannot create events"), nullptr;
+        }
     }
-    const std::string why = plan_batch_check(b->plan);
+    if (hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess) {
+        batch_free(b);
+        return bad(ctx, "batch: cannot create events"), nullptr;
+    }
+    const std::string why = plan_batch_check(b->lanes[0].plan);
     if (!why.empty()) {
         batch_free(b);
         return bad(ctx, why), nullptr;
     }
     int npk = 0, nbk = 0;
-    plan_vector_dims(b->plan, &npk, &nbk);
+    plan_vector_dims(b->lanes[0].plan, &npk, &nbk);
     b->vec_len = npk + nbk;
     b->cov_len = (cfg->cov_with_bk || npk == 0) ? b->vec_len : npk;
     b->capacity = cfg->max_mocks;
     const size_t cl = (size_t)b->cov_len;
-    if (plan_batch_prepare(ctx, b->plan) || dev_alloc(ctx, b->dyn, sizeof(double) * DYN_LEN) ||
-        dev_alloc(ctx, b->vec, sizeof(double) * b->vec_len) ||
-        dev_alloc(ctx, b->store, sizeof(double) * (size_t)b->vec_len * (size_t)std::max(b->capacity, 1LL)) ||
+    for (Lane& l : b->lanes)
+        if (plan_batch_prepare(l.ctx, l.plan) || dev_alloc(ctx, l.dyn, sizeof(double) * DYN_LEN) ||
+            dev_alloc(ctx, l.vec, sizeof(double) * b->vec_len)) {
+            if (l.own && !l.ctx->err.empty()) ctx->set_error(l.ctx->err);
+            batch_free(b);
+            return nullptr;
+        }
+    if (dev_alloc(ctx, b->store, sizeof(double) * (size_t)b->vec_len * (size_t)std::max(b->capacity, 1LL)) ||
         dev_alloc(ctx, b->shift, sizeof(double) * cl) || dev_alloc(ctx, b->s1, sizeof(double) * cl) ||
         dev_alloc(ctx, b->s2, sizeof(double) * cl * cl) || dev_alloc(ctx, b->count, sizeof(long long))) {
         batch_free(b);
@@ -172,29 +254,53 @@ int hm_batch_dims(const hm_batch* b, int* vec_len, int* cov_len) {
 
 int hm_batch_reset(hm_ctx* ctx, hm_batch* b) {
     if (!ctx || !b) return -1;
+    if (sync_all(ctx, b)) return -1;
     const size_t cl = (size_t)b->cov_len;
     if (fill_zero(ctx, b->s1.p, sizeof(double) * cl) || fill_zero(ctx, b->s2.p, sizeof(double) * cl * cl) ||
         fill_zero(ctx, b->shift.p, sizeof(double) * cl) || fill_zero(ctx, b->count.p, sizeof(long long)))
         return -1;
     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     b->pushed = 0;
+    b->last_cov = nullptr;
+    for (Lane& l : b->lanes) l.pushed = 0;
     return 0;
 }
 
 int hm_batch_push(hm_ctx* ctx, hm_batch* b, const double* d_pos, long long n) {
     if (!ctx || !b) return -1;
     if (n <= 0 || !d_pos) return bad(ctx, "batch: a mock needs particles");
+    Lane& l = b->lanes[(size_t)(b->pushed % (long long)b->lanes.size())];
+    hm_ctx* lc = l.ctx;
+    if (lc != ctx) {
+        // the positions are produced on the caller's stream
+        HM_HIP_CHECK(ctx, hipEventRecord(b->ev_in, ctx->stream));
+        HM_HIP_CHECK(ctx, hipStreamWaitEvent(lc->stream, b->ev_in, 0));
+        lc->timing = ctx->timing;
+        lc->timing_only = ctx->timing_only;
+        lc->wide_mask = ctx->wide_mask;
+    }
     ExecOpts xa;
-    xa.dyn = (const double*)b->dyn.p;
+    xa.dyn = (const double*)l.dyn.p;
     xa.mesh = false;
     xa.readback = false;
-    if (plan_execute_opts(ctx, b->plan, d_pos, n, xa)) return -1;
-    if (b->cfg.use_graph && b->pushed >= 1) {
-        if (!b->exec && capture(ctx, b)) return -1;
-        HM_HIP_CHECK(ctx, hipGraphLaunch(b->exec, ctx->stream));
-    } else if (mesh_tail(ctx, b)) {
+    const auto fail = [&]() {
+        if (lc != ctx) ctx->set_error(lc->err);
         return -1;
+    };
+    if (plan_execute_opts(lc, l.plan, d_pos, n, xa)) return fail();
+    if (lc != ctx) {
+        // the caller's later work on its stream waits until the positions are read
+        HM_HIP_CHECK(ctx, hipEventRecord(l.ev_read, lc->stream));
+        HM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, l.ev_read, 0));
     }
+    if (b->cfg.use_graph && l.pushed >= 1) {
+        if (!l.exec && capture(l)) return fail();
+        HM_HIP_CHECK(ctx, hipGraphLaunch(l.exec, lc->stream));
+    } else if (mesh_tail(l)) {
+        return fail();
+    }
+    if (cov_tail(b, l)) return fail();
+    l.pushed++;
     b->pushed++;
     return 0;
 }
@@ -203,25 +309,38 @@ int hm_batch_autotune(hm_ctx* ctx, hm_batch* b, const double* d_pos, long long n
                       size_t report_len) {
     if (!ctx || !b) return -1;
     if (!d_pos || n <= 0) return bad(ctx, "autotune: needs a catalogue");
+    if (sync_all(ctx, b)) return -1;
     std::string r;
-    const int rc = plan_autotune(ctx, b->plan, d_pos, n, reps, r);
+    const int rc = plan_autotune(ctx, b->lanes[0].plan, d_pos, n, reps, r);
     if (report && report_len > 0) {
         const size_t m = std::min(report_len - 1, r.size());
         memcpy(report, r.data(), m);
         report[m] = 0;
     }
-    // a captured graph holds the old shapes: capture again at the next push
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    unsigned wide = 0;
+    int bin = 256;
+    plan_tune_get(b->lanes[0].plan, &wide, &bin);
+    // the other lanes take the same shapes; captured graphs hold the old ones: capture again
+    for (Lane& l : b->lanes) {
+        plan_tune_set(l.plan, wide, bin);
+        drop_graph(l);
+    }
     return rc;
 }
 
 long long hm_batch_count(const hm_batch* b) { return b ? b->pushed : -1; }
 
+int hm_batch_collect_timing(hm_ctx* ctx, hm_batch* b) {
+    if (!ctx || !b) return -1;
+    if (sync_all(ctx, b)) return -1;
+    for (Lane& l : b->lanes)
+        if (l.own && timing_merge(ctx, l.ctx)) return -1;
+    return 0;
+}
+
 int hm_batch_vectors(hm_ctx* ctx, hm_batch* b, double* out, long long max_rows) {
     if (!ctx || !b || !out) return -1;
+    if (sync_all(ctx, b)) return -1;
     const long long rows = std::min(std::min(b->pushed, b->capacity), max_rows);
     if (rows <= 0) return 0;
     HM_HIP_CHECK(ctx, hipMemcpyAsync(out, b->store.p, sizeof(double) * (size_t)rows * b->vec_len,
@@ -232,6 +351,7 @@ int hm_batch_vectors(hm_ctx* ctx, hm_batch* b, double* out, long long max_rows) 
 
 int hm_batch_covariance(hm_ctx* ctx, hm_batch* b, double* mean, double* cov) {
     if (!ctx || !b) return -1;
+    if (sync_all(ctx, b)) return -1;
     const size_t cl = (size_t)b->cov_len;
     std::vector<double> shift(cl), s1(cl), s2(cl * cl);
     long long count = 0;

@@ -109,6 +109,9 @@ int plan_batch_prepare(hm_ctx* ctx, BoxPlanImpl* p);  // device tables of the fi
 // the data vector [P_l(k_b)], then [B_000(k_a, k_b) - SN], a <= b, into out (device)
 int plan_finalize_device(hm_ctx* ctx, BoxPlanImpl* p, const double* dyn, double* out);
 
+// Kernel timings of src (another context on the same device) added to dst's.
+int timing_merge(hm_ctx* dst, hm_ctx* src);
+
 // Autotuner of the launch shapes (wide column tiles, binning block size).
 int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, int reps, std::string& report);
 void plan_tune_get(const BoxPlanImpl* p, unsigned* wide, int* bin);

@@ -288,6 +288,8 @@ typedef struct {
     long long max_mocks;   /* data vectors kept on the device (further mocks still enter the covariance) */
     int cov_with_bk;       /* 0: covariance of the P_l entries; 1: of the whole vector */
     int use_graph;         /* 1: the mesh stage is captured once into a hipGraph and replayed */
+    int lanes;             /* plans on streams of their own that take mocks in turn (0 or 1: one); the vectors and
+                              the covariance sums are the same for any number */
 } hm_batch_config;
 
 hm_batch *hm_batch_create(hm_ctx *ctx, const hm_batch_config *cfg);
@@ -295,10 +297,16 @@ void hm_batch_destroy(hm_ctx *ctx, hm_batch *b);
 /* vec_len: entries per data vector; cov_len: entries the covariance covers. */
 int hm_batch_dims(const hm_batch *b, int *vec_len, int *cov_len);
 /* One mock (d_pos: n x 3 FP64, already periodic; any n > 0).  Asynchronous:
- * d_pos must stay valid until the next hm_sync or result call. */
+ * d_pos must stay valid until the next hm_sync or result call.  With lanes,
+ * mock i goes to lane i mod lanes; the lane waits for work queued on ctx's
+ * stream before it reads d_pos, and ctx's stream waits until it has read it,
+ * so a caller may refill d_pos with further work on ctx's stream. */
 int hm_batch_push(hm_ctx *ctx, hm_batch *b, const double *d_pos, long long n);
 /* Mocks pushed since creation or the last reset. */
 long long hm_batch_count(const hm_batch *b);
+/* Kernel timings recorded on the lanes' streams (hm_timing_enable on ctx)
+ * added to ctx's, after the lanes finish. */
+int hm_batch_collect_timing(hm_ctx *ctx, hm_batch *b);
 /* The first min(count, max_mocks, max_rows) data vectors, [rows][vec_len]. */
 int hm_batch_vectors(hm_ctx *ctx, hm_batch *b, double *out, long long max_rows);
 /* mean [cov_len] and the unbiased (N - 1) sample covariance [cov_len][cov_len]

@@ -29,15 +29,16 @@ def per_mock_vector(res, nk, npk):
     return np.concatenate(v)
 
 
-@pytest.mark.parametrize("prec,graph", [(hm.FP32, True), (hm.FP32, False), (hm.FP64, True)])
-def test_batch_equals_per_mock_bitwise(ctx, prec, graph):
+@pytest.mark.parametrize("prec,graph,lanes", [(hm.FP32, True, 1), (hm.FP32, False, 1), (hm.FP64, True, 1),
+                                              (hm.FP32, True, 3), (hm.FP32, False, 2)])
+def test_batch_equals_per_mock_bitwise(ctx, prec, graph, lanes):
     plain = prec == hm.FP64
     kw = dict(PLAN, precision=prec, regrid=not plain, prune=not plain)
     pos = mocks(ctx)
     plan = hm.BoxPlan(ctx, 64, L, "TSC", **kw)
     ref = np.array([per_mock_vector(plan.execute(p), PLAN["n_kbin"], 3) for p in pos])
     plan.close()
-    b = hm.Batch(ctx, 64, L, max_mocks=16, use_graph=graph, **kw)
+    b = hm.Batch(ctx, 64, L, max_mocks=16, use_graph=graph, lanes=lanes, **kw)
     assert b.vec_len == 3 * 6 + 21 and b.cov_len == 18
     for p in pos:
         b.push(p)
@@ -52,10 +53,10 @@ def test_batch_equals_per_mock_bitwise(ctx, prec, graph):
     b.close()
 
 
-@pytest.mark.parametrize("with_bk", [False, True])
-def test_batch_covariance_matches_numpy(ctx, with_bk):
+@pytest.mark.parametrize("with_bk,lanes", [(False, 1), (True, 1), (True, 3)])
+def test_batch_covariance_matches_numpy(ctx, with_bk, lanes):
     pos = mocks(ctx)
-    b = hm.Batch(ctx, 64, L, max_mocks=4, cov_with_bk=with_bk, **PLAN)
+    b = hm.Batch(ctx, 64, L, max_mocks=4, cov_with_bk=with_bk, lanes=lanes, **PLAN)
     for rep in range(2):
         for p in pos:
             b.push(p)
@@ -99,3 +100,19 @@ def _interlaced(ctx):
     if not h:
         ctx.raise_error("hm_batch_create")
     ctx.lib.hm_batch_destroy(ctx.h, h)
+
+
+def test_lanes_refill_one_buffer(ctx):
+    """A caller that regenerates every mock into one buffer on its own stream
+    gets the same vectors with three lanes as with one."""
+    cfg = hm.mock_config(N_SIDE, L, amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5, bao_amp=0.05)
+    buf = ctx.empty((max(SIZES), 3), np.float64)
+    out = []
+    for lanes in (1, 3):
+        b = hm.Batch(ctx, 64, L, max_mocks=16, lanes=lanes, **PLAN)
+        for i, n in enumerate(SIZES):
+            hm.mock_zeldovich(ctx, cfg, 100 + i, n, out=buf)
+            b.push(buf, n)
+        out.append(b.vectors())
+        b.close()
+    assert np.array_equal(out[0], out[1])
