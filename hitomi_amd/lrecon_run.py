@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--pk-mesh", type=int, default=256)
+    ap.add_argument("--fine-mesh", type=int, default=512, help="mesh of the costlier plain reconstruction (0: skip)")
     ap.add_argument("--md", default=None)
     args = ap.parse_args()
 
@@ -104,6 +105,9 @@ def main():
                       precision=hm.FP32)
     kc = 0.005 + 0.005 * (np.arange(60) + 0.5)
     rms = {"plain": [], "learned": []}
+    fine = hm.LRecon(ctx, args.fine_mesh, L, b1=2.0, RG=15.0, psi_scale=10.0) if args.fine_mesh else None
+    if fine:
+        rms["plain_fine"] = []
     amp = {"pre": [], "plain": [], "learned": [], "true": []}
     cost = {"plain": 0.0, "learned": 0.0}
     for j in range(args.test):
@@ -117,6 +121,11 @@ def main():
         cost["learned"] += time.perf_counter() - t
         P, T = pos.download(), shift.download()
         S0, S1 = sp.download(), sq.download()
+        if fine:
+            fp, fq = fine.predict(pos)
+            rms["plain_fine"].append(float(np.sqrt(np.mean(np.sum((fp.download() - T) ** 2, 1)))))
+            fp.free()
+            fq.free()
         rms["plain"].append(float(np.sqrt(np.mean(np.sum((S0 - T) ** 2, 1)))))
         rms["learned"].append(float(np.sqrt(np.mean(np.sum((S1 - T) ** 2, 1)))))
         for key, S in (("pre", None), ("plain", S0), ("learned", S1), ("true", T)):
@@ -135,6 +144,14 @@ def main():
         model.features(pos)
     ctx.sync()
     cost["plain"] = (time.perf_counter() - t) / 5
+    if fine:
+        fine.features(pos)
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(5):
+            fine.features(pos)
+        ctx.sync()
+        cost["plain_fine"] = (time.perf_counter() - t) / 5
     out = {
         "workload": f"learned reconstruction, {args.side}^3 lattice, L = {L:g}, {args.tracers:.0f} tracers, b = 2, "
                     f"f = 0.76, R = 15; network on {args.mesh}^3; {args.train} training and {args.test} held-out mocks",
@@ -144,11 +161,14 @@ def main():
         "held_out_rms_learned": round(float(np.mean(rms["learned"])), 4),
         "held_out_rms_gain": round(1.0 - float(np.mean(rms["learned"])) / float(np.mean(rms["plain"])), 4),
         "learned_better_on": f"{int(np.sum(np.array(rms['learned']) < np.array(rms['plain'])))} of {args.test}",
+        "held_out_rms_plain_fine": round(float(np.mean(rms["plain_fine"])), 4) if fine else None,
+        "fine_mesh": args.fine_mesh,
         "bao_amplitude": {k: round(float(np.mean(v)), 3) for k, v in amp.items()},
         "bao_amplitude_sem": {k: round(float(np.std(v) / np.sqrt(len(v))), 3) for k, v in amp.items()},
         "ms_per_mock": {"generate": round(gen_ms, 2), "train_step": round(train_ms, 2),
                         "plain_reconstruction": round(cost["plain"] * 1000.0 / 1.0, 2),
-                        "learned_reconstruction": round(cost["learned"] * 1000.0 / max(args.test, 1), 2)},
+                        "learned_reconstruction": round(cost["learned"] * 1000.0 / max(args.test, 1), 2),
+                        "plain_reconstruction_fine_mesh": round(cost.get("plain_fine", 0.0) * 1000.0, 2)},
     }
     print(json.dumps(out), flush=True)
     if args.md:
