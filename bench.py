@@ -303,9 +303,22 @@ def run(args):
         copy_calibration(ctx, args.copy_calibration)
 
     plain = None
+    err = None
     if args.plain_steps > 0:
         pplan = hm.BoxPlan(ctx, N_MESH, BOX, "TSC", precision=hm.FP64, regrid=False, prune=False, **KW)
-        pplan.execute(dpos[0])
+        r64 = pplan.execute(dpos[0])
+        # the FP32 batched path against the plain FP64 one on the same catalogue, per multipole
+        nk = KW["n_kbin"]
+        p32 = ref_vec[:3 * nk].reshape(3, nk)
+        p64 = r64["pk"][..., 0]
+        b32 = ref_vec[3 * nk:]
+        b64 = r64["bk"][..., 0][np.triu_indices(nk)]
+        err = {f"P_{l}": float(np.abs(p32[e] - p64[e]).max() / np.abs(p64[0]).max()) for e, l in enumerate((0, 2, 4))}
+        err["P_2 rel. where |P_2| > 0.1 |P_0|"] = float(np.max(np.where(np.abs(p64[1]) > 0.1 * np.abs(p64[0]),
+                                                                     np.abs(p32[1] - p64[1]) / np.abs(p64[1]), 0.0)))
+        err["B_000"] = float(np.abs(b32 - b64).max() / np.abs(b64).max())
+        err = {k: float(f"{v:.3e}") for k, v in err.items()}
+        err["normalisation"] = "max over k of |FP32 - FP64| / max|P_0 (FP64)|; B: / max|B (FP64)|"
         ctx.sync()
         t = time.perf_counter()
         for i in range(args.plain_steps):
@@ -353,6 +366,7 @@ def run(args):
     }
     if plain:
         out["plain_fp64"] = plain
+        out["fp32_error_vs_fp64"] = err
         out["speedup_vs_plain_fp64"] = round(value / plain["mocks_per_hour"], 2)
     if args.cov_mocks > 0:
         out["covariance_run"] = covariance_run(ctx, args.cov_mocks, args.lanes)

@@ -74,7 +74,8 @@ class MockConfig(ct.Structure):
 
 class LReconConfig(ct.Structure):
     _fields_ = [("n_mesh", ct.c_int), ("boxsize", ct.c_double), ("b1", ct.c_double), ("RG", ct.c_double),
-                ("psi_scale", ct.c_double), ("lr", ct.c_double), ("init_seed", ct.c_ulonglong), ("precision", ct.c_int)]
+                ("psi_scale", ct.c_double), ("lr", ct.c_double), ("init_seed", ct.c_ulonglong), ("precision", ct.c_int),
+                ("conv_cache_taps", ct.c_int)]
 
 
 class BoxResult(ct.Structure):
@@ -773,11 +774,12 @@ class LRecon:
     """Learned correction of the plain reconstruction (hm_lrecon)."""
 
     def __init__(self, ctx: Context, n_mesh, boxsize, b1=2.0, RG=15.0, psi_scale=10.0, lr=1e-3, init_seed=1,
-                 precision=FP32):
+                 precision=FP32, conv_cache_taps=False):
         self.ctx = ctx
         cfg = LReconConfig()
         cfg.n_mesh, cfg.boxsize, cfg.b1, cfg.RG = int(n_mesh), float(boxsize), float(b1), float(RG)
         cfg.psi_scale, cfg.lr, cfg.init_seed, cfg.precision = float(psi_scale), float(lr), int(init_seed), precision
+        cfg.conv_cache_taps = 1 if conv_cache_taps else 0
         self.cfg = cfg
         self.n = int(n_mesh)
         self.h = ctx.lib.hm_lrecon_create(ctx.h, ct.byref(cfg))

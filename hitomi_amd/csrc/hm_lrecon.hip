@@ -9,13 +9,13 @@
 //   network   four 3^3 convolutions with periodic padding, 4 -> 16 -> 16 ->
 //             16 -> 3 channels, ReLU after the first three; activations in
 //             BF16, channels last; MFMA 16x16x16 BF16 with FP32 accumulation
-//   shift     S_i = norm Psi(x_i) + psi_scale out(x_i), both TSC-gathered with
-//             periodic wrap
+//   shift     S_i = norm Psi(x_i) + psi_scale out(x_i), both gathered with the
+//             TSC stencil of the reference's reconstruction gather
 //   loss      mean_i |S_i - T_i|^2 against the true shifts T_i
 //
 // Training is the network's own backward pass: the loss gradient on the
-// output mesh is the adjoint of the gather (a TSC scatter, summed in 64-bit
-// fixed point so that the order of the atomics does not matter), the input
+// output mesh is the adjoint of the gather, the sorted 64-bit fixed-point
+// mass assignment of the residuals (the same stencil; order-independent), the input
 // gradient of a convolution is the same convolution kernel with flipped,
 // transposed weights (masked by the ReLU of the layer below), the weight
 // gradient an MFMA reduction over voxels; Adam (Kingma and Ba, ICLR 2015)
@@ -154,6 +154,96 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         }
 }
 
+// The same convolution with the input tile of a workgroup staged in LDS: a
+// workgroup covers 4 x 4 rows (x, y) and 16 TPW voxels along z, loads that
+// block with a one-voxel periodic halo once (channels last, 16-byte loads),
+// and its four waves (one per row of x) read the A operands of all 27 taps
+// from LDS; 16 consecutive voxels x 32 B are contiguous, so a wave's 8-byte
+// reads cover 512 B without bank conflicts.
+constexpr int CL_RX = 4, CL_RY = 4;
+
+template <int CIN, int TPW>
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
+    constexpr int ZS = 16 * TPW, HX = CL_RX + 2, HY = CL_RY + 2, HZ = ZS + 2;
+    constexpr int HV = HX * HY * HZ;  // halo voxels
+    extern __shared__ unsigned char smraw[];
+    unsigned short* sm = reinterpret_cast<unsigned short*>(smraw);
+    const int nzs = a.n2 / ZS, nys = a.n1 / CL_RY;
+    const int zb = (int)(blockIdx.x % nzs);
+    const int yb = (int)((blockIdx.x / nzs) % nys);
+    const int xb = (int)(blockIdx.x / ((unsigned)nzs * nys));
+    const int x0 = xb * CL_RX, y0 = yb * CL_RY, z0 = zb * ZS;
+    // stage the halo block: CIN * 2 bytes per voxel, in 16-byte pieces
+    constexpr int PPV = CIN * 2 / 16 > 0 ? CIN * 2 / 16 : 1;  // pieces per voxel (CIN = 4: half a piece)
+    if (CIN == 16) {
+        for (int e = threadIdx.x; e < HV * PPV; e += 256) {
+            const int v = e / PPV, pc = e % PPV;
+            const int hz = v % HZ, hy = (v / HZ) % HY, hx = v / (HZ * HY);
+            const long long g = ((long long)wrapi(x0 + hx - 1, a.n0) * a.n1 + wrapi(y0 + hy - 1, a.n1)) * a.n2 +
+                                wrapi(z0 + hz - 1, a.n2);
+            reinterpret_cast<uint4*>(sm)[e] = reinterpret_cast<const uint4*>(a.in + g * CIN)[pc];
+        }
+    } else {
+        for (int v = threadIdx.x; v < HV; v += 256) {
+            const int hz = v % HZ, hy = (v / HZ) % HY, hx = v / (HZ * HY);
+            const long long g = ((long long)wrapi(x0 + hx - 1, a.n0) * a.n1 + wrapi(y0 + hy - 1, a.n1)) * a.n2 +
+                                wrapi(z0 + hz - 1, a.n2);
+            reinterpret_cast<uint2*>(sm)[v] = *reinterpret_cast<const uint2*>(a.in + g * CIN);
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;  // wave wv: rows x0 + wv, y0 .. y0 + 3
+    const int m = lane & 15, g = lane >> 4;
+    f32x4 acc[CL_RY][TPW];
+#pragma unroll
+    for (int ry = 0; ry < CL_RY; ry++)
+#pragma unroll
+        for (int tt = 0; tt < TPW; tt++) acc[ry][tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const bf16x4* W = reinterpret_cast<const bf16x4*>(a.wp);
+    for (int s = 0; s < a.ks; s++) {
+        const bf16x4 bv = W[s * 64 + lane];
+        const int t = (CIN == 16) ? s : 4 * s + g;
+        const int ci0 = (CIN == 16) ? 4 * g : 0;
+        const bool valid = t < 27;
+        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+#pragma unroll
+        for (int ry = 0; ry < CL_RY; ry++) {
+            const int hrow = ((wv + 1 + dx) * HY + (ry + 1 + dy)) * HZ;
+#pragma unroll
+            for (int tt = 0; tt < TPW; tt++) {
+                bf16x4 av = bf16x4{0, 0, 0, 0};
+                if (valid) av = *reinterpret_cast<const bf16x4*>(sm + (hrow + tt * 16 + m + 1 + dz) * CIN + ci0);
+                acc[ry][tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[ry][tt], 0, 0, 0);
+            }
+        }
+    }
+    const int c = lane & 15;
+    const float bb = a.bias ? a.bias[c] : 0.0f;
+#pragma unroll
+    for (int ry = 0; ry < CL_RY; ry++) {
+        const long long r0 = ((long long)(x0 + wv) * a.n1 + (y0 + ry)) * a.n2 + z0;
+#pragma unroll
+        for (int tt = 0; tt < TPW; tt++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const long long v = r0 + tt * 16 + 4 * g + e;
+                const float val = acc[ry][tt][e] + bb;
+                if (a.mode == CONV_RELU_BF16) {
+                    static_cast<unsigned short*>(a.out)[v * 16 + c] = f2bf(fmaxf(val, 0.0f));
+                } else if (a.mode == CONV_F32) {
+                    if (c < a.cout) static_cast<float*>(a.out)[v * a.ostride + c] = val;
+                } else {
+                    static_cast<unsigned short*>(a.out)[v * 16 + c] =
+                        f2bf(bf2f(a.mask[v * 16 + c]) > 0.0f ? val : 0.0f);
+                }
+            }
+    }
+}
+
+template <int CIN, int TPW> constexpr size_t conv_lds_bytes() {
+    return (size_t)(CL_RX + 2) * (CL_RY + 2) * (16 * TPW + 2) * CIN * 2;
+}
+
 // ---------------------------------------------------------------------------
 // weight gradient: per wave, over its voxel tiles,
 //   G[k][co] += sum_v X[v + off_t][ci] dZ[v][co],  k = t cin + ci,
@@ -210,13 +300,26 @@ __global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
         for (int e = 0; e < 4; e++) P[(kb * 16 + 4 * g + e) * 16 + m] = acc[kb][e];
 }
 
-// fixed-order FP64 sum over waves -> dW [27][cin][cout], db [cout]
-__global__ void k_dw_reduce(const float* __restrict__ part, long long waves, int nkb, int cin, int cout,
+// fixed-order FP64 sums over the waves in two stages: groups of DW_G waves
+// (blockIdx.y = group), then the groups -> dW [27][cin][cout], db [cout]
+constexpr int DW_G = 64;
+
+__global__ void k_dw_reduce1(const float* __restrict__ part, long long waves, int nkb, double* __restrict__ gsum) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nkb * 256) return;
+    const long long w0 = (long long)blockIdx.y * DW_G;
+    const long long w1 = (w0 + DW_G < waves) ? w0 + DW_G : waves;
+    double s = 0.0;
+    for (long long w = w0; w < w1; w++) s += (double)part[w * (long long)nkb * 256 + idx];
+    gsum[(long long)blockIdx.y * nkb * 256 + idx] = s;
+}
+
+__global__ void k_dw_reduce(const double* __restrict__ gsum, long long groups, int nkb, int cin, int cout,
                             float* __restrict__ dW, float* __restrict__ db) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nkb * 256) return;
     double s = 0.0;
-    for (long long w = 0; w < waves; w++) s += (double)part[w * (long long)nkb * 256 + idx];
+    for (long long q = 0; q < groups; q++) s += gsum[q * (long long)nkb * 256 + idx];
     const int kb = idx / 256, i = (idx / 16) % 16, j = idx % 16;
     if (j >= cout) return;
     if (kb == nkb - 1) {
@@ -235,54 +338,77 @@ struct Mesh3 {
     double L[3];
 };
 
-__device__ __forceinline__ void tsc_periodic(const Mesh3& g, const double* p, int idx[3][3], double w[3][3]) {
-    for (int a = 0; a < 3; a++) {
-        int i0;
-        stencil_axis(2, axis_xp(p[a], g.n[a], g.L[a]), &i0, w[a]);
-        for (int s = 0; s < 3; s++) {
-            int i = (i0 + s) % g.n[a];
-            if (i < 0) i += g.n[a];
-            idx[a][s] = i;
-        }
-    }
+// TSC stencil of the reference's reconstruction gather (particle_reconstruction.hpp:141-205,
+// 334-388): no wrap, a stencil point whose linear index leaves [0, n^3) is
+// skipped.  The library's sorted assignment uses the same points and weights,
+// so it is this gather's exact adjoint.
+__device__ __forceinline__ int tsc_ref(const Mesh3& g, const double* p, long long lin[27], double w[27]) {
+    double wa[3][3];
+    int i0[3];
+    for (int a = 0; a < 3; a++) stencil_axis(2, axis_xp(p[a], g.n[a], g.L[a]), &i0[a], wa[a]);
+    const long long ng = (long long)g.n[0] * g.n[1] * g.n[2];
+    int c = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                const long long v = ((long long)(i0[0] + i) * g.n[1] + (i0[1] + j)) * g.n[2] + (i0[2] + k);
+                if (v < 0 || v >= ng) continue;
+                lin[c] = v;
+                w[c] = (wa[0][i] * wa[1][j]) * wa[2][k];
+                c++;
+            }
+    return c;
 }
 
-// S_i = plain (channels 0..2 of feat) [+ psi_scale out]; with
-// target: residual r = S - T (clamped to +-L per component) and |r|^2 per block
+// S_i = plain (channels 0..2 of feat) [+ psi_scale out]; with target: |S - T|^2
+// per block and, with wxy / wz, the scaled residuals as the weights of the
+// loss gradient's assignment: (coef r_x, coef r_y) and (coef r_z, 0), r
+// clamped to +-L per component
 __global__ __launch_bounds__(256) void k_shift_loss(Mesh3 g, const double* __restrict__ pos, long long np,
-                                                    const float* __restrict__ feat, const float* __restrict__ out,
+                                                    const float4* __restrict__ feat, const float4* __restrict__ out,
                                                     double psi_scale, const double* __restrict__ target,
                                                     double* __restrict__ s_plain, double* __restrict__ s_pred,
-                                                    double* __restrict__ resid, double* __restrict__ part) {
+                                                    double coef, double* __restrict__ wxy, double* __restrict__ wz,
+                                                    double* __restrict__ part) {
 #pragma clang fp contract(off)
     __shared__ double sm[2][256];
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     double l0 = 0.0, l1 = 0.0;
     if (i < np) {
-        int idx[3][3];
-        double w[3][3];
-        tsc_periodic(g, pos + 3 * i, idx, w);
+        long long lin[27];
+        double w[27];
+        const int cnt = tsc_ref(g, pos + 3 * i, lin, w);
         double sp[3] = {0.0, 0.0, 0.0}, so[3] = {0.0, 0.0, 0.0};
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++)
-                for (int c = 0; c < 3; c++) {
-                    const long long v = ((long long)idx[0][a] * g.n[1] + idx[1][b]) * g.n[2] + idx[2][c];
-                    const double ww = (w[0][a] * w[1][b]) * w[2][c];
-                    for (int d = 0; d < 3; d++) {
-                        sp[d] += ww * (double)feat[4 * v + d];
-                        if (out) so[d] += ww * (double)out[4 * v + d];
-                    }
-                }
+        for (int c = 0; c < cnt; c++) {
+            const float4 f = feat[lin[c]];
+            sp[0] += w[c] * (double)f.x;
+            sp[1] += w[c] * (double)f.y;
+            sp[2] += w[c] * (double)f.z;
+            if (out) {
+                const float4 o = out[lin[c]];
+                so[0] += w[c] * (double)o.x;
+                so[1] += w[c] * (double)o.y;
+                so[2] += w[c] * (double)o.z;
+            }
+        }
+        double r[3];
         for (int d = 0; d < 3; d++) {
             const double pl = sp[d], pr = pl + so[d] * psi_scale;  // features hold norm Psi in Mpc/h
             if (s_plain) s_plain[3 * i + d] = pl;
             if (s_pred) s_pred[3 * i + d] = pr;
+            r[d] = 0.0;
             if (target) {
                 const double t = target[3 * i + d];
                 l0 += (pl - t) * (pl - t);
                 l1 += (pr - t) * (pr - t);
-                if (resid) resid[3 * i + d] = fmin(fmax(pr - t, -g.L[d]), g.L[d]);
+                r[d] = fmin(fmax(pr - t, -g.L[d]), g.L[d]);
             }
+        }
+        if (wxy) {
+            wxy[2 * i] = r[0] * coef;
+            wxy[2 * i + 1] = r[1] * coef;
+            wz[2 * i] = r[2] * coef;
+            wz[2 * i + 1] = 0.0;
         }
     }
     sm[0][threadIdx.x] = l0;
@@ -299,30 +425,6 @@ __global__ __launch_bounds__(256) void k_shift_loss(Mesh3 g, const double* __res
         part[2 * blockIdx.x] = sm[0][0];
         part[2 * blockIdx.x + 1] = sm[1][0];
     }
-}
-
-// gradient of the loss on the output mesh: (2 psi_scale / N) sum_i W(v - x_i) r_i,
-// in 64-bit fixed point (integer atomics commute: any order gives the same sums)
-__global__ __launch_bounds__(256) void k_scatter_grad(Mesh3 g, const double* __restrict__ pos, long long np,
-                                                      const double* __restrict__ resid, double coef, double scale,
-                                                      unsigned long long* __restrict__ G) {
-#pragma clang fp contract(off)
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= np) return;
-    int idx[3][3];
-    double w[3][3];
-    tsc_periodic(g, pos + 3 * i, idx, w);
-    const long long ng = (long long)g.n[0] * g.n[1] * g.n[2];
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++)
-            for (int c = 0; c < 3; c++) {
-                const long long v = ((long long)idx[0][a] * g.n[1] + idx[1][b]) * g.n[2] + idx[2][c];
-                const double ww = (w[0][a] * w[1][b]) * w[2][c];
-                for (int d = 0; d < 3; d++) {
-                    const long long q = llrint(((ww * resid[3 * i + d]) * coef) * scale);
-                    atomicAdd(&G[d * ng + v], (unsigned long long)q);
-                }
-            }
 }
 
 // fixed point -> BF16 dZ [ng][16] (channels 0..2) and FP32 copy [ng][4]
@@ -381,7 +483,7 @@ struct hm_lrecon {
     int tpw = 1;
     size_t np_ = 0;
     DevBuf params, grads, adam_m, adam_v, wp, wpt, bias16;  // FP32 master, packed BF16 forward / transposed
-    DevBuf feat, x, act[3], out, dzA, dzB, G, part, lpart, resid, g32;
+    DevBuf feat, x, act[3], out, dzA, dzB, G, part, gsum, lpart, wxy, wz, g32;
     long long part_waves = 0, tiles_per_wave = 0;
     ReconWork rw;
     int step = 0;
@@ -415,8 +517,21 @@ int repack(hm_ctx* ctx, hm_lrecon* h) {
     return 0;
 }
 
+template <int CIN, int TPW>
+int conv_lds_launch(hm_ctx* ctx, const char* name, const ConvArgs& a) {
+    const unsigned blocks = (unsigned)((a.n0 / CL_RX) * (a.n1 / CL_RY) * (a.n2 / (16 * TPW)));
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_conv_lds<CIN, TPW>), dim3(blocks), dim3(256), (conv_lds_bytes<CIN, TPW>()), ctx->stream, a);
+    });
+}
+
 template <int CIN>
 int conv_tpw(hm_ctx* ctx, const char* name, const ConvArgs& a, int tpw, long long waves) {
+    if (ctx->conv_lds && a.n0 % CL_RX == 0 && a.n1 % CL_RY == 0 && a.n2 % (16 * tpw) == 0) {
+        if (tpw == 4 && conv_lds_bytes<CIN, 4>() <= ctx->lds_max) return conv_lds_launch<CIN, 4>(ctx, name, a);
+        if (tpw == 2 && conv_lds_bytes<CIN, 2>() <= ctx->lds_max) return conv_lds_launch<CIN, 2>(ctx, name, a);
+        if (tpw == 1 && conv_lds_bytes<CIN, 1>() <= ctx->lds_max) return conv_lds_launch<CIN, 1>(ctx, name, a);
+    }
     const unsigned blocks = (unsigned)((waves + 3) / 4);
     return ctx->launch(name, [&] {
         if (tpw == 4)
@@ -444,6 +559,7 @@ int conv(hm_ctx* ctx, hm_lrecon* h, const char* name, int cin, const unsigned sh
     a.out = out;
     a.mask = mask;
     const long long waves = (long long)a.n0 * a.n1 * (a.n2 / (16 * h->tpw));
+    ctx->conv_lds = h->cfg.conv_cache_taps == 0;
     return cin == 16 ? conv_tpw<16>(ctx, name, a, h->tpw, waves) : conv_tpw<4>(ctx, name, a, h->tpw, waves);
 }
 
@@ -467,9 +583,15 @@ int conv_dw(hm_ctx* ctx, hm_lrecon* h, int l, const unsigned short* x, const uns
                 hipLaunchKernelGGL(k_conv_dw<4>, dim3(blocks), dim3(256), 0, ctx->stream, a);
         }))
         return -1;
+    const long long waves = (long long)blocks * 4, groups = (waves + DW_G - 1) / DW_G;
+    if (ctx->launch("lrecon_dw_reduce", [&] {
+            hipLaunchKernelGGL(k_dw_reduce1, dim3((nkb * 256 + 255) / 256, (unsigned)groups), dim3(256), 0, ctx->stream,
+                               (const float*)h->part.p, waves, nkb, (double*)h->gsum.p);
+        }))
+        return -1;
     return ctx->launch("lrecon_dw_reduce", [&] {
         hipLaunchKernelGGL(k_dw_reduce, dim3((nkb * 256 + 255) / 256), dim3(256), 0, ctx->stream,
-                           (const float*)h->part.p, (long long)(blocks * 4), nkb, cin, COUT_L[l], dW, db);
+                           (const double*)h->gsum.p, groups, nkb, cin, COUT_L[l], dW, db);
     });
 }
 
@@ -532,6 +654,7 @@ int alloc_model(hm_ctx* ctx, hm_lrecon* h) {
     h->tiles_per_wave = tpw;
     h->part_waves = ((tiles + tpw - 1) / tpw + 3) / 4 * 4;
     if (ensure(ctx, h->part, sizeof(float) * (size_t)h->part_waves * 28 * 256)) return -1;
+    if (ensure(ctx, h->gsum, sizeof(double) * (size_t)((h->part_waves + DW_G - 1) / DW_G) * 28 * 256)) return -1;
     return 0;
 }
 
@@ -582,19 +705,25 @@ int prepare(hm_ctx* ctx, hm_lrecon* h, const double* d_pos, long long n) {
     });
 }
 
-// shifts and loss; with target, residuals and the output-mesh gradient into dzA
+// shifts and loss; with grad, the output-mesh gradient into dzA: the residuals
+// assigned with the sorted fixed-point assignment (the gather's adjoint)
 int shifts(hm_ctx* ctx, hm_lrecon* h, const double* d_pos, long long n, const double* target, double* s_plain,
            double* s_pred, bool use_net, double* loss_plain, double* loss_pred, bool grad) {
     const long long blocks = (n + 255) / 256;
     if (target && (ensure(ctx, h->lpart, sizeof(double) * 2 * (size_t)blocks) ||
-                   (grad && ensure(ctx, h->resid, sizeof(double) * 3 * (size_t)n))))
+                   (grad && (ensure(ctx, h->wxy, sizeof(double) * 2 * (size_t)n) ||
+                             ensure(ctx, h->wz, sizeof(double) * 2 * (size_t)n)))))
         return -1;
     const Mesh3 g = mesh3(h);
+    // d loss / d out = (2 psi_scale / N) sum_i W r_i; |sum| <= 2 psi_scale max|r| <= 2 psi_scale L
+    const double coef = 2.0 * h->cfg.psi_scale / (double)n;
+    const double bound = 2.0 * h->cfg.psi_scale * std::max(h->mesh.L[0], std::max(h->mesh.L[1], h->mesh.L[2])) + 1.0;
+    const double scale = std::ldexp(1.0, (int)std::floor(std::log2(std::ldexp(1.0, 61) / bound)));
     if (ctx->launch("lrecon_shift", [&] {
             hipLaunchKernelGGL(k_shift_loss, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g, d_pos, n,
-                               (const float*)h->feat.p, use_net ? (const float*)h->out.p : nullptr, h->cfg.psi_scale,
-                               target, s_plain, s_pred, grad ? (double*)h->resid.p : nullptr,
-                               target ? (double*)h->lpart.p : nullptr);
+                               (const float4*)h->feat.p, use_net ? (const float4*)h->out.p : nullptr, h->cfg.psi_scale,
+                               target, s_plain, s_pred, coef, grad ? (double*)h->wxy.p : nullptr,
+                               grad ? (double*)h->wz.p : nullptr, target ? (double*)h->lpart.p : nullptr);
         }))
         return -1;
     if (target) {
@@ -611,15 +740,17 @@ int shifts(hm_ctx* ctx, hm_lrecon* h, const double* d_pos, long long n, const do
         if (loss_pred) *loss_pred = b / (double)n;
     }
     if (!grad) return 0;
-    // d loss / d out = (2 psi_scale / N) sum_i W r_i; |sum| <= 2 psi_scale max|r| <= 2 psi_scale L
-    const double coef = 2.0 * h->cfg.psi_scale / (double)n;
-    const double bound = 2.0 * h->cfg.psi_scale * std::max(h->mesh.L[0], std::max(h->mesh.L[1], h->mesh.L[2])) + 1.0;
-    const double scale = std::ldexp(1.0, (int)std::floor(std::log2(std::ldexp(1.0, 61) / bound)));
-    if (fill_zero(ctx, h->G.p, sizeof(unsigned long long) * 3 * (size_t)h->ng)) return -1;
-    if (ctx->launch("lrecon_scatter", [&] {
-            hipLaunchKernelGGL(k_scatter_grad, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g, d_pos, n,
-                               (const double*)h->resid.p, coef, scale, (unsigned long long*)h->G.p);
-        }))
+    AssignGeom ag;
+    for (int a = 0; a < 3; a++) {
+        ag.n[a] = h->mesh.n[a];
+        ag.L[a] = h->mesh.L[a];
+    }
+    ag.ng = h->ng;
+    ag.assign = HM_TSC;
+    if (sort_work_alloc(ctx, h->rw.sort, ag, n, true)) return -1;
+    long long* G = (long long*)h->G.p;
+    if (assign_sorted(ctx, ag, d_pos, (const double*)h->wxy.p, n, scale, G, G + h->ng, h->rw.sort) ||
+        assign_sorted(ctx, ag, d_pos, (const double*)h->wz.p, n, scale, G + 2 * h->ng, nullptr, h->rw.sort))
         return -1;
     const unsigned gb = (unsigned)std::min<long long>((h->ng + 255) / 256, 16384);
     return ctx->launch("lrecon_grad_dz", [&] {
@@ -643,7 +774,7 @@ int adam(hm_ctx* ctx, hm_lrecon* h) {
 
 void free_model(hm_lrecon* h) {
     DevBuf* b[] = {&h->params, &h->grads, &h->adam_m, &h->adam_v, &h->wp, &h->wpt, &h->bias16, &h->feat, &h->x,
-                   &h->out, &h->dzA, &h->dzB, &h->G, &h->part, &h->lpart, &h->resid, &h->g32,
+                   &h->out, &h->dzA, &h->dzB, &h->G, &h->part, &h->gsum, &h->lpart, &h->wxy, &h->wz, &h->g32,
                    &h->act[0], &h->act[1], &h->act[2]};
     for (DevBuf* d : b) dev_free(*d);
     recon_work_free(h->rw);

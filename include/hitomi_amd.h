@@ -363,8 +363,11 @@ int hm_mock_zeldovich_biased(hm_ctx *ctx, const hm_mock_config *cfg, unsigned lo
  * V / N, no randoms) on an n_mesh^3 TSC mesh; four 3^3 convolutions with
  * periodic padding (4 -> 16 -> 16 -> 16 -> 3 channels, ReLU after the first
  * three), BF16 operands and FP32 accumulation on the matrix cores; shift
- * S_i = norm Psi(x_i) + psi_scale out(x_i), TSC-gathered with periodic wrap;
- * loss mean_i |S_i - T_i|^2.  Parameters, FP32, in the order W1 b1 W2 b2 W3
+ * S_i = norm Psi(x_i) + psi_scale out(x_i), both gathered with the TSC
+ * stencil of the reference's reconstruction gather (particle_reconstruction.hpp:
+ * 334-388, so the plain shift is the reference's displacement); loss
+ * mean_i |S_i - T_i|^2, whose mesh gradient is the library's mass assignment
+ * of the residuals (the gather's adjoint).  Parameters, FP32, in the order W1 b1 W2 b2 W3
  * b3 W4 b4, W_l [27][cin][cout] with tap t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1)
  * (input voxel v + (dx, dy, dz)); the last layer starts at zero, so a new
  * model predicts the plain shifts. */
@@ -377,6 +380,8 @@ typedef struct {
     double lr;                    /* Adam step size */
     unsigned long long init_seed; /* He-normal initialisation of the first three layers */
     int precision;                /* of the plain reconstruction's transforms: HM_FP32 or HM_FP64 */
+    int conv_cache_taps;          /* 0: convolution input tiles staged in LDS where they fit; 1: every tap read
+                                     through the caches (the same results, bit for bit) */
 } hm_lrecon_config;
 
 hm_lrecon *hm_lrecon_create(hm_ctx *ctx, const hm_lrecon_config *cfg);

@@ -75,17 +75,11 @@ def test_features_match_oracle(ctx):
     m.close()
 
 
-def tsc_weights(pos, n, L):
-    """Per-particle TSC stencil (reference convention, periodic): idx [N,3,3], w [N,3,3]."""
-    xp = pos * n / L
-    i1 = (xp + 0.5).astype(np.int64)
-    dx = xp - i1
-    w = np.stack([0.5 * (0.5 - dx) ** 2, 0.75 - dx ** 2, 0.5 * (0.5 + dx) ** 2], -1)
-    idx = (i1[..., None] - 1 + np.arange(3)) % n
-    return idx, w
-
-
 def test_loss_and_output_gradient(ctx):
+    """The plain shift is the reference's reconstruction gather of the features
+    (particle_reconstruction.hpp:334-388, via the oracle), and the loss
+    gradient on the output mesh is the reference's mass assignment (the
+    oracle's calc_field) of the scaled residuals: the gather's adjoint."""
     n, L = 32, 500.0
     cfg = hm.mock_config(64, L, f_growth=0.76)
     dpos, dshift, N = hm.mock_zeldovich_biased(ctx, cfg, 11, 20000, 1.0)
@@ -98,18 +92,23 @@ def test_loss_and_output_gradient(ctx):
     S0, S1 = sp.download(), sq.download()
     assert abs(l0 - np.mean(np.sum((S0 - T) ** 2, 1))) <= 1e-12 * l0
     assert abs(l1 - np.mean(np.sum((S1 - T) ** 2, 1))) <= 1e-12 * l1
-    # the plain shift is the TSC gather of the features
-    feat = m.features(dpos).download().reshape(-1, 4)
-    idx, w = tsc_weights(pos, n, L)
-    lin = (idx[:, 0, :, None, None] * n + idx[:, 1, None, :, None]) * n + idx[:, 2, None, None, :]
-    ww = w[:, 0, :, None, None] * w[:, 1, None, :, None] * w[:, 2, None, None, :]
-    gathered = np.einsum("nabc,nabcd->nd", ww, feat[lin][..., :3])
+    p = po.param(n, L, po.TSC)
+    feat = m.features(dpos).download()
+    psi = np.ascontiguousarray(np.moveaxis(feat[..., :3].astype(np.float64), -1, 0)).astype(np.complex128)
+    gathered = po.recon_gather(p, psi, 1.0, pos) - pos
     assert np.abs(gathered - S0).max() <= 1e-9 * np.abs(S0).max()
-    # d loss / d out = (2 psi_scale / N) sum_i W r_i
-    r = S1 - T
-    ref = np.zeros((n ** 3, 3))
-    np.add.at(ref, lin.reshape(N, -1), (ww.reshape(N, -1, 1) * r[:, None, :]) * (2 * 10.0 / N))
-    g = gout.download().reshape(-1, 4)[:, :3]
+    # d loss / d out = (2 psi_scale / N) sum_i W r_i, r = S - T
+    coef = 2.0 * 10.0 / N
+    r = (S1 - T) * coef
+    dV = (L / n) ** 3
+    ref = np.zeros((n, n, n, 3))
+    for a, wgt in enumerate((np.stack([r[:, 0], r[:, 1]], 1), np.stack([r[:, 2], 0 * r[:, 2]], 1))):
+        f = po.calc_field(p, pos, wgt) * dV
+        if a == 0:
+            ref[..., 0], ref[..., 1] = f.real, f.imag
+        else:
+            ref[..., 2] = f.real
+    g = gout.download()[..., :3]
     assert np.abs(g - ref).max() <= 1e-6 * np.abs(ref).max()
     m.close()
 
@@ -152,3 +151,21 @@ def test_training_beats_plain_reconstruction(ctx):
     assert np.all(learned < plain)
     assert learned.mean() < 0.9 * plain.mean()
     m.close()
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_lds_staged_convolutions_equal_cache_reads(ctx, n):
+    """The LDS-staged convolution kernels give the cache-reading ones' results bit for bit."""
+    rng = np.random.default_rng(100 + n)
+    p = he_params(rng)
+    x = rng.normal(0.0, 1.0, (n, n, n, 4)).astype(np.float32)
+    gout = np.zeros((n, n, n, 4), np.float32)
+    gout[..., :3] = rng.normal(0.0, 1.0, (n, n, n, 3))
+    res = []
+    for cache in (False, True):
+        m = hm.LRecon(ctx, n, 500.0, conv_cache_taps=cache)
+        m.set_params(p)
+        res.append((m.net_forward(x), m.net_backward(x, gout)))
+        m.close()
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
