@@ -559,6 +559,7 @@ __host__ __device__ inline int support_unmap(int m, int s, int n) {
 // where rounding makes neighbouring bin edges overlap).
 template <typename T> struct TriStage {
     const C<T>* dk;
+    const C<T>* dfull = nullptr;  // set: a full (c2c) spectrum [n0][n1][n2] instead of the half one
     int n[3], nzh, M[3];
     int s3[3], c3[3];  // delta/W support and compact extents
     int s[3], cs[3];   // shell support and compact extents
@@ -591,7 +592,9 @@ __global__ __launch_bounds__(256) void k_tri_stage(TriStage<T> a, C<T>* __restri
     }
     const int i = idx[0], j = idx[1], k = idx[2];
     C<T> d;
-    if (k < a.nzh) {
+    if (a.dfull) {
+        d = a.dfull[((long long)i * a.n[1] + j) * a.n[2] + k];
+    } else if (k < a.nzh) {
         d = a.dk[((long long)i * a.n[1] + j) * a.nzh + k];
     } else {
         const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
@@ -599,7 +602,7 @@ __global__ __launch_bounds__(256) void k_tri_stage(TriStage<T> a, C<T>* __restri
     }
     const double W = wpow((a.t.wf[0][i] * a.t.wf[1][j]) * a.t.wf[2][k], a.wp);
     D[((long long)x * a.M[1] + y) * a.M[2] + z] = mk<T>((T)((double)d.x / W), (T)((double)d.y / W));
-    if (in_support(x, a.s[0], a.M[0]) && in_support(y, a.s[1], a.M[1]) && in_support(z, a.s[2], a.M[2])) {
+    if (sid && in_support(x, a.s[0], a.M[0]) && in_support(y, a.s[1], a.M[1]) && in_support(z, a.s[2], a.M[2])) {
         const double kmag = kmag3(a.t.kv[0][i], a.t.kv[1][j], a.t.kv[2][k]);
         int lo = -1, hi = -2;
         for (int b = 0; b < a.nk; b++)
@@ -933,6 +936,168 @@ static void make_ang(int l1, int l2, const std::vector<int>& ml, const std::vect
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// survey bispectrum with ELL > 0 (bispec.hpp:4-353): Y_LM(los)-weighted
+// fields per M, their cross spectra with d_00 and the shot-noise ijk term
+// with a direction-dependent angular weight
+// ---------------------------------------------------------------------------
+// weights over D then R: w^pw Y(L, M)(los) (conjugated with conj), times cR
+// for R; los = p / |p| of the unshifted positions (field.hpp:186-306)
+__global__ __launch_bounds__(256) void k_ylm_weights(const double* __restrict__ losD, const double* __restrict__ wD,
+                                                    long long nD, const double* __restrict__ losR,
+                                                    const double* __restrict__ wR, long long nR, int L, int M, int pw,
+                                                    int cj, double cR, double* __restrict__ w) {
+#pragma clang fp contract(off)
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nD + nR) return;
+    const bool d = q < nD;
+    const double* p = d ? losD + 3 * q : losR + 3 * (q - nD);
+    const double m = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const Cplx y = ylm_red(L, M, p[0] / m, p[1] / m, p[2] / m);
+    const double ww = d ? wD[q] : wR[q - nD];
+    const double wp = pw == 1 ? ww : pow(ww, pw);
+    double re = y.re * wp, im = (cj ? -y.im : y.im) * wp;
+    if (!d) {
+        re = cR * re;
+        im = cR * im;
+    }
+    w[2 * q] = re;
+    w[2 * q + 1] = im;
+}
+
+// P_{l m}(k) sums of d_00 conj(N) with N a full (c2c) spectrum (calcPowerSpectrum,
+// field.hpp:765-852): per row partials [nk][2] += w Y(l, m)(k) (d conj N - sn S)/W^2
+struct YBinArgs {
+    int n[3], nzh, s01, nrow_y, nzmax;
+    long long nrows;
+    AxisTabs t;
+    int wp;
+    const int2* f2b;
+    int nk, l, m;
+    double w, sn_re, sn_im;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_ycross(const C<T>* __restrict__ dk, const C<T>* __restrict__ nf,
+                                                   YBinArgs a, double* __restrict__ part, long long* __restrict__ cnt) {
+#pragma clang fp contract(off)
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.nrows) return;
+    const int i = support_map((int)(row / a.nrow_y), a.s01, a.n[0]);
+    const int j = support_map((int)(row % a.nrow_y), a.s01, a.n[1]);
+    double* P = part + row * 2 * (long long)a.nk;
+    long long* Cn = cnt ? cnt + row * (long long)a.nk : nullptr;
+    const long long rbase = ((long long)i * a.n[1] + j) * a.nzh;
+    const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
+    for (int kz = 0; kz <= a.nzmax; kz++) {
+        const C<T> d = dk[rbase + kz];
+        const int nmir = (kz > 0 && kz < a.n[2] - kz) ? 2 : 1;
+        for (int mm = 0; mm < nmir; mm++) {
+            const int ii = mm ? im : i, jj = mm ? jm : j, kk = mm ? (a.n[2] - kz) : kz;
+            const double kx = a.t.kv[0][ii], ky = a.t.kv[1][jj], kzv = a.t.kv[2][kk];
+            const double kmag = kmag3(kx, ky, kzv);
+            const int kindex = (int)(kmag / FINE_DK + 0.5);
+            if (kindex >= FINE_N) continue;
+            const int2 b = a.f2b[kindex];
+            if (b.x < 0) continue;
+            const double dr = (double)d.x, di = mm ? -(double)d.y : (double)d.y;  // d_00(-k) = conj d_00(k)
+            const C<T> e = nf[((long long)ii * a.n[1] + jj) * a.n[2] + kk];
+            double pr = dr * (double)e.x + di * (double)e.y, pi = di * (double)e.x - dr * (double)e.y;
+            const double S = (a.t.sf[0][ii] * a.t.sf[1][jj]) * a.t.sf[2][kk];
+            pr -= a.sn_re * S;
+            pi -= a.sn_im * S;
+            const double W = wpow((a.t.wf[0][ii] * a.t.wf[1][jj]) * a.t.wf[2][kk], a.wp);
+            const double W2 = pow(W, 2);
+            pr /= W2;
+            pi /= W2;
+            const Cplx y = ylm_red(a.l, a.m, kx, ky, kzv);
+            const double vr = pr * y.re - pi * y.im, vi = pr * y.im + pi * y.re;
+            for (int bb = b.x; bb <= b.y; bb++) {
+                P[2 * bb] += a.w * vr;
+                P[2 * bb + 1] += a.w * vi;
+                if (Cn) Cn[bb] += 1;
+            }
+        }
+    }
+}
+
+// (dn_LM conj N_00 - sn S)/W^2 / V on the full mesh, N_00 from its half spectrum
+// (calcShotNoiseForBispectrum_ijk, field.hpp:1116-1160)
+struct YXiArgs {
+    int n[3], nzh;
+    AxisTabs t;
+    int wp;
+    double sn_re, sn_im, inv_v;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ycross_xi(const C<T>* __restrict__ dl, const C<T>* __restrict__ n00, YXiArgs a,
+                                                  C<T>* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long ng = (long long)a.n[0] * a.n[1] * a.n[2];
+    const long long st = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ng; e += st) {
+        const int k = (int)(e % a.n[2]);
+        const long long r = e / a.n[2];
+        const int j = (int)(r % a.n[1]), i = (int)(r / a.n[1]);
+        C<T> nn;
+        if (k < a.nzh) {
+            nn = n00[((long long)i * a.n[1] + j) * a.nzh + k];
+        } else {
+            const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
+            nn = cconj<T>(n00[((long long)im * a.n[1] + jm) * a.nzh + (a.n[2] - k)]);
+        }
+        const C<T> d = dl[e];
+        double pr = (double)d.x * (double)nn.x + (double)d.y * (double)nn.y;
+        double pi = (double)d.y * (double)nn.x - (double)d.x * (double)nn.y;
+        const double S = (a.t.sf[0][i] * a.t.sf[1][j]) * a.t.sf[2][k];
+        pr -= a.sn_re * S;
+        pi -= a.sn_im * S;
+        const double W2 = pow(wpow((a.t.wf[0][i] * a.t.wf[1][j]) * a.t.wf[2][k], a.wp), 2);
+        out[e] = mk<T>((T)(a.inv_v * (pr / W2)), (T)(a.inv_v * (pi / W2)));
+    }
+}
+
+// |n|^2 histogram of A(r) xi(r), A(r) = sum_c w_c Y(l1, m1_c)(r) Y(l2, m2_c)(r)
+// (r with the Nyquist index negative, field.hpp:797-799): FP64 atomics (their
+// order can change the last bits of a sum)
+constexpr int YH_MAXC = 32;
+struct YHistArgs {
+    int n[3];
+    int l1, l2, nc;
+    int m1[YH_MAXC], m2[YH_MAXC];
+    double w[YH_MAXC];
+    long long nkey;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_yhist(const C<T>* __restrict__ xi, YHistArgs a, double* __restrict__ Hre,
+                                              double* __restrict__ Him) {
+#pragma clang fp contract(off)
+    const long long ng = (long long)a.n[0] * a.n[1] * a.n[2];
+    const long long st = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ng; e += st) {
+        const int k = (int)(e % a.n[2]);
+        const long long r = e / a.n[2];
+        const int j = (int)(r % a.n[1]), i = (int)(r / a.n[1]);
+        const int si = i < a.n[0] / 2 ? i : i - a.n[0], sj = j < a.n[1] / 2 ? j : j - a.n[1],
+                  sk = k < a.n[2] / 2 ? k : k - a.n[2];
+        const long long key = (long long)si * si + (long long)sj * sj + (long long)sk * sk;
+        if (key >= a.nkey) continue;
+        double ar = 0.0, ai = 0.0;
+        for (int c = 0; c < a.nc; c++) {
+            const Cplx y1 = ylm_red(a.l1, a.m1[c], (double)si, (double)sj, (double)sk);
+            const Cplx y2 = ylm_red(a.l2, a.m2[c], (double)si, (double)sj, (double)sk);
+            ar += a.w[c] * (y1.re * y2.re - y1.im * y2.im);
+            ai += a.w[c] * (y1.re * y2.im + y1.im * y2.re);
+        }
+        const C<T> x = xi[e];
+        const double vr = ar * (double)x.x - ai * (double)x.y, vi = ar * (double)x.y + ai * (double)x.x;
+        if (vr != 0.0) atomicAdd(&Hre[key], vr);
+        if (vi != 0.0) atomicAdd(&Him[key], vi);
+    }
+}
+
 struct BoxPlanImpl {
     hm_box_config cfg;
     // launch shapes (the autotuner's choice; results do not depend on them)
@@ -1003,6 +1168,9 @@ struct BoxPlanImpl {
     long long z_rows = 0;
     DevBuf zf2b_d, zrb_d, zsy[2], zsc[2], zfields, zr, zpart, zcnt, zred, zcnt_red, zpairs_d;
     DevBuf qo_d;  // batched mode: raw triple-sum column of each bin pair
+    // survey B with ELL > 0: full spectra of the Y_LM-weighted fields, xi, the
+    // imaginary fixed-point mesh, binning partials, the |n|^2 histogram
+    DevBuf cf1, cf2, cx, fixed_im, ypart, yred, yh;
 };
 
 static int alloc(hm_ctx* ctx, BoxPlanImpl* p, DevBuf& b, size_t bytes) {
@@ -1501,7 +1669,8 @@ void plan_destroy(BoxPlanImpl* p) {
                       &p->jtab2, &p->gpairs_d, &p->hist2, &p->sn_pairs2, &p->snp_part, &p->fsup_d, &p->fsz_d, &p->dk2, &p->pkx_part,
                       &p->pkx_part_im, &p->pkx_red, &p->dki, &p->spos, &p->zf2b_d, &p->zrb_d, &p->zsy[0],
                       &p->zsy[1], &p->zsc[0], &p->zsc[1], &p->zfields, &p->zr, &p->zpart, &p->zcnt, &p->zred,
-                      &p->zcnt_red, &p->zpairs_d, &p->qo_d};
+                      &p->zcnt_red, &p->zpairs_d, &p->qo_d, &p->cf1, &p->cf2, &p->cx, &p->fixed_im,
+                      &p->ypart, &p->yred, &p->yh};
     for (DevBuf* b : bufs) dev_free(*b);
     sort_work_free(p->sort);
     delete p;
@@ -1655,6 +1824,34 @@ static int pack_pow(hm_ctx* ctx, const double* pD, const double* wD, long long n
         hipLaunchKernelGGL(k_pack_pow, dim3((unsigned)((nD + nR + 255) / 256)), dim3(256), 0, ctx->stream, pD, wD, nD, pR,
                            wR, nR, cR, pw, pos, w);
     });
+}
+
+// Survey B with ELL > 0: the full spectrum of the field with weights
+// w^pw Y(L, M)(los) (conjugated with cj) of D and cR times those of R, in out.
+template <typename T>
+static int svg_field(hm_ctx* ctx, BoxPlanImpl* p, const AssignGeom& ag, const BoxWeighted* wt, long long nD,
+                     long long nR, int M, int pw, int cj, double cR, double abs, C<T>* out) {
+    const Mesh& m = p->mesh;
+    const long long ntot = nD + nR, ng = m.ng();
+    if (!p->fixed_im.p && alloc(ctx, p, p->fixed_im, sizeof(long long) * (size_t)ng)) return -1;
+    if (ctx->launch("svg_weights", [&] {
+            hipLaunchKernelGGL(k_ylm_weights, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, ctx->stream, wt->losD,
+                               wt->wD, nD, wt->losR, wt->wR, nR, wt->L, M, pw, cj, cR, (double*)p->rw.p);
+        }))
+        return -1;
+    const double scale = fixed_scale_for(1.5 * abs);  // |re| + |im| <= sqrt(2) |w|: |Y_red| <= 1
+    if (assign_sorted(ctx, ag, (const double*)p->rpos.p, (const double*)p->rw.p, ntot, scale, (long long*)p->fixed.p,
+                      (long long*)p->fixed_im.p, p->sort))
+        return -1;
+    const double dV = m.volume() / (double)ng;
+    const unsigned gb = (unsigned)std::max<long long>(1, std::min<long long>((ng + 255) / 256, 16384));
+    if (ctx->launch("svg_to_complex", [&] {
+            hipLaunchKernelGGL(k_fixed_to_complex<T>, dim3(gb), dim3(256), 0, ctx->stream, (const long long*)p->fixed.p,
+                               (const long long*)p->fixed_im.p, ng, 1.0 / scale, 1.0 / dV, dV, out);
+        }))
+        return -1;
+    const int n3[3] = {m.n[0], m.n[1], m.n[2]};
+    return fft3d_c2c<T>(ctx, out, n3, -1);
 }
 
 // The per-catalogue scalars of an execution into device memory (batched mode).
@@ -1918,8 +2115,158 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     }
     const int npairs = (int)p->pairs.size();
     std::vector<double> gacc;  // weighted cross sums of a general multipole, per ordered pair
+    // survey B with ELL > 0 (bispec.hpp:4-353): per M the Y_LM-weighted field, its ijk shot noise, the two-point
+    // shot-noise terms and the triple sums; the raw sums land in sv_row / sv_col (per bin, complex, not yet
+    // divided by the mode counts) and sv_ijk (per pair, complex, before i^(l1 + l2))
+    const bool svg = weighted && wt->L > 0;
+    std::vector<double> sv_row, sv_col, sv_ijk;
+    auto svg_stage = [&](C<T>* F3, TriStage<T> st, auto&& f3_passes, auto&& gram_combo) -> int {
+        const int L = wt->L, nk = p->nk;
+        const long long ng = m.ng();
+        if ((!p->cf1.p && alloc(ctx, p, p->cf1, sizeof(C<T>) * (size_t)ng)) ||
+            (!p->cx.p && alloc(ctx, p, p->cx, sizeof(C<T>) * (size_t)ng)) ||
+            (!p->yh.p && alloc(ctx, p, p->yh, sizeof(double) * 2 * (size_t)p->nkey)) ||
+            (!p->ypart.p && alloc(ctx, p, p->ypart, sizeof(double) * 2 * (size_t)nk * (size_t)p->nrows)) ||
+            (!p->yred.p && alloc(ctx, p, p->yred, sizeof(double) * 2 * (size_t)nk)))
+            return -1;
+        if ((p->l1 == 0 || p->l2 == 0) && !p->cf2.p && alloc(ctx, p, p->cf2, sizeof(C<T>) * (size_t)ng)) return -1;
+        C<T>* cf1 = (C<T>*)p->cf1.p;
+        C<T>* cf2 = (C<T>*)p->cf2.p;
+        C<T>* cx = (C<T>*)p->cx.p;
+        double* Hre = (double*)p->yh.p;
+        double* Him = Hre + p->nkey;
+        sv_row.assign(2 * (size_t)nk, 0.0);
+        sv_col.assign(2 * (size_t)nk, 0.0);
+        sv_ijk.assign(2 * (size_t)npairs, 0.0);
+        const int n3[3] = {m.n[0], m.n[1], m.n[2]};
+        const unsigned gb = (unsigned)std::max<long long>(1, std::min<long long>((ng + 255) / 256, 16384));
+        // P_{l m}(k) of d_00 conj N_LM into acc (bins, complex), weight w (field.hpp:765-852)
+        auto ybin = [&](int l, int mm, double w, double snr, double sni, std::vector<double>& acc) -> int {
+            YBinArgs ya;
+            for (int a = 0; a < 3; a++) ya.n[a] = m.n[a];
+            ya.nzh = (int)nzh;
+            ya.s01 = p->s01;
+            ya.nrow_y = p->nrow_y;
+            ya.nzmax = p->nzmax;
+            ya.nrows = p->nrows;
+            ya.t = tabs;
+            ya.wp = wp;
+            ya.f2b = (const int2*)p->f2b_d.p;
+            ya.nk = nk;
+            ya.l = l;
+            ya.m = mm;
+            ya.w = w;
+            ya.sn_re = snr;
+            ya.sn_im = sni;
+            if (fill_zero(ctx, p->ypart.p, sizeof(double) * 2 * (size_t)nk * (size_t)p->nrows)) return -1;
+            if (ctx->launch("svg_bin", [&] {
+                    hipLaunchKernelGGL(k_bin_ycross<T>, dim3((unsigned)((p->nrows + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, dk, cf2, ya, (double*)p->ypart.p, (long long*)nullptr);
+                }))
+                return -1;
+            if (reduce_cols<double>(ctx, "svg_bin_reduce", (const double*)p->ypart.p, p->nrows, 2 * nk,
+                                    (double*)p->yred.p))
+                return -1;
+            std::vector<double> h(2 * (size_t)nk);
+            HM_HIP_CHECK(ctx, hipMemcpyAsync(h.data(), p->yred.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                                             ctx->stream));
+            HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+            for (size_t e = 0; e < h.size(); e++) acc[e] += h[e];
+            return 0;
+        };
+        for (int M = -L; M <= L; M++) {
+            YHistArgs ha;
+            ha.l1 = p->l1;
+            ha.l2 = p->l2;
+            ha.nc = 0;
+            for (int m1 = -p->l1; m1 <= p->l1; m1++) {
+                const int m2 = -M - m1;
+                if (m2 < -p->l2 || m2 > p->l2) continue;
+                double w = wigner_3j(p->l1, p->l2, L, 0, 0, 0) * wigner_3j(p->l1, p->l2, L, m1, m2, M);
+                w *= (double)(2 * L + 1) * (double)(2 * p->l1 + 1) * (double)(2 * p->l2 + 1);
+                if (std::fabs(w) < 1.0e-10) continue;
+                if (ha.nc >= YH_MAXC) {
+                    ctx->set_error("survey bispectrum: too many (m1, m2) terms");
+                    return -1;
+                }
+                ha.m1[ha.nc] = m1;
+                ha.m2[ha.nc] = m2;
+                ha.w[ha.nc] = w;
+                ha.nc++;
+            }
+            if (ha.nc == 0) continue;
+            const double snr = wt->snLM[2 * (M + L)], sni = wt->snLM[2 * (M + L) + 1];
+            // the Y_LM-weighted fluctuation w (D - alpha R) (field.hpp:186-228), transformed
+            if (svg_field<T>(ctx, p, ag, wt, np, nr, M, 1, 0, wt->c1, wt->abs1, cf1)) return -1;
+            // ijk: xi_LM = IFFT[(dn_LM conj N_00 - SN_LM S)/W^2/V], histogram of A_M xi by |n|^2, Bessel sweep
+            YXiArgs xa;
+            for (int a = 0; a < 3; a++) xa.n[a] = m.n[a];
+            xa.nzh = (int)nzh;
+            xa.t = tabs;
+            xa.wp = wp;
+            xa.sn_re = snr;
+            xa.sn_im = sni;
+            xa.inv_v = 1.0 / m.volume();
+            if (ctx->launch("svg_xi_src", [&] {
+                    hipLaunchKernelGGL(k_ycross_xi<T>, dim3(gb), dim3(256), 0, ctx->stream, (const C<T>*)cf1,
+                                       (const C<T>*)dk2, xa, cx);
+                }))
+                return -1;
+            if (fft3d_c2c<T>(ctx, cx, n3, +1)) return -1;
+            for (int a = 0; a < 3; a++) ha.n[a] = m.n[a];
+            ha.nkey = p->nkey;
+            if (fill_zero(ctx, p->yh.p, sizeof(double) * 2 * (size_t)p->nkey)) return -1;
+            if (ctx->launch("svg_hist", [&] {
+                    hipLaunchKernelGGL(k_yhist<T>, dim3(gb), dim3(256), 0, ctx->stream, (const C<T>*)cx, ha, Hre, Him);
+                }))
+                return -1;
+            const long long snb = (p->nkey + SNP_KPB - 1) / SNP_KPB;
+            for (int par = 0; par < 2; par++) {
+                if (ctx->launch("sn_pairs", [&] {
+                        hipLaunchKernelGGL(k_sn_pairs_blk, dim3((unsigned)snb), dim3(256),
+                                           sizeof(double) * 2 * (size_t)p->nk * SNP_CHP, ctx->stream,
+                                           (const double*)p->jtab.p,
+                                           (const double*)(p->l2 != p->l1 ? p->jtab2.p : p->jtab.p), p->nk, p->nkey,
+                                           (const double*)(par ? Him : Hre), (const short2*)p->pairs_d.p, (int)npairs,
+                                           dV, (double*)p->snp_part.p);
+                    }))
+                    return -1;
+                if (reduce_cols<double>(ctx, "sn_pairs_reduce", (const double*)p->snp_part.p, snb, (int)npairs,
+                                        (double*)p->sn_pairs.p))
+                    return -1;
+                std::vector<double> h((size_t)npairs);
+                HM_HIP_CHECK(ctx, hipMemcpyAsync(h.data(), p->sn_pairs.p, sizeof(double) * h.size(),
+                                                 hipMemcpyDeviceToHost, ctx->stream));
+                HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+                for (int q = 0; q < npairs; q++) sv_ijk[2 * (size_t)q + par] += h[q];
+            }
+            // two-point terms (bispec.hpp:33-93): P_{l1 m1}(k1) for l2 = 0, P_{l2 m2}(k2) for l1 = 0, of d_00
+            // against N_LM = conj(Y_LM) w^2 (D + alpha^2 R) (field.hpp:249-306)
+            if (p->l1 == 0 || p->l2 == 0) {
+                if (svg_field<T>(ctx, p, ag, wt, np, nr, M, 2, 1, wt->c2, wt->abs2, cf2)) return -1;
+                for (int c = 0; c < ha.nc; c++) {
+                    if (p->l2 == 0 && ybin(p->l1, ha.m1[c], ha.w[c], snr, sni, sv_row)) return -1;
+                    if (p->l1 == 0 && ybin(p->l2, ha.m2[c], ha.w[c], snr, sni, sv_col)) return -1;
+                }
+            }
+            // the third field dn_LM / W on the triple mesh, then the triple sums of every (m1, m2)
+            TriStage<T> s3 = st;
+            s3.dfull = cf1;
+            const long long nst = (long long)s3.c3[0] * s3.c3[1] * s3.c3[2];
+            if (ctx->launch("tri_stage", [&] {
+                    hipLaunchKernelGGL(k_tri_stage<T>, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, ctx->stream,
+                                       s3, F3, (char2*)nullptr);
+                }))
+                return -1;
+            if (f3_passes()) return -1;
+            for (int c = 0; c < ha.nc; c++)
+                if (gram_combo(ha.m1[c], ha.m2[c], ha.w[c])) return -1;
+        }
+        return 0;
+    };
     if (p->cfg.bk) {
-        // 4. shot-noise ijk term (field.hpp:1116-1160, bispec.hpp:1023-1056)
+        // 4. shot-noise ijk term (field.hpp:1116-1160, bispec.hpp:1023-1056); per M below for a survey ELL > 0
+        if (!svg) {
         XiSrc<T> xs;
         xs.dk = dk;
         for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
@@ -1978,6 +2325,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                 (double*)sbuf))
             return -1;
         }
+        }
         // 5. shell fields and delta/W on the triple mesh (field.hpp:346-401, 610-639, 328-344)
         const long long nvox = (long long)p->M[0] * p->M[1] * p->M[2];
         C<T>* F = (C<T>*)p->fields.p;
@@ -2006,7 +2354,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                    Draw, (char2*)p->sid.p);
             }))
             return -1;
-        if (p->bk_gen)
+        if (p->bk_gen && !svg)
             HM_HIP_CHECK(ctx, hipMemcpyAsync(F3, Draw, sizeof(C<T>) * (size_t)nvox, hipMemcpyDeviceToDevice, ctx->stream));
         const int4* fsup = (const int4*)p->fsup_d.p;
         const int* fsz = (const int*)p->fsz_d.p;
@@ -2019,15 +2367,17 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             ms.cs[a] = st.cs[a];
         }
         if (p->bk_gen) {
-            // delta/W on the triple mesh, all three passes once
-            ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
-            if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
-            ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
-            if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;
             const long long nrows = (long long)p->M[0] * p->M[1];
-            RowGeom rg3{nrows, p->M[2], p->M[2], st.s3[2]};
-            if (fft_rows<T>(ctx, "tri3_pass_z", p->M[2], +1, F3, F3, rg3)) return -1;
-            // per m1 (bispec.hpp:1107-1180): shells weighted by Y(l1, m1) and by Y(l2, -m1), all cross sums
+            // the third field on the triple mesh, all three passes
+            auto f3_passes = [&]() -> int {
+                ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
+                if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
+                ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
+                if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;
+                RowGeom rg3{nrows, p->M[2], p->M[2], st.s3[2]};
+                return fft_rows<T>(ctx, "tri3_pass_z", p->M[2], +1, F3, F3, rg3);
+            };
+            // shells weighted by Y(l1, m1) and by Y(l2, m2), all cross sums with the third field, times wq
             ms.ylm = 1;
             ms.nk = p->nk;
             ms.l1 = p->l1;
@@ -2035,9 +2385,9 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             ms.t = tabs;
             for (int a = 0; a < 3; a++) ms.n[a] = m.n[a];
             gacc.assign(2 * (size_t)npairs, 0.0);
-            for (size_t q = 0; q < p->mlist.size(); q++) {
-                ms.m1 = p->mlist[q];
-                ms.m2 = -p->mlist[q];
+            auto gram_combo = [&](int m1, int m2, double wq) -> int {
+                ms.m1 = m1;
+                ms.m2 = m2;
                 ColGeom gx = tri_geom_x(p->M, st.s, nvox, nsh, fsup);
                 if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
                 ColGeom gy = tri_geom_y(p->M, st.s, nvox, nsh, fsup);
@@ -2072,8 +2422,17 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                     HM_HIP_CHECK(ctx, hipMemcpyAsync(part.data(), p->gram_red.p, sizeof(double) * part.size(),
                                                      hipMemcpyDeviceToHost, ctx->stream));
                     HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-                    for (int e = 0; e < 2 * cnt; e++) gacc[2 * (size_t)c0 + e] += p->wlist[q] * part[e];
+                    for (int e = 0; e < 2 * cnt; e++) gacc[2 * (size_t)c0 + e] += wq * part[e];
                 }
+                return 0;
+            };
+            if (!svg) {
+                // per m1 (bispec.hpp:1107-1180): Y(l1, m1) and Y(l2, -m1), delta/W third
+                if (f3_passes()) return -1;
+                for (size_t q = 0; q < p->mlist.size(); q++)
+                    if (gram_combo(p->mlist[q], -p->mlist[q], p->wlist[q])) return -1;
+            } else if (svg_stage(F3, st, f3_passes, gram_combo)) {
+                return -1;
             }
         } else {
             // x pass of every shell (one launch), then of delta/W in place
@@ -2335,7 +2694,38 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         const double ir = ip == 0 ? 1.0 : (ip == 2 ? -1.0 : 0.0), ii = ip == 1 ? 1.0 : (ip == 3 ? -1.0 : 0.0);
         const long long M3 = (long long)p->M[0] * p->M[1] * p->M[2];
         const double norm = weighted ? wt->norm : (V / N / N) * (V / N);
-        for (int q = 0; q < npairs; q++) {
+        for (int q = 0; svg && q < npairs; q++) {
+            // survey ELL > 0 (bispec.hpp:33-211): row and column terms (bin averages), i^(l1+l2) times the ijk sums
+            const int a = p->pairs[q].x, b = p->pairs[q].y;
+            const double den = (double)M3 * (double)p->shell_nmode[a] * (double)p->shell_nmode[b];
+            const double br = gram[2 * q] / den, bi = gram[2 * q + 1] / den;
+            double sr = 0.0, si = 0.0;
+            if (p->l2 == 0 && cnt[a]) {
+                sr += sv_row[2 * (size_t)a] / (double)cnt[a];
+                si += sv_row[2 * (size_t)a + 1] / (double)cnt[a];
+            }
+            if (p->l1 == 0 && cnt[b]) {
+                sr += sv_col[2 * (size_t)b] / (double)cnt[b];
+                si += sv_col[2 * (size_t)b + 1] / (double)cnt[b];
+            }
+            const double jr = sv_ijk[2 * (size_t)q], ji = sv_ijk[2 * (size_t)q + 1];
+            sr += ir * jr - ii * ji;
+            si += ir * ji + ii * jr;
+            const size_t o = 2 * ((size_t)a * nk + b);
+            if (res->bk_sum) {
+                res->bk_sum[o] = br;
+                res->bk_sum[o + 1] = bi;
+            }
+            if (res->sn_sum) {
+                res->sn_sum[o] = sr;
+                res->sn_sum[o + 1] = si;
+            }
+            if (res->bk) {
+                res->bk[o] = norm * (br - sr);
+                res->bk[o + 1] = norm * (bi - si);
+            }
+        }
+        for (int q = 0; !svg && q < npairs; q++) {
             const int a = p->pairs[q].x, b = p->pairs[q].y;
             const double den = (double)M3 * (double)p->shell_nmode[a] * (double)p->shell_nmode[b];
             const double br = gram[2 * q] / den, bi = gram[2 * q + 1] / den;

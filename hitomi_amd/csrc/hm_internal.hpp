@@ -84,6 +84,13 @@ struct BoxWeighted {
     const double* wD;
     const double* wR;
     double c1, c2, abs1, abs2, sn3, norm;
+    // B with ELL = L > 0 (bispec.hpp:4-353): the unshifted positions D then R
+    // whose directions are the lines of sight, and per M = -L..L the shot noise
+    // sum_D w^3 Y_LM - alpha^3 sum_R w^3 Y_LM (re, im)
+    int L = 0;
+    const double* losD = nullptr;
+    const double* losR = nullptr;
+    const double* snLM = nullptr;
 };
 struct BoxPlanImpl;
 BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg);

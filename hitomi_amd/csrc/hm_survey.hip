@@ -223,6 +223,42 @@ __global__ __launch_bounds__(256) void k_survey_sn(const double* __restrict__ pD
     }
 }
 
+// sum_D w^3 Y(L, M)(los) and sum_R w^3 Y(L, M)(los) for M = -L..L (field.hpp:1202-1240): per block
+// [M + L][D re, D im, R re, R im], fixed order
+__global__ __launch_bounds__(256) void k_survey_w3y(const double* __restrict__ pD, const double* __restrict__ wD,
+                                                   long long nD, const double* __restrict__ pR,
+                                                   const double* __restrict__ wR, long long nR, int L,
+                                                   double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double sm[256];
+    const int nm = 2 * L + 1;
+    double acc[4 * (2 * SV_MAX_ELL + 1)];
+    for (int i = 0; i < 4 * nm; i++) acc[i] = 0.0;
+    const long long st = (long long)gridDim.x * 256;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nD + nR; q += st) {
+        const bool d = q < nD;
+        const double* p = d ? pD + 3 * q : pR + 3 * (q - nD);
+        const double w3 = pow(d ? wD[q] : wR[q - nD], 3);
+        double los[3];
+        los_of(p, los);
+        for (int M = -L; M <= L; M++) {
+            const Cplx y = ylm_red(L, M, los[0], los[1], los[2]);
+            acc[4 * (M + L) + (d ? 0 : 2)] += w3 * y.re;
+            acc[4 * (M + L) + (d ? 1 : 3)] += w3 * y.im;
+        }
+    }
+    for (int i = 0; i < 4 * nm; i++) {
+        sm[threadIdx.x] = acc[i];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(long long)blockIdx.x * 4 * nm + i] = sm[0];
+        __syncthreads();
+    }
+}
+
 // sum over cells of rho^2 dV, rho = (q 2^-s)(1/dV) (field.hpp:487-492); per-block partials
 __global__ __launch_bounds__(256) void k_survey_sqsum(const long long* __restrict__ q, long long n, double inv_scale,
                                                      double factor, double dV, double* __restrict__ part) {
@@ -481,10 +517,11 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
     p->measure = cfg->measure;
     if (p->measure < HM_SURVEY_PK || p->measure > HM_SURVEY_ZETA_WINDOW) return fail("survey plan: unknown measure");
     if (delegated(p->measure)) {
-        // calcBiSpectrum / calcThreePoint(Window)CorrelationFunction with ELL = 0: no Y_LM(los) weight, so
-        // the estimators are those of the box on the weighted catalogues (bispec.hpp:4-874)
-        if (cfg->ell != 0)
-            return fail("survey plan: the survey bispectrum and three-point functions need ELL = 0 in this build");
+        // calcBiSpectrum / calcThreePoint(Window)CorrelationFunction (bispec.hpp:4-874) on the box machinery:
+        // with ELL = 0 (no Y_LM(los) weight) the box estimators of the weighted catalogues; the bispectrum with
+        // ELL > 0 adds per M the Y_LM(los)-weighted third field and its shot-noise terms
+        if (cfg->ell != 0 && p->measure != HM_SURVEY_BK)
+            return fail("survey plan: the survey three-point functions need ELL = 0 in this build");
         hm_box_config bc;
         std::memset(&bc, 0, sizeof(bc));
         bc.mesh = cfg->mesh;
@@ -497,7 +534,7 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
         bc.n_rbin = cfg->n_rbin;
         bc.rmin = cfg->rmin;
         bc.rmax = cfg->rmax;
-        const int ells[3] = {cfg->ell1, cfg->ell2, 0};
+        const int ells[3] = {cfg->ell1, cfg->ell2, p->measure == HM_SURVEY_BK ? cfg->ell : 0};
         for (int a = 0; a < 3; a++) bc.bk_ell[a] = bc.zeta_ell[a] = ells[a];
         bc.bk = p->measure == HM_SURVEY_BK ? 1 : 0;
         bc.zeta = p->measure == HM_SURVEY_BK ? 0 : 1;
@@ -727,6 +764,43 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
             bw.abs1 = awD + std::fabs(alpha) * awR;
             bw.abs2 = sw2D + alpha * alpha * sw2R;
             bw.sn3 = sw3D - pow(alpha, 3) * sw3R;
+        }
+        std::vector<double> snLM;
+        if (p->measure == HM_SURVEY_BK && p->l > 0) {
+            // Y_LM(los)-weighted shot noise of every M (field.hpp:1202-1240); lines of sight from the input
+            // (unshifted, unreconstructed) positions
+            const int L = p->l, nm = 2 * L + 1;
+            const unsigned nb = (unsigned)std::max<long long>(1, std::min<long long>((nD + nR + 255) / 256, 256));
+            DevBuf yp;
+            if (dev_alloc(ctx, yp, sizeof(double) * 4 * (size_t)nm * nb)) return -1;
+            int rc = ctx->launch("survey_w3y", [&] {
+                hipLaunchKernelGGL(k_survey_w3y, dim3(nb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR, L,
+                                   (double*)yp.p);
+            });
+            std::vector<double> h(4 * (size_t)nm * nb);
+            if (!rc && (hipMemcpyAsync(h.data(), yp.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream) !=
+                            hipSuccess ||
+                        hipStreamSynchronize(ctx->stream) != hipSuccess))
+                rc = -1;
+            dev_free(yp);
+            if (rc) return -1;
+            snLM.assign(2 * (size_t)nm, 0.0);
+            for (int M = 0; M < nm; M++) {
+                double dr = 0.0, di = 0.0, rr = 0.0, ri = 0.0;
+                for (unsigned b = 0; b < nb; b++) {
+                    const double* q = &h[((size_t)b * nm + M) * 4];
+                    dr += q[0];
+                    di += q[1];
+                    rr += q[2];
+                    ri += q[3];
+                }
+                snLM[2 * M] = dr - pow(alpha, 3) * rr;
+                snLM[2 * M + 1] = di - pow(alpha, 3) * ri;
+            }
+            bw.L = L;
+            bw.losD = pD;
+            bw.losR = pR;
+            bw.snLM = snLM.data();
         }
         hm_box_result res;
         std::memset(&res, 0, sizeof(res));

@@ -169,9 +169,9 @@ def test_survey_missing_randoms_exits_1(tmp_path):
     assert r.returncode == 1 and "Cannot open the file" in r.stdout and "'P_R.readParticleBOSS' fails." in r.stdout
 
 
-@pytest.mark.parametrize("measure", ["bk", "3PCF", "window3PCF"])
+@pytest.mark.parametrize("measure", ["3PCF", "window3PCF"])
 def test_survey_unsupported_measure_exits_1(tmp_path, measure):
-    """The survey bispectrum and three-point functions with ELL > 0 are rejected, not computed differently."""
+    """The survey three-point functions with ELL > 0 are rejected, not computed differently."""
     write_xyzw(tmp_path / "data.dat", np.ones((3, 3)) * 500.0, np.ones(3))
     ini = tmp_path / "p.ini"
     hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="data.dat",

@@ -205,6 +205,28 @@ def test_survey_zeta(ctx, prec, measure, ells):
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
 
 
-def test_survey_bispectrum_rejects_ell_nonzero(ctx):
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+@pytest.mark.parametrize("ells", [(2, 0, 2), (0, 2, 2), (1, 1, 2), (1, 0, 1)])
+def test_survey_bispectrum_ell_nonzero(ctx, prec, ells):
+    """calcBiSpectrum (bispec.hpp:4-353) with ELL > 0 for every ith_kbin: the Y_LM(los)-weighted third
+    field per M, the row / column shot-noise terms against N_LM and the direction-weighted ijk term;
+    |dB| <= tol * max|B|, tol = 1e-9 (FP64) / 2e-4 (FP32)."""
+    l1, l2, L_ = ells
+    n, L = 32, 800.0
+    kw = dict(kmin=0.02, kmax=0.1, n_kbin=5)
+    cat = get_cat("patch")
+    plan = hm.SurveyPlan(ctx, n, L, "TSC", ell=L_, ell1=l1, ell2=l2, precision=prec, measure="bk", **kw)
+    got, nm, info = plan.execute(*cat)
+    plan.close()
+    ref = np.zeros((kw["n_kbin"], kw["n_kbin"]), complex)
+    for ith in range(kw["n_kbin"]):
+        p = po.param(n, L, po.TSC, ell1=l1, ell2=l2, ELL=L_, ith_kbin=ith, **kw)
+        s = po.Survey(p, *cat)
+        ref[ith], _, _ = s.bk(p)
+    tol = 1e-9 if prec == hm.FP64 else 2e-4
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+
+
+def test_survey_three_point_rejects_ell_nonzero(ctx):
     with pytest.raises(RuntimeError):
-        hm.SurveyPlan(ctx, 32, 800.0, "TSC", ell=2, ell1=2, ell2=0, measure="bk", kmin=0.02, kmax=0.1, n_kbin=5)
+        hm.SurveyPlan(ctx, 32, 800.0, "TSC", ell=2, ell1=2, ell2=0, measure="3PCF", rmin=60.0, rmax=240.0, n_rbin=4)
