@@ -158,6 +158,7 @@ _SIGS = {
     "hm_timing_enable": (ct.c_int, [_VP, ct.c_int]),
     "hm_timing_get": (ct.c_int, [_VP, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]),
     "hm_timing_reset": (ct.c_int, [_VP]),
+    "hm_timing_get_busy": (ct.c_int, [_VP, ct.c_char_p, ct.POINTER(ct.c_double)]),
     "hm_timing_select": (ct.c_int, [_VP, ct.c_char_p]),
     "hm_assign": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
     "hm_assign_atomic": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
@@ -337,6 +338,12 @@ class Context:
         n = ct.c_longlong()
         self.check(self.lib.hm_timing_get(self.h, name.encode(), ct.byref(ms), ct.byref(n)), "hm_timing_get")
         return ms.value, n.value
+
+    def kernel_busy(self, name):
+        """ms during which at least one timed launch of `name` ran (overlaps count once)."""
+        ms = ct.c_double()
+        self.check(self.lib.hm_timing_get_busy(self.h, name.encode(), ct.byref(ms)), "hm_timing_get_busy")
+        return ms.value
 
     def timing_select(self, name=None):
         self.check(self.lib.hm_timing_select(self.h, name.encode() if name else None), "hm_timing_select")

@@ -95,6 +95,7 @@ void hm_ctx_destroy(hm_ctx* ctx) {
         (void)hipEventDestroy(pe.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->tref) (void)hipEventDestroy(ctx->tref);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -168,35 +169,47 @@ int hm_timing_enable(hm_ctx* ctx, int on) {
     return 0;
 }
 
-static int timing_collect(hm_ctx* ctx) {
-    if (ctx->pending.empty()) return 0;
-    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    for (auto& pe : ctx->pending) {
-        float ms = 0.f;
+}  // extern "C"
+
+// src's finished launches into dst's totals; their spans are placed on dst's
+// clock (dst->tref), which works across streams of one device.
+static int timing_collect_into(hm_ctx* dst, hm_ctx* src) {
+    if (src->pending.empty()) return 0;
+    HM_HIP_CHECK(src, hipStreamSynchronize(src->stream));
+    const bool clock = dst->tref && hipEventSynchronize(dst->tref) == hipSuccess;
+    for (auto& pe : src->pending) {
+        float ms = 0.f, t0 = 0.f;
         if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
-            auto& s = ctx->kstats[pe.name];
+            auto& s = dst->kstats[pe.name];
             s.first += ms;
             s.second += 1;
+            if (clock && hipEventElapsedTime(&t0, dst->tref, pe.a) == hipSuccess)
+                dst->kspans[pe.name].push_back({(double)t0, (double)t0 + (double)ms});
         }
-        ctx->event_pool.push_back(pe.a);
-        ctx->event_pool.push_back(pe.b);
+        src->event_pool.push_back(pe.a);
+        src->event_pool.push_back(pe.b);
     }
-    ctx->pending.clear();
+    src->pending.clear();
+    (void)hipGetLastError();
     return 0;
 }
 
-}  // extern "C"
+static int timing_collect(hm_ctx* ctx) { return timing_collect_into(ctx, ctx); }
 
 namespace hm {
 int timing_merge(hm_ctx* dst, hm_ctx* src) {
     if (!dst || !src || dst == src) return 0;
-    if (timing_collect(src)) return -1;
+    if (timing_collect_into(dst, src)) {
+        dst->set_error(src->err);
+        return -1;
+    }
     for (auto& kv : src->kstats) {
         auto& s = dst->kstats[kv.first];
         s.first += kv.second.first;
         s.second += kv.second.second;
     }
     src->kstats.clear();
+    src->kspans.clear();
     return 0;
 }
 }  // namespace hm
@@ -216,6 +229,32 @@ int hm_timing_reset(hm_ctx* ctx) {
     if (!ctx) return -1;
     if (timing_collect(ctx)) return -1;
     ctx->kstats.clear();
+    ctx->kspans.clear();
+    if (!ctx->tref) HM_HIP_CHECK(ctx, hipEventCreate(&ctx->tref));
+    HM_HIP_CHECK(ctx, hipEventRecord(ctx->tref, ctx->stream));
+    return 0;
+}
+
+int hm_timing_get_busy(hm_ctx* ctx, const char* name, double* busy_ms) {
+    if (!ctx || !name || !busy_ms) return -1;
+    if (timing_collect(ctx)) return -1;
+    *busy_ms = 0.0;
+    auto it = ctx->kspans.find(name);
+    if (it == ctx->kspans.end()) return 0;
+    std::vector<std::pair<double, double>> v = it->second;
+    std::sort(v.begin(), v.end());
+    double lo = 0.0, hi = -1.0, sum = 0.0;
+    for (const auto& s : v) {
+        if (s.first > hi) {
+            if (hi > lo) sum += hi - lo;
+            lo = s.first;
+            hi = s.second;
+        } else if (s.second > hi) {
+            hi = s.second;
+        }
+    }
+    if (hi > lo) sum += hi - lo;
+    *busy_ms = sum;
     return 0;
 }
 

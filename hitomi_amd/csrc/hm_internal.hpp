@@ -25,6 +25,11 @@ struct hm_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, long long>> kstats;  // name -> (ms total, launches)
+    // each timed launch's [start, end) in ms after tref (recorded by
+    // hm_timing_reset): launches on other streams that overlap count once in
+    // the busy time
+    hipEvent_t tref = nullptr;
+    std::map<std::string, std::vector<std::pair<double, double>>> kspans;
     bool capturing = false;
     // launch shapes chosen by an autotuner: bit b of wide_mask set = the wide
     // column tile for the b-th tunable length (256, 288, 512)

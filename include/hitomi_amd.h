@@ -66,6 +66,11 @@ int hm_memset_zero(hm_ctx *ctx, void *d_ptr, size_t bytes);
 int hm_timing_enable(hm_ctx *ctx, int on);
 int hm_timing_get(hm_ctx *ctx, const char *name, double *total_ms, long long *launches);
 int hm_timing_reset(hm_ctx *ctx);
+/* Busy time of `name` since the last hm_timing_reset(): the length of the
+ * union of its launches' [start, end) intervals, so that launches overlapping
+ * on several streams (batch lanes, merged by hm_batch_collect_timing) count
+ * once.  No counterpart in the reference (a measurement aid). */
+int hm_timing_get_busy(hm_ctx *ctx, const char *name, double *busy_ms);
 /* Restrict the timed launches to one kernel name (NULL or "" times all). */
 int hm_timing_select(hm_ctx *ctx, const char *name);
 

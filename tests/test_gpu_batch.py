@@ -116,3 +116,33 @@ def test_lanes_refill_one_buffer(ctx):
         out.append(b.vectors())
         b.close()
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_lane_timing_busy_time(ctx, lanes):
+    """hm_timing_get_busy: the union of the timed launches' intervals, merged
+    over the lanes' streams.  One lane launches one kernel at a time, so the
+    busy time equals the summed durations; with three lanes it lies between
+    the sum over three and the sum."""
+    pos = mocks(ctx)
+    b = hm.Batch(ctx, 64, L, max_mocks=16, use_graph=False, lanes=lanes, **PLAN)
+    for p in pos[:lanes]:
+        b.push(p)
+    b.collect_timing()
+    ctx.timing(True)
+    ctx.timing_select("bin_pk")
+    ctx.timing_reset()
+    for p in pos:
+        b.push(p)
+    b.collect_timing()
+    total, n = ctx.kernel_time("bin_pk")
+    busy = ctx.kernel_busy("bin_pk")
+    ctx.timing(False)
+    ctx.timing_select(None)
+    b.close()
+    assert n > 0 and n % len(pos) == 0
+    assert 0.0 < busy <= total * (1.0 + 1e-6) + 1e-6
+    if lanes == 1:
+        assert busy >= total * (1.0 - 1e-6) - 1e-6
+    else:
+        assert busy >= total / lanes * (1.0 - 1e-6) - 1e-6
