@@ -1098,6 +1098,77 @@ __global__ __launch_bounds__(256) void k_yhist(const C<T>* __restrict__ xi, YHis
     }
 }
 
+// Shot noise of the survey three-point function with ELL > 0 (bispec.hpp:388-460,
+// 644-720, corr_for_3pcf in field.hpp:979-1113): the complex xi_LM on the full
+// mesh, weighted by A(r) = sum_c w_c Y(l1, m1_c)(r) Y(l2, m2_c)(r) (r with the
+// Nyquist index negative), binned by |r| like k_bin_xi_ang into per-row
+// partials (re, im) and, when count is set, mesh points per bin.
+struct ZetaYArgs {
+    int n[3], s[3], c[3];
+    double dr[3];
+    long long nrows;
+    const int2* f2b;
+    int nr;
+    int l1, l2, nc;
+    int m1[YH_MAXC], m2[YH_MAXC];
+    double w[YH_MAXC];
+    int count;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_xi_y(const C<T>* __restrict__ xi, ZetaYArgs a, double* __restrict__ part,
+                                                 long long* __restrict__ cnt) {
+#pragma clang fp contract(off)
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.nrows) return;
+    const int i = support_map((int)(row / a.c[1]), a.s[0], a.n[0]);
+    const int j = support_map((int)(row % a.c[1]), a.s[1], a.n[1]);
+    const double rx = mesh_vec(i, a.n[0], a.dr[0]), ry = mesh_vec(j, a.n[1], a.dr[1]);
+    const C<T>* line = xi + ((long long)i * a.n[1] + j) * (long long)a.n[2];
+    double* P = part + row * 2 * (long long)a.nr;
+    long long* Cn = cnt + row * (long long)a.nr;
+    for (int kc = 0; kc < a.c[2]; kc++) {
+        const int k = support_map(kc, a.s[2], a.n[2]);
+        const double rz = mesh_vec(k, a.n[2], a.dr[2]);
+        const double rmag = kmag3(rx, ry, rz);
+        const int ri = (int)(rmag / FINE_DR + 0.5);
+        if (ri >= FINE_NZ) continue;
+        const int2 b = a.f2b[ri];
+        if (b.x < 0) continue;
+        double ar = 0.0, ai = 0.0;
+        for (int q = 0; q < a.nc; q++) {
+            const Cplx y1 = ylm_red(a.l1, a.m1[q], rx, ry, rz);
+            const Cplx y2 = ylm_red(a.l2, a.m2[q], rx, ry, rz);
+            ar += a.w[q] * (y1.re * y2.re - y1.im * y2.im);
+            ai += a.w[q] * (y1.re * y2.im + y1.im * y2.re);
+        }
+        const C<T> x = line[k];
+        const double vr = ar * (double)x.x - ai * (double)x.y, vi = ar * (double)x.y + ai * (double)x.x;
+        for (int bb = b.x; bb <= b.y; bb++) {
+            P[2 * bb] += vr;
+            P[2 * bb + 1] += vi;
+            if (a.count) Cn[bb] += 1;
+        }
+    }
+}
+
+// (f / W) / V on the full mesh (the window correction of calcInverseFourierTransform)
+template <typename T>
+__global__ __launch_bounds__(256) void k_full_over_w(const C<T>* __restrict__ f, int n0, int n1, int n2, AxisTabs t,
+                                                    int wp, double inv_v, C<T>* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long ng = (long long)n0 * n1 * n2;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ng; e += st) {
+        const int k = (int)(e % n2);
+        const long long r = e / n2;
+        const int j = (int)(r % n1), i = (int)(r / n1);
+        const double W = wpow((t.wf[0][i] * t.wf[1][j]) * t.wf[2][k], wp);
+        const C<T> v = f[e];
+        out[e] = mk<T>((T)(((double)v.x / W) * inv_v), (T)(((double)v.y / W) * inv_v));
+    }
+}
+
 struct BoxPlanImpl {
     hm_box_config cfg;
     // launch shapes (the autotuner's choice; results do not depend on them)
@@ -2472,6 +2543,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     // 6'. three-point function (bispec.hpp:1205-1424)
     std::vector<double> zacc, zsum, zcntv;
     if (p->zeta) {
+        const long long n_rand = nr;
         const int nr = p->cfg.n_rbin;
         const int nf = p->z_gen ? 2 * nr : nr;
         const long long ng = m.ng();
@@ -2505,18 +2577,12 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             RowGeom rg{(long long)m.n[0] * m.n[1] * nfield, m.n[2], m.n[2], -1};
             return fft_rows<T>(ctx, nz, m.n[2], +1, out, out, rg);
         };
-        // (d/W)/V transformed back once (bispec.hpp:1308-1317)
-        bs.plain = 1;
-        if (full_inverse("zeta3_pass_x", "zeta3_pass_y", "zeta3_pass_z", Z3, 1)) return -1;
-        bs.plain = 0;
         const int nzp = (int)p->zpairs.size();
         const long long gblocks = (ng + p->gram_vpb - 1) / p->gram_vpb;
         double* gpart = (double*)p->zpart.p + 2 * (size_t)nr * (size_t)std::max(p->z_rows, (long long)1);
         zacc.assign(2 * (size_t)nzp, 0.0);
-        for (size_t q = 0; q < p->zmlist.size(); q++) {
-            bs.m[0] = p->zmlist[q];
-            bs.m[1] = -p->zmlist[q];
-            if (full_inverse("zeta_pass_x", "zeta_pass_y", "zeta_pass_z", ZF, nf)) return -1;
+        // sum_x F1 F2 Z3 over every r-bin pair of the fields in ZF, times wq, into zacc
+        auto zeta_gram = [&](double wq) -> int {
             for (int c0 = 0; c0 < nzp; c0 += 256) {
                 const int cnt = std::min(256, nzp - c0);
                 const short2* gp = (const short2*)p->zpairs_d.p + c0;
@@ -2541,8 +2607,33 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                 HM_HIP_CHECK(ctx, hipMemcpyAsync(part.data(), p->zred.p, sizeof(double) * part.size(),
                                                  hipMemcpyDeviceToHost, ctx->stream));
                 HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-                for (int e = 0; e < 2 * cnt; e++) zacc[2 * (size_t)c0 + e] += p->zwlist[q] * part[e];
+                for (int e = 0; e < 2 * cnt; e++) zacc[2 * (size_t)c0 + e] += wq * part[e];
             }
+            return 0;
+        };
+        ZetaSnArgs za;
+        for (int a = 0; a < 3; a++) {
+            za.n[a] = m.n[a];
+            za.s[a] = p->zrs[a];
+            za.c[a] = support_count(p->zrs[a], m.n[a]);
+            za.dr[a] = m.L[a] / (double)m.n[a];
+        }
+        za.nrows = p->z_rows;
+        za.f2b = (const int2*)p->zf2b_d.p;
+        za.nr = nr;
+        za.A = p->zang;
+        if (fill_zero(ctx, p->zpart.p, sizeof(double) * 2 * (size_t)nr * p->z_rows)) return -1;
+        if (fill_zero(ctx, p->zcnt.p, sizeof(long long) * (size_t)nr * p->z_rows)) return -1;
+        if (!svg) {
+        // (d/W)/V transformed back once (bispec.hpp:1308-1317)
+        bs.plain = 1;
+        if (full_inverse("zeta3_pass_x", "zeta3_pass_y", "zeta3_pass_z", Z3, 1)) return -1;
+        bs.plain = 0;
+        for (size_t q = 0; q < p->zmlist.size(); q++) {
+            bs.m[0] = p->zmlist[q];
+            bs.m[1] = -p->zmlist[q];
+            if (full_inverse("zeta_pass_x", "zeta_pass_y", "zeta_pass_z", ZF, nf)) return -1;
+            if (zeta_gram(p->zwlist[q])) return -1;
         }
         // shot noise: xi of (d conj(n) - N S)/W^2/V, weighted by A(mu), binned (field.hpp:979-1113)
         XiSrc<T> xs;
@@ -2559,19 +2650,6 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.dk2 = dk2;
         C<T>* xh = (C<T>*)p->xi_half.p;
         const int cx = support_count(p->zrs[0], m.n[0]), cy = support_count(p->zrs[1], m.n[1]);
-        ZetaSnArgs za;
-        for (int a = 0; a < 3; a++) {
-            za.n[a] = m.n[a];
-            za.s[a] = p->zrs[a];
-            za.c[a] = support_count(p->zrs[a], m.n[a]);
-            za.dr[a] = m.L[a] / (double)m.n[a];
-        }
-        za.nrows = p->z_rows;
-        za.f2b = (const int2*)p->zf2b_d.p;
-        za.nr = nr;
-        za.A = p->zang;
-        if (fill_zero(ctx, p->zpart.p, sizeof(double) * 2 * (size_t)nr * p->z_rows)) return -1;
-        if (fill_zero(ctx, p->zcnt.p, sizeof(long long) * (size_t)nr * p->z_rows)) return -1;
         for (int par = 0; par < ((p->z_odd || recon) ? 2 : 1); par++) {
             xs.odd = par;
             if (launch_cols<T, +1>(ctx, "zsn_pass_x", m.n[0], xs, xh,
@@ -2591,6 +2669,87 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                        (long long*)p->zcnt.p);
                 }))
                 return -1;
+        }
+        } else {
+            // survey three-point function with ELL > 0 (bispec.hpp:355-874): per M the Y_LM(los)-weighted
+            // fluctuation dn_LM; (dn_LM / W) / V transformed back is the third field of every (m1, m2) with
+            // m1 + m2 + M = 0, and xi_LM = IFFT[(dn_LM conj N_00 - SN_LM S)/W^2/V] weighted by
+            // sum_c w_c Y(l1, m1_c)(r) Y(l2, m2_c)(r) is the shot noise
+            const int L = wt->L;
+            const int n3[3] = {m.n[0], m.n[1], m.n[2]};
+            if ((!p->cf1.p && alloc(ctx, p, p->cf1, sizeof(C<T>) * (size_t)ng)) ||
+                (!p->cx.p && alloc(ctx, p, p->cx, sizeof(C<T>) * (size_t)ng)))
+                return -1;
+            C<T>* cf1 = (C<T>*)p->cf1.p;
+            C<T>* cxf = (C<T>*)p->cx.p;
+            const unsigned gb = (unsigned)std::max<long long>(1, std::min<long long>((ng + 255) / 256, 16384));
+            ZetaYArgs ya;
+            for (int a = 0; a < 3; a++) {
+                ya.n[a] = za.n[a];
+                ya.s[a] = za.s[a];
+                ya.c[a] = za.c[a];
+                ya.dr[a] = za.dr[a];
+            }
+            ya.nrows = za.nrows;
+            ya.f2b = za.f2b;
+            ya.nr = nr;
+            ya.l1 = p->zl[0];
+            ya.l2 = p->zl[1];
+            bool first = true;
+            for (int M = -L; M <= L; M++) {
+                ya.nc = 0;
+                for (int m1 = -p->zl[0]; m1 <= p->zl[0]; m1++) {
+                    const int m2 = -M - m1;
+                    if (m2 < -p->zl[1] || m2 > p->zl[1]) continue;
+                    double w = wigner_3j(p->zl[0], p->zl[1], L, 0, 0, 0) * wigner_3j(p->zl[0], p->zl[1], L, m1, m2, M);
+                    w *= (double)(2 * L + 1) * (double)(2 * p->zl[0] + 1) * (double)(2 * p->zl[1] + 1);
+                    if (std::fabs(w) < 1.0e-10) continue;
+                    if (ya.nc >= YH_MAXC) {
+                        ctx->set_error("survey three-point function: too many (m1, m2) terms");
+                        return -1;
+                    }
+                    ya.m1[ya.nc] = m1;
+                    ya.m2[ya.nc] = m2;
+                    ya.w[ya.nc] = w;
+                    ya.nc++;
+                }
+                if (ya.nc == 0) continue;
+                if (svg_field<T>(ctx, p, ag, wt, np, n_rand, M, 1, 0, wt->c1, wt->abs1, cf1)) return -1;
+                YXiArgs xa;
+                for (int a = 0; a < 3; a++) xa.n[a] = m.n[a];
+                xa.nzh = (int)nzh;
+                xa.t = tabs;
+                xa.wp = wp;
+                xa.sn_re = wt->snLM[2 * (M + L)];
+                xa.sn_im = wt->snLM[2 * (M + L) + 1];
+                xa.inv_v = 1.0 / m.volume();
+                if (ctx->launch("svz_xi_src", [&] {
+                        hipLaunchKernelGGL(k_ycross_xi<T>, dim3(gb), dim3(256), 0, ctx->stream, (const C<T>*)cf1,
+                                           (const C<T>*)dk2, xa, cxf);
+                    }))
+                    return -1;
+                if (fft3d_c2c<T>(ctx, cxf, n3, +1)) return -1;
+                ya.count = first ? 1 : 0;
+                first = false;
+                if (ctx->launch("svz_sn_bin", [&] {
+                        hipLaunchKernelGGL(k_bin_xi_y<T>, dim3((unsigned)((p->z_rows + 255) / 256)), dim3(256), 0,
+                                           ctx->stream, (const C<T>*)cxf, ya, (double*)p->zpart.p,
+                                           (long long*)p->zcnt.p);
+                    }))
+                    return -1;
+                if (ctx->launch("svz_third", [&] {
+                        hipLaunchKernelGGL(k_full_over_w<T>, dim3(gb), dim3(256), 0, ctx->stream, (const C<T>*)cf1,
+                                           m.n[0], m.n[1], m.n[2], tabs, wp, 1.0 / m.volume(), Z3);
+                    }))
+                    return -1;
+                if (fft3d_c2c<T>(ctx, Z3, n3, +1)) return -1;
+                for (int c = 0; c < ya.nc; c++) {
+                    bs.m[0] = ya.m1[c];
+                    bs.m[1] = ya.m2[c];
+                    if (full_inverse("zeta_pass_x", "zeta_pass_y", "zeta_pass_z", ZF, nf)) return -1;
+                    if (zeta_gram(ya.w[c])) return -1;
+                }
+            }
         }
         if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->zpart.p, p->z_rows, 2 * nr, (double*)p->zred.p))
             return -1;

@@ -223,11 +223,11 @@ __global__ __launch_bounds__(256) void k_survey_sn(const double* __restrict__ pD
     }
 }
 
-// sum_D w^3 Y(L, M)(los) and sum_R w^3 Y(L, M)(los) for M = -L..L (field.hpp:1202-1240): per block
-// [M + L][D re, D im, R re, R im], fixed order
+// sum_D w^pw Y(L, M)(los) and sum_R w^pw Y(L, M)(los) for M = -L..L (field.hpp:1202-1240; pw = 3, or 2
+// for the window): per block [M + L][D re, D im, R re, R im], fixed order
 __global__ __launch_bounds__(256) void k_survey_w3y(const double* __restrict__ pD, const double* __restrict__ wD,
                                                    long long nD, const double* __restrict__ pR,
-                                                   const double* __restrict__ wR, long long nR, int L,
+                                                   const double* __restrict__ wR, long long nR, int L, int pw,
                                                    double* __restrict__ part) {
 #pragma clang fp contract(off)
     __shared__ double sm[256];
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_survey_w3y(const double* __restrict__ p
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nD + nR; q += st) {
         const bool d = q < nD;
         const double* p = d ? pD + 3 * q : pR + 3 * (q - nD);
-        const double w3 = pow(d ? wD[q] : wR[q - nD], 3);
+        const double w3 = pow(d ? wD[q] : wR[q - nD], pw);
         double los[3];
         los_of(p, los);
         for (int M = -L; M <= L; M++) {
@@ -518,10 +518,8 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
     if (p->measure < HM_SURVEY_PK || p->measure > HM_SURVEY_ZETA_WINDOW) return fail("survey plan: unknown measure");
     if (delegated(p->measure)) {
         // calcBiSpectrum / calcThreePoint(Window)CorrelationFunction (bispec.hpp:4-874) on the box machinery:
-        // with ELL = 0 (no Y_LM(los) weight) the box estimators of the weighted catalogues; the bispectrum with
-        // ELL > 0 adds per M the Y_LM(los)-weighted third field and its shot-noise terms
-        if (cfg->ell != 0 && p->measure != HM_SURVEY_BK)
-            return fail("survey plan: the survey three-point functions need ELL = 0 in this build");
+        // with ELL = 0 (no Y_LM(los) weight) the box estimators of the weighted catalogues; ELL > 0 adds per
+        // M the Y_LM(los)-weighted third field and its shot-noise terms
         hm_box_config bc;
         std::memset(&bc, 0, sizeof(bc));
         bc.mesh = cfg->mesh;
@@ -534,7 +532,7 @@ SurveyPlanImpl* survey_create(hm_ctx* ctx, const hm_survey_config* cfg) {
         bc.n_rbin = cfg->n_rbin;
         bc.rmin = cfg->rmin;
         bc.rmax = cfg->rmax;
-        const int ells[3] = {cfg->ell1, cfg->ell2, p->measure == HM_SURVEY_BK ? cfg->ell : 0};
+        const int ells[3] = {cfg->ell1, cfg->ell2, cfg->ell};
         for (int a = 0; a < 3; a++) bc.bk_ell[a] = bc.zeta_ell[a] = ells[a];
         bc.bk = p->measure == HM_SURVEY_BK ? 1 : 0;
         bc.zeta = p->measure == HM_SURVEY_BK ? 0 : 1;
@@ -742,9 +740,9 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
             return -1;
     }
     if (delegated(p->measure)) {
-        // 3'. bispectrum / three-point function with ELL = 0 (bispec.hpp:4-874): w (D - alpha R) and
-        // w^2 (D + alpha^2 R), shot noise sum_D w^3 - alpha^3 sum_R w^3, or R alone with sum_R w^2 for the
-        // window; norm Vsurvey^2 / (sum w)^3 (particle.hpp:319-332)
+        // 3'. bispectrum / three-point function (bispec.hpp:4-874): w (D - alpha R) and w^2 (D + alpha^2 R),
+        // shot noise sum_D w^3 - alpha^3 sum_R w^3, or R alone with sum_R w^2 for the window; norm
+        // Vsurvey^2 / (sum w)^3 (particle.hpp:319-332); ELL > 0 weights the third field by Y_LM(los)
         const bool win = p->measure == HM_SURVEY_ZETA_WINDOW;
         BoxWeighted bw;
         const double sw = win ? swR : swD;
@@ -766,16 +764,20 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
             bw.sn3 = sw3D - pow(alpha, 3) * sw3R;
         }
         std::vector<double> snLM;
-        if (p->measure == HM_SURVEY_BK && p->l > 0) {
-            // Y_LM(los)-weighted shot noise of every M (field.hpp:1202-1240); lines of sight from the input
-            // (unshifted, unreconstructed) positions
+        if (p->l > 0) {
+            // Y_LM(los)-weighted shot noise of every M (field.hpp:1202-1240): sum_D w^3 Y - alpha^3 sum_R w^3 Y,
+            // or sum_R w^2 Y for the window; lines of sight from the input (unshifted, unreconstructed) positions
             const int L = p->l, nm = 2 * L + 1;
-            const unsigned nb = (unsigned)std::max<long long>(1, std::min<long long>((nD + nR + 255) / 256, 256));
+            const long long n1 = win ? nR : nD, n2 = win ? 0 : nR;
+            const double* p1 = win ? pR : pD;
+            const double* w1 = win ? wR : wD;
+            const double a3 = win ? 0.0 : pow(alpha, 3);
+            const unsigned nb = (unsigned)std::max<long long>(1, std::min<long long>((n1 + n2 + 255) / 256, 256));
             DevBuf yp;
             if (dev_alloc(ctx, yp, sizeof(double) * 4 * (size_t)nm * nb)) return -1;
             int rc = ctx->launch("survey_w3y", [&] {
-                hipLaunchKernelGGL(k_survey_w3y, dim3(nb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR, L,
-                                   (double*)yp.p);
+                hipLaunchKernelGGL(k_survey_w3y, dim3(nb), dim3(256), 0, ctx->stream, p1, w1, n1, pR, wR, n2, L,
+                                   win ? 2 : 3, (double*)yp.p);
             });
             std::vector<double> h(4 * (size_t)nm * nb);
             if (!rc && (hipMemcpyAsync(h.data(), yp.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream) !=
@@ -794,12 +796,12 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
                     rr += q[2];
                     ri += q[3];
                 }
-                snLM[2 * M] = dr - pow(alpha, 3) * rr;
-                snLM[2 * M + 1] = di - pow(alpha, 3) * ri;
+                snLM[2 * M] = dr - a3 * rr;
+                snLM[2 * M + 1] = di - a3 * ri;
             }
             bw.L = L;
-            bw.losD = pD;
-            bw.losR = pR;
+            bw.losD = p1;
+            bw.losR = win ? nullptr : pR;
             bw.snLM = snLM.data();
         }
         hm_box_result res;

@@ -6,11 +6,9 @@
 // reference's formats, and exits with status 1 where the reference calls
 // EXIT().  The measurement runs on the GPU through libhitomi_amd.so.
 //
-// Supported: measure = pk, 2PCF and window2PCF (ell1 = ELL, ell2 = 0), bk
-// for any (ell1, ell2, ELL) that wigner_3j allows, and 3PCF and window3PCF
-// with ELL = 0, each with or without reconstruction (flag_recon).  The
-// three-point functions with ELL > 0 are rejected with an error instead of
-// being computed differently.
+// Supported: measure = pk, 2PCF and window2PCF (ell1 = ELL, ell2 = 0), and
+// bk, 3PCF and window3PCF for any (ell1, ell2, ELL) that wigner_3j allows,
+// each with or without reconstruction (flag_recon).
 #include "../../include/hitomi_amd.h"
 #include "hm_param.hpp"
 #include "hm_survey_catalog.hpp"
@@ -150,8 +148,11 @@ int main(int argc, char* argv[]) {
         EXIT();
     }
     const bool three = measure >= HM_SURVEY_BK;
-    if (three && measure != HM_SURVEY_BK && p.ELL != 0) {
-        printf("measure = %s with ELL = %d is not available in this build (ELL = 0 only).\n", p.measure.c_str(), p.ELL);
+    // bispec.hpp:12-18, 363-369, 624-630: wigner_3j(ell1, ell2, ELL, 0, 0, 0) vanishes unless
+    // ell1 + ell2 + ELL is even and |ell1 - ell2| <= ELL <= ell1 + ell2
+    if (three && (((p.ell1 + p.ell2 + p.ELL) & 1) || p.ELL < std::abs(p.ell1 - p.ell2) || p.ELL > p.ell1 + p.ell2)) {
+        printf("This multipole combination is not allowed.\n");
+        printf("It should be wigner_3j(ell1, ell2, ELL, 0,0,0) != 0\n");
         printf("%s\n", fails.c_str());
         EXIT();
     }

@@ -226,8 +226,8 @@ enum {
     HM_SURVEY_XI = 1,        /* measure = 2PCF: calcTwoPointCorrelationFunction (powerspec.hpp:93-179) */
     HM_SURVEY_XI_WINDOW = 2, /* measure = window2PCF: calcTwoPointWindowCorrelationFunction (powerspec.hpp:182-270) */
     HM_SURVEY_BK = 3,        /* measure = bk: calcBiSpectrum (bispec.hpp:4-353), any (ell1, ell2, ELL) */
-    HM_SURVEY_ZETA = 4,      /* measure = 3PCF: calcThreePointCorrelationFunction (bispec.hpp:355-614), ELL = 0 */
-    HM_SURVEY_ZETA_WINDOW = 5 /* measure = window3PCF: calcThreePointWindowCorrelationFunction (bispec.hpp:617-874), ELL = 0 */
+    HM_SURVEY_ZETA = 4,      /* measure = 3PCF: calcThreePointCorrelationFunction (bispec.hpp:355-614), any (ell1, ell2, ELL) */
+    HM_SURVEY_ZETA_WINDOW = 5 /* measure = window3PCF: calcThreePointWindowCorrelationFunction (bispec.hpp:617-874), any (ell1, ell2, ELL) */
 };
 
 typedef struct {

@@ -169,15 +169,19 @@ def test_survey_missing_randoms_exits_1(tmp_path):
     assert r.returncode == 1 and "Cannot open the file" in r.stdout and "'P_R.readParticleBOSS' fails." in r.stdout
 
 
-@pytest.mark.parametrize("measure", ["3PCF", "window3PCF"])
-def test_survey_unsupported_measure_exits_1(tmp_path, measure):
-    """The survey three-point functions with ELL > 0 are rejected, not computed differently."""
+@pytest.mark.parametrize("measure", ["bk", "3PCF", "window3PCF"])
+@pytest.mark.parametrize("ells", [(1, 0, 0), (2, 0, 1), (1, 1, 3)])
+def test_survey_disallowed_multipole_exits_1(tmp_path, measure, ells):
+    """bispec.hpp:12-18, 363-369, 624-630: wigner_3j(ell1, ell2, ELL, 0, 0, 0) = 0 is rejected with the
+    reference's message before any GPU work."""
     write_xyzw(tmp_path / "data.dat", np.ones((3, 3)) * 500.0, np.ones(3))
     ini = tmp_path / "p.ini"
     hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="data.dat",
-                  output_dir=str(tmp_path / "out"), measure=measure, ell1=2, ell2=0, ELL=2)
+                  output_dir=str(tmp_path / "out"), measure=measure, ell1=ells[0], ell2=ells[1], ELL=ells[2])
     r = run_survey(ini, tmp_path)
-    assert r.returncode == 1 and "not available" in r.stdout
+    assert r.returncode == 1
+    assert "This multipole combination is not allowed." in r.stdout
+    assert "It should be wigner_3j(ell1, ell2, ELL, 0,0,0) != 0" in r.stdout
 
 
 @pytest.mark.gpu

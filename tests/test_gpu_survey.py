@@ -227,6 +227,40 @@ def test_survey_bispectrum_ell_nonzero(ctx, prec, ells):
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
 
 
-def test_survey_three_point_rejects_ell_nonzero(ctx):
+@pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
+@pytest.mark.parametrize("measure", ["3PCF", "window3PCF"])
+@pytest.mark.parametrize("ells", [(2, 0, 2), (1, 1, 2), (1, 0, 1)])
+def test_survey_zeta_ell_nonzero(ctx, prec, measure, ells):
+    """calcThreePoint(Window)CorrelationFunction (bispec.hpp:355-874) with ELL > 0 for every ith_rbin: per M
+    the Y_LM(los)-weighted third field and the shot noise of xi_LM weighted by Y_l1m1(r) Y_l2m2(r);
+    |d norm (zeta - sn)| <= tol * max of it, tol = 1e-9 (FP64) / 1e-4 (FP32).  The shot noise (diagonal
+    only, ~0.2% of zeta here) is checked on its own: on the diagonal, the GPU's norm (zeta - sn) minus the
+    oracle's norm zeta is -norm sn within 1e-5 (FP64) / 0.2 (FP32) of |norm sn|."""
+    l1, l2, L_ = ells
+    n, L = 32, 800.0
+    rw = dict(rmin=60.0, rmax=240.0, n_rbin=4)
+    cat = get_cat("patch")
+    win = measure == "window3PCF"
+    plan = hm.SurveyPlan(ctx, n, L, "TSC", ell=L_, ell1=l1, ell2=l2, precision=prec, measure=measure, **rw)
+    got, _, info = plan.execute(*cat)
+    plan.close()
+    nr = rw["n_rbin"]
+    ref = np.zeros((nr, nr), complex)
+    zs, sns = np.zeros(nr, complex), np.zeros(nr, complex)
+    for ith in range(nr):
+        p = po.param(n, L, po.TSC, ell1=l1, ell2=l2, ELL=L_, ith_rbin=ith, **rw)
+        s = po.Survey(p, *cat)
+        ref[ith], z, sn = s.zeta(p, window=win)
+        zs[ith], sns[ith] = z[ith], sn[ith]
+    tol = 1e-9 if prec == hm.FP64 else 1e-4
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    norm = info["norm"]
+    sn_got = norm * zs - np.diag(got)
+    stol = 1e-5 if prec == hm.FP64 else 0.2
+    assert np.abs(sn_got - norm * sns).max() <= stol * np.abs(norm * sns).max()
+
+
+def test_survey_three_point_rejects_disallowed_multipole(ctx):
+    """wigner_3j(ell1, ell2, ELL, 0, 0, 0) = 0 (bispec.hpp:363-369) is rejected."""
     with pytest.raises(RuntimeError):
-        hm.SurveyPlan(ctx, 32, 800.0, "TSC", ell=2, ell1=2, ell2=0, measure="3PCF", rmin=60.0, rmax=240.0, n_rbin=4)
+        hm.SurveyPlan(ctx, 32, 800.0, "TSC", ell=1, ell1=1, ell2=1, measure="3PCF", rmin=60.0, rmax=240.0, n_rbin=4)
