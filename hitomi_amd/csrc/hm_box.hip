@@ -651,18 +651,20 @@ template <typename T> struct ShellMaskSrc {
 
 // Fused last (z) transform + triple sums.  A workgroup takes rows (x, y) of
 // the M mesh one at a time: the in-support z coefficients of every shell and
-// of delta/W are loaded into LDS (24 lines, delta/W on the last), transformed
-// there, and sum_z F_a F_3 F_b is accumulated for 4 x 4 blocks of shell pairs
-// (block (bi, bj), bi <= bj), each block split over z among `nsl` lanes.  The
-// fields never return to HBM.
-constexpr int TRI_B = 24;
+// of delta/W are loaded into LDS (NB lines, delta/W on the last: NB = 21 for
+// up to 20 shells, else 24), transformed there, and sum_z F_a F_3 F_b is
+// accumulated for 4 x 4 blocks of shell pairs (block (bi, bj), bi <= bj), each
+// block split over z among `nsl` lanes.  The fields never return to HBM.
+constexpr int TRI_B = 24;  // most lines of a tile (up to 23 shells)
 constexpr int TRI_RPW = 16;  // rows per workgroup
 
-template <int L> struct TriFits { static constexpr bool value = (size_t)TRI_B * LdsPad<L, false>::LP * sizeof(float2) <= 65536; };
+template <int L> struct TriFits {
+    static constexpr bool value = ((size_t)TRI_B * LdsPad<L, false>::LP + L) * sizeof(float2) <= 65536;
+};
 
 constexpr int TRI_NT = 512;  // lanes per workgroup of the fused kernel
 
-template <typename T, int L>
+template <typename T, int L, int NB>
 __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ F, long long fstride, int nk,
                                                    const int* __restrict__ zsup, int s_f3, long long nrows, int ngrp,
                                                    int nblk, int nsl, const C<T>* __restrict__ tw,
@@ -691,7 +693,7 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
     const long long r1 = (r0 + TRI_RPW < nrows) ? r0 + TRI_RPW : nrows;
     // the next row's in-support coefficients are loaded into registers while
     // the current row is transformed and reduced
-    constexpr int NPRE = (TRI_B * L + TRI_NT - 1) / TRI_NT;
+    constexpr int NPRE = (NB * L + TRI_NT - 1) / TRI_NT;
     static_assert(NPRE <= 32, "one mask bit per staged element");
     C<T> pre[NPRE];
     // which of this lane's staged elements lie in their field's z support
@@ -702,34 +704,43 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
         const int e = t + q * TRI_NT;
         const int f = e / L, z = e % L;
         bool in = false;
-        if (e < TRI_B * L) {
+        if (e < NB * L) {
             if (f < nk) in = in_support(z, zsup[f], L);
-            else if (f == TRI_B - 1) in = in_support(z, s_f3, L);
+            else if (f == NB - 1) in = in_support(z, s_f3, L);
         }
         need |= (unsigned)in << q;
     }
-    auto load_row = [&](long long r) {
+    // tl: the lane index passed through an empty asm statement once per row, so
+    // that the compiler recomputes the row-invariant LDS, twiddle and load
+    // addresses inside the loop instead of holding them in registers across
+    // it (that would cost occupancy)
+    auto load_row = [&](long long r, int tl) {
 #pragma unroll
         for (int q = 0; q < NPRE; q++) {
-            const int e = t + q * TRI_NT;
+            const int e = tl + q * TRI_NT;
             const int f = e / L, z = e % L;
             C<T> v = mk<T>((T)0, (T)0);
             if ((need >> q) & 1u) v = F[(long long)(f < nk ? f : nk) * fstride + r * L + z];
             pre[q] = v;
         }
     };
-    if (r0 < r1) load_row(r0);
+    // the twiddle table in LDS after the tile (read by every stage of every row)
+    C<T>* stw = sm + NB * LP;
+    for (int e = t; e < L; e += TRI_NT) stw[e] = tw[e];
+    if (r0 < r1) load_row(r0, t);
     for (long long r = r0; r < r1; r++) {
+        int tl = t;
+        asm volatile("" : "+v"(tl));
 #pragma unroll
         for (int q = 0; q < NPRE; q++) {
-            const int e = t + q * TRI_NT;
-            if (e < TRI_B * L) sm[(e / L) * LP + e % L] = pre[q];
+            const int e = tl + q * TRI_NT;
+            if (e < NB * L) sm[(e / L) * LP + e % L] = pre[q];
         }
         __syncthreads();
-        if (r + 1 < r1) load_row(r + 1);
-        lds_fft<T, +1, L, TRI_B, TRI_NT, false>(sm, tw);
+        if (r + 1 < r1) load_row(r + 1, tl);
+        lds_fft<T, +1, L, NB, TRI_NT, false>(sm, stw, tl);
         if (active) {
-            const C<T>* f3 = sm + (TRI_B - 1) * LP;
+            const C<T>* f3 = sm + (NB - 1) * LP;
             const C<T>* fa = sm + 4 * bi * LP;
             const C<T>* fb = sm + 4 * bj * LP;
             for (int z = sl; z < L; z += nsl) {
@@ -777,15 +788,20 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
 template <typename T, int L>
 static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, int nk, const int* zsup, int s_f3,
                                long long nrows, int ngrp, int nblk, int nsl, double* part) {
-    const size_t fft_lds = (size_t)TRI_B * LdsPad<L, false>::LP * sizeof(C<T>);
+    const int nb = nk <= 20 ? 21 : TRI_B;
+    const size_t fft_lds = ((size_t)nb * LdsPad<L, false>::LP + L) * sizeof(C<T>);
     const size_t red_lds = (size_t)TRI_NT * 4 * sizeof(double);
     const size_t lds = fft_lds > red_lds ? fft_lds : red_lds;
     const C<T>* tw = ctx->twiddle<T>(L);
     if (!tw) return -1;
     const unsigned blocks = (unsigned)((nrows + TRI_RPW - 1) / TRI_RPW);
     return ctx->launch("tri_z_gram", [&] {
-        hipLaunchKernelGGL((k_tri_z_gram<T, L>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride, nk, zsup, s_f3,
-                           nrows, ngrp, nblk, nsl, tw, part);
+        if (nb == 21)
+            hipLaunchKernelGGL((k_tri_z_gram<T, L, 21>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride, nk,
+                               zsup, s_f3, nrows, ngrp, nblk, nsl, tw, part);
+        else
+            hipLaunchKernelGGL((k_tri_z_gram<T, L, TRI_B>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride,
+                               nk, zsup, s_f3, nrows, ngrp, nblk, nsl, tw, part);
     });
 }
 
