@@ -207,9 +207,12 @@ constexpr int pow2floor(int x) {
 }
 
 // Tile width (lines per workgroup) for a column pass: up to 128 B per point,
-// a power of two, within 64 KB of LDS.
+// a power of two, within 64 KB of LDS.  FP32 lines of 288 points take 8
+// columns (64 B per point): the triple mesh's shell passes run faster with the
+// smaller tiles (more workgroups in flight per CU); the autotuner can still
+// choose 16 (the wide tile).
 template <typename T, int L> struct ColTile {
-    static constexpr int want = (int)(128 / sizeof(C<T>));
+    static constexpr int want = (sizeof(T) == 4 && L == 288) ? 8 : (int)(128 / sizeof(C<T>));
     static constexpr int fit = (int)(65536 / (LdsPad<L>::LP * sizeof(C<T>)));
     static constexpr int B = pow2floor(want < fit ? want : fit);
 };
