@@ -268,8 +268,9 @@ __global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ sta
     }
     __syncthreads();
     const int total = roff[NR];
+    // j only grows, so the row search resumes where the previous particle's ended
+    int r = 0;
     for (int j = threadIdx.x; j < total; j += 256) {
-        int r = 0;
         while (roff[r + 1] <= j) r++;
         const long long p = (long long)rbeg[r] + (j - roff[r]);
         double wt[3][3];
