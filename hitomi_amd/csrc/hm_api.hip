@@ -75,8 +75,10 @@ hm_ctx* hm_ctx_create(int device, int flags) {
     if (!ctx) return nullptr;
     ctx->device = device;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
-        ctx->lds_max = (size_t)prop.sharedMemPerBlock;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        if (prop.sharedMemPerBlock > 0) ctx->lds_max = (size_t)prop.sharedMemPerBlock;
+        if (prop.multiProcessorCount > 0) ctx->cu_count = prop.multiProcessorCount;
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return nullptr;

@@ -656,7 +656,24 @@ template <typename T> struct ShellMaskSrc {
 // accumulated for 4 x 4 blocks of shell pairs (block (bi, bj), bi <= bj), each
 // block split over z among `nsl` lanes.  The fields never return to HBM.
 constexpr int TRI_B = 24;  // most lines of a tile (up to 23 shells)
-constexpr int TRI_RPW = 16;  // rows per workgroup
+// Rows per workgroup of the fused kernel.  Two workgroups fit a CU, so the
+// grid runs in rounds of 2 x CUs workgroups, and a partly filled last round
+// costs a full one: take the count in [12, 24] with the fewest rows per CU over
+// all rounds (18 for 288^2 rows on 256 CUs).  A workgroup's FP32 partial sums
+// depend on it, so results are deterministic per device.
+static int tri_rows_per_wg(long long nrows, int cus) {
+    const long long slots = 2LL * (cus > 0 ? cus : 1);
+    int best = 16;
+    long long best_cost = -1;
+    for (int r = 12; r <= 24; r++) {
+        const long long rounds = ((nrows + r - 1) / r + slots - 1) / slots;
+        if (best_cost < 0 || rounds * r < best_cost) {
+            best_cost = rounds * r;
+            best = r;
+        }
+    }
+    return best;
+}
 
 template <int L> struct TriFits {
     static constexpr bool value = ((size_t)TRI_B * LdsPad<L, false>::LP + L) * sizeof(float2) <= 65536;
@@ -667,7 +684,7 @@ constexpr int TRI_NT = 512;  // lanes per workgroup of the fused kernel
 template <typename T, int L, int NB>
 __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ F, long long fstride, int nk,
                                                    const int* __restrict__ zsup, int s_f3, long long nrows, int ngrp,
-                                                   int nblk, int nsl, const C<T>* __restrict__ tw,
+                                                   int nblk, int nsl, int rpw, const C<T>* __restrict__ tw,
                                                    double* __restrict__ part) {
     constexpr int LP = LdsPad<L, false>::LP;
     extern __shared__ unsigned char smraw[];
@@ -684,13 +701,13 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
         }
         bj = bi + q;
     }
-    // FP32 sums over the workgroup's rows (TRI_RPW rows x L / nsl voxels per
+    // FP32 sums over the workgroup's rows (rpw rows x L / nsl voxels per
     // lane), FP64 from the slice reduction on
     T ar[16], ai[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) ar[q] = ai[q] = (T)0;
-    const long long r0 = (long long)blockIdx.x * TRI_RPW;
-    const long long r1 = (r0 + TRI_RPW < nrows) ? r0 + TRI_RPW : nrows;
+    const long long r0 = (long long)blockIdx.x * rpw;
+    const long long r1 = (r0 + rpw < nrows) ? r0 + rpw : nrows;
     // the next row's in-support coefficients are loaded into registers while
     // the current row is transformed and reduced
     constexpr int NPRE = (NB * L + TRI_NT - 1) / TRI_NT;
@@ -787,21 +804,21 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
 
 template <typename T, int L>
 static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, int nk, const int* zsup, int s_f3,
-                               long long nrows, int ngrp, int nblk, int nsl, double* part) {
+                               long long nrows, int ngrp, int nblk, int nsl, int rpw, double* part) {
     const int nb = nk <= 20 ? 21 : TRI_B;
     const size_t fft_lds = ((size_t)nb * LdsPad<L, false>::LP + L) * sizeof(C<T>);
     const size_t red_lds = (size_t)TRI_NT * 4 * sizeof(double);
     const size_t lds = fft_lds > red_lds ? fft_lds : red_lds;
     const C<T>* tw = ctx->twiddle<T>(L);
     if (!tw) return -1;
-    const unsigned blocks = (unsigned)((nrows + TRI_RPW - 1) / TRI_RPW);
+    const unsigned blocks = (unsigned)((nrows + rpw - 1) / rpw);
     return ctx->launch("tri_z_gram", [&] {
         if (nb == 21)
             hipLaunchKernelGGL((k_tri_z_gram<T, L, 21>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride, nk,
-                               zsup, s_f3, nrows, ngrp, nblk, nsl, tw, part);
+                               zsup, s_f3, nrows, ngrp, nblk, nsl, rpw, tw, part);
         else
             hipLaunchKernelGGL((k_tri_z_gram<T, L, TRI_B>), dim3(blocks), dim3(TRI_NT), lds, ctx->stream, F, fstride,
-                               nk, zsup, s_f3, nrows, ngrp, nblk, nsl, tw, part);
+                               nk, zsup, s_f3, nrows, ngrp, nblk, nsl, rpw, tw, part);
     });
 }
 
@@ -820,13 +837,13 @@ static int tri_fused_ok(int prec, int L) {
 
 template <typename T>
 static int launch_tri_z_gram(hm_ctx* ctx, int L, const C<T>* F, long long fstride, int nk, const int* zsup, int s_f3,
-                             long long nrows, int ngrp, int nblk, int nsl, double* part) {
+                             long long nrows, int ngrp, int nblk, int nsl, int rpw, double* part) {
     if constexpr (sizeof(T) == sizeof(float)) {
         switch (L) {
 #define HM_CASE(LL)                                                                                                    \
     case LL:                                                                                                           \
         if constexpr (TriFits<LL>::value)                                                                              \
-            return launch_tri_z_gram_L<T, LL>(ctx, F, fstride, nk, zsup, s_f3, nrows, ngrp, nblk, nsl, part);          \
+            return launch_tri_z_gram_L<T, LL>(ctx, F, fstride, nk, zsup, s_f3, nrows, ngrp, nblk, nsl, rpw, part);     \
         break;
             HM_POW2_LENGTHS(HM_CASE)
             HM_MIXED_LENGTHS(HM_CASE)
@@ -1214,6 +1231,7 @@ struct BoxPlanImpl {
     int gram_vpb = 4096;
     bool fused = false;  // z transform of the triple fields fused with the triple sums
     int ngrp = 0, nblk = 0, nsl = 0;  // 4-shell groups, group pairs, z slices per pair block (fused)
+    int tri_rpw = 16;                 // rows per workgroup of the fused kernel
     int gram_cols = 0;                // doubles per partial row of the triple sums
     DevBuf sid;                       // char2 first/last shell per mode of the shell support cube
     DevBuf rpos, rw;                  // D then R positions and weights of a reconstructed box
@@ -1302,7 +1320,8 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
         p->fused = !p->bk_gen && tri_fused_ok(p->cfg.precision, p->M[2]) && p->nk < TRI_B;
         p->gram_cols = 2 * std::min((int)p->pairs.size(), 256);
         if (p->fused) {
-            p->gram_blocks = ((long long)p->M[0] * p->M[1] + TRI_RPW - 1) / TRI_RPW;
+            p->tri_rpw = tri_rows_per_wg((long long)p->M[0] * p->M[1], ctx->cu_count);
+            p->gram_blocks = ((long long)p->M[0] * p->M[1] + p->tri_rpw - 1) / p->tri_rpw;
             p->ngrp = (p->nk + 3) / 4;
             p->nblk = p->ngrp * (p->ngrp + 1) / 2;
             p->nsl = TRI_NT / p->nblk;
@@ -2536,7 +2555,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             const long long nrows = (long long)p->M[0] * p->M[1];
             if (p->fused) {
                 if (launch_tri_z_gram<T>(ctx, p->M[2], F, nvox, p->nk, fsz, st.s3[2], nrows, p->ngrp, p->nblk, p->nsl,
-                                         (double*)p->gram_part.p))
+                                         p->tri_rpw, (double*)p->gram_part.p))
                     return -1;
             } else {
                 RowGeom rg{nrows * p->nk, p->M[2], p->M[2], st.s[2], fsz, nrows};

@@ -35,6 +35,7 @@ struct hm_ctx {
     // column tile for the b-th tunable length (256, 288, 512)
     unsigned wide_mask = 0;
     size_t lds_max = 65536;  // dynamic LDS per workgroup the device allows
+    int cu_count = 256;      // compute units of the device
     bool conv_lds = true;    // convolutions of the learned reconstruction with LDS-staged input tiles
     bool wide_cols(int L) const {
         const int b = L == 256 ? 0 : (L == 288 ? 1 : (L == 512 ? 2 : -1));
