@@ -210,4 +210,38 @@ template <typename V> inline int reduce_cols(hm_ctx* ctx, const char* name, cons
     });
 }
 
+// The same sums over partials stored column-major (column c at c * nrows):
+// one workgroup of 1024 lanes per column, each lane with four independent
+// running sums over rows r = lane + 1024 (4 i + u) (fixed order), so that many
+// contiguous loads are in flight.
+template <typename V>
+__global__ __launch_bounds__(1024) void k_reduce_cols_cm(const V* __restrict__ part, long long nrows,
+                                                        V* __restrict__ out) {
+    __shared__ V sm[1024];
+    const int c = blockIdx.x;
+    const V* col = part + (long long)c * nrows;
+    V acc[4] = {0, 0, 0, 0};
+    long long r = threadIdx.x;
+    for (; r + 3 * 1024 < nrows; r += 4 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc[u] += col[r + u * 1024];
+    }
+    for (int u = 0; r < nrows; r += 1024, u++) acc[u] += col[r];
+    sm[threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sm[0];
+}
+
+template <typename V>
+inline int reduce_cols_cm(hm_ctx* ctx, const char* name, const V* part, long long nrows, int ncols, V* out) {
+    if (ncols <= 0) return 0;
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL(k_reduce_cols_cm<V>, dim3((unsigned)ncols), dim3(1024), 0, ctx->stream, part, nrows, out);
+    });
+}
+
 }  // namespace hm

@@ -50,6 +50,9 @@ struct BinArgs {
     double sn;
     int sn_w2 = 0;  // 1: shot-noise shape W(k)^2 (interlaced field) instead of the aliased S(k)
     const double* sn_ptr = nullptr;  // set: the shot noise read from device memory instead of sn
+    int kc = 1;                      // kz chunks per row, each with its own partial sums (kcl kz each)
+    int kcl = 0;
+    long long cm = 0;  // > 0: partials column-major, column c of lane t at c * cm + t (else t * columns + c)
 };
 
 template <typename T>
@@ -60,17 +63,25 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
                                                 long long* __restrict__ cnt, const C<T>* __restrict__ dk2 = nullptr,
                                                 double* __restrict__ part_im = nullptr) {
 #pragma clang fp contract(off)
-    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= a.nrows) return;
+    // one lane per (row, kz chunk); partial sums at [row * kc + chunk]
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= a.nrows * a.kc) return;
+    const long long row = tid / a.kc;
+    const int ch = (int)(tid % a.kc);
+    const int kcl = a.kc > 1 ? a.kcl : a.nzmax + 1;
+    const int kz0 = ch * kcl, kz1 = (kz0 + kcl - 1 < a.nzmax) ? kz0 + kcl - 1 : a.nzmax;
     const double sn = a.sn_ptr ? *a.sn_ptr : a.sn;
     const int i = support_map((int)(row / a.nrow_y), a.s01, a.n[0]);
     const int j = support_map((int)(row % a.nrow_y), a.s01, a.n[1]);
-    double* P = part + row * (long long)a.nk * a.ne;
-    double* Pi = part_im ? part_im + row * (long long)a.nk * a.ne : nullptr;
-    long long* Cn = cnt + row * (long long)a.nk;
+    // element c of this lane's partial sums: strided by cm (column-major) or contiguous
+    const long long ps = a.cm ? a.cm : 1;
+    const long long pb = a.cm ? tid : tid * (long long)a.nk * a.ne, cb = a.cm ? tid : tid * (long long)a.nk;
+    double* P = part + pb;
+    double* Pi = part_im ? part_im + pb : nullptr;
+    long long* Cn = cnt + cb;
     const long long rbase = ((long long)i * a.n[1] + j) * a.nzh;
     const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
-    for (int kz = 0; kz <= a.nzmax; kz++) {
+    for (int kz = kz0; kz <= kz1; kz++) {
         const C<T> d = dk[rbase + kz];
         const double re = (double)d.x, imv = (double)d.y;
         double p2 = re * re + imv * imv, pim = 0.0;  // f1 conj(f2), f1 = f2
@@ -97,10 +108,10 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
                 for (int e = 0; e < a.ne; e++) {
                     const int l = a.ells[e];
                     const double y = (l == 0) ? 1.0 : ((fabs(kmag) < 1.0e-15) ? 0.0 : legendre(l, mu));
-                    P[bb * a.ne + e] += val * y;
-                    if (Pi) Pi[bb * a.ne + e] += vim * y;
+                    P[(bb * a.ne + e) * ps] += val * y;
+                    if (Pi) Pi[(bb * a.ne + e) * ps] += vim * y;
                 }
-                Cn[bb] += 1;
+                Cn[bb * ps] += 1;
             }
         }
     }
@@ -1232,6 +1243,7 @@ struct BoxPlanImpl {
     bool fused = false;  // z transform of the triple fields fused with the triple sums
     int ngrp = 0, nblk = 0, nsl = 0;  // 4-shell groups, group pairs, z slices per pair block (fused)
     int tri_rpw = 16;                 // rows per workgroup of the fused kernel
+    int bin_kc = 1;                   // kz chunks per row of the P_l binning
     int gram_cols = 0;                // doubles per partial row of the triple sums
     DevBuf sid;                       // char2 first/last shell per mode of the shell support cube
     DevBuf rpos, rw;                  // D then R positions and weights of a reconstructed box
@@ -1289,8 +1301,11 @@ template <typename T> static int plan_alloc(hm_ctx* ctx, BoxPlanImpl* p) {
     if (alloc(ctx, p, p->fixed, sizeof(long long) * (size_t)m.ng())) return -1;
     if (alloc(ctx, p, p->dk, sizeof(C<T>) * (size_t)m.nh())) return -1;
     const size_t ncols = (size_t)p->nk * (size_t)std::max(p->ne, 1);
-    if (alloc(ctx, p, p->pk_part, sizeof(double) * ncols * (size_t)p->nrows)) return -1;
-    if (alloc(ctx, p, p->pk_cnt, sizeof(long long) * (size_t)p->nk * (size_t)p->nrows)) return -1;
+    // kz chunks of the P_l binning: enough lanes to fill the device, 16+ kz per chunk
+    p->bin_kc = std::max(1, std::min(8, (p->nzmax + 1) / 16));
+    const size_t prow = (size_t)p->nrows * (size_t)p->bin_kc;
+    if (alloc(ctx, p, p->pk_part, sizeof(double) * ncols * prow)) return -1;
+    if (alloc(ctx, p, p->pk_cnt, sizeof(long long) * (size_t)p->nk * prow)) return -1;
     if (alloc(ctx, p, p->pk_red, sizeof(double) * ncols)) return -1;
     if (alloc(ctx, p, p->pk_cnt_red, sizeof(long long) * (size_t)p->nk)) return -1;
     if (p->cfg.bk) {
@@ -2126,19 +2141,26 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     const int ncols = p->nk * ba.ne;
     const bool do_pk = p->ne > 0;
     if (do_pk) {
-    if (fill_zero(ctx, p->pk_part.p, sizeof(double) * (size_t)ncols * p->nrows)) return -1;
-    if (fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * p->nrows)) return -1;
+    ba.kc = p->bin_kc;
+    ba.kcl = (p->nzmax + 1 + p->bin_kc - 1) / p->bin_kc;
+    const long long prow = p->nrows * p->bin_kc;
+    ba.cm = prow;
+    if (fill_zero(ctx, p->pk_part.p, sizeof(double) * (size_t)ncols * prow)) return -1;
+    if (fill_zero(ctx, p->pk_cnt.p, sizeof(long long) * (size_t)p->nk * prow)) return -1;
     if (ctx->launch("bin_pk", [&] {
             const int bb = p->tune_bin;
-            hipLaunchKernelGGL(k_bin_box<T>, dim3((unsigned)((p->nrows + bb - 1) / bb)), dim3(bb), 0, ctx->stream, dk,
-                               ba, (double*)p->pk_part.p, (long long*)p->pk_cnt.p);
+            hipLaunchKernelGGL(k_bin_box<T>, dim3((unsigned)((prow + bb - 1) / bb)), dim3(bb), 0, ctx->stream, dk, ba,
+                               (double*)p->pk_part.p, (long long*)p->pk_cnt.p);
         }))
         return -1;
-    if (reduce_cols<double>(ctx, "bin_reduce", (const double*)p->pk_part.p, p->nrows, ncols, (double*)p->pk_red.p))
+    if (reduce_cols_cm<double>(ctx, "bin_reduce", (const double*)p->pk_part.p, prow, ncols, (double*)p->pk_red.p))
         return -1;
-    if (reduce_cols<long long>(ctx, "bin_reduce", (const long long*)p->pk_cnt.p, p->nrows, p->nk,
-                               (long long*)p->pk_cnt_red.p))
+    if (reduce_cols_cm<long long>(ctx, "bin_reduce", (const long long*)p->pk_cnt.p, prow, p->nk,
+                                  (long long*)p->pk_cnt_red.p))
         return -1;
+    ba.kc = 1;
+    ba.kcl = 0;
+    ba.cm = 0;
     }
     // 3'. reconstructed box with B: the two-point shot-noise terms bin the cross spectrum of D - alpha R
     // and D + alpha^2 R with SN N_D - alpha^3 N_R (bispec.hpp:1455-1508), complex
