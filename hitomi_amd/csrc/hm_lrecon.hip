@@ -257,15 +257,16 @@ struct DwArgs {
     float* part;               // [waves][NKB][16][16]
 };
 
-template <int CIN>
-__global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
+// One wave's share of the K blocks [KB0, KB1) (block NKB - 1: the bias).
+template <int CIN, int KB0, int KB1>
+__device__ __forceinline__ void dw_body(const DwArgs& a, long long wave) {
     constexpr int NKB = (27 * CIN + 15) / 16 + 1;
+    constexpr int NB = KB1 - KB0;
     const int lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int m = lane & 15, g = lane >> 4;
-    f32x4 acc[NKB];
+    f32x4 acc[NB];
 #pragma unroll
-    for (int kb = 0; kb < NKB; kb++) acc[kb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int q = 0; q < NB; q++) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const long long t0 = wave * a.tiles_per_wave;
     const long long t1 = (t0 + a.tiles_per_wave < a.tiles) ? t0 + a.tiles_per_wave : a.tiles;
     bf16x4 one = bf16x4{0, 0, 0, 0};
@@ -279,7 +280,12 @@ __global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
 #pragma unroll
         for (int e = 0; e < 4; e++) bv[e] = (short)a.dz[(v0 + 4 * g + e) * 16 + m];
 #pragma unroll
-        for (int kb = 0; kb < NKB - 1; kb++) {
+        for (int q = 0; q < NB; q++) {
+            const int kb = KB0 + q;
+            if (kb == NKB - 1) {
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(one, bv, acc[q], 0, 0, 0);
+                continue;
+            }
             const int k = 16 * kb + m;
             const int t = k / CIN, ci = k % CIN;
             bf16x4 av = bf16x4{0, 0, 0, 0};
@@ -289,15 +295,28 @@ __global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
 #pragma unroll
                 for (int e = 0; e < 4; e++) av[e] = (short)a.x[(rb + wrapi(z0 + 4 * g + e + dz, a.n2)) * CIN + ci];
             }
-            acc[kb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[kb], 0, 0, 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
         }
-        acc[NKB - 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(one, bv, acc[NKB - 1], 0, 0, 0);
     }
     float* P = a.part + wave * (long long)NKB * 256;
 #pragma unroll
-    for (int kb = 0; kb < NKB; kb++)
+    for (int q = 0; q < NB; q++)
 #pragma unroll
-        for (int e = 0; e < 4; e++) P[(kb * 16 + 4 * g + e) * 16 + m] = acc[kb][e];
+        for (int e = 0; e < 4; e++) P[((KB0 + q) * 16 + 4 * g + e) * 16 + m] = acc[q][e];
+}
+
+// Two workgroups per group of four waves: the even one takes the first half of
+// the K blocks, the odd one the rest (half the accumulators per wave, so
+// twice the waves fit a SIMD); each wave's blocks go to its slot of part.
+template <int CIN>
+__global__ __launch_bounds__(256, 4) void k_conv_dw(DwArgs a) {
+    constexpr int NKB = (27 * CIN + 15) / 16 + 1;
+    constexpr int NH = NKB / 2;
+    const long long wave = (long long)(blockIdx.x >> 1) * 4 + (threadIdx.x >> 6);
+    if (blockIdx.x & 1u)
+        dw_body<CIN, NH, NKB>(a, wave);
+    else
+        dw_body<CIN, 0, NH>(a, wave);
 }
 
 // fixed-order FP64 sums over the waves in two stages: groups of DW_G waves
@@ -578,9 +597,9 @@ int conv_dw(hm_ctx* ctx, hm_lrecon* h, int l, const unsigned short* x, const uns
     const unsigned blocks = (unsigned)((h->part_waves + 3) / 4);
     if (ctx->launch("lrecon_dw", [&] {
             if (cin == 16)
-                hipLaunchKernelGGL(k_conv_dw<16>, dim3(blocks), dim3(256), 0, ctx->stream, a);
+                hipLaunchKernelGGL(k_conv_dw<16>, dim3(2 * blocks), dim3(256), 0, ctx->stream, a);
             else
-                hipLaunchKernelGGL(k_conv_dw<4>, dim3(blocks), dim3(256), 0, ctx->stream, a);
+                hipLaunchKernelGGL(k_conv_dw<4>, dim3(2 * blocks), dim3(256), 0, ctx->stream, a);
         }))
         return -1;
     const long long waves = (long long)blocks * 4, groups = (waves + DW_G - 1) / DW_G;
@@ -804,7 +823,10 @@ hm_lrecon* hm_lrecon_create(hm_ctx* ctx, const hm_lrecon_config* cfg) {
     }
     h->mesh.assign = HM_TSC;
     h->ng = (long long)n * n * n;
-    h->tpw = n % 64 == 0 ? 4 : (n % 32 == 0 ? 2 : 1);
+    // 16 z per wave: with LDS staging a 4 x 4 x 16 block (+ halo) takes 21 KB,
+    // so seven workgroups fit a CU (4 x 4 x 64 took 76 KB and allowed two: a
+    // training step took 21.9 ms against 18.5); the cache-read path takes 64
+    h->tpw = cfg->conv_cache_taps ? (n % 64 == 0 ? 4 : (n % 32 == 0 ? 2 : 1)) : 1;
     size_t o = 0;
     for (int l = 0; l < NL; l++) {
         h->woff[l] = o;
