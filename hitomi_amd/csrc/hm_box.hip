@@ -668,18 +668,18 @@ template <typename T> struct ShellMaskSrc {
 // block split over z among `nsl` lanes.  The fields never return to HBM.
 constexpr int TRI_B = 24;  // most lines of a tile (up to 23 shells)
 // Rows per workgroup of the fused kernel.  Two workgroups fit a CU, so the
-// grid runs in rounds of 2 x CUs workgroups, and a partly filled last round
+// grid runs in waves of 2 x CUs workgroups, and a partly filled last wave
 // costs a full one: take the count in [12, 24] with the fewest rows per CU over
-// all rounds (18 for 288^2 rows on 256 CUs).  A workgroup's FP32 partial sums
+// all waves (18 for 288^2 rows on 256 CUs).  A workgroup's FP32 partial sums
 // depend on it, so results are deterministic per device.
 static int tri_rows_per_wg(long long nrows, int cus) {
     const long long slots = 2LL * (cus > 0 ? cus : 1);
     int best = 16;
     long long best_cost = -1;
     for (int r = 12; r <= 24; r++) {
-        const long long rounds = ((nrows + r - 1) / r + slots - 1) / slots;
-        if (best_cost < 0 || rounds * r < best_cost) {
-            best_cost = rounds * r;
+        const long long waves = ((nrows + r - 1) / r + slots - 1) / slots;
+        if (best_cost < 0 || waves * r < best_cost) {
+            best_cost = waves * r;
             best = r;
         }
     }
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
         __syncthreads();
     }
     // fixed-order reduction over the z slices of each block through LDS, four
-    // of the 16 pair sums (re or im) per round
+    // of the 16 pair sums (re or im) per pass
     double* red = reinterpret_cast<double*>(smraw);
     for (int h = 0; h < 8; h++) {
         const int q0 = (h % 4) * 4;
