@@ -220,9 +220,11 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const double* __restrict__
     long long key;
     first_cell<S>(pos, q, g, &key);
     const long long d = (long long)start[key] + r;
-    spos[3 * d] = pos[3 * q];
-    spos[3 * d + 1] = pos[3 * q + 1];
-    spos[3 * d + 2] = pos[3 * q + 2];
+    if (spos) {  // null: the positions are already in cell order, only the weights move
+        spos[3 * d] = pos[3 * q];
+        spos[3 * d + 1] = pos[3 * q + 1];
+        spos[3 * d + 2] = pos[3 * q + 2];
+    }
     if (w) {
         sw[2 * d] = w[2 * q];
         sw[2 * d + 1] = w[2 * q + 1];
@@ -339,7 +341,7 @@ void sort_work_free(SortWork& w) {
 
 template <int S>
 static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
-                           double scale, long long* d_re, long long* d_im, SortWork& w) {
+                           double scale, long long* d_re, long long* d_im, SortWork& w, bool reuse) {
     int* start = (int*)w.start.p;
     int* slot = (int*)w.slot.p;
     int* border = (int*)w.border.p;
@@ -351,21 +353,24 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
     const long long n = n_keys(g) + 1;
     const unsigned nb = (unsigned)((n + SCAN_BLK - 1) / SCAN_BLK);
     const unsigned pb = (unsigned)((np + 255) / 256);
-    if (fill_zero(ctx, start, sizeof(int) * (size_t)n)) return -1;
-    if (fill_zero(ctx, border, sizeof(int))) return -1;
-    if (np > 0 && ctx->launch("assign_count", [&] {
-            hipLaunchKernelGGL(k_sort_count<S>, dim3(pb), dim3(256), 0, ctx->stream, d_pos, np, g, start, slot, border);
-        }))
-        return -1;
-    if (ctx->launch("assign_scan", [&] {
-            hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, start, n, bsum);
-            hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)nb);
-            hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, start, n, bsum);
-        }))
-        return -1;
-    if (np > 0 && ctx->launch("assign_scatter", [&] {
+    if (!reuse) {
+        if (fill_zero(ctx, start, sizeof(int) * (size_t)n)) return -1;
+        if (fill_zero(ctx, border, sizeof(int))) return -1;
+        if (np > 0 && ctx->launch("assign_count", [&] {
+                hipLaunchKernelGGL(k_sort_count<S>, dim3(pb), dim3(256), 0, ctx->stream, d_pos, np, g, start, slot,
+                                   border);
+            }))
+            return -1;
+        if (ctx->launch("assign_scan", [&] {
+                hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, start, n, bsum);
+                hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)nb);
+                hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, start, n, bsum);
+            }))
+            return -1;
+    }
+    if (np > 0 && (!reuse || sw) && ctx->launch("assign_scatter", [&] {
             hipLaunchKernelGGL(k_sort_scatter<S>, dim3(pb), dim3(256), 0, ctx->stream, d_pos, d_w, np, g, start, slot,
-                               spos, sw);
+                               reuse ? nullptr : spos, sw);
         }))
         return -1;
     const dim3 gg((unsigned)((g.n[2] + GZ - 1) / GZ), (unsigned)((g.n[1] + GY - 1) / GY),
@@ -388,14 +393,14 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
 }
 
 int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
-                  double scale, long long* d_re, long long* d_im, SortWork& w) {
+                  double scale, long long* d_re, long long* d_im, SortWork& w, bool reuse) {
     if (np > 0x7ffffff0LL || g.ng > 0x7ffffff0LL) {
         ctx->set_error("assign: more than 2^31 particles or cells");
         return -1;
     }
-    if (g.assign == 0) return assign_sorted_S<1>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w);
-    if (g.assign == 1) return assign_sorted_S<2>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w);
-    return assign_sorted_S<3>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w);
+    if (g.assign == 0) return assign_sorted_S<1>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w, reuse);
+    if (g.assign == 1) return assign_sorted_S<2>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w, reuse);
+    return assign_sorted_S<3>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w, reuse);
 }
 
 template <int S>

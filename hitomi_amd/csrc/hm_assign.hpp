@@ -109,7 +109,10 @@ struct SortWork {
 };
 int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights);
 void sort_work_free(SortWork& w);
+// reuse: the previous call on w sorted these same positions (same pointer,
+// count and geometry, contents unchanged, weights given both times): only the
+// weights are moved into cell order before the gather.
 int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
-                  double scale, long long* d_re, long long* d_im, SortWork& w);
+                  double scale, long long* d_re, long long* d_im, SortWork& w, bool reuse = false);
 
 }  // namespace hm

@@ -846,9 +846,10 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
             return -1;
         const double scale = fixed_scale_for(wabs);
         const bool cplx = M != 0;
+        // the positions are the same for every field: sort them once, then move only the weights
         const int arc = window ? assign_sorted(ctx, ag, pos + 3 * nD, wts + 2 * nD, nR, scale, fre, cplx ? fim : nullptr,
-                                               p->sort)
-                               : assign_sorted(ctx, ag, pos, wts, N, scale, fre, cplx ? fim : nullptr, p->sort);
+                                               p->sort, f > 0)
+                               : assign_sorted(ctx, ag, pos, wts, N, scale, fre, cplx ? fim : nullptr, p->sort, f > 0);
         if (arc) return -1;
         C<T>* out = fields + (long long)f * ng;
         if (ctx->launch("survey_to_complex", [&] {
