@@ -2108,8 +2108,11 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                      weighted ? wt->c2 : alpha * alpha, 2, (double*)p->rpos.p, (double*)p->rw.p))
             return -1;
         const double scale2 = fixed_scale_for(weighted ? wt->abs2 : (double)np + alpha * alpha * (double)nr);
+        // the same packed positions as the density above, sorted on p->sort in this execution (unless
+        // the assignment was skipped or the interlacing pass sorted its shifted copy since): only the
+        // weights move
         if (assign_sorted(ctx, ag, (const double*)p->rpos.p, (const double*)p->rw.p, np + nr, scale2,
-                          (long long*)p->fixed.p, nullptr, p->sort))
+                          (long long*)p->fixed.p, nullptr, p->sort, run_assign && !inter))
             return -1;
         FixedRealSrc<T> fsrc2{(const long long*)p->fixed.p, m.n[2] / 2, 1.0 / scale2, 1.0 / dV, 0.0, dV};
         if (launch_r2c<T>(ctx, "fft_r2c_z", m.n[2] / 2, fsrc2, dk2, (long long)m.n[0] * m.n[1])) return -1;

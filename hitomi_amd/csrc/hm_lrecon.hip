@@ -769,7 +769,7 @@ int shifts(hm_ctx* ctx, hm_lrecon* h, const double* d_pos, long long n, const do
     if (sort_work_alloc(ctx, h->rw.sort, ag, n, true)) return -1;
     long long* G = (long long*)h->G.p;
     if (assign_sorted(ctx, ag, d_pos, (const double*)h->wxy.p, n, scale, G, G + h->ng, h->rw.sort) ||
-        assign_sorted(ctx, ag, d_pos, (const double*)h->wz.p, n, scale, G + 2 * h->ng, nullptr, h->rw.sort))
+        assign_sorted(ctx, ag, d_pos, (const double*)h->wz.p, n, scale, G + 2 * h->ng, nullptr, h->rw.sort, true))
         return -1;
     const unsigned gb = (unsigned)std::min<long long>((h->ng + 255) / 256, 16384);
     return ctx->launch("lrecon_grad_dz", [&] {
