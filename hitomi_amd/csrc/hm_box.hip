@@ -1950,8 +1950,10 @@ static int pack_pow(hm_ctx* ctx, const double* pD, const double* wD, long long n
 // Survey B with ELL > 0: the full spectrum of the field with weights
 // w^pw Y(L, M)(los) (conjugated with cj) of D and cR times those of R, in out.
 template <typename T>
+// reuse: p->sort already holds the cell sort of these packed positions (from
+// this execution's density assignment), so only the weights move.
 static int svg_field(hm_ctx* ctx, BoxPlanImpl* p, const AssignGeom& ag, const BoxWeighted* wt, long long nD,
-                     long long nR, int M, int pw, int cj, double cR, double abs, C<T>* out) {
+                     long long nR, int M, int pw, int cj, double cR, double abs, C<T>* out, bool reuse) {
     const Mesh& m = p->mesh;
     const long long ntot = nD + nR, ng = m.ng();
     if (!p->fixed_im.p && alloc(ctx, p, p->fixed_im, sizeof(long long) * (size_t)ng)) return -1;
@@ -1962,7 +1964,7 @@ static int svg_field(hm_ctx* ctx, BoxPlanImpl* p, const AssignGeom& ag, const Bo
         return -1;
     const double scale = fixed_scale_for(1.5 * abs);  // |re| + |im| <= sqrt(2) |w|: |Y_red| <= 1
     if (assign_sorted(ctx, ag, (const double*)p->rpos.p, (const double*)p->rw.p, ntot, scale, (long long*)p->fixed.p,
-                      (long long*)p->fixed_im.p, p->sort))
+                      (long long*)p->fixed_im.p, p->sort, reuse))
         return -1;
     const double dV = m.volume() / (double)ng;
     const unsigned gb = (unsigned)std::max<long long>(1, std::min<long long>((ng + 255) / 256, 16384));
@@ -2328,7 +2330,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             if (ha.nc == 0) continue;
             const double snr = wt->snLM[2 * (M + L)], sni = wt->snLM[2 * (M + L) + 1];
             // the Y_LM-weighted fluctuation w (D - alpha R) (field.hpp:186-228), transformed
-            if (svg_field<T>(ctx, p, ag, wt, np, nr, M, 1, 0, wt->c1, wt->abs1, cf1)) return -1;
+            if (svg_field<T>(ctx, p, ag, wt, np, nr, M, 1, 0, wt->c1, wt->abs1, cf1, run_assign && !inter)) return -1;
             // ijk: xi_LM = IFFT[(dn_LM conj N_00 - SN_LM S)/W^2/V], histogram of A_M xi by |n|^2, Bessel sweep
             YXiArgs xa;
             for (int a = 0; a < 3; a++) xa.n[a] = m.n[a];
@@ -2374,7 +2376,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             // two-point terms (bispec.hpp:33-93): P_{l1 m1}(k1) for l2 = 0, P_{l2 m2}(k2) for l1 = 0, of d_00
             // against N_LM = conj(Y_LM) w^2 (D + alpha^2 R) (field.hpp:249-306)
             if (p->l1 == 0 || p->l2 == 0) {
-                if (svg_field<T>(ctx, p, ag, wt, np, nr, M, 2, 1, wt->c2, wt->abs2, cf2)) return -1;
+                if (svg_field<T>(ctx, p, ag, wt, np, nr, M, 2, 1, wt->c2, wt->abs2, cf2, run_assign && !inter))
+                    return -1;
                 for (int c = 0; c < ha.nc; c++) {
                     if (p->l2 == 0 && ybin(p->l1, ha.m1[c], ha.w[c], snr, sni, sv_row)) return -1;
                     if (p->l1 == 0 && ybin(p->l2, ha.m2[c], ha.w[c], snr, sni, sv_col)) return -1;
@@ -2774,7 +2777,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                     ya.nc++;
                 }
                 if (ya.nc == 0) continue;
-                if (svg_field<T>(ctx, p, ag, wt, np, n_rand, M, 1, 0, wt->c1, wt->abs1, cf1)) return -1;
+                if (svg_field<T>(ctx, p, ag, wt, np, n_rand, M, 1, 0, wt->c1, wt->abs1, cf1, run_assign && !inter))
+                    return -1;
                 YXiArgs xa;
                 for (int a = 0; a < 3; a++) xa.n[a] = m.n[a];
                 xa.nzh = (int)nzh;
