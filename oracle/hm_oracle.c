@@ -698,6 +698,59 @@ int or_calc_power_spectrum(const or_param *p, const double *f1, const double *f2
     return 0;
 }
 
+/* Mode counts alone, in one pass over the mesh: the P_l bins of
+ * calcPowerSpectrum (field.hpp:793-837: |k| from the Nyquist-negative
+ * frequencies, fine index int(|k|/1e-4 + 0.5) below 1e5, fold over fine
+ * centres in (lower, upper]) and the n_mode of every bispectrum shell of
+ * calcInverseFourierTransformForBispectrum (field.hpp:361-387: the raw |k| in
+ * (max(k_i - dk/2, 0), k_i + dk/2]) for shell centres kbin[i] and width
+ * kbin[1] - kbin[0], as calcBiSpectrumForBOX passes them (bispec.hpp:1100-1110).
+ * The same FP64 expressions as or_calc_power_spectrum and
+ * or_ifft_for_bispectrum, without the field arithmetic, so it runs at 512^3. */
+int or_mode_counts(const or_param *p, const double *kbin, long long *pk_nmode, long long *shell_nmode) {
+    const int n_temp = 100000;
+    const double dk_temp = 0.0001;
+    const int nk = p->n_kbin;
+    double dk_bin = kbin[1] - kbin[0];
+    long long *n_mode_temp = (long long *)calloc((size_t)n_temp, sizeof(long long));
+    double *slo = (double *)malloc(sizeof(double) * (size_t)nk), *shi = (double *)malloc(sizeof(double) * (size_t)nk);
+    if (!n_mode_temp || !slo || !shi) {
+        free(n_mode_temp);
+        free(slo);
+        free(shi);
+        return -1;
+    }
+    for (int j = 0; j < nk; j++) {
+        slo[j] = (kbin[j] > dk_bin * 0.5) ? (kbin[j] - dk_bin * 0.5) : 0.0;
+        shi[j] = kbin[j] + dk_bin * 0.5;
+        pk_nmode[j] = 0;
+        shell_nmode[j] = 0;
+    }
+    double kvec[3], dk[3];
+    dk[0] = 2.0 * M_PI / p->boxsize[0];
+    dk[1] = 2.0 * M_PI / p->boxsize[1];
+    dk[2] = 2.0 * M_PI / p->boxsize[2];
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                kvec_of(p, i, j, k, dk, kvec);
+                double kmag = sqrt(kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2]);
+                int kindex = (int)(kmag / dk_temp + 0.5);
+                if (kindex < n_temp) n_mode_temp[kindex]++;
+                for (int b = 0; b < nk; b++)
+                    if ((kmag > slo[b]) && (kmag <= shi[b])) shell_nmode[b]++;
+            }
+    for (int j = 0; j < nk; j++)
+        for (int i = 0; i < n_temp; i++) {
+            double kmag_temp = (double)i * dk_temp;
+            if ((kmag_temp > slo[j]) && (kmag_temp <= shi[j])) pk_nmode[j] += n_mode_temp[i];
+        }
+    free(n_mode_temp);
+    free(slo);
+    free(shi);
+    return 0;
+}
+
 /* field.hpp:854-977 */
 int or_calc_correlation_function(const or_param *p, const double *f1, const double *f2, const double *rbin,
                                  double sn_re, double sn_im, int ell, int m, double *xi, int *n_mode) {

@@ -50,6 +50,9 @@ int or_calc_field(const or_param *p, const double *pos, const double *weight, lo
  * (field.hpp:84-93, 121-133, 162-176): writes 1/8/27 coords per particle,
  * -1 where the reference's range check drops the contribution. */
 int or_cell_coords(const or_param *p, const double *pos, long long np, long long *coords);
+/* P_l bin mode counts (field.hpp:793-837) and bispectrum shell n_mode
+ * (field.hpp:361-387) for the bins of tools.hpp:118-124, counts only. */
+int or_mode_counts(const or_param *p, const double *kbin, long long *pk_nmode, long long *shell_nmode);
 
 /* Unnormalised in-place 3D complex DFT, sign -1 (FFTW_FORWARD) or +1. */
 int or_fft3d(const or_param *p, double *field, int sign);

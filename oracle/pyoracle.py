@@ -45,6 +45,7 @@ def lib():
             "or_set_rbin": (None, [P, _D]),
             "or_calc_field": (ct.c_int, [P, _D, _D, ct.c_longlong, _D]),
             "or_cell_coords": (ct.c_int, [P, _D, ct.c_longlong, ct.POINTER(ct.c_longlong)]),
+            "or_mode_counts": (ct.c_int, [P, _D, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong)]),
             "or_fft3d": (ct.c_int, [P, _D, ct.c_int]),
             "or_fft1d": (ct.c_int, [_D, ct.c_int, ct.c_int]),
             "or_fourier_transform": (None, [P, _D]),
@@ -157,6 +158,18 @@ def cell_coords(p: Param, pos):
     out = np.zeros((pos.shape[0], s), dtype=np.int64)
     lib().or_cell_coords(ct.byref(p), dp(pos), pos.shape[0], out.ctypes.data_as(ct.POINTER(ct.c_longlong)))
     return out
+
+
+def mode_counts(p: Param):
+    """(P_l bin mode counts, bispectrum shell n_mode) for the bins of p, counts only (or_mode_counts)."""
+    kb = kbin(p)
+    pk = np.zeros(p.n_kbin, dtype=np.int64)
+    sh = np.zeros(p.n_kbin, dtype=np.int64)
+    LL = ct.POINTER(ct.c_longlong)
+    rc = lib().or_mode_counts(ct.byref(p), dp(kb), pk.ctypes.data_as(LL), sh.ctypes.data_as(LL))
+    if rc != 0:
+        raise MemoryError("or_mode_counts")
+    return pk, sh
 
 
 def _c(a):

@@ -50,3 +50,19 @@ def clustered_catalogue(n, L, seed, nclump=40, sigma=20.0):
     pos = np.mod(pos, L)
     pos[pos >= L] -= L
     return pos
+
+
+def report_err(name, value):
+    """Print a measured error next to its tolerance check, so a run with -s lists
+    the margins of every FP32 comparison."""
+    test = os.environ.get("PYTEST_CURRENT_TEST", "").split(" ")[0]
+    print(f"\nTOLREPORT {test} {name}: {float(value):.3e}", flush=True)
+
+
+def rel_check(name, got, ref, tol, scale=None):
+    """max |got - ref| / scale (default max |ref|), reported, then checked against tol."""
+    scale = np.abs(ref).max() if scale is None else scale
+    e = float(np.abs(np.asarray(got) - np.asarray(ref)).max() / scale)
+    report_err(name, e)
+    assert e <= tol, (name, e, tol)
+    return e

@@ -5,8 +5,9 @@ calcPowerSpectrumForBOX (powerspec.hpp:273-332) and calcBiSpectrumForBOX
 Tolerances (stated per quantity):
   - mode counts (P bins and B shells): bit-exact;
   - P_l: |dP_l(k)| <= tol * (|P_0(k)| + V/N), tol = 1e-10 for FP64 grids and
-    1e-5 for FP32 grids (FP64 bin sums in both);
-  - B: |dB(k1,k2)| <= tol * max|B| with tol = 1e-9 (FP64) / 2e-4 (FP32); the
+    1e-6 for FP32 grids (FP64 bin sums in both; measured <= 1.5e-7);
+  - B: |dB(k1,k2)| <= tol * max|B| with tol = 1e-9 (FP64) / 5e-6 (FP32, measured
+    <= 2.7e-6 for the odd multipole (1, 0, 1), <= 4e-7 otherwise); the
     bk and sn sums are checked separately with the same tolerances.
 """
 import numpy as np
@@ -15,12 +16,12 @@ import pytest
 import hitomi_amd as hm
 from oracle import pyoracle as po
 
-from .conftest import clustered_catalogue, uniform_catalogue
+from .conftest import clustered_catalogue, rel_check, report_err, uniform_catalogue
 
 pytestmark = pytest.mark.gpu
 
-PK_TOL = {hm.FP64: 1e-10, hm.FP32: 1e-5}
-BK_TOL = {hm.FP64: 1e-9, hm.FP32: 2e-4}
+PK_TOL = {hm.FP64: 1e-10, hm.FP32: 1e-6}
+BK_TOL = {hm.FP64: 1e-9, hm.FP32: 5e-6}
 
 
 def oracle_pk(n, L, pos, kmin, kmax, nk, ells=(0, 2, 4)):
@@ -36,7 +37,9 @@ def check_pk(res, ref_pk, ref_nm, prec, V, N):
     assert np.array_equal(res["pk_nmode"], ref_nm.astype(np.int64))
     got = res["pk"][..., 0] + 1j * res["pk"][..., 1]
     scale = np.abs(ref_pk[0].real) + V / N
-    assert np.all(np.abs(got - ref_pk) <= PK_TOL[prec] * scale[None, :]), np.abs(got - ref_pk) / scale
+    e = (np.abs(got - ref_pk) / scale[None, :]).max()
+    report_err("pk", e)
+    assert e <= PK_TOL[prec]
 
 
 @pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
@@ -112,9 +115,9 @@ def test_bk000_all_pairs(ctx, case, mode):
     assert np.array_equal(res["shell_nmode"], shell_counts(n, L, kmin, kmax, nk))
     c = lambda a: a[..., 0] + 1j * a[..., 1]
     tol = BK_TOL[prec]
-    assert np.abs(c(res["bk_sum"]) - ref_bk).max() <= tol * np.abs(ref_bk).max()
-    assert np.abs(c(res["sn_sum"]) - ref_sn).max() <= tol * np.abs(ref_sn).max()
-    assert np.abs(c(res["bk"]) - ref).max() <= tol * np.abs(ref).max()
+    rel_check("bk_sum", c(res["bk_sum"]), ref_bk, tol)
+    rel_check("sn_sum", c(res["sn_sum"]), ref_sn, tol)
+    rel_check("bk", c(res["bk"]), ref, tol)
 
 
 def test_fp32_pipeline_bitwise_deterministic(ctx):
@@ -130,7 +133,8 @@ def test_fp32_pipeline_bitwise_deterministic(ctx):
 
 def test_metric_mesh_512_properties(ctx):
     """512^3, L = 2000, 4e6 particles, 20 bins 0.015..0.205: FP32 P_l agree with the
-    FP64 plan to 1e-5 (P_0 + SN); B is symmetric in (k1, k2); repeat runs are bitwise equal."""
+    FP64 plan to 1e-6 (P_0 + SN); B within 5e-6 of max|B|; B is symmetric in (k1, k2); repeat runs
+    are bitwise equal."""
     n, L, N = 512, 2000.0, 4_000_000
     pos = clustered_catalogue(N, L, seed=42, nclump=400, sigma=15.0)
     kw = dict(n_kbin=20, kmin=0.015, kmax=0.205, pk_ells=(0, 2, 4), bk=True)
@@ -146,11 +150,11 @@ def test_metric_mesh_512_properties(ctx):
     p64.close()
     assert np.array_equal(r1["pk_nmode"], r64["pk_nmode"])
     scale = np.abs(r64["pk"][0, :, 0]) + L ** 3 / N
-    assert np.all(np.abs(r1["pk"][..., 0] - r64["pk"][..., 0]) <= 1e-5 * scale[None, :])
+    assert np.all(np.abs(r1["pk"][..., 0] - r64["pk"][..., 0]) <= 1e-6 * scale[None, :])
     b = r1["bk"][..., 0]
     assert np.allclose(b, b.T, rtol=0, atol=0)
     b64 = r64["bk"][..., 0]
-    assert np.abs(b - b64).max() <= 2e-4 * np.abs(b64).max()
+    rel_check("B_000 fp32 vs fp64", b, b64, 5e-6)
 
 
 _GEN_CACHE = {}
@@ -196,9 +200,9 @@ def test_bk_general_multipole(ctx, case, mode):
     res = plan.execute(pos)
     c = lambda a: a[..., 0] + 1j * a[..., 1]
     tol = BK_TOL[prec]
-    assert np.abs(c(res["bk_sum"]) - ref_bk).max() <= tol * np.abs(ref_bk).max()
-    assert np.abs(c(res["sn_sum"]) - ref_sn).max() <= tol * np.abs(ref_sn).max()
-    assert np.abs(c(res["bk"]) - ref).max() <= tol * np.abs(ref).max()
+    rel_check("bk_sum", c(res["bk_sum"]), ref_bk, tol)
+    rel_check("sn_sum", c(res["sn_sum"]), ref_sn, tol)
+    rel_check("bk", c(res["bk"]), ref, tol)
 
 
 @pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])

@@ -14,6 +14,8 @@ import pytest
 import hitomi_amd as hm
 from oracle import pyoracle as po
 
+from .conftest import rel_check
+
 pytestmark = pytest.mark.gpu
 
 TOL = {hm.FP64: 1e-10, hm.FP32: 2e-5}
@@ -156,8 +158,8 @@ def test_survey_after_reconstruction(ctx, prec, measure):
     plan.close()
     assert np.array_equal(gnm, nm.astype(np.int64))
     assert info["vsurvey"] == pytest.approx(s.vsurvey, rel=1e-12)
-    tol = 1e-9 if prec == hm.FP64 else 1e-4
-    assert np.abs(got - ref).max() <= tol * np.abs(ref0).max()
+    tol = 1e-9 if prec == hm.FP64 else 5e-6
+    rel_check("out", got, ref, tol, np.abs(ref0).max())
 
 
 @pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
@@ -178,8 +180,8 @@ def test_survey_bispectrum(ctx, prec, ells):
         s = po.Survey(p, *cat)
         ref[ith], _, _ = s.bk(p)
     assert info["norm"] == pytest.approx(s.vsurvey ** 2 / s.wsumD ** 3, rel=1e-12)
-    tol = 1e-9 if prec == hm.FP64 else 2e-4
-    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    tol = 1e-9 if prec == hm.FP64 else 5e-6
+    rel_check("out", got, ref, tol)
     assert np.all(nm > 0)
 
 
@@ -188,7 +190,7 @@ def test_survey_bispectrum(ctx, prec, ells):
 @pytest.mark.parametrize("ells", [(0, 0, 0), (1, 1, 0)])
 def test_survey_zeta(ctx, prec, measure, ells):
     """calcThreePoint(Window)CorrelationFunction (bispec.hpp:355-874) with ELL = 0 for every ith_rbin:
-    |d norm (zeta - sn)| <= tol * max of it, tol = 1e-9 (FP64) / 1e-4 (FP32)."""
+    |d norm (zeta - sn)| <= tol * max of it, tol = 1e-9 (FP64) / 5e-6 (FP32)."""
     l1, l2, L_ = ells
     n, L = 32, 800.0
     rw = dict(rmin=60.0, rmax=240.0, n_rbin=4)
@@ -201,8 +203,8 @@ def test_survey_zeta(ctx, prec, measure, ells):
         p = po.param(n, L, po.TSC, ell1=l1, ell2=l2, ELL=L_, ith_rbin=ith, **rw)
         s = po.Survey(p, *cat)
         ref[ith], _, _ = s.zeta(p, window=measure == "window3PCF")
-    tol = 1e-9 if prec == hm.FP64 else 1e-4
-    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    tol = 1e-9 if prec == hm.FP64 else 5e-6
+    rel_check("out", got, ref, tol)
 
 
 @pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
@@ -210,7 +212,7 @@ def test_survey_zeta(ctx, prec, measure, ells):
 def test_survey_bispectrum_ell_nonzero(ctx, prec, ells):
     """calcBiSpectrum (bispec.hpp:4-353) with ELL > 0 for every ith_kbin: the Y_LM(los)-weighted third
     field per M, the row / column shot-noise terms against N_LM and the direction-weighted ijk term;
-    |dB| <= tol * max|B|, tol = 1e-9 (FP64) / 2e-4 (FP32)."""
+    |dB| <= tol * max|B|, tol = 1e-9 (FP64) / 5e-6 (FP32)."""
     l1, l2, L_ = ells
     n, L = 32, 800.0
     kw = dict(kmin=0.02, kmax=0.1, n_kbin=5)
@@ -223,8 +225,8 @@ def test_survey_bispectrum_ell_nonzero(ctx, prec, ells):
         p = po.param(n, L, po.TSC, ell1=l1, ell2=l2, ELL=L_, ith_kbin=ith, **kw)
         s = po.Survey(p, *cat)
         ref[ith], _, _ = s.bk(p)
-    tol = 1e-9 if prec == hm.FP64 else 2e-4
-    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    tol = 1e-9 if prec == hm.FP64 else 5e-6
+    rel_check("out", got, ref, tol)
 
 
 @pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
@@ -233,9 +235,10 @@ def test_survey_bispectrum_ell_nonzero(ctx, prec, ells):
 def test_survey_zeta_ell_nonzero(ctx, prec, measure, ells):
     """calcThreePoint(Window)CorrelationFunction (bispec.hpp:355-874) with ELL > 0 for every ith_rbin: per M
     the Y_LM(los)-weighted third field and the shot noise of xi_LM weighted by Y_l1m1(r) Y_l2m2(r);
-    |d norm (zeta - sn)| <= tol * max of it, tol = 1e-9 (FP64) / 1e-4 (FP32).  The shot noise (diagonal
+    |d norm (zeta - sn)| <= tol * max of it, tol = 1e-9 (FP64) / 5e-6 (FP32).  The shot noise (diagonal
     only, ~0.2% of zeta here) is checked on its own: on the diagonal, the GPU's norm (zeta - sn) minus the
-    oracle's norm zeta is -norm sn within 1e-5 (FP64) / 0.2 (FP32) of |norm sn|."""
+    oracle's norm zeta is -norm sn within 1e-5 (FP64) / 2e-3 (FP32: a difference of two FP32 results that
+    are ~500x larger; measured <= 3.3e-4) of |norm sn|."""
     l1, l2, L_ = ells
     n, L = 32, 800.0
     rw = dict(rmin=60.0, rmax=240.0, n_rbin=4)
@@ -252,12 +255,12 @@ def test_survey_zeta_ell_nonzero(ctx, prec, measure, ells):
         s = po.Survey(p, *cat)
         ref[ith], z, sn = s.zeta(p, window=win)
         zs[ith], sns[ith] = z[ith], sn[ith]
-    tol = 1e-9 if prec == hm.FP64 else 1e-4
-    assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    tol = 1e-9 if prec == hm.FP64 else 5e-6
+    rel_check("out", got, ref, tol)
     norm = info["norm"]
     sn_got = norm * zs - np.diag(got)
-    stol = 1e-5 if prec == hm.FP64 else 0.2
-    assert np.abs(sn_got - norm * sns).max() <= stol * np.abs(norm * sns).max()
+    stol = 1e-5 if prec == hm.FP64 else 2e-3
+    rel_check("shot noise", sn_got, norm * sns, stol)
 
 
 def test_survey_three_point_rejects_disallowed_multipole(ctx):

@@ -4,10 +4,10 @@ against the oracle's restatement of calcThreePointCorrelationFunctionForBOX
 
 Tolerances (stated per quantity):
   - zeta_save: |d| <= tol * max |zeta_save| over the whole (r1, r2) matrix,
-    tol = 1e-9 for FP64 grids and 1e-4 for FP32 grids (the triple sums run in
+    tol = 1e-9 for FP64 grids and 5e-6 for FP32 grids (measured <= 6e-7; the triple sums run in
     the grid precision within 64-voxel chunks, FP64 across chunks);
   - sn_save (diagonal only, zero elsewhere as in the reference): |d| <= tol *
-    max |sn_save|, tol = 1e-10 (FP64) / 2e-5 (FP32);
+    max |sn_save|, tol = 1e-10 (FP64) / 2e-6 (FP32, measured <= 3e-7);
   - norm * (zeta - sn): |d| <= tol_zeta * norm * (max|zeta_save| + max|sn_save|).
 """
 import numpy as np
@@ -16,12 +16,12 @@ import pytest
 import hitomi_amd as hm
 from oracle import pyoracle as po
 
-from .conftest import clustered_catalogue
+from .conftest import clustered_catalogue, rel_check
 
 pytestmark = pytest.mark.gpu
 
-Z_TOL = {hm.FP64: 1e-9, hm.FP32: 1e-4}
-SN_TOL = {hm.FP64: 1e-10, hm.FP32: 2e-5}
+Z_TOL = {hm.FP64: 1e-9, hm.FP32: 5e-6}
+SN_TOL = {hm.FP64: 1e-10, hm.FP32: 2e-6}
 
 CASES = [
     # n, L, N, rmin, rmax, n_rbin, (ell1, ell2, ELL)
@@ -68,11 +68,11 @@ def test_zeta_box(ctx, case, mode):
     zscale = np.abs(ref_z).max()
     snscale = np.abs(ref_sn).max()
     assert zscale > 0.0 and snscale > 0.0
-    assert np.abs(got_z - ref_z).max() <= Z_TOL[prec] * zscale, np.abs(got_z - ref_z).max() / zscale
-    assert np.abs(got_sn - ref_sn).max() <= SN_TOL[prec] * snscale, np.abs(got_sn - ref_sn).max() / snscale
+    rel_check("zeta_sum", got_z, ref_z, Z_TOL[prec], zscale)
+    rel_check("zeta_sn", got_sn, ref_sn, SN_TOL[prec], snscale)
     assert np.all(got_sn[~np.eye(nr, dtype=bool)] == 0.0)
     norm = (L ** 3 / N / N) * (L ** 3 / N)
-    assert np.abs(got - ref).max() <= Z_TOL[prec] * norm * (zscale + snscale)
+    rel_check("zeta", got, ref, Z_TOL[prec], norm * (zscale + snscale))
 
 
 def test_zeta_rejects_forbidden_multipoles(ctx):

@@ -212,6 +212,57 @@ def test_pk_mode_counts(n, L, kmin, kmax, nk):
     assert (nm == _mode_counts_numpy(n, L, kb)).all()
 
 
+def _shell_and_pk_counts_numpy(n, L, kbin):
+    """P bin counts (fine histogram + fold, field.hpp:793-837) and bispectrum shell counts
+    (raw |k| in (max(k - dk/2, 0), k + dk/2], field.hpp:361-387), vectorised over x slabs."""
+    n = list(n) if np.ndim(n) else [int(n)] * 3
+    L = list(L) if np.ndim(L) else [float(L)] * 3
+    kv = []
+    for a in range(3):
+        idx = np.arange(n[a])
+        kv.append(np.where(idx < n[a] // 2, idx, idx - n[a]) * (2 * np.pi / L[a]))
+    dkb = kbin[1] - kbin[0]
+    lo = np.array([kb - dkb * 0.5 if kb > dkb * 0.5 else 0.0 for kb in kbin])
+    hi = kbin + dkb * 0.5
+    hist = np.zeros(100000, dtype=np.int64)
+    shells = np.zeros(len(kbin), dtype=np.int64)
+    ky, kz = np.meshgrid(kv[1], kv[2], indexing="ij")
+    for kx in kv[0]:
+        kmag = np.sqrt(kx * kx + ky * ky + kz * kz).ravel()
+        kindex = (kmag / 0.0001 + 0.5).astype(np.int64)
+        hist += np.bincount(kindex[kindex < 100000], minlength=100000)
+        ks = np.sort(kmag)
+        shells += np.searchsorted(ks, hi, side="right") - np.searchsorted(ks, lo, side="right")
+    centres = np.arange(100000) * 0.0001
+    pk = np.array([int(hist[(centres > lo[j]) & (centres <= hi[j])].sum()) for j in range(len(kbin))])
+    return pk, shells
+
+
+@pytest.mark.parametrize("n,L,kmin,kmax,nk", [(32, 1000.0, 0.02, 0.08, 4), (24, 700.0, 0.01, 0.2, 9),
+                                               ((20, 36, 24), (1350.0, 2450.0, 1400.0), 0.01, 0.1, 10)])
+def test_mode_counts_only(n, L, kmin, kmax, nk):
+    """or_mode_counts equals the counts of calcPowerSpectrum and of every bispectrum shell of
+    calcInverseFourierTransformForBispectrum, and an independent vectorised count."""
+    p = po.param(n, L, po.TSC, kmin=kmin, kmax=kmax, n_kbin=nk)
+    kb = po.kbin(p)
+    pk, sh = po.mode_counts(p)
+    f = np.ones(po.shape(p), complex)
+    _, nm = po.calc_power_spectrum(p, f, f, kb, 0.0, 0, 0)
+    assert np.array_equal(pk, nm)
+    assert np.array_equal(sh, [po.ifft_for_bispectrum(p, f, k, kb[1] - kb[0])[1] for k in kb])
+    ref_pk, ref_sh = _shell_and_pk_counts_numpy(n, L, kb)
+    assert np.array_equal(pk, ref_pk) and np.array_equal(sh, ref_sh)
+
+
+def test_mode_counts_metric_mesh():
+    """512^3, L = 2000, 20 bins 0.015..0.205 (the bispectrum workload): the count-only oracle against the
+    independent vectorised count."""
+    p = po.param(512, 2000.0, po.TSC, kmin=0.015, kmax=0.205, n_kbin=20)
+    pk, sh = po.mode_counts(p)
+    ref_pk, ref_sh = _shell_and_pk_counts_numpy(512, 2000.0, po.kbin(p))
+    assert np.array_equal(pk, ref_pk) and np.array_equal(sh, ref_sh)
+
+
 def test_pk_plane_wave_closed_form():
     n, L = 16, 400.0
     p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.2, n_kbin=10)
