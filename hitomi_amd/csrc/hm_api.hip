@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 namespace hm {
@@ -66,6 +67,43 @@ static bool mesh_ok(hm_ctx* ctx, const hm_mesh* m, hm::AssignGeom& g) {
 
 extern "C" {
 
+// Scratch of hm_assign_noalloc: the sort work of the sorted mass assignment
+// (per-chunk counts and offsets, per-particle slots, positions and weights in
+// cell order, the border list, scan sums) for callers that cannot allocate.
+// Every process that loads the library holds it on each device it uses, until
+// the process ends.  In host code `scratch` is only a stand-in for the device
+// symbol: its device address is looked up once per device (scratch_setup) and
+// only that address is handed to kernels.
+#define HM_NOALLOC_SCRATCH_BYTES (128ull << 20)
+#define HM_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[HM_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[HM_NOALLOC_MAX_DEVICES];
+static int g_scratch_state[HM_NOALLOC_MAX_DEVICES];  // 0: not looked up, 1: ok, -1: the lookup failed
+
+// With `device` current: the device address of this device's copy of scratch.
+static void scratch_setup(int device) {
+    if (device < 0 || device >= HM_NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_state[device] != 0) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) {
+        g_scratch_ptr[device] = p;
+        g_scratch_state[device] = 1;
+    } else {
+        (void)hipGetLastError();
+        g_scratch_state[device] = -1;
+    }
+}
+
+static void* scratch_ptr(int device) {
+    if (device < 0 || device >= HM_NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_state[device] == 1 ? g_scratch_ptr[device] : nullptr;
+}
+
 hm_ctx* hm_ctx_create(int device, int flags) {
     (void)flags;
     int count = 0;
@@ -83,6 +121,7 @@ hm_ctx* hm_ctx_create(int device, int flags) {
         delete ctx;
         return nullptr;
     }
+    scratch_setup(device);
     return ctx;
 }
 
@@ -306,6 +345,36 @@ int hm_assign(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const doubl
 int hm_assign_atomic(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
                      double wabs_sum, double* d_out_re, double* d_out_im) {
     return assign_impl(ctx, mesh, d_pos, d_w, np, wabs_sum, d_out_re, d_out_im, false);
+}
+
+int hm_assign_noalloc(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, const double* d_w, long long np,
+                      double wabs_sum, double* d_out_re, double* d_out_im) {
+    if (!ctx) return -1;
+    hm::AssignGeom g;
+    if (!mesh_ok(ctx, mesh, g)) return -1;
+    if (np < 0 || (np > 0 && !d_pos) || !d_out_re) return bad(ctx, "hm_assign_noalloc: bad arguments");
+    if (d_out_im == d_out_re) return bad(ctx, "hm_assign_noalloc: d_out_re and d_out_im must not alias");
+    char* base = (char*)scratch_ptr(ctx->device);
+    if (!base) return bad(ctx, "hm_assign_noalloc: the device address of the library's scratch is unavailable");
+    hm::SortWork w;
+    const size_t need = hm::sort_work_carve(g, np, d_w != nullptr, nullptr, &w);
+    if (need > HM_NOALLOC_SCRATCH_BYTES)
+        return bad(ctx, ("hm_assign_noalloc: " + std::to_string(np) + " particles on this mesh need " +
+                         std::to_string(need) + " bytes of scratch, more than the " +
+                         std::to_string(HM_NOALLOC_SCRATCH_BYTES) + " the library holds; use hm_assign")
+                            .c_str());
+    hm::sort_work_carve(g, np, d_w != nullptr, base, &w);
+    if (!(wabs_sum > 0.0)) wabs_sum = (double)np;
+    const double scale = hm::fixed_scale_for(wabs_sum);
+    // the output grids hold the int64 cell sums until they are converted in place
+    long long* qre = reinterpret_cast<long long*>(d_out_re);
+    long long* qim = reinterpret_cast<long long*>(d_out_im);
+    int rc = hm::assign_sorted(ctx, g, d_pos, d_w, np, scale, qre, qim, w);
+    const double dV = (g.L[0] * g.L[1] * g.L[2]) / (double)g.ng;
+    const double factor = 1.0 / dV;
+    if (!rc) rc = hm::fixed_to_density_inplace(ctx, qre, g.ng, 1.0 / scale, factor);
+    if (!rc && qim) rc = hm::fixed_to_density_inplace(ctx, qim, g.ng, 1.0 / scale, factor);
+    return rc;
 }
 
 int hm_cell_coords(hm_ctx* ctx, const hm_mesh* mesh, const double* d_pos, long long np, long long* d_coords) {

@@ -330,6 +330,25 @@ int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np,
     return 0;
 }
 
+size_t sort_work_carve(const AssignGeom& g, long long np, bool weights, char* base, SortWork* w) {
+    const long long nk = n_keys(g);
+    const long long nb = (nk + 1 + SCAN_BLK - 1) / SCAN_BLK;
+    const size_t npp = (size_t)(np > 0 ? np : 1);
+    const size_t sizes[6] = {sizeof(int) * (size_t)(nk + 1), sizeof(int) * npp, sizeof(double) * 3 * npp,
+                             weights ? sizeof(double) * 2 * npp : 0, sizeof(int) * (size_t)(np + 1),
+                             sizeof(int) * (size_t)nb};
+    DevBuf* parts[6] = {&w->start, &w->slot, &w->spos, &w->sw, &w->border, &w->bsum};
+    size_t off = 0;
+    for (int i = 0; i < 6; i++) {
+        if (base) {
+            parts[i]->p = sizes[i] ? base + off : nullptr;
+            parts[i]->bytes = sizes[i];
+        }
+        off += (sizes[i] + 255) & ~(size_t)255;
+    }
+    return off;
+}
+
 void sort_work_free(SortWork& w) {
     dev_free(w.start);
     dev_free(w.slot);
@@ -451,6 +470,29 @@ int fixed_to_density(hm_ctx* ctx, const long long* d_q, long long n, double inv_
     return ctx->launch("fixed_to_density", [&] {
         hipLaunchKernelGGL(k_fixed_to_density, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d_q, n, inv_scale,
                            factor, d_out);
+    });
+}
+
+// The same conversion where the int64 cells and the FP64 densities share
+// memory: each lane reads a cell and overwrites it with its density.
+__global__ void k_fixed_to_density_inplace(long long* q, long long n, double inv_scale, double factor) {
+#pragma clang fp contract(off)
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    double* out = reinterpret_cast<double*>(q);
+    for (long long i = i0; i < n; i += st) {
+        const long long v = q[i];
+        out[i] = ((double)v * inv_scale) * factor;
+    }
+}
+
+int fixed_to_density_inplace(hm_ctx* ctx, long long* d_q, long long n, double inv_scale, double factor) {
+    long long blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks < 1) blocks = 1;
+    return ctx->launch("fixed_to_density", [&] {
+        hipLaunchKernelGGL(k_fixed_to_density_inplace, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d_q, n,
+                           inv_scale, factor);
     });
 }
 

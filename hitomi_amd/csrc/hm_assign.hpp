@@ -109,6 +109,12 @@ struct SortWork {
 };
 int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights);
 void sort_work_free(SortWork& w);
+// Bytes of a SortWork for np particles on g carved out of one region, each part
+// 256-byte aligned.  With base set, w's buffers point into [base, base + bytes):
+// they are borrowed, never passed to sort_work_free.
+size_t sort_work_carve(const AssignGeom& g, long long np, bool weights, char* base, SortWork* w);
+// Fixed-point cells -> density where the int64 cells and the FP64 output share memory.
+int fixed_to_density_inplace(hm_ctx* ctx, long long* d_q, long long n, double inv_scale, double factor);
 // reuse: the previous call on w sorted these same positions (same pointer,
 // count and geometry, contents unchanged, weights given both times): only the
 // weights are moved into cell order before the gather.

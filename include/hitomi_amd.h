@@ -92,6 +92,20 @@ int hm_assign(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const doubl
 int hm_assign_atomic(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const double *d_w, long long np,
                      double wabs_sum, double *d_out_re, double *d_out_im);
 
+/* hm_assign without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same grids as
+ * hm_assign, bit for bit.  The sort work lives in a fixed 128 MiB device array
+ * of the library (one per GPU, held until the process ends): about 48 B per
+ * particle with weights (32 B without) plus 4 B per (x, y, 64-cell z chunk);
+ * larger inputs are rejected with an error.  The int64 cell sums are
+ * accumulated in d_out_re / d_out_im themselves and converted in place.
+ * Asynchronous on the context's stream (no synchronisation).  Every call on a
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (other streams,
+ * other contexts, other threads) race and must be ordered by the caller. */
+int hm_assign_noalloc(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const double *d_w, long long np,
+                      double wabs_sum, double *d_out_re, double *d_out_im);
+
 /* The 1/8/27 stencil cell indices of each particle as field.hpp:84-93,
  * 121-133, 162-176 compute them; -1 where the reference's range check
  * drops the contribution.  d_coords: np * (1|8|27) int64. */
