@@ -339,6 +339,45 @@ __global__ __launch_bounds__(256) void k_xi_group(const C<T>* __restrict__ xh, i
     }
 }
 
+// Keys of the shot-noise radial sums.  Equal cell sizes: key = i'^2 + j'^2 +
+// k'^2 (|r| = sqrt(key) dr).  Unequal cell sizes (the reference's anisotropic
+// survey boxes): key = (|i'| e1 + |j'|) e2 + |k'|, one key per distinct
+// (|i'|, |j'|, |k'|), |r| = sqrt((|i'| d0)^2 + (|j'| d1)^2 + (|k'| d2)^2); the
+// rows (i, j) are grouped by a = |i'| e1 + |j'|.  Signed indices follow the
+// rvec rule of bispec.hpp:1040-1042 (the Nyquist index negative).
+struct SnKeys {
+    int aniso = 0;
+    int e1 = 0, e2 = 0;  // extents of |j'| and |k'| (+1)
+    double d[3] = {0.0, 0.0, 0.0};
+};
+__device__ __forceinline__ double sn_key_r(const SnKeys& K, long long key, double dr) {
+#pragma clang fp contract(off)
+    if (!K.aniso) return sqrt((double)key) * dr;
+    const long long i = key / ((long long)K.e1 * K.e2), j = (key / K.e2) % K.e1, k = key % K.e2;
+    const double x = (double)i * K.d[0], y = (double)j * K.d[1], z = (double)k * K.d[2];
+    return sqrt(x * x + y * y + z * z);
+}
+
+// Step 2 with unequal cell sizes: H[(a, q)] = R[a][q] + R[a][n2 - q] (one
+// term for q = 0 and the Nyquist plane q = n2 / 2).
+template <typename T>
+__global__ __launch_bounds__(256) void k_xi_hist_aniso(const T* __restrict__ R, int n2, const int* __restrict__ a_idx,
+                                                      SnKeys K, long long nkey, double* __restrict__ H) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nkey) return;
+    const long long a = n / K.e2;
+    const int q = (int)(n % K.e2);
+    const int ai = a_idx[a];
+    double acc = 0.0;
+    if (ai >= 0) {
+        const T* row = R + (long long)ai * n2;
+        if (q == 0) acc = (double)row[0];
+        else if (2 * q == n2) acc = (double)row[q];
+        else acc = (double)row[q] + (double)row[n2 - q];
+    }
+    H[n] = acc;
+}
+
 // Step 2: H[n] = sum over k with a = n - k'^2 a row key of R[a][k]: the sum of
 // xi over all voxels with minimum-image |n|^2 = n (rvec rule of
 // bispec.hpp:1040-1043), in a fixed order.
@@ -418,24 +457,50 @@ __global__ __launch_bounds__(256) void k_xi_hist_ang(const T* __restrict__ R, in
     H[n] = acc;
 }
 
+// As k_xi_hist_aniso with each voxel's xi weighted by A(mu), mu = r_z / |r|
+// (the Nyquist plane negative, field.hpp:1042), A0 at r = 0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_xi_hist_ang_aniso(const T* __restrict__ R, int n2,
+                                                          const int* __restrict__ a_idx, SnKeys K, long long nkey,
+                                                          AngPoly A, double* __restrict__ H) {
+#pragma clang fp contract(off)
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nkey) return;
+    const long long a = n / K.e2;
+    const int q = (int)(n % K.e2);
+    const int ai = a_idx[a];
+    double acc = 0.0;
+    if (ai >= 0) {
+        const T* row = R + (long long)ai * n2;
+        const double rn = sn_key_r(K, n, 0.0);
+        const double rz = (double)q * K.d[2];
+        if (n == 0) acc = A.a0 * (double)row[0];
+        else if (q == 0) acc = ang_eval(A, 0.0) * (double)row[0];
+        else if (2 * q == n2) acc = ang_eval(A, -rz / rn) * (double)row[q];
+        else acc = ang_eval(A, rz / rn) * (double)row[q] + ang_eval(A, -rz / rn) * (double)row[n2 - q];
+    }
+    H[n] = acc;
+}
+
 // J[a][n2] = j_l(k_a sqrt(n2) dr) from the spline of bessel.hpp:12-25
 __global__ void k_jl_table(const double* __restrict__ sy, const double* __restrict__ sc, const double* __restrict__ kb,
-                           int nk, long long nkey, double dr, double* __restrict__ J) {
+                           int nk, long long nkey, double dr, SnKeys K, double* __restrict__ J) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nk * nkey) return;
     const int a = (int)(t / nkey);
     const long long key = t % nkey;
-    J[t] = jl_spline_eval(sy, sc, kb[a] * (sqrt((double)key) * dr));
+    J[t] = jl_spline_eval(sy, sc, kb[a] * sn_key_r(K, key, dr));
 }
 
 // J[a][n2] = j0(k_a sqrt(n2) dr) (bessel.hpp spline of j0 replaced by the
 // closed form sin x / x)
-__global__ void k_j0_table(const double* __restrict__ kb, int nk, long long nkey, double dr, double* __restrict__ J) {
+__global__ void k_j0_table(const double* __restrict__ kb, int nk, long long nkey, double dr, SnKeys K,
+                           double* __restrict__ J) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nk * nkey) return;
     const int a = (int)(t / nkey);
     const long long key = t % nkey;
-    const double x = kb[a] * (sqrt((double)key) * dr);
+    const double x = kb[a] * sn_key_r(K, key, dr);
     J[t] = (x == 0.0) ? 1.0 : sin(x) / x;
 }
 
@@ -1112,6 +1177,7 @@ struct YHistArgs {
     int m1[YH_MAXC], m2[YH_MAXC];
     double w[YH_MAXC];
     long long nkey;
+    SnKeys keys;
 };
 
 template <typename T>
@@ -1126,12 +1192,18 @@ __global__ __launch_bounds__(256) void k_yhist(const C<T>* __restrict__ xi, YHis
         const int j = (int)(r % a.n[1]), i = (int)(r / a.n[1]);
         const int si = i < a.n[0] / 2 ? i : i - a.n[0], sj = j < a.n[1] / 2 ? j : j - a.n[1],
                   sk = k < a.n[2] / 2 ? k : k - a.n[2];
-        const long long key = (long long)si * si + (long long)sj * sj + (long long)sk * sk;
+        const long long key =
+            a.keys.aniso ? ((long long)abs(si) * a.keys.e1 + abs(sj)) * a.keys.e2 + abs(sk)
+                         : (long long)si * si + (long long)sj * sj + (long long)sk * sk;
         if (key >= a.nkey) continue;
+        // the direction of r (in cell units when the cells are cubes)
+        const double vx = a.keys.aniso ? (double)si * a.keys.d[0] : (double)si;
+        const double vy = a.keys.aniso ? (double)sj * a.keys.d[1] : (double)sj;
+        const double vz = a.keys.aniso ? (double)sk * a.keys.d[2] : (double)sk;
         double ar = 0.0, ai = 0.0;
         for (int c = 0; c < a.nc; c++) {
-            const Cplx y1 = ylm_red(a.l1, a.m1[c], (double)si, (double)sj, (double)sk);
-            const Cplx y2 = ylm_red(a.l2, a.m2[c], (double)si, (double)sj, (double)sk);
+            const Cplx y1 = ylm_red(a.l1, a.m1[c], vx, vy, vz);
+            const Cplx y2 = ylm_red(a.l2, a.m2[c], vx, vy, vz);
             ar += a.w[c] * (y1.re * y2.re - y1.im * y2.im);
             ai += a.w[c] * (y1.re * y2.im + y1.im * y2.re);
         }
@@ -1216,7 +1288,7 @@ __global__ __launch_bounds__(256) void k_full_over_w(const C<T>* __restrict__ f,
 struct BoxPlanImpl {
     hm_box_config cfg;
     // launch shapes (the autotuner's choice; results do not depend on them)
-    unsigned tune_wide = 0;   // wide column tiles, bit per tunable length (256, 288, 512)
+    unsigned tune_wide = 0;   // wide column tiles, bit per tunable length (256, 270 or 288, 512)
     int tune_bin = 256;       // threads per block of the P_l binning
     Mesh mesh;
     int nk = 0, ne = 0;
@@ -1228,8 +1300,9 @@ struct BoxPlanImpl {
     int M[3];
     int nzmax = 0, s01 = -1, nrow_y = 0;
     long long nrows = 0;
-    long long nkey = 0;   // |n|^2 keys
-    long long amax = 0;   // largest i'^2 + j'^2
+    long long nkey = 0;   // shot-noise radial keys (SnKeys)
+    long long amax = 0;   // largest row key
+    SnKeys keys;
     int na = 0;           // distinct i'^2 + j'^2 over the rows of the mesh
     SortWork sort;
     std::vector<short2> pairs;
@@ -1276,6 +1349,10 @@ struct BoxPlanImpl {
     DevBuf gpairs_d;  // (a, nk + b): the cross sums of the two Y-weighted shell sets
     // three-point function
     bool zeta = false, z_gen = false, z_odd = false;
+    // an odd mesh length: the reference's separation rule (i < n/2 ? i : i - n, bispec.hpp:1040-1042) gives
+    // the planes next to the Nyquist index different |r| at x and -x, so the odd part of xi enters the
+    // shot-noise sums even for even l1 + l2
+    bool odd_mesh = false;
     int zl[3] = {0, 0, 0};
     std::vector<int> zmlist;
     std::vector<double> zwlist;
@@ -1405,11 +1482,13 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
         return nullptr;
     };
     for (int a = 0; a < 3; a++) {
-        if (!fft_length_supported(m.n[a]) || (m.n[a] & (m.n[a] - 1)))
-            return fail("box plan: n_mesh must be a power of two in [8, 1024]");
+        if (m.n[a] < 8 || !fft_length_supported(m.n[a]))
+            return fail("box plan: n_mesh must be in [8, " + std::to_string(RT_MAX_L) + "]");
         if (!(m.L[a] > 0)) return fail("box plan: boxsize must be > 0");
     }
     if (m.n[2] < 16) return fail("box plan: n_mesh_z must be >= 16");
+    if (m.n[2] % 2) return fail("box plan: n_mesh_z must be even (real-to-complex transform of the z rows)");
+    p->odd_mesh = (m.n[0] % 2) != 0 || (m.n[1] % 2) != 0;
     if (m.assign < 0 || m.assign > 2) return fail("box plan: assign must be NGP, CIC or TSC");
     if (cfg->n_kbin < 2 || cfg->n_kbin > 64) return fail("box plan: n_kbin must be in [2, 64]");
     if (cfg->n_pk_ells < 0 || cfg->n_pk_ells > HM_MAX_ELLS) return fail("box plan: n_pk_ells out of range");
@@ -1511,18 +1590,30 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     if (cfg->bk && !p->bk_gen && p->pairs.size() > 256) return fail("box plan: at most 22 k-bins for the bispectrum");
     if (cfg->bk) {
         const double dr0 = m.L[0] / m.n[0], dr1 = m.L[1] / m.n[1], dr2 = m.L[2] / m.n[2];
-        if (!(dr0 == dr1 && dr1 == dr2)) return fail("box plan: the bispectrum needs equal cell sizes on all axes");
-        const long long h0 = m.n[0] / 2, h1 = m.n[1] / 2, h2 = m.n[2] / 2;
-        p->nkey = h0 * h0 + h1 * h1 + h2 * h2 + 1;
-        // rows (i, j) grouped by the minimum-image i'^2 + j'^2
-        p->amax = h0 * h0 + h1 * h1;
+        // signed indices of the rvec rule (bispec.hpp:1040-1042): |i'| <= n - n/2
+        const long long h0 = m.n[0] - m.n[0] / 2, h1 = m.n[1] - m.n[1] / 2, h2 = m.n[2] / 2;
+        p->keys.aniso = !(dr0 == dr1 && dr1 == dr2);
+        if (p->keys.aniso) {
+            p->keys.e1 = (int)(h1 + 1);
+            p->keys.e2 = (int)(h2 + 1);
+            p->keys.d[0] = dr0;
+            p->keys.d[1] = dr1;
+            p->keys.d[2] = dr2;
+            p->nkey = (h0 + 1) * (h1 + 1) * (h2 + 1);
+            p->amax = (h0 + 1) * (h1 + 1) - 1;
+        } else {
+            p->nkey = h0 * h0 + h1 * h1 + h2 * h2 + 1;
+            p->amax = h0 * h0 + h1 * h1;
+        }
+        // rows (i, j) grouped by their row key: the minimum-image i'^2 + j'^2, or (|i'|, |j'|)
         std::vector<int> cnt((size_t)p->amax + 1, 0), key((size_t)m.n[0] * m.n[1]);
         for (int i = 0; i < m.n[0]; i++) {
             long long a = (i < m.n[0] / 2) ? i : i - m.n[0];
             for (int j = 0; j < m.n[1]; j++) {
                 long long b = (j < m.n[1] / 2) ? j : j - m.n[1];
-                key[(size_t)i * m.n[1] + j] = (int)(a * a + b * b);
-                cnt[(size_t)(a * a + b * b)]++;
+                const long long rk = p->keys.aniso ? std::llabs(a) * p->keys.e1 + std::llabs(b) : a * a + b * b;
+                key[(size_t)i * m.n[1] + j] = (int)rk;
+                cnt[(size_t)rk]++;
             }
         }
         std::vector<int> a_idx((size_t)p->amax + 1, -1), a_start(1, 0);
@@ -1725,7 +1816,7 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
             if (l == 0)
                 return ctx->launch("j0_table", [&] {
                     hipLaunchKernelGGL(k_j0_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
-                                       (const double*)p->kb_d.p, p->nk, p->nkey, dr, (double*)out.p);
+                                       (const double*)p->kb_d.p, p->nk, p->nkey, dr, p->keys, (double*)out.p);
                 });
             std::vector<double> sy, sc;
             jl_spline(l, sy, sc);
@@ -1742,7 +1833,7 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
                 rc = ctx->launch("jl_table", [&] {
                     hipLaunchKernelGGL(k_jl_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
                                        (const double*)dy.p, (const double*)dc.p, (const double*)p->kb_d.p, p->nk,
-                                       p->nkey, dr, (double*)out.p);
+                                       p->nkey, dr, p->keys, (double*)out.p);
                 });
             if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = -1;
             dev_free(dy);
@@ -2348,6 +2439,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             if (fft3d_c2c<T>(ctx, cx, n3, +1)) return -1;
             for (int a = 0; a < 3; a++) ha.n[a] = m.n[a];
             ha.nkey = p->nkey;
+            ha.keys = p->keys;
             if (fill_zero(ctx, p->yh.p, sizeof(double) * 2 * (size_t)p->nkey)) return -1;
             if (ctx->launch("svg_hist", [&] {
                     hipLaunchKernelGGL(k_yhist<T>, dim3(gb), dim3(256), 0, ctx->stream, (const C<T>*)cx, ha, Hre, Him);
@@ -2416,7 +2508,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         // Re xi; for an odd l1 + l2 also Im xi (field.hpp:1116-1160 keeps both: an odd angular weight sees
         // the odd part of xi, and the even part on the z Nyquist plane), and always for the complex cross
         // spectrum of a reconstructed box
-        for (int par = 0; par < ((p->sn_odd || recon) ? 2 : 1); par++) {
+        for (int par = 0; par < ((p->sn_odd || p->odd_mesh || recon) ? 2 : 1); par++) {
         xs.odd = par;
         void* hbuf = par ? p->hist2.p : p->hist.p;
         void* sbuf = par ? p->sn_pairs2.p : p->sn_pairs.p;
@@ -2436,7 +2528,15 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             return -1;
         if (fft_c2r_rows<T>(ctx, "xi_pass_z", m.n[2] / 2, xg, xr, (long long)p->na, -1)) return -1;
         if (ctx->launch("xi_hist", [&] {
-                if (p->bk_gen)
+                if (p->keys.aniso && p->bk_gen)
+                    hipLaunchKernelGGL(k_xi_hist_ang_aniso<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->keys, p->nkey, p->ang,
+                                       (double*)hbuf);
+                else if (p->keys.aniso)
+                    hipLaunchKernelGGL(k_xi_hist_aniso<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->keys, p->nkey,
+                                       (double*)hbuf);
+                else if (p->bk_gen)
                     hipLaunchKernelGGL(k_xi_hist_ang<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0,
                                        ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey, p->ang,
                                        (double*)hbuf);
@@ -2713,7 +2813,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         xs.dk2 = dk2;
         C<T>* xh = (C<T>*)p->xi_half.p;
         const int cx = support_count(p->zrs[0], m.n[0]), cy = support_count(p->zrs[1], m.n[1]);
-        for (int par = 0; par < ((p->z_odd || recon) ? 2 : 1); par++) {
+        for (int par = 0; par < ((p->z_odd || p->odd_mesh || recon) ? 2 : 1); par++) {
             xs.odd = par;
             if (launch_cols<T, +1>(ctx, "zsn_pass_x", m.n[0], xs, xh,
                                    geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh),
@@ -2854,7 +2954,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                              hipMemcpyDeviceToHost, ctx->stream));
         HM_HIP_CHECK(ctx, hipMemcpyAsync(snp.data(), p->sn_pairs.p, sizeof(double) * npairs, hipMemcpyDeviceToHost,
                                          ctx->stream));
-        if (p->sn_odd || recon)
+        if (p->sn_odd || p->odd_mesh || recon)
             HM_HIP_CHECK(ctx, hipMemcpyAsync(snp2.data(), p->sn_pairs2.p, sizeof(double) * npairs,
                                              hipMemcpyDeviceToHost, ctx->stream));
     }
@@ -3148,11 +3248,14 @@ int plan_finalize_device(hm_ctx* ctx, BoxPlanImpl* p, const double* dyn, double*
 // per mode; the tests compare them bit for bit).
 int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np, int reps, std::string& report) {
     if (reps < 1) reps = 1;
-    const int lens[3] = {256, 288, 512};
+    // bit b: the wide tile of length 256, 270 or 288 (one bit), 512
+    auto tunable = [](int L) { return L == 256 ? 0 : ((L == 270 || L == 288) ? 1 : (L == 512 ? 2 : -1)); };
     unsigned avail = 0;
-    for (int b = 0; b < 3; b++)
-        for (int a = 0; a < 3; a++)
-            if (p->mesh.n[a] == lens[b] || p->M[a] == lens[b]) avail |= 1u << b;
+    for (int a = 0; a < 3; a++) {
+        const int b0 = tunable(p->mesh.n[a]), b1 = tunable(p->M[a]);
+        if (b0 >= 0) avail |= 1u << b0;
+        if (b1 >= 0) avail |= 1u << b1;
+    }
     if (p->cfg.precision == HM_FP64) avail = 0;  // the wide tiles exist for FP32 only
     const int bins[3] = {256, 128, 64};
     const unsigned keep_wide = p->tune_wide;
@@ -3194,7 +3297,7 @@ int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np
     }
     p->tune_wide = best_wide;
     p->tune_bin = best_bin;
-    snprintf(buf, sizeof(buf), "chosen: wide=%u%u%u (256, 288, 512) bin=%d, %.3f ms", best_wide & 1u,
+    snprintf(buf, sizeof(buf), "chosen: wide=%u%u%u (256, 270/288, 512) bin=%d, %.3f ms", best_wide & 1u,
              (best_wide >> 1) & 1u, (best_wide >> 2) & 1u, best_bin, best);
     report += buf;
     return 0;

@@ -106,11 +106,9 @@ int fill_zero(hm_ctx* ctx, void* p, size_t bytes) {
     });
 }
 
-bool fft_length_supported(int L) {
-    for (int v : kFftLengths)
-        if (v == L) return true;
-    return false;
-}
+// Every length in [2, RT_MAX_L]: a compile-time radix plan where one exists
+// (kFftLengths), the run-time plan otherwise.
+bool fft_length_supported(int L) { return L >= 2 && L <= RT_MAX_L; }
 
 int fft_length_at_least(int n) {
     for (int v : kFftLengths)

@@ -99,6 +99,82 @@ template <typename T, int DIR> struct Dft<T, 5, DIR> {
     }
 };
 
+// Odd radices 7 .. 23: y_0 = sum_t a_t; for u = 1 .. (R-1)/2, with
+// s_t = a_t + a_{R-t}, d_t = a_t - a_{R-t} (t = 1 .. (R-1)/2),
+// y_u = a_0 + sum_t s_t cos(2 pi t u / R) + DIR i sum_t d_t sin(2 pi t u / R)
+// and y_{R-u} the same with the sine term negated.  The cosines and sines are
+// compile-time constants (Taylor series of the reduced angle).
+constexpr double ce_sin(double x) {
+    double t = x, s = x;
+    for (int k = 1; k < 30; k++) {
+        t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        s += t;
+    }
+    return s;
+}
+constexpr double ce_cos(double x) {
+    double t = 1.0, s = 1.0;
+    for (int k = 1; k < 30; k++) {
+        t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        s += t;
+    }
+    return s;
+}
+template <int R> struct RootTab {
+    double c[R], s[R];  // cos, sin of 2 pi m / R
+    constexpr RootTab() : c(), s() {
+        for (int m = 0; m < R; m++) {
+            const int mm = (2 * m > R) ? m - R : m;  // angle reduced to (-pi, pi]
+            const double a = 6.283185307179586476925286766559 * (double)mm / (double)R;
+            c[m] = ce_cos(a);
+            s[m] = ce_sin(a);
+        }
+    }
+};
+template <int R> inline constexpr RootTab<R> kRoot{};
+
+template <typename T, int R, int DIR> struct DftOdd {
+    __device__ static inline void run(C<T>* a) {
+        constexpr int H = (R - 1) / 2;
+        C<T> sp[H], dm[H];
+#pragma unroll
+        for (int t = 1; t <= H; t++) {
+            sp[t - 1] = cadd<T>(a[t], a[R - t]);
+            dm[t - 1] = csub<T>(a[t], a[R - t]);
+        }
+        C<T> y0 = a[0];
+#pragma unroll
+        for (int t = 0; t < H; t++) y0 = cadd<T>(y0, sp[t]);
+        C<T> out[R];
+        out[0] = y0;
+#pragma unroll
+        for (int u = 1; u <= H; u++) {
+            T ax = a[0].x, ay = a[0].y, bx = (T)0, by = (T)0;
+#pragma unroll
+            for (int t = 1; t <= H; t++) {
+                const T c = (T)kRoot<R>.c[(t * u) % R], sn = (T)kRoot<R>.s[(t * u) % R];
+                ax += sp[t - 1].x * c;
+                ay += sp[t - 1].y * c;
+                bx += dm[t - 1].x * sn;
+                by += dm[t - 1].y * sn;
+            }
+            // DIR i (bx + i by) = DIR (-by + i bx)
+            const C<T> ib = mk<T>((T)(-DIR) * by, (T)DIR * bx);
+            out[u] = mk<T>(ax + ib.x, ay + ib.y);
+            out[R - u] = mk<T>(ax - ib.x, ay - ib.y);
+        }
+#pragma unroll
+        for (int u = 0; u < R; u++) a[u] = out[u];
+    }
+};
+template <typename T, int DIR> struct Dft<T, 7, DIR> : DftOdd<T, 7, DIR> {};
+template <typename T, int DIR> struct Dft<T, 11, DIR> : DftOdd<T, 11, DIR> {};
+template <typename T, int DIR> struct Dft<T, 13, DIR> : DftOdd<T, 13, DIR> {};
+template <typename T, int DIR> struct Dft<T, 17, DIR> : DftOdd<T, 17, DIR> {};
+template <typename T, int DIR> struct Dft<T, 19, DIR> : DftOdd<T, 19, DIR> {};
+template <typename T, int DIR> struct Dft<T, 23, DIR> : DftOdd<T, 23, DIR> {};
+template <typename T, int DIR> struct Dft<T, 9, DIR> : DftOdd<T, 9, DIR> {};
+
 // ---------------------------------------------------------------------------
 // Stockham stages over a tile in LDS: sm[b * LP + p], b < B, p < L
 // ---------------------------------------------------------------------------
@@ -194,11 +270,12 @@ __device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw, i
     else if constexpr (L == 720) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 3, 5>(sm, tw, tid);
     else if constexpr (L == 800) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 5, 5>(sm, tw, tid);
     else if constexpr (L == 960) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 270) lds_stockham<T, DIR, L, B, NT, SW, 1, 9, 5, 3, 2>(sm, tw, tid);
     else static_assert(L == 8, "unsupported FFT length");
 }
 
 // Lengths with a radix plan, ascending.
-static constexpr int kFftLengths[] = {8, 16, 24, 32, 40, 48, 64, 72, 80, 96, 120, 128, 144, 160, 192, 200, 216, 240, 256, 288, 320, 360, 384, 400, 432, 480, 512, 576, 600, 640, 720, 768, 800, 960, 1024};
+static constexpr int kFftLengths[] = {8, 16, 24, 32, 40, 48, 64, 72, 80, 96, 120, 128, 144, 160, 192, 200, 216, 240, 256, 270, 288, 320, 360, 384, 400, 432, 480, 512, 576, 600, 640, 720, 768, 800, 960, 1024};
 
 constexpr int pow2floor(int x) {
     int r = 1;
@@ -207,12 +284,12 @@ constexpr int pow2floor(int x) {
 }
 
 // Tile width (lines per workgroup) for a column pass: up to 128 B per point,
-// a power of two, within 64 KB of LDS.  FP32 lines of 288 points take 8
+// a power of two, within 64 KB of LDS.  FP32 lines of 270 and 288 points take 8
 // columns (64 B per point): the triple mesh's shell passes run faster with the
 // smaller tiles (more workgroups in flight per CU); the autotuner can still
 // choose 16 (the wide tile).
 template <typename T, int L> struct ColTile {
-    static constexpr int want = (sizeof(T) == 4 && L == 288) ? 8 : (int)(128 / sizeof(C<T>));
+    static constexpr int want = (sizeof(T) == 4 && (L == 288 || L == 270)) ? 8 : (int)(128 / sizeof(C<T>));
     static constexpr int fit = (int)(65536 / (LdsPad<L>::LP * sizeof(C<T>)));
     static constexpr int B = pow2floor(want < fit ? want : fit);
 };
@@ -449,8 +526,292 @@ template <typename T, int L> constexpr size_t col_lds() { return (size_t)ColTile
 // the wide column tile (twice ColTile's width) of the FP32 lengths the box
 // plans use most; the autotuner chooses per length whether to use it
 template <typename T, int L> constexpr bool has_wide_cols() {
-    return sizeof(T) == 4 && (L == 256 || L == 288 || L == 512);
+    return sizeof(T) == 4 && (L == 256 || L == 270 || L == 288 || L == 512);
 }
 template <typename T, int L> constexpr size_t row_lds() { return (size_t)RowTile<T, L>::B * LdsPad<L>::LP * sizeof(C<T>); }
+
+// ---------------------------------------------------------------------------
+// Any length: the radix plan is a kernel argument.  Lengths without a
+// compile-time plan (every L in [2, RT_MAX_L]) run through these kernels:
+// radix 2, 3, 4, 5, 7, 8, 9, 11, 13, 17, 19 and 23 butterflies with the
+// twiddle table of the length, and for any other prime factor p a direct
+// p-point DFT per output (O(p) per point and stage).  A tile holds at most
+// RtMax<T>::PTS points, so each lane keeps at most RtMax<T>::PTS / 256
+// butterflies of a stage in registers between its LDS read and write.
+// ---------------------------------------------------------------------------
+constexpr int RT_MAX_L = 2048;
+constexpr int RT_MAX_STAGES = 24;
+template <typename T> struct RtMax { static constexpr int PTS = sizeof(T) == 4 ? 4096 : 2048; };
+
+struct RtPlan {
+    int L, LP, B, ns;
+    int r[RT_MAX_STAGES];
+};
+
+// Radices of L: 8s, then 4, 2, then 3 (9s first), 5, 7, 11, 13, 17, 19, 23,
+// then any remaining prime factors.  B: lines per tile (a power of two with
+// B * L <= max_pts, at most want_b).  false: L outside [2, RT_MAX_L].
+inline bool rt_plan_make(int L, int max_pts, int want_b, RtPlan& p) {
+    if (L < 2 || L > RT_MAX_L) return false;
+    p.L = L;
+    p.LP = L + L / 32 + 1;
+    p.ns = 0;
+    int m = L;
+    auto take = [&](int r) {
+        while (m % r == 0 && p.ns < RT_MAX_STAGES) {
+            p.r[p.ns++] = r;
+            m /= r;
+        }
+    };
+    take(8);
+    take(4);
+    take(2);
+    take(9);
+    for (int r : {3, 5, 7, 11, 13, 17, 19, 23}) take(r);
+    for (int f = 29; m > 1 && f <= m; f += 2) take(f);
+    if (m != 1) return false;
+    int b = 1;
+    while (2 * b <= want_b && 2 * b * L <= max_pts) b *= 2;
+    p.B = b;
+    return true;
+}
+
+__device__ __forceinline__ int rt_lpos(int p) { return p + (p >> 5); }
+
+template <typename T, int DIR, int R, int QM>
+__device__ __forceinline__ void rt_stage(C<T>* sm, const C<T>* __restrict__ tw, int L, int LP, int B, int NS, int tid) {
+    const int NBF = L / R;
+    const int TOT = B * NBF;
+    const int tstep = L / (NS * R);
+    C<T> v[QM][R];
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+        const int idx = tid + q * FFT_NT;
+        if (idx < TOT) {
+            const int b = idx / NBF, j = idx - (idx / NBF) * NBF;
+            const int k = j % NS;
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                C<T> a = sm[b * LP + rt_lpos(j + t * NBF)];
+                if (t > 0 && NS > 1) {
+                    C<T> w = tw[t * k * tstep];
+                    if (DIR > 0) w.y = -w.y;
+                    a = cmul<T>(a, w);
+                }
+                v[q][t] = a;
+            }
+            Dft<T, R, DIR>::run(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+        const int idx = tid + q * FFT_NT;
+        if (idx < TOT) {
+            const int b = idx / NBF, j = idx - (idx / NBF) * NBF;
+            const int k = j % NS;
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int t = 0; t < R; t++) sm[b * LP + rt_lpos(base + t * NS)] = v[q][t];
+        }
+    }
+    __syncthreads();
+}
+
+// A stage of any radix R (a prime factor above 23): each lane forms whole
+// outputs, y = sum_t (a_t w^(t k)) exp(DIR 2 pi i t u / R), from the table.
+template <typename T, int DIR, int QO>
+__device__ __forceinline__ void rt_stage_any(C<T>* sm, const C<T>* __restrict__ tw, int L, int LP, int B, int NS, int R,
+                                             int tid) {
+    const int NBF = L / R;
+    const int tstep = L / (NS * R), rstep = L / R;
+    const int TOT = B * L;
+    C<T> out[QO];
+#pragma unroll
+    for (int q = 0; q < QO; q++) {
+        const int o = tid + q * FFT_NT;
+        C<T> acc = mk<T>((T)0, (T)0);
+        if (o < TOT) {
+            const int b = o / L, pp = o - (o / L) * L;
+            const int k = pp % NS, u = (pp / NS) % R, j = (pp / (NS * R)) * NS + k;
+            for (int t = 0; t < R; t++) {
+                C<T> a = sm[b * LP + rt_lpos(j + t * NBF)];
+                if (t > 0 && NS > 1) {
+                    C<T> w = tw[t * k * tstep];
+                    if (DIR > 0) w.y = -w.y;
+                    a = cmul<T>(a, w);
+                }
+                C<T> w = tw[((t * u) % R) * rstep];
+                if (DIR > 0) w.y = -w.y;
+                acc = cadd<T>(acc, cmul<T>(a, w));
+            }
+        }
+        out[q] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QO; q++) {
+        const int o = tid + q * FFT_NT;
+        if (o < TOT) sm[(o / L) * LP + rt_lpos(o - (o / L) * L)] = out[q];
+    }
+    __syncthreads();
+}
+
+template <typename T, int DIR>
+__device__ void rt_fft(C<T>* sm, const C<T>* __restrict__ tw, const RtPlan& pl, int tid) {
+    constexpr int P = RtMax<T>::PTS;
+#define HM_RT_Q(R) ((P / (R) + FFT_NT - 1) / FFT_NT)
+    int NS = 1;
+    for (int s = 0; s < pl.ns; s++) {
+        const int R = pl.r[s];
+        switch (R) {
+        case 2: rt_stage<T, DIR, 2, HM_RT_Q(2)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 3: rt_stage<T, DIR, 3, HM_RT_Q(3)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 4: rt_stage<T, DIR, 4, HM_RT_Q(4)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 5: rt_stage<T, DIR, 5, HM_RT_Q(5)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 7: rt_stage<T, DIR, 7, HM_RT_Q(7)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 8: rt_stage<T, DIR, 8, HM_RT_Q(8)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 9: rt_stage<T, DIR, 9, HM_RT_Q(9)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 11: rt_stage<T, DIR, 11, HM_RT_Q(11)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 13: rt_stage<T, DIR, 13, HM_RT_Q(13)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 17: rt_stage<T, DIR, 17, HM_RT_Q(17)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 19: rt_stage<T, DIR, 19, HM_RT_Q(19)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        case 23: rt_stage<T, DIR, 23, HM_RT_Q(23)>(sm, tw, pl.L, pl.LP, pl.B, NS, tid); break;
+        default: rt_stage_any<T, DIR, P / FFT_NT>(sm, tw, pl.L, pl.LP, pl.B, NS, R, tid); break;
+        }
+        NS *= R;
+    }
+#undef HM_RT_Q
+}
+
+template <typename T, int DIR, class Src>
+__global__ __launch_bounds__(FFT_NT) void k_fft_cols_rt(Src src, C<T>* __restrict__ dst, ColGeom g,
+                                                       const C<T>* __restrict__ tw, RtPlan pl) {
+    const int B = pl.B, LP = pl.LP, L = pl.L;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const uint3 wg = xcd_block();
+    const int f = (int)wg.z;
+    int outer_s = g.outer_s, inner_s = g.inner_s, point_s = g.point_s, n_inner = g.n_inner;
+    if (g.fsup) {
+        const int4 q = g.fsup[f];
+        if (g.role_outer >= 0) {
+            outer_s = int4_get(q, g.role_outer);
+            if ((int)wg.y >= support_count(outer_s, g.outer_full)) return;
+        }
+        if (g.role_inner >= 0) {
+            inner_s = int4_get(q, g.role_inner);
+            n_inner = support_count(inner_s, g.inner_full);
+            if ((int)wg.x * B >= n_inner) return;
+        }
+        if (g.role_point >= 0) point_s = int4_get(q, g.role_point);
+    }
+    const int o = support_map((int)wg.y, outer_s, g.outer_full);
+    const long long obase = (long long)f * g.FS + (long long)o * g.OS;
+    const int tid = threadIdx.x;
+    const int bl = tid % B;
+    const int bc = (int)wg.x * B + bl;
+    const bool bvalid = bc < n_inner;
+    const int b = bvalid ? support_map(bc, inner_s, g.inner_full) : 0;
+    const int pstep = FFT_NT / B;
+    for (int p = tid / B; p < L; p += pstep) {
+        C<T> v = mk<T>((T)0, (T)0);
+        if (bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
+        sm[bl * LP + rt_lpos(p)] = v;
+    }
+    __syncthreads();
+    rt_fft<T, DIR>(sm, tw, pl, tid);
+    if (bvalid) {
+        for (int p = tid / B; p < L; p += pstep) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + rt_lpos(p)];
+    }
+}
+
+template <typename T, int DIR>
+__global__ __launch_bounds__(FFT_NT) void k_fft_rows_rt(const C<T>* __restrict__ src, C<T>* __restrict__ dst, RowGeom g,
+                                                       const C<T>* __restrict__ tw, RtPlan pl) {
+    const int B = pl.B, LP = pl.LP, L = pl.L;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < g.n_rows && in_support(p, g.fps ? g.fps[(r0 + b) / g.rpf] : g.point_s, L))
+            v = src[(r0 + b) * g.RS_in + p];
+        sm[b * LP + rt_lpos(p)] = v;
+    }
+    __syncthreads();
+    rt_fft<T, DIR>(sm, tw, pl, threadIdx.x);
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        if (r0 + b < g.n_rows) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + rt_lpos(p)];
+    }
+}
+
+// k_r2c_rows and k_c2r_rows with a run-time half length L = pl.L.
+template <typename T, class RSrc>
+__global__ __launch_bounds__(FFT_NT) void k_r2c_rows_rt(RSrc src, C<T>* __restrict__ dst, long long n_rows,
+                                                       const C<T>* __restrict__ tw, const C<T>* __restrict__ tw2,
+                                                       RtPlan pl) {
+    const int B = pl.B, LP = pl.LP, L = pl.L;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < n_rows) v = src(r0 + b, p);
+        sm[b * LP + rt_lpos(p)] = v;
+    }
+    __syncthreads();
+    rt_fft<T, -1>(sm, tw, pl, threadIdx.x);
+    for (int e = threadIdx.x; e < B * (L + 1); e += FFT_NT) {
+        const int b = e / (L + 1), k = e % (L + 1);
+        if (r0 + b >= n_rows) continue;
+        const C<T> zk = sm[b * LP + rt_lpos(k % L)];
+        const C<T> zc = cconj<T>(sm[b * LP + rt_lpos((L - k) % L)]);
+        const C<T> E = mk<T>((T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y));
+        const C<T> d = mk<T>((T)0.5 * (zk.x - zc.x), (T)0.5 * (zk.y - zc.y));
+        const C<T> O = mk<T>(d.y, -d.x);
+        const C<T> w = tw2[k];
+        dst[(r0 + b) * (L + 1) + k] = cadd<T>(E, cmul<T>(w, O));
+    }
+}
+
+template <typename T, class HSrc>
+__global__ __launch_bounds__(FFT_NT) void k_c2r_rows_rt(HSrc src, T* __restrict__ dst, long long n_rows, int point_s,
+                                                       const C<T>* __restrict__ tw, const C<T>* __restrict__ tw2,
+                                                       RtPlan pl) {
+    const int B = pl.B, LP = pl.LP, L = pl.L;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, k = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < n_rows) {
+            const C<T> xk = (point_s < 0 || k <= point_s) ? src(r0 + b, k) : mk<T>((T)0, (T)0);
+            const int kc = L - k;
+            const C<T> xc = (point_s < 0 || kc <= point_s) ? cconj<T>(src(r0 + b, kc)) : mk<T>((T)0, (T)0);
+            const C<T> sv = cadd<T>(xk, xc);
+            const C<T> d = csub<T>(xk, xc);
+            C<T> w = tw2[k];
+            w.y = -w.y;
+            const C<T> wd = cmul<T>(w, d);
+            v = mk<T>(sv.x - wd.y, sv.y + wd.x);
+        }
+        sm[b * LP + rt_lpos(k)] = v;
+    }
+    __syncthreads();
+    rt_fft<T, +1>(sm, tw, pl, threadIdx.x);
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, m = e % L;
+        if (r0 + b < n_rows) {
+            const C<T> z = sm[b * LP + rt_lpos(m)];
+            dst[(r0 + b) * (2 * L) + 2 * m] = z.x;
+            dst[(r0 + b) * (2 * L) + 2 * m + 1] = z.y;
+        }
+    }
+}
 
 }  // namespace hm

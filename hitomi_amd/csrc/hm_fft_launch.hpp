@@ -5,7 +5,7 @@
 #include "hm_internal.hpp"
 
 #define HM_POW2_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
-#define HM_MIXED_LENGTHS(X) X(24) X(40) X(48) X(72) X(80) X(96) X(120) X(144) X(160) X(192) X(200) X(216) X(240) X(288) X(320) X(360) X(384) X(400) X(432) X(480) X(576) X(600) X(640) X(720) X(768) X(800) X(960)
+#define HM_MIXED_LENGTHS(X) X(24) X(40) X(48) X(72) X(80) X(96) X(120) X(144) X(160) X(192) X(200) X(216) X(240) X(270) X(288) X(320) X(360) X(384) X(400) X(432) X(480) X(576) X(600) X(640) X(720) X(768) X(800) X(960)
 
 namespace hm {
 
@@ -37,6 +37,83 @@ int launch_cols_L(hm_ctx* ctx, const char* name, const Src& src, C<T>* dst, cons
     });
 }
 
+// Lengths without a compile-time plan: the run-time radix plan.
+template <typename T, int DIR, class Src>
+int launch_cols_rt(hm_ctx* ctx, const char* name, int L, const Src& src, C<T>* dst, const ColGeom& g) {
+    RtPlan pl;
+    if (!rt_plan_make(L, RtMax<T>::PTS, (int)(128 / sizeof(C<T>)), pl)) {
+        ctx->set_error("fft: unsupported line length " + std::to_string(L));
+        return -1;
+    }
+    if (g.n_inner <= 0 || g.n_outer <= 0) return 0;
+    if (g.n_outer > 65535) {
+        ctx->set_error("fft column pass: outer extent exceeds 65535");
+        return -1;
+    }
+    const C<T>* tw = ctx->twiddle<T>(L);
+    if (!tw) return -1;
+    dim3 grid((unsigned)((g.n_inner + pl.B - 1) / pl.B), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
+    const size_t lds = (size_t)pl.B * pl.LP * sizeof(C<T>);
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_fft_cols_rt<T, DIR, Src>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw, pl);
+    });
+}
+
+template <typename T, int DIR>
+int launch_rows_rt(hm_ctx* ctx, const char* name, int L, const C<T>* src, C<T>* dst, const RowGeom& g) {
+    RtPlan pl;
+    if (!rt_plan_make(L, RtMax<T>::PTS, 4096, pl)) {
+        ctx->set_error("fft: unsupported line length " + std::to_string(L));
+        return -1;
+    }
+    if (g.n_rows <= 0) return 0;
+    const C<T>* tw = ctx->twiddle<T>(L);
+    if (!tw) return -1;
+    dim3 grid((unsigned)((g.n_rows + pl.B - 1) / pl.B));
+    const size_t lds = (size_t)pl.B * pl.LP * sizeof(C<T>);
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_fft_rows_rt<T, DIR>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw, pl);
+    });
+}
+
+template <typename T, class RSrc>
+int launch_r2c_rt(hm_ctx* ctx, const char* name, int L2, const RSrc& src, C<T>* dst, long long n_rows) {
+    RtPlan pl;
+    if (!rt_plan_make(L2, RtMax<T>::PTS, 4096, pl)) {
+        ctx->set_error("fft r2c: unsupported half length " + std::to_string(L2));
+        return -1;
+    }
+    if (n_rows <= 0) return 0;
+    const C<T>* tw = ctx->twiddle<T>(L2);
+    const C<T>* tw2 = ctx->twiddle<T>(2 * L2);
+    if (!tw || !tw2) return -1;
+    dim3 grid((unsigned)((n_rows + pl.B - 1) / pl.B));
+    const size_t lds = (size_t)pl.B * pl.LP * sizeof(C<T>);
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_r2c_rows_rt<T, RSrc>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, n_rows, tw, tw2,
+                           pl);
+    });
+}
+
+template <typename T, class HSrc>
+int launch_c2r_rt(hm_ctx* ctx, const char* name, int L2, const HSrc& src, T* dst, long long n_rows, int point_s) {
+    RtPlan pl;
+    if (!rt_plan_make(L2, RtMax<T>::PTS, 4096, pl)) {
+        ctx->set_error("fft c2r: unsupported half length " + std::to_string(L2));
+        return -1;
+    }
+    if (n_rows <= 0) return 0;
+    const C<T>* tw = ctx->twiddle<T>(L2);
+    const C<T>* tw2 = ctx->twiddle<T>(2 * L2);
+    if (!tw || !tw2) return -1;
+    dim3 grid((unsigned)((n_rows + pl.B - 1) / pl.B));
+    const size_t lds = (size_t)pl.B * pl.LP * sizeof(C<T>);
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_c2r_rows_rt<T, HSrc>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, n_rows, point_s,
+                           tw, tw2, pl);
+    });
+}
+
 template <typename T, int DIR, class Src>
 int launch_cols(hm_ctx* ctx, const char* name, int L, const Src& src, C<T>* dst, const ColGeom& g) {
     switch (L) {
@@ -47,8 +124,7 @@ int launch_cols(hm_ctx* ctx, const char* name, int L, const Src& src, C<T>* dst,
 #undef HM_CASE
     default: break;
     }
-    ctx->set_error("fft: unsupported line length " + std::to_string(L));
-    return -1;
+    return launch_cols_rt<T, DIR, Src>(ctx, name, L, src, dst, g);
 }
 
 template <typename T, int L, int DIR>
@@ -74,8 +150,7 @@ int launch_rows(hm_ctx* ctx, const char* name, int L, const C<T>* src, C<T>* dst
 #undef HM_CASE
     default: break;
     }
-    ctx->set_error("fft: unsupported line length " + std::to_string(L));
-    return -1;
+    return launch_rows_rt<T, DIR>(ctx, name, L, src, dst, g);
 }
 
 // r2c rows of N = 2*L2 reals.
@@ -102,8 +177,7 @@ int launch_r2c(hm_ctx* ctx, const char* name, int L2, const RSrc& src, C<T>* dst
 #undef HM_CASE
     default: break;
     }
-    ctx->set_error("fft r2c: unsupported half length " + std::to_string(L2));
-    return -1;
+    return launch_r2c_rt<T, RSrc>(ctx, name, L2, src, dst, n_rows);
 }
 
 template <typename T, int L2, class HSrc>
@@ -130,8 +204,7 @@ int launch_c2r(hm_ctx* ctx, const char* name, int L2, const HSrc& src, T* dst, l
 #undef HM_CASE
     default: break;
     }
-    ctx->set_error("fft c2r: unsupported half length " + std::to_string(L2));
-    return -1;
+    return launch_c2r_rt<T, HSrc>(ctx, name, L2, src, dst, n_rows, point_s);
 }
 
 }  // namespace hm

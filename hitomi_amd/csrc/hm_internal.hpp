@@ -38,7 +38,7 @@ struct hm_ctx {
     int cu_count = 256;      // compute units of the device
     bool conv_lds = true;    // convolutions of the learned reconstruction with LDS-staged input tiles
     bool wide_cols(int L) const {
-        const int b = L == 256 ? 0 : (L == 288 ? 1 : (L == 512 ? 2 : -1));
+        const int b = L == 256 ? 0 : ((L == 288 || L == 270) ? 1 : (L == 512 ? 2 : -1));
         return b >= 0 && ((wide_mask >> b) & 1u);
     }
     // device workspace of the mock generator, kept between calls

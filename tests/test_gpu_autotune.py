@@ -1,5 +1,5 @@
 """Autotuner: the chosen launch shapes (wide column tiles of the FP32 passes
-of length 256, 288 and 512; the P_l binning block size) leave every result
+of length 256, 270 or 288, and 512; the P_l binning block size) leave every result
 bit for bit unchanged, for the plan and for the graph-replayed batch."""
 import numpy as np
 import pytest
@@ -19,7 +19,7 @@ def mock(ctx):
 
 def test_plan_shapes_do_not_change_results(ctx, mock):
     plan = hm.BoxPlan(ctx, 512, 2000.0, "TSC", precision=hm.FP32, **KW)
-    assert plan.triple_mesh == (288, 288, 288)
+    assert plan.triple_mesh == (270, 270, 270)
     assert plan.shapes == (0, 256)
     ref = plan.execute(mock)
     for shapes in [(0b110, 64), (0b010, 128), (0b100, 256)]:

@@ -10,6 +10,10 @@ pytestmark = pytest.mark.gpu
 POW2 = [8, 16, 32, 64, 128, 256, 512, 1024]
 MIXED = [24, 40, 48, 72, 80, 96, 120, 144, 160, 192, 200, 216, 240, 288, 320, 360, 384, 400, 432, 480, 576, 600, 640,
          720, 768, 800, 960]
+# lengths with no compile-time plan: the run-time radix plan (radix 2-23 butterflies, a direct DFT stage for
+# larger prime factors), including every axis length of the reference's BOSS meshes (RUN_boss.py:85-120)
+RUNTIME = [2, 3, 5, 7, 11, 13, 17, 19, 23, 25, 26, 29, 46, 49, 97, 105, 130, 140, 169, 180, 190, 210, 250, 260, 280,
+           340, 361, 460, 500, 529, 650, 1000, 1021, 1290, 2048]
 TOL = {hm.FP64: 1e-12, hm.FP32: 2e-6}
 
 
@@ -18,10 +22,11 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
-@pytest.mark.parametrize("L", POW2 + MIXED)
+@pytest.mark.parametrize("L", POW2 + MIXED + [270] + RUNTIME)
 def test_c2c_each_length_each_axis(ctx, prec, L):
     rng = np.random.default_rng(L)
-    for shape in [(L, 8, 8), (8, L, 8), (8, 8, L)]:
+    m = 8 if L <= 1024 else 4
+    for shape in [(L, m, m), (m, L, m), (m, m, L)]:
         a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
         f = ctx.fft_c2c(a, -1, prec)
         assert _rel(f, np.fft.fftn(a)) < TOL[prec] * np.log2(L), (shape, prec)
@@ -30,7 +35,8 @@ def test_c2c_each_length_each_axis(ctx, prec, L):
 
 
 @pytest.mark.parametrize("prec", [hm.FP64, hm.FP32])
-@pytest.mark.parametrize("nz", [16, 32, 64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("nz", [16, 32, 64, 128, 256, 512, 1024, 10, 26, 46, 130, 210, 260, 280, 360, 500, 540, 650,
+                                2048])
 def test_r2c_c2r_against_numpy(ctx, prec, nz):
     rng = np.random.default_rng(nz)
     a = rng.standard_normal((16, 8, nz))

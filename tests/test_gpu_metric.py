@@ -82,7 +82,7 @@ def test_metric_mode_counts_bit_exact(b000):
 def test_metric_fp32_regrid_vs_plain_fp64(b000):
     r32, m32 = b000["fp32"]
     r64, m64 = b000["fp64_plain"]
-    assert m32 == (288, 288, 288) and m64 == (N_MESH,) * 3
+    assert m32 == (270, 270, 270) and m64 == (N_MESH,) * 3
     scale = np.abs(r64["pk"][0, :, 0]) + BOX ** 3 / N_PART
     dp = (np.abs(r32["pk"][..., 0] - r64["pk"][..., 0]) / scale[None, :]).max()
     report_err("metric P_l fp32 vs fp64_plain", dp)
