@@ -72,6 +72,11 @@ class MockConfig(ct.Structure):
                 ("f_growth", ct.c_double)]
 
 
+class SkyPatch(ct.Structure):
+    _fields_ = [("rmin", ct.c_double), ("rmax", ct.c_double), ("ra_min", ct.c_double), ("ra_max", ct.c_double),
+                ("dec_min", ct.c_double), ("dec_max", ct.c_double)]
+
+
 class LReconConfig(ct.Structure):
     _fields_ = [("n_mesh", ct.c_int), ("boxsize", ct.c_double), ("b1", ct.c_double), ("RG", ct.c_double),
                 ("psi_scale", ct.c_double), ("lr", ct.c_double), ("init_seed", ct.c_ulonglong), ("precision", ct.c_int),
@@ -163,6 +168,7 @@ _SIGS = {
     "hm_assign": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
     "hm_assign_atomic": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
     "hm_assign_noalloc": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP, _VP]),
+    "hm_mock_sky_randoms": (ct.c_int, [_VP, ct.POINTER(SkyPatch), ct.c_ulonglong, ct.c_longlong, _VP]),
     "hm_cell_coords": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_longlong, _VP]),
     "hm_fft_c2c": (ct.c_int, [_VP, ct.c_int, _VP, ct.POINTER(ct.c_int), ct.c_int]),
     "hm_fft_r2c": (ct.c_int, [_VP, ct.c_int, _VP, _VP, ct.POINTER(ct.c_int)]),
@@ -776,6 +782,17 @@ def mock_zeldovich_biased(ctx: Context, cfg: MockConfig, seed, mean_n, b_lag, ca
     shift.shape = (n.value, 3)
     pos.nbytes = shift.nbytes = n.value * 24
     return pos, shift, n.value
+
+
+def sky_patch(rmin, rmax, ra_min, ra_max, dec_min, dec_max) -> SkyPatch:
+    return SkyPatch(float(rmin), float(rmax), float(ra_min), float(ra_max), float(dec_min), float(dec_max))
+
+
+def mock_sky_randoms(ctx: Context, patch: SkyPatch, seed, n, out: DeviceArray = None) -> DeviceArray:
+    """n points uniform in the volume of a sky patch (hm_mock_sky_randoms), [n][3] FP64 on the device."""
+    out = ctx.empty((int(n), 3), np.float64) if out is None else out
+    ctx.check(ctx.lib.hm_mock_sky_randoms(ctx.h, ct.byref(patch), int(seed), int(n), out.ptr), "hm_mock_sky_randoms")
+    return out
 
 
 class LRecon:

@@ -298,6 +298,34 @@ uint64_t splitmix_next(uint64_t& s) {
     return z ^ (z >> 31);
 }
 
+// Points uniform in volume inside a sky patch: point i takes two Philox
+// blocks, counters (i lo, i hi, 3, 0) and (i lo, i hi, 4, 0), key (seed):
+// u1, u2 from the first, u3 from the second; r^3 uniform in [rmin^3, rmax^3],
+// RA uniform, sin(Dec) uniform; Cartesian as fits2dat_galaxy_weight.py:130-132.
+__global__ __launch_bounds__(256) void k_sky_randoms(long long n, unsigned long long seed, hm_sky_patch g,
+                                                    double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long long st = (long long)gridDim.x * blockDim.x;
+    const double deg = 3.14159265358979323846 / 180.0;
+    const double r3a = g.rmin * g.rmin * g.rmin, r3b = g.rmax * g.rmax * g.rmax;
+    const double sa = sin(g.dec_min * deg), sb = sin(g.dec_max * deg);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) {
+        const uint64_t u = (uint64_t)i;
+        uint32_t c[4] = {(uint32_t)u, (uint32_t)(u >> 32), 3u, 0u};
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        uint32_t e[4] = {(uint32_t)u, (uint32_t)(u >> 32), 4u, 0u};
+        philox4x32_10(e, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double u1 = u53(c[0], c[1]), u2 = u53(c[2], c[3]), u3 = u53(e[0], e[1]);
+        const double r = cbrt(r3a + u1 * (r3b - r3a));
+        const double ra = (g.ra_min + u2 * (g.ra_max - g.ra_min)) * deg;
+        const double sd = sa + u3 * (sb - sa);
+        const double cd = sqrt(1.0 - sd * sd);
+        out[3 * i] = r * cd * cos(ra);
+        out[3 * i + 1] = r * cd * sin(ra);
+        out[3 * i + 2] = r * sd;
+    }
+}
+
 int check_cfg(hm_ctx* ctx, const hm_mock_config* c) {
     if (!c) return bad(ctx, "mock: no configuration");
     if (c->n_side < 16 || c->n_side > 1024 || (c->n_side & (c->n_side - 1)))
@@ -414,6 +442,19 @@ int hm_mock_zeldovich(hm_ctx* ctx, const hm_mock_config* cfg, unsigned long long
         const long long blocks = std::min((np + 255) / 256, 65536LL);
         hipLaunchKernelGGL(k_place, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, np, (int)n, cfg->boxsize, pm,
                            g[0], g[1], g[2], inv_n3, d_pos);
+    });
+}
+
+int hm_mock_sky_randoms(hm_ctx* ctx, const hm_sky_patch* patch, unsigned long long seed, long long n, double* d_pos) {
+    if (!ctx) return -1;
+    if (!patch || !(patch->rmin >= 0.0) || !(patch->rmax > patch->rmin) || !(patch->ra_max > patch->ra_min) ||
+        !(patch->dec_max > patch->dec_min) || patch->dec_min < -90.0 || patch->dec_max > 90.0)
+        return bad(ctx, "mock: the sky patch needs 0 <= rmin < rmax, ra_min < ra_max, -90 <= dec_min < dec_max <= 90");
+    if (n < 0 || (n > 0 && !d_pos)) return bad(ctx, "mock: bad arguments");
+    if (n == 0) return 0;
+    return ctx->launch("mock_sky_randoms", [&] {
+        const long long blocks = std::min((n + 255) / 256, 65536LL);
+        hipLaunchKernelGGL(k_sky_randoms, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, n, seed, *patch, d_pos);
     });
 }
 

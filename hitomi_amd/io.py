@@ -1,7 +1,7 @@
 """Catalogue and result file formats of the reference, for writing inputs to
 and reading outputs of the drop-in programs.
 
-Gadget-1 snapshot files `<base>.<i>` (particleGadget.hpp:35-51, 211-298): a
+ASCII catalogues `x y z w` in %.10e (write_xyzw).  Gadget-1 snapshot files `<base>.<i>` (particleGadget.hpp:35-51, 211-298): a
 256-byte header between 4-byte record markers, then float32 positions,
 float32 velocities (the reader multiplies them by sqrt(a)) and int64 ids,
 each block between record markers.  Result files pk%d.dat / bk%d%d%d_%02d.dat
@@ -36,6 +36,14 @@ def write_gadget1(base, pos, vel=None, boxsize=1000.0, a=1.0, omega_m=0.3, omega
                 f.write(struct.pack("<i", len(block)))
                 f.write(block)
                 f.write(struct.pack("<i", len(block)))
+
+
+def write_xyzw(path, pos, w):
+    """A catalogue as the reference's converters write it (np.savetxt(..., fmt="%.10e") of the columns
+    x, y, z, w: fits2dat_galaxy_weight.py:134, 158), the input of ParticleBOSSClass (particle.hpp:93-154)."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    w = np.asarray(w, dtype=np.float64).reshape(-1)
+    np.savetxt(path, np.column_stack([pos, w]), fmt="%.10e")
 
 
 def read_table(path):

@@ -81,3 +81,83 @@ def poisson_sizes(n_mocks, mean, lo, hi, seed, scatter=0.0):
         out[bad] = draw(int(bad.sum()))
         bad = (out < lo) | (out > hi)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Survey-geometry mocks: data Zel'dovich-displaced from a seeded Gaussian field
+# and kept where they land inside a sky patch, randoms uniform in the patch's
+# volume, FKP weights w = 1 / (1 + nbar P0) with the patch's mean density
+# (the reference's catalogues come as `x y z w` text from
+# struct/data/boss/fits2dat_galaxy_weight.py; the survey measurement reads them
+# through particle.hpp:93-154).
+# ---------------------------------------------------------------------------
+def sky_patch_volume(patch):
+    """(RA range in rad) (sin Dec_max - sin Dec_min) (rmax^3 - rmin^3) / 3."""
+    deg = np.pi / 180.0
+    return ((patch.ra_max - patch.ra_min) * deg * (np.sin(patch.dec_max * deg) - np.sin(patch.dec_min * deg))
+            * (patch.rmax ** 3 - patch.rmin ** 3) / 3.0)
+
+
+def in_sky_patch(pos, patch):
+    """Mask of the positions inside the patch (r, RA in [0, 360), Dec from the Cartesian position)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    r = np.sqrt(np.sum(pos * pos, axis=1))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        dec = np.degrees(np.arcsin(np.clip(pos[:, 2] / r, -1.0, 1.0)))
+    ra = np.mod(np.degrees(np.arctan2(pos[:, 1], pos[:, 0])), 360.0)
+    return ((r >= patch.rmin) & (r < patch.rmax) & (ra >= patch.ra_min) & (ra < patch.ra_max)
+            & (dec >= patch.dec_min) & (dec < patch.dec_max))
+
+
+def sky_patch_bounds(patch, samples=181):
+    """Cartesian bounding box of the patch from a dense sample of its surface."""
+    deg = np.pi / 180.0
+    ra = np.linspace(patch.ra_min, patch.ra_max, samples) * deg
+    dec = np.linspace(patch.dec_min, patch.dec_max, samples) * deg
+    r = np.array([patch.rmin, patch.rmax])
+    R, A, D = np.meshgrid(r, ra, dec, indexing="ij")
+    xyz = np.stack([R * np.cos(D) * np.cos(A), R * np.cos(D) * np.sin(A), R * np.sin(D)], axis=-1).reshape(-1, 3)
+    return xyz.min(axis=0), xyz.max(axis=0)
+
+
+def survey_catalogue(ctx, patch, n_data, n_rand, seed, P0=1.0e4, n_side=256, margin=30.0, pk=None,
+                     randoms_on_device=True):
+    """A survey-geometry mock: returns dict(posD [n_data][3] host, wD host, posR, wR (device arrays when
+    randoms_on_device, else host), nbar, volume, box (the Zel'dovich box side)).
+
+    Data: a periodic Zel'dovich box of side max(extent) + 2 margin generated on the device
+    (hm_mock_zeldovich, seeded site subset of an n_side^3 lattice, P(k) = pk), moved to the patch's
+    bounding box, the particles inside the patch kept in the generator's (seeded, random) order, the
+    first n_data of them.  Randoms: hm_mock_sky_randoms with seed + 1.  Weights: FKP with the mean
+    density nbar = n_data / V_patch."""
+    import hitomi_amd as hm
+
+    pk = pk or dict(amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5)
+    vol = sky_patch_volume(patch)
+    nbar = n_data / vol
+    lo, hi = sky_patch_bounds(patch)
+    side = float(np.max(hi - lo)) + 2.0 * margin
+    n_box = int(np.ceil(1.3 * nbar * side ** 3))
+    if n_box > n_side ** 3:
+        raise ValueError(f"survey_catalogue: {n_box} lattice sites needed, n_side^3 = {n_side ** 3}")
+    cfg = hm.mock_config(n_side, side, **pk)
+    dbox = hm.mock_zeldovich(ctx, cfg, int(seed), n_box)
+    ctx.sync()
+    box = dbox.download()
+    dbox.free()
+    pos = box + (lo - margin)
+    sel = pos[in_sky_patch(pos, patch)]
+    if sel.shape[0] < n_data:
+        raise ValueError(f"survey_catalogue: only {sel.shape[0]} particles inside the patch")
+    posD = np.ascontiguousarray(sel[:n_data])
+    w = 1.0 / (1.0 + nbar * P0)
+    wD = np.full(n_data, w)
+    if randoms_on_device:
+        posR = hm.mock_sky_randoms(ctx, patch, int(seed) + 1, n_rand)
+        wR = ctx.array(np.full(n_rand, w))
+    else:
+        d = hm.mock_sky_randoms(ctx, patch, int(seed) + 1, n_rand)
+        ctx.sync()
+        posR, wR = d.download(), np.full(n_rand, w)
+        d.free()
+    return dict(posD=posD, wD=wD, posR=posR, wR=wR, nbar=nbar, volume=vol, box=side)

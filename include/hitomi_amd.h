@@ -373,6 +373,23 @@ int hm_mock_zeldovich(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long
 int hm_mock_zeldovich_biased(hm_ctx *ctx, const hm_mock_config *cfg, unsigned long long seed, double mean_n,
                              double b_lag, long long capacity, double *d_pos, double *d_shift, long long *n_out);
 
+/* A sky patch of the survey geometry: comoving distance shell [rmin, rmax]
+ * (Mpc/h) times RA in [ra_min, ra_max) and Dec in [dec_min, dec_max)
+ * (degrees).  Cartesian positions follow fits2dat_galaxy_weight.py:130-132:
+ * x = r cos(Dec) cos(RA), y = r cos(Dec) sin(RA), z = r sin(Dec). */
+typedef struct {
+    double rmin, rmax;
+    double ra_min, ra_max;
+    double dec_min, dec_max;
+} hm_sky_patch;
+/* n points uniform in the volume of the patch (the random catalogue of a
+ * survey-geometry mock; the reference reads its randoms from files,
+ * struct/data/boss/fits2dat_galaxy_random_weight.py): point i from two
+ * Philox4x32-10 blocks with counters (i, 3) and (i, 4) and key seed, r^3, RA
+ * and sin(Dec) uniform.  d_pos [n][3] FP64; asynchronous on the context's
+ * stream. */
+int hm_mock_sky_randoms(hm_ctx *ctx, const hm_sky_patch *patch, unsigned long long seed, long long n, double *d_pos);
+
 /* ---------------------------------------------------------------- */
 /* Learned reconstruction.  No counterpart in the reference, whose
  * reconstruction is the plain one (particle_reconstruction.hpp:210-402).  A

@@ -2317,3 +2317,32 @@ void or_mock_select_uniform(long long n, unsigned long long seed, double *out) {
         out[i] = (double)(((uint64_t)(w[0] >> 5) << 26) | (uint64_t)(w[1] >> 6)) / 9007199254740992.0;
     }
 }
+
+/* Sky-patch randoms: the restatement of the library's generator (counters
+ * (i lo, i hi, 3, 0) and (i lo, i hi, 4, 0), key (seed lo, seed hi); r^3, RA
+ * and sin(Dec) uniform; x = r cos(Dec) cos(RA), y = r cos(Dec) sin(RA),
+ * z = r sin(Dec) as fits2dat_galaxy_weight.py:130-132). */
+void or_sky_randoms(const double patch[6], unsigned long long seed, long long n, double *out) {
+    const unsigned int key[2] = {(unsigned int)(seed & 0xFFFFFFFFu), (unsigned int)(seed >> 32)};
+    const double deg = M_PI / 180.0;
+    const double r3a = patch[0] * patch[0] * patch[0], r3b = patch[1] * patch[1] * patch[1];
+    const double sa = sin(patch[4] * deg), sb = sin(patch[5] * deg);
+    for (long long i = 0; i < n; i++) {
+        const unsigned long long u = (unsigned long long)i;
+        const unsigned int c1[4] = {(unsigned int)(u & 0xFFFFFFFFu), (unsigned int)(u >> 32), 3u, 0u};
+        const unsigned int c2[4] = {(unsigned int)(u & 0xFFFFFFFFu), (unsigned int)(u >> 32), 4u, 0u};
+        unsigned int w[4], v[4];
+        or_philox4x32_10(c1, key, w);
+        or_philox4x32_10(c2, key, v);
+        const double u1 = (double)(((uint64_t)(w[0] >> 5) << 26) | (uint64_t)(w[1] >> 6)) / 9007199254740992.0;
+        const double u2 = (double)(((uint64_t)(w[2] >> 5) << 26) | (uint64_t)(w[3] >> 6)) / 9007199254740992.0;
+        const double u3 = (double)(((uint64_t)(v[0] >> 5) << 26) | (uint64_t)(v[1] >> 6)) / 9007199254740992.0;
+        const double r = cbrt(r3a + u1 * (r3b - r3a));
+        const double ra = (patch[2] + u2 * (patch[3] - patch[2])) * deg;
+        const double sd = sa + u3 * (sb - sa);
+        const double cd = sqrt(1.0 - sd * sd);
+        out[3 * i] = r * cd * cos(ra);
+        out[3 * i + 1] = r * cd * sin(ra);
+        out[3 * i + 2] = r * sd;
+    }
+}

@@ -175,5 +175,7 @@ void or_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsi
 void or_mock_white_noise(long long n, unsigned long long seed, double *out);
 void or_mock_sites(long long np, long long sites, unsigned long long seed, long long *out);
 void or_mock_select_uniform(long long n, unsigned long long seed, double *out);
+/* sky-patch randoms: patch = {rmin, rmax, ra_min, ra_max, dec_min, dec_max} (degrees) */
+void or_sky_randoms(const double patch[6], unsigned long long seed, long long n, double *out);
 
 #endif

@@ -98,6 +98,7 @@ def lib():
             "or_mock_white_noise": (None, [ct.c_longlong, ct.c_ulonglong, _D]),
             "or_mock_sites": (None, [ct.c_longlong, ct.c_longlong, ct.c_ulonglong, ct.POINTER(ct.c_longlong)]),
             "or_mock_select_uniform": (None, [ct.c_longlong, ct.c_ulonglong, _D]),
+            "or_sky_randoms": (None, [_D, ct.c_ulonglong, ct.c_longlong, _D]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -710,3 +711,11 @@ def lrecon_net_grad(x, params, gout, bf16=True):
         da = conv3_input_grad(dz, W)
         dz = r(f32(np.where(ins[l] > 0.0, da, 0.0))).astype(np.float64)
     return lrecon_pack(grads)
+
+
+def sky_randoms(patch, seed, n):
+    """or_sky_randoms: n points uniform in the sky patch (rmin, rmax, ra_min, ra_max, dec_min, dec_max)."""
+    pt = np.ascontiguousarray(patch, dtype=np.float64)
+    out = np.zeros((int(n), 3))
+    lib().or_sky_randoms(dp(pt), int(seed), int(n), dp(out))
+    return out

@@ -93,3 +93,52 @@ def test_field_recovers_power_spectrum():
     chi = np.array(chi)
     assert np.abs(chi).max() < 5.0
     assert 0.3 < np.mean(chi ** 2) < 2.0
+
+
+def test_sky_randoms_inside_patch_and_uniform():
+    """The sky-patch randoms: every point inside the patch; r^3, RA and sin(Dec) uniform (Kolmogorov-Smirnov)
+    and mutually uncorrelated; the count per volume element is the patch's mean density."""
+    from scipy import stats
+
+    from hitomi_amd import mocks
+    import hitomi_amd as hm
+
+    patch = hm.sky_patch(1200.0, 1800.0, 0.0, 60.0, -30.0, 30.0)
+    x = po.sky_randoms([1200.0, 1800.0, 0.0, 60.0, -30.0, 30.0], 99, 200_000)
+    assert mocks.in_sky_patch(x, patch).all()
+    r = np.sqrt(np.sum(x * x, axis=1))
+    u1 = (r ** 3 - 1200.0 ** 3) / (1800.0 ** 3 - 1200.0 ** 3)
+    u2 = np.degrees(np.arctan2(x[:, 1], x[:, 0])) / 60.0
+    s = x[:, 2] / r
+    u3 = (s - np.sin(np.radians(-30.0))) / (np.sin(np.radians(30.0)) - np.sin(np.radians(-30.0)))
+    for u in (u1, u2, u3):
+        assert stats.kstest(u, "uniform").pvalue > 1e-3
+    c = np.corrcoef(np.stack([u1, u2, u3]))
+    assert np.abs(c[np.triu_indices(3, 1)]).max() < 0.01
+    # an inner sub-shell holds its volume fraction of the points
+    inner = (r < 1400.0).mean()
+    assert inner == pytest.approx((1400.0 ** 3 - 1200.0 ** 3) / (1800.0 ** 3 - 1200.0 ** 3), abs=0.005)
+    assert mocks.sky_patch_volume(patch) == pytest.approx(np.pi / 3 * (1800.0 ** 3 - 1200.0 ** 3) / 3, rel=1e-12)
+
+
+def test_sky_randoms_seeded_and_counter_based():
+    a = po.sky_randoms([500.0, 800.0, 0.0, 40.0, -20.0, 20.0], 5, 1000)
+    b = po.sky_randoms([500.0, 800.0, 0.0, 40.0, -20.0, 20.0], 5, 300)
+    c = po.sky_randoms([500.0, 800.0, 0.0, 40.0, -20.0, 20.0], 6, 300)
+    assert np.array_equal(a[:300], b)
+    assert not np.allclose(b, c)
+
+
+def test_write_xyzw_format(tmp_path):
+    """np.savetxt(..., fmt="%.10e") of x, y, z, w (fits2dat_galaxy_weight.py:134, 158)."""
+    from hitomi_amd import io as hio
+
+    pos = np.array([[1.0, -2.5e3, 3.25e-4], [1234.5678901234, 0.0, -7.0]])
+    w = np.array([0.125, 1.0 / 3.0])
+    f = tmp_path / "cat.dat"
+    hio.write_xyzw(f, pos, w)
+    lines = f.read_text().splitlines()
+    assert lines[0] == "1.0000000000e+00 -2.5000000000e+03 3.2500000000e-04 1.2500000000e-01"
+    assert lines[1] == "1.2345678901e+03 0.0000000000e+00 -7.0000000000e+00 3.3333333333e-01"
+    back = np.loadtxt(f)
+    assert np.allclose(back[:, :3], pos, rtol=1e-10) and np.allclose(back[:, 3], w, rtol=1e-10)
