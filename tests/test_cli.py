@@ -304,3 +304,107 @@ def test_survey_reconstruction_end_to_end(tmp_path):
     s.posD, s.posR = dD - d2, dR - d2
     ref, _ = s.pk(p)
     assert np.allclose(got[:, 1], ref.real, rtol=1e-6, atol=1e-7 * np.abs(ref.real).max())
+
+
+@pytest.mark.gpu
+def test_box_default_fp32_end_to_end(tmp_path):
+    """A parameter file without a precision key runs FP32 grids and FFTs (the drop-in's default): pk and bk
+    outputs against the oracle within the FP32 tolerances (P: 1e-6 of |P_0| + V/N; B: 5e-6 of max|B|).
+    (Keys match anywhere in a line, as parameter.hpp:161-207 reads them, so a data_dir whose path holds the
+    word of a key sets that key too: the test's name, part of its directory, avoids them.)"""
+    from oracle import pyoracle as po
+    from .conftest import clustered_catalogue
+
+    L, n, N = 1000.0, 32, 30000
+    pos = clustered_catalogue(N, L, seed=78, nclump=20, sigma=25.0).astype(np.float32)
+    hio.write_gadget1(str(tmp_path / "snap"), pos, boxsize=L, num_files=1)
+    posd = pos.astype(np.float64)
+    kw = dict(data_dir=str(tmp_path), boxsize_x=L, boxsize_y=L, boxsize_z=L, n_mesh_x=n, n_mesh_y=n, n_mesh_z=n,
+              kmin=0.02, kmax=0.08, n_kbin=4)
+    ref0 = None
+    for ell in (0, 2):
+        ini = tmp_path / f"pk{ell}.ini"
+        hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="pk", ell1=ell, ELL=ell, **kw)
+        r = run(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        got = hio.read_table(tmp_path / "out" / f"pk{ell}.dat")
+        p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.08, n_kbin=4, ell1=ell, ELL=ell)
+        ref, _ = po.box_pk(p, posd)
+        ref0 = ref if ell == 0 else ref0
+        # the file holds %.7e: 5e-8 relative of rounding on top of the FP32 error
+        assert np.all(np.abs(got[:, 1] - ref.real) <= 1e-6 * (np.abs(ref0.real) + L ** 3 / N))
+    ini = tmp_path / "bk.ini"
+    hio.write_ini(ini, output_dir=str(tmp_path / "out"), measure="bk", ith_kbin=1, **kw)
+    r = run(ini, tmp_path)
+    assert r.returncode == 0, r.stdout
+    got = hio.read_table(tmp_path / "out" / "bk000_01.dat")
+    p = po.param(n, L, po.TSC, kmin=0.02, kmax=0.08, n_kbin=4, ith_kbin=1)
+    ref, _, _ = po.box_bk(p, posd)
+    assert np.abs(got[:, 2] - ref.real).max() <= 5e-6 * np.abs(ref.real).max()
+
+
+@pytest.mark.gpu
+def test_survey_boss_mesh_default_fp32_matches_oracle(tmp_path):
+    """hitomi_survey on the reference's North zbin-1 BOSS box (1350 x 2450 x 1400 Mpc/h, RUN_boss.py:85-92)
+    with its mesh divided by 10 (25 x 46 x 26: odd, non-power-of-two, anisotropic cells), a survey-geometry
+    mock written as x y z w text, no precision key (FP32): pk0.dat and pk2.dat against the oracle within
+    2e-5 of |P_0| + norm SN_00."""
+    import hitomi_amd as hm
+    from hitomi_amd import mocks
+    from oracle import pyoracle as po
+
+    n, L = (25, 46, 26), (1350.0, 2450.0, 1400.0)
+    with hm.Context(0) as ctx:
+        cat = mocks.survey_catalogue(ctx, hm.sky_patch(500.0, 800.0, 0.0, 40.0, -20.0, 20.0), 20_000, 100_000,
+                                     seed=31, n_side=64, randoms_on_device=False)
+    hio.write_xyzw(tmp_path / "data.dat", cat["posD"], cat["wD"])
+    hio.write_xyzw(tmp_path / "rand.dat", cat["posR"], cat["wR"])
+    D, R = np.loadtxt(tmp_path / "data.dat"), np.loadtxt(tmp_path / "rand.dat")
+    kw = dict(kmin=0.01, kmax=0.1, n_kbin=10)
+    p0 = po.param(n, L, po.TSC, **kw)
+    s0 = po.Survey(p0, D[:, :3], D[:, 3], R[:, :3], R[:, 3])
+    ref0, _ = s0.pk(p0)
+    sn0 = s0.shotnoise(0, 0).real * s0.vsurvey / s0.wsumD ** 2
+    for ell in (0, 2):
+        ini = tmp_path / f"pk{ell}.ini"
+        hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="rand.dat",
+                      output_dir=str(tmp_path / "out"), measure="pk", ell1=ell, ELL=ell, boxsize_x=L[0],
+                      boxsize_y=L[1], boxsize_z=L[2], n_mesh_x=n[0], n_mesh_y=n[1], n_mesh_z=n[2], **kw)
+        r = run_survey(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        got = hio.read_table(tmp_path / "out" / f"pk{ell}.dat")
+        p = po.param(n, L, po.TSC, ell1=ell, ELL=ell, **kw)
+        ref, _ = po.Survey(p, D[:, :3], D[:, 3], R[:, :3], R[:, 3]).pk(p)
+        assert np.allclose(got[:, 0], po.kbin(p), atol=5e-6)
+        assert np.all(np.abs(got[:, 1] - ref.real) <= 2e-5 * (np.abs(ref0.real) + sn0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mesh", [((250, 460, 260), (1350.0, 2450.0, 1400.0)),
+                                  ((340, 650, 360), (1800.0, 3400.0, 1900.0)),
+                                  ((190, 360, 210), (1000.0, 1900.0, 1100.0)),
+                                  ((190, 500, 280), (1000.0, 2600.0, 1500.0))],
+                         ids=["NGC_z1", "NGC_z3", "SGC_z1", "SGC_z3"])
+def test_survey_accepts_boss_mesh_box_pairs(tmp_path, mesh):
+    """Every mesh / box pair of RUN_boss.py:85-120 runs through hitomi_survey (pk, default FP32), and its
+    pk0.dat agrees with an FP64 run within 1e-5 of |P_0|."""
+    import hitomi_amd as hm
+    from hitomi_amd import mocks
+
+    n, L = mesh
+    with hm.Context(0) as ctx:
+        cat = mocks.survey_catalogue(ctx, hm.sky_patch(500.0, 800.0, 0.0, 40.0, -20.0, 20.0), 20_000, 100_000,
+                                     seed=sum(n), n_side=64, randoms_on_device=False)
+    hio.write_xyzw(tmp_path / "data.dat", cat["posD"], cat["wD"])
+    hio.write_xyzw(tmp_path / "rand.dat", cat["posR"], cat["wR"])
+    out = {}
+    for prec in (None, "FP64"):
+        ini = tmp_path / f"pk_{prec}.ini"
+        hio.write_ini(ini, data_dir=str(tmp_path), data_file="data.dat", random_file="rand.dat",
+                      output_dir=str(tmp_path / f"out_{prec}"), measure="pk", boxsize_x=L[0], boxsize_y=L[1],
+                      boxsize_z=L[2], n_mesh_x=n[0], n_mesh_y=n[1], n_mesh_z=n[2], kmin=0.01, kmax=0.2, n_kbin=20,
+                      precision=prec)
+        r = run_survey(ini, tmp_path)
+        assert r.returncode == 0, r.stdout
+        out[prec] = hio.read_table(tmp_path / f"out_{prec}" / "pk0.dat")
+    assert np.all(np.abs(out[None][:, 1] - out["FP64"][:, 1]) <= 1e-5 * np.abs(out["FP64"][:, 1]))
