@@ -146,7 +146,8 @@ __device__ inline void los_of(const double* p, double* l) {
 // Complex weights of one field over D then R: w Y(l, M)(los) for D and
 // -alpha w Y(l, M)(los) for R (field.hpp:186-228; D - alpha R is formed by
 // the fixed-point sum).  mode 1: the random catalogue alone with (w, 0)
-// (field.hpp:476-485); mode 2: the random catalogue alone with w Y(l, M).
+// (field.hpp:476-485); mode 2: the random catalogue alone with w Y(l, M);
+// mode 3 (l = M = 0): mode 0 in re and mode 1 in im.
 __global__ __launch_bounds__(256) void k_survey_weights(const double* __restrict__ pD, const double* __restrict__ wD,
                                                        long long nD, const double* __restrict__ pR,
                                                        const double* __restrict__ wR, long long nR, double alpha, int l,
@@ -157,6 +158,14 @@ __global__ __launch_bounds__(256) void k_survey_weights(const double* __restrict
         if (q < nR) {
             out[2 * q] = wR[q];
             out[2 * q + 1] = 0.0;
+        }
+        return;
+    }
+    if (mode == 3) {  // Y_00 = 1: D - alpha R in re, and the random catalogue alone (w, field.hpp:476-485) in im
+        if (q < nD + nR) {
+            const bool d = q < nD;
+            out[2 * q] = d ? wD[q] : -(alpha * wR[q - nD]);
+            out[2 * q + 1] = d ? 0.0 : wR[q - nD];
         }
         return;
     }
@@ -693,25 +702,35 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
     }
     long long* fre = (long long*)p->fre.p;
     long long* fim = (long long*)p->fim.p;
-    // 2. survey volume (field.hpp:449-503)
-    if (ctx->launch("survey_weights", [&] {
-            hipLaunchKernelGGL(k_survey_weights, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, ctx->stream, pD, wD,
-                               nD, pR, wR, nR, alpha, 0, 0, 1, wts + 2 * nD);
-        }))
-        return -1;
-    const double sv_scale = fixed_scale_for(awR);
-    if (assign_sorted(ctx, ag, pos + 3 * nD, wts + 2 * nD, nR, sv_scale, fre, nullptr, p->sort)) return -1;
-    if (ctx->launch("survey_sqsum", [&] {
-            hipLaunchKernelGGL(k_survey_sqsum, dim3(rbg), dim3(256), 0, ctx->stream, fre, ng, 1.0 / sv_scale, 1.0 / dV,
-                               dV, part);
-        }))
-        return -1;
-    if (reduce_cols<double>(ctx, "survey_reduce", part, rbg, 1, red)) return -1;
-    double isq = 0.0;
-    HM_HIP_CHECK(ctx, hipMemcpyAsync(&isq, red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const double vs = swR * swR / isq;      // field.hpp:499
-    const double nbar = swD * swD / vs / sw2D;  // particle.hpp:296
+    // 2. survey volume (field.hpp:449-503): Vsurvey = (sum w_R)^2 / sum_x rho_R(x)^2 dV from a fixed-point grid
+    // of the randoms with weights w_R.  Without reconstruction the P_l / xi_l fields below share their sort by
+    // cell with it: the grid is the imaginary channel of the first field (the window's first field is that grid).
+    double vs = 0.0, nbar = 0.0;
+    auto volume_from = [&](const long long* q, double scale) -> int {
+        if (ctx->launch("survey_sqsum", [&] {
+                hipLaunchKernelGGL(k_survey_sqsum, dim3(rbg), dim3(256), 0, ctx->stream, q, ng, 1.0 / scale, 1.0 / dV,
+                                   dV, part);
+            }))
+            return -1;
+        if (reduce_cols<double>(ctx, "survey_reduce", part, rbg, 1, red)) return -1;
+        double isq = 0.0;
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(&isq, red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        vs = swR * swR / isq;           // field.hpp:499
+        nbar = swD * swD / vs / sw2D;   // particle.hpp:296
+        return 0;
+    };
+    const bool fold_volume = !p->cfg.recon && !delegated(p->measure);
+    if (!fold_volume) {
+        if (ctx->launch("survey_weights", [&] {
+                hipLaunchKernelGGL(k_survey_weights, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, ctx->stream, pD,
+                                   wD, nD, pR, wR, nR, alpha, 0, 0, 1, wts + 2 * nD);
+            }))
+            return -1;
+        const double sv_scale = fixed_scale_for(awR);
+        if (assign_sorted(ctx, ag, pos + 3 * nD, wts + 2 * nD, nR, sv_scale, fre, nullptr, p->sort)) return -1;
+        if (volume_from(fre, sv_scale)) return -1;
+    }
     if (p->cfg.recon) {
         // main.cpp:172-183: reconstruct on the n_mesh_recon mesh, then shift into the FFT box again
         Mesh mr;
@@ -835,22 +854,26 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
     const int n3[3] = {m.n[0], m.n[1], m.n[2]};
     for (int f = 0; f < p->nf; f++) {
         const int l = f == 0 ? 0 : p->l, M = f == 0 ? 0 : f - 1;
+        // the first field carries the survey volume's grid in its imaginary channel (mode 3)
+        const bool with_volume = fold_volume && f == 0 && !window;
         if (ctx->launch("survey_weights", [&] {
                 if (window)
                     hipLaunchKernelGGL(k_survey_weights, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, ctx->stream,
                                        pD, wD, nD, pR, wR, nR, alpha, l, M, 2, wts + 2 * nD);
                 else
                     hipLaunchKernelGGL(k_survey_weights, dim3(pb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR,
-                                       alpha, l, M, 0, wts);
+                                       alpha, l, M, with_volume ? 3 : 0, wts);
             }))
             return -1;
-        const double scale = fixed_scale_for(wabs);
+        const double scale = fixed_scale_for(with_volume ? wabs + awR : wabs);
         const bool cplx = M != 0;
         // the positions are the same for every field: sort them once, then move only the weights
         const int arc = window ? assign_sorted(ctx, ag, pos + 3 * nD, wts + 2 * nD, nR, scale, fre, cplx ? fim : nullptr,
                                                p->sort, f > 0)
-                               : assign_sorted(ctx, ag, pos, wts, N, scale, fre, cplx ? fim : nullptr, p->sort, f > 0);
+                               : assign_sorted(ctx, ag, pos, wts, N, scale, fre, (cplx || with_volume) ? fim : nullptr,
+                                               p->sort, f > 0);
         if (arc) return -1;
+        if (fold_volume && f == 0 && volume_from(window ? fre : fim, scale)) return -1;
         C<T>* out = fields + (long long)f * ng;
         if (ctx->launch("survey_to_complex", [&] {
                 hipLaunchKernelGGL(k_fixed_to_complex<T>, dim3(rbg), dim3(256), 0, ctx->stream, fre,
