@@ -208,10 +208,13 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(int* __restrict__ a, lon
     }
 }
 
+// Each interior particle to its place in cell order, start[key] + rank; the
+// place replaces the rank in slot, so that a later assignment of the same
+// positions moves only its weights (k_sort_move_w).
 template <int S>
 __global__ __launch_bounds__(256) void k_sort_scatter(const double* __restrict__ pos, const double* __restrict__ w,
                                                      long long np, AssignGeom g, const int* __restrict__ start,
-                                                     const int* __restrict__ slot, double* __restrict__ spos,
+                                                     int* __restrict__ slot, double* __restrict__ spos,
                                                      double* __restrict__ sw) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
@@ -220,15 +223,24 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const double* __restrict__
     long long key;
     first_cell<S>(pos, q, g, &key);
     const long long d = (long long)start[key] + r;
-    if (spos) {  // null: the positions are already in cell order, only the weights move
-        spos[3 * d] = pos[3 * q];
-        spos[3 * d + 1] = pos[3 * q + 1];
-        spos[3 * d + 2] = pos[3 * q + 2];
-    }
+    slot[q] = (int)d;
+    spos[3 * d] = pos[3 * q];
+    spos[3 * d + 1] = pos[3 * q + 1];
+    spos[3 * d + 2] = pos[3 * q + 2];
     if (w) {
         sw[2 * d] = w[2 * q];
         sw[2 * d + 1] = w[2 * q + 1];
     }
+}
+
+__global__ __launch_bounds__(256) void k_sort_move_w(const double* __restrict__ w, long long np,
+                                                    const int* __restrict__ place, double* __restrict__ sw) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const int d = place[q];
+    if (d < 0) return;
+    sw[2 * (long long)d] = w[2 * q];
+    sw[2 * (long long)d + 1] = w[2 * q + 1];
 }
 
 // One workgroup owns one output tile.  The interior particles whose first
@@ -319,6 +331,7 @@ __global__ __launch_bounds__(256) void k_assign_border(const double* __restrict_
 }
 
 int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights) {
+    w.sig.valid = false;  // the buffers may move: the next assignment sorts afresh
     const long long nk = n_keys(g);
     const long long nb = (nk + 1 + SCAN_BLK - 1) / SCAN_BLK;
     if (dev_alloc(ctx, w.start, sizeof(int) * (size_t)(nk + 1))) return -1;
@@ -387,9 +400,13 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
             }))
             return -1;
     }
-    if (np > 0 && (!reuse || sw) && ctx->launch("assign_scatter", [&] {
+    if (np > 0 && !reuse && ctx->launch("assign_scatter", [&] {
             hipLaunchKernelGGL(k_sort_scatter<S>, dim3(pb), dim3(256), 0, ctx->stream, d_pos, d_w, np, g, start, slot,
-                               reuse ? nullptr : spos, sw);
+                               spos, sw);
+        }))
+        return -1;
+    if (np > 0 && reuse && sw && ctx->launch("assign_scatter", [&] {
+            hipLaunchKernelGGL(k_sort_move_w, dim3(pb), dim3(256), 0, ctx->stream, d_w, np, (const int*)slot, sw);
         }))
         return -1;
     const dim3 gg((unsigned)((g.n[2] + GZ - 1) / GZ), (unsigned)((g.n[1] + GY - 1) / GY),
@@ -416,6 +433,25 @@ int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const d
     if (np > 0x7ffffff0LL || g.ng > 0x7ffffff0LL) {
         ctx->set_error("assign: more than 2^31 particles or cells");
         return -1;
+    }
+    SortWork::Sig& sg = w.sig;
+    if (reuse) {
+        bool same = sg.valid && sg.pos == d_pos && sg.np == np && sg.assign == g.assign && sg.weights && d_w;
+        for (int a = 0; a < 3; a++) same = same && sg.n[a] == g.n[a] && sg.L[a] == g.L[a];
+        if (!same) {
+            ctx->set_error("assign: a sort is reused for other positions, count, mesh or without weights");
+            return -1;
+        }
+    } else {
+        sg.valid = true;
+        sg.pos = d_pos;
+        sg.np = np;
+        sg.assign = g.assign;
+        sg.weights = d_w != nullptr;
+        for (int a = 0; a < 3; a++) {
+            sg.n[a] = g.n[a];
+            sg.L[a] = g.L[a];
+        }
     }
     if (g.assign == 0) return assign_sorted_S<1>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w, reuse);
     if (g.assign == 1) return assign_sorted_S<2>(ctx, g, d_pos, d_w, np, scale, d_re, d_im, w, reuse);

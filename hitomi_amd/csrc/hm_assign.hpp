@@ -100,11 +100,22 @@ int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const do
 // bitwise identical to it.
 struct SortWork {
     DevBuf start;   // int32 [keys + 1]: particle count, then exclusive prefix, per (x, y, z/64) chunk
-    DevBuf slot;    // int32 [np]: rank within the cell, or -1 for a border particle
+    DevBuf slot;    // int32 [np]: rank within the cell, then (after the scatter) the particle's place in cell
+                    // order; -1 for a border particle
     DevBuf spos;    // double [3 np] positions in cell order
     DevBuf sw;      // double [2 np] weights in cell order
     DevBuf border;  // int32 [np + 1]: border count, then border particle indices
     DevBuf bsum;    // int32 scan block sums
+    // what the last full sort was made for: a reuse must name the same positions, count, mesh and weights
+    struct Sig {
+        bool valid = false;
+        const double* pos = nullptr;
+        long long np = 0;
+        int n[3] = {0, 0, 0};
+        double L[3] = {0.0, 0.0, 0.0};
+        int assign = -1;
+        bool weights = false;
+    } sig;
     size_t bytes() const { return start.bytes + slot.bytes + spos.bytes + sw.bytes + border.bytes + bsum.bytes; }
 };
 int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights);
@@ -117,7 +128,8 @@ size_t sort_work_carve(const AssignGeom& g, long long np, bool weights, char* ba
 int fixed_to_density_inplace(hm_ctx* ctx, long long* d_q, long long n, double inv_scale, double factor);
 // reuse: the previous call on w sorted these same positions (same pointer,
 // count and geometry, contents unchanged, weights given both times): only the
-// weights are moved into cell order before the gather.
+// weights are moved into cell order before the gather.  A reuse whose
+// positions, count, mesh or weights differ from the last sort's is an error.
 int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const double* d_w, long long np,
                   double scale, long long* d_re, long long* d_im, SortWork& w, bool reuse = false);
 
