@@ -323,6 +323,9 @@ struct ColGeom {
     // outside a batch's own support are neither read nor written.
     const int4* fsup = nullptr;
     int role_outer = -1, role_inner = -1, role_point = -1;
+    // output support on the transformed axis: only the points |n| <= out_s are stored (-1: all), for a
+    // forward transform whose consumer reads a low-frequency cube only
+    int out_s = -1;
 };
 
 // Row-pass geometry: element (r, p) at r*RS + p.  With fps set, row r takes
@@ -332,6 +335,7 @@ struct RowGeom {
     int point_s;
     const int* fps = nullptr;
     long long rpf = 1;
+    int out_s = -1;  // only the output points |n| <= out_s are stored (-1: all)
 };
 
 __device__ __forceinline__ int int4_get(const int4& q, int i) { return i == 0 ? q.x : (i == 1 ? q.y : q.z); }
@@ -405,7 +409,8 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, tw);
     if (bvalid) {
-        for (int p = tid / B; p < L; p += PSTEP) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + lpos(p)];
+        for (int p = tid / B; p < L; p += PSTEP)
+            if (in_support(p, g.out_s, L)) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + lpos(p)];
     }
 }
 
@@ -429,7 +434,31 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows(const C<T>* __restrict__ sr
     lds_fft<T, DIR, L, B>(sm, tw);
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, p = e % L;
-        if (r0 + b < g.n_rows) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
+        if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
+    }
+}
+
+// Row FFT whose input comes from a functor, src(row, point) -> complex
+// (fixed-point grids converted on load); output as k_fft_rows.
+template <typename T, int L, int DIR, class RSrc>
+__global__ __launch_bounds__(FFT_NT) void k_fft_rows_f(RSrc src, C<T>* __restrict__ dst, RowGeom g,
+                                                      const C<T>* __restrict__ tw) {
+    constexpr int B = RowTile<T, L>::B;
+    constexpr int LP = LdsPad<L>::LP;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < g.n_rows) v = src(r0 + b, p);
+        sm[b * LP + lpos(p)] = v;
+    }
+    __syncthreads();
+    lds_fft<T, DIR, L, B>(sm, tw);
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
     }
 }
 
@@ -722,7 +751,8 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols_rt(Src src, C<T>* __restric
     __syncthreads();
     rt_fft<T, DIR>(sm, tw, pl, tid);
     if (bvalid) {
-        for (int p = tid / B; p < L; p += pstep) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + rt_lpos(p)];
+        for (int p = tid / B; p < L; p += pstep)
+            if (in_support(p, g.out_s, L)) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + rt_lpos(p)];
     }
 }
 
@@ -744,7 +774,28 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows_rt(const C<T>* __restrict__
     rt_fft<T, DIR>(sm, tw, pl, threadIdx.x);
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, p = e % L;
-        if (r0 + b < g.n_rows) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + rt_lpos(p)];
+        if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + rt_lpos(p)];
+    }
+}
+
+template <typename T, int DIR, class RSrc>
+__global__ __launch_bounds__(FFT_NT) void k_fft_rows_f_rt(RSrc src, C<T>* __restrict__ dst, RowGeom g,
+                                                         const C<T>* __restrict__ tw, RtPlan pl) {
+    const int B = pl.B, LP = pl.LP, L = pl.L;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const long long r0 = (long long)blockIdx.x * B;
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        C<T> v = mk<T>((T)0, (T)0);
+        if (r0 + b < g.n_rows) v = src(r0 + b, p);
+        sm[b * LP + rt_lpos(p)] = v;
+    }
+    __syncthreads();
+    rt_fft<T, DIR>(sm, tw, pl, threadIdx.x);
+    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+        const int b = e / L, p = e % L;
+        if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + rt_lpos(p)];
     }
 }
 

@@ -207,4 +207,43 @@ int launch_c2r(hm_ctx* ctx, const char* name, int L2, const HSrc& src, T* dst, l
     return launch_c2r_rt<T, HSrc>(ctx, name, L2, src, dst, n_rows, point_s);
 }
 
+// Rows from a functor source (forward or backward), any supported length.
+template <typename T, int L, int DIR, class RSrc>
+int launch_rows_f_L(hm_ctx* ctx, const char* name, const RSrc& src, C<T>* dst, const RowGeom& g) {
+    constexpr int B = RowTile<T, L>::B;
+    if (g.n_rows <= 0) return 0;
+    dim3 grid((unsigned)((g.n_rows + B - 1) / B));
+    const size_t lds = row_lds<T, L>();
+    const C<T>* tw = ctx->twiddle<T>(L);
+    if (!tw) return -1;
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_fft_rows_f<T, L, DIR, RSrc>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw);
+    });
+}
+
+template <typename T, int DIR, class RSrc>
+int launch_rows_f(hm_ctx* ctx, const char* name, int L, const RSrc& src, C<T>* dst, const RowGeom& g) {
+    switch (L) {
+#define HM_CASE(LL) \
+    case LL: return launch_rows_f_L<T, LL, DIR, RSrc>(ctx, name, src, dst, g);
+        HM_POW2_LENGTHS(HM_CASE)
+        HM_MIXED_LENGTHS(HM_CASE)
+#undef HM_CASE
+    default: break;
+    }
+    RtPlan pl;
+    if (!rt_plan_make(L, RtMax<T>::PTS, 4096, pl)) {
+        ctx->set_error("fft: unsupported line length " + std::to_string(L));
+        return -1;
+    }
+    if (g.n_rows <= 0) return 0;
+    const C<T>* tw = ctx->twiddle<T>(L);
+    if (!tw) return -1;
+    dim3 grid((unsigned)((g.n_rows + pl.B - 1) / pl.B));
+    const size_t lds = (size_t)pl.B * pl.LP * sizeof(C<T>);
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL((k_fft_rows_f_rt<T, DIR, RSrc>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw, pl);
+    });
+}
+
 }  // namespace hm

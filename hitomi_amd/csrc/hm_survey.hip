@@ -629,6 +629,22 @@ size_t survey_bytes(const SurveyPlanImpl* p) {
     return p ? p->bytes + p->sort.bytes() + recon_work_bytes(p->rw) + (p->box ? plan_bytes(p->box) : 0) : 0;
 }
 
+// Complex source of a z-row transform from fixed-point cell sums: the value of
+// k_fixed_to_complex (((q * 2^-s) / dV) * dV, im from the second grid or 0).
+template <typename T> struct FixedCplxSrc {
+    const long long* re;
+    const long long* im;
+    int n2;
+    double inv_scale, factor, dV;
+    __device__ inline C<T> operator()(long long r, int m) const {
+#pragma clang fp contract(off)
+        const long long i = r * n2 + m;
+        const double a = (((double)re[i] * inv_scale) * factor) * dV;
+        const double b = im ? (((double)im[i] * inv_scale) * factor) * dV : 0.0;
+        return mk<T>((T)a, (T)b);
+    }
+};
+
 static unsigned red_blocks(long long n) { return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 1024)); }
 
 template <typename T>
@@ -875,22 +891,63 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
         if (arc) return -1;
         if (fold_volume && f == 0 && volume_from(window ? fre : fim, scale)) return -1;
         C<T>* out = fields + (long long)f * ng;
-        if (ctx->launch("survey_to_complex", [&] {
-                hipLaunchKernelGGL(k_fixed_to_complex<T>, dim3(rbg), dim3(256), 0, ctx->stream, fre,
-                                   cplx ? fim : nullptr, ng, 1.0 / scale, 1.0 / dV, dV, out);
+        if (p->measure == HM_SURVEY_PK) {
+            // forward transform (field.hpp:310-326) with the fixed-point cells converted on load; P_l reads only
+            // the modes of the binning support (and their mirrors, in the same cube), so each pass stores only
+            // those: the y and x passes run on the support's columns alone
+            FixedCplxSrc<T> fs{fre, cplx ? fim : nullptr, m.n[2], 1.0 / scale, 1.0 / dV, dV};
+            RowGeom rz{(long long)m.n[0] * m.n[1], m.n[2], m.n[2], -1};
+            rz.out_s = p->s[2];
+            if (launch_rows_f<T, -1>(ctx, "survey_fft_z", m.n[2], fs, out, rz)) return -1;
+            const int cz = support_count(p->s[2], m.n[2]), cy = support_count(p->s[1], m.n[1]);
+            ColGeom gy;
+            gy.OS = (long long)m.n[1] * m.n[2];
+            gy.PS = m.n[2];
+            gy.n_outer = m.n[0];
+            gy.outer_full = m.n[0];
+            gy.outer_s = -1;
+            gy.n_inner = cz;
+            gy.inner_full = m.n[2];
+            gy.inner_s = p->s[2];
+            gy.point_s = -1;
+            gy.out_s = p->s[1];
+            if (fft_cols<T>(ctx, "survey_fft_y", m.n[1], -1, out, out, gy)) return -1;
+            ColGeom gx;
+            gx.OS = m.n[2];
+            gx.PS = (long long)m.n[1] * m.n[2];
+            gx.n_outer = cy;
+            gx.outer_full = m.n[1];
+            gx.outer_s = p->s[1];
+            gx.n_inner = cz;
+            gx.inner_full = m.n[2];
+            gx.inner_s = p->s[2];
+            gx.point_s = -1;
+            gx.out_s = p->s[0];
+            if (fft_cols<T>(ctx, "survey_fft_x", m.n[0], -1, out, out, gx)) return -1;
+        } else {
+            if (ctx->launch("survey_to_complex", [&] {
+                    hipLaunchKernelGGL(k_fixed_to_complex<T>, dim3(rbg), dim3(256), 0, ctx->stream, fre,
+                                       cplx ? fim : nullptr, ng, 1.0 / scale, 1.0 / dV, dV, out);
+                }))
+                return -1;
+            if (fft3d_c2c<T>(ctx, out, n3, -1)) return -1;
+        }
+    }
+    // 4. shot noise sums for M = 0..l; for l = 0 (Y_00 = 1) they are the sums of w^2 already formed in step 1
+    std::vector<double> snr(nsn, 0.0);
+    if (p->l == 0) {
+        snr[0] = sw2D;
+        snr[2] = sw2R;
+    } else {
+        if (ctx->launch("survey_sn", [&] {
+                hipLaunchKernelGGL(k_survey_sn, dim3(rb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR, p->l,
+                                   part);
             }))
             return -1;
-        if (fft3d_c2c<T>(ctx, out, n3, -1)) return -1;
+        if (reduce_cols<double>(ctx, "survey_reduce", part, rb, nsn, red)) return -1;
+        HM_HIP_CHECK(ctx, hipMemcpyAsync(snr.data(), red, sizeof(double) * nsn, hipMemcpyDeviceToHost, ctx->stream));
+        HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
-    // 4. shot noise sums for M = 0..l
-    if (ctx->launch("survey_sn", [&] {
-            hipLaunchKernelGGL(k_survey_sn, dim3(rb), dim3(256), 0, ctx->stream, pD, wD, nD, pR, wR, nR, p->l, part);
-        }))
-        return -1;
-    if (reduce_cols<double>(ctx, "survey_reduce", part, rb, nsn, red)) return -1;
-    std::vector<double> snr(nsn);
-    HM_HIP_CHECK(ctx, hipMemcpyAsync(snr.data(), red, sizeof(double) * nsn, hipMemcpyDeviceToHost, ctx->stream));
-    HM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     // shot noise per M: sum_D + alpha^2 sum_R (field.hpp:1187), or sum_R for the window (field.hpp:1202-1218)
     std::vector<double> snM(2 * (size_t)(p->l + 1));
     for (int M = 0; M <= p->l; M++) {

@@ -30,8 +30,8 @@ sys.path.insert(0, ROOT)
 import hitomi_amd as hm  # noqa: E402
 from hitomi_amd import mocks  # noqa: E402
 
-KERNELS = ["survey_wsums", "survey_minpos", "survey_shift", "survey_weights", "survey_sqsum", "survey_to_complex",
-           "survey_sn", "survey_reduce", "assign_count", "assign_scan", "assign_scatter", "assign_gather",
+KERNELS = ["survey_g", "survey_wsums", "survey_minpos", "survey_shift", "survey_weights", "survey_sqsum", "survey_to_complex",
+           "survey_sn", "survey_reduce", "survey_fft_z", "survey_fft_y", "survey_fft_x", "assign_count", "assign_scan", "assign_scatter", "assign_gather",
            "assign_border", "fill_zero", "fft_c2c_z", "fft_c2c_y", "fft_c2c_x", "bin_survey", "bin_reduce"]
 
 
@@ -69,7 +69,7 @@ def main():
             ms = (time.perf_counter() - t) * 1000.0 / args.reps
             out["ms_per_measurement"][f"{name} P_{ell}"] = round(ms, 2)
             out["first_bins"][f"{name} P_{ell}"] = [float(f"{v:.6e}") for v in pk.real[:3]]
-            if prec == hm.FP32 and ell == 0:
+            if prec == hm.FP32 and ell in (0, 2):
                 ctx.timing(True)
                 ctx.timing_reset()
                 plan.execute(dpd, dwd, dpr, dwr)
@@ -80,8 +80,8 @@ def main():
                     if nl:
                         br[k] = [round(kms, 3), nl]
                 ctx.timing(False)
-                out["fp32_P0_kernels_ms_launches"] = br
-                out["fp32_P0_kernel_total_ms"] = round(sum(v[0] for v in br.values()), 3)
+                out[f"fp32_P{ell}_kernels_ms_launches"] = br
+                out[f"fp32_P{ell}_kernel_total_ms"] = round(sum(v[0] for v in br.values()), 3)
             plan.close()
             print(f"{name} P_{ell}: {ms:.2f} ms", file=sys.stderr, flush=True)
     p32 = np.array(out["first_bins"]["fp32 P_0"])
