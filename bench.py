@@ -113,7 +113,7 @@ def tri_z_gram_flops(M, nk):
 # HBM bytes per launch of the dominant kernels, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE on this workload and scaled by the counters' reading
 # of a 1 GiB copy (x2.000 for FETCH_SIZE, x1.000 for WRITE_SIZE).
-MEASURED_TRAFFIC = {"tri_z_gram": 1398.5e6}
+MEASURED_TRAFFIC = {"tri_z_gram": 1245.0e6}
 
 
 def make_mocks(ctx, count, seed0=1000):
@@ -180,36 +180,51 @@ def covariance_run(ctx, n_mocks, lanes, seed0=5000):
     return out
 
 
-def cpu_baseline(pos, budget_s=30.0):
+def host_facts():
+    """nproc and the CPU model of the host the baseline runs on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(pos):
     """The FP64 CPU oracle (serial, 1 core) on a bounded sample of the same
-    workload: one P_0 job and one B_000 job (ith_kbin = 0, 20 k2 bins) of the
-    reference's job structure on the same catalogue at 128^3, plus the
-    assignment alone; scaled to 512^3 (x64 for the mesh-proportional part,
-    which ignores the extra log N of the FFTs and so favours the CPU) and to
-    the 3 + 20 jobs of one mock."""
+    workload: one P_0 job of the reference's job structure at the full 512^3
+    size on the same catalogue (assignment, FFT, binning), and one B_000 job
+    (ith_kbin = 0, 20 k2 bins) at 128^3, whose mesh-proportional part is scaled
+    x64 to 512^3 (this ignores the extra log N of the FFTs and so favours the
+    CPU); one mock = 3 P jobs + 20 B jobs."""
     from oracle import pyoracle as po
 
+    p512 = po.param(N_MESH, BOX, po.TSC, kmin=KW["kmin"], kmax=KW["kmax"], n_kbin=KW["n_kbin"])
+    t = time.time()
+    po.box_pk(p512, pos)
+    t_pk = time.time() - t
     n = 128
     p = po.param(n, BOX, po.TSC, kmin=KW["kmin"], kmax=KW["kmax"], n_kbin=KW["n_kbin"])
     t = time.time()
     po.box_density_fluctuation(p, pos)
     t_assign = time.time() - t
     t = time.time()
-    po.box_pk(p, pos)
-    t_pk = time.time() - t
-    t = time.time()
     pb = po.param(n, BOX, po.TSC, kmin=KW["kmin"], kmax=KW["kmax"], n_kbin=KW["n_kbin"], ith_kbin=0)
     po.box_bk(pb, pos)
     t_bk = time.time() - t
     scale = (N_MESH / n) ** 3
-    pk_job = t_assign + scale * max(t_pk - t_assign, 0.0)
     bk_job = 6 * t_assign + scale * max(t_bk - 6 * t_assign, 0.0)
-    sec_per_mock = 3 * pk_job + 20 * bk_job
-    sample = (f"oracle FP64, 1 core: assign {t_assign:.2f}s, P_0 job {t_pk:.2f}s, B_000 job (ith_kbin=0) "
-              f"{t_bk:.2f}s at 128^3 on the 4.0e6-particle mock; scaled x{scale:.0f} (mesh part) to 512^3 and to "
-              f"3 P + 20 B jobs = {sec_per_mock:.0f} s/mock")
+    sec_per_mock = 3 * t_pk + 20 * bk_job
+    sample = (f"oracle FP64, 1 core: P_0 job at 512^3 {t_pk:.1f}s (timed at full size), B_000 job (ith_kbin=0) "
+              f"{t_bk:.2f}s at 128^3 (assignment {t_assign:.2f}s) scaled x{scale:.0f} (mesh part) to 512^3; "
+              f"3 P + 20 B jobs = {sec_per_mock:.0f} s/mock, on the 4.0e6-particle mock")
     return {"value": 3600.0 / sec_per_mock, "unit": "mocks/hour", "cores": 1, "kind": "port", "sample": sample,
-            "sample_seconds": round(t_assign + t_pk + t_bk, 1)}
+            "sample_seconds": round(t_pk + t_assign + t_bk, 1), "p0_job_512_s": round(t_pk, 2),
+            "bk_job_512_s_scaled": round(bk_job, 1), "host": host_facts()}
 
 
 def copy_calibration(ctx, nbytes):
@@ -318,6 +333,7 @@ def run(args):
         copy_calibration(ctx, args.copy_calibration)
 
     plain = None
+    plain_regrid = None
     err = None
     if args.plain_steps > 0:
         pplan = hm.BoxPlan(ctx, N_MESH, BOX, "TSC", precision=hm.FP64, regrid=False, prune=False, **KW)
@@ -343,6 +359,20 @@ def run(args):
         pplan.close()
         plain = {"mocks_per_hour": round(3600e3 / ms_plain, 1), "ms_per_mock": round(ms_plain, 2),
                  "what": "FP64 grids and FFTs, full 512^3 shell fields (no re-grid, no pruning), one mock at a time"}
+        # the FP32 path's algorithm (exact re-grid, pruned passes) in FP64, one mock at a time: what the FP32
+        # arithmetic alone earns
+        fplan = hm.BoxPlan(ctx, N_MESH, BOX, "TSC", precision=hm.FP64, **KW)
+        fplan.execute(dpos[0])
+        ctx.sync()
+        t = time.perf_counter()
+        for i in range(args.plain_steps):
+            fplan.execute(dpos[i % len(dpos)])
+        ctx.sync()
+        ms_f64 = (time.perf_counter() - t) * 1000.0 / args.plain_steps
+        fplan.close()
+        plain_regrid = {"ms_per_mock": round(ms_f64, 2), "mocks_per_hour": round(3600e3 / ms_f64, 1),
+                        "what": "FP64 grids and FFTs with the FP32 path's algorithm (exact re-grid to the triple "
+                                "mesh, pruned passes), one mock at a time (no batch lanes)"}
 
     out = {
         "metric": METRIC,
@@ -369,31 +399,45 @@ def run(args):
         "batched_vector_equals_per_mock": same,
         "autotune": tune.split("chosen: ")[-1] if "chosen" in tune else tune,
         "graph_replay_ms_per_step": round(ms_graph, 3),
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get(dom_name),
-                     "algorithmic_bytes_per_launch": int(b_dom), "ms_per_launch": round(ms_launch, 4),
-                     "launches": n_dom, "busy_ms": round(busy_dom, 3),
-                     "avg_launch_ms": round(ms_dom / n_dom, 4),
-                     "avg_launch_ms_alone": round(dom[0] / dom[2], 4),
-                     "achieved_alone": round(b_dom / (dom[0] / dom[2]) / 1e6, 1),
-                     "frac_alone": round(b_dom / (dom[0] / dom[2]) / 1e6 / HBM_PEAK_GBS, 4),
-                     "note": "HIP events on each lane's stream around every launch in the timed steps; ms_per_launch "
-                             "= busy_ms (the union of the launch intervals: overlapping launches of different lanes "
-                             "count once) / launches; avg_launch_ms = the mean interval, which includes the slowdown "
-                             "of sharing the GPU with the other lanes; *_alone: one launch with the GPU to itself"},
     }
+    hbm = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "traffic": MEASURED_TRAFFIC.get(dom_name),
+           "algorithmic_bytes_per_launch": int(b_dom), "ms_per_launch": round(ms_launch, 4),
+           "launches": n_dom, "busy_ms": round(busy_dom, 3),
+           "avg_launch_ms": round(ms_dom / n_dom, 4),
+           "avg_launch_ms_alone": round(dom[0] / dom[2], 4),
+           "achieved_alone": round(b_dom / (dom[0] / dom[2]) / 1e6, 1),
+           "frac_alone": round(b_dom / (dom[0] / dom[2]) / 1e6 / HBM_PEAK_GBS, 4),
+           "note": "HIP events on each lane's stream around every launch in the timed steps; ms_per_launch "
+                   "= busy_ms (the union of the launch intervals: overlapping launches of different lanes "
+                   "count once) / launches; avg_launch_ms = the mean interval, which includes the slowdown "
+                   "of sharing the GPU with the other lanes; *_alone: one launch with the GPU to itself"}
     if fl_dom:
-        out["roofline"]["compute"] = {
-            "unit": "TFLOP/s", "flops_per_launch": int(fl_dom), "peak": FP32_PEAK_TFLOPS,
-            "achieved": round(fl_dom / ms_launch / 1e9, 2), "frac": round(fl_dom / ms_launch / 1e9 / FP32_PEAK_TFLOPS, 4),
+        # the roof that binds: FP32 arithmetic (the SQ counters show the vector ALUs issuing most of the time
+        # and HBM traffic at a few percent of its peak)
+        out["roofline"] = {
+            "bound": "mfma", "kernel": dom_name, "unit": "TFLOP/s", "flops_per_launch": int(fl_dom),
+            "peak": FP32_PEAK_TFLOPS, "achieved": round(fl_dom / ms_launch / 1e9, 2),
+            "frac": round(fl_dom / ms_launch / 1e9 / FP32_PEAK_TFLOPS, 4),
+            "traffic": MEASURED_TRAFFIC.get(dom_name), "ms_per_launch": round(ms_launch, 4), "launches": n_dom,
             "achieved_alone": round(fl_dom / (dom[0] / dom[2]) / 1e9, 2),
             "frac_alone": round(fl_dom / (dom[0] / dom[2]) / 1e9 / FP32_PEAK_TFLOPS, 4),
-            "note": "the kernel is bound by FP32 vector issue and LDS, not HBM: the pair sums are complex FMAs on "
-                    "the vector ALUs (no MFMA); the code is built without packed FP32, which halves the vector peak "
-                    "it can reach (78.65 TFLOP/s)"}
+            "note": "compute-bound on FP32 arithmetic: the z transforms of 21 lines per row in LDS and the 210 "
+                    "complex pair sums run on the vector ALUs (the SQ counters show them issuing most of the "
+                    "time; moving the pair sums to v_mfma_f32_16x16x4_f32 did not help because the LDS "
+                    "transforms are the larger part); bound 'mfma' stands for the compute roof; peak = MI355X's "
+                    "published FP32 rate; the code is built without packed FP32, which halves the vector rate it "
+                    "can reach (78.65 TFLOP/s); traffic = measured HBM bytes per launch; the HBM view of the same "
+                    "launches is roofline_hbm",
+        }
+        out["roofline_hbm"] = hbm
+    else:
+        out["roofline"] = hbm
     if plain:
         out["plain_fp64"] = plain
+        out["fp64_regrid_prune"] = plain_regrid
+        out["speedup_vs_fp64_regrid_prune"] = round(value / plain_regrid["mocks_per_hour"], 2)
         out["fp32_error_vs_fp64"] = err
         out["speedup_vs_plain_fp64"] = round(value / plain["mocks_per_hour"], 2)
     if args.cov_mocks > 0:

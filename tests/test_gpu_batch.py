@@ -146,3 +146,37 @@ def test_lane_timing_busy_time(ctx, lanes):
         assert busy >= total * (1.0 - 1e-6) - 1e-6
     else:
         assert busy >= total / lanes * (1.0 - 1e-6) - 1e-6
+
+
+def test_covariance_workload_full_size_bitwise(ctx):
+    """The covariance workload at its stated size: mocks of uneven size
+    (Poisson around a Gamma-scattered mean of 1.5e6, truncated to
+    [0.8e6, 2.5e6]) generated on the device from a 256^3 lattice, L = 1500,
+    256^3 mesh, 20 bins 0.015-0.205, P_0, P_2, P_4 and B_000; the graph-replayed
+    batch with autotuned launch shapes and three lanes gives each mock's
+    vector bit for bit as the per-mock plan does, and the covariance equals
+    numpy.cov of those vectors within 1e-10 of the largest variance."""
+    from hitomi_amd import mocks as hmm
+
+    sizes = hmm.poisson_sizes(5, 1_500_000, 800_000, 2_500_000, seed=77, scatter=0.25)
+    cfg = hm.mock_config(256, 1500.0, amp=2.0e4, k0=0.05, slope=-1.5, kcut=0.5, bao_amp=0.05, bao_period=0.0628)
+    kw = dict(n_kbin=20, kmin=0.015, kmax=0.205, pk_ells=(0, 2, 4), bk=True)
+    pos = [hm.mock_zeldovich(ctx, cfg, 900 + i, int(n)) for i, n in enumerate(sizes)]
+    plan = hm.BoxPlan(ctx, 256, 1500.0, "TSC", precision=hm.FP32, **kw)
+    ref = np.array([per_mock_vector(plan.execute(p), 20, 3) for p in pos])
+    plan.close()
+    b = hm.Batch(ctx, 256, 1500.0, max_mocks=8, use_graph=True, lanes=3, precision=hm.FP32, **kw)
+    b.autotune(pos[0], reps=1)
+    for p in pos:
+        b.push(p)
+    got = b.vectors()
+    assert np.array_equal(got, ref)
+    mean, cov = b.covariance()
+    x = ref[:, : b.cov_len]
+    want = np.cov(x.T)
+    assert cov.shape == (60, 60)
+    assert np.abs(cov - want).max() <= 1e-10 * np.diag(want).max()
+    assert np.abs(mean - x.mean(0)).max() <= 1e-12 * np.abs(x).max()
+    b.close()
+    for p in pos:
+        p.free()
