@@ -186,6 +186,8 @@ _SIGS = {
     "hm_recon_displace": (ct.c_int, [_VP, ct.POINTER(ReconConfig), _VP, _VP, ct.c_longlong, _VP, _VP, ct.c_longlong,
                                      ct.c_double, ct.c_double]),
     "hm_sph_bessel_jl": (ct.c_double, [ct.c_int, ct.c_double]),
+    "hm_wigner_3j": (ct.c_double, [ct.c_int] * 6),
+    "hm_ylm_reduced": (None, [ct.c_int, ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "hm_periodic_wrap": (ct.c_int, [_VP, _VP, ct.c_longlong, ct.POINTER(ct.c_double)]),
     "hm_survey_plan_create": (_VP, [_VP, ct.POINTER(SurveyConfig)]),
     "hm_survey_plan_destroy": (None, [_VP, _VP]),

@@ -4,6 +4,7 @@
 #include "hm_assign.hpp"
 #include "hm_internal.hpp"
 #include "hm_bessel.hpp"
+#include "hm_bins.hpp"
 #include "hm_recon.hpp"
 
 #include "../../include/hitomi_amd.h"
@@ -572,5 +573,15 @@ int hm_periodic_wrap(hm_ctx* ctx, double* d_pos, long long n, const double boxsi
 }
 
 double hm_sph_bessel_jl(int l, double x) { return hm::sph_jl(l, x); }
+
+double hm_wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+    return hm::wigner_3j(j1, j2, j3, m1, m2, m3);
+}
+
+void hm_ylm_reduced(int l, int m, const double v[3], double out[2]) {
+    const hm::Cplx y = hm::ylm_red(l, m, v[0], v[1], v[2]);
+    out[0] = y.re;
+    out[1] = y.im;
+}
 
 }  // extern "C"

@@ -204,6 +204,15 @@ int hm_box_plan_set_shapes(hm_box_plan *plan, unsigned wide_mask, int bin_block)
  * bispectrum shot noise and the three-point function sample; host code,
  * no GPU needed.  NaN for l < 0 or x < 0. */
 double hm_sph_bessel_jl(int l, double x);
+/* gsl_sf_coupling_3j with integer arguments (common.hpp:29; the weights of
+ * every P, B and zeta multipole), by the Racah formula in long double; host
+ * code.  0 outside the triangle and selection rules. */
+double hm_wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3);
+/* The reduced spherical harmonic of tools.hpp:12-68, conj(Y_lm(v/|v|)) *
+ * sqrt(4 pi / (2l + 1)), with its (0, 0) and |v| < 1e-15 cases; the same
+ * function the kernels evaluate per mode and per separation, here on the
+ * host.  Writes the real and imaginary parts to out[0], out[1]. */
+void hm_ylm_reduced(int l, int m, const double v[3], double out[2]);
 
 /* ---------------------------------------------------------------- */
 /* reconstruction (particle_reconstruction.hpp)                       */

@@ -4,11 +4,13 @@ The reference ships no test vectors (its plot_test.py bodies are commented
 out), so the values that pin the replaced FFTW/GSL routines are computed here
 from independent implementations:
   - numpy.fft for the unnormalised 3D/1D DFT (fftw_plan_dft_3d semantics);
-  - sympy exact rationals for the Wigner 3j symbols (gsl_sf_coupling_3j);
+  - sympy exact rationals for the Wigner 3j symbols (gsl_sf_coupling_3j):
+    every m for j1, j2 <= 4, and up to j1, j2 <= 8, j3 <= 16 (the multipoles
+    the plans accept) every triangle with m = 0 plus a spread of m values;
   - sympy's exact associated Legendre functions, normalised as
-    gsl_sf_legendre_sphPlm;
+    gsl_sf_legendre_sphPlm, for l <= 16;
   - mpmath at 30 digits for the spherical Bessel functions (gsl_sf_bessel_jl);
-  - sympy's Ynm for the reduced spherical harmonics of tools.hpp:12-68.
+  - sympy's Ynm for the reduced spherical harmonics of tools.hpp:12-68, l <= 8.
 Run from the repository root:  python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -42,19 +44,33 @@ def fft_fixtures():
 
 def special():
     w3j = []
+    seen = set()
+
+    def add(j1, j2, j3, m1, m2, m3):
+        key = (j1, j2, j3, m1, m2, m3)
+        if key in seen or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3 or m1 + m2 + m3 != 0:
+            return
+        seen.add(key)
+        w3j.append([j1, j2, j3, m1, m2, m3, float(wigner_3j(j1, j2, j3, m1, m2, m3))])
+
+    # every m for j1, j2 <= 4, j3 <= 6
     for j1 in range(0, 5):
         for j2 in range(0, 5):
             for j3 in range(abs(j1 - j2), min(j1 + j2, 6) + 1):
                 for m1 in range(-j1, j1 + 1):
                     for m2 in range(-j2, j2 + 1):
-                        m3 = -m1 - m2
-                        if abs(m3) > j3:
-                            continue
-                        v = float(wigner_3j(j1, j2, j3, m1, m2, m3))
-                        w3j.append([j1, j2, j3, m1, m2, m3, v])
+                        add(j1, j2, j3, m1, m2, -m1 - m2)
+    # the range the plans accept (ell1, ell2 <= 8, ELL <= 16): every triangle with m = 0, and for each a
+    # spread of m1, m2 (the extremes, small values and one in between)
+    for j1 in range(0, 9):
+        for j2 in range(0, 9):
+            for j3 in range(abs(j1 - j2), j1 + j2 + 1):
+                for m1 in sorted({-j1, 0, 1, j1}):
+                    for m2 in sorted({-j2, -1, 0, j2}):
+                        add(j1, j2, j3, m1, m2, -m1 - m2)
     x = sp.Symbol("x")
     plm = []
-    for l in range(0, 7):
+    for l in range(0, 17):
         for m in range(0, l + 1):
             # P_l^m with the Condon-Shortley phase, then the sphPlm normalisation
             P = sp.assoc_legendre(l, m, x)
@@ -75,7 +91,7 @@ def special():
     ylm = []
     th, ph = sp.symbols("theta phi", real=True)
     vecs = [(0.3, -0.4, 0.5), (1.0, 2.0, -3.0), (-0.2, 0.1, 0.9), (0.0, 0.0, 1.0), (0.6, -0.8, 0.0), (-1.0, -1.0, 1.0)]
-    for l in range(0, 5):
+    for l in range(0, 9):
         for m in range(-l, l + 1):
             Y = sp.Ynm(l, m, th, ph).expand(func=True)
             for v in vecs:
@@ -86,7 +102,7 @@ def special():
                 val = val.conjugate() * float(np.sqrt(4 * np.pi / (2 * l + 1)))
                 ylm.append([l, m, list(v), val.real, val.imag])
     with open(os.path.join(HERE, "special.json"), "w") as f:
-        json.dump({"wigner3j": w3j, "sphPlm": plm, "bessel_jl": jl, "ylm_reduced": ylm}, f, indent=0)
+        json.dump({"wigner3j": w3j, "sphPlm": plm, "bessel_jl": jl, "ylm_reduced": ylm}, f, separators=(",", ":"))
 
 
 if __name__ == "__main__":
