@@ -204,6 +204,14 @@ _SIGS = {
     "hm_batch_covariance": (ct.c_int, [_VP, _VP, _VP, _VP]),
     "hm_batch_reset": (ct.c_int, [_VP, _VP]),
     "hm_batch_autotune": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.c_int, ct.c_char_p, ct.c_size_t]),
+    "hm_cov_create": (_VP, [_VP, ct.c_int, _VP, ct.c_longlong]),
+    "hm_cov_destroy": (None, [_VP, _VP]),
+    "hm_cov_push": (ct.c_int, [_VP, _VP, _VP]),
+    "hm_cov_push_host": (ct.c_int, [_VP, _VP, _VP]),
+    "hm_cov_count": (ct.c_longlong, [_VP]),
+    "hm_cov_vectors": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong]),
+    "hm_cov_result": (ct.c_int, [_VP, _VP, _VP, _VP]),
+    "hm_cov_reset": (ct.c_int, [_VP, _VP]),
     "hm_mock_white_noise": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, _VP]),
     "hm_mock_zeldovich": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, ct.c_longlong, _VP]),
     "hm_mock_zeldovich_biased": (ct.c_int, [_VP, ct.POINTER(MockConfig), ct.c_ulonglong, ct.c_double, ct.c_double,

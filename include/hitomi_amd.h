@@ -348,6 +348,34 @@ int hm_batch_autotune(hm_ctx *ctx, hm_batch *b, const double *d_pos, long long n
                       size_t report_len);
 
 /* ---------------------------------------------------------------- */
+/* Covariance of window-normalised data vectors, accumulated on the device.
+ * Replaces the numpy stage of
+ * struct/covariance/boss/modified_gravity/calc_data_and_cov.py: the
+ * normalisation xi_l / w_0 and zeta_l1l2L / w_000 (:72-105), and numpy.cov of
+ * the stacked multipoles and of each pair of blocks (:284-307), which are the
+ * blocks of one covariance of the concatenated vector.  Each push computes
+ * x = v / norm (FP64, elementwise) and adds d = x - x_0 (x_0 the first
+ * vector) into device sums of d and d d^T; pushes are ordered on ctx's
+ * stream and do not synchronise with the host. */
+typedef struct hm_cov hm_cov;
+/* dim: vector length (1..4096); norm: dim window values on the host (copied;
+ * none may be zero) or NULL for no division; keep_rows: normalised vectors
+ * kept on the device for hm_cov_vectors. */
+hm_cov *hm_cov_create(hm_ctx *ctx, int dim, const double *norm, long long keep_rows);
+void hm_cov_destroy(hm_ctx *ctx, hm_cov *c);
+/* One vector already in device memory (dim FP64), read in stream order. */
+int hm_cov_push(hm_ctx *ctx, hm_cov *c, const double *d_vec);
+/* One vector in host memory (copied before return). */
+int hm_cov_push_host(hm_ctx *ctx, hm_cov *c, const double *vec);
+long long hm_cov_count(const hm_cov *c);
+/* The first min(count, keep_rows, max_rows) normalised vectors, [rows][dim]. */
+int hm_cov_vectors(hm_ctx *ctx, hm_cov *c, double *out, long long max_rows);
+/* mean [dim] and the N - 1 sample covariance [dim][dim] (numpy.cov); needs
+ * two or more vectors. */
+int hm_cov_result(hm_ctx *ctx, hm_cov *c, double *mean, double *cov);
+int hm_cov_reset(hm_ctx *ctx, hm_cov *c);
+
+/* ---------------------------------------------------------------- */
 /* Seeded Zel'dovich mocks generated on the device.  No counterpart in the
  * reference (its only synthetic input is an unseeded uniform catalogue,
  * struct/measurement/nbody/pk/calc_random.py:38-49).  White noise from the
