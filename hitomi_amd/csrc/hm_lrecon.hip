@@ -8,7 +8,7 @@
 //             reconstruction on the mesh and (rho / rho_bar - 1) / 8
 //   network   four 3^3 convolutions with periodic padding, 4 -> 16 -> 16 ->
 //             16 -> 3 channels, ReLU after the first three; activations in
-//             BF16, channels last; MFMA 16x16x16 BF16 with FP32 accumulation
+//             BF16, channels last; MFMA 16x16x32 BF16 with FP32 accumulation
 //   shift     S_i = norm Psi(x_i) + psi_scale out(x_i), both gathered with the
 //             TSC stencil of the reference's reconstruction gather
 //   loss      mean_i |S_i - T_i|^2 against the true shifts T_i
@@ -21,11 +21,13 @@
 // gradient an MFMA reduction over voxels; Adam (Kingma and Ba, ICLR 2015)
 // updates FP32 master weights.
 //
-// Convolution as an implicit GEMM: for 16 consecutive voxels along z (M), 16
-// output channels (N) and K = 27 taps x cin input channels in steps of 16,
-// lane l of a wave holds A[voxel l % 16][k = 16 s + 4 (l / 16) + e] (one tap's
-// four consecutive channels, an 8-byte load), B[k][l % 16] (weights packed per
-// lane once per update), and receives D[4 (l / 16) + e][l % 16].
+// Convolution as an implicit GEMM on gfx950's v_mfma_f32_16x16x32_bf16: for 16
+// consecutive voxels along z (M), 16 output channels (N) and K = 27 taps x cin
+// input channels in steps of 32, lane l of a wave holds
+// A[voxel l % 16][k = 32 s + 8 (l / 16) + e], e = 0..7 (with 16 channels one
+// tap's eight consecutive channels, a 16-byte load; with 4 channels two taps),
+// B[k][l % 16] (weights packed per lane once per update), and receives
+// D[4 (l / 16) + e][l % 16], e = 0..3.
 #include "hm_internal.hpp"
 #include "hm_assign.hpp"
 #include "hm_recon.hpp"
@@ -35,8 +37,20 @@
 
 using namespace hm;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// K steps of 32 for the 3^3 taps of cin channels
+constexpr int conv_steps(int cin) { return (27 * cin + 31) / 32; }
+// packed B operands per layer: at most 14 steps x 64 lanes x 8 values
+constexpr size_t WP_LAYER = 16 * 512;
+
+__device__ __forceinline__ f32x4 mfma32(u16x8 a, u16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
 
 namespace {
 
@@ -59,15 +73,15 @@ __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >
 // weights
 // ---------------------------------------------------------------------------
 // W [27][cin][cout] (tap t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1)) -> per-lane
-// B operands of ks steps.  Forward: B[k][j] = W[t][ci][j], k = t cin + ci.
-// Input gradient (transpose): the convolution of dZ (16 channels, cout used)
-// with B[k][j] = W[26 - t][j][c], k = 16 t + c, j = ci.
+// B operands of ks steps ([ks][64][8]).  Forward: B[k][j] = W[t][ci][j],
+// k = t cin + ci.  Input gradient (transpose): the convolution of dZ (16
+// channels, cout used) with B[k][j] = W[26 - t][j][c], k = 16 t + c, j = ci.
 __global__ void k_pack_w(const float* __restrict__ W, int cin, int cout, int ks, int transpose,
                          unsigned short* __restrict__ wp) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= ks * 256) return;
-    const int s = idx / 256, l = (idx / 4) % 64, e = idx % 4;
-    const int k = 16 * s + 4 * (l >> 4) + e, j = l & 15;
+    if (idx >= ks * 512) return;
+    const int s = idx / 512, l = (idx / 8) % 64, e = idx % 8;
+    const int k = 32 * s + 8 * (l >> 4) + e, j = l & 15;
     float v = 0.0f;
     if (!transpose) {
         const int t = k / cin, ci = k % cin;
@@ -97,7 +111,7 @@ enum { CONV_RELU_BF16 = 0, CONV_F32 = 1, CONV_MASK_BF16 = 2 };
 
 struct ConvArgs {
     const unsigned short* in;  // [ng][CIN] BF16
-    const unsigned short* wp;  // [ks][64][4] BF16
+    const unsigned short* wp;  // [ks][64][8] BF16
     int ks;
     const float* bias;         // [16] or null
     int n0, n1, n2;
@@ -105,6 +119,31 @@ struct ConvArgs {
     void* out;                 // RELU / MASK: [ng][16] BF16; F32: [ng][ostride] FP32
     const unsigned short* mask;  // MASK: [ng][16] BF16, gate > 0
 };
+
+// The A operand of K step s for lane group g at voxel z of row base rb (the
+// first voxel of the tap's row): 16 channels -> tap 2 s + g / 2, channels
+// 8 (g % 2) .. + 7; 4 channels -> taps 8 s + 2 g and 8 s + 2 g + 1, all four.
+// Taps past 26 are zero.  row(t) gives the voxel index of (x + dx, y + dy, 0);
+// zof(dz) the wrapped z of this lane's voxel.
+template <int CIN, typename Row, typename Zof>
+__device__ __forceinline__ u16x8 conv_a(const unsigned short* __restrict__ in, int s, int g, Row row, Zof zof) {
+    u16x8 av = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (CIN == 16) {
+        const int t = 2 * s + (g >> 1);
+        if (t < 27) av = *reinterpret_cast<const u16x8*>(in + (row(t) + zof(t % 3 - 1)) * 16 + 8 * (g & 1));
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int t = 8 * s + 2 * g + h;
+            if (t < 27) {
+                const u16x4 q = *reinterpret_cast<const u16x4*>(in + (row(t) + zof(t % 3 - 1)) * 4);
+#pragma unroll
+                for (int e = 0; e < 4; e++) av[4 * h + e] = q[e];
+            }
+        }
+    }
+    return av;
+}
 
 template <int CIN, int TPW>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
@@ -119,20 +158,18 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     f32x4 acc[TPW];
 #pragma unroll
     for (int tt = 0; tt < TPW; tt++) acc[tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const bf16x4* W = reinterpret_cast<const bf16x4*>(a.wp);
+    const u16x8* W = reinterpret_cast<const u16x8*>(a.wp);
+    const auto tap_row = [&](int t) -> long long {
+        return ((long long)wrapi(x + t / 9 - 1, a.n0) * a.n1 + wrapi(y + (t / 3) % 3 - 1, a.n1)) * a.n2;
+    };
     for (int s = 0; s < a.ks; s++) {
-        const bf16x4 bv = W[s * 64 + lane];
-        const int t = (CIN == 16) ? s : 4 * s + g;
-        const int ci0 = (CIN == 16) ? 4 * g : 0;
-        const bool valid = t < 27;
-        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-        const long long rb = ((long long)wrapi(x + dx, a.n0) * a.n1 + wrapi(y + dy, a.n1)) * a.n2;
+        const u16x8 bv = W[s * 64 + lane];
 #pragma unroll
         for (int tt = 0; tt < TPW; tt++) {
-            const int z = wrapi((seg * TPW + tt) * 16 + m + dz, a.n2);
-            bf16x4 av = bf16x4{0, 0, 0, 0};
-            if (valid) av = *reinterpret_cast<const bf16x4*>(a.in + (rb + z) * CIN + ci0);
-            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[tt], 0, 0, 0);
+            const int zb = (seg * TPW + tt) * 16 + m;
+            const u16x8 av =
+                conv_a<CIN>(a.in, s, g, tap_row, [&](int dz) { return (long long)wrapi(zb + dz, a.n2); });
+            acc[tt] = mfma32(av, bv, acc[tt]);
         }
     }
     const int c = lane & 15;
@@ -199,21 +236,19 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
     for (int ry = 0; ry < CL_RY; ry++)
 #pragma unroll
         for (int tt = 0; tt < TPW; tt++) acc[ry][tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const bf16x4* W = reinterpret_cast<const bf16x4*>(a.wp);
+    const u16x8* W = reinterpret_cast<const u16x8*>(a.wp);
     for (int s = 0; s < a.ks; s++) {
-        const bf16x4 bv = W[s * 64 + lane];
-        const int t = (CIN == 16) ? s : 4 * s + g;
-        const int ci0 = (CIN == 16) ? 4 * g : 0;
-        const bool valid = t < 27;
-        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        const u16x8 bv = W[s * 64 + lane];
 #pragma unroll
         for (int ry = 0; ry < CL_RY; ry++) {
-            const int hrow = ((wv + 1 + dx) * HY + (ry + 1 + dy)) * HZ;
+            const auto row = [&](int t) -> long long {
+                return (long long)((wv + 1 + t / 9 - 1) * HY + (ry + 1 + (t / 3) % 3 - 1)) * HZ;
+            };
 #pragma unroll
             for (int tt = 0; tt < TPW; tt++) {
-                bf16x4 av = bf16x4{0, 0, 0, 0};
-                if (valid) av = *reinterpret_cast<const bf16x4*>(sm + (hrow + tt * 16 + m + 1 + dz) * CIN + ci0);
-                acc[ry][tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[ry][tt], 0, 0, 0);
+                const u16x8 av =
+                    conv_a<CIN>(sm, s, g, row, [&](int dz) { return (long long)(tt * 16 + m + 1 + dz); });
+                acc[ry][tt] = mfma32(av, bv, acc[ry][tt]);
             }
         }
     }
@@ -245,9 +280,11 @@ template <int CIN, int TPW> constexpr size_t conv_lds_bytes() {
 }
 
 // ---------------------------------------------------------------------------
-// weight gradient: per wave, over its voxel tiles,
+// weight gradient: per wave, over its tiles of 32 voxels (the K of one MFMA),
 //   G[k][co] += sum_v X[v + off_t][ci] dZ[v][co],  k = t cin + ci,
-// plus a bias block (row 0: sum_v dZ[v][co]); partials per wave
+// plus a bias block (row 0: sum_v dZ[v][co]); partials per wave.  Lane group g
+// takes voxels 8 g .. 8 g + 7 of a tile, which lie in one z row (rows are
+// multiples of 16 voxels).
 // ---------------------------------------------------------------------------
 struct DwArgs {
     const unsigned short* x;   // [ng][CIN]
@@ -269,33 +306,33 @@ __device__ __forceinline__ void dw_body(const DwArgs& a, long long wave) {
     for (int q = 0; q < NB; q++) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const long long t0 = wave * a.tiles_per_wave;
     const long long t1 = (t0 + a.tiles_per_wave < a.tiles) ? t0 + a.tiles_per_wave : a.tiles;
-    bf16x4 one = bf16x4{0, 0, 0, 0};
-    if (m == 0) one = bf16x4{0x3F80, 0x3F80, 0x3F80, 0x3F80};
+    const unsigned short o1 = m == 0 ? (unsigned short)0x3F80 : (unsigned short)0;
+    const u16x8 one = u16x8{o1, o1, o1, o1, o1, o1, o1, o1};
     for (long long tile = t0; tile < t1; tile++) {
-        const long long v0 = tile * 16;
+        const long long v0 = tile * 32 + 8 * g;
         const int z0 = (int)(v0 % a.n2);
         const long long row = v0 / a.n2;
         const int y = (int)(row % a.n1), x = (int)(row / a.n1);
-        bf16x4 bv;
+        u16x8 bv;
 #pragma unroll
-        for (int e = 0; e < 4; e++) bv[e] = (short)a.dz[(v0 + 4 * g + e) * 16 + m];
+        for (int e = 0; e < 8; e++) bv[e] = a.dz[(v0 + e) * 16 + m];
 #pragma unroll
         for (int q = 0; q < NB; q++) {
             const int kb = KB0 + q;
             if (kb == NKB - 1) {
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(one, bv, acc[q], 0, 0, 0);
+                acc[q] = mfma32(one, bv, acc[q]);
                 continue;
             }
             const int k = 16 * kb + m;
             const int t = k / CIN, ci = k % CIN;
-            bf16x4 av = bf16x4{0, 0, 0, 0};
+            u16x8 av = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
             if (t < 27) {
                 const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
                 const long long rb = ((long long)wrapi(x + dx, a.n0) * a.n1 + wrapi(y + dy, a.n1)) * a.n2;
 #pragma unroll
-                for (int e = 0; e < 4; e++) av[e] = (short)a.x[(rb + wrapi(z0 + 4 * g + e + dz, a.n2)) * CIN + ci];
+                for (int e = 0; e < 8; e++) av[e] = a.x[(rb + wrapi(z0 + e + dz, a.n2)) * CIN + ci];
             }
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+            acc[q] = mfma32(av, bv, acc[q]);
         }
     }
     float* P = a.part + wave * (long long)NKB * 256;
@@ -485,7 +522,7 @@ constexpr int NL = 4;
 constexpr int CIN_L[NL] = {4, 16, 16, 16};
 constexpr int COUT_L[NL] = {16, 16, 16, 3};
 
-int ks_of(int cin) { return (27 * cin + 15) / 16; }
+int ks_of(int cin) { return conv_steps(cin); }
 size_t wcount(int l) { return (size_t)27 * CIN_L[l] * COUT_L[l]; }
 size_t param_count() {
     size_t s = 0;
@@ -517,13 +554,13 @@ int repack(hm_ctx* ctx, hm_lrecon* h) {
     const float* P = (const float*)h->params.p;
     for (int l = 0; l < NL; l++) {
         const int ks = ks_of(CIN_L[l]), kst = ks_of(16);
-        unsigned short* wp = (unsigned short*)h->wp.p + (size_t)l * 27 * 256;
-        unsigned short* wpt = (unsigned short*)h->wpt.p + (size_t)l * 27 * 256;
+        unsigned short* wp = (unsigned short*)h->wp.p + (size_t)l * WP_LAYER;
+        unsigned short* wpt = (unsigned short*)h->wpt.p + (size_t)l * WP_LAYER;
         if (ctx->launch("lrecon_pack", [&] {
-                hipLaunchKernelGGL(k_pack_w, dim3((ks * 256 + 255) / 256), dim3(256), 0, ctx->stream, P + h->woff[l],
+                hipLaunchKernelGGL(k_pack_w, dim3((ks * 512 + 255) / 256), dim3(256), 0, ctx->stream, P + h->woff[l],
                                    CIN_L[l], COUT_L[l], ks, 0, wp);
                 if (l > 0)
-                    hipLaunchKernelGGL(k_pack_w, dim3((kst * 256 + 255) / 256), dim3(256), 0, ctx->stream,
+                    hipLaunchKernelGGL(k_pack_w, dim3((kst * 512 + 255) / 256), dim3(256), 0, ctx->stream,
                                        P + h->woff[l], CIN_L[l], COUT_L[l], kst, 1, wpt);
             }))
             return -1;
@@ -589,7 +626,7 @@ int conv_dw(hm_ctx* ctx, hm_lrecon* h, int l, const unsigned short* x, const uns
     a.n0 = h->mesh.n[0];
     a.n1 = h->mesh.n[1];
     a.n2 = h->mesh.n[2];
-    a.tiles = h->ng / 16;
+    a.tiles = h->ng / 32;
     a.tiles_per_wave = h->tiles_per_wave;
     a.part = (float*)h->part.p;
     const int cin = CIN_L[l];
@@ -619,7 +656,7 @@ int net_forward(hm_ctx* ctx, hm_lrecon* h) {
     const unsigned short* in = (const unsigned short*)h->x.p;
     const float* b16 = (const float*)h->bias16.p;
     for (int l = 0; l < NL; l++) {
-        const unsigned short* wp = (const unsigned short*)h->wp.p + (size_t)l * 27 * 256;
+        const unsigned short* wp = (const unsigned short*)h->wp.p + (size_t)l * WP_LAYER;
         if (l < NL - 1) {
             if (conv(ctx, h, "lrecon_conv", CIN_L[l], in, wp, ks_of(CIN_L[l]), b16 + 16 * l, CONV_RELU_BF16, 16, 16,
                      h->act[l].p, nullptr))
@@ -644,7 +681,7 @@ int net_backward(hm_ctx* ctx, hm_lrecon* h) {
         const unsigned short* xin = l == 0 ? (const unsigned short*)h->x.p : (const unsigned short*)h->act[l - 1].p;
         if (conv_dw(ctx, h, l, xin, dz, G + h->woff[l], G + h->boff[l])) return -1;
         if (l == 0) break;
-        const unsigned short* wpt = (const unsigned short*)h->wpt.p + (size_t)l * 27 * 256;
+        const unsigned short* wpt = (const unsigned short*)h->wpt.p + (size_t)l * WP_LAYER;
         if (conv(ctx, h, "lrecon_conv_dx", 16, dz, wpt, ks_of(16), nullptr, CONV_MASK_BF16, 16, 16, dzn,
                  (const unsigned short*)h->act[l - 1].p))
             return -1;
@@ -658,8 +695,8 @@ int alloc_model(hm_ctx* ctx, hm_lrecon* h) {
     const size_t P = param_count();
     if (ensure(ctx, h->params, sizeof(float) * P) || ensure(ctx, h->grads, sizeof(float) * P) ||
         ensure(ctx, h->adam_m, sizeof(float) * P) || ensure(ctx, h->adam_v, sizeof(float) * P) ||
-        ensure(ctx, h->wp, sizeof(unsigned short) * NL * 27 * 256) ||
-        ensure(ctx, h->wpt, sizeof(unsigned short) * NL * 27 * 256) || ensure(ctx, h->bias16, sizeof(float) * 16 * NL) ||
+        ensure(ctx, h->wp, sizeof(unsigned short) * NL * WP_LAYER) ||
+        ensure(ctx, h->wpt, sizeof(unsigned short) * NL * WP_LAYER) || ensure(ctx, h->bias16, sizeof(float) * 16 * NL) ||
         ensure(ctx, h->feat, sizeof(float) * 4 * (size_t)ng) || ensure(ctx, h->x, sizeof(unsigned short) * 4 * (size_t)ng) ||
         ensure(ctx, h->out, sizeof(float) * 4 * (size_t)ng) || ensure(ctx, h->dzA, sizeof(unsigned short) * 16 * (size_t)ng) ||
         ensure(ctx, h->dzB, sizeof(unsigned short) * 16 * (size_t)ng) ||
@@ -667,9 +704,9 @@ int alloc_model(hm_ctx* ctx, hm_lrecon* h) {
         return -1;
     for (int l = 0; l < 3; l++)
         if (ensure(ctx, h->act[l], sizeof(unsigned short) * 16 * (size_t)ng)) return -1;
-    // weight-gradient partials: waves of >= 64 tiles, at most 4096 waves
-    const long long tiles = ng / 16;
-    long long tpw = std::max(64LL, (tiles + 4095) / 4096);
+    // weight-gradient partials: waves of >= 32 tiles of 32 voxels, at most 4096 waves
+    const long long tiles = ng / 32;
+    long long tpw = std::max(32LL, (tiles + 4095) / 4096);
     h->tiles_per_wave = tpw;
     h->part_waves = ((tiles + tpw - 1) / tpw + 3) / 4 * 4;
     if (ensure(ctx, h->part, sizeof(float) * (size_t)h->part_waves * 28 * 256)) return -1;
