@@ -173,6 +173,32 @@ _SIGS = {
     "hm_fft_c2c": (ct.c_int, [_VP, ct.c_int, _VP, ct.POINTER(ct.c_int), ct.c_int]),
     "hm_fft_r2c": (ct.c_int, [_VP, ct.c_int, _VP, _VP, ct.POINTER(ct.c_int)]),
     "hm_fft_c2r": (ct.c_int, [_VP, ct.c_int, _VP, _VP, ct.POINTER(ct.c_int)]),
+    "hm_grid_alloc": (_VP, [_VP, ct.POINTER(Mesh)]),
+    "hm_grid_free": (ct.c_int, [_VP, _VP]),
+    "hm_assign_field": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.c_double, _VP]),
+    "hm_fourier_transform": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_int]),
+    "hm_window_correct": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP]),
+    "hm_bin_pk": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.POINTER(ct.c_double), ct.c_int,
+                             ct.POINTER(ct.c_double), ct.c_int, ct.c_int, ct.POINTER(ct.c_double),
+                             ct.POINTER(ct.c_longlong)]),
+    "hm_bin_xi": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.POINTER(ct.c_double), ct.c_int,
+                             ct.POINTER(ct.c_double), ct.c_int, ct.c_int, ct.POINTER(ct.c_double),
+                             ct.POINTER(ct.c_longlong)]),
+    "hm_bin_xi_3pcf": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.POINTER(ct.c_double), ct.c_int,
+                                  ct.POINTER(ct.c_double), ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                  ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]),
+    "hm_sn_field_ijk": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.POINTER(ct.c_double), _VP]),
+    "hm_shell_c2r": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_double, ct.c_double, ct.c_int, ct.c_int, _VP,
+                                ct.POINTER(ct.c_longlong)]),
+    "hm_bessel_c2r": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_double, ct.c_int, ct.c_int, ct.c_int, _VP]),
+    "hm_triple_sums": (ct.c_int, [_VP, ct.POINTER(Mesh), ct.POINTER(_VP), ct.c_int, ct.POINTER(ct.c_int), ct.c_int,
+                                  _VP, ct.POINTER(ct.c_double)]),
+    "hm_sn_bessel_sweep": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                      ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.c_int,
+                                      ct.POINTER(ct.c_double)]),
+    "hm_particle_sums": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.c_int, ct.c_int, ct.c_int,
+                                    ct.POINTER(ct.c_double)]),
+    "hm_survey_volume": (ct.c_int, [_VP, ct.POINTER(Mesh), _VP, _VP, ct.c_longlong, ct.POINTER(ct.c_double)]),
     "hm_box_plan_create": (_VP, [_VP, ct.POINTER(BoxConfig)]),
     "hm_box_plan_destroy": (None, [_VP, _VP]),
     "hm_box_plan_execute": (ct.c_int, [_VP, _VP, _VP, ct.c_longlong, ct.POINTER(BoxResult)]),

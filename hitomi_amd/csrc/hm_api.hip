@@ -132,6 +132,7 @@ void hm_ctx_destroy(hm_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
     if (ctx->mock_ws) (void)hipFree(ctx->mock_ws);
+    for (auto& kv : ctx->jl_splines) (void)hipFree(kv.second);
     for (auto& pe : ctx->pending) {
         (void)hipEventDestroy(pe.a);
         (void)hipEventDestroy(pe.b);

@@ -44,6 +44,9 @@ struct hm_ctx {
     // device workspace of the mock generator, kept between calls
     void* mock_ws = nullptr;
     size_t mock_ws_bytes = 0;
+    // j_l spline nodes (y then c, JL_NODES doubles each) of the seam entry
+    // points, keyed by l, kept until the context ends
+    std::map<int, void*> jl_splines;
 
     void set_error(const std::string& m) { err = m; }
     hipEvent_t get_event();

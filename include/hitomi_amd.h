@@ -122,6 +122,89 @@ int hm_fft_r2c(hm_ctx *ctx, int precision, const void *d_in, void *d_out, const 
 int hm_fft_c2r(hm_ctx *ctx, int precision, void *d_in, void *d_out, const int n[3]);
 
 /* ---------------------------------------------------------------- */
+/* single seams on the reference's own grid layout                    */
+/* ---------------------------------------------------------------- */
+/* A "grid" below is DensityFieldClass's field (field.hpp:20-31): complex
+ * FP64 [n0][n1][n2] with (re, im) interleaved like fftw_complex, over the
+ * whole mesh, in device memory.  Each entry point performs one member
+ * function of the reference on such grids, so a caller can replace one seam
+ * at a time; the plans further down fuse whole estimators instead.  Sums
+ * are deterministic (fixed-order reductions).  Calls that return host
+ * values synchronise the context's stream; the others are ordered on it
+ * and also synchronise before returning. */
+
+/* DensityFieldClass constructor (field.hpp:20-31): a zeroed grid; NULL on failure. */
+double *hm_grid_alloc(hm_ctx *ctx, const hm_mesh *mesh);
+/* finalizeDensityField (field.hpp:35-45). */
+int hm_grid_free(hm_ctx *ctx, double *d_grid);
+/* calcField (field.hpp:47-184) into a grid: rho = sum_p w_p W(x - x_p) / dV
+ * with complex weights d_w (np x (re, im); NULL: 1), as hm_assign computes it
+ * (64-bit fixed point, order-independent); wabs_sum as for hm_assign. */
+int hm_assign_field(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const double *d_w, long long np,
+                    double wabs_sum, double *d_grid);
+/* calcFourierTransform (field.hpp:310-326, sign -1: x dV, then FFTW_FORWARD)
+ * and calcInverseFourierTransform (field.hpp:328-344, sign +1: / V, then
+ * FFTW_BACKWARD), in place. */
+int hm_fourier_transform(hm_ctx *ctx, const hm_mesh *mesh, double *d_grid, int sign);
+/* calcAssignmentFunctionCorrection (field.hpp:610-639): grid / W(k), with
+ * the reference's truncated index for negative k. */
+int hm_window_correct(hm_ctx *ctx, const hm_mesh *mesh, double *d_grid);
+/* TwoPointStatisticsClass::calcPowerSpectrum (field.hpp:765-852):
+ * (f1 conj(f2) - sn S(k)) / W(k)^2 Y_red(ell, m)(k) on the fine 1e-4 grid,
+ * folded into the bins around kbin[0..n_kbin-1] (n_kbin >= 2) and averaged.
+ * sn: complex shot noise (NULL: 0).  pk: [n_kbin][2]; n_mode: [n_kbin] (may be NULL). */
+int hm_bin_pk(hm_ctx *ctx, const hm_mesh *mesh, const double *d_f1, const double *d_f2, const double *kbin,
+              int n_kbin, const double sn[2], int ell, int m, double *pk, long long *n_mode);
+/* calcCorrelationFunction (field.hpp:854-977): the same product / V,
+ * FFTW_BACKWARD, then Y_red(ell, m)(r) on the 0.5 Mpc/h fine grid folded into
+ * the bins around rbin (n_rbin >= 2) and averaged.  xi: [n_rbin][2]. */
+int hm_bin_xi(hm_ctx *ctx, const hm_mesh *mesh, const double *d_f1, const double *d_f2, const double *rbin,
+              int n_rbin, const double sn[2], int ell, int m, double *xi, long long *n_mode);
+/* calcCorrelationFunctionForThreePointFunction (field.hpp:979-1113): as
+ * hm_bin_xi with the weight Y_red(ell1, m1)(r) Y_red(ell2, m2)(r) (the
+ * reference's Ylm1, Ylm2 tables), the 3PCF bin edges and the scaling
+ * (-1)^(ell1 + ell2) / n_mode^2 / dV. */
+int hm_bin_xi_3pcf(hm_ctx *ctx, const hm_mesh *mesh, const double *d_f1, const double *d_f2, const double *rbin,
+                   int n_rbin, const double sn[2], int ell1, int m1, int ell2, int m2, double *xi,
+                   long long *n_mode);
+/* calcShotNoiseForBispectrum_ijk (field.hpp:1116-1160): d_xi = FFTW_BACKWARD
+ * of (f1 conj(f2) - sn S(k)) / W(k)^2 / V. */
+int hm_sn_field_ijk(hm_ctx *ctx, const hm_mesh *mesh, const double *d_f1, const double *d_f2, const double sn[2],
+                    double *d_xi);
+/* calcInverseFourierTransformForBispectrum (field.hpp:346-401): d_out =
+ * FFTW_BACKWARD of Y_red(ell, m)(k) d(k) / W(k) on the shell
+ * max(kmag - dk/2, 0) < |k| <= kmag + dk/2, divided by the shell's mode
+ * count, which is returned in n_mode (may be NULL).  d_out != d_delta. */
+int hm_shell_c2r(hm_ctx *ctx, const hm_mesh *mesh, const double *d_delta, double kmag, double dk, int ell, int m,
+                 double *d_out, long long *n_mode);
+/* calcInverseFourierTransformForThreePointFunction (field.hpp:403-445):
+ * d_out = FFTW_BACKWARD of j_ell_j(|k| rmag) Y_red(ell, m)(k) (d(k) / W(k)) / V,
+ * j from the 10^6-node spline of SphericalBesselClass (bessel.hpp:4-41). */
+int hm_bessel_c2r(hm_ctx *ctx, const hm_mesh *mesh, const double *d_delta, double rmag, int ell_j, int ell, int m,
+                  double *d_out);
+/* The bispectrum sums of bispec.hpp:1159-1168 for many pairs at once:
+ * out[p] = sum_x dV F_a F_b F3 with (a, b) = pairs[2p], pairs[2p + 1]
+ * indexing d_fields (host array of n_fields grid pointers).  out: [n_pairs][2]. */
+int hm_triple_sums(hm_ctx *ctx, const hm_mesh *mesh, const double *const *d_fields, int n_fields, const int *pairs,
+                   int n_pairs, const double *d_f3, double *out);
+/* The shot-noise sweep of bispec.hpp:1036-1056 for many (k1, k2) pairs:
+ * out[p] = i^(ell1 + ell2) dV sum_x j_ell1(k1[p] |r|) j_ell2(k2[p] |r|)
+ * xi(x) Y_red(ell1, m1)(r) Y_red(ell2, m2)(r), xi from hm_sn_field_ijk.
+ * out: [n_pairs][2]. */
+int hm_sn_bessel_sweep(hm_ctx *ctx, const hm_mesh *mesh, const double *d_xi, int ell1, int m1, int ell2, int m2,
+                       const double *k1, const double *k2, int n_pairs, double *out);
+/* The particle sums of the shot noise (field.hpp:1164-1240):
+ * out = sum_p w_p^power Y_red(ell, m)(los_p); d_los: n x 3 (may be NULL for
+ * ell = 0), d_w: n weights (NULL: 1).  power 2 gives calcShotNoiseFor-
+ * Powerspectrum's and the window's sums, power 3 calcShotNoiseForBispectrum's. */
+int hm_particle_sums(hm_ctx *ctx, const double *d_los, const double *d_w, long long n, int ell, int m, int power,
+                     double out[2]);
+/* calcSurveyVolume (field.hpp:449-503): the weighted randoms' field,
+ * I = sum_x rho(x)^2 dV, survey_volume = (sum w)^2 / I.  d_w: n weights or NULL. */
+int hm_survey_volume(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, const double *d_w, long long n,
+                     double *survey_volume);
+
+/* ---------------------------------------------------------------- */
 /* periodic-box estimators (main_box.cpp:171-228: pk, 2PCF, bk, 3PCF) */
 /* ---------------------------------------------------------------- */
 #define HM_MAX_ELLS 4

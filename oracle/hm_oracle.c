@@ -2346,3 +2346,115 @@ void or_sky_randoms(const double patch[6], unsigned long long seed, long long n,
         out[3 * i + 2] = r * sd;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* single seams (the pieces the drivers above compose), exported for    */
+/* the per-seam entry points of the library                             */
+/* ------------------------------------------------------------------ */
+
+/* ToolClass::storeReducedSphericalHarmonicsIn{Fourier,Configuration}Space
+ * (tools.hpp:70-115): config = 0 k-space, 1 r-space; ylm: 2*ng doubles. */
+void or_store_ylm(const or_param *p, int ell, int m, int config, double *ylm) {
+    if (config) store_ylm_config(p, ell, m, ylm);
+    else store_ylm_fourier(p, ell, m, ylm);
+}
+
+/* field.hpp:403-445 with the spline of j_ell_j (bessel.hpp:4-41) */
+int or_ifft_for_3pcf(const or_param *p, const double *density, double rmag, const double *ylm, int ell_j,
+                     double *out) {
+    or_bessel *sj = or_bessel_new(ell_j);
+    if (!sj) return -1;
+    ifft_for_3pcf(p, density, rmag, ylm, sj, out);
+    or_bessel_free(sj);
+    return 0;
+}
+
+/* field.hpp:979-1113 (p->ell1 + p->ell2 sets the sign, as the reference's param does) */
+int or_corr_for_3pcf(const or_param *p, const double *f1, const double *f2, const double *rbin, double sn_re,
+                     double sn_im, const double *ylm1, const double *ylm2, double *xi, int *n_mode) {
+    corr_for_3pcf(p, f1, f2, rbin, sn_re, sn_im, ylm1, ylm2, xi, n_mode);
+    return 0;
+}
+
+/* bispec.hpp:1036-1056 for one (kmag1, kmag2): sum_x j_l1(k1 r) j_l2(k2 r)
+ * xi Ylm1 Ylm2, times i^(l1 + l2) dV; ylm1 / ylm2 r-space tables. */
+int or_sn_bessel_sweep(const or_param *p, const double *xi, const double *ylm1, const double *ylm2, int ell1,
+                       int ell2, double kmag1, double kmag2, double *out) {
+    or_bessel *sj1 = or_bessel_new(ell1), *sj2 = or_bessel_new(ell2);
+    if (!sj1 || !sj2) {
+        or_bessel_free(sj1);
+        or_bessel_free(sj2);
+        return -1;
+    }
+    double sr = 0.0, si = 0.0, rvec[3], dr[3];
+    dr[0] = p->boxsize[0] / (double)(p->n_mesh[0]);
+    dr[1] = p->boxsize[1] / (double)(p->n_mesh[1]);
+    dr[2] = p->boxsize[2] / (double)(p->n_mesh[2]);
+    for (int i = 0; i < p->n_mesh[0]; i++)
+        for (int j = 0; j < p->n_mesh[1]; j++)
+            for (int k = 0; k < p->n_mesh[2]; k++) {
+                long long coord = or_coord(p, i, j, k);
+                kvec_of(p, i, j, k, dr, rvec);
+                double rmag = sqrt(rvec[0] * rvec[0] + rvec[1] * rvec[1] + rvec[2] * rvec[2]);
+                double j1 = or_bessel_eval(sj1, kmag1 * rmag);
+                double j2 = or_bessel_eval(sj2, kmag2 * rmag);
+                double ar = j1 * j2 * xi[2 * coord], ai = j1 * j2 * xi[2 * coord + 1], br, bi, cr, ci;
+                CMUL(ar, ai, ylm1[2 * coord], ylm1[2 * coord + 1], br, bi);
+                CMUL(br, bi, ylm2[2 * coord], ylm2[2 * coord + 1], cr, ci);
+                sr += cr;
+                si += ci;
+            }
+    double ir, ii;
+    ipow(ell1 + ell2, &ir, &ii);
+    double fac = p->volume / (double)(p->n_mesh_tot);
+    double tr = sr * ir - si * ii, ti = sr * ii + si * ir;
+    out[0] = tr * fac;
+    out[1] = ti * fac;
+    or_bessel_free(sj1);
+    or_bessel_free(sj2);
+    return 0;
+}
+
+/* bispec.hpp:1159-1168: sum_x fac f1 f2 f3, fac = V / N_grid */
+void or_triple_sum(const or_param *p, const double *f1, const double *f2, const double *f3, double *out) {
+    double fac = p->volume / (double)(p->n_mesh_tot), sr = 0.0, si = 0.0;
+    for (long long c = 0; c < p->n_mesh_tot; c++) {
+        double ar = fac * f1[2 * c], ai = fac * f1[2 * c + 1], br, bi, cr, ci;
+        CMUL(ar, ai, f2[2 * c], f2[2 * c + 1], br, bi);
+        CMUL(br, bi, f3[2 * c], f3[2 * c + 1], cr, ci);
+        sr += cr;
+        si += ci;
+    }
+    out[0] = sr;
+    out[1] = si;
+}
+
+/* field.hpp:1164-1240: sum_p pow(w_p, power) Y_red(ell, m)(los_p); w NULL: 1 */
+void or_particle_sum(const double *los, const double *w, long long n, int ell, int m, int power, double *out) {
+    double sr = 0.0, si = 0.0, y[2];
+    for (long long q = 0; q < n; q++) {
+        or_ylm(ell, m, los + 3 * q, y);
+        double f = pow(w ? w[q] : 1.0, power);
+        sr += f * y[0];
+        si += f * y[1];
+    }
+    out[0] = sr;
+    out[1] = si;
+}
+
+/* calcSurveyVolume (field.hpp:449-503): (sum w)^2 / sum_x rho_R(x)^2 dV */
+double or_survey_volume(const or_param *p, const double *pos, const double *w, long long n) {
+    double *weight = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+    for (long long q = 0; q < n; q++) {
+        weight[2 * q] = w ? w[q] : 1.0;
+        weight[2 * q + 1] = 0.0;
+    }
+    double *field = grid_alloc(p);
+    or_calc_field(p, pos, weight, n, field);
+    free(weight);
+    double dV = p->volume / (double)(p->n_mesh_tot), norm = 0.0, nw = 0.0;
+    for (long long i = 0; i < p->n_mesh_tot; i++) norm += field[2 * i] * field[2 * i] * dV;
+    for (long long q = 0; q < n; q++) nw += w ? w[q] : 1.0;
+    free(field);
+    return nw * nw / norm;
+}

@@ -159,6 +159,18 @@ int or_survey_zeta(const or_param *p, const double *posD, const double *losD, co
                    const double *posR, const double *losR, const double *wR, long long nR, double alpha, double vs,
                    const double *rbin, int window, double *zeta_save, double *sn_save, double *out);
 
+/* Single seams exported for the library's per-seam entry points. */
+void or_store_ylm(const or_param *p, int ell, int m, int config, double *ylm); /* tools.hpp:70-115 */
+int or_ifft_for_3pcf(const or_param *p, const double *density, double rmag, const double *ylm, int ell_j,
+                     double *out);                                               /* field.hpp:403-445 */
+int or_corr_for_3pcf(const or_param *p, const double *f1, const double *f2, const double *rbin, double sn_re,
+                     double sn_im, const double *ylm1, const double *ylm2, double *xi, int *n_mode); /* field.hpp:979-1113 */
+int or_sn_bessel_sweep(const or_param *p, const double *xi, const double *ylm1, const double *ylm2, int ell1,
+                       int ell2, double kmag1, double kmag2, double *out);     /* bispec.hpp:1036-1056 */
+void or_triple_sum(const or_param *p, const double *f1, const double *f2, const double *f3, double *out); /* bispec.hpp:1159-1168 */
+void or_particle_sum(const double *los, const double *w, long long n, int ell, int m, int power, double *out); /* field.hpp:1164-1240 */
+double or_survey_volume(const or_param *p, const double *pos, const double *w, long long n); /* field.hpp:449-503 */
+
 /* Interlacing (no counterpart in the reference): the half-cell-shifted
  * second grid combined in Fourier space, and P_l / xi_l from it with the
  * shot noise N W(k)^2. */

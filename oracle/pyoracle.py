@@ -99,6 +99,15 @@ def lib():
             "or_mock_sites": (None, [ct.c_longlong, ct.c_longlong, ct.c_ulonglong, ct.POINTER(ct.c_longlong)]),
             "or_mock_select_uniform": (None, [ct.c_longlong, ct.c_ulonglong, _D]),
             "or_sky_randoms": (None, [_D, ct.c_ulonglong, ct.c_longlong, _D]),
+            "or_shotnoise_bispectrum_ijk": (ct.c_int, [P, _D, _D, ct.c_double, ct.c_double, _D]),
+            "or_store_ylm": (None, [P, ct.c_int, ct.c_int, ct.c_int, _D]),
+            "or_ifft_for_3pcf": (ct.c_int, [P, _D, ct.c_double, _D, ct.c_int, _D]),
+            "or_corr_for_3pcf": (ct.c_int, [P, _D, _D, _D, ct.c_double, ct.c_double, _D, _D, _D,
+                                            ct.POINTER(ct.c_int)]),
+            "or_sn_bessel_sweep": (ct.c_int, [P, _D, _D, _D, ct.c_int, ct.c_int, ct.c_double, ct.c_double, _D]),
+            "or_triple_sum": (None, [P, _D, _D, _D, _D]),
+            "or_particle_sum": (None, [_D, _D, ct.c_longlong, ct.c_int, ct.c_int, ct.c_int, _D]),
+            "or_survey_volume": (ct.c_double, [P, _D, _D, ct.c_longlong]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -281,6 +290,100 @@ def ifft_for_bispectrum(p: Param, density, kmag, dk):
     out = np.zeros(shape(p) + (2,))
     n = lib().or_ifft_for_bispectrum(ct.byref(p), dp(v), float(kmag), float(dk), None, dp(out))
     return out[..., 0] + 1j * out[..., 1], n
+
+
+def inverse_fourier_transform(p: Param, field):
+    """field.hpp:328-344: / V, then the backward transform."""
+    a, v = _c(np.array(field, dtype=np.complex128, copy=True))
+    lib().or_inverse_fourier_transform(ct.byref(p), dp(v))
+    return a
+
+
+def store_ylm(p: Param, ell, m, config):
+    """tools.hpp:70-115: Y_red(ell, m) of every k (config=False) or r (config=True) of the mesh."""
+    out = np.zeros(shape(p) + (2,))
+    lib().or_store_ylm(ct.byref(p), int(ell), int(m), 1 if config else 0, dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def ifft_for_bispectrum_ylm(p: Param, density, kmag, dk, ell, m):
+    """field.hpp:346-401 with the k-space Y_red(ell, m) table: (field, n_mode)."""
+    a, v = _c(density)
+    y, yv = _c(store_ylm(p, ell, m, False))
+    out = np.zeros(shape(p) + (2,))
+    n = lib().or_ifft_for_bispectrum(ct.byref(p), dp(v), float(kmag), float(dk), dp(yv), dp(out))
+    return out[..., 0] + 1j * out[..., 1], n
+
+
+def ifft_for_3pcf(p: Param, density, rmag, ell_j, ell, m):
+    """field.hpp:403-445 with the k-space Y_red(ell, m) table and the j_ell_j spline."""
+    a, v = _c(density)
+    y, yv = _c(store_ylm(p, ell, m, False))
+    out = np.zeros(shape(p) + (2,))
+    lib().or_ifft_for_3pcf(ct.byref(p), dp(v), float(rmag), dp(yv), int(ell_j), dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def corr_for_3pcf(p: Param, f1, f2, rb, sn, ell1, m1, ell2, m2):
+    """field.hpp:979-1113 with r-space Y tables of (ell1, m1), (ell2, m2); p.ell1/p.ell2 set the sign."""
+    a1, v1 = _c(f1)
+    a2, v2 = _c(f2)
+    y1, w1 = _c(store_ylm(p, ell1, m1, True))
+    y2, w2 = _c(store_ylm(p, ell2, m2, True))
+    rb = np.ascontiguousarray(rb, dtype=np.float64)
+    xi = np.zeros(2 * rb.size)
+    nm = np.zeros(rb.size, dtype=np.int32)
+    p2 = Param.from_buffer_copy(p)
+    p2.ell1, p2.ell2, p2.n_rbin = ell1, ell2, rb.size
+    lib().or_corr_for_3pcf(ct.byref(p2), dp(v1), dp(v2), dp(rb), float(np.real(sn)), float(np.imag(sn)), dp(w1),
+                           dp(w2), dp(xi), nm.ctypes.data_as(ct.POINTER(ct.c_int)))
+    return xi[0::2] + 1j * xi[1::2], nm
+
+
+def shotnoise_bispectrum_ijk(p: Param, f1, f2, sn):
+    """field.hpp:1116-1160."""
+    a1, v1 = _c(f1)
+    a2, v2 = _c(f2)
+    out = np.zeros(shape(p) + (2,))
+    lib().or_shotnoise_bispectrum_ijk(ct.byref(p), dp(v1), dp(v2), float(np.real(sn)), float(np.imag(sn)), dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def sn_bessel_sweep(p: Param, xi, ell1, m1, ell2, m2, k1, k2):
+    """bispec.hpp:1036-1056 for one (k1, k2) with r-space Y tables; i^(l1+l2) dV applied."""
+    a, v = _c(xi)
+    y1, w1 = _c(store_ylm(p, ell1, m1, True))
+    y2, w2 = _c(store_ylm(p, ell2, m2, True))
+    out = np.zeros(2)
+    lib().or_sn_bessel_sweep(ct.byref(p), dp(v), dp(w1), dp(w2), int(ell1), int(ell2), float(k1), float(k2), dp(out))
+    return complex(out[0], out[1])
+
+
+def triple_sum(p: Param, f1, f2, f3):
+    """bispec.hpp:1159-1168."""
+    a1, v1 = _c(f1)
+    a2, v2 = _c(f2)
+    a3, v3 = _c(f3)
+    out = np.zeros(2)
+    lib().or_triple_sum(ct.byref(p), dp(v1), dp(v2), dp(v3), dp(out))
+    return complex(out[0], out[1])
+
+
+def particle_sum(los, w, ell, m, power):
+    """field.hpp:1164-1240."""
+    los = np.ascontiguousarray(los, dtype=np.float64).reshape(-1, 3)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    out = np.zeros(2)
+    lib().or_particle_sum(dp(los), dp(w) if w is not None else None, los.shape[0], int(ell), int(m), int(power),
+                          dp(out))
+    return complex(out[0], out[1])
+
+
+def survey_volume(p: Param, pos, w=None):
+    """calcSurveyVolume (field.hpp:449-503)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    return lib().or_survey_volume(ct.byref(p), dp(pos), dp(w) if w is not None else None, pos.shape[0])
 
 
 def box_pk(p: Param, pos):
