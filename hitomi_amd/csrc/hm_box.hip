@@ -207,12 +207,29 @@ template <typename T> struct XiSrc {
     int sn_w2 = 0;              // 1: shot-noise shape W(k)^2 (interlaced field)
     const double* sn_ptr = nullptr;  // set: sn and dc read from device memory
     const double* dc_ptr = nullptr;
+    // per x index i, {1/W_x^2(i), 1/W_x^2(-i), S_x(i)/W_x^2(i), S_x(-i)/W_x^2(-i)} (the x factors
+    // of both modes of the pair (k, -k), one 32-byte load; required)
+    const double* xq = nullptr;
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
     __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
 #pragma clang fp contract(off)
         const int i = p, j = b / nzh, k = b % nzh;
         const C<T> d = dk[addr];
         const double re = (double)d.x, im = (double)d.y;
+        if (!dk2 && !sn_w2 && k != 0 && 2 * k != n[2]) {
+            // inside 0 < k_z < n_z/2 the spectrum at -k is the conjugate, so with p2 = |d|^2 the two
+            // sides combine to Re = (p2 (Q1 + Q2) - sn (S1 Q1 + S2 Q2)) / 2V (even part) and
+            // Im = -(p2 (Q1 - Q2) - sn (S1 Q1 - S2 Q2)) / 2V (odd part); Q, S separable per axis
+            const int jj = j ? n[1] - j : 0, kk = n[2] - k;
+            const double4 q = *reinterpret_cast<const double4*>(xq + 4 * (long long)i);
+            const double qa = t.iw[1][j] * t.iw[2][k], qb = t.iw[1][jj] * t.iw[2][kk];
+            const double sa = (t.sf[1][j] * t.sf[2][k]) * qa, sb = (t.sf[1][jj] * t.sf[2][kk]) * qb;
+            const double p2 = re * re + im * im;
+            const double snv = sn_ptr ? *sn_ptr : sn;
+            const double Q1 = q.x * qa, Q2 = q.y * qb, SQ1 = q.z * sa, SQ2 = q.w * sb;
+            if (odd) return mk<T>((T)0, (T)(-(0.5 * inv_v) * (p2 * (Q1 - Q2) - snv * (SQ1 - SQ2))));
+            return mk<T>((T)((0.5 * inv_v) * (p2 * (Q1 + Q2) - snv * (SQ1 + SQ2))), (T)0);
+        }
         double p2, pim = 0.0;
         if (dk2) {
             const C<T> e = dk2[addr];
@@ -1465,6 +1482,13 @@ static AxisTabs axis_tabs(const BoxPlanImpl* p) {
 }
 
 
+// XiSrc's per-x table, after the four axis tables (32-byte aligned: the buffer
+// is, and the offset is 4 (n0 + n1 + n2) doubles)
+static const double* xi_xq(const BoxPlanImpl* p) {
+    const long long ntot = (long long)p->mesh.n[0] + p->mesh.n[1] + p->mesh.n[2];
+    return reinterpret_cast<const double*>(p->tab.p) + 4 * ntot;
+}
+
 void plan_destroy(BoxPlanImpl* p);
 
 BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
@@ -1738,6 +1762,21 @@ BoxPlanImpl* plan_create(hm_ctx* ctx, const hm_box_config* cfg) {
     // device tables
     std::vector<double> tabs;
     host_axis_tables(m, tabs, p->tab_off);
+    {
+        // the x factors of XiSrc's pair (k, -k), 4 doubles per x index, after the axis tables
+        const long long ntot = (long long)m.n[0] + m.n[1] + m.n[2];
+        const double* iw0 = tabs.data() + 3 * ntot + p->tab_off[0];
+        const double* sf0 = tabs.data() + 2 * ntot + p->tab_off[0];
+        std::vector<double> xq(4 * (size_t)m.n[0]);
+        for (int i = 0; i < m.n[0]; i++) {
+            const int ii = i ? m.n[0] - i : 0;
+            xq[4 * i] = iw0[i];
+            xq[4 * i + 1] = iw0[ii];
+            xq[4 * i + 2] = sf0[i] * iw0[i];
+            xq[4 * i + 3] = sf0[ii] * iw0[ii];
+        }
+        tabs.insert(tabs.end(), xq.begin(), xq.end());
+    }
     if (alloc(ctx, p, p->tab, sizeof(double) * tabs.size())) { plan_destroy(p); return nullptr; }
     if (alloc(ctx, p, p->f2b_d, sizeof(int2) * FINE_N)) { plan_destroy(p); return nullptr; }
     if (alloc(ctx, p, p->lu_d, sizeof(double) * 2 * p->nk)) { plan_destroy(p); return nullptr; }
@@ -2294,6 +2333,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
         xs.nzh = (int)nzh;
         xs.t = tabs;
+        xs.xq = xi_xq(p);
         xs.wp = wp;
         xs.sn = sn2;
         xs.sn_ptr = dyn ? dyn + DYN_SN2 : nullptr;
@@ -2498,6 +2538,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
         xs.nzh = (int)nzh;
         xs.t = tabs;
+        xs.xq = xi_xq(p);
         xs.wp = wp;
         xs.sn = sn3;
         xs.inv_v = 1.0 / m.volume();
@@ -2804,6 +2845,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
         xs.nzh = (int)nzh;
         xs.t = tabs;
+        xs.xq = xi_xq(p);
         xs.wp = wp;
         xs.sn = sn3;
         xs.inv_v = 1.0 / m.volume();

@@ -2,7 +2,7 @@
 //
 // A workgroup (256 lanes = 4 wave64) loads a tile of B lines of length L
 // into LDS, runs the Stockham autosort stages for a compile-time radix plan
-// (radices 2, 3, 4, 5, 8), and stores the tile back.  Two tile shapes:
+// (radices 2, 3, 4, 5, 6, 8, 9), and stores the tile back.  Two tile shapes:
 //   * columns: lines whose points are PS elements apart while B consecutive
 //     lines are contiguous in memory, so each point of the tile is one
 //     coalesced run of B elements (B*8 = 128 B for FP32, B*16 for FP64);
@@ -78,6 +78,29 @@ template <typename T, int DIR> struct Dft<T, 3, DIR> {
         a[0] = cadd<T>(a[0], t);
         a[1] = cadd<T>(m, r);
         a[2] = csub<T>(m, r);
+    }
+};
+
+// Radix 6 as two 3-point DFTs of the even and odd points, y_u = E_u + w^u O_u
+// and y_{u+3} = E_u - w^u O_u (w = exp(DIR 2 pi i / 6)): one Stockham stage
+// (one LDS round trip, five twiddles) where a radix-3 and a radix-2 stage
+// took two (seven twiddles).
+template <typename T, int DIR> struct Dft<T, 6, DIR> {
+    __device__ static inline void run(C<T>* a) {
+        const T s3 = (T)0.86602540378443864676;
+        C<T> e[3] = {a[0], a[2], a[4]};
+        C<T> o[3] = {a[1], a[3], a[5]};
+        Dft<T, 3, DIR>::run(e);
+        Dft<T, 3, DIR>::run(o);
+        // w^1 = 1/2 + DIR i s3, w^2 = -1/2 + DIR i s3
+        const C<T> o1 = mk<T>((T)0.5 * o[1].x - (T)DIR * s3 * o[1].y, (T)0.5 * o[1].y + (T)DIR * s3 * o[1].x);
+        const C<T> o2 = mk<T>((T)-0.5 * o[2].x - (T)DIR * s3 * o[2].y, (T)-0.5 * o[2].y + (T)DIR * s3 * o[2].x);
+        a[0] = cadd<T>(e[0], o[0]);
+        a[3] = csub<T>(e[0], o[0]);
+        a[1] = cadd<T>(e[1], o1);
+        a[4] = csub<T>(e[1], o1);
+        a[2] = cadd<T>(e[2], o2);
+        a[5] = csub<T>(e[2], o2);
     }
 };
 
@@ -238,7 +261,7 @@ __device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw, i
     if constexpr (L == 8) lds_stockham<T, DIR, L, B, NT, SW, 1, 8>(sm, tw, tid);
     else if constexpr (L == 16) lds_stockham<T, DIR, L, B, NT, SW, 1, 4, 4>(sm, tw, tid);
     else if constexpr (L == 32) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4>(sm, tw, tid);
-    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3>(sm, tw, tid);
+    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6>(sm, tw, tid);
     else if constexpr (L == 64) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8>(sm, tw, tid);
     else if constexpr (L == 80) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 5>(sm, tw, tid);
     else if constexpr (L == 96) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3>(sm, tw, tid);
@@ -246,9 +269,9 @@ __device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw, i
     else if constexpr (L == 160) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 5>(sm, tw, tid);
     else if constexpr (L == 192) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3>(sm, tw, tid);
     else if constexpr (L == 256) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 4>(sm, tw, tid);
-    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 6>(sm, tw, tid);
     else if constexpr (L == 320) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 5>(sm, tw, tid);
-    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 2, 3>(sm, tw, tid);
+    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 6>(sm, tw, tid);
     else if constexpr (L == 480) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3, 5>(sm, tw, tid);
     else if constexpr (L == 512) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 8>(sm, tw, tid);
     else if constexpr (L == 576) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3, 3>(sm, tw, tid);
@@ -259,18 +282,18 @@ __device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw, i
     else if constexpr (L == 40) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 5>(sm, tw, tid);
     else if constexpr (L == 72) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3>(sm, tw, tid);
     else if constexpr (L == 120) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 5>(sm, tw, tid);
-    else if constexpr (L == 144) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 144) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 3>(sm, tw, tid);
     else if constexpr (L == 200) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 5, 5>(sm, tw, tid);
     else if constexpr (L == 216) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3, 3>(sm, tw, tid);
-    else if constexpr (L == 240) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 240) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 5>(sm, tw, tid);
     else if constexpr (L == 360) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3, 5>(sm, tw, tid);
     else if constexpr (L == 400) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 5, 5>(sm, tw, tid);
-    else if constexpr (L == 432) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 3, 3>(sm, tw, tid);
+    else if constexpr (L == 432) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 3, 3>(sm, tw, tid);
     else if constexpr (L == 600) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 5, 5>(sm, tw, tid);
-    else if constexpr (L == 720) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 3, 3, 5>(sm, tw, tid);
+    else if constexpr (L == 720) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 3, 5>(sm, tw, tid);
     else if constexpr (L == 800) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 5, 5>(sm, tw, tid);
     else if constexpr (L == 960) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3, 5>(sm, tw, tid);
-    else if constexpr (L == 270) lds_stockham<T, DIR, L, B, NT, SW, 1, 9, 5, 3, 2>(sm, tw, tid);
+    else if constexpr (L == 270) lds_stockham<T, DIR, L, B, NT, SW, 1, 9, 5, 6>(sm, tw, tid);
     else static_assert(L == 8, "unsupported FFT length");
 }
 
