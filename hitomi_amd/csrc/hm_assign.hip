@@ -1,6 +1,7 @@
 // hm_assign.hip -- mass assignment kernels (field.hpp:47-184).
 #include "hm_assign.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace hm {
@@ -330,16 +331,286 @@ __global__ __launch_bounds__(256) void k_assign_border(const double* __restrict_
         scatter_particle<S>(pos, w, border[1 + i], g, scale, gre, gim);
 }
 
+// ---------------------------------------------------------------------------
+// Two-level counting sort by key (x, y, z/64), without global atomics.
+// Level 1: chunks of S2_C1 particles count their x-slab buckets (XB planes,
+// LK = XB * n1 * nzc keys each) in LDS; an exclusive scan of the chunk counts
+// (bucket-major) gives every chunk its run in each bucket, and the chunk
+// writes its particles' (index, local key) there, ranked by LDS atomics.
+// Level 2: chunks of S2_C2 particles of one bucket count their local keys in
+// LDS; one exclusive scan over (bucket, key, chunk) gives every particle its
+// final place, and the chunk moves positions and weights there.  Border
+// particles (stencil leaving the mesh) go to the end of level 1 and form the
+// border list.  Within a key the order depends on LDS atomics; the cell sums
+// do not (64-bit integer sums).
+// ---------------------------------------------------------------------------
+constexpr int S2_C1 = 8192;   // particles per level-1 chunk
+constexpr int S2_C2 = 16384;  // particles per level-2 chunk
+constexpr int S2_LK = 16384;  // most local keys of a bucket (int32 counters in 64 KB of LDS)
+
+struct TwoLevel {
+    int XB, nb, LK;
+    long long nch1;  // level-1 chunks
+    long long nch2;  // level-2 chunks at most: nch1 * S2_C1 / S2_C2 + nb
+};
+
+static bool two_level_geom(const AssignGeom& g, long long np, TwoLevel& t) {
+    const long long rowkeys = (long long)g.n[1] * chunks_z(g);
+    if (rowkeys > S2_LK || np <= 0) return false;
+    t.XB = (int)std::min<long long>(S2_LK / rowkeys, g.n[0]);
+    t.nb = (g.n[0] + t.XB - 1) / t.XB;
+    t.LK = (int)(t.XB * rowkeys);
+    t.nch1 = (np + S2_C1 - 1) / S2_C1;
+    t.nch2 = (np + S2_C2 - 1) / S2_C2 + t.nb;
+    return true;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_s2_hist1(const double* __restrict__ pos, long long np, AssignGeom g,
+                                                 TwoLevel tl, int* __restrict__ key, int* __restrict__ h1) {
+    extern __shared__ int hist[];  // nb + 1 (the last: border particles)
+    for (int e = threadIdx.x; e <= tl.nb; e += 256) hist[e] = 0;
+    __syncthreads();
+    const long long c = blockIdx.x, q0 = c * S2_C1, q1 = (q0 + S2_C1 < np) ? q0 + S2_C1 : np;
+    for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
+        long long K;
+        const bool in = first_cell<S>(pos, q, g, &K);
+        key[q] = in ? (int)K : -1;
+        atomicAdd(&hist[in ? (int)(K / tl.LK) : tl.nb], 1);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e <= tl.nb; e += 256) h1[(long long)e * tl.nch1 + c] = hist[e];
+}
+
+// Level-1 placement: each chunk's particles in runs per bucket (about
+// S2_C1 / buckets records each), as one 32-byte (x, y, z, w_re) and one
+// 16-byte (index, local key, w_im) record; level 2 then reads them in order.
+__global__ __launch_bounds__(256) void k_s2_scatter1(const int* __restrict__ key, long long np, TwoLevel tl,
+                                                    const int* __restrict__ off1, const double* __restrict__ pos,
+                                                    const double* __restrict__ w, double* __restrict__ tpos,
+                                                    int* __restrict__ tw) {
+    extern __shared__ int cur[];  // nb + 1
+    const long long c = blockIdx.x, q0 = c * S2_C1, q1 = (q0 + S2_C1 < np) ? q0 + S2_C1 : np;
+    for (int e = threadIdx.x; e <= tl.nb; e += 256) cur[e] = off1[(long long)e * tl.nch1 + c];
+    __syncthreads();
+    for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
+        const int K = key[q];
+        const int b = K < 0 ? tl.nb : K / tl.LK;
+        const long long d = atomicAdd(&cur[b], 1);
+        // one 32-byte (x, y, z, w_re) and one 16-byte (q, local key, w_im) record per particle
+        const double wr = w ? w[2 * q] : 0.0, wi = w ? w[2 * q + 1] : 0.0;
+        reinterpret_cast<double4*>(tpos)[d] = make_double4(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2], wr);
+        const long long wib = __double_as_longlong(wi);
+        reinterpret_cast<int4*>(tw)[d] = make_int4((int)q, K < 0 ? -1 : K - b * tl.LK, (int)(wib & 0xffffffffLL),
+                                                   (int)(wib >> 32));
+    }
+}
+
+// Level-2 chunk starts per bucket (one workgroup): cst[b] = sum over b' < b of
+// ceil(n_b' / S2_C2), cst[nb] = their total.
+__global__ __launch_bounds__(1024) void k_s2_chunks(const int* __restrict__ off1, TwoLevel tl, int* __restrict__ cst) {
+    __shared__ int sm[1024];
+    int carry = 0;
+    for (int c0 = 0; c0 < tl.nb; c0 += 1024) {
+        const int b = c0 + threadIdx.x;
+        int v = 0;
+        if (b < tl.nb) {
+            const int n = off1[(long long)(b + 1) * tl.nch1] - off1[(long long)b * tl.nch1];
+            v = (n + S2_C2 - 1) / S2_C2;
+        }
+        sm[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int t = (threadIdx.x >= off) ? sm[threadIdx.x - off] : 0;
+            __syncthreads();
+            sm[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (b < tl.nb) cst[b] = carry + sm[threadIdx.x] - v;
+        carry += sm[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cst[tl.nb] = carry;
+}
+
+// The bucket of level-2 chunk w: the largest b with cst[b] <= w.
+__device__ __forceinline__ int s2_bucket(const int* __restrict__ cst, int nb, int w) {
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cst[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Level-2 key counts, layout per bucket [LK][chunks] (key-major) at LK * cst[b];
+// workgroups past the last chunk zero the unused tail of h2 instead.
+__global__ __launch_bounds__(256) void k_s2_hist2(const int* __restrict__ off1, const int* __restrict__ cst,
+                                                 TwoLevel tl, const int* __restrict__ tk, int* __restrict__ h2) {
+    // tk: the level-1 16-byte records (index, local key, w_im)
+    extern __shared__ int hist[];  // LK
+    const int w = blockIdx.x;
+    const int total = cst[tl.nb];
+    if (w >= total) {
+        for (int e = threadIdx.x; e < tl.LK; e += 256) h2[(long long)w * tl.LK + e] = 0;
+        return;
+    }
+    const int b = s2_bucket(cst, tl.nb, w);
+    const int ch = cst[b + 1] - cst[b], c = w - cst[b];
+    const long long bs = off1[(long long)b * tl.nch1], be = off1[(long long)(b + 1) * tl.nch1];
+    const long long p0 = bs + (long long)c * S2_C2, p1 = (p0 + S2_C2 < be) ? p0 + S2_C2 : be;
+    for (int e = threadIdx.x; e < tl.LK; e += 256) hist[e] = 0;
+    __syncthreads();
+    for (long long p = p0 + threadIdx.x; p < p1; p += 256) atomicAdd(&hist[tk[4 * p + 1]], 1);
+    __syncthreads();
+    int* out = h2 + (long long)tl.LK * cst[b] + c;
+    for (int e = threadIdx.x; e < tl.LK; e += 256) out[(long long)e * ch] = hist[e];
+}
+
+// start[K] of every key (the first place of key K; n_interior at K = nk).
+__global__ __launch_bounds__(256) void k_s2_start(const int* __restrict__ off1, const int* __restrict__ cst,
+                                                 const int* __restrict__ off2, TwoLevel tl, long long nk,
+                                                 int* __restrict__ start) {
+    const long long K = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (K > nk) return;
+    if (K == nk) {
+        start[nk] = off1[(long long)tl.nb * tl.nch1];
+        return;
+    }
+    const int b = (int)(K / tl.LK), lk = (int)(K - (long long)b * tl.LK);
+    const int ch = cst[b + 1] - cst[b];
+    start[K] = ch > 0 ? off2[(long long)tl.LK * cst[b] + (long long)lk * ch] : off1[(long long)b * tl.nch1];
+}
+
+// Level-2 placement: positions and weights (read in bucket order) to their
+// places in key order, all within the bucket's range; slot[p] = place of the
+// particle at level-1 position p (a later move of weights reads the particle
+// index from the level-1 record).
+__global__ __launch_bounds__(256) void k_s2_scatter2(const int* __restrict__ off1, const int* __restrict__ cst,
+                                                    const int* __restrict__ off2, TwoLevel tl,
+                                                    const int* __restrict__ tk, const double* __restrict__ pos,
+                                                    bool weights, int* __restrict__ slot, double* __restrict__ spos,
+                                                    double* __restrict__ sw) {
+    extern __shared__ int cur[];  // LK
+    const int wg = blockIdx.x;
+    if (wg >= cst[tl.nb]) return;
+    const int b = s2_bucket(cst, tl.nb, wg);
+    const int ch = cst[b + 1] - cst[b], c = wg - cst[b];
+    const long long bs = off1[(long long)b * tl.nch1], be = off1[(long long)(b + 1) * tl.nch1];
+    const long long p0 = bs + (long long)c * S2_C2, p1 = (p0 + S2_C2 < be) ? p0 + S2_C2 : be;
+    const int* o = off2 + (long long)tl.LK * cst[b] + c;
+    for (int e = threadIdx.x; e < tl.LK; e += 256) cur[e] = o[(long long)e * ch];
+    __syncthreads();
+    for (long long p = p0 + threadIdx.x; p < p1; p += 256) {
+        const int4 m = reinterpret_cast<const int4*>(tk)[p];
+        const double4 a = reinterpret_cast<const double4*>(pos)[p];
+        const long long d = atomicAdd(&cur[m.y], 1);
+        slot[p] = (int)d;
+        spos[3 * d] = a.x;
+        spos[3 * d + 1] = a.y;
+        spos[3 * d + 2] = a.z;
+        if (weights) {
+            sw[2 * d] = a.w;
+            sw[2 * d + 1] = __longlong_as_double(((long long)m.w << 32) | (unsigned)m.z);
+        }
+    }
+}
+
+// The border particles (the tail of level 1) to the border list; border[0]
+// their count, and the interior count (the level-2 positions in use) after
+// the list at border[np + 1].
+__global__ __launch_bounds__(256) void k_s2_border(const int* __restrict__ off1, TwoLevel tl, long long np,
+                                                  const int* __restrict__ tq, int* __restrict__ border) {
+    const long long nint = off1[(long long)tl.nb * tl.nch1];
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) {
+        border[0] = (int)(np - nint);
+        border[np + 1] = (int)nint;
+    }
+    if (i < np - nint) border[1 + i] = tq[4 * (nint + i)];
+}
+
+// A later assignment of the same positions after a two-level sort: weights to
+// the places of their particles (the reads follow level-1 order).
+__global__ __launch_bounds__(256) void k_s2_move_w(const double* __restrict__ w, long long np,
+                                                  const int* __restrict__ tq, const int* __restrict__ slot,
+                                                  const int* __restrict__ border, double* __restrict__ sw) {
+    const long long nint = border[np + 1];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nint) return;
+    const long long q = tq[4 * p], d = slot[p];
+    sw[2 * d] = w[2 * q];
+    sw[2 * d + 1] = w[2 * q + 1];
+}
+
+static int scan_inplace(hm_ctx* ctx, const char* name, int* a, long long n, int* bsum) {
+    const unsigned nb = (unsigned)((n + SCAN_BLK - 1) / SCAN_BLK);
+    return ctx->launch(name, [&] {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, a, n, bsum);
+        hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(1024), 0, ctx->stream, bsum, (int)nb);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_NT), 0, ctx->stream, a, n, bsum);
+    });
+}
+
+template <int S>
+static int sort_two_level(hm_ctx* ctx, const AssignGeom& g, const TwoLevel& tl, const double* d_pos,
+                          const double* d_w, long long np, SortWork& w) {
+    int* key = (int*)w.key.p;
+    int* h1 = (int*)w.h1.p;
+    int* h2 = (int*)w.h2.p;
+    int* cst = (int*)w.cst.p;
+    int* bsum = (int*)w.bsum.p;
+    const size_t lds1 = sizeof(int) * (size_t)(tl.nb + 1), lds2 = sizeof(int) * (size_t)tl.LK;
+    const long long nk = n_keys(g);
+    if (ctx->launch("assign_count", [&] {
+            hipLaunchKernelGGL(k_s2_hist1<S>, dim3((unsigned)tl.nch1), dim3(256), lds1, ctx->stream, d_pos, np, g, tl,
+                               key, h1);
+        }))
+        return -1;
+    if (scan_inplace(ctx, "assign_scan", h1, (long long)(tl.nb + 1) * tl.nch1, bsum)) return -1;
+    if (ctx->launch("assign_bucket", [&] {
+            hipLaunchKernelGGL(k_s2_scatter1, dim3((unsigned)tl.nch1), dim3(256), lds1, ctx->stream, key, np, tl, h1,
+                               d_pos, d_w, (double*)w.tpos.p, (int*)w.tw.p);
+            hipLaunchKernelGGL(k_s2_chunks, dim3(1), dim3(1024), 0, ctx->stream, h1, tl, cst);
+            hipLaunchKernelGGL(k_s2_hist2, dim3((unsigned)tl.nch2), dim3(256), lds2, ctx->stream, h1, cst, tl,
+                               (const int*)w.tw.p, h2);
+        }))
+        return -1;
+    if (scan_inplace(ctx, "assign_scan", h2, (long long)tl.LK * tl.nch2, bsum)) return -1;
+    return ctx->launch("assign_scatter", [&] {
+        hipLaunchKernelGGL(k_s2_start, dim3((unsigned)((nk + 1 + 255) / 256)), dim3(256), 0, ctx->stream, h1, cst, h2,
+                           tl, nk, (int*)w.start.p);
+        hipLaunchKernelGGL(k_s2_scatter2, dim3((unsigned)tl.nch2), dim3(256), lds2, ctx->stream, h1, cst, h2, tl,
+                           (const int*)w.tw.p, (const double*)w.tpos.p, d_w != nullptr, (int*)w.slot.p,
+                           (double*)w.spos.p, (double*)w.sw.p);
+        hipLaunchKernelGGL(k_s2_border, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, h1, tl, np,
+                           (const int*)w.tw.p, (int*)w.border.p);
+    });
+}
+
 int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights) {
     w.sig.valid = false;  // the buffers may move: the next assignment sorts afresh
     const long long nk = n_keys(g);
-    const long long nb = (nk + 1 + SCAN_BLK - 1) / SCAN_BLK;
+    long long nscan = nk + 1;
+    TwoLevel tl;
+    const bool two = two_level_geom(g, np, tl);
+    if (two) nscan = std::max<long long>({nscan, (long long)(tl.nb + 1) * tl.nch1, (long long)tl.LK * tl.nch2});
+    const long long nb = (nscan + SCAN_BLK - 1) / SCAN_BLK;
     if (dev_alloc(ctx, w.start, sizeof(int) * (size_t)(nk + 1))) return -1;
     if (dev_alloc(ctx, w.slot, sizeof(int) * (size_t)(np > 0 ? np : 1))) return -1;
     if (dev_alloc(ctx, w.spos, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1))) return -1;
     if (weights && dev_alloc(ctx, w.sw, sizeof(double) * 2 * (size_t)(np > 0 ? np : 1))) return -1;
-    if (dev_alloc(ctx, w.border, sizeof(int) * (size_t)(np + 1))) return -1;
+    if (dev_alloc(ctx, w.border, sizeof(int) * (size_t)(np + 2))) return -1;
     if (dev_alloc(ctx, w.bsum, sizeof(int) * (size_t)nb)) return -1;
+    if (two) {
+        if (dev_alloc(ctx, w.key, sizeof(int) * (size_t)np)) return -1;
+        if (dev_alloc(ctx, w.tpos, 32 * (size_t)np)) return -1;
+        if (dev_alloc(ctx, w.tw, 16 * (size_t)np)) return -1;
+        if (dev_alloc(ctx, w.h1, sizeof(int) * (size_t)(tl.nb + 1) * (size_t)tl.nch1)) return -1;
+        if (dev_alloc(ctx, w.h2, sizeof(int) * (size_t)tl.LK * (size_t)tl.nch2)) return -1;
+        if (dev_alloc(ctx, w.cst, sizeof(int) * (size_t)(tl.nb + 1))) return -1;
+    }
     return 0;
 }
 
@@ -369,6 +640,12 @@ void sort_work_free(SortWork& w) {
     dev_free(w.sw);
     dev_free(w.border);
     dev_free(w.bsum);
+    dev_free(w.key);
+    dev_free(w.tpos);
+    dev_free(w.tw);
+    dev_free(w.h1);
+    dev_free(w.h2);
+    dev_free(w.cst);
 }
 
 template <int S>
@@ -385,7 +662,16 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
     const long long n = n_keys(g) + 1;
     const unsigned nb = (unsigned)((n + SCAN_BLK - 1) / SCAN_BLK);
     const unsigned pb = (unsigned)((np + 255) / 256);
-    if (!reuse) {
+    TwoLevel tl;
+    const bool two = !reuse && two_level_geom(g, np, tl) && w.key.p && w.tpos.bytes >= 32 * (size_t)np &&
+                     w.tw.bytes >= 16 * (size_t)np && w.h2.bytes >= sizeof(int) * (size_t)tl.LK * tl.nch2 &&
+                     w.h1.bytes >= sizeof(int) * (size_t)(tl.nb + 1) * tl.nch1 &&
+                     w.bsum.bytes >= sizeof(int) * (size_t)(((long long)tl.LK * tl.nch2 + SCAN_BLK - 1) / SCAN_BLK);
+    if (two) {
+        if (sort_two_level<S>(ctx, g, tl, d_pos, d_w, np, w)) return -1;
+        w.two_level = true;
+    } else if (!reuse) {
+        w.two_level = false;
         if (fill_zero(ctx, start, sizeof(int) * (size_t)n)) return -1;
         if (fill_zero(ctx, border, sizeof(int))) return -1;
         if (np > 0 && ctx->launch("assign_count", [&] {
@@ -400,13 +686,17 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
             }))
             return -1;
     }
-    if (np > 0 && !reuse && ctx->launch("assign_scatter", [&] {
+    if (np > 0 && !reuse && !two && ctx->launch("assign_scatter", [&] {
             hipLaunchKernelGGL(k_sort_scatter<S>, dim3(pb), dim3(256), 0, ctx->stream, d_pos, d_w, np, g, start, slot,
                                spos, sw);
         }))
         return -1;
     if (np > 0 && reuse && sw && ctx->launch("assign_scatter", [&] {
-            hipLaunchKernelGGL(k_sort_move_w, dim3(pb), dim3(256), 0, ctx->stream, d_w, np, (const int*)slot, sw);
+            if (w.two_level)
+                hipLaunchKernelGGL(k_s2_move_w, dim3(pb), dim3(256), 0, ctx->stream, d_w, np, (const int*)w.tw.p,
+                                   (const int*)slot, (const int*)border, sw);
+            else
+                hipLaunchKernelGGL(k_sort_move_w, dim3(pb), dim3(256), 0, ctx->stream, d_w, np, (const int*)slot, sw);
         }))
         return -1;
     const dim3 gg((unsigned)((g.n[2] + GZ - 1) / GZ), (unsigned)((g.n[1] + GY - 1) / GY),

@@ -106,6 +106,15 @@ struct SortWork {
     DevBuf sw;      // double [2 np] weights in cell order
     DevBuf border;  // int32 [np + 1]: border count, then border particle indices
     DevBuf bsum;    // int32 scan block sums
+    // two-level counting sort (no global atomics): per-particle keys, the particles in bucket order
+    // (32-byte (x, y, z, w_re) records in tpos, 16-byte (index, local key, w_im) records in tw), the
+    // chunk histograms of both levels and the per-bucket chunk starts; left empty (the one-level sort
+    // runs instead) by sort_work_carve and for meshes whose rows hold more keys than one bucket's LDS
+    // histogram
+    DevBuf key, tpos, tw, h1, h2, cst;
+    // the last full sort was the two-level one: slot[p] is the place of the particle at level-1
+    // position p (the index in its tw record), for p below the interior count
+    bool two_level = false;
     // what the last full sort was made for: a reuse must name the same positions, count, mesh and weights
     struct Sig {
         bool valid = false;
@@ -116,7 +125,10 @@ struct SortWork {
         int assign = -1;
         bool weights = false;
     } sig;
-    size_t bytes() const { return start.bytes + slot.bytes + spos.bytes + sw.bytes + border.bytes + bsum.bytes; }
+    size_t bytes() const {
+        return start.bytes + slot.bytes + spos.bytes + sw.bytes + border.bytes + bsum.bytes + key.bytes + tpos.bytes +
+               tw.bytes + h1.bytes + h2.bytes + cst.bytes;
+    }
 };
 int sort_work_alloc(hm_ctx* ctx, SortWork& w, const AssignGeom& g, long long np, bool weights);
 void sort_work_free(SortWork& w);

@@ -31,7 +31,7 @@ import hitomi_amd as hm  # noqa: E402
 from hitomi_amd import mocks  # noqa: E402
 
 KERNELS = ["survey_g", "survey_wsums", "survey_minpos", "survey_shift", "survey_weights", "survey_sqsum", "survey_to_complex",
-           "survey_sn", "survey_reduce", "survey_fft_z", "survey_fft_y", "survey_fft_x", "assign_count", "assign_scan", "assign_scatter", "assign_gather",
+           "survey_sn", "survey_reduce", "survey_fft_z", "survey_fft_y", "survey_fft_x", "assign_count", "assign_scan", "assign_bucket", "assign_scatter", "assign_gather",
            "assign_border", "fill_zero", "fft_c2c_z", "fft_c2c_y", "fft_c2c_x", "bin_survey", "bin_reduce"]
 
 
