@@ -86,6 +86,8 @@ def algorithmic_bytes(name, n, M, s, nk, npart, elem=8, ws=None):
         "fft_r2c_y": 2 * half,
         "fft_r2c_x": 2 * half,
         "xi_pass_x": 2 * half,
+        "fft_x_xi": 3 * half,                         # the spectrum in once; it and xi's x pass out
+        "xi_planes_x": 2 * 2 * n * n * elem,          # the planes k_z = 0, n_z/2 of xi's x pass
         "xi_pass_y": 2 * half,
         "xi_pass_z": n_groups(n) * ((n // 2 + 1) * elem + n * (elem // 2)),  # grouped rows, c2r
         # triple-product fields: shell j on its own cube of width ws[j], one delta/W of support W3

@@ -23,8 +23,8 @@ MAX_ELLS = 4
 
 # Launch names the box pipeline reports to hm_timing_get().
 BOX_KERNELS = ("fill_zero", "assign_count", "assign_scan", "assign_bucket", "assign_scatter",
-               "assign_gather", "assign_border", "fft_r2c_z", "fft_r2c_y", "fft_r2c_x", "bin_pk", "bin_reduce",
-               "xi_pass_x", "xi_pass_y", "xi_pass_z", "xi_group", "xi_hist", "sn_pairs", "tri_stage", "tri_pass_x",
+               "assign_gather", "assign_border", "fft_r2c_z", "fft_r2c_y", "fft_r2c_x", "fft_x_xi", "bin_pk",
+               "bin_reduce", "xi_pass_x", "xi_planes_x", "xi_pass_y", "xi_pass_z", "xi_group", "xi_hist", "sn_pairs", "tri_stage", "tri_pass_x",
                "tri3_pass_x", "tri_pass_y", "tri3_pass_y", "tri_pass_z", "tri3_pass_z", "tri_z_gram", "gram",
                "gram_reduce")
 

@@ -212,24 +212,32 @@ template <typename T> struct XiSrc {
     const double* xq = nullptr;
     // x pass over the half spectrum: o = 0, b = y * nzh + kz, p = x
     __device__ inline C<T> operator()(int, int, int b, int p, long long addr) const {
+        return at(p, b / nzh, b % nzh, addr);
+    }
+    // the mode (i, j, k) has the fast form: 0 < k_z < n_z/2, no second spectrum, the aliased shape
+    __device__ inline bool fast_ok(int k) const { return !dk2 && !sn_w2 && k != 0 && 2 * k != n[2]; }
+    // inside 0 < k_z < n_z/2 the spectrum at -k is the conjugate, so with p2 = |d|^2 the two sides
+    // combine to Re = (p2 (Q1 + Q2) - sn (S1 Q1 + S2 Q2)) / 2V (even part) and
+    // Im = -(p2 (Q1 - Q2) - sn (S1 Q1 - S2 Q2)) / 2V (odd part); Q, S separable per axis
+    __device__ inline C<T> fast(int i, int j, int k, C<T> d) const {
 #pragma clang fp contract(off)
-        const int i = p, j = b / nzh, k = b % nzh;
-        const C<T> d = dk[addr];
         const double re = (double)d.x, im = (double)d.y;
-        if (!dk2 && !sn_w2 && k != 0 && 2 * k != n[2]) {
-            // inside 0 < k_z < n_z/2 the spectrum at -k is the conjugate, so with p2 = |d|^2 the two
-            // sides combine to Re = (p2 (Q1 + Q2) - sn (S1 Q1 + S2 Q2)) / 2V (even part) and
-            // Im = -(p2 (Q1 - Q2) - sn (S1 Q1 - S2 Q2)) / 2V (odd part); Q, S separable per axis
-            const int jj = j ? n[1] - j : 0, kk = n[2] - k;
-            const double4 q = *reinterpret_cast<const double4*>(xq + 4 * (long long)i);
-            const double qa = t.iw[1][j] * t.iw[2][k], qb = t.iw[1][jj] * t.iw[2][kk];
-            const double sa = (t.sf[1][j] * t.sf[2][k]) * qa, sb = (t.sf[1][jj] * t.sf[2][kk]) * qb;
-            const double p2 = re * re + im * im;
-            const double snv = sn_ptr ? *sn_ptr : sn;
-            const double Q1 = q.x * qa, Q2 = q.y * qb, SQ1 = q.z * sa, SQ2 = q.w * sb;
-            if (odd) return mk<T>((T)0, (T)(-(0.5 * inv_v) * (p2 * (Q1 - Q2) - snv * (SQ1 - SQ2))));
-            return mk<T>((T)((0.5 * inv_v) * (p2 * (Q1 + Q2) - snv * (SQ1 + SQ2))), (T)0);
-        }
+        const int jj = j ? n[1] - j : 0, kk = n[2] - k;
+        const double4 q = *reinterpret_cast<const double4*>(xq + 4 * (long long)i);
+        const double qa = t.iw[1][j] * t.iw[2][k], qb = t.iw[1][jj] * t.iw[2][kk];
+        const double sa = (t.sf[1][j] * t.sf[2][k]) * qa, sb = (t.sf[1][jj] * t.sf[2][kk]) * qb;
+        const double p2 = re * re + im * im;
+        const double snv = sn_ptr ? *sn_ptr : sn;
+        const double Q1 = q.x * qa, Q2 = q.y * qb, SQ1 = q.z * sa, SQ2 = q.w * sb;
+        if (odd) return mk<T>((T)0, (T)(-(0.5 * inv_v) * (p2 * (Q1 - Q2) - snv * (SQ1 - SQ2))));
+        return mk<T>((T)((0.5 * inv_v) * (p2 * (Q1 + Q2) - snv * (SQ1 + SQ2))), (T)0);
+    }
+    // the value at mode (i, j, k) stored at addr
+    __device__ inline C<T> at(int i, int j, int k, long long addr) const {
+#pragma clang fp contract(off)
+        const C<T> d = dk[addr];
+        if (fast_ok(k)) return fast(i, j, k, d);
+        const double re = (double)d.x, im = (double)d.y;
         double p2, pim = 0.0;
         if (dk2) {
             const C<T> e = dk2[addr];
@@ -274,6 +282,78 @@ template <typename T> struct XiSrc {
         return mk<T>((T)(0.5 * (g1 + g2)), (T)(0.5 * (b1 - b2)));
     }
 };
+
+// XiSrc on one plane k_z = kz of the half spectrum: a column pass with the y
+// index as the outer line index (stride nzh) and one column per line, the
+// base pointers advanced by kz (the source reads dk at addr + kz).
+template <typename T> struct XiPlaneSrc {
+    XiSrc<T> xs;
+    int kz;
+    __device__ inline C<T> operator()(int, int o, int, int p, long long addr) const { return xs.at(p, o, kz, addr + kz); }
+};
+
+// The forward x pass of the spectrum fused with the inverse x pass of the
+// shot-noise xi (columns b = y * nzh + k_z of the half spectrum, points x
+// PS apart): each tile is transformed forward and stored (the spectrum d),
+// then the xi spectrum of its points is formed in LDS from d and transformed
+// back into xh -- d is read from HBM once for both.  Columns on the planes
+// k_z = 0, n_z/2 need the mirror points of other columns; they are left to a
+// pass with XiPlaneSrc afterwards.
+template <typename T, int L>
+__global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>* __restrict__ xh, XiSrc<T> xs,
+                                                    long long PS, int n_inner, const C<T>* __restrict__ tw) {
+    constexpr int B = ColTile<T, L>::B;
+    constexpr int LP = LdsPad<L>::LP;
+    constexpr int PSTEP = FFT_NT / B;
+    extern __shared__ unsigned char smraw[];
+    C<T>* sm = reinterpret_cast<C<T>*>(smraw);
+    const uint3 wg = xcd_block();
+    const int tid = threadIdx.x, bl = tid % B;
+    const int bc = (int)wg.x * B + bl;
+    const bool bvalid = bc < n_inner;
+    const int b = bvalid ? bc : 0;
+    for (int p = tid / B; p < L; p += PSTEP) sm[bl * LP + lpos(p)] = bvalid ? dk[b + (long long)p * PS] : mk<T>((T)0, (T)0);
+    __syncthreads();
+    lds_fft<T, -1, L, B>(sm, tw);
+    const int j = b / xs.nzh, k = b % xs.nzh;
+    const bool form = bvalid && xs.fast_ok(k);
+    // each lane reads and rewrites only the points it stores
+    for (int p = tid / B; p < L; p += PSTEP) {
+        const C<T> d = sm[bl * LP + lpos(p)];
+        if (bvalid) dk[b + (long long)p * PS] = d;
+        sm[bl * LP + lpos(p)] = form ? xs.fast(p, j, k, d) : mk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    lds_fft<T, +1, L, B>(sm, tw);
+    if (bvalid)
+        for (int p = tid / B; p < L; p += PSTEP) xh[b + (long long)p * PS] = sm[bl * LP + lpos(p)];
+}
+
+// The fused pass for a compile-time length; 1 when it ran, 0 when L has no
+// compile-time plan (the caller runs the two passes separately), -1 on error.
+template <typename T>
+static int launch_fwd_x_xi(hm_ctx* ctx, int L, C<T>* dk, C<T>* xh, const XiSrc<T>& xs, long long PS, int n_inner) {
+    switch (L) {
+#define HM_CASE(LL)                                                                                                    \
+    case LL: {                                                                                                         \
+        const C<T>* tw = ctx->twiddle<T>(LL);                                                                          \
+        if (!tw) return -1;                                                                                            \
+        constexpr int B = ColTile<T, LL>::B;                                                                           \
+        const size_t lds = col_lds<T, LL>();                                                                           \
+        const unsigned gx = (unsigned)((n_inner + B - 1) / B);                                                         \
+        return ctx->launch("fft_x_xi", [&] {                                                                           \
+                   hipLaunchKernelGGL((k_fwd_x_xi<T, LL>), dim3(gx), dim3(FFT_NT), lds, ctx->stream, dk, xh, xs, PS,   \
+                                      n_inner, tw);                                                                    \
+               })                                                                                                      \
+                   ? -1                                                                                                \
+                   : 1;                                                                                                \
+    }
+        HM_POW2_LENGTHS(HM_CASE)
+        HM_MIXED_LENGTHS(HM_CASE)
+#undef HM_CASE
+    default: return 0;
+    }
+}
 
 // Rows of a half spectrum restricted to |x|, |y| within a support, in the
 // compact order of support_map (the z c2r of a pruned inverse transform).
@@ -2192,8 +2272,37 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (fft_cols<T>(ctx, "fft_r2c_y", m.n[1], -1, dk, dk,
                     geom((long long)m.n[1] * nzh, nzh, m.n[0], m.n[0], -1, (int)nzh, (int)nzh, -1, -1)))
         return -1;
-    if (fft_cols<T>(ctx, "fft_r2c_x", m.n[0], -1, dk, dk,
-                    geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
+    // the source of the shot-noise xi of B (field.hpp:1116-1160): (|d|^2 - N S) / W^2 / V of each mode pair
+    auto make_xs = [&](const C<T>* d2) {
+        XiSrc<T> xs;
+        xs.dk = dk;
+        for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
+        xs.nzh = (int)nzh;
+        xs.t = tabs;
+        xs.xq = xi_xq(p);
+        xs.wp = wp;
+        xs.sn = sn3;
+        xs.inv_v = 1.0 / m.volume();
+        xs.dc = recon ? 0.0 : (double)np;
+        xs.sn_ptr = dyn ? dyn + DYN_SN3 : nullptr;
+        xs.dc_ptr = dyn ? dyn + DYN_DC : nullptr;
+        xs.dk2 = d2;
+        return xs;
+    };
+    // with one parity of xi and no second spectrum, xi's inverse x pass rides on the forward x pass (the
+    // spectrum is read once for both); its planes k_z = 0, n_z/2 follow in step 4.  Not with xi_l, whose
+    // transform (step 3b) works in the same buffer before step 4 reads it.
+    const bool fuse_xi = p->cfg.bk && !(weighted && wt->L > 0) && !recon && p->cfg.interlace == 0 && !p->sn_odd &&
+                         !p->odd_mesh && (m.n[2] % 2) == 0 && p->cfg.n_xi_ells == 0;
+    int fused_xi = 0;
+    if (fuse_xi) {
+        fused_xi = launch_fwd_x_xi<T>(ctx, m.n[0], dk, (C<T>*)p->xi_half.p, make_xs(nullptr),
+                                      (long long)m.n[1] * nzh, (int)(m.n[1] * nzh));
+        if (fused_xi < 0) return -1;
+    }
+    if (!fused_xi && fft_cols<T>(ctx, "fft_r2c_x", m.n[0], -1, dk, dk,
+                                 geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh),
+                                      -1, -1)))
         return -1;
     // 2''. interlacing: the half-cell-shifted grid, transformed and combined (no counterpart in the reference)
     const bool inter = p->cfg.interlace != 0;
@@ -2533,19 +2642,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
     if (p->cfg.bk) {
         // 4. shot-noise ijk term (field.hpp:1116-1160, bispec.hpp:1023-1056); per M below for a survey ELL > 0
         if (!svg) {
-        XiSrc<T> xs;
-        xs.dk = dk;
-        for (int a = 0; a < 3; a++) xs.n[a] = m.n[a];
-        xs.nzh = (int)nzh;
-        xs.t = tabs;
-        xs.xq = xi_xq(p);
-        xs.wp = wp;
-        xs.sn = sn3;
-        xs.inv_v = 1.0 / m.volume();
-        xs.dc = recon ? 0.0 : (double)np;
-        xs.sn_ptr = dyn ? dyn + DYN_SN3 : nullptr;
-        xs.dc_ptr = dyn ? dyn + DYN_DC : nullptr;
-        xs.dk2 = dk2;
+        XiSrc<T> xs = make_xs(dk2);
         // Re xi; for an odd l1 + l2 also Im xi (field.hpp:1116-1160 keeps both: an odd angular weight sees
         // the odd part of xi, and the even part on the z Nyquist plane), and always for the complex cross
         // spectrum of a reconstructed box
@@ -2554,8 +2651,17 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         void* hbuf = par ? p->hist2.p : p->hist.p;
         void* sbuf = par ? p->sn_pairs2.p : p->sn_pairs.p;
         C<T>* xh = (C<T>*)p->xi_half.p;
-        if (launch_cols<T, +1>(ctx, "xi_pass_x", m.n[0], xs, xh,
-                               geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh), (int)(m.n[1] * nzh), -1, -1)))
+        if (fused_xi) {
+            // the planes k_z = 0 and n_z/2 the fused pass left: one column per y line
+            for (int kz : {0, m.n[2] / 2}) {
+                XiPlaneSrc<T> ps{xs, kz};
+                if (launch_cols<T, +1>(ctx, "xi_planes_x", m.n[0], ps, xh + kz,
+                                       geom((long long)nzh, (long long)m.n[1] * nzh, m.n[1], m.n[1], -1, 1, 1, -1, -1)))
+                    return -1;
+            }
+        } else if (launch_cols<T, +1>(ctx, "xi_pass_x", m.n[0], xs, xh,
+                                      geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh),
+                                           (int)(m.n[1] * nzh), -1, -1)))
             return -1;
         if (fft_cols<T>(ctx, "xi_pass_y", m.n[1], +1, xh, xh,
                         geom((long long)m.n[1] * nzh, nzh, m.n[0], m.n[0], -1, (int)nzh, (int)nzh, -1, -1)))
