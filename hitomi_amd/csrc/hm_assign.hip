@@ -544,6 +544,28 @@ __global__ __launch_bounds__(256) void k_s2_move_w(const double* __restrict__ w,
     sw[2 * d + 1] = w[2 * q + 1];
 }
 
+// The same move with the weights read in order: one workgroup per level-1
+// chunk stages the chunk's S2_C1 complex weights in LDS (128 KiB), then each
+// wave walks the chunk's run in some of the buckets (level-1 order, about
+// S2_C1 / buckets records each) and writes every weight to its particle's
+// place. The runs hold only the chunk's own particles, so q - q0 stays in
+// [0, S2_C1).
+constexpr int S2_MW_NT = 1024;
+__global__ __launch_bounds__(S2_MW_NT) void k_s2_move_wc(const double* __restrict__ w, long long np, TwoLevel tl,
+                                                         const int* __restrict__ off1, const int* __restrict__ tq,
+                                                         const int* __restrict__ slot, double* __restrict__ sw) {
+    extern __shared__ double2 lw[];  // S2_C1
+    const long long c = blockIdx.x, q0 = c * S2_C1, q1 = (q0 + S2_C1 < np) ? q0 + S2_C1 : np;
+    for (long long q = q0 + threadIdx.x; q < q1; q += S2_MW_NT) lw[q - q0] = make_double2(w[2 * q], w[2 * q + 1]);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double2* sw2 = reinterpret_cast<double2*>(sw);
+    for (int b = wave; b < tl.nb; b += S2_MW_NT / 64) {
+        const long long r0 = off1[(long long)b * tl.nch1 + c], r1 = off1[(long long)b * tl.nch1 + c + 1];
+        for (long long j = r0 + lane; j < r1; j += 64) sw2[slot[j]] = lw[tq[4 * j] - q0];
+    }
+}
+
 static int scan_inplace(hm_ctx* ctx, const char* name, int* a, long long n, int* bsum) {
     const unsigned nb = (unsigned)((n + SCAN_BLK - 1) / SCAN_BLK);
     return ctx->launch(name, [&] {
@@ -691,8 +713,15 @@ static int assign_sorted_S(hm_ctx* ctx, const AssignGeom& g, const double* d_pos
                                spos, sw);
         }))
         return -1;
+    TwoLevel tr;
+    const bool chunk_move = reuse && w.two_level && two_level_geom(g, np, tr) &&
+                            ctx->lds_max >= sizeof(double2) * (size_t)S2_C1;
     if (np > 0 && reuse && sw && ctx->launch("assign_scatter", [&] {
-            if (w.two_level)
+            if (chunk_move)
+                hipLaunchKernelGGL(k_s2_move_wc, dim3((unsigned)tr.nch1), dim3(S2_MW_NT), sizeof(double2) * S2_C1,
+                                   ctx->stream, d_w, np, tr, (const int*)w.h1.p, (const int*)w.tw.p, (const int*)slot,
+                                   sw);
+            else if (w.two_level)
                 hipLaunchKernelGGL(k_s2_move_w, dim3(pb), dim3(256), 0, ctx->stream, d_w, np, (const int*)w.tw.p,
                                    (const int*)slot, (const int*)border, sw);
             else

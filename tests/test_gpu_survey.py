@@ -267,3 +267,31 @@ def test_survey_three_point_rejects_disallowed_multipole(ctx):
     """wigner_3j(ell1, ell2, ELL, 0, 0, 0) = 0 (bispec.hpp:363-369) is rejected."""
     with pytest.raises(RuntimeError):
         hm.SurveyPlan(ctx, 32, 800.0, "TSC", ell=1, ell1=1, ell2=1, measure="3PCF", rmin=60.0, rmax=240.0, n_rbin=4)
+
+
+def test_survey_pk_sort_reuse_many_buckets(ctx):
+    """P_2 on a (256, 256, 64) mesh with 1.2e5 points: the cell sort splits them into 4 x-slab buckets and 15
+    chunks of 8192, and the Y_2m-weighted fields after the first reuse that sort (each chunk's weights staged
+    in LDS, then written to their places bucket run by bucket run).  Against the oracle, which assigns every
+    field afresh: mode counts bit-exact, P_2 to 1e-10 of |P_0| + SN in FP64."""
+    rng = np.random.default_rng(5)
+    lo, size = np.array([300.0, 500.0, 900.0]), 700.0
+    nD, nR = 30000, 90000
+    posD = lo + rng.random((nD, 3)) * size
+    posR = lo + rng.random((nR, 3)) * size
+    wD = 1.0 / (1.0 + 0.3 * rng.random(nD))
+    wR = 1.0 / (1.0 + 0.3 * rng.random(nR))
+    n, L = (256, 256, 64), (800.0, 800.0, 800.0)
+    kw = dict(kmin=0.02, kmax=0.2, n_kbin=8)
+    p = po.param(n, L, po.TSC, ell1=2, ELL=2, **kw)
+    s = po.Survey(p, posD, wD, posR, wR)
+    ref, nm = s.pk(p)
+    ref0, _ = s.pk(po.param(n, L, po.TSC, **kw))
+    sn0 = s.shotnoise(0, 0).real * s.vsurvey / s.wsumD ** 2
+    plan = hm.SurveyPlan(ctx, n, L, "TSC", ell=2, precision=hm.FP64, **kw)
+    got, gnm, _ = plan.execute(posD, wD, posR, wR)
+    plan.close()
+    assert np.array_equal(gnm, nm.astype(np.int64))
+    err = np.abs(got - ref)
+    scale = np.abs(ref0) + sn0
+    assert np.all(err <= 1e-10 * scale), (err / scale).max()
