@@ -82,11 +82,14 @@ int assign_fixed(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const do
 // sort-by-cell assignment
 // ---------------------------------------------------------------------------
 // Output tile per workgroup: GX x GY x GZ cells.  Particles are sorted by
-// the (x, y, z / GZ) chunk of their first stencil cell.
+// the (x, y, z / GZ) chunk of their first stencil cell, and within a chunk
+// the last S - 1 first cells (the tail: the only ones whose stencil reaches
+// the next chunk) carry a key of their own, right after the chunk's head.
 constexpr int GX = 4, GY = 8, GZ = 64;
 
 __host__ __device__ inline int chunks_z(const AssignGeom& g) { return (g.n[2] + GZ - 1) / GZ; }
-__host__ __device__ inline long long n_keys(const AssignGeom& g) { return (long long)g.n[0] * g.n[1] * chunks_z(g); }
+__host__ __device__ inline long long keys_row(const AssignGeom& g) { return 2LL * chunks_z(g); }
+__host__ __device__ inline long long n_keys(const AssignGeom& g) { return (long long)g.n[0] * g.n[1] * keys_row(g); }
 
 template <int S>
 __device__ __forceinline__ bool first_cell(const double* __restrict__ pos, long long q, const AssignGeom& g,
@@ -100,7 +103,8 @@ __device__ __forceinline__ bool first_cell(const double* __restrict__ pos, long 
         stencil_axis(g.assign, axis_xp(pos[3 * q + a], g.n[a], g.L[a]), &iw0[a], wt);
         inside = inside && iw0[a] >= 0 && iw0[a] + S - 1 <= g.n[a] - 1;
     }
-    *key = ((long long)iw0[0] * g.n[1] + iw0[1]) * chunks_z(g) + iw0[2] / GZ;
+    const int tail = (S > 1 && iw0[2] % GZ >= GZ - (S - 1)) ? 1 : 0;
+    *key = (((long long)iw0[0] * g.n[1] + iw0[1]) * chunks_z(g) + iw0[2] / GZ) * 2 + tail;
     return inside;
 }
 
@@ -247,11 +251,11 @@ __global__ __launch_bounds__(256) void k_sort_move_w(const double* __restrict__ 
 // One workgroup owns one output tile.  The interior particles whose first
 // stencil cell lies in the tile extended by S-1 cells towards -x, -y, -z are
 // exactly those that can reach the tile; in key order they form one
-// contiguous run per extended (x, y) row (the z chunks of the tile and the
-// one before it), read from the prefix array.  Each
-// such particle adds its in-tile stencil contributions to an LDS tile with
-// 64-bit integer atomics (order-independent), and the tile is then stored
-// once with plain writes.
+// contiguous run per extended (x, y) row (the tail key of the chunk before
+// the tile, then both keys of the tile's chunk), read from the prefix array.
+// Each such particle adds its in-tile stencil contributions to an LDS tile
+// with 64-bit integer atomics (order-independent), and the tile is then
+// stored once with plain writes.
 template <int S, bool CPLX>
 __global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ start, const double* __restrict__ spos,
                                                     const double* __restrict__ sw, AssignGeom g, double scale,
@@ -270,8 +274,9 @@ __global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ sta
         int b = 0, n = 0;
         if (xx >= 0 && xx <= g.n[0] - S && yy >= 0 && yy <= g.n[1] - S && zlo <= zhi) {
             const long long lo = ((long long)xx * g.n[1] + yy) * chunks_z(g);
-            b = start[lo + zlo / GZ];
-            n = start[lo + zhi / GZ + 1] - b;
+            const int cl = zlo / GZ, ch = zhi / GZ;
+            b = start[2 * (lo + cl) + (cl < z0 / GZ ? 1 : 0)];
+            n = start[2 * (lo + ch) + 2] - b;
         }
         rbeg[r] = b;
         roff[r + 1] = n;
@@ -347,6 +352,8 @@ __global__ __launch_bounds__(256) void k_assign_border(const double* __restrict_
 constexpr int S2_C1 = 8192;   // particles per level-1 chunk
 constexpr int S2_C2 = 16384;  // particles per level-2 chunk
 constexpr int S2_LK = 16384;  // most local keys of a bucket (int32 counters in 64 KB of LDS)
+constexpr int S2_U = 4;       // particles per thread loaded before any is placed
+constexpr int S2_U2 = 2;      // the same for the level-2 placement (more writes in flight there slow it)
 
 struct TwoLevel {
     int XB, nb, LK;
@@ -355,7 +362,7 @@ struct TwoLevel {
 };
 
 static bool two_level_geom(const AssignGeom& g, long long np, TwoLevel& t) {
-    const long long rowkeys = (long long)g.n[1] * chunks_z(g);
+    const long long rowkeys = (long long)g.n[1] * keys_row(g);
     if (rowkeys > S2_LK || np <= 0) return false;
     t.XB = (int)std::min<long long>(S2_LK / rowkeys, g.n[0]);
     t.nb = (g.n[0] + t.XB - 1) / t.XB;
@@ -393,16 +400,37 @@ __global__ __launch_bounds__(256) void k_s2_scatter1(const int* __restrict__ key
     const long long c = blockIdx.x, q0 = c * S2_C1, q1 = (q0 + S2_C1 < np) ? q0 + S2_C1 : np;
     for (int e = threadIdx.x; e <= tl.nb; e += 256) cur[e] = off1[(long long)e * tl.nch1 + c];
     __syncthreads();
-    for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
-        const int K = key[q];
-        const int b = K < 0 ? tl.nb : K / tl.LK;
-        const long long d = atomicAdd(&cur[b], 1);
-        // one 32-byte (x, y, z, w_re) and one 16-byte (q, local key, w_im) record per particle
-        const double wr = w ? w[2 * q] : 0.0, wi = w ? w[2 * q + 1] : 0.0;
-        reinterpret_cast<double4*>(tpos)[d] = make_double4(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2], wr);
-        const long long wib = __double_as_longlong(wi);
-        reinterpret_cast<int4*>(tw)[d] = make_int4((int)q, K < 0 ? -1 : K - b * tl.LK, (int)(wib & 0xffffffffLL),
-                                                   (int)(wib >> 32));
+    // S2_U particles per thread are loaded before any is placed: stores count
+    // in the same in-order counter as loads, so one particle at a time would
+    // wait for the previous particle's scattered stores on every load
+    // without weights the loads read pos (3 np doubles, so 2 q + 1 stays inside) and the values are dropped
+    const double* wsrc = w ? w : pos;
+    for (long long qb = q0 + threadIdx.x; qb < q1; qb += S2_U * 256) {
+        int K[S2_U];
+        double x[S2_U], y[S2_U], z[S2_U], wr[S2_U], wi[S2_U];
+#pragma unroll
+        for (int u = 0; u < S2_U; u++) {
+            // past the chunk's end: load the last particle again (unused), so no load waits on a branch
+            const long long q = (qb + u * 256 < q1) ? qb + u * 256 : q1 - 1;
+            K[u] = key[q];
+            x[u] = pos[3 * q];
+            y[u] = pos[3 * q + 1];
+            z[u] = pos[3 * q + 2];
+            wr[u] = wsrc[2 * q];
+            wi[u] = wsrc[2 * q + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < S2_U; u++) {
+            const long long q = qb + u * 256;
+            if (q >= q1) break;
+            const int b = K[u] < 0 ? tl.nb : K[u] / tl.LK;
+            const long long d = atomicAdd(&cur[b], 1);
+            // one 32-byte (x, y, z, w_re) and one 16-byte (q, local key, w_im) record per particle
+            reinterpret_cast<double4*>(tpos)[d] = make_double4(x[u], y[u], z[u], w ? wr[u] : 0.0);
+            const long long wib = __double_as_longlong(w ? wi[u] : 0.0);
+            reinterpret_cast<int4*>(tw)[d] = make_int4((int)q, K[u] < 0 ? -1 : K[u] - b * tl.LK,
+                                                       (int)(wib & 0xffffffffLL), (int)(wib >> 32));
+        }
     }
 }
 
@@ -444,28 +472,38 @@ __device__ __forceinline__ int s2_bucket(const int* __restrict__ cst, int nb, in
     return lo;
 }
 
+constexpr int S2_NT2 = 1024;
+
 // Level-2 key counts, layout per bucket [LK][chunks] (key-major) at LK * cst[b];
-// workgroups past the last chunk zero the unused tail of h2 instead.
-__global__ __launch_bounds__(256) void k_s2_hist2(const int* __restrict__ off1, const int* __restrict__ cst,
+// workgroups past the last chunk zero the unused tail of h2 instead. The
+// 64 KiB histogram allows two workgroups per CU, so each runs S2_NT2 threads.
+__global__ __launch_bounds__(S2_NT2) void k_s2_hist2(const int* __restrict__ off1, const int* __restrict__ cst,
                                                  TwoLevel tl, const int* __restrict__ tk, int* __restrict__ h2) {
     // tk: the level-1 16-byte records (index, local key, w_im)
     extern __shared__ int hist[];  // LK
     const int w = blockIdx.x;
     const int total = cst[tl.nb];
     if (w >= total) {
-        for (int e = threadIdx.x; e < tl.LK; e += 256) h2[(long long)w * tl.LK + e] = 0;
+        for (int e = threadIdx.x; e < tl.LK; e += S2_NT2) h2[(long long)w * tl.LK + e] = 0;
         return;
     }
     const int b = s2_bucket(cst, tl.nb, w);
     const int ch = cst[b + 1] - cst[b], c = w - cst[b];
     const long long bs = off1[(long long)b * tl.nch1], be = off1[(long long)(b + 1) * tl.nch1];
     const long long p0 = bs + (long long)c * S2_C2, p1 = (p0 + S2_C2 < be) ? p0 + S2_C2 : be;
-    for (int e = threadIdx.x; e < tl.LK; e += 256) hist[e] = 0;
+    for (int e = threadIdx.x; e < tl.LK; e += S2_NT2) hist[e] = 0;
     __syncthreads();
-    for (long long p = p0 + threadIdx.x; p < p1; p += 256) atomicAdd(&hist[tk[4 * p + 1]], 1);
+    for (long long pb = p0 + threadIdx.x; pb < p1; pb += S2_U * S2_NT2) {
+        int k[S2_U];
+#pragma unroll
+        for (int u = 0; u < S2_U; u++) k[u] = tk[4 * ((pb + u * S2_NT2 < p1) ? pb + u * S2_NT2 : p1 - 1) + 1];
+#pragma unroll
+        for (int u = 0; u < S2_U; u++)
+            if (pb + u * S2_NT2 < p1) atomicAdd(&hist[k[u]], 1);
+    }
     __syncthreads();
     int* out = h2 + (long long)tl.LK * cst[b] + c;
-    for (int e = threadIdx.x; e < tl.LK; e += 256) out[(long long)e * ch] = hist[e];
+    for (int e = threadIdx.x; e < tl.LK; e += S2_NT2) out[(long long)e * ch] = hist[e];
 }
 
 // start[K] of every key (the first place of key K; n_interior at K = nk).
@@ -487,7 +525,7 @@ __global__ __launch_bounds__(256) void k_s2_start(const int* __restrict__ off1, 
 // places in key order, all within the bucket's range; slot[p] = place of the
 // particle at level-1 position p (a later move of weights reads the particle
 // index from the level-1 record).
-__global__ __launch_bounds__(256) void k_s2_scatter2(const int* __restrict__ off1, const int* __restrict__ cst,
+__global__ __launch_bounds__(S2_NT2) void k_s2_scatter2(const int* __restrict__ off1, const int* __restrict__ cst,
                                                     const int* __restrict__ off2, TwoLevel tl,
                                                     const int* __restrict__ tk, const double* __restrict__ pos,
                                                     bool weights, int* __restrict__ slot, double* __restrict__ spos,
@@ -500,19 +538,30 @@ __global__ __launch_bounds__(256) void k_s2_scatter2(const int* __restrict__ off
     const long long bs = off1[(long long)b * tl.nch1], be = off1[(long long)(b + 1) * tl.nch1];
     const long long p0 = bs + (long long)c * S2_C2, p1 = (p0 + S2_C2 < be) ? p0 + S2_C2 : be;
     const int* o = off2 + (long long)tl.LK * cst[b] + c;
-    for (int e = threadIdx.x; e < tl.LK; e += 256) cur[e] = o[(long long)e * ch];
+    for (int e = threadIdx.x; e < tl.LK; e += S2_NT2) cur[e] = o[(long long)e * ch];
     __syncthreads();
-    for (long long p = p0 + threadIdx.x; p < p1; p += 256) {
-        const int4 m = reinterpret_cast<const int4*>(tk)[p];
-        const double4 a = reinterpret_cast<const double4*>(pos)[p];
-        const long long d = atomicAdd(&cur[m.y], 1);
-        slot[p] = (int)d;
-        spos[3 * d] = a.x;
-        spos[3 * d + 1] = a.y;
-        spos[3 * d + 2] = a.z;
-        if (weights) {
-            sw[2 * d] = a.w;
-            sw[2 * d + 1] = __longlong_as_double(((long long)m.w << 32) | (unsigned)m.z);
+    for (long long pb = p0 + threadIdx.x; pb < p1; pb += S2_U2 * S2_NT2) {
+        int4 m[S2_U2];
+        double4 a[S2_U2];
+#pragma unroll
+        for (int u = 0; u < S2_U2; u++) {
+            const long long p = (pb + u * S2_NT2 < p1) ? pb + u * S2_NT2 : p1 - 1;  // the tail reloads the last
+            m[u] = reinterpret_cast<const int4*>(tk)[p];
+            a[u] = reinterpret_cast<const double4*>(pos)[p];
+        }
+#pragma unroll
+        for (int u = 0; u < S2_U2; u++) {
+            const long long p = pb + u * S2_NT2;
+            if (p >= p1) break;
+            const long long d = atomicAdd(&cur[m[u].y], 1);
+            slot[p] = (int)d;
+            spos[3 * d] = a[u].x;
+            spos[3 * d + 1] = a[u].y;
+            spos[3 * d + 2] = a[u].z;
+            if (weights) {
+                sw[2 * d] = a[u].w;
+                sw[2 * d + 1] = __longlong_as_double(((long long)m[u].w << 32) | (unsigned)m[u].z);
+            }
         }
     }
 }
@@ -595,7 +644,7 @@ static int sort_two_level(hm_ctx* ctx, const AssignGeom& g, const TwoLevel& tl, 
             hipLaunchKernelGGL(k_s2_scatter1, dim3((unsigned)tl.nch1), dim3(256), lds1, ctx->stream, key, np, tl, h1,
                                d_pos, d_w, (double*)w.tpos.p, (int*)w.tw.p);
             hipLaunchKernelGGL(k_s2_chunks, dim3(1), dim3(1024), 0, ctx->stream, h1, tl, cst);
-            hipLaunchKernelGGL(k_s2_hist2, dim3((unsigned)tl.nch2), dim3(256), lds2, ctx->stream, h1, cst, tl,
+            hipLaunchKernelGGL(k_s2_hist2, dim3((unsigned)tl.nch2), dim3(S2_NT2), lds2, ctx->stream, h1, cst, tl,
                                (const int*)w.tw.p, h2);
         }))
         return -1;
@@ -603,7 +652,7 @@ static int sort_two_level(hm_ctx* ctx, const AssignGeom& g, const TwoLevel& tl, 
     return ctx->launch("assign_scatter", [&] {
         hipLaunchKernelGGL(k_s2_start, dim3((unsigned)((nk + 1 + 255) / 256)), dim3(256), 0, ctx->stream, h1, cst, h2,
                            tl, nk, (int*)w.start.p);
-        hipLaunchKernelGGL(k_s2_scatter2, dim3((unsigned)tl.nch2), dim3(256), lds2, ctx->stream, h1, cst, h2, tl,
+        hipLaunchKernelGGL(k_s2_scatter2, dim3((unsigned)tl.nch2), dim3(S2_NT2), lds2, ctx->stream, h1, cst, h2, tl,
                            (const int*)w.tw.p, (const double*)w.tpos.p, d_w != nullptr, (int*)w.slot.p,
                            (double*)w.spos.p, (double*)w.sw.p);
         hipLaunchKernelGGL(k_s2_border, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, h1, tl, np,

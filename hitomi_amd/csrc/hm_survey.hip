@@ -61,25 +61,30 @@ double wigner_3j(int j1, int j2, int j3, int m1, int m2, int m3) {
 // ---------------------------------------------------------------------------
 // particle passes
 // ---------------------------------------------------------------------------
-// per-block partials: sum w_D, sum w_D^2, sum w_R, sum |w_D|, sum |w_R|; fixed order
+// per-block partials: sum w_D, sum w_D^2, sum w_R, sum |w_D|, sum |w_R|; fixed order.
+// w * w is the correctly rounded square that pow(w, 2) returns; the cubes
+// (pow(w, 3), only the three-point shot noise uses them) only when asked for.
 constexpr int SV_NSUM = 8;  // sum w_D, w_D^2, w_R, |w_D|, |w_R|, w_R^2, w_D^3, w_R^3
 __global__ __launch_bounds__(256) void k_survey_wsums(const double* __restrict__ wD, long long nD,
-                                                     const double* __restrict__ wR, long long nR,
+                                                     const double* __restrict__ wR, long long nR, int cubes,
                                                      double* __restrict__ part) {
+#pragma clang fp contract(off)
     __shared__ double sm[SV_NSUM][256];
     double v[SV_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const long long st = (long long)gridDim.x * 256;
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nD; q += st) {
-        v[0] += wD[q];
-        v[1] += pow(wD[q], 2);
-        v[3] += fabs(wD[q]);
-        v[6] += pow(wD[q], 3);
+        const double w = wD[q];
+        v[0] += w;
+        v[1] += w * w;
+        v[3] += fabs(w);
+        if (cubes) v[6] += pow(w, 3);
     }
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nR; q += st) {
-        v[2] += wR[q];
-        v[4] += fabs(wR[q]);
-        v[5] += pow(wR[q], 2);
-        v[7] += pow(wR[q], 3);
+        const double w = wR[q];
+        v[2] += w;
+        v[4] += fabs(w);
+        v[5] += w * w;
+        if (cubes) v[7] += pow(w, 3);
     }
     for (int k = 0; k < SV_NSUM; k++) sm[k][threadIdx.x] = v[k];
     __syncthreads();
@@ -273,13 +278,25 @@ __global__ __launch_bounds__(256) void k_survey_sqsum(const long long* __restric
                                                      double factor, double dV, double* __restrict__ part) {
 #pragma clang fp contract(off)
     __shared__ double sm[256];
-    double a = 0.0;
+    // four cells per step, each into its own sum, so four loads are in flight
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
     const long long st = (long long)gridDim.x * 256;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += st) {
-        const double r = ((double)q[i] * inv_scale) * factor;
-        a += r * r * dV;
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        long long v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = q[i + u * st];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double r = ((double)v[u] * inv_scale) * factor;
+            a[u] += r * r * dV;
+        }
     }
-    sm[threadIdx.x] = a;
+    for (; i < n; i += st) {
+        const double r = ((double)q[i] * inv_scale) * factor;
+        a[0] += r * r * dV;
+    }
+    sm[threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
@@ -671,7 +688,8 @@ static int survey_run(hm_ctx* ctx, SurveyPlanImpl* p, const double* pD, const do
     // 1. weight sums (particle.hpp:334-353, 284-299) and the randoms' minimum (particle.hpp:160-200)
     std::vector<double> hp((size_t)rb * 3);
     if (ctx->launch("survey_wsums", [&] {
-            hipLaunchKernelGGL(k_survey_wsums, dim3(rb), dim3(256), 0, ctx->stream, wD, nD, wR, nR, part);
+            hipLaunchKernelGGL(k_survey_wsums, dim3(rb), dim3(256), 0, ctx->stream, wD, nD, wR, nR,
+                               delegated(p->measure) ? 1 : 0, part);
         }))
         return -1;
     if (reduce_cols<double>(ctx, "survey_reduce", part, rb, SV_NSUM, red)) return -1;

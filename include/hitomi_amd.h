@@ -97,7 +97,7 @@ int hm_assign_atomic(hm_ctx *ctx, const hm_mesh *mesh, const double *d_pos, cons
  * serving loops, harnesses that forbid hipMalloc).  The same grids as
  * hm_assign, bit for bit.  The sort work lives in a fixed 128 MiB device array
  * of the library (one per GPU, held until the process ends): about 48 B per
- * particle with weights (32 B without) plus 4 B per (x, y, 64-cell z chunk);
+ * particle with weights (32 B without) plus 8 B per (x, y, 64-cell z chunk);
  * larger inputs are rejected with an error.  The int64 cell sums are
  * accumulated in d_out_re / d_out_im themselves and converted in place.
  * Asynchronous on the context's stream (no synchronisation).  Every call on a
