@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace hm {
 
@@ -298,23 +299,46 @@ __global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ sta
         for (int a = 0; a < 3; a++) stencil_axis(g.assign, axis_xp(spos[3 * p + a], g.n[a], g.L[a]), &iw0[a], wt[a]);
         const double wre = sw ? sw[2 * p] : 1.0;
         const double wim = sw ? sw[2 * p + 1] : 0.0;
-        for (int i = 0; i < S; i++) {
-            const int ox = iw0[0] + i - x0;
-            if (ox < 0 || ox >= GX) continue;
-            for (int jj = 0; jj < S; jj++) {
-                const int oy = iw0[1] + jj - y0;
-                if (oy < 0 || oy >= GY) continue;
-                const double wij = wt[0][i] * wt[1][jj];
-                for (int k = 0; k < S; k++) {
-                    const int oz = iw0[2] + k - z0;
-                    if (oz < 0 || oz >= GZ) continue;
-                    const double wijk = wij * wt[2][k];
-                    const int c = (ox * GY + oy) * GZ + oz;
-                    atomicAdd(&acc[c], (unsigned long long)__double2ll_rn(wre * wijk * scale));
-                    if (CPLX) atomicAdd(&acc[NC + c], (unsigned long long)__double2ll_rn(wim * wijk * scale));
+        // a zero weight adds exact zeros: a particle with one non-zero channel adds to that channel alone
+        const bool re0 = CPLX && wre == 0.0;
+        const int ch0 = re0 ? NC : 0;
+        const bool both = CPLX && !re0 && wim != 0.0;
+        // the scale is a power of two, so (w * w_ijk) * scale == (w * scale) * w_ijk exactly
+        const double s0 = (re0 ? wim : wre) * scale, s1 = wim * scale;
+        // |w_ijk| <= 1: below 2^51 every contribution rounds with one add of 1.5 * 2^52 (to nearest, ties to
+        // even, as __double2ll_rn); a wave takes that path when all its particles allow it
+        const bool small = fabs(s0) < 0x1p51 && fabs(s1) < 0x1p51;
+        auto stencil = [&](auto fast) {
+            for (int i = 0; i < S; i++) {
+                const int ox = iw0[0] + i - x0;
+                if (ox < 0 || ox >= GX) continue;
+                for (int jj = 0; jj < S; jj++) {
+                    const int oy = iw0[1] + jj - y0;
+                    if (oy < 0 || oy >= GY) continue;
+                    const double wij = wt[0][i] * wt[1][jj];
+                    for (int k = 0; k < S; k++) {
+                        const int oz = iw0[2] + k - z0;
+                        if (oz < 0 || oz >= GZ) continue;
+                        const double wijk = wij * wt[2][k];
+                        const int c = (ox * GY + oy) * GZ + oz;
+                        if constexpr (decltype(fast)::value) {
+                            atomicAdd(&acc[ch0 + c], (unsigned long long)(__double_as_longlong(s0 * wijk + 0x1.8p52) -
+                                                                         0x4338000000000000LL));
+                            if (both)
+                                atomicAdd(&acc[NC + c], (unsigned long long)(__double_as_longlong(s1 * wijk + 0x1.8p52) -
+                                                                            0x4338000000000000LL));
+                        } else {
+                            atomicAdd(&acc[ch0 + c], (unsigned long long)__double2ll_rn(s0 * wijk));
+                            if (both) atomicAdd(&acc[NC + c], (unsigned long long)__double2ll_rn(s1 * wijk));
+                        }
+                    }
                 }
             }
-        }
+        };
+        if (__all(small))
+            stencil(std::true_type{});
+        else
+            stencil(std::false_type{});
     }
     __syncthreads();
     for (int e = threadIdx.x; e < NC; e += 256) {
@@ -800,6 +824,11 @@ int assign_sorted(hm_ctx* ctx, const AssignGeom& g, const double* d_pos, const d
                   double scale, long long* d_re, long long* d_im, SortWork& w, bool reuse) {
     if (np > 0x7ffffff0LL || g.ng > 0x7ffffff0LL) {
         ctx->set_error("assign: more than 2^31 particles or cells");
+        return -1;
+    }
+    int sexp = 0;
+    if (!(scale > 0.0) || std::frexp(scale, &sexp) != 0.5) {  // the gather premultiplies weights by the scale
+        ctx->set_error("assign: the fixed-point scale must be a power of two");
         return -1;
     }
     SortWork::Sig& sg = w.sig;
