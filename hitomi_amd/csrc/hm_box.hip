@@ -903,15 +903,26 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
     // tl: the lane index passed through an empty asm statement once per row, so
     // that the compiler recomputes the row-invariant LDS, twiddle and load
     // addresses inside the loop instead of holding them in registers across
-    // it (that would cost occupancy)
+    // it (that would cost occupancy).  Staged element q of a lane is
+    // e = tl + q TRI_NT: its (line, point) pair is stepped from the previous
+    // one (one division per row), and fields are addressed with 32-bit
+    // element offsets from the row's base (the host checks NB fstride < 2^31).
+    constexpr int DF = TRI_NT / L, DZ = TRI_NT % L;
+    const unsigned fs = (unsigned)fstride;
     auto load_row = [&](long long r, int tl) {
+        const C<T>* Fr = F + r * L;
+        int f = tl / L, z = tl - f * L;
 #pragma unroll
         for (int q = 0; q < NPRE; q++) {
-            const int e = tl + q * TRI_NT;
-            const int f = e / L, z = e % L;
             C<T> v = mk<T>((T)0, (T)0);
-            if ((need >> q) & 1u) v = F[(long long)(f < nk ? f : nk) * fstride + r * L + z];
+            if ((need >> q) & 1u) v = Fr[(unsigned)(f < nk ? f : nk) * fs + (unsigned)z];
             pre[q] = v;
+            z += DZ;
+            f += DF;
+            if (z >= L) {
+                z -= L;
+                f++;
+            }
         }
     };
     // the twiddle table in LDS after the tile (read by every stage of every row)
@@ -921,10 +932,18 @@ __global__ __launch_bounds__(TRI_NT) void k_tri_z_gram(const C<T>* __restrict__ 
     for (long long r = r0; r < r1; r++) {
         int tl = t;
         asm volatile("" : "+v"(tl));
+        {
+            int f = tl / L, z = tl - f * L;
 #pragma unroll
-        for (int q = 0; q < NPRE; q++) {
-            const int e = tl + q * TRI_NT;
-            if (e < NB * L) sm[(e / L) * LP + e % L] = pre[q];
+            for (int q = 0; q < NPRE; q++) {
+                if (f < NB) sm[f * LP + z] = pre[q];
+                z += DZ;
+                f += DF;
+                if (z >= L) {
+                    z -= L;
+                    f++;
+                }
+            }
         }
         __syncthreads();
         if (r + 1 < r1) load_row(r + 1, tl);
@@ -982,6 +1001,10 @@ static int launch_tri_z_gram_L(hm_ctx* ctx, const C<T>* F, long long fstride, in
     const size_t fft_lds = ((size_t)nb * LdsPad<L, false>::LP + L) * sizeof(C<T>);
     const size_t red_lds = (size_t)TRI_NT * 4 * sizeof(double);
     const size_t lds = fft_lds > red_lds ? fft_lds : red_lds;
+    if ((long long)nb * fstride >= (1LL << 31)) {  // the kernel addresses fields with 32-bit offsets
+        ctx->set_error("tri_z_gram: fields too large for 32-bit offsets");
+        return -1;
+    }
     const C<T>* tw = ctx->twiddle<T>(L);
     if (!tw) return -1;
     const unsigned blocks = (unsigned)((nrows + rpw - 1) / rpw);
