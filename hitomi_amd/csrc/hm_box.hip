@@ -312,13 +312,31 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
     const int bc = (int)wg.x * B + bl;
     const bool bvalid = bc < n_inner;
     const int b = bvalid ? bc : 0;
-    for (int p = tid / B; p < L; p += PSTEP) sm[bl * LP + lpos(p)] = bvalid ? dk[b + (long long)p * PS] : mk<T>((T)0, (T)0);
+    // all of the lane's points are loaded before any is written to LDS (loads in flight together)
+    constexpr int NI = (L + PSTEP - 1) / PSTEP;
+    const int p0 = tid / B;
+    {
+        C<T> buf[NI];
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            buf[i] = (p < L && bvalid) ? dk[b + (long long)p * PS] : mk<T>((T)0, (T)0);
+        }
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            if (p < L) sm[bl * LP + lpos(p)] = buf[i];
+        }
+    }
     __syncthreads();
     lds_fft<T, -1, L, B>(sm, tw);
     const int j = b / xs.nzh, k = b % xs.nzh;
     const bool form = bvalid && xs.fast_ok(k);
     // each lane reads and rewrites only the points it stores
-    for (int p = tid / B; p < L; p += PSTEP) {
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int p = p0 + i * PSTEP;
+        if (p >= L) continue;
         const C<T> d = sm[bl * LP + lpos(p)];
         if (bvalid) dk[b + (long long)p * PS] = d;
         sm[bl * LP + lpos(p)] = form ? xs.fast(p, j, k, d) : mk<T>((T)0, (T)0);

@@ -424,10 +424,22 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     const bool bvalid = bc < n_inner;
     const int b = bvalid ? support_map(bc, inner_s, g.inner_full) : 0;
     constexpr int PSTEP = FFT_NT / B;
-    for (int p = tid / B; p < L; p += PSTEP) {
+    // every point of the lane is loaded before any is written to LDS, so the
+    // loads are in flight together (one at a time, each would wait its full latency)
+    constexpr int NI = (L + PSTEP - 1) / PSTEP;
+    const int p0 = tid / B;
+    C<T> buf[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int p = p0 + i * PSTEP;
         C<T> v = mk<T>((T)0, (T)0);
-        if (bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
-        sm[bl * LP + lpos(p)] = v;
+        if (p < L && bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
+        buf[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int p = p0 + i * PSTEP;
+        if (p < L) sm[bl * LP + lpos(p)] = buf[i];
     }
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, tw);
@@ -446,12 +458,22 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows(const C<T>* __restrict__ sr
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
     const long long r0 = (long long)blockIdx.x * B;
-    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+    // all of the lane's points are loaded before any is written to LDS (loads in flight together)
+    constexpr int NE = (B * L + FFT_NT - 1) / FFT_NT;
+    C<T> buf[NE];
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
         const int b = e / L, p = e % L;
         C<T> v = mk<T>((T)0, (T)0);
-        if (r0 + b < g.n_rows && in_support(p, g.fps ? g.fps[(r0 + b) / g.rpf] : g.point_s, L))
+        if (e < B * L && r0 + b < g.n_rows && in_support(p, g.fps ? g.fps[(r0 + b) / g.rpf] : g.point_s, L))
             v = src[(r0 + b) * g.RS_in + p];
-        sm[b * LP + lpos(p)] = v;
+        buf[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
+        if (e < B * L) sm[(e / L) * LP + lpos(e % L)] = buf[i];
     }
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, tw);
@@ -471,11 +493,20 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows_f(RSrc src, C<T>* __restric
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
     const long long r0 = (long long)blockIdx.x * B;
-    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+    constexpr int NE = (B * L + FFT_NT - 1) / FFT_NT;
+    C<T> buf[NE];
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
         const int b = e / L, p = e % L;
         C<T> v = mk<T>((T)0, (T)0);
-        if (r0 + b < g.n_rows) v = src(r0 + b, p);
-        sm[b * LP + lpos(p)] = v;
+        if (e < B * L && r0 + b < g.n_rows) v = src(r0 + b, p);
+        buf[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
+        if (e < B * L) sm[(e / L) * LP + lpos(e % L)] = buf[i];
     }
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, tw);
@@ -496,11 +527,20 @@ __global__ __launch_bounds__(FFT_NT) void k_r2c_rows(RSrc src, C<T>* __restrict_
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
     const long long r0 = (long long)blockIdx.x * B;
-    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+    constexpr int NE = (B * L + FFT_NT - 1) / FFT_NT;
+    C<T> buf[NE];
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
         const int b = e / L, p = e % L;
         C<T> v = mk<T>((T)0, (T)0);
-        if (r0 + b < n_rows) v = src(r0 + b, p);
-        sm[b * LP + lpos(p)] = v;
+        if (e < B * L && r0 + b < n_rows) v = src(r0 + b, p);
+        buf[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
+        if (e < B * L) sm[(e / L) * LP + lpos(e % L)] = buf[i];
     }
     __syncthreads();
     lds_fft<T, -1, L, B>(sm, tw);
@@ -528,21 +568,36 @@ __global__ __launch_bounds__(FFT_NT) void k_c2r_rows(HSrc src, T* __restrict__ d
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
     const long long r0 = (long long)blockIdx.x * B;
-    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
-        const int b = e / L, k = e % L;
-        C<T> v = mk<T>((T)0, (T)0);
-        if (r0 + b < n_rows) {
-            const C<T> xk = (point_s < 0 || k <= point_s) ? src(r0 + b, k) : mk<T>((T)0, (T)0);
-            const int kc = L - k;
-            const C<T> xc = (point_s < 0 || kc <= point_s) ? cconj<T>(src(r0 + b, kc)) : mk<T>((T)0, (T)0);
-            const C<T> s = cadd<T>(xk, xc);
-            const C<T> d = csub<T>(xk, xc);
-            C<T> w = tw2[k];
-            w.y = -w.y;  // exp(+2 pi i k / 2L)
-            const C<T> wd = cmul<T>(w, d);
-            v = mk<T>(s.x - wd.y, s.y + wd.x);  // s + i * wd
+    // chunks of CH elements per lane: the chunk's loads are issued before any is used
+    constexpr int NE = (B * L + FFT_NT - 1) / FFT_NT, CH = NE < 8 ? NE : 8;
+#pragma unroll
+    for (int i0 = 0; i0 < NE; i0 += CH) {
+        C<T> xks[CH], xcs[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + (i0 + u) * FFT_NT;
+            const int b = e / L, k = e % L, kc = L - k;
+            const bool row = i0 + u < NE && e < B * L && r0 + b < n_rows;
+            xks[u] = (row && (point_s < 0 || k <= point_s)) ? src(r0 + b, k) : mk<T>((T)0, (T)0);
+            xcs[u] = (row && (point_s < 0 || kc <= point_s)) ? cconj<T>(src(r0 + b, kc)) : mk<T>((T)0, (T)0);
         }
-        sm[b * LP + lpos(k)] = v;
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int e = threadIdx.x + (i0 + u) * FFT_NT;
+            if (i0 + u >= NE || e >= B * L) continue;
+            const int b = e / L, k = e % L;
+            C<T> v = mk<T>((T)0, (T)0);
+            if (r0 + b < n_rows) {
+                const C<T> xk = xks[u], xc = xcs[u];
+                const C<T> sum = cadd<T>(xk, xc);
+                const C<T> d = csub<T>(xk, xc);
+                C<T> w = tw2[k];
+                w.y = -w.y;  // exp(+2 pi i k / 2L)
+                const C<T> wd = cmul<T>(w, d);
+                v = mk<T>(sum.x - wd.y, sum.y + wd.x);  // sum + i * wd
+            }
+            sm[b * LP + lpos(k)] = v;
+        }
     }
     __syncthreads();
     lds_fft<T, +1, L, B>(sm, tw);
