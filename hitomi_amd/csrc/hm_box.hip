@@ -823,14 +823,27 @@ template <typename T> struct ShellMaskSrc {
         const int sg = (m <= M[a] / 2) ? m : m - M[a];
         return sg >= 0 ? sg : sg + n[a];
     }
-    __device__ inline C<T> operator()(int f, int o, int b, int p, long long) const {
-#pragma clang fp contract(off)
+    // two phases (the column kernel loads every point's pair before it cooks
+    // any): delta/W (D covers the whole mesh) and the shell-id range of the cell
+    struct Raw {
+        C<T> d;
+        unsigned r;  // the char2 as loaded (16 bits, zero-extended), unpacked in cook
+    };
+    __device__ inline Raw pre(int, int o, int b, int p) const {
         const int sx = support_unmap(p, s[0], M[0]), sy = support_unmap(o, s[1], M[1]),
                   sz = support_unmap(b, s[2], M[2]);
-        const char2 r = sid[((long long)sx * cs[1] + sy) * cs[2] + sz];
+        const unsigned short* sr = reinterpret_cast<const unsigned short*>(sid);
+        return Raw{D[((long long)p * M[1] + o) * M[2] + b], (unsigned)sr[((long long)sx * cs[1] + sy) * cs[2] + sz]};
+    }
+    __device__ inline C<T> operator()(int f, int o, int b, int p, long long) const {
+        return cook(pre(f, o, b, p), f, o, b, p);
+    }
+    __device__ inline C<T> cook(const Raw& w, int f, int o, int b, int p) const {
+#pragma clang fp contract(off)
+        const C<T> d = w.d;
+        const int rlo = (int)(signed char)(w.r & 0xffu), rhi = (int)(signed char)(w.r >> 8);  // char2 x, y
         const int sh = ylm ? f % nk : f;
-        if (sh < (int)r.x || sh > (int)r.y) return mk<T>((T)0, (T)0);
-        const C<T> d = D[((long long)p * M[1] + o) * M[2] + b];
+        if (sh < rlo || sh > rhi) return mk<T>((T)0, (T)0);
         if (!ylm) return d;
         const bool second = f >= nk;
         const Cplx y = ylm_red(second ? l2 : l1, second ? m2 : m1, t.kv[0][to_n(p, 0)], t.kv[1][to_n(o, 1)],

@@ -18,6 +18,8 @@
 
 #include "hm_common.hpp"
 
+#include <type_traits>
+
 namespace hm {
 
 constexpr int FFT_NT = 256;  // lanes per workgroup
@@ -393,6 +395,13 @@ __device__ __forceinline__ uint3 xcd_block() {
 // Column FFT: grid.x = tiles of B compact inner lines, grid.y = compact outer index.
 // BW: columns per tile (0: ColTile's width).  The transform of each column is
 // the same arithmetic for any width; only the tiling of the grid changes.
+// A column source may split into pre(f, o, b, p) -> Raw (its loads only) and
+// cook(raw, f, o, b, p) -> value (its arithmetic); k_fft_cols then issues the
+// loads of all of a lane's points before cooking any.  Sources without a Raw
+// type are called once per point.
+template <class S, class = void> struct TwoPhaseSrc : std::false_type {};
+template <class S> struct TwoPhaseSrc<S, std::void_t<typename S::Raw>> : std::true_type {};
+
 template <typename T, int L, int DIR, class Src, int BW = 0>
 __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__ dst, ColGeom g,
                                                     const C<T>* __restrict__ tw) {
@@ -429,12 +438,26 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     constexpr int NI = (L + PSTEP - 1) / PSTEP;
     const int p0 = tid / B;
     C<T> buf[NI];
+    if constexpr (TwoPhaseSrc<Src>::value) {
+        // the source's loads (pre) for every point first, its arithmetic (cook) after
+        typename Src::Raw raw[NI];
+        bool use[NI];
 #pragma unroll
-    for (int i = 0; i < NI; i++) {
-        const int p = p0 + i * PSTEP;
-        C<T> v = mk<T>((T)0, (T)0);
-        if (p < L && bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
-        buf[i] = v;
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            use[i] = p < L && bvalid && in_support(p, point_s, L);
+            if (use[i]) raw[i] = src.pre(f, o, b, p);
+        }
+#pragma unroll
+        for (int i = 0; i < NI; i++) buf[i] = use[i] ? src.cook(raw[i], f, o, b, p0 + i * PSTEP) : mk<T>((T)0, (T)0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            C<T> v = mk<T>((T)0, (T)0);
+            if (p < L && bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
+            buf[i] = v;
+        }
     }
 #pragma unroll
     for (int i = 0; i < NI; i++) {
@@ -497,11 +520,12 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows_f(RSrc src, C<T>* __restric
     C<T> buf[NE];
 #pragma unroll
     for (int i = 0; i < NE; i++) {
+        // rows past the end read the last row again (no branch around the load) and are zeroed
         const int e = threadIdx.x + i * FFT_NT;
         const int b = e / L, p = e % L;
-        C<T> v = mk<T>((T)0, (T)0);
-        if (e < B * L && r0 + b < g.n_rows) v = src(r0 + b, p);
-        buf[i] = v;
+        const bool in = ((B * L) % FFT_NT == 0 || e < B * L) && r0 + b < g.n_rows;
+        const C<T> v = src(in ? r0 + b : g.n_rows - 1, in ? p : 0);
+        buf[i] = in ? v : mk<T>((T)0, (T)0);
     }
 #pragma unroll
     for (int i = 0; i < NE; i++) {
@@ -531,11 +555,12 @@ __global__ __launch_bounds__(FFT_NT) void k_r2c_rows(RSrc src, C<T>* __restrict_
     C<T> buf[NE];
 #pragma unroll
     for (int i = 0; i < NE; i++) {
+        // rows past the end read the last row again (no branch around the load) and are zeroed
         const int e = threadIdx.x + i * FFT_NT;
         const int b = e / L, p = e % L;
-        C<T> v = mk<T>((T)0, (T)0);
-        if (e < B * L && r0 + b < n_rows) v = src(r0 + b, p);
-        buf[i] = v;
+        const bool in = ((B * L) % FFT_NT == 0 || e < B * L) && r0 + b < n_rows;
+        const C<T> v = src(in ? r0 + b : n_rows - 1, in ? p : 0);
+        buf[i] = in ? v : mk<T>((T)0, (T)0);
     }
 #pragma unroll
     for (int i = 0; i < NE; i++) {
