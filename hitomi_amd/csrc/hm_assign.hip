@@ -267,25 +267,33 @@ __global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ sta
     __shared__ int rbeg[NR], roff[NR + 1];
     const int x0 = blockIdx.z * GX, y0 = blockIdx.y * GY, z0 = blockIdx.x * GZ;
     for (int e = threadIdx.x; e < (CPLX ? 2 * NC : NC); e += 256) acc[e] = 0ull;
-    if (threadIdx.x < NR) {
+    // the first wave finds each extended row's run and scans the run lengths (NR <= 64)
+    static_assert(NR <= 64, "one lane per extended row");
+    if (threadIdx.x < 64) {
         const int r = threadIdx.x;
-        const int xx = x0 - (S - 1) + r / RY, yy = y0 - (S - 1) + r % RY;
-        const int zlo = (z0 - (S - 1) > 0) ? z0 - (S - 1) : 0;
-        const int zhi = (z0 + GZ - 1 < g.n[2] - S) ? z0 + GZ - 1 : g.n[2] - S;
         int b = 0, n = 0;
-        if (xx >= 0 && xx <= g.n[0] - S && yy >= 0 && yy <= g.n[1] - S && zlo <= zhi) {
-            const long long lo = ((long long)xx * g.n[1] + yy) * chunks_z(g);
-            const int cl = zlo / GZ, ch = zhi / GZ;
-            b = start[2 * (lo + cl) + (cl < z0 / GZ ? 1 : 0)];
-            n = start[2 * (lo + ch) + 2] - b;
+        if (r < NR) {
+            const int xx = x0 - (S - 1) + r / RY, yy = y0 - (S - 1) + r % RY;
+            const int zlo = (z0 - (S - 1) > 0) ? z0 - (S - 1) : 0;
+            const int zhi = (z0 + GZ - 1 < g.n[2] - S) ? z0 + GZ - 1 : g.n[2] - S;
+            if (xx >= 0 && xx <= g.n[0] - S && yy >= 0 && yy <= g.n[1] - S && zlo <= zhi) {
+                const long long lo = ((long long)xx * g.n[1] + yy) * chunks_z(g);
+                const int cl = zlo / GZ, ch = zhi / GZ;
+                b = start[2 * (lo + cl) + (cl < z0 / GZ ? 1 : 0)];
+                n = start[2 * (lo + ch) + 2] - b;
+            }
         }
-        rbeg[r] = b;
-        roff[r + 1] = n;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        roff[0] = 0;
-        for (int r = 0; r < NR; r++) roff[r + 1] += roff[r];
+        int v = n;  // inclusive scan over the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(v, off);
+            if (r >= off) v += u;
+        }
+        if (r < NR) {
+            rbeg[r] = b;
+            roff[r + 1] = v;
+        }
+        if (r == 0) roff[0] = 0;
     }
     __syncthreads();
     const int total = roff[NR];
