@@ -315,6 +315,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
     // all of the lane's points are loaded before any is written to LDS (loads in flight together)
     constexpr int NI = (L + PSTEP - 1) / PSTEP;
     const int p0 = tid / B;
+    C<T>* stw = sm + B * LP;  // the twiddle table, staged once for both transforms
     {
         C<T> buf[NI];
 #pragma unroll
@@ -322,6 +323,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
             const int p = p0 + i * PSTEP;
             buf[i] = (p < L && bvalid) ? dk[b + (long long)p * PS] : mk<T>((T)0, (T)0);
         }
+        for (int e = tid; e < L; e += FFT_NT) stw[e] = tw[e];
 #pragma unroll
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
         }
     }
     __syncthreads();
-    lds_fft<T, -1, L, B>(sm, tw);
+    lds_fft<T, -1, L, B>(sm, stw);
     const int j = b / xs.nzh, k = b % xs.nzh;
     const bool form = bvalid && xs.fast_ok(k);
     // each lane reads and rewrites only the points it stores
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
         sm[bl * LP + lpos(p)] = form ? xs.fast(p, j, k, d) : mk<T>((T)0, (T)0);
     }
     __syncthreads();
-    lds_fft<T, +1, L, B>(sm, tw);
+    lds_fft<T, +1, L, B>(sm, stw);
     if (bvalid)
         for (int p = tid / B; p < L; p += PSTEP) xh[b + (long long)p * PS] = sm[bl * LP + lpos(p)];
 }

@@ -438,6 +438,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     constexpr int NI = (L + PSTEP - 1) / PSTEP;
     const int p0 = tid / B;
     C<T> buf[NI];
+    C<T>* stw = sm + B * LP;  // the twiddle table, staged while the tile's loads are in flight
     if constexpr (TwoPhaseSrc<Src>::value) {
         // the source's loads (pre) for every point first, its arithmetic (cook) after
         typename Src::Raw raw[NI];
@@ -459,13 +460,14 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
             buf[i] = v;
         }
     }
+    for (int e = tid; e < L; e += FFT_NT) stw[e] = tw[e];
 #pragma unroll
     for (int i = 0; i < NI; i++) {
         const int p = p0 + i * PSTEP;
         if (p < L) sm[bl * LP + lpos(p)] = buf[i];
     }
     __syncthreads();
-    lds_fft<T, DIR, L, B>(sm, tw);
+    lds_fft<T, DIR, L, B>(sm, stw);
     if (bvalid) {
         for (int p = tid / B; p < L; p += PSTEP)
             if (in_support(p, g.out_s, L)) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + lpos(p)];
@@ -493,13 +495,15 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows(const C<T>* __restrict__ sr
             v = src[(r0 + b) * g.RS_in + p];
         buf[i] = v;
     }
+    C<T>* stw = sm + B * LP;  // the twiddle table, staged while the tile's loads are in flight
+    for (int e = threadIdx.x; e < L; e += FFT_NT) stw[e] = tw[e];
 #pragma unroll
     for (int i = 0; i < NE; i++) {
         const int e = threadIdx.x + i * FFT_NT;
         if (e < B * L) sm[(e / L) * LP + lpos(e % L)] = buf[i];
     }
     __syncthreads();
-    lds_fft<T, DIR, L, B>(sm, tw);
+    lds_fft<T, DIR, L, B>(sm, stw);
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, p = e % L;
         if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
@@ -527,13 +531,15 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows_f(RSrc src, C<T>* __restric
         const C<T> v = src(in ? r0 + b : g.n_rows - 1, in ? p : 0);
         buf[i] = in ? v : mk<T>((T)0, (T)0);
     }
+    C<T>* stw = sm + B * LP;  // the twiddle table, staged while the tile's loads are in flight
+    for (int e = threadIdx.x; e < L; e += FFT_NT) stw[e] = tw[e];
 #pragma unroll
     for (int i = 0; i < NE; i++) {
         const int e = threadIdx.x + i * FFT_NT;
         if (e < B * L) sm[(e / L) * LP + lpos(e % L)] = buf[i];
     }
     __syncthreads();
-    lds_fft<T, DIR, L, B>(sm, tw);
+    lds_fft<T, DIR, L, B>(sm, stw);
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, p = e % L;
         if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
@@ -562,13 +568,15 @@ __global__ __launch_bounds__(FFT_NT) void k_r2c_rows(RSrc src, C<T>* __restrict_
         const C<T> v = src(in ? r0 + b : n_rows - 1, in ? p : 0);
         buf[i] = in ? v : mk<T>((T)0, (T)0);
     }
+    C<T>* stw = sm + B * LP;  // the twiddle table, staged while the tile's loads are in flight
+    for (int e = threadIdx.x; e < L; e += FFT_NT) stw[e] = tw[e];
 #pragma unroll
     for (int i = 0; i < NE; i++) {
         const int e = threadIdx.x + i * FFT_NT;
         if (e < B * L) sm[(e / L) * LP + lpos(e % L)] = buf[i];
     }
     __syncthreads();
-    lds_fft<T, -1, L, B>(sm, tw);
+    lds_fft<T, -1, L, B>(sm, stw);
     // X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[L-k]) / 2, O = (Z[k] - conj Z[L-k]) / (2i)
     for (int e = threadIdx.x; e < B * (L + 1); e += FFT_NT) {
         const int b = e / (L + 1), k = e % (L + 1);
@@ -593,6 +601,8 @@ __global__ __launch_bounds__(FFT_NT) void k_c2r_rows(HSrc src, T* __restrict__ d
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
     const long long r0 = (long long)blockIdx.x * B;
+    C<T>* stw = sm + B * LP;  // the twiddle table (read by every stage)
+    for (int e = threadIdx.x; e < L; e += FFT_NT) stw[e] = tw[e];
     // chunks of CH elements per lane: the chunk's loads are issued before any is used
     constexpr int NE = (B * L + FFT_NT - 1) / FFT_NT, CH = NE < 8 ? NE : 8;
 #pragma unroll
@@ -625,7 +635,7 @@ __global__ __launch_bounds__(FFT_NT) void k_c2r_rows(HSrc src, T* __restrict__ d
         }
     }
     __syncthreads();
-    lds_fft<T, +1, L, B>(sm, tw);
+    lds_fft<T, +1, L, B>(sm, stw);
     for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
         const int b = e / L, m = e % L;
         if (r0 + b < n_rows) {
@@ -654,13 +664,14 @@ template <typename T> struct RealSrc {
 };
 
 // LDS bytes of the three tile shapes.
-template <typename T, int L> constexpr size_t col_lds() { return (size_t)ColTile<T, L>::B * LdsPad<L>::LP * sizeof(C<T>); }
+// the tile, then the length's twiddle table (staged once, read by every stage)
+template <typename T, int L> constexpr size_t col_lds() { return ((size_t)ColTile<T, L>::B * LdsPad<L>::LP + L) * sizeof(C<T>); }
 // the wide column tile (twice ColTile's width) of the FP32 lengths the box
 // plans use most; the autotuner chooses per length whether to use it
 template <typename T, int L> constexpr bool has_wide_cols() {
     return sizeof(T) == 4 && (L == 256 || L == 270 || L == 288 || L == 512);
 }
-template <typename T, int L> constexpr size_t row_lds() { return (size_t)RowTile<T, L>::B * LdsPad<L>::LP * sizeof(C<T>); }
+template <typename T, int L> constexpr size_t row_lds() { return ((size_t)RowTile<T, L>::B * LdsPad<L>::LP + L) * sizeof(C<T>); }
 
 // ---------------------------------------------------------------------------
 // Any length: the radix plan is a kernel argument.  Lengths without a

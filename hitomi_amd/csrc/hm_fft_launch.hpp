@@ -21,7 +21,7 @@ int launch_cols_L(hm_ctx* ctx, const char* name, const Src& src, C<T>* dst, cons
     if (!tw) return -1;
     if constexpr (has_wide_cols<T, L>()) {
         constexpr int BW = 2 * B0;
-        const size_t lds = (size_t)BW * LdsPad<L>::LP * sizeof(C<T>);
+        const size_t lds = ((size_t)BW * LdsPad<L>::LP + L) * sizeof(C<T>);  // tile + twiddle table
         if (ctx->wide_cols(L) && lds <= ctx->lds_max) {
             dim3 grid((unsigned)((g.n_inner + BW - 1) / BW), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
             return ctx->launch(name, [&] {
