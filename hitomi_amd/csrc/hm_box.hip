@@ -219,19 +219,27 @@ template <typename T> struct XiSrc {
     // inside 0 < k_z < n_z/2 the spectrum at -k is the conjugate, so with p2 = |d|^2 the two sides
     // combine to Re = (p2 (Q1 + Q2) - sn (S1 Q1 + S2 Q2)) / 2V (even part) and
     // Im = -(p2 (Q1 - Q2) - sn (S1 Q1 - S2 Q2)) / 2V (odd part); Q, S separable per axis
-    __device__ inline C<T> fast(int i, int j, int k, C<T> d) const {
+    // the y, z factors of the fast form (the same for a whole x line) and the x table entry of a point
+    struct FastYZ {
+        double qa, qb, sa, sb, snv;
+    };
+    __device__ inline FastYZ fast_yz(int j, int k) const {
 #pragma clang fp contract(off)
-        const double re = (double)d.x, im = (double)d.y;
         const int jj = j ? n[1] - j : 0, kk = n[2] - k;
-        const double4 q = *reinterpret_cast<const double4*>(xq + 4 * (long long)i);
         const double qa = t.iw[1][j] * t.iw[2][k], qb = t.iw[1][jj] * t.iw[2][kk];
         const double sa = (t.sf[1][j] * t.sf[2][k]) * qa, sb = (t.sf[1][jj] * t.sf[2][kk]) * qb;
-        const double p2 = re * re + im * im;
-        const double snv = sn_ptr ? *sn_ptr : sn;
-        const double Q1 = q.x * qa, Q2 = q.y * qb, SQ1 = q.z * sa, SQ2 = q.w * sb;
-        if (odd) return mk<T>((T)0, (T)(-(0.5 * inv_v) * (p2 * (Q1 - Q2) - snv * (SQ1 - SQ2))));
-        return mk<T>((T)((0.5 * inv_v) * (p2 * (Q1 + Q2) - snv * (SQ1 + SQ2))), (T)0);
+        return FastYZ{qa, qb, sa, sb, sn_ptr ? *sn_ptr : sn};
     }
+    __device__ inline double4 fast_x(int i) const { return *reinterpret_cast<const double4*>(xq + 4 * (long long)i); }
+    __device__ inline C<T> fast_q(const double4& q, const FastYZ& f, C<T> d) const {
+#pragma clang fp contract(off)
+        const double re = (double)d.x, im = (double)d.y;
+        const double p2 = re * re + im * im;
+        const double Q1 = q.x * f.qa, Q2 = q.y * f.qb, SQ1 = q.z * f.sa, SQ2 = q.w * f.sb;
+        if (odd) return mk<T>((T)0, (T)(-(0.5 * inv_v) * (p2 * (Q1 - Q2) - f.snv * (SQ1 - SQ2))));
+        return mk<T>((T)((0.5 * inv_v) * (p2 * (Q1 + Q2) - f.snv * (SQ1 + SQ2))), (T)0);
+    }
+    __device__ inline C<T> fast(int i, int j, int k, C<T> d) const { return fast_q(fast_x(i), fast_yz(j, k), d); }
     // the value at mode (i, j, k) stored at addr
     __device__ inline C<T> at(int i, int j, int k, long long addr) const {
 #pragma clang fp contract(off)
@@ -334,19 +342,44 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
     lds_fft<T, -1, L, B>(sm, stw);
     const int j = b / xs.nzh, k = b % xs.nzh;
     const bool form = bvalid && xs.fast_ok(k);
-    // each lane reads and rewrites only the points it stores
+    const typename XiSrc<T>::FastYZ yz = form ? xs.fast_yz(j, k) : typename XiSrc<T>::FastYZ{0.0, 0.0, 0.0, 0.0, 0.0};
+    // each lane reads and rewrites only the points it stores; two points at a time: their LDS values and
+    // x table entries are read before any store
+    constexpr int CH = 2;
 #pragma unroll
-    for (int i = 0; i < NI; i++) {
-        const int p = p0 + i * PSTEP;
-        if (p >= L) continue;
-        const C<T> d = sm[bl * LP + lpos(p)];
-        if (bvalid) dk[b + (long long)p * PS] = d;
-        sm[bl * LP + lpos(p)] = form ? xs.fast(p, j, k, d) : mk<T>((T)0, (T)0);
+    for (int i0 = 0; i0 < NI; i0 += CH) {
+        C<T> dv[CH];
+        double4 qv[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int p = p0 + (i0 + u) * PSTEP;
+            const bool ok = i0 + u < NI && p < L;
+            dv[u] = ok ? sm[bl * LP + lpos(p)] : mk<T>((T)0, (T)0);
+            qv[u] = (ok && form) ? xs.fast_x(p) : make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int p = p0 + (i0 + u) * PSTEP;
+            if (i0 + u >= NI || p >= L) continue;
+            if (bvalid) dk[b + (long long)p * PS] = dv[u];
+            sm[bl * LP + lpos(p)] = form ? xs.fast_q(qv[u], yz, dv[u]) : mk<T>((T)0, (T)0);
+        }
     }
     __syncthreads();
     lds_fft<T, +1, L, B>(sm, stw);
-    if (bvalid)
-        for (int p = tid / B; p < L; p += PSTEP) xh[b + (long long)p * PS] = sm[bl * LP + lpos(p)];
+    if (bvalid) {
+        C<T> out[NI];
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            if (p < L) out[i] = sm[bl * LP + lpos(p)];
+        }
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            if (p < L) xh[b + (long long)p * PS] = out[i];
+        }
+    }
 }
 
 // The fused pass for a compile-time length; 1 when it ran, 0 when L has no

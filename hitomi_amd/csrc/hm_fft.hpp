@@ -469,8 +469,17 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, stw);
     if (bvalid) {
-        for (int p = tid / B; p < L; p += PSTEP)
-            if (in_support(p, g.out_s, L)) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + lpos(p)];
+        // all of the lane's points read from LDS first, then stored
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            if (p < L) buf[i] = sm[bl * LP + lpos(p)];
+        }
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int p = p0 + i * PSTEP;
+            if (p < L && in_support(p, g.out_s, L)) dst[obase + b + (long long)p * g.PS] = buf[i];
+        }
     }
 }
 
@@ -504,9 +513,16 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows(const C<T>* __restrict__ sr
     }
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, stw);
-    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
+        if (e < B * L) buf[i] = sm[(e / L) * LP + lpos(e % L)];
+    }
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
         const int b = e / L, p = e % L;
-        if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
+        if (e < B * L && r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = buf[i];
     }
 }
 
@@ -540,9 +556,16 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_rows_f(RSrc src, C<T>* __restric
     }
     __syncthreads();
     lds_fft<T, DIR, L, B>(sm, stw);
-    for (int e = threadIdx.x; e < B * L; e += FFT_NT) {
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
+        if (e < B * L) buf[i] = sm[(e / L) * LP + lpos(e % L)];
+    }
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = threadIdx.x + i * FFT_NT;
         const int b = e / L, p = e % L;
-        if (r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = sm[b * LP + lpos(p)];
+        if (e < B * L && r0 + b < g.n_rows && in_support(p, g.out_s, L)) dst[(r0 + b) * g.RS_out + p] = buf[i];
     }
 }
 
