@@ -81,6 +81,11 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
     long long* Cn = cnt + cb;
     const long long rbase = ((long long)i * a.n[1] + j) * a.nzh;
     const int im = (a.n[0] - i) % a.n[0], jm = (a.n[1] - j) % a.n[1];
+    // the row's x and y table values (the loop's stores to the partial sums could alias the tables, so the
+    // compiler would reload them for every kz)
+    const double kxv[2] = {a.t.kv[0][i], a.t.kv[0][im]}, kyv[2] = {a.t.kv[1][j], a.t.kv[1][jm]};
+    const double wxv[2] = {a.t.wf[0][i], a.t.wf[0][im]}, wyv[2] = {a.t.wf[1][j], a.t.wf[1][jm]};
+    const double sxv[2] = {a.t.sf[0][i], a.t.sf[0][im]}, syv[2] = {a.t.sf[1][j], a.t.sf[1][jm]};
     for (int kz = kz0; kz <= kz1; kz++) {
         const C<T> d = dk[rbase + kz];
         const double re = (double)d.x, imv = (double)d.y;
@@ -92,15 +97,15 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
         }
         const int nmir = (kz > 0 && kz < a.n[2] - kz) ? 2 : 1;
         for (int m = 0; m < nmir; m++) {
-            const int ii = m ? im : i, jj = m ? jm : j, kk = m ? (a.n[2] - kz) : kz;
-            const double kx = a.t.kv[0][ii], ky = a.t.kv[1][jj], kzv = a.t.kv[2][kk];
+            const int kk = m ? (a.n[2] - kz) : kz;
+            const double kx = kxv[m], ky = kyv[m], kzv = a.t.kv[2][kk];
             const double kmag = kmag3(kx, ky, kzv);
             const int kindex = (int)(kmag / FINE_DK + 0.5);
             if (kindex >= FINE_N) continue;
             const int2 b = a.f2b[kindex];
             if (b.x < 0) continue;
-            const double W = wpow((a.t.wf[0][ii] * a.t.wf[1][jj]) * a.t.wf[2][kk], a.wp);
-            const double S = a.sn_w2 ? W * W : (a.t.sf[0][ii] * a.t.sf[1][jj]) * a.t.sf[2][kk];
+            const double W = wpow((wxv[m] * wyv[m]) * a.t.wf[2][kk], a.wp);
+            const double S = a.sn_w2 ? W * W : (sxv[m] * syv[m]) * a.t.sf[2][kk];
             const double val = (p2 - sn * S) / (W * W);
             const double vim = (m ? -pim : pim) / (W * W);
             const double mu = (fabs(kmag) < 1.0e-15) ? 0.0 : kzv / kmag;
