@@ -1491,7 +1491,7 @@ __global__ __launch_bounds__(256) void k_full_over_w(const C<T>* __restrict__ f,
 struct BoxPlanImpl {
     hm_box_config cfg;
     // launch shapes (the autotuner's choice; results do not depend on them)
-    unsigned tune_wide = 0;   // wide column tiles, bit per tunable length (256, 270 or 288, 512)
+    unsigned tune_wide = 0;   // wide column tiles, bit per tunable length (256, 270 or 288, 512; 3: the shell x pass)
     int tune_bin = 256;       // threads per block of the P_l binning
     Mesh mesh;
     int nk = 0, ne = 0;
@@ -3510,7 +3510,8 @@ int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np
         if (b0 >= 0) avail |= 1u << b0;
         if (b1 >= 0) avail |= 1u << b1;
     }
-    if (p->cfg.precision == HM_FP64) avail = 0;  // the wide tiles exist for FP32 only
+    if ((avail & 2u) && p->cfg.bk) avail |= 8u;  // the triple stage's shell x pass on its own
+    if (p->cfg.precision == HM_FP64) avail = 0;   // the wide tiles exist for FP32 only
     const int bins[3] = {256, 128, 64};
     const unsigned keep_wide = p->tune_wide;
     const int keep_bin = p->tune_bin;
@@ -3521,7 +3522,7 @@ int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np
     report.clear();
     snprintf(buf, sizeof(buf), "lds per workgroup %zu B; ", ctx->lds_max);
     report += buf;
-    for (unsigned w = 0; w < 8; w++) {
+    for (unsigned w = 0; w < 16; w++) {
         if (w & ~avail) continue;
         for (int bb : bins) {
             p->tune_wide = w;
@@ -3540,7 +3541,8 @@ int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np
                 }
             const double ms =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
-            snprintf(buf, sizeof(buf), "wide=%u%u%u bin=%d: %.3f ms; ", w & 1u, (w >> 1) & 1u, (w >> 2) & 1u, bb, ms);
+            snprintf(buf, sizeof(buf), "wide=%u%u%u%u bin=%d: %.3f ms; ", w & 1u, (w >> 1) & 1u, (w >> 2) & 1u,
+                     (w >> 3) & 1u, bb, ms);
             report += buf;
             if (ms < best) {
                 best = ms;
@@ -3551,8 +3553,8 @@ int plan_autotune(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long long np
     }
     p->tune_wide = best_wide;
     p->tune_bin = best_bin;
-    snprintf(buf, sizeof(buf), "chosen: wide=%u%u%u (256, 270/288, 512) bin=%d, %.3f ms", best_wide & 1u,
-             (best_wide >> 1) & 1u, (best_wide >> 2) & 1u, best_bin, best);
+    snprintf(buf, sizeof(buf), "chosen: wide=%u%u%u%u (256, 270/288, 512, shell x pass) bin=%d, %.3f ms",
+             best_wide & 1u, (best_wide >> 1) & 1u, (best_wide >> 2) & 1u, (best_wide >> 3) & 1u, best_bin, best);
     report += buf;
     return 0;
 }
@@ -3562,7 +3564,7 @@ void plan_tune_get(const BoxPlanImpl* p, unsigned* wide, int* bin) {
     *bin = p->tune_bin;
 }
 void plan_tune_set(BoxPlanImpl* p, unsigned wide, int bin) {
-    p->tune_wide = wide & 7u;
+    p->tune_wide = wide & 15u;
     p->tune_bin = (bin == 64 || bin == 128) ? bin : 256;
 }
 

@@ -22,7 +22,7 @@ int launch_cols_L(hm_ctx* ctx, const char* name, const Src& src, C<T>* dst, cons
     if constexpr (has_wide_cols<T, L>()) {
         constexpr int BW = 2 * B0;
         const size_t lds = ((size_t)BW * LdsPad<L>::LP + L) * sizeof(C<T>);  // tile + twiddle table
-        if (ctx->wide_cols(L) && lds <= ctx->lds_max) {
+        if (ctx->wide_cols(L, name) && lds <= ctx->lds_max) {
             dim3 grid((unsigned)((g.n_inner + BW - 1) / BW), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
             return ctx->launch(name, [&] {
                 hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src, BW>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g,

@@ -6,6 +6,7 @@
 #include "hm_fft.hpp"
 #include "../../include/hitomi_amd.h"
 
+#include <cstring>
 #include <functional>
 #include <memory>
 
@@ -32,13 +33,15 @@ struct hm_ctx {
     std::map<std::string, std::vector<std::pair<double, double>>> kspans;
     bool capturing = false;
     // launch shapes chosen by an autotuner: bit b of wide_mask set = the wide
-    // column tile for the b-th tunable length (256, 288, 512)
+    // column tile for the b-th tunable length (256; 270 or 288; 512), bit 3 the
+    // wide tile of the triple stage's shell x pass (270 or 288) on its own
     unsigned wide_mask = 0;
     size_t lds_max = 65536;  // dynamic LDS per workgroup the device allows
     int cu_count = 256;      // compute units of the device
     bool conv_lds = true;    // convolutions of the learned reconstruction with LDS-staged input tiles
-    bool wide_cols(int L) const {
-        const int b = L == 256 ? 0 : ((L == 288 || L == 270) ? 1 : (L == 512 ? 2 : -1));
+    bool wide_cols(int L, const char* pass = nullptr) const {
+        int b = L == 256 ? 0 : ((L == 288 || L == 270) ? 1 : (L == 512 ? 2 : -1));
+        if (b == 1 && pass && std::strcmp(pass, "tri_pass_x") == 0) b = 3;
         return b >= 0 && ((wide_mask >> b) & 1u);
     }
     // device workspace of the mock generator, kept between calls

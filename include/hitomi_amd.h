@@ -270,7 +270,8 @@ size_t hm_box_plan_bytes(const hm_box_plan *plan);
 int hm_box_plan_triple_mesh(const hm_box_plan *plan, int m[3]);
 /* Autotuner (no counterpart in the reference): times every combination of
  * the launch shapes it can vary -- the wide (2x) column tile of the FP32
- * column passes of length 256, 270 or 288, and 512 (bit 0, 1, 2 of wide_mask, used
+ * column passes of length 256, 270 or 288, and 512 (bit 0, 1, 2 of wide_mask;
+ * bit 3 the triple stage's shell x pass of length 270 or 288 on its own; used
  * when the device allows the LDS) and 64, 128 or 256 threads per block of the
  * P_l binning -- over reps executions of d_pos, and keeps the fastest for
  * this plan.  The results do not depend on these shapes.  report (may be

@@ -22,7 +22,7 @@ def test_plan_shapes_do_not_change_results(ctx, mock):
     assert plan.triple_mesh == (270, 270, 270)
     assert plan.shapes == (0, 256)
     ref = plan.execute(mock)
-    for shapes in [(0b110, 64), (0b010, 128), (0b100, 256)]:
+    for shapes in [(0b110, 64), (0b010, 128), (0b100, 256), (0b1000, 256), (0b1110, 128)]:
         plan.shapes = shapes
         assert plan.shapes == shapes
         out = plan.execute(mock)
@@ -30,7 +30,7 @@ def test_plan_shapes_do_not_change_results(ctx, mock):
             assert np.array_equal(out[k], ref[k]), (shapes, k)
     report = plan.autotune(mock, reps=1)
     print(report)
-    assert "chosen" in report and report.count("ms;") == 12
+    assert "chosen" in report and report.count("ms;") == 24  # (270/288, 512, shell x pass) x 3 block sizes
     out = plan.execute(mock)
     for k in ref:
         assert np.array_equal(out[k], ref[k]), k
