@@ -394,28 +394,6 @@ struct Mesh3 {
     double L[3];
 };
 
-// TSC stencil of the reference's reconstruction gather (particle_reconstruction.hpp:141-205,
-// 334-388): no wrap, a stencil point whose linear index leaves [0, n^3) is
-// skipped.  The library's sorted assignment uses the same points and weights,
-// so it is this gather's exact adjoint.
-__device__ __forceinline__ int tsc_ref(const Mesh3& g, const double* p, long long lin[27], double w[27]) {
-    double wa[3][3];
-    int i0[3];
-    for (int a = 0; a < 3; a++) stencil_axis(2, axis_xp(p[a], g.n[a], g.L[a]), &i0[a], wa[a]);
-    const long long ng = (long long)g.n[0] * g.n[1] * g.n[2];
-    int c = 0;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) {
-                const long long v = ((long long)(i0[0] + i) * g.n[1] + (i0[1] + j)) * g.n[2] + (i0[2] + k);
-                if (v < 0 || v >= ng) continue;
-                lin[c] = v;
-                w[c] = (wa[0][i] * wa[1][j]) * wa[2][k];
-                c++;
-            }
-    return c;
-}
-
 // S_i = plain (channels 0..2 of feat) [+ psi_scale out]; with target: |S - T|^2
 // per block and, with wxy / wz, the scaled residuals as the weights of the
 // loss gradient's assignment: (coef r_x, coef r_y) and (coef r_z, 0), r
@@ -431,20 +409,40 @@ __global__ __launch_bounds__(256) void k_shift_loss(Mesh3 g, const double* __res
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     double l0 = 0.0, l1 = 0.0;
     if (i < np) {
-        long long lin[27];
-        double w[27];
-        const int cnt = tsc_ref(g, pos + 3 * i, lin, w);
+        // the TSC stencil of the reference's reconstruction gather (particle_reconstruction.hpp:141-205,
+        // 334-388): the 27 cells in (x, y, z) order, no wrap, a cell whose linear index leaves [0, n^3)
+        // skipped (the library's sorted assignment uses the same cells and weights, so it is this gather's
+        // exact adjoint); one x plane (9 cells) at a time with the plane's loads issued before any is added
+        // (a 27-entry index list would sit in scratch memory)
+        double wa[3][3];
+        int i0[3];
+        for (int a = 0; a < 3; a++) stencil_axis(2, axis_xp(pos[3 * i + a], g.n[a], g.L[a]), &i0[a], wa[a]);
+        const long long ng = (long long)g.n[0] * g.n[1] * g.n[2];
         double sp[3] = {0.0, 0.0, 0.0}, so[3] = {0.0, 0.0, 0.0};
-        for (int c = 0; c < cnt; c++) {
-            const float4 f = feat[lin[c]];
-            sp[0] += w[c] * (double)f.x;
-            sp[1] += w[c] * (double)f.y;
-            sp[2] += w[c] * (double)f.z;
-            if (out) {
-                const float4 o = out[lin[c]];
-                so[0] += w[c] * (double)o.x;
-                so[1] += w[c] * (double)o.y;
-                so[2] += w[c] * (double)o.z;
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) {
+            float4 fv[9], ov[9];
+            bool ok[9];
+#pragma unroll
+            for (int jk = 0; jk < 9; jk++) {
+                const int j = jk / 3, k = jk % 3;
+                const long long v = ((long long)(i0[0] + ix) * g.n[1] + (i0[1] + j)) * g.n[2] + (i0[2] + k);
+                ok[jk] = v >= 0 && v < ng;
+                fv[jk] = ok[jk] ? feat[v] : make_float4(0.f, 0.f, 0.f, 0.f);
+                ov[jk] = (ok[jk] && out) ? out[v] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int jk = 0; jk < 9; jk++) {
+                if (!ok[jk]) continue;
+                const double w = (wa[0][ix] * wa[1][jk / 3]) * wa[2][jk % 3];
+                sp[0] += w * (double)fv[jk].x;
+                sp[1] += w * (double)fv[jk].y;
+                sp[2] += w * (double)fv[jk].z;
+                if (out) {
+                    so[0] += w * (double)ov[jk].x;
+                    so[1] += w * (double)ov[jk].y;
+                    so[2] += w * (double)ov[jk].z;
+                }
             }
         }
         double r[3];
