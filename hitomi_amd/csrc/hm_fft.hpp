@@ -630,14 +630,21 @@ __global__ __launch_bounds__(FFT_NT) void k_c2r_rows(HSrc src, T* __restrict__ d
     constexpr int NE = (B * L + FFT_NT - 1) / FFT_NT, CH = NE < 8 ? NE : 8;
 #pragma unroll
     for (int i0 = 0; i0 < NE; i0 += CH) {
-        C<T> xks[CH], xcs[CH];
+        C<T> xks[CH], xcs[CH], tws[CH];
 #pragma unroll
         for (int u = 0; u < CH; u++) {
             const int e = threadIdx.x + (i0 + u) * FFT_NT;
             const int b = e / L, k = e % L, kc = L - k;
-            const bool row = i0 + u < NE && e < B * L && r0 + b < n_rows;
-            xks[u] = (row && (point_s < 0 || k <= point_s)) ? src(r0 + b, k) : mk<T>((T)0, (T)0);
-            xcs[u] = (row && (point_s < 0 || kc <= point_s)) ? cconj<T>(src(r0 + b, kc)) : mk<T>((T)0, (T)0);
+            const bool row = (i0 + u < NE) & (e < B * L) & (r0 + b < n_rows);
+            // unconditional loads at a clamped row (every row holds L + 1 points), so that
+            // the chunk's loads are all in flight together; the masks (bitwise, so that no
+            // branch invites the compiler to sink a load under it) select afterwards
+            const long long rr = row ? r0 + b : n_rows - 1;
+            const C<T> a = src(rr, k), c = src(rr, kc);
+            const bool ka = row & ((point_s < 0) | (k <= point_s)), kb = row & ((point_s < 0) | (kc <= point_s));
+            xks[u] = mk<T>(ka ? a.x : (T)0, ka ? a.y : (T)0);
+            xcs[u] = mk<T>(kb ? c.x : (T)0, kb ? -c.y : (T)0);
+            tws[u] = tw2[k];
         }
 #pragma unroll
         for (int u = 0; u < CH; u++) {
@@ -649,7 +656,7 @@ __global__ __launch_bounds__(FFT_NT) void k_c2r_rows(HSrc src, T* __restrict__ d
                 const C<T> xk = xks[u], xc = xcs[u];
                 const C<T> sum = cadd<T>(xk, xc);
                 const C<T> d = csub<T>(xk, xc);
-                C<T> w = tw2[k];
+                C<T> w = tws[u];
                 w.y = -w.y;  // exp(+2 pi i k / 2L)
                 const C<T> wd = cmul<T>(w, d);
                 v = mk<T>(sum.x - wd.y, sum.y + wd.x);  // sum + i * wd
