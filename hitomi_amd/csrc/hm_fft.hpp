@@ -255,48 +255,119 @@ __device__ __forceinline__ void lds_stockham(C<T>* sm, const C<T>* __restrict__ 
     if constexpr (sizeof...(RS) > 0) lds_stockham<T, DIR, L, B, NT, SW, NS * R, RS...>(sm, tw, tid);
 }
 
-// Radix plans of the supported line lengths.  tid: the lane's index in the
-// workgroup (threadIdx.x unless the caller passes its own copy).
+// Column-minor tile: point p of line b at sm[p * B + b], B a power of two
+// dividing NT.  A lane keeps its line b = tid % B through every stage, the R
+// points of a butterfly lie B * L / R slots apart (immediate offsets from one
+// address), and the 64 lanes of a wavefront read 64 consecutive slots.  The
+// index arithmetic per butterfly is the quotient and remainder of j by NS,
+// where the line-major layout also divides by L / R for the line index.
+template <typename T, int DIR, int L, int B, int NT, int NS, int R, int... RS>
+__device__ __forceinline__ void lds_stockham_cm(C<T>* sm, const C<T>* __restrict__ tw, int tid) {
+    static_assert((B & (B - 1)) == 0 && NT % B == 0, "column-minor tiles need a power-of-two width dividing NT");
+    constexpr int NBF = L / R;
+    constexpr int JS = NT / B;  // butterflies of one line per pass over the workgroup
+    constexpr int Q = (NBF + JS - 1) / JS;
+    const int b = tid % B, j0 = tid / B;
+    C<T> v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int j = j0 + q * JS;
+        if ((NBF % JS == 0) || j < NBF) {
+            const int k = j % NS;
+            const C<T>* src = sm + j * B + b;
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                C<T> a = src[t * NBF * B];
+                if (NS > 1 && t > 0) {
+                    const int e = t * k * (L / (NS * R));
+                    C<T> w = tw[e];
+                    if (DIR > 0) w.y = -w.y;
+                    a = cmul<T>(a, w);
+                }
+                v[q][t] = a;
+            }
+            Dft<T, R, DIR>::run(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int j = j0 + q * JS;
+        if ((NBF % JS == 0) || j < NBF) {
+            C<T>* dst = sm + ((j / NS) * NS * R + j % NS) * B + b;
+#pragma unroll
+            for (int t = 0; t < R; t++) dst[t * NS * B] = v[q][t];
+        }
+    }
+    __syncthreads();
+    if constexpr (sizeof...(RS) > 0) lds_stockham_cm<T, DIR, L, B, NT, NS * R, RS...>(sm, tw, tid);
+}
+
+// Radix plans of the supported line lengths (kFftLengths below lists them).
+template <int... RS> struct Radices {};
+template <int L> struct FftPlan;  // no plan: not a supported length
+#define HM_FFT_PLAN(L, ...) \
+    template <> struct FftPlan<L> { using type = Radices<__VA_ARGS__>; };
+HM_FFT_PLAN(8, 8)
+HM_FFT_PLAN(16, 4, 4)
+HM_FFT_PLAN(32, 8, 4)
+HM_FFT_PLAN(48, 8, 6)
+HM_FFT_PLAN(64, 8, 8)
+HM_FFT_PLAN(80, 8, 2, 5)
+HM_FFT_PLAN(96, 8, 4, 3)
+HM_FFT_PLAN(128, 8, 4, 4)
+HM_FFT_PLAN(160, 8, 4, 5)
+HM_FFT_PLAN(192, 8, 8, 3)
+HM_FFT_PLAN(256, 8, 8, 4)
+HM_FFT_PLAN(288, 8, 6, 6)
+HM_FFT_PLAN(320, 8, 8, 5)
+HM_FFT_PLAN(384, 8, 8, 6)
+HM_FFT_PLAN(480, 8, 4, 3, 5)
+HM_FFT_PLAN(512, 8, 8, 8)
+HM_FFT_PLAN(576, 8, 8, 3, 3)
+HM_FFT_PLAN(640, 8, 8, 2, 5)
+HM_FFT_PLAN(768, 8, 8, 4, 3)
+HM_FFT_PLAN(1024, 8, 8, 4, 4)
+HM_FFT_PLAN(24, 8, 3)
+HM_FFT_PLAN(40, 8, 5)
+HM_FFT_PLAN(72, 8, 3, 3)
+HM_FFT_PLAN(120, 8, 3, 5)
+HM_FFT_PLAN(144, 8, 6, 3)
+HM_FFT_PLAN(200, 8, 5, 5)
+HM_FFT_PLAN(216, 8, 3, 3, 3)
+HM_FFT_PLAN(240, 8, 6, 5)
+HM_FFT_PLAN(360, 8, 3, 3, 5)
+HM_FFT_PLAN(400, 8, 2, 5, 5)
+HM_FFT_PLAN(432, 8, 6, 3, 3)
+HM_FFT_PLAN(600, 8, 3, 5, 5)
+HM_FFT_PLAN(720, 8, 6, 3, 5)
+HM_FFT_PLAN(800, 8, 4, 5, 5)
+HM_FFT_PLAN(960, 8, 8, 3, 5)
+HM_FFT_PLAN(270, 9, 5, 6)
+#undef HM_FFT_PLAN
+
+template <typename T, int DIR, int L, int B, int NT, bool SW, int... RS>
+__device__ __forceinline__ void lds_fft_run(C<T>* sm, const C<T>* __restrict__ tw, int tid, Radices<RS...>) {
+    lds_stockham<T, DIR, L, B, NT, SW, 1, RS...>(sm, tw, tid);
+}
+template <typename T, int DIR, int L, int B, int NT, int... RS>
+__device__ __forceinline__ void lds_fft_cm_run(C<T>* sm, const C<T>* __restrict__ tw, int tid, Radices<RS...>) {
+    lds_stockham_cm<T, DIR, L, B, NT, 1, RS...>(sm, tw, tid);
+}
+
+// Transform of B lines of L points in LDS, line-major (sm[b * LP + lpos(p)]).
+// tid: the lane's index in the workgroup (threadIdx.x unless the caller passes
+// its own copy).
 template <typename T, int DIR, int L, int B, int NT = FFT_NT, bool SW = true>
 __device__ __forceinline__ void lds_fft(C<T>* sm, const C<T>* __restrict__ tw, int tid = -1) {
     if (tid < 0) tid = threadIdx.x;
-    if constexpr (L == 8) lds_stockham<T, DIR, L, B, NT, SW, 1, 8>(sm, tw, tid);
-    else if constexpr (L == 16) lds_stockham<T, DIR, L, B, NT, SW, 1, 4, 4>(sm, tw, tid);
-    else if constexpr (L == 32) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4>(sm, tw, tid);
-    else if constexpr (L == 48) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6>(sm, tw, tid);
-    else if constexpr (L == 64) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8>(sm, tw, tid);
-    else if constexpr (L == 80) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 5>(sm, tw, tid);
-    else if constexpr (L == 96) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3>(sm, tw, tid);
-    else if constexpr (L == 128) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 4>(sm, tw, tid);
-    else if constexpr (L == 160) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 5>(sm, tw, tid);
-    else if constexpr (L == 192) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3>(sm, tw, tid);
-    else if constexpr (L == 256) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 4>(sm, tw, tid);
-    else if constexpr (L == 288) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 6>(sm, tw, tid);
-    else if constexpr (L == 320) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 5>(sm, tw, tid);
-    else if constexpr (L == 384) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 6>(sm, tw, tid);
-    else if constexpr (L == 480) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 3, 5>(sm, tw, tid);
-    else if constexpr (L == 512) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 8>(sm, tw, tid);
-    else if constexpr (L == 576) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3, 3>(sm, tw, tid);
-    else if constexpr (L == 640) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 2, 5>(sm, tw, tid);
-    else if constexpr (L == 768) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 4, 3>(sm, tw, tid);
-    else if constexpr (L == 1024) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 4, 4>(sm, tw, tid);
-    else if constexpr (L == 24) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3>(sm, tw, tid);
-    else if constexpr (L == 40) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 5>(sm, tw, tid);
-    else if constexpr (L == 72) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3>(sm, tw, tid);
-    else if constexpr (L == 120) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 5>(sm, tw, tid);
-    else if constexpr (L == 144) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 3>(sm, tw, tid);
-    else if constexpr (L == 200) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 5, 5>(sm, tw, tid);
-    else if constexpr (L == 216) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3, 3>(sm, tw, tid);
-    else if constexpr (L == 240) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 5>(sm, tw, tid);
-    else if constexpr (L == 360) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 3, 5>(sm, tw, tid);
-    else if constexpr (L == 400) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 2, 5, 5>(sm, tw, tid);
-    else if constexpr (L == 432) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 3, 3>(sm, tw, tid);
-    else if constexpr (L == 600) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 3, 5, 5>(sm, tw, tid);
-    else if constexpr (L == 720) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 6, 3, 5>(sm, tw, tid);
-    else if constexpr (L == 800) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 4, 5, 5>(sm, tw, tid);
-    else if constexpr (L == 960) lds_stockham<T, DIR, L, B, NT, SW, 1, 8, 8, 3, 5>(sm, tw, tid);
-    else if constexpr (L == 270) lds_stockham<T, DIR, L, B, NT, SW, 1, 9, 5, 6>(sm, tw, tid);
-    else static_assert(L == 8, "unsupported FFT length");
+    lds_fft_run<T, DIR, L, B, NT, SW>(sm, tw, tid, typename FftPlan<L>::type{});
+}
+// The same transform on a column-minor tile (sm[p * B + b]).
+template <typename T, int DIR, int L, int B, int NT = FFT_NT>
+__device__ __forceinline__ void lds_fft_cm(C<T>* sm, const C<T>* __restrict__ tw, int tid = -1) {
+    if (tid < 0) tid = threadIdx.x;
+    lds_fft_cm_run<T, DIR, L, B, NT>(sm, tw, tid, typename FftPlan<L>::type{});
 }
 
 // Lengths with a radix plan, ascending.
@@ -398,7 +469,8 @@ __device__ __forceinline__ uint3 xcd_block() {
 // A column source may split into pre(f, o, b, p) -> Raw (its loads only) and
 // cook(raw, f, o, b, p) -> value (its arithmetic); k_fft_cols then issues the
 // loads of all of a lane's points before cooking any.  Sources without a Raw
-// type are called once per point.
+// type are called once per point.  The tile is column-minor in LDS (point p
+// of column b at p * B + b), so a wavefront's accesses are consecutive slots.
 template <class S, class = void> struct TwoPhaseSrc : std::false_type {};
 template <class S> struct TwoPhaseSrc<S, std::void_t<typename S::Raw>> : std::true_type {};
 
@@ -437,6 +509,8 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     // loads are in flight together (one at a time, each would wait its full latency)
     constexpr int NI = (L + PSTEP - 1) / PSTEP;
     const int p0 = tid / B;
+    // element addresses of the lane's points: a0 + i * ast (the uniform step folds into scalar arithmetic)
+    const long long a0 = obase + b + (long long)p0 * g.PS, ast = (long long)PSTEP * g.PS;
     C<T> buf[NI];
     C<T>* stw = sm + B * LP;  // the twiddle table, staged while the tile's loads are in flight
     if constexpr (TwoPhaseSrc<Src>::value) {
@@ -456,7 +530,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
             C<T> v = mk<T>((T)0, (T)0);
-            if (p < L && bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
+            if (p < L && bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, a0 + i * ast);
             buf[i] = v;
         }
     }
@@ -464,21 +538,21 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
 #pragma unroll
     for (int i = 0; i < NI; i++) {
         const int p = p0 + i * PSTEP;
-        if (p < L) sm[bl * LP + lpos(p)] = buf[i];
+        if (p < L) sm[p * B + bl] = buf[i];
     }
     __syncthreads();
-    lds_fft<T, DIR, L, B>(sm, stw);
+    lds_fft_cm<T, DIR, L, B>(sm, stw);
     if (bvalid) {
         // all of the lane's points read from LDS first, then stored
 #pragma unroll
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
-            if (p < L) buf[i] = sm[bl * LP + lpos(p)];
+            if (p < L) buf[i] = sm[p * B + bl];
         }
 #pragma unroll
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
-            if (p < L && in_support(p, g.out_s, L)) dst[obase + b + (long long)p * g.PS] = buf[i];
+            if (p < L && in_support(p, g.out_s, L)) dst[a0 + i * ast] = buf[i];
         }
     }
 }
