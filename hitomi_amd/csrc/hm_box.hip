@@ -296,13 +296,13 @@ template <typename T> struct XiSrc {
     }
 };
 
-// XiSrc on one plane k_z = kz of the half spectrum: a column pass with the y
-// index as the outer line index (stride nzh) and one column per line, the
-// base pointers advanced by kz (the source reads dk at addr + kz).
+// XiSrc on the planes k_z = 0 and n_z/2 of the half spectrum in one column
+// pass: batch f is the plane k_z = f * kzs (batch stride kzs), the y index the
+// inner line index (stride nzh), so a tile holds 8 or 16 y columns of a plane.
 template <typename T> struct XiPlaneSrc {
     XiSrc<T> xs;
-    int kz;
-    __device__ inline C<T> operator()(int, int o, int, int p, long long addr) const { return xs.at(p, o, kz, addr + kz); }
+    int kzs;
+    __device__ inline C<T> operator()(int f, int, int b, int p, long long addr) const { return xs.at(p, b, f * kzs, addr); }
 };
 
 // The forward x pass of the spectrum fused with the inverse x pass of the
@@ -328,23 +328,25 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
     // all of the lane's points are loaded before any is written to LDS (loads in flight together)
     constexpr int NI = (L + PSTEP - 1) / PSTEP;
     const int p0 = tid / B;
+    // element addresses of the lane's points: a0 + i * ast (the uniform step folds into scalar arithmetic)
+    const long long a0 = b + (long long)p0 * PS, ast = (long long)PSTEP * PS;
     C<T>* stw = sm + B * LP;  // the twiddle table, staged once for both transforms
     {
         C<T> buf[NI];
 #pragma unroll
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
-            buf[i] = (p < L && bvalid) ? dk[b + (long long)p * PS] : mk<T>((T)0, (T)0);
+            buf[i] = (p < L && bvalid) ? dk[a0 + i * ast] : mk<T>((T)0, (T)0);
         }
         for (int e = tid; e < L; e += FFT_NT) stw[e] = tw[e];
 #pragma unroll
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
-            if (p < L) sm[bl * LP + lpos(p)] = buf[i];
+            if (p < L) sm[p * B + bl] = buf[i];
         }
     }
     __syncthreads();
-    lds_fft<T, -1, L, B>(sm, stw);
+    lds_fft_cm<T, -1, L, B>(sm, stw);
     const int j = b / xs.nzh, k = b % xs.nzh;
     const bool form = bvalid && xs.fast_ok(k);
     const typename XiSrc<T>::FastYZ yz = form ? xs.fast_yz(j, k) : typename XiSrc<T>::FastYZ{0.0, 0.0, 0.0, 0.0, 0.0};
@@ -359,30 +361,30 @@ __global__ __launch_bounds__(FFT_NT) void k_fwd_x_xi(C<T>* __restrict__ dk, C<T>
         for (int u = 0; u < CH; u++) {
             const int p = p0 + (i0 + u) * PSTEP;
             const bool ok = i0 + u < NI && p < L;
-            dv[u] = ok ? sm[bl * LP + lpos(p)] : mk<T>((T)0, (T)0);
+            dv[u] = ok ? sm[p * B + bl] : mk<T>((T)0, (T)0);
             qv[u] = (ok && form) ? xs.fast_x(p) : make_double4(0.0, 0.0, 0.0, 0.0);
         }
 #pragma unroll
         for (int u = 0; u < CH; u++) {
             const int p = p0 + (i0 + u) * PSTEP;
             if (i0 + u >= NI || p >= L) continue;
-            if (bvalid) dk[b + (long long)p * PS] = dv[u];
-            sm[bl * LP + lpos(p)] = form ? xs.fast_q(qv[u], yz, dv[u]) : mk<T>((T)0, (T)0);
+            if (bvalid) dk[a0 + (i0 + u) * ast] = dv[u];
+            sm[p * B + bl] = form ? xs.fast_q(qv[u], yz, dv[u]) : mk<T>((T)0, (T)0);
         }
     }
     __syncthreads();
-    lds_fft<T, +1, L, B>(sm, stw);
+    lds_fft_cm<T, +1, L, B>(sm, stw);
     if (bvalid) {
         C<T> out[NI];
 #pragma unroll
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
-            if (p < L) out[i] = sm[bl * LP + lpos(p)];
+            if (p < L) out[i] = sm[p * B + bl];
         }
 #pragma unroll
         for (int i = 0; i < NI; i++) {
             const int p = p0 + i * PSTEP;
-            if (p < L) xh[b + (long long)p * PS] = out[i];
+            if (p < L) xh[a0 + i * ast] = out[i];
         }
     }
 }
@@ -478,12 +480,16 @@ __global__ __launch_bounds__(256) void k_bin_xi(const T* __restrict__ xr, XiBinA
 // minimum-image i'^2 + j'^2 equals a (rows listed per a in a fixed order),
 // one workgroup per distinct a, FP64 sums.  The z transform of G then gives
 // R[a][k], the sum of xi over those rows, for ~10x fewer rows than the mesh.
+// One workgroup per group; blockDim.x (up to XG_NT) lanes over k_z, launched
+// with all nzh = n_z/2 + 1 of them where that fits, so that the Nyquist column
+// is summed beside the others rather than in a second pass of one lane.
+constexpr int XG_NT = 512;
 template <typename T>
-__global__ __launch_bounds__(256) void k_xi_group(const C<T>* __restrict__ xh, int nzh, const int* __restrict__ a_start,
-                                                 const int* __restrict__ rows, C<T>* __restrict__ G) {
+__global__ __launch_bounds__(XG_NT) void k_xi_group(const C<T>* __restrict__ xh, int nzh, const int* __restrict__ a_start,
+                                                   const int* __restrict__ rows, C<T>* __restrict__ G) {
     const int ai = blockIdx.x;
     const int r0 = a_start[ai], r1 = a_start[ai + 1];
-    for (int k = threadIdx.x; k < nzh; k += 256) {
+    for (int k = threadIdx.x; k < nzh; k += blockDim.x) {
         double ar = 0.0, aim = 0.0;
         int r = r0;
         // four rows' values loaded before any is added; the sums keep the row order
@@ -547,28 +553,38 @@ __global__ __launch_bounds__(256) void k_xi_hist_aniso(const T* __restrict__ R, 
 
 // Step 2: H[n] = sum over k with a = n - k'^2 a row key of R[a][k]: the sum of
 // xi over all voxels with minimum-image |n|^2 = n (rvec rule of
-// bispec.hpp:1040-1043), in a fixed order.
+// bispec.hpp:1040-1043), in a fixed order.  XH_G lanes share a key: lane l
+// sums the terms q = l, l + XH_G, ... and the partials are added in a fixed
+// butterfly order (every lane ends with the same total), so a wavefront has
+// XH_G times the loads in flight that one lane per key would have (the keys'
+// dependent row lookups are latency-bound).
+constexpr int XH_G = 8;
 template <typename T>
 __global__ __launch_bounds__(256) void k_xi_hist(const T* __restrict__ R, int n2, const int* __restrict__ a_idx,
                                                 long long amax, long long nkey, double* __restrict__ H) {
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nkey) return;
-    long long m = (long long)sqrt((double)n);
-    while (m * m > n) m--;
-    while ((m + 1) * (m + 1) <= n) m++;
-    if (m > n2 / 2) m = n2 / 2;
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = gt / XH_G;
+    const int l = (int)(gt % XH_G);
     double acc = 0.0;
-    for (long long q = 0; q <= m; q++) {
-        const long long a = n - q * q;
-        if (a > amax) continue;
-        const int ai = a_idx[a];
-        if (ai < 0) continue;
-        const T* row = R + (long long)ai * n2;
-        if (q == 0) acc += (double)row[0];
-        else if (q == n2 / 2) acc += (double)row[n2 / 2];
-        else acc += (double)row[q] + (double)row[n2 - q];
+    if (n < nkey) {
+        long long m = (long long)sqrt((double)n);
+        while (m * m > n) m--;
+        while ((m + 1) * (m + 1) <= n) m++;
+        if (m > n2 / 2) m = n2 / 2;
+        for (long long q = l; q <= m; q += XH_G) {
+            const long long a = n - q * q;
+            if (a > amax) continue;
+            const int ai = a_idx[a];
+            if (ai < 0) continue;
+            const T* row = R + (long long)ai * n2;
+            if (q == 0) acc += (double)row[0];
+            else if (q == n2 / 2) acc += (double)row[n2 / 2];
+            else acc += (double)row[q] + (double)row[n2 - q];
+        }
     }
-    H[n] = acc;
+#pragma unroll
+    for (int o = XH_G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, XH_G);
+    if (l == 0 && n < nkey) H[n] = acc;
 }
 
 // As k_xi_hist for a general multipole: each voxel's xi is weighted by
@@ -818,9 +834,21 @@ __global__ __launch_bounds__(256) void k_tri_stage(TriStage<T> a, C<T>* __restri
     const long long tot = (long long)a.c3[0] * a.c3[1] * a.c3[2];
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= tot) return;
-    const int cz = (int)(e % a.c3[2]);
-    const long long r = e / a.c3[2];
-    const int cy = (int)(r % a.c3[1]), cx = (int)(r / a.c3[1]);
+    int cx, cy, cz;
+    if (tot <= 0xffffffffLL) {
+        // 32-bit quotients (a 64-bit division by a run-time value costs several times more)
+        const unsigned eu = (unsigned)e, c2 = (unsigned)a.c3[2], c1 = (unsigned)a.c3[1];
+        const unsigned r = eu / c2;
+        cz = (int)(eu - r * c2);
+        const unsigned rx = r / c1;
+        cy = (int)(r - rx * c1);
+        cx = (int)rx;
+    } else {
+        cz = (int)(e % a.c3[2]);
+        const long long r = e / a.c3[2];
+        cy = (int)(r % a.c3[1]);
+        cx = (int)(r / a.c3[1]);
+    }
     const int x = support_map(cx, a.s3[0], a.M[0]), y = support_map(cy, a.s3[1], a.M[1]),
               z = support_map(cz, a.s3[2], a.M[2]);
     int idx[3];
@@ -2758,13 +2786,13 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         void* sbuf = par ? p->sn_pairs2.p : p->sn_pairs.p;
         C<T>* xh = (C<T>*)p->xi_half.p;
         if (fused_xi) {
-            // the planes k_z = 0 and n_z/2 the fused pass left: one column per y line
-            for (int kz : {0, m.n[2] / 2}) {
-                XiPlaneSrc<T> ps{xs, kz};
-                if (launch_cols<T, +1>(ctx, "xi_planes_x", m.n[0], ps, xh + kz,
-                                       geom((long long)nzh, (long long)m.n[1] * nzh, m.n[1], m.n[1], -1, 1, 1, -1, -1)))
-                    return -1;
-            }
+            // the planes k_z = 0 and n_z/2 the fused pass left, both in one launch (y columns of stride nzh)
+            XiPlaneSrc<T> ps{xs, m.n[2] / 2};
+            ColGeom gp = geom(0, (long long)m.n[1] * nzh, 1, 1, -1, m.n[1], m.n[1], -1, -1);
+            gp.IS = nzh;
+            gp.FS = m.n[2] / 2;
+            gp.nf = 2;
+            if (launch_cols<T, +1>(ctx, "xi_planes_x", m.n[0], ps, xh, gp)) return -1;
         } else if (launch_cols<T, +1>(ctx, "xi_pass_x", m.n[0], xs, xh,
                                       geom(0, (long long)m.n[1] * nzh, 1, 1, -1, (int)(m.n[1] * nzh),
                                            (int)(m.n[1] * nzh), -1, -1)))
@@ -2775,7 +2803,8 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
         C<T>* xg = (C<T>*)p->xi_real.p;
         T* xr = (T*)p->xi_R.p;
         if (ctx->launch("xi_group", [&] {
-                hipLaunchKernelGGL(k_xi_group<T>, dim3((unsigned)p->na), dim3(256), 0, ctx->stream, xh, (int)nzh,
+                const unsigned nt = (unsigned)std::min<long long>(XG_NT, (nzh + 63) / 64 * 64);
+                hipLaunchKernelGGL(k_xi_group<T>, dim3((unsigned)p->na), dim3(nt), 0, ctx->stream, xh, (int)nzh,
                                    (const int*)p->a_start.p, (const int*)p->a_rows.p, xg);
             }))
             return -1;
@@ -2794,7 +2823,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                        ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey, p->ang,
                                        (double*)hbuf);
                 else
-                    hipLaunchKernelGGL(k_xi_hist<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0,
+                    hipLaunchKernelGGL(k_xi_hist<T>, dim3((unsigned)((p->nkey * XH_G + 255) / 256)), dim3(256), 0,
                                        ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey,
                                        (double*)hbuf);
             }))

@@ -403,11 +403,12 @@ __host__ __device__ inline int support_map(int c, int s, int n) {
 __host__ __device__ inline int support_count(int s, int n) { return (s < 0 || 2 * s + 1 >= n) ? n : 2 * s + 1; }
 __host__ __device__ inline bool in_support(int p, int s, int n) { return s < 0 || p <= s || p >= n - s; }
 
-// Column-pass geometry: element (f, o, b, p) at f*FS + o*OS + b + p*PS, where
+// Column-pass geometry: element (f, o, b, p) at f*FS + o*OS + b*IS + p*PS, where
 // o and b are the actual outer/inner line indices and f < nf a batch index
 // (grid.z) over equally laid-out arrays.
 struct ColGeom {
     long long OS, PS;
+    long long IS = 1;  // stride of the inner line index (1: a tile's columns are contiguous)
     long long FS = 0;
     int nf = 1;
     int n_outer, outer_full, outer_s;  // compact count, full extent, support
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     constexpr int NI = (L + PSTEP - 1) / PSTEP;
     const int p0 = tid / B;
     // element addresses of the lane's points: a0 + i * ast (the uniform step folds into scalar arithmetic)
-    const long long a0 = obase + b + (long long)p0 * g.PS, ast = (long long)PSTEP * g.PS;
+    const long long a0 = obase + (long long)b * g.IS + (long long)p0 * g.PS, ast = (long long)PSTEP * g.PS;
     C<T> buf[NI];
     C<T>* stw = sm + B * LP;  // the twiddle table, staged while the tile's loads are in flight
     if constexpr (TwoPhaseSrc<Src>::value) {
@@ -963,14 +964,14 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols_rt(Src src, C<T>* __restric
     const int pstep = FFT_NT / B;
     for (int p = tid / B; p < L; p += pstep) {
         C<T> v = mk<T>((T)0, (T)0);
-        if (bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + b + (long long)p * g.PS);
+        if (bvalid && in_support(p, point_s, L)) v = src(f, o, b, p, obase + (long long)b * g.IS + (long long)p * g.PS);
         sm[bl * LP + rt_lpos(p)] = v;
     }
     __syncthreads();
     rt_fft<T, DIR>(sm, tw, pl, tid);
     if (bvalid) {
         for (int p = tid / B; p < L; p += pstep)
-            if (in_support(p, g.out_s, L)) dst[obase + b + (long long)p * g.PS] = sm[bl * LP + rt_lpos(p)];
+            if (in_support(p, g.out_s, L)) dst[obase + (long long)b * g.IS + (long long)p * g.PS] = sm[bl * LP + rt_lpos(p)];
     }
 }
 
