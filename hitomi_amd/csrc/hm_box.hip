@@ -2158,13 +2158,15 @@ static ColGeom geom(long long OS, long long PS, int n_outer, int outer_full, int
 
 // x pass over nf fields (stride FS) of support sup: lines (y, z) in the
 // support, points x in the support.
-static ColGeom tri_geom_x(const int M[3], const int sup[3], long long FS, int nf, const int4* fsup = nullptr) {
+static ColGeom tri_geom_x(const int M[3], const int sup[3], long long FS, int nf, const int4* fsup = nullptr,
+                           const int4* fsup_host = nullptr) {
     ColGeom g = geom(M[2], (long long)M[1] * M[2], support_count(sup[1], M[1]), M[1], sup[1],
                      support_count(sup[2], M[2]), M[2], sup[2], sup[0]);
     g.FS = FS;
     g.nf = nf;
     if (fsup) {
         g.fsup = fsup;
+        g.fsup_host = fsup_host;
         g.role_outer = 1;
         g.role_inner = 2;
         g.role_point = 0;
@@ -2172,12 +2174,14 @@ static ColGeom tri_geom_x(const int M[3], const int sup[3], long long FS, int nf
     return g;
 }
 // y pass: lines (x all, z in the support), points y in the support.
-static ColGeom tri_geom_y(const int M[3], const int sup[3], long long FS, int nf, const int4* fsup = nullptr) {
+static ColGeom tri_geom_y(const int M[3], const int sup[3], long long FS, int nf, const int4* fsup = nullptr,
+                           const int4* fsup_host = nullptr) {
     ColGeom g = geom((long long)M[1] * M[2], M[2], M[0], M[0], -1, support_count(sup[2], M[2]), M[2], sup[2], sup[1]);
     g.FS = FS;
     g.nf = nf;
     if (fsup) {
         g.fsup = fsup;
+        g.fsup_host = fsup_host;
         g.role_inner = 2;
         g.role_point = 1;
     }
@@ -2904,9 +2908,9 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             auto gram_combo = [&](int m1, int m2, double wq) -> int {
                 ms.m1 = m1;
                 ms.m2 = m2;
-                ColGeom gx = tri_geom_x(p->M, st.s, nvox, nsh, fsup);
+                ColGeom gx = tri_geom_x(p->M, st.s, nvox, nsh, fsup, p->fsup.data());
                 if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
-                ColGeom gy = tri_geom_y(p->M, st.s, nvox, nsh, fsup);
+                ColGeom gy = tri_geom_y(p->M, st.s, nvox, nsh, fsup, p->fsup.data());
                 if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
                 RowGeom rg{nrows * nsh, p->M[2], p->M[2], st.s[2], fsz, nrows};
                 if (fft_rows<T>(ctx, "tri_pass_z", p->M[2], +1, F, F, rg)) return -1;
@@ -2952,12 +2956,12 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
             }
         } else {
             // x pass of every shell (one launch), then of delta/W in place
-            ColGeom gx = tri_geom_x(p->M, st.s, nvox, p->nk, fsup);
+            ColGeom gx = tri_geom_x(p->M, st.s, nvox, p->nk, fsup, p->fsup.data());
             if (launch_cols<T, +1>(ctx, "tri_pass_x", p->M[0], ms, F, gx)) return -1;
             ColGeom gx3 = tri_geom_x(p->M, st.s3, nvox, 1);
             if (fft_cols<T>(ctx, "tri3_pass_x", p->M[0], +1, F3, F3, gx3)) return -1;
             // y pass
-            ColGeom gy = tri_geom_y(p->M, st.s, nvox, p->nk, fsup);
+            ColGeom gy = tri_geom_y(p->M, st.s, nvox, p->nk, fsup, p->fsup.data());
             if (fft_cols<T>(ctx, "tri_pass_y", p->M[1], +1, F, F, gy)) return -1;
             ColGeom gy3 = tri_geom_y(p->M, st.s3, nvox, 1);
             if (fft_cols<T>(ctx, "tri3_pass_y", p->M[1], +1, F3, F3, gy3)) return -1;

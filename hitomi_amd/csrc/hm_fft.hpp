@@ -423,6 +423,11 @@ struct ColGeom {
     // output support on the transformed axis: only the points |n| <= out_s are stored (-1: all), for a
     // forward transform whose consumer reads a low-frequency cube only
     int out_s = -1;
+    // host copy of fsup: the launcher then runs a compact grid of exactly the
+    // tiles inside the batches' supports, in batch order (grid.x = all of them,
+    // compact = 1), instead of the box of the largest with the rest exiting
+    const int4* fsup_host = nullptr;
+    int compact = 0;
 };
 
 // Row-pass geometry: element (r, p) at r*RS + p.  With fps set, row r takes
@@ -435,7 +440,21 @@ struct RowGeom {
     int out_s = -1;  // only the output points |n| <= out_s are stored (-1: all)
 };
 
-__device__ __forceinline__ int int4_get(const int4& q, int i) { return i == 0 ? q.x : (i == 1 ? q.y : q.z); }
+__host__ __device__ __forceinline__ int int4_get(const int4& q, int i) { return i == 0 ? q.x : (i == 1 ? q.y : q.z); }
+
+// Tiles of B lines of a per-batch-support column geometry (the compact grid);
+// 0: the geometry has no host copy of its supports.
+inline long long compact_tiles(const ColGeom& g, int B) {
+    if (!g.fsup || !g.fsup_host) return 0;
+    long long tot = 0;
+    for (int f = 0; f < g.nf; f++) {
+        const int4 q = g.fsup_host[f];
+        const int no = g.role_outer >= 0 ? support_count(int4_get(q, g.role_outer), g.outer_full) : g.n_outer;
+        const int ni = g.role_inner >= 0 ? support_count(int4_get(q, g.role_inner), g.inner_full) : g.n_inner;
+        tot += (long long)no * ((ni + B - 1) / B);
+    }
+    return tot;
+}
 
 // Plain global source / destination.
 template <typename T> struct GSrc {
@@ -482,7 +501,25 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_cols(Src src, C<T>* __restrict__
     constexpr int LP = LdsPad<L>::LP;
     extern __shared__ unsigned char smraw[];
     C<T>* sm = reinterpret_cast<C<T>*>(smraw);
-    const uint3 wg = xcd_block();
+    uint3 wg = xcd_block();
+    if (g.compact) {
+        // compact grid: linear tile id -> (batch, compact outer index, tile) in batch order
+        unsigned t = wg.x;
+        int fz = -1;
+        for (int ff = 0; ff < g.nf; ff++) {
+            const int4 q = g.fsup[ff];
+            const int no = g.role_outer >= 0 ? support_count(int4_get(q, g.role_outer), g.outer_full) : g.n_outer;
+            const int ni = g.role_inner >= 0 ? support_count(int4_get(q, g.role_inner), g.inner_full) : g.n_inner;
+            const unsigned nt = (unsigned)((ni + B - 1) / B), cnt = (unsigned)no * nt;
+            if (t < cnt) {
+                fz = ff;
+                wg = make_uint3(t % nt, t / nt, (unsigned)ff);
+                break;
+            }
+            t -= cnt;
+        }
+        if (fz < 0) return;
+    }
     const int f = (int)wg.z;
     int outer_s = g.outer_s, inner_s = g.inner_s, point_s = g.point_s, n_inner = g.n_inner;
     if (g.fsup) {

@@ -24,16 +24,28 @@ int launch_cols_L(hm_ctx* ctx, const char* name, const Src& src, C<T>* dst, cons
         const size_t lds = ((size_t)BW * LdsPad<L>::LP + L) * sizeof(C<T>);  // tile + twiddle table
         if (ctx->wide_cols(L, name) && lds <= ctx->lds_max) {
             dim3 grid((unsigned)((g.n_inner + BW - 1) / BW), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
+            ColGeom gc = g;
+            const long long ct = compact_tiles(g, BW);
+            if (ct > 0 && ct <= 0x7fffffffLL) {
+                gc.compact = 1;
+                grid = dim3((unsigned)ct, 1, 1);
+            }
             return ctx->launch(name, [&] {
-                hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src, BW>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g,
+                hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src, BW>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, gc,
                                    tw);
             });
         }
     }
     dim3 grid((unsigned)((g.n_inner + B0 - 1) / B0), (unsigned)g.n_outer, (unsigned)(g.nf > 0 ? g.nf : 1));
+    ColGeom gc = g;
+    const long long ct = compact_tiles(g, B0);
+    if (ct > 0 && ct <= 0x7fffffffLL) {
+        gc.compact = 1;
+        grid = dim3((unsigned)ct, 1, 1);
+    }
     const size_t lds = col_lds<T, L>();
     return ctx->launch(name, [&] {
-        hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, g, tw);
+        hipLaunchKernelGGL((k_fft_cols<T, L, DIR, Src>), grid, dim3(FFT_NT), lds, ctx->stream, src, dst, gc, tw);
     });
 }
 
