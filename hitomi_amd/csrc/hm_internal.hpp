@@ -42,6 +42,8 @@ struct hm_ctx {
     bool wide_cols(int L, const char* pass = nullptr) const {
         int b = L == 256 ? 0 : ((L == 288 || L == 270) ? 1 : (L == 512 ? 2 : -1));
         if (b == 1 && pass && std::strcmp(pass, "tri_pass_x") == 0) b = 3;
+        // the two self-conjugate planes fill 128 narrow tiles, too few to halve
+        if (pass && std::strcmp(pass, "xi_planes_x") == 0) return false;
         return b >= 0 && ((wide_mask >> b) & 1u);
     }
     // device workspace of the mock generator, kept between calls
