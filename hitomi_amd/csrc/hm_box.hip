@@ -613,31 +613,37 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_xi_hist_ang(const T* __restrict__ R, int n2, const int* __restrict__ a_idx,
                                                     long long amax, long long nkey, AngPoly A,
                                                     double* __restrict__ H) {
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nkey) return;
-    long long m = (long long)sqrt((double)n);
-    while (m * m > n) m--;
-    while ((m + 1) * (m + 1) <= n) m++;
-    if (m > n2 / 2) m = n2 / 2;
-    const double rn = sqrt((double)n);
+    // XH_G lanes per key as in k_xi_hist (terms q = l, l + XH_G, ...; fixed butterfly sum)
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = gt / XH_G;
+    const int l = (int)(gt % XH_G);
     double acc = 0.0;
-    for (long long q = 0; q <= m; q++) {
-        const long long a = n - q * q;
-        if (a > amax) continue;
-        const int ai = a_idx[a];
-        if (ai < 0) continue;
-        const T* row = R + (long long)ai * n2;
-        if (n == 0) {
-            acc += A.a0 * (double)row[0];
-        } else if (q == 0) {
-            acc += ang_eval(A, 0.0) * (double)row[0];
-        } else if (q == n2 / 2) {  // the Nyquist plane is the negative z (field.hpp:1042)
-            acc += ang_eval(A, -(double)q / rn) * (double)row[n2 / 2];
-        } else {
-            acc += ang_eval(A, (double)q / rn) * (double)row[q] + ang_eval(A, -(double)q / rn) * (double)row[n2 - q];
+    if (n < nkey) {
+        long long m = (long long)sqrt((double)n);
+        while (m * m > n) m--;
+        while ((m + 1) * (m + 1) <= n) m++;
+        if (m > n2 / 2) m = n2 / 2;
+        const double rn = sqrt((double)n);
+        for (long long q = l; q <= m; q += XH_G) {
+            const long long a = n - q * q;
+            if (a > amax) continue;
+            const int ai = a_idx[a];
+            if (ai < 0) continue;
+            const T* row = R + (long long)ai * n2;
+            if (n == 0) {
+                acc += A.a0 * (double)row[0];
+            } else if (q == 0) {
+                acc += ang_eval(A, 0.0) * (double)row[0];
+            } else if (q == n2 / 2) {  // the Nyquist plane is the negative z (field.hpp:1042)
+                acc += ang_eval(A, -(double)q / rn) * (double)row[n2 / 2];
+            } else {
+                acc += ang_eval(A, (double)q / rn) * (double)row[q] + ang_eval(A, -(double)q / rn) * (double)row[n2 - q];
+            }
         }
     }
-    H[n] = acc;
+#pragma unroll
+    for (int o = XH_G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, XH_G);
+    if (l == 0 && n < nkey) H[n] = acc;
 }
 
 // As k_xi_hist_aniso with each voxel's xi weighted by A(mu), mu = r_z / |r|
@@ -2823,7 +2829,7 @@ static int plan_execute(hm_ctx* ctx, BoxPlanImpl* p, const double* d_pos, long l
                                        ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->keys, p->nkey,
                                        (double*)hbuf);
                 else if (p->bk_gen)
-                    hipLaunchKernelGGL(k_xi_hist_ang<T>, dim3((unsigned)((p->nkey + 255) / 256)), dim3(256), 0,
+                    hipLaunchKernelGGL(k_xi_hist_ang<T>, dim3((unsigned)((p->nkey * XH_G + 255) / 256)), dim3(256), 0,
                                        ctx->stream, xr, m.n[2], (const int*)p->a_idx.p, p->amax, p->nkey, p->ang,
                                        (double*)hbuf);
                 else
