@@ -86,6 +86,23 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
     const double kxv[2] = {a.t.kv[0][i], a.t.kv[0][im]}, kyv[2] = {a.t.kv[1][j], a.t.kv[1][jm]};
     const double wxv[2] = {a.t.wf[0][i], a.t.wf[0][im]}, wyv[2] = {a.t.wf[1][j], a.t.wf[1][jm]};
     const double sxv[2] = {a.t.sf[0][i], a.t.sf[0][im]}, syv[2] = {a.t.sf[1][j], a.t.sf[1][jm]};
+    // the sums of the bin the last modes fell in, held in registers while consecutive modes stay in that
+    // one bin (read from memory when the bin is entered, written back when it is left: every partial sum
+    // sees the same additions in the same order as with a read-modify-write per mode)
+    int cb_ = -1;
+    double cre[BIN_MAX_ELLS], cim[BIN_MAX_ELLS];
+    long long ccn = 0;
+    auto flush = [&]() {
+        if (cb_ < 0) return;
+#pragma unroll
+        for (int e = 0; e < BIN_MAX_ELLS; e++) {
+            if (e >= a.ne) break;
+            P[(cb_ * a.ne + e) * ps] = cre[e];
+            if (Pi) Pi[(cb_ * a.ne + e) * ps] = cim[e];
+        }
+        Cn[cb_ * ps] = ccn;
+        cb_ = -1;
+    };
     for (int kz = kz0; kz <= kz1; kz++) {
         const C<T> d = dk[rbase + kz];
         const double re = (double)d.x, imv = (double)d.y;
@@ -109,6 +126,30 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
             const double val = (p2 - sn * S) / (W * W);
             const double vim = (m ? -pim : pim) / (W * W);
             const double mu = (fabs(kmag) < 1.0e-15) ? 0.0 : kzv / kmag;
+            if (b.x == b.y) {
+                if (b.x != cb_) {
+                    flush();
+                    cb_ = b.x;
+#pragma unroll
+                    for (int e = 0; e < BIN_MAX_ELLS; e++) {
+                        if (e >= a.ne) break;
+                        cre[e] = P[(cb_ * a.ne + e) * ps];
+                        cim[e] = Pi ? Pi[(cb_ * a.ne + e) * ps] : 0.0;
+                    }
+                    ccn = Cn[cb_ * ps];
+                }
+#pragma unroll
+                for (int e = 0; e < BIN_MAX_ELLS; e++) {
+                    if (e >= a.ne) break;
+                    const int l = a.ells[e];
+                    const double y = (l == 0) ? 1.0 : ((fabs(kmag) < 1.0e-15) ? 0.0 : legendre(l, mu));
+                    cre[e] += val * y;
+                    if (Pi) cim[e] += vim * y;
+                }
+                ccn += 1;
+                continue;
+            }
+            flush();
             for (int bb = b.x; bb <= b.y; bb++) {
                 for (int e = 0; e < a.ne; e++) {
                     const int l = a.ells[e];
@@ -120,6 +161,7 @@ __global__ __launch_bounds__(256) void k_bin_box(const C<T>* __restrict__ dk, Bi
             }
         }
     }
+    flush();
 }
 
 // ---------------------------------------------------------------------------
